@@ -1,0 +1,233 @@
+"""Throughput of the MI355X hash-join / hash-aggregate path.
+
+The headline workload is the equi hash join with a filtered build: build
+1.5M rows (int64 key = a permutation of 1..1.5M, segment uniform 0..4,
+filter segment = 1), probe 60M rows (int64 fk uniform over 1..1.5M,
+DECIMAL(15,2) value); outputs COUNT(*), exact SUM(value) and the matched
+(probe_row_id, build_row_id) pairs. One step is one whole query on inputs
+already resident in HBM: build filter -> append -> table build -> probe
+(pairs + fused COUNT/SUM). `value` is (build rows + probe rows) / second.
+
+The other two shapes are measured in the same run (unless --workload join):
+the filtered low-cardinality group-by over 6M lineitem rows (fused filter +
+project + aggregate) and the high-cardinality Zipf group-by over 100M rows.
+
+roofline: algorithmic bytes of the dominant kernel (the probe: 16 B per
+probe row read + 16 B per pair written) over its average duration, timed
+with HIP events on the library's stream. cpu_baseline: the CPU oracle (a
+restatement of the reference algorithm, one thread) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["join", "lineitem", "groups", "all"], default="all")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    return ap.parse_args()
+
+
+def timing(family):
+    import ctypes
+    from qumirdb_amd import lib
+    n = ctypes.c_int64()
+    ms = lib().qdb_gpu_timing_avg_ms(family.encode(), ctypes.byref(n))
+    return ms, n.value
+
+
+def bench_join(steps, warmup, build_rows=1_500_000, probe_rows=60_000_000):
+    import ctypes
+    from qumirdb_amd import lib, workloads as W
+    from qumirdb_amd.host import DeviceBuffer, FilterKernel, RowJoinKernels, TRowJoinProcessor
+
+    jd = W.JoinDevice(build_rows, probe_rows)
+    pred = FilterKernel(W.BUILD_TYPES, *W.join_build_predicate())
+    kern = RowJoinKernels(W.join_spec(fused=True))
+    sel = DeviceBuffer(build_rows)
+    results = {}
+
+    def step():
+        # the build-side filter writes the selection into the build batch
+        view = type(jd.build.rs).from_buffer_copy(jd.build.rs)
+        view.Selection = sel.ptr
+        pred.fn(ctypes.byref(view), None, None, None)
+        jd.build.set_selection_buffer(sel)
+        proc = TRowJoinProcessor(kern, build_is_left=False)
+        proc.append(jd.build)
+        proc.build()
+        n = lib().qdb_gpu_join_processor_probe_only(proc.p, jd.probe.ptr)
+        f = proc.fused_results()
+        results["pairs"], results["count"], results["sum"] = n, f[0]["value"], f[1]["value"]
+        proc.close()
+
+    for _ in range(warmup):
+        step()
+    lib().qdb_gpu_synchronize()
+    lib().qdb_gpu_timing_reset()
+    lib().qdb_gpu_timing_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    lib().qdb_gpu_synchronize()
+    t1 = time.perf_counter()
+    lib().qdb_gpu_timing_enable(0)
+    probe_ms, launches = timing("join_probe")
+    build_ms, _ = timing("join_build")
+    ms_per_step = (t1 - t0) * 1e3 / steps
+    pairs = results["pairs"]
+    probe_bytes = probe_rows * 16 + pairs * 16
+    achieved = probe_bytes / (probe_ms * 1e-3) / 1e9 if probe_ms else 0.0
+    lib().qdb_gpu_timing_reset()
+    return {
+        "rows": build_rows + probe_rows, "ms_per_step": ms_per_step,
+        "rows_per_s": (build_rows + probe_rows) / (ms_per_step * 1e-3),
+        "probe_kernel_ms": probe_ms, "probe_launches": launches, "build_kernel_ms": build_ms,
+        "pairs": pairs, "count": results["count"], "sum_value_unscaled": str(results["sum"]),
+        "algorithmic_bytes_per_probe": probe_bytes, "achieved_gbs": achieved,
+    }
+
+
+def bench_lineitem(steps, warmup, rows=6_000_000):
+    from qumirdb_amd import lib, workloads as W
+    from qumirdb_amd.host import AggregateKernels, TAggregateProcessor
+
+    dev = W.LineitemDevice(rows)
+    kern = AggregateKernels(W.lineitem_fused_spec())
+    out = {}
+
+    def step():
+        p = TAggregateProcessor(kern, initial_capacity=8)
+        p.add(dev.rowset)
+        out["rows"] = W.lineitem_finalize(p.finish())
+        p.close()
+
+    for _ in range(warmup):
+        step()
+    lib().qdb_gpu_timing_reset()
+    lib().qdb_gpu_timing_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    lib().qdb_gpu_synchronize()
+    t1 = time.perf_counter()
+    lib().qdb_gpu_timing_enable(0)
+    k_ms, _ = timing("agg_update")
+    lib().qdb_gpu_timing_reset()
+    ms = (t1 - t0) * 1e3 / steps
+    nbytes = rows * (1 + 1 + 4 + 8 * 4)
+    return {"rows": rows, "ms_per_step": ms, "rows_per_s": rows / (ms * 1e-3), "kernel_ms": k_ms,
+            "algorithmic_bytes": nbytes, "achieved_gbs": nbytes / (k_ms * 1e-3) / 1e9 if k_ms else 0.0,
+            "groups": len(out["rows"])}
+
+
+def bench_groups(steps, warmup, rows=100_000_000, distinct=10_000_000):
+    from qumirdb_amd import lib, workloads as W
+    from qumirdb_amd.host import AggregateKernels, TAggregateProcessor
+
+    dev = W.GroupsDevice(rows, distinct)
+    kern = AggregateKernels(W.groups_spec())
+    out = {}
+
+    def step():
+        p = TAggregateProcessor(kern, initial_capacity=1 << 24)
+        p.add(dev.rowset)
+        res = p.finish()
+        out["groups"] = len(res[0][0])
+        p.close()
+
+    for _ in range(warmup):
+        step()
+    lib().qdb_gpu_timing_reset()
+    lib().qdb_gpu_timing_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    lib().qdb_gpu_synchronize()
+    t1 = time.perf_counter()
+    lib().qdb_gpu_timing_enable(0)
+    k_ms, _ = timing("agg_update")
+    f_ms, _ = timing("agg_finalize")
+    lib().qdb_gpu_timing_reset()
+    ms = (t1 - t0) * 1e3 / steps
+    groups = out["groups"]
+    nbytes = rows * 24 + groups * 48
+    return {"rows": rows, "ms_per_step": ms, "rows_per_s": rows / (ms * 1e-3), "update_kernel_ms": k_ms,
+            "finalize_kernel_ms": f_ms, "groups": groups, "algorithmic_bytes": nbytes,
+            "achieved_gbs": nbytes / ((k_ms + f_ms) * 1e-3) / 1e9 if k_ms else 0.0}
+
+
+def cpu_join_baseline(build_rows=1_500_000, sample_probe=6_000_000):
+    """The oracle's row-table join (one thread) over the full filtered build
+    and a probe sample of `sample_probe` rows."""
+    import numpy as np
+    from oracle import oracle as O
+    from qumirdb_amd import workloads as W
+    key, seg = O.gen_join_build(W.SEED_JOIN, 0, build_rows, build_rows)
+    fk, val = O.gen_join_probe(W.SEED_JOIN, 0, sample_probe, build_rows)
+    t0 = time.perf_counter()
+    oj = O.OracleJoin(W.join_spec().to_c())
+    oj.append(O.HostRowSet(W.BUILD_TYPES, [key, seg], selection=(seg == 1).astype(np.uint8)))
+    oj.build()
+    n = O.LIB.oracle_join_probe(oj.h, O.HostRowSet(W.PROBE_TYPES, [fk, val]).ptr)
+    t1 = time.perf_counter()
+    rows = build_rows + sample_probe
+    return {"value": rows / (t1 - t0), "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": f"build {build_rows} rows + probe sample {sample_probe} of 60M rows, {n} pairs, {t1 - t0:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
+              "(the path runs on a single MI355X; see DESIGN.md for how it would shard).", file=sys.stderr)
+        sys.exit(2)
+    import qumirdb_amd
+    qumirdb_amd.init_device(0)
+    out = {}
+    if args.workload in ("join", "all"):
+        out["join"] = bench_join(args.steps, args.warmup)
+    if args.workload in ("lineitem", "all"):
+        out["lineitem"] = bench_lineitem(args.steps, args.warmup)
+    if args.workload in ("groups", "all"):
+        out["groups"] = bench_groups(max(1, args.steps // 2), max(1, args.warmup // 2))
+    head = out.get("join") or out.get("lineitem") or out.get("groups")
+    name = "join" if "join" in out else list(out)[0]
+    cpu = None
+    if not args.no_cpu and name == "join":
+        cpu = cpu_join_baseline()
+    roof = None
+    if name == "join":
+        j = out["join"]
+        roof = {"bound": "hbm", "achieved": round(j["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(j["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_probe",
+                "kernel_ms": round(j["probe_kernel_ms"], 4)}
+    line = {
+        "metric": "rows/sec + achieved HBM GB/s (% of peak) for agg/join; queries/s and p99 latency",
+        "value": head["rows_per_s"], "unit": "rows/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int64", "data": "synthetic (seeded, generated in HBM)",
+        "config": {"workload": "equi hash join, filtered build 1.5M rows (segment = 1), probe 60M rows; "
+                               "COUNT(*), exact SUM(value), matched pairs" if name == "join" else name,
+                   "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu,
+        "workloads": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in out.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,450 @@
+/*
+ * qdb_gpu.h - C-ABI of the MI355X (gfx950) hash-aggregate / hash-join path.
+ *
+ * The reference engine calls its operator kernels through plain function
+ * pointers held in a kernel slot (qdb/kernel/generated.h:82-128): arguments
+ * are pointers or integers of at most 64 bits, results void/int/bool. This
+ * library fills such slots with entry points that run hand-written HIP
+ * kernels. Every signature below keeps the reference's argument meaning,
+ * op-codes, state-blob sizes and error convention:
+ *
+ *   agg_dispatch      int64_t (void* ht, TRowSet* batch, int64_t arg, int64_t op)
+ *                     qdb/kernel/compiler.h:208-212, compiler.cpp:2776-2782
+ *   agg_finish_rowset int64_t (void* ht, TRowSet* out)
+ *                     qdb/kernel/compiler.h:232-233
+ *   jr_dispatch       bool (void* arena, void* table, void* scratch, void* pairs,
+ *                           TRowSet* batch, const int64_t* key_columns, int64_t op)
+ *                     qdb/kernel/compiler.h:385-393, compiler.cpp:3242-3265
+ *   jr_materialize    int64_t (void* pairs, TRowSet* stream, int64_t start,
+ *                              int64_t limit, TRowSet* out)
+ *                     qdb/kernel/compiler.cpp:3244,3266-3274
+ *   filter <kernel>   void (TRowSet* rs, void* arena, void** regexes,
+ *                           const uint8_t* input_sel)
+ *                     qdb/kernel/compiler.cpp:2252-2260
+ *   <project>         void (TRowSet* in, void** out, void* arena, void** regexes,
+ *                           const uint8_t* input_sel)
+ *                     qdb/kernel/compiler.cpp:2329-2336
+ *   group hash        bool (TRowSet* rs, uint64_t* hashes, const int64_t* key_cols)
+ *                     qdb/kernel/compiler.cpp:3057-3125
+ *
+ * The reference generates one native function per query; here a "compile"
+ * call binds a query spec to a slot of a fixed pool of trampolines and
+ * returns plain function pointers with exactly those signatures, which is
+ * what FinalizeKernels stores into TKernelSlot::Fns.
+ *
+ * Column buffers handed to these entry points live in device memory (HBM):
+ * qdb_gpu_stage_rowset copies host batches in once through pinned buffers.
+ * Every entry point is synchronous with respect to its caller, as the
+ * reference's CPU kernels are: when it returns, the batch may be released.
+ */
+#ifndef QDB_GPU_H
+#define QDB_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Column-batch ABI, byte-compatible with qdb/io/io.h:17-39.                 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct TColumn {
+    char* Data;
+    int32_t DataBitOffset;   /* bit offset for bit-packed (bool) columns */
+    const uint8_t* Mask;     /* Arrow LSB validity bitmap; NULL = all valid */
+    int32_t MaskBitOffset;
+    void* Offsets;           /* variable-length offsets; NULL for fixed width */
+    uint8_t OffsetWidth;     /* 4 or 8 for strings, 0 otherwise */
+} TColumn;
+
+typedef struct TRowSet {
+    TColumn* Columns;
+    int64_t ColumnCount;
+    int64_t RowCount;
+    uint8_t* Selection;      /* NULL = all rows; Selection[i] == 0 skips row i */
+    uint64_t* Hash;          /* NULL, or rh_hash of the key of every selected row */
+    void (*Destroy)(struct TRowSet*);
+    void* Private;
+    int64_t RefCount;
+} TRowSet;
+
+/* Value kinds, qdb/io/value_type.h:7-18. */
+enum {
+    QDB_KIND_SIGNED = 0,
+    QDB_KIND_UNSIGNED = 1,
+    QDB_KIND_FLOAT = 2,
+    QDB_KIND_BOOL = 3,
+    QDB_KIND_STRING = 4,
+    QDB_KIND_DATE = 5,
+    QDB_KIND_DECIMAL = 6
+};
+
+/* How a column's values are read, qdb/io/value_type.h:20-35. A DECIMAL is an
+ * unscaled signed integer of Width 8 (precision <= 18) or 16 (i128),
+ * qdb/plan/types/decimal.h:80-99. */
+typedef struct QdbValueType {
+    uint8_t Kind;
+    int32_t Width;
+    int32_t Scale;
+    bool Nullable;
+} QdbValueType;
+
+/* ------------------------------------------------------------------------ */
+/* Op-codes and state-blob sizes (qdb/kernel/compiler.h:217-230,363-383,     */
+/* 603-616).                                                                 */
+/* ------------------------------------------------------------------------ */
+
+enum {
+    QDB_AGG_OP_INIT = 0,
+    QDB_AGG_OP_UPDATE = 1,
+    QDB_AGG_OP_INIT_ROWS = 3,
+    QDB_AGG_OP_SWITCH_TO_ROWS_AT = 4,
+    QDB_AGG_OP_EXPECT_ROW_GROUPS = 5
+};
+
+enum {
+    QDB_JR_OP_INIT = 0,
+    QDB_JR_OP_APPEND = 1,
+    QDB_JR_OP_BUILD = 2,
+    QDB_JR_OP_PROBE = 3,
+    QDB_JR_OP_APPEND_NULL_KEYS = 5,
+    QDB_JR_OP_EMIT_BY_MARK = 6
+};
+
+#define QDB_HASH_TABLE_SIZE 304
+#define QDB_PAIR_BUFFER_SIZE 24
+#define QDB_ROW_ARENA_SIZE 56
+#define QDB_ROW_TABLE_SIZE 48
+#define QDB_ROW_SCRATCH_SIZE 64
+#define QDB_ROWSET_SIZE 64
+
+/* The PairBuffer blob (qdb/exec/row_join_exec.h:95-100). Data holds 2 i64 per
+ * pair, (left id, right id); here Data is device memory. A stream row id is
+ * (-1 << 32) + row, as jr_push_pair writes it (qdb/kernel/join/join_rows.oz:
+ * 117-125); the build side is the build row's ordinal over every appended
+ * batch (the reference stores a row pointer there, which has no meaning on
+ * another device). */
+typedef struct QdbPairBuffer {
+    int64_t Count;
+    int64_t Capacity;
+    int64_t* Data;
+} QdbPairBuffer;
+
+/* ------------------------------------------------------------------------ */
+/* Expression programs: predicates (filter) and computed columns (project).  */
+/* A program is a straight-line list of register instructions evaluated per */
+/* row. Values are exact i128 (integers and unscaled decimals), f64, or a   */
+/* three-valued bool; every register carries a validity flag (SQL NULL).    */
+/* ------------------------------------------------------------------------ */
+
+enum {
+    QDB_EXPR_COLUMN = 1,   /* r[dst] = column[col] widened to i128 / f64 / bool   */
+    QDB_EXPR_CONST_INT = 2,/* r[dst] = (imm_hi:imm_lo) as i128                    */
+    QDB_EXPR_CONST_F64 = 3,/* r[dst] = f64 bits in imm_lo                         */
+    QDB_EXPR_ADD = 4,      /* r[dst] = r[a] + r[b] (i128, wraps; or f64)           */
+    QDB_EXPR_SUB = 5,
+    QDB_EXPR_MUL = 6,
+    QDB_EXPR_NEG = 7,
+    QDB_EXPR_SCALE_UP = 8, /* r[dst] = r[a] * 10^imm_lo (decimal rescale)          */
+    QDB_EXPR_CMP = 9,      /* r[dst] = r[a] <cmp imm_lo> r[b], three-valued        */
+    QDB_EXPR_AND = 10,     /* SQL three-valued AND                                 */
+    QDB_EXPR_OR = 11,
+    QDB_EXPR_NOT = 12,
+    QDB_EXPR_IS_NULL = 13, /* r[dst] = r[a] IS NULL (never NULL)                   */
+    QDB_EXPR_TO_F64 = 14   /* r[dst] = decimal r[a] at scale imm_lo as nearest f64 */
+};
+
+enum { QDB_CMP_EQ = 0, QDB_CMP_NE = 1, QDB_CMP_LT = 2, QDB_CMP_LE = 3, QDB_CMP_GT = 4, QDB_CMP_GE = 5 };
+
+/* Register classes. */
+enum { QDB_REG_INT = 0, QDB_REG_F64 = 1, QDB_REG_BOOL = 2 };
+
+#define QDB_EXPR_MAX_INSNS 32
+#define QDB_EXPR_MAX_REGS 8
+
+typedef struct QdbExprInsn {
+    uint8_t Op;
+    uint8_t Dst;
+    uint8_t A;
+    uint8_t B;
+    uint8_t Class;     /* register class of the result (QDB_REG_*) */
+    uint8_t Pad[3];
+    int32_t Column;    /* QDB_EXPR_COLUMN: input column index */
+    int32_t Pad2;
+    int64_t ImmLo;
+    int64_t ImmHi;
+} QdbExprInsn;
+
+typedef struct QdbExprProgram {
+    int32_t NumInsns;
+    int32_t Result;    /* register holding the result */
+    QdbExprInsn Insns[QDB_EXPR_MAX_INSNS];
+} QdbExprProgram;
+
+/* ------------------------------------------------------------------------ */
+/* Filter kernel: GenFilterKernelAst (qdb/kernel/gen.cpp:911-1091).          */
+/* sel[i] = (input_sel == NULL || input_sel[i]) ? (pred(row i) IS TRUE) : 0, */
+/* written to rs->Selection (which must be a device buffer of RowCount      */
+/* bytes). Rows rejected upstream are not evaluated.                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct QdbColumnSchema {
+    int32_t NumColumns;
+    QdbValueType Types[64];
+} QdbColumnSchema;
+
+typedef struct QdbFilterSpec {
+    QdbColumnSchema Input;
+    QdbExprProgram Predicate;   /* result register class QDB_REG_BOOL */
+} QdbFilterSpec;
+
+typedef void (*QdbFilterFn)(TRowSet* rs, void* arena, void** regexes, const uint8_t* input_sel);
+
+/* ------------------------------------------------------------------------ */
+/* Project kernel: GenProjectKernelAst (qdb/kernel/gen.cpp:1373-1613).       */
+/* out[k] = data buffer of output k (RowCount * width bytes, device; a bool */
+/* output is an LSB bitmap), out[NumOutputs + k] = its validity bitmap      */
+/* (device) when the output is nullable (gen.cpp:1519-1535). Every row is   */
+/* evaluated when input_sel is NULL; otherwise only the selected rows.      */
+/* ------------------------------------------------------------------------ */
+
+#define QDB_PROJECT_MAX_OUTPUTS 8
+
+typedef struct QdbProjectSpec {
+    QdbColumnSchema Input;
+    int32_t NumOutputs;
+    QdbValueType OutputTypes[QDB_PROJECT_MAX_OUTPUTS];
+    QdbExprProgram Outputs[QDB_PROJECT_MAX_OUTPUTS];
+} QdbProjectSpec;
+
+typedef void (*QdbProjectFn)(TRowSet* in, void** out, void* arena, void** regexes, const uint8_t* input_sel);
+
+/* ------------------------------------------------------------------------ */
+/* Aggregate kernel: agg_dispatch / agg_finish_rowset                        */
+/* (qdb/kernel/gen.cpp:2075-2299, 2836-3132).                                */
+/* ------------------------------------------------------------------------ */
+
+enum {
+    QDB_AGG_COUNT_STAR = 0,
+    QDB_AGG_COUNT = 1,
+    QDB_AGG_SUM = 2,
+    QDB_AGG_MIN = 3,
+    QDB_AGG_MAX = 4
+};
+
+#define QDB_AGG_MAX_KEYS 4
+#define QDB_AGG_MAX_AGGS 16
+
+typedef struct QdbAggregateFn {
+    int32_t Func;       /* QDB_AGG_* */
+    int32_t Column;     /* argument column; -1 for COUNT(*) */
+} QdbAggregateFn;
+
+typedef struct QdbAggregateSpec {
+    QdbColumnSchema Input;
+    int32_t NumKeys;
+    int32_t KeyColumns[QDB_AGG_MAX_KEYS];
+    int32_t NumAggs;
+    QdbAggregateFn Aggs[QDB_AGG_MAX_AGGS];
+    /* 1 = a global aggregate (no keys): one group that exists over an empty
+     * input, COUNT 0 and SUM/MIN/MAX NULL (qdb/kernel/spec.h:85-89). */
+    int32_t Global;
+    /* Output buffers of agg_finish_rowset: 0 = device memory, 1 = host. */
+    int32_t OutputOnHost;
+    /* Optional fused stages, each equal to the separate operator: a
+     * predicate (filter kernel) and computed columns appended after the
+     * input columns (project kernel). NumComputed = 0 disables project. */
+    int32_t HasPredicate;
+    QdbExprProgram Predicate;
+    int32_t NumComputed;
+    QdbValueType ComputedTypes[QDB_PROJECT_MAX_OUTPUTS];
+    QdbExprProgram Computed[QDB_PROJECT_MAX_OUTPUTS];
+} QdbAggregateSpec;
+
+typedef int64_t (*QdbAggDispatchFn)(void* ht, TRowSet* batch, int64_t arg, int64_t op);
+typedef int64_t (*QdbAggFinishFn)(void* ht, TRowSet* out);
+
+/* TAggregateKernelMeta (qdb/exec/plan_data.h:173-205) plus the callables. */
+typedef struct QdbAggregateKernels {
+    QdbAggDispatchFn Dispatch;
+    QdbAggFinishFn FinishRowSet;
+    int64_t NumAggs;
+    int64_t StateWords;
+    int64_t RowBytes;       /* 0: no separate row layout on this device */
+    int32_t Slot;
+    /* Output schema of agg_finish_rowset: keys, then one column per agg. */
+    int32_t NumOutputColumns;
+    QdbValueType OutputTypes[QDB_AGG_MAX_KEYS + QDB_AGG_MAX_AGGS];
+} QdbAggregateKernels;
+
+/* ------------------------------------------------------------------------ */
+/* Row join: jr_dispatch / jr_materialize (qdb/kernel/join_gen.cpp:2017-2375, */
+/* qdb/kernel/join/join_rows.oz).                                            */
+/* ------------------------------------------------------------------------ */
+
+enum { QDB_JOIN_INNER = 0, QDB_JOIN_LEFT_OUTER = 1, QDB_JOIN_RIGHT_OUTER = 2 };
+
+#define QDB_JOIN_MAX_KEYS 2
+#define QDB_JOIN_MAX_FUSED 4
+
+typedef struct QdbRowJoinSpec {
+    QdbColumnSchema Build;
+    QdbColumnSchema Probe;
+    int32_t NumKeys;
+    int32_t BuildKeyColumns[QDB_JOIN_MAX_KEYS];
+    int32_t ProbeKeyColumns[QDB_JOIN_MAX_KEYS];
+    int32_t JoinType;        /* QDB_JOIN_* */
+    int32_t ProbeIsLeft;     /* output column order: probe columns first */
+    /* Optional global aggregate over the matched pairs, fused into Probe
+     * (COUNT(*) and SUM/MIN/MAX of a probe or build column), the same value
+     * an aggregate over jr_materialize's output computes. Column >= 0 names a
+     * probe column, Column <= -2 the build column (-Column - 2). */
+    int32_t NumFused;
+    QdbAggregateFn Fused[QDB_JOIN_MAX_FUSED];
+    /* Output buffers of jr_materialize: 0 = device memory, 1 = host. */
+    int32_t OutputOnHost;
+} QdbRowJoinSpec;
+
+typedef bool (*QdbJoinDispatchFn)(void* arena, void* table, void* scratch, void* pairs,
+    TRowSet* batch, const int64_t* key_columns, int64_t op);
+typedef int64_t (*QdbJoinMaterializeFn)(void* pairs, TRowSet* stream, int64_t start,
+    int64_t limit, TRowSet* out);
+
+typedef struct QdbRowJoinKernels {
+    QdbJoinDispatchFn Dispatch;
+    QdbJoinMaterializeFn Materialize;
+    int32_t Slot;
+    int32_t NumOutputColumns;
+    QdbValueType OutputTypes[128];
+} QdbRowJoinKernels;
+
+/* Fused global-aggregate results of the probes since Init, per fused agg:
+ * Count = contributing rows, Value = i128 sum (lo, hi) or the f64 bits of
+ * min/max/sum in Lo, Valid = 0 when the result is NULL. */
+typedef struct QdbFusedResult {
+    int64_t Lo;
+    int64_t Hi;
+    int64_t Count;
+    int64_t Valid;
+} QdbFusedResult;
+
+/* ------------------------------------------------------------------------ */
+/* Library entry points.                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Initialises the device runtime on device `device` (hipSetDevice, stream,
+ * memory pool). Returns 0 on success. Every other call initialises lazily. */
+int qdb_gpu_init(int device);
+/* Text of the last error on this thread ("" when none). */
+const char* qdb_gpu_last_error(void);
+/* Library version string. */
+const char* qdb_gpu_version(void);
+
+/* Compile = bind a spec to a trampoline slot and return its entry points.
+ * Return -1 on an unsupported spec (see qdb_gpu_last_error); on success the
+ * filter, project and group-hash compilers return their slot index (>= 0,
+ * for qdb_gpu_release_slot) and the aggregate and join compilers return 0
+ * with the slot in the kernels' Slot field. */
+int qdb_gpu_compile_filter(const QdbFilterSpec* spec, QdbFilterFn* out);
+int qdb_gpu_compile_project(const QdbProjectSpec* spec, QdbProjectFn* out);
+int qdb_gpu_compile_aggregate(const QdbAggregateSpec* spec, QdbAggregateKernels* out);
+int qdb_gpu_compile_row_join(const QdbRowJoinSpec* spec, QdbRowJoinKernels* out);
+/* The group-hash entry point: rh_hash of the key columns of every selected
+ * row into `hashes` (device), qdb/kernel/compiler.cpp:3057-3125. */
+typedef bool (*QdbGroupHashFn)(TRowSet* rs, uint64_t* hashes, const int64_t* key_cols);
+int qdb_gpu_compile_group_hash(const QdbColumnSchema* input, int32_t num_keys, QdbGroupHashFn* out);
+/* Frees the slot of a compiled kernel (its Slot field / the fn pointer). */
+void qdb_gpu_release_slot(int32_t kind, int32_t slot);
+enum { QDB_SLOT_FILTER = 0, QDB_SLOT_PROJECT = 1, QDB_SLOT_AGGREGATE = 2, QDB_SLOT_JOIN = 3, QDB_SLOT_HASH = 4 };
+
+/* Fused join aggregate results (see QdbRowJoinSpec.NumFused); `arena` is the
+ * join's RowArena blob. Returns the number of results written. */
+int64_t qdb_gpu_join_fused_results(void* arena, QdbFusedResult* out, int64_t max_out);
+
+/* Frees everything behind a join's arena/table/scratch/pairs blobs (the
+ * reference frees them with its allocation domain). */
+void qdb_gpu_join_destroy(void* arena, void* table, void* scratch, void* pairs);
+/* Frees a table behind a HashTable blob without finalizing it. */
+void qdb_gpu_aggregate_destroy(void* ht);
+
+/* Device memory (from the library's caching pool) and copies. */
+void* qdb_gpu_alloc(size_t bytes);
+void qdb_gpu_free(void* ptr);
+int qdb_gpu_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int qdb_gpu_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int qdb_gpu_memset(void* dst, int value, size_t bytes);
+int qdb_gpu_synchronize(void);
+
+/* Staging (qdb/io): copies a host TRowSet's fixed-width
+ * columns, masks and selection into HBM through pinned buffers with
+ * hipMemcpyAsync, producing a device TRowSet whose Destroy frees it.
+ * `types` gives each column's width. Returns 0 on success. */
+int qdb_gpu_stage_rowset(const TRowSet* host, const QdbColumnSchema* types, TRowSet* device_out);
+/* Copies a device TRowSet (as produced by agg_finish_rowset/jr_materialize)
+ * to host memory; the result's Destroy frees the host buffers. */
+int qdb_gpu_download_rowset(const TRowSet* device, const QdbValueType* types, TRowSet* host_out);
+
+/* Synthetic inputs generated in HBM from a seed (counter-based, identical to
+ * the host generator in qdb_gen.h). Column buffers are caller-allocated. */
+int qdb_gpu_gen_lineitem(uint64_t seed, int64_t row0, int64_t rows, int8_t* flag, int8_t* status,
+    int32_t* shipdate, int64_t* quantity, int64_t* price, int64_t* discount, int64_t* tax);
+int qdb_gpu_gen_join_build(uint64_t seed, int64_t row0, int64_t rows, int64_t* key, int32_t* segment);
+int qdb_gpu_gen_join_probe(uint64_t seed, int64_t row0, int64_t rows, int64_t build_rows, int64_t* fk, int64_t* value);
+int qdb_gpu_gen_zipf_groups(uint64_t seed, int64_t row0, int64_t rows, const double* zipf_cdf,
+    int64_t distinct, int64_t* key, int64_t* vi, double* vf);
+
+/* Kernel timing of the last launches, for roofline reporting: the average
+ * duration in milliseconds of the named kernel family ("agg_update",
+ * "join_probe", ...) measured with HIP events on the stream it runs on since
+ * the last reset, and the number of launches. */
+void qdb_gpu_timing_enable(int on);
+void qdb_gpu_timing_reset(void);
+double qdb_gpu_timing_avg_ms(const char* family, int64_t* launches);
+
+/* Sizes of the structs above (TColumn ... QdbFusedResult, in declaration
+ * order) so that bindings can check their layouts; returns the count. */
+int64_t qdb_gpu_struct_sizes(int64_t* out, int64_t n);
+
+/* ------------------------------------------------------------------------ */
+/* Host processors over the entry points above, mirroring qdb/exec:         */
+/* TAggregateProcessor (qdb/exec/aggregate_exec.cpp:52-156) and             */
+/* TRowJoinProcessor (qdb/exec/row_join_exec.cpp:22-276). Batches passed in */
+/* stay owned by the caller. Errors: -1 / NULL, text in                      */
+/* qdb_gpu_exec_last_error.                                                  */
+/* ------------------------------------------------------------------------ */
+
+const char* qdb_gpu_exec_last_error(void);
+/* row_layout != 0 starts with InitRows(initial_groups), else Init. */
+void* qdb_gpu_agg_processor_new(const QdbAggregateKernels* kernels, int64_t initial_capacity, int row_layout,
+    int64_t initial_groups);
+int qdb_gpu_agg_processor_add(void* processor, const TRowSet* batch);
+/* Finalizes into `out` (release it with qdb_gpu_rowset_release); returns rows. */
+int64_t qdb_gpu_agg_processor_finish(void* processor, TRowSet* out);
+int64_t qdb_gpu_agg_processor_groups(void* processor);
+void qdb_gpu_agg_processor_free(void* processor);
+
+void* qdb_gpu_join_processor_new(const QdbRowJoinKernels* kernels, int build_is_left, int64_t output_batch_rows);
+int qdb_gpu_join_processor_append(void* processor, const TRowSet* batch);
+int qdb_gpu_join_processor_build(void* processor);
+/* Probes and materializes the batch's matches into queued output batches. */
+int64_t qdb_gpu_join_processor_probe(void* processor, const TRowSet* batch);
+/* Probes only; returns the pair count, pairs readable until the next probe. */
+int64_t qdb_gpu_join_processor_probe_only(void* processor, const TRowSet* batch);
+const QdbPairBuffer* qdb_gpu_join_processor_pairs(void* processor);
+/* The processor's RowArena blob (for qdb_gpu_join_fused_results). */
+void* qdb_gpu_join_processor_arena(void* processor);
+/* Next queued output batch: 1 when one was returned, 0 when none is left. */
+int qdb_gpu_join_processor_next(void* processor, TRowSet* out);
+void qdb_gpu_join_processor_free(void* processor);
+/* Calls rs->Destroy when set. */
+void qdb_gpu_rowset_release(TRowSet* rs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QDB_GPU_H */

@@ -1,0 +1,928 @@
+// Hash aggregation on gfx950: agg_dispatch / agg_finish_rowset.
+//
+// Reference: agg_dispatch (qdb/kernel/gen.cpp:2075-2299), reducers
+// (gen.cpp:3301-3592), finalize (gen.cpp:2836-3132), op-codes and the
+// HashTable blob (qdb/kernel/compiler.h:208-233,603-616,
+// qdb/exec/aggregate_exec.cpp:18-39).
+//
+// Device design. A group's state is a record of 64-bit words, each with a
+// combine operation (add, min, max, f64 add), so that a row's contribution,
+// a wave's partial, a workgroup's partial and the global state all combine
+// the same way:
+//   - integer SUM: one wrapping i64 add word (the reference's i64 state);
+//   - DECIMAL SUM (exact i128): three add words holding bits [0,32), [32,64)
+//     and [64,128) of every value, recombined modulo 2^128 at the end;
+//   - f64 SUM: four 32-bit-chunk limbs (exact over a 128-bit window) plus an
+//     f64 side word for anything outside it, rounded once at the end;
+//   - MIN/MAX: i64 or order-encoded f64 words with atomic min/max;
+//   - COUNT and the group's row count: add words.
+// Rows are pre-aggregated per wave (ballot/match on equal keys), then in a
+// workgroup-private open-addressing table in LDS; keys that miss a full LDS
+// table, and every LDS group at the end, go to the HBM table (64-bit CAS on a
+// control word per slot, dense group records). A launch that fills the HBM
+// table records what it could not insert; the host grows the table and
+// replays exactly those rows and records, so every row is applied once.
+#include <hip/hip_runtime.h>
+
+#include <string.h>
+
+#include <memory>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "aggregate_kernels.h"
+#include "slots.h"
+
+namespace qdbgpu {
+
+namespace {
+
+using namespace agg;
+
+__device__ __forceinline__ void store_width(char* data, int width, unsigned int g, u128 v)
+{
+    switch (width) {
+    case 1: ((uint8_t*)data)[g] = (uint8_t)v; break;
+    case 2: ((uint16_t*)data)[g] = (uint16_t)v; break;
+    case 4: ((uint32_t*)data)[g] = (uint32_t)v; break;
+    case 8: ((uint64_t*)data)[g] = (uint64_t)v; break;
+    default: {
+        uint64_t* p = (uint64_t*)(data + 16 * (size_t)g);
+        p[0] = (uint64_t)v;
+        p[1] = (uint64_t)(v >> 64);
+    }
+    }
+}
+
+// Validity bits are written a byte (8 groups) per thread to avoid races.
+__global__ void __launch_bounds__(kBlock) k_agg_finalize(FinArgs F)
+{
+    const unsigned int nbytes = (F.groups + 7u) / 8u;
+    for (unsigned int b = blockIdx.x * blockDim.x + threadIdx.x; b < nbytes; b += gridDim.x * blockDim.x) {
+        uint8_t masks[QDB_AGG_MAX_KEYS + QDB_AGG_MAX_AGGS];
+        for (int c = 0; c < F.nkeys + F.nred; ++c) masks[c] = 0;
+        for (unsigned int g = b * 8u; g < b * 8u + 8u && g < F.groups; ++g) {
+            const uint64_t* r = F.recs + (size_t)g * F.R;
+            for (int f = 0; f < F.nkeys; ++f) {
+                const KeyFieldDesc kf = F.keys[f];
+                bool is_null = kf.nullable && ((r[kf.null_word] >> kf.null_bit) & 1ull);
+                u128 v = 0;
+                if (!is_null) {
+                    if (kf.width == 16) {
+                        v = ((u128)r[kf.word + 1] << 64) | (u128)r[kf.word];
+                    } else {
+                        uint64_t m = kf.width == 8 ? ~0ull : ((1ull << (8 * kf.width)) - 1ull);
+                        v = (r[kf.word] >> kf.shift) & m;
+                    }
+                }
+                if (kf.kind == QDB_KIND_BOOL) {
+                    // bool keys are emitted as an LSB bitmap in the data buffer.
+                    if (v) F.out_data[f][g >> 3] |= (char)(1u << (g & 7));
+                } else {
+                    store_width(F.out_data[f], F.key_out_width[f], g, v);
+                }
+                if (!is_null) masks[f] |= (uint8_t)(1u << (g & 7));
+            }
+            const uint64_t* s = r + F.KW;
+            const int64_t rows = (int64_t)s[F.rows_word];
+            for (int a = 0; a < F.nred; ++a) {
+                const RedOut ro = F.red[a];
+                const int col = F.nkeys + a;
+                const int64_t contributions = ro.wvalid >= 0 ? (int64_t)s[ro.wvalid] : rows;
+                bool valid = true;
+                u128 v = 0;
+                switch (ro.kind) {
+                case RK_COUNT_ROWS: v = (u128)(uint64_t)rows; break;
+                case RK_COUNT_VALID: v = (u128)s[ro.w0]; break;
+                case RK_SUM64:
+                    valid = contributions > 0;
+                    v = (u128)s[ro.w0];
+                    break;
+                case RK_SUM128:
+                    valid = contributions > 0;
+                    v = (u128)s[ro.w0] + ((u128)s[ro.w0 + 1] << 32) + ((u128)s[ro.w0 + 2] << 64);
+                    break;
+                case RK_SUMF64: {
+                    valid = contributions > 0;
+                    int64_t limb[kLimbs];
+                    for (int k = 0; k < kLimbs; ++k) limb[k] = (int64_t)s[ro.w0 + k];
+                    double exact = limbs_to_f64(limb);
+                    double side = __longlong_as_double((long long)s[ro.w0 + kLimbs]);
+                    double total = side != 0.0 || side != side ? exact + side : exact;
+                    v = (u128)(uint64_t)__double_as_longlong(total);
+                    break;
+                }
+                case RK_MINMAX_I64: {
+                    valid = contributions > 0;
+                    int64_t x = (int64_t)s[ro.w0];
+                    v = (u128)(i128)x;
+                    break;
+                }
+                case RK_MINMAX_I128:
+                    valid = contributions > 0;
+                    v = ((u128)s[ro.w0 + 1] << 64) | (u128)s[ro.w0];
+                    break;
+                case RK_MINMAX_F64: {
+                    valid = contributions > 0;
+                    uint64_t k = s[ro.w0];
+                    double d;
+                    if (ro.is_max) d = k == 0ull ? __longlong_as_double(0x7ff8000000000000ll) : f64_from_order(k);
+                    else d = k >= 0xfff8000000000001ull ? __longlong_as_double(0x7ff8000000000000ll) : f64_from_order(k);
+                    v = (u128)(uint64_t)__double_as_longlong(d);
+                    break;
+                }
+                default: break;
+                }
+                if (!valid) v = 0;
+                store_width(F.out_data[col], ro.out_width, g, v);
+                if (valid) masks[col] |= (uint8_t)(1u << (g & 7));
+            }
+        }
+        for (int c = 0; c < F.nkeys + F.nred; ++c) {
+            if (F.out_mask[c]) F.out_mask[c][b] = masks[c];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host: plans, tables, entry points.
+// ---------------------------------------------------------------------------
+
+// A compiled aggregate spec.
+struct AggPlan {
+    QdbAggregateSpec spec;
+    int KW = 1;
+    int S = 8;                 // word-count bucket (template argument)
+    AggArgs proto;             // launch arguments without per-batch fields
+    int slot_col[kMaxDevCols]; // device slot -> input column
+    int nslots = 0;
+    QdbExprProgram* d_pred = nullptr;
+    QdbExprProgram* d_progs = nullptr;
+    FinArgs fin;               // finalize arguments without buffers
+    int nout = 0;
+    QdbValueType out_types[QDB_AGG_MAX_KEYS + QDB_AGG_MAX_AGGS];
+    bool has_pairs = false;
+    uint64_t word_init_host[kMaxWords];
+
+    ~AggPlan()
+    {
+        dfree(d_pred);
+        dfree(d_progs);
+    }
+};
+
+// The device table behind a HashTable blob.
+struct AggTable {
+    AggPlan* plan = nullptr;
+    unsigned long long* ctrl = nullptr;
+    uint64_t capacity = 0;
+    uint64_t* recs = nullptr;
+    unsigned int max_groups = 0;
+    unsigned int* counters = nullptr;   // device [4]
+    unsigned int groups = 0;
+    int64_t expect_groups = 0;          // ExpectRowGroups hint
+    int match_rounds = 8;
+};
+
+// The HashTable blob (qdb/exec/aggregate_exec.cpp:18-39). The processor reads
+// Size; the first word holds the device table.
+struct HtBlob {
+    AggTable* table;
+    void* unused_ptrs[7];
+    int64_t Capacity;
+    int64_t Size;
+    int64_t NumAggs;
+    int64_t NumKeys;
+    int64_t KeySize;
+    uint8_t RowLayout[QDB_ROW_ARENA_SIZE + QDB_ROW_TABLE_SIZE + QDB_ROW_SCRATCH_SIZE];
+    int64_t RowSwitchGroups;
+    int64_t RowExpectedGroups;
+    int64_t RowRebuilds;
+    int64_t RowMovedFromSlots;
+};
+static_assert(sizeof(HtBlob) == QDB_HASH_TABLE_SIZE, "HashTable blob size");
+
+SlotTable<AggPlan>& agg_slots()
+{
+    static SlotTable<AggPlan>* t = new SlotTable<AggPlan>();
+    return *t;
+}
+
+uint64_t word_init_host(const WordDesc& d)
+{
+    if (d.kind == WK_PAIR_LO) return d.part ? 0ull : ~0ull;
+    if (d.kind == WK_PAIR_HI) return d.part ? 0x8000000000000000ull : 0x7fffffffffffffffull;
+    switch (d.op) {
+    case OP_MIN_I64: return 0x7fffffffffffffffull;
+    case OP_MAX_I64: return 0x8000000000000000ull;
+    case OP_MIN_U64: return ~0ull;
+    default: return 0ull;
+    }
+}
+
+int key_bytes_of(const QdbValueType& t)
+{
+    if (t.Kind == QDB_KIND_BOOL) return 1;
+    if (t.Kind == QDB_KIND_FLOAT) return t.Width == 4 ? 4 : 8;
+    return t.Width;
+}
+
+// Result types, AggregateResultType (qdb/plan/ops/aggregate.cpp:21-61).
+void output_types(const QdbAggregateSpec& spec, const QdbValueType* col_types, QdbValueType* out)
+{
+    int n = 0;
+    for (int k = 0; k < spec.NumKeys; ++k) out[n++] = col_types[spec.KeyColumns[k]];
+    for (int a = 0; a < spec.NumAggs; ++a) {
+        QdbValueType t;
+        memset(&t, 0, sizeof(t));
+        const QdbAggregateFn& fn = spec.Aggs[a];
+        if (fn.Func == QDB_AGG_COUNT_STAR || fn.Func == QDB_AGG_COUNT) {
+            t.Kind = QDB_KIND_SIGNED;
+            t.Width = 8;
+        } else {
+            const QdbValueType& in = col_types[fn.Column];
+            t.Nullable = in.Nullable || spec.Global;
+            if (in.Kind == QDB_KIND_FLOAT) {
+                t.Kind = QDB_KIND_FLOAT;
+                t.Width = 8;
+            } else if (in.Kind == QDB_KIND_DECIMAL) {
+                t.Kind = QDB_KIND_DECIMAL;
+                t.Scale = in.Scale;
+                t.Width = fn.Func == QDB_AGG_SUM ? 16 : in.Width;
+            } else if (in.Kind == QDB_KIND_DATE) {
+                t.Kind = QDB_KIND_DATE;
+                t.Width = 8;
+            } else {
+                t.Kind = QDB_KIND_SIGNED;
+                t.Width = 8;
+            }
+        }
+        out[n++] = t;
+    }
+}
+
+bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P)
+{
+    P->spec = spec;
+    AggArgs& A = P->proto;
+    memset(&A, 0, sizeof(A));
+    const int nin = spec.Input.NumColumns;
+    // Types of input and computed columns.
+    QdbValueType types[64 + QDB_PROJECT_MAX_OUTPUTS];
+    for (int c = 0; c < nin; ++c) types[c] = spec.Input.Types[c];
+    for (int k = 0; k < spec.NumComputed; ++k) types[nin + k] = spec.ComputedTypes[k];
+    const int ncols = nin + spec.NumComputed;
+    auto bad_col = [&](int c) { return c < 0 || c >= ncols; };
+
+    int col_slot[64];
+    for (int c = 0; c < 64; ++c) col_slot[c] = -1;
+    auto slot_of = [&](int c) -> int {
+        if (c < 0 || c >= nin) return -1;
+        if (col_slot[c] < 0) {
+            if (P->nslots >= kMaxDevCols) return -2;
+            col_slot[c] = P->nslots;
+            P->slot_col[P->nslots++] = c;
+        }
+        return col_slot[c];
+    };
+    // Programs: remap input columns to device slots.
+    auto remap = [&](const QdbExprProgram& in, QdbExprProgram* out) -> bool {
+        *out = in;
+        if (in.NumInsns < 0 || in.NumInsns > QDB_EXPR_MAX_INSNS) return fail("program has %d instructions", in.NumInsns);
+        for (int k = 0; k < in.NumInsns; ++k) {
+            QdbExprInsn& ins = out->Insns[k];
+            if (ins.Dst >= QDB_EXPR_MAX_REGS || ins.A >= QDB_EXPR_MAX_REGS || ins.B >= QDB_EXPR_MAX_REGS) {
+                return fail("program register out of range");
+            }
+            if (ins.Op == QDB_EXPR_TO_F64) return fail("decimal-to-double is not supported inside kernels");
+            if (ins.Op == QDB_EXPR_COLUMN) {
+                if (ins.Column < 0 || ins.Column >= nin) return fail("program column %d is not an input column", ins.Column);
+                int s = slot_of(ins.Column);
+                if (s < 0) return fail("too many columns in one aggregate (max %d)", kMaxDevCols);
+                ins.Column = s;
+            }
+        }
+        return true;
+    };
+    if (spec.NumKeys < 0 || spec.NumKeys > QDB_AGG_MAX_KEYS) return fail("bad key count %d", spec.NumKeys);
+    if (spec.NumAggs < 0 || spec.NumAggs > QDB_AGG_MAX_AGGS) return fail("bad aggregate count %d", spec.NumAggs);
+    if (spec.Global && spec.NumKeys != 0) return fail("a global aggregate has no keys");
+    if (spec.NumComputed < 0 || spec.NumComputed > QDB_PROJECT_MAX_OUTPUTS) return fail("bad computed count");
+
+    // Keys: the null bits of nullable fields first, then every field at its
+    // natural alignment (a 16-byte field on a word boundary).
+    int nnull = 0;
+    for (int k = 0; k < spec.NumKeys; ++k) {
+        const int c = spec.KeyColumns[k];
+        if (c >= 0 && c < nin && types[c].Nullable) ++nnull;
+    }
+    int byte = (nnull + 7) / 8;
+    int nullbit = 0;
+    for (int k = 0; k < spec.NumKeys; ++k) {
+        const int c = spec.KeyColumns[k];
+        if (bad_col(c) || c >= nin) return fail("key column %d must be an input column", c);
+        const QdbValueType& t = types[c];
+        if (t.Kind == QDB_KIND_STRING) return fail("string keys are not supported on this device");
+        const int w = key_bytes_of(t);
+        if (w != 1 && w != 2 && w != 4 && w != 8 && w != 16) return fail("key width %d unsupported", w);
+        const int align = w > 8 ? 8 : w;
+        byte = (byte + align - 1) / align * align;
+        KeyFieldDesc& kf = A.keys[k];
+        int s = slot_of(c);
+        if (s < 0) return fail("too many columns");
+        kf.slot = (int8_t)s;
+        kf.kind = t.Kind;
+        kf.width = (uint8_t)w;
+        kf.nullable = t.Nullable;
+        kf.word = (uint8_t)(byte / 8);
+        kf.shift = (uint8_t)((byte % 8) * 8);
+        if (t.Nullable) {
+            kf.null_word = (uint8_t)(nullbit / 64);
+            kf.null_bit = (uint8_t)(nullbit % 64);
+            ++nullbit;
+        }
+        byte += w;
+    }
+    P->KW = (byte + 7) / 8;
+    if (P->KW < 1) P->KW = 1;
+    if (P->KW > 4) return fail("a key of %d bytes exceeds the 32 bytes this device packs", byte);
+    A.nkeys = spec.NumKeys;
+
+    // Sources and state words.
+    int src_col[kMaxSrc];
+    A.nsrc = 0;
+    auto src_of = [&](int c) -> int {
+        for (int j = 0; j < A.nsrc; ++j) {
+            if (src_col[j] == c) return j;
+        }
+        if (A.nsrc >= kMaxSrc) return -1;
+        src_col[A.nsrc] = c;
+        SrcDesc& sd = A.src[A.nsrc];
+        if (c >= nin) {
+            sd.slot = -1;
+            sd.prog = (int8_t)(c - nin);
+        } else {
+            int s = slot_of(c);
+            if (s < 0) return -1;
+            sd.slot = (int8_t)s;
+            sd.prog = -1;
+        }
+        return A.nsrc++;
+    };
+    int nw = 0;
+    auto word = [&](uint8_t kind, uint8_t op, int src, int part) -> int {
+        if (nw >= kMaxWords) return -1;
+        WordDesc& d = A.words[nw];
+        d.kind = kind;
+        d.op = op;
+        d.src = (int8_t)src;
+        d.part = (uint8_t)part;
+        return nw++;
+    };
+    const int rows_word = word(WK_ROWS, OP_ADD, 0, 0);
+    FinArgs& F = P->fin;
+    memset(&F, 0, sizeof(F));
+    F.nred = spec.NumAggs;
+    F.rows_word = rows_word;
+    for (int a = 0; a < spec.NumAggs; ++a) {
+        const QdbAggregateFn& fn = spec.Aggs[a];
+        RedOut& ro = F.red[a];
+        ro.wvalid = -1;
+        if (fn.Func == QDB_AGG_COUNT_STAR) {
+            ro.kind = RK_COUNT_ROWS;
+            ro.out_width = 8;
+            continue;
+        }
+        if (bad_col(fn.Column)) return fail("aggregate %d: bad column %d", a, fn.Column);
+        const QdbValueType& t = types[fn.Column];
+        if (t.Kind == QDB_KIND_STRING) return fail("string aggregates are not supported on this device");
+        // An identical earlier aggregate (AVG lowers to SUM + COUNT of the same
+        // argument) shares its state words; COUNT of a non-nullable argument is
+        // the group's row count.
+        bool shared = false;
+        for (int b = 0; b < a && !shared; ++b) {
+            if (spec.Aggs[b].Func == fn.Func && spec.Aggs[b].Column == fn.Column) {
+                ro = F.red[b];
+                shared = true;
+            }
+        }
+        if (shared) continue;
+        if (fn.Func == QDB_AGG_COUNT && !t.Nullable) {
+            ro.kind = RK_COUNT_ROWS;
+            ro.out_width = 8;
+            continue;
+        }
+        const int src = src_of(fn.Column);
+        if (src < 0) return fail("too many aggregate arguments (max %d)", kMaxSrc);
+        if (fn.Func == QDB_AGG_COUNT) {
+            ro.kind = RK_COUNT_VALID;
+            ro.out_width = 8;
+            ro.w0 = (int16_t)word(WK_VALID, OP_ADD, src, 0);
+            continue;
+        }
+        const bool is_max = fn.Func == QDB_AGG_MAX;
+        if (fn.Func == QDB_AGG_SUM) {
+            if (t.Kind == QDB_KIND_DATE) return fail("function sum(DATE) does not exist");
+            if (t.Kind == QDB_KIND_FLOAT) {
+                ro.kind = RK_SUMF64;
+                ro.out_width = 8;
+                ro.w0 = (int16_t)word(WK_LIMB, OP_ADD, src, 0);
+                for (int k = 1; k < kLimbs; ++k) word(WK_LIMB, OP_ADD, src, k);
+                word(WK_OUTSIDE, OP_FADD, src, 0);
+            } else if (t.Kind == QDB_KIND_DECIMAL) {
+                ro.kind = RK_SUM128;
+                ro.out_width = 16;
+                ro.w0 = (int16_t)word(WK_I128_P0, OP_ADD, src, 0);
+                word(WK_I128_P1, OP_ADD, src, 0);
+                word(WK_I128_P2, OP_ADD, src, 0);
+            } else {
+                ro.kind = RK_SUM64;
+                ro.out_width = 8;
+                ro.w0 = (int16_t)word(WK_SUM64, OP_ADD, src, 0);
+            }
+        } else if (fn.Func == QDB_AGG_MIN || fn.Func == QDB_AGG_MAX) {
+            if (t.Kind == QDB_KIND_FLOAT) {
+                ro.kind = RK_MINMAX_F64;
+                ro.out_width = 8;
+                ro.is_max = is_max;
+                ro.w0 = (int16_t)word(is_max ? WK_MAX_F64 : WK_MIN_F64, is_max ? OP_MAX_U64 : OP_MIN_U64, src, 0);
+            } else if (t.Kind == QDB_KIND_DECIMAL && t.Width == 16) {
+                ro.kind = RK_MINMAX_I128;
+                ro.out_width = 16;
+                ro.is_max = is_max;
+                ro.w0 = (int16_t)word(WK_PAIR_LO, is_max ? OP_PAIR_MAX : OP_PAIR_MIN, src, is_max);
+                word(WK_PAIR_HI, OP_PAIR_HI, src, is_max);
+                word(WK_LOCK, OP_LOCK, src, 0);
+                P->has_pairs = true;
+            } else {
+                ro.kind = RK_MINMAX_I64;
+                ro.out_width = 8;
+                ro.is_max = is_max;
+                ro.w0 = (int16_t)word(is_max ? WK_MAX_I64 : WK_MIN_I64, is_max ? OP_MAX_I64 : OP_MIN_I64, src, 0);
+            }
+        } else {
+            return fail("aggregate %d: unknown function %d", a, fn.Func);
+        }
+        if (t.Nullable) ro.wvalid = (int16_t)word(WK_VALID, OP_ADD, src, 0);
+        if (ro.w0 < 0 || (t.Nullable && ro.wvalid < 0)) return fail("too many state words (max %d)", kMaxWords);
+    }
+    A.nwords = nw;
+    P->S = nw <= 8 ? 8 : nw <= 16 ? 16 : 32;
+    for (int w = 0; w < nw; ++w) P->word_init_host[w] = word_init_host(A.words[w]);
+
+    // Programs.
+    if (spec.HasPredicate) {
+        QdbExprProgram pred;
+        if (!remap(spec.Predicate, &pred)) return false;
+        P->d_pred = (QdbExprProgram*)upload(&pred, sizeof(pred));
+        if (!P->d_pred) return false;
+    }
+    if (spec.NumComputed > 0) {
+        std::vector<QdbExprProgram> progs((size_t)spec.NumComputed);
+        for (int k = 0; k < spec.NumComputed; ++k) {
+            if (!remap(spec.Computed[k], &progs[(size_t)k])) return false;
+        }
+        P->d_progs = (QdbExprProgram*)upload(progs.data(), progs.size() * sizeof(QdbExprProgram));
+        if (!P->d_progs) return false;
+    }
+
+    // Finalize layout.
+    F.KW = P->KW;
+    F.R = P->KW + nw;
+    F.nkeys = spec.NumKeys;
+    for (int k = 0; k < spec.NumKeys; ++k) {
+        F.keys[k] = A.keys[k];
+        F.key_out_width[k] = (uint8_t)key_bytes_of(types[spec.KeyColumns[k]]);
+    }
+    output_types(spec, types, P->out_types);
+    P->nout = spec.NumKeys + spec.NumAggs;
+    for (int a = 0; a < spec.NumAggs; ++a) F.red[a].nullable_out = P->out_types[spec.NumKeys + a].Nullable;
+    return true;
+}
+
+// Fills the per-batch fields of the launch arguments.
+bool bind_batch(const AggPlan* P, const TRowSet* batch, AggArgs* A)
+{
+    *A = P->proto;
+    for (int s = 0; s < P->nslots; ++s) {
+        const int c = P->slot_col[s];
+        if (c >= batch->ColumnCount) return fail("batch has %lld columns, the spec reads column %d", (long long)batch->ColumnCount, c);
+        const TColumn& col = batch->Columns[c];
+        const QdbValueType& t = P->spec.Input.Types[c];
+        DevCol& d = A->cols.c[s];
+        d.data = col.Data;
+        d.mask = col.Mask;
+        d.mask_off = col.MaskBitOffset;
+        d.bit_off = col.DataBitOffset;
+        d.kind = t.Kind;
+        d.width = (uint8_t)t.Width;
+        d.nullable = t.Nullable;
+    }
+    A->nrows = batch->RowCount;
+    A->sel = batch->Selection;
+    A->pred = P->d_pred;
+    A->progs = P->d_progs;
+    return true;
+}
+
+
+bool table_alloc(AggTable* T, uint64_t capacity)
+{
+    const AggPlan* P = T->plan;
+    const int R = P->KW + P->proto.nwords;
+    T->capacity = capacity;
+    T->max_groups = (unsigned int)(capacity / 2 < 0xfffffff0ull ? capacity / 2 : 0xfffffff0ull);
+    T->ctrl = (unsigned long long*)dalloc(capacity * 8);
+    T->recs = (uint64_t*)dalloc((size_t)T->max_groups * R * 8);
+    if (!T->ctrl || !T->recs) return false;
+    return zero_fill(T->ctrl, capacity * 8);
+}
+
+AggTable* table_new(AggPlan* P, int64_t groups_hint)
+{
+    AggTable* T = new AggTable();
+    T->plan = P;
+    uint64_t cap = 1024;
+    while (cap < (uint64_t)groups_hint * 2) cap *= 2;
+    T->counters = (unsigned int*)dalloc(16);
+    if (!T->counters || !zero_fill(T->counters, 16) || !table_alloc(T, cap)) {
+        dfree(T->counters);
+        dfree(T->ctrl);
+        dfree(T->recs);
+        delete T;
+        return nullptr;
+    }
+    // Few expected groups: wave-level matching pays; many: it does not.
+    T->match_rounds = groups_hint > (1 << 16) ? 1 : 8;
+    if (P->has_pairs) T->match_rounds = 0;
+    return T;
+}
+
+void table_free(AggTable* T)
+{
+    if (!T) return;
+    dfree(T->ctrl);
+    dfree(T->recs);
+    dfree(T->counters);
+    delete T;
+}
+
+// Doubles the HBM table until it holds `need` groups; re-inserts the groups.
+bool table_grow(AggTable* T, uint64_t need)
+{
+    const AggPlan* P = T->plan;
+    const int R = P->KW + P->proto.nwords;
+    uint64_t cap = T->capacity * 2;
+    while (cap / 2 < need) cap *= 2;
+    unsigned long long* old_ctrl = T->ctrl;
+    uint64_t* old_recs = T->recs;
+    if (!table_alloc(T, cap)) return false;
+    if (T->groups) {
+        if (!hip_ok(hipMemcpyAsync(T->recs, old_recs, (size_t)T->groups * R * 8, hipMemcpyDeviceToDevice, stream()), "grow copy")) {
+            return false;
+        }
+        agg::launch_rehash(P->KW, T->ctrl, T->capacity - 1, T->recs, R, T->groups);
+    }
+    if (!sync("grow")) return false;
+    dfree(old_ctrl);
+    dfree(old_recs);
+    return true;
+}
+
+void table_args(const AggTable* T, AggArgs* A)
+{
+    A->ctrl = T->ctrl;
+    A->cmask = T->capacity - 1;
+    A->recs = T->recs;
+    A->counters = T->counters;
+    A->max_groups = T->max_groups;
+    A->match_rounds = T->match_rounds;
+}
+
+bool read_counters(AggTable* T, unsigned int out[4])
+{
+    if (!hip_ok(hipMemcpyAsync(out, T->counters, 16, hipMemcpyDeviceToHost, stream()), "counters")) return false;
+    if (!sync("aggregate update")) return false;
+    T->groups = out[0] < T->max_groups ? out[0] : T->max_groups;
+    return true;
+}
+
+bool set_counters(AggTable* T)
+{
+    unsigned int h[4] = {T->groups, 0u, 0u, 0u};
+    return hip_ok(hipMemcpyAsync(T->counters, h, 16, hipMemcpyHostToDevice, stream()), "counters");
+}
+
+// agg_dispatch(Update): every selected row of the batch, exactly once.
+bool table_update(AggTable* T, const TRowSet* batch)
+{
+    const AggPlan* P = T->plan;
+    if (batch->RowCount <= 0) return true;
+    AggArgs A;
+    if (!bind_batch(P, batch, &A)) return false;
+    const int KW = P->KW, S = P->S;
+    const int R = KW + A.nwords;
+    // LDS table: as many groups as fit in 64 KiB, a power of two.
+    int lds_groups = 1024;
+    while (lds_groups > 16 && (size_t)lds_groups * R * 8 + (size_t)lds_groups * 2 * 4 + 16 > 64 * 1024) lds_groups /= 2;
+    if (P->has_pairs) lds_groups = 0;
+    A.lds_groups = lds_groups;
+    A.lds_slots = lds_groups * 2;
+    const size_t lds = lds_groups ? (size_t)lds_groups * R * 8 + (size_t)A.lds_slots * 4 + 16 : 0;
+    const int64_t chunk = (int64_t)1 << 30;
+    for (int64_t base = 0; base < batch->RowCount; base += chunk) {
+        const int64_t n = batch->RowCount - base < chunk ? batch->RowCount - base : chunk;
+        const int grid = grid_for(n, kBlock, 2);
+        // Spill buffers: every row of the launch, every LDS group of every
+        // workgroup, twice (a replay may spill again).
+        const size_t ov_recs_cap = (size_t)grid * (lds_groups ? lds_groups : 1) + 1024;
+        uint32_t* ov_rows[2] = {(uint32_t*)dalloc((size_t)n * 4), (uint32_t*)dalloc((size_t)n * 4)};
+        uint64_t* ov_recs[2] = {(uint64_t*)dalloc(ov_recs_cap * R * 8), (uint64_t*)dalloc(ov_recs_cap * R * 8)};
+        auto release = [&]() {
+            dfree(ov_rows[0]);
+            dfree(ov_rows[1]);
+            dfree(ov_recs[0]);
+            dfree(ov_recs[1]);
+        };
+        if (!ov_rows[0] || !ov_rows[1] || !ov_recs[0] || !ov_recs[1]) {
+            release();
+            return false;
+        }
+        table_args(T, &A);
+        A.row_base = base;
+        A.nrows = n;
+        A.row_index = nullptr;
+        A.ov_rows = ov_rows[0];
+        A.ov_recs = ov_recs[0];
+        A.ov_rec_cap = (unsigned int)ov_recs_cap;
+        if (!set_counters(T)) { release(); return false; }
+        {
+            TimedRegion timer("agg_update");
+            agg::launch_update(KW, S, A, grid, lds);
+        }
+        if (!hip_ok(hipGetLastError(), "agg update launch")) { release(); return false; }
+        unsigned int cnt[4];
+        if (!read_counters(T, cnt)) { release(); return false; }
+        int cur = 0;
+        while (cnt[1] || cnt[2]) {
+            const unsigned int spilled_rows = cnt[1];
+            const unsigned int spilled_recs = cnt[2] < ov_recs_cap ? cnt[2] : (unsigned int)ov_recs_cap;
+            if (!table_grow(T, (uint64_t)T->groups + spilled_rows + spilled_recs + 1024)) { release(); return false; }
+            const int nxt = cur ^ 1;
+            table_args(T, &A);
+            A.ov_rows = ov_rows[nxt];
+            A.ov_recs = ov_recs[nxt];
+            if (!set_counters(T)) { release(); return false; }
+            if (spilled_rows) {
+                AggArgs B = A;
+                B.row_index = ov_rows[cur];
+                B.nrows = spilled_rows;
+                B.row_base = 0;
+                B.lds_slots = 0;
+                B.lds_groups = 0;
+                const int g2 = grid_for(spilled_rows, kBlock, 2);
+                agg::launch_update(KW, S, B, g2, 0);
+            }
+            if (spilled_recs) {
+                agg::launch_records(KW, S, A, ov_recs[cur], spilled_recs);
+            }
+            if (!hip_ok(hipGetLastError(), "agg replay launch")) { release(); return false; }
+            if (!read_counters(T, cnt)) { release(); return false; }
+            cur = nxt;
+        }
+        release();
+    }
+    return true;
+}
+
+// agg_finish_rowset: groups -> columnar TRowSet; destroys the table.
+int64_t table_finish(AggTable* T, TRowSet* out)
+{
+    AggPlan* P = T->plan;
+    const int R = P->KW + P->proto.nwords;
+    if (P->spec.Global && T->groups == 0) {
+        // The one group of a global aggregate over an empty input.
+        std::vector<uint64_t> rec((size_t)R, 0ull);
+        for (int w = 0; w < P->proto.nwords; ++w) rec[(size_t)(P->KW + w)] = P->word_init_host[w];
+        if (!hip_ok(hipMemcpyAsync(T->recs, rec.data(), (size_t)R * 8, hipMemcpyHostToDevice, stream()), "global group")) return -1;
+        T->groups = 1;
+    }
+    const unsigned int n = T->groups;
+    OwnedRowSet* o = init_owned_rowset(out, n, P->nout);
+    FinArgs F = P->fin;
+    F.recs = T->recs;
+    F.groups = n;
+    const size_t mask_bytes = (size_t)((n + 7) / 8) + 8;
+    for (int c = 0; c < P->nout; ++c) {
+        const QdbValueType& t = P->out_types[c];
+        const size_t bytes = t.Kind == QDB_KIND_BOOL ? mask_bytes : (size_t)n * (size_t)t.Width + 8;
+        char* data = (char*)dalloc(bytes);
+        if (!data) return -1;
+        o->dev.push_back(data);
+        if (t.Kind == QDB_KIND_BOOL && !zero_fill(data, bytes)) return -1;
+        F.out_data[c] = data;
+        o->cols[(size_t)c].Data = data;
+        uint8_t* mask = nullptr;
+        if (t.Nullable) {
+            mask = (uint8_t*)dalloc(mask_bytes);
+            if (!mask) return -1;
+            o->dev.push_back(mask);
+        }
+        F.out_mask[c] = mask;
+        o->cols[(size_t)c].Mask = mask;
+    }
+    if (n > 0) {
+        TimedRegion timer("agg_finalize");
+        const int grid = grid_for((n + 7) / 8, kBlock, 8);
+        hipLaunchKernelGGL(k_agg_finalize, dim3(grid), dim3(kBlock), 0, stream(), F);
+        if (!hip_ok(hipGetLastError(), "agg finalize launch")) return -1;
+    }
+    if (!sync("agg finalize")) return -1;
+    if (P->spec.OutputOnHost && !move_rowset_to_host(out, P->out_types)) return -1;
+    return (int64_t)n;
+}
+
+int64_t agg_dispatch_impl(AggPlan* P, void* ht, TRowSet* batch, int64_t arg, int64_t op)
+{
+    clear_error();
+    HtBlob* B = (HtBlob*)ht;
+    if (!P) return op == QDB_AGG_OP_UPDATE ? -1 : 0;
+    switch (op) {
+    case QDB_AGG_OP_INIT:
+    case QDB_AGG_OP_INIT_ROWS: {
+        if (op == QDB_AGG_OP_INIT && (arg < 8 || (arg & (arg - 1)) != 0)) {
+            fail("Init needs a power-of-two capacity of at least 8, got %lld", (long long)arg);
+            return 0;
+        }
+        if (B->table) table_free(B->table);
+        memset(B, 0, sizeof(*B));
+        // Init's arg is swiss slots (load <= 7/8); InitRows' arg is groups.
+        const int64_t groups = op == QDB_AGG_OP_INIT ? arg - arg / 8 : arg;
+        AggTable* T = table_new(P, groups);
+        if (!T) return 0;
+        B->table = T;
+        B->Capacity = (int64_t)T->capacity;
+        B->Size = 0;
+        B->NumAggs = P->spec.NumAggs;
+        B->NumKeys = P->spec.NumKeys;
+        B->KeySize = P->KW * 8;
+        return 1;
+    }
+    case QDB_AGG_OP_SWITCH_TO_ROWS_AT:
+        // One layout serves every cardinality here; the hint is recorded.
+        B->RowSwitchGroups = arg;
+        return 1;
+    case QDB_AGG_OP_EXPECT_ROW_GROUPS:
+        B->RowExpectedGroups = arg;
+        if (B->table) {
+            B->table->expect_groups = arg;
+            if (arg > (1 << 16)) B->table->match_rounds = P->has_pairs ? 0 : 1;
+        }
+        return 1;
+    case QDB_AGG_OP_UPDATE: {
+        AggTable* T = B->table;
+        if (!T) {
+            fail("aggregate update before Init");
+            return -1;
+        }
+        if (!batch) return (int64_t)T->groups;
+        if (!table_update(T, batch)) return -1;
+        B->Size = T->groups;
+        B->Capacity = (int64_t)T->capacity;
+        return (int64_t)T->groups;
+    }
+    default:
+        return 0;
+    }
+}
+
+int64_t agg_finish_impl(AggPlan* P, void* ht, TRowSet* out)
+{
+    clear_error();
+    HtBlob* B = (HtBlob*)ht;
+    AggTable* T = B->table;
+    if (!P || !T) {
+        fail("aggregate finish without a table");
+        return -1;
+    }
+    int64_t n = table_finish(T, out);
+    table_free(T);
+    memset(B, 0, sizeof(*B));
+    return n;
+}
+
+template <int N>
+struct AggDispatchTramp {
+    static int64_t call(void* ht, TRowSet* batch, int64_t arg, int64_t op)
+    {
+        return agg_dispatch_impl(agg_slots().get(N), ht, batch, arg, op);
+    }
+};
+
+template <int N>
+struct AggFinishTramp {
+    static int64_t call(void* ht, TRowSet* out)
+    {
+        return agg_finish_impl(agg_slots().get(N), ht, out);
+    }
+};
+
+const auto kAggDispatchFns = make_tramp_table<QdbAggDispatchFn, AggDispatchTramp>(std::make_integer_sequence<int, kSlots>{});
+const auto kAggFinishFns = make_tramp_table<QdbAggFinishFn, AggFinishTramp>(std::make_integer_sequence<int, kSlots>{});
+
+}  // namespace
+}  // namespace qdbgpu
+
+using namespace qdbgpu;
+
+extern "C" int qdb_gpu_compile_aggregate(const QdbAggregateSpec* spec, QdbAggregateKernels* out)
+{
+    clear_error();
+    if (!spec || !out) return fail("null argument"), -1;
+    if (!ensure_init()) return -1;
+    AggPlan* P = new AggPlan();
+    if (!compile_plan(*spec, P)) {
+        delete P;
+        return -1;
+    }
+    const int slot = agg_slots().add(P);
+    if (slot < 0) {
+        delete P;
+        return fail("all %d aggregate slots are in use", kSlots), -1;
+    }
+    memset(out, 0, sizeof(*out));
+    out->Dispatch = kAggDispatchFns.fns[slot];
+    out->FinishRowSet = kAggFinishFns.fns[slot];
+    out->NumAggs = spec->NumAggs;
+    out->StateWords = P->proto.nwords;
+    out->RowBytes = (int64_t)(P->KW + P->proto.nwords) * 8;
+    out->Slot = slot;
+    out->NumOutputColumns = P->nout;
+    for (int c = 0; c < P->nout; ++c) out->OutputTypes[c] = P->out_types[c];
+    return 0;
+}
+
+extern "C" void qdb_gpu_aggregate_destroy(void* ht)
+{
+    HtBlob* B = (HtBlob*)ht;
+    if (!B) return;
+    table_free(B->table);
+    memset(B, 0, sizeof(*B));
+}
+
+namespace qdbgpu {
+void release_aggregate_slot(int slot)
+{
+    agg_slots().release(slot);
+}
+}  // namespace qdbgpu
+
+namespace qdbgpu {
+namespace agg {
+
+void launch_update_kw1(int S, const AggArgs& A, int grid, size_t lds);
+void launch_update_kw2(int S, const AggArgs& A, int grid, size_t lds);
+void launch_update_kw3(int S, const AggArgs& A, int grid, size_t lds);
+void launch_update_kw4(int S, const AggArgs& A, int grid, size_t lds);
+void launch_records_kw1(int S, const AggArgs& A, const uint64_t* recs, unsigned int n);
+void launch_records_kw2(int S, const AggArgs& A, const uint64_t* recs, unsigned int n);
+void launch_records_kw3(int S, const AggArgs& A, const uint64_t* recs, unsigned int n);
+void launch_records_kw4(int S, const AggArgs& A, const uint64_t* recs, unsigned int n);
+void launch_rehash_kw1(unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups);
+void launch_rehash_kw2(unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups);
+void launch_rehash_kw3(unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups);
+void launch_rehash_kw4(unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups);
+
+void launch_update(int KW, int S, const AggArgs& A, int grid, size_t lds)
+{
+    switch (KW) {
+    case 1: launch_update_kw1(S, A, grid, lds); break;
+    case 2: launch_update_kw2(S, A, grid, lds); break;
+    case 3: launch_update_kw3(S, A, grid, lds); break;
+    default: launch_update_kw4(S, A, grid, lds); break;
+    }
+}
+
+void launch_records(int KW, int S, const AggArgs& A, const uint64_t* recs, unsigned int n)
+{
+    switch (KW) {
+    case 1: launch_records_kw1(S, A, recs, n); break;
+    case 2: launch_records_kw2(S, A, recs, n); break;
+    case 3: launch_records_kw3(S, A, recs, n); break;
+    default: launch_records_kw4(S, A, recs, n); break;
+    }
+}
+
+void launch_rehash(int KW, unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups)
+{
+    switch (KW) {
+    case 1: launch_rehash_kw1(ctrl, cmask, recs, R, groups); break;
+    case 2: launch_rehash_kw2(ctrl, cmask, recs, R, groups); break;
+    case 3: launch_rehash_kw3(ctrl, cmask, recs, R, groups); break;
+    default: launch_rehash_kw4(ctrl, cmask, recs, R, groups); break;
+    }
+}
+
+}  // namespace agg
+}  // namespace qdbgpu

@@ -1,0 +1,656 @@
+// Aggregate kernels and launch arguments, shared by the per-key-width
+// translation units (aggregate_kw*.hip) and the host code (aggregate.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+#include "runtime.h"
+
+namespace qdbgpu {
+namespace agg {
+
+constexpr int kMaxSrc = 8;
+constexpr int kMaxWords = 32;
+constexpr int kBlock = 256;
+
+enum WordOp : uint8_t {
+    OP_ADD = 0,     // wrapping u64 add
+    OP_FADD = 1,    // f64 add
+    OP_MIN_I64 = 2,
+    OP_MAX_I64 = 3,
+    OP_MIN_U64 = 4,
+    OP_MAX_U64 = 5,
+    // i128 MIN/MAX: the LO word carries the op, then the HI word and a lock
+    // word; applied under the lock, never pre-aggregated.
+    OP_PAIR_MIN = 6,
+    OP_PAIR_MAX = 7,
+    OP_PAIR_HI = 8,
+    OP_LOCK = 9,
+};
+
+enum WordKind : uint8_t {
+    WK_ROWS = 0,       // 1 per row
+    WK_VALID = 1,      // 1 if the source is valid
+    WK_SUM64 = 2,      // i64 value
+    WK_I128_P0 = 3,    // bits [0,32) of the i128 value
+    WK_I128_P1 = 4,    // bits [32,64)
+    WK_I128_P2 = 5,    // bits [64,128)
+    WK_LIMB = 6,       // limb `part` of the f64 value
+    WK_OUTSIDE = 7,    // f64 value outside the limb window
+    WK_MIN_I64 = 8,
+    WK_MAX_I64 = 9,
+    WK_MIN_F64 = 10,
+    WK_MAX_F64 = 11,
+    WK_PAIR_LO = 12,
+    WK_PAIR_HI = 13,
+    WK_LOCK = 14,
+};
+
+struct WordDesc {
+    uint8_t kind;
+    uint8_t op;
+    int8_t src;
+    uint8_t part;
+};
+
+struct KeyFieldDesc {
+    int8_t slot;      // device column slot
+    uint8_t kind;
+    uint8_t width;    // bytes of the packed value (16 for i128)
+    uint8_t nullable;
+    uint8_t word;     // packed word index
+    uint8_t shift;    // bit offset inside the word
+    uint8_t null_word;
+    uint8_t null_bit;
+};
+
+struct SrcDesc {
+    int8_t slot;      // device column slot, or -1
+    int8_t prog;      // computed program index, or -1
+};
+
+// Everything an update launch needs, passed by value.
+struct AggArgs {
+    DevCols cols;
+    int64_t nrows;
+    int64_t row_base;          // first row of this launch
+    const uint8_t* sel;
+    const uint32_t* row_index; // replay: rows to apply (row_base-relative)
+    const QdbExprProgram* pred;
+    const QdbExprProgram* progs;
+    int nkeys;
+    KeyFieldDesc keys[QDB_AGG_MAX_KEYS];
+    int nsrc;
+    SrcDesc src[kMaxSrc];
+    int nwords;
+    WordDesc words[kMaxWords];
+    int lds_slots;             // power of two
+    int lds_groups;            // max groups in LDS
+    int match_rounds;
+    // HBM table
+    unsigned long long* ctrl;
+    uint64_t cmask;
+    uint64_t* recs;            // dense records: KW key words, then words
+    unsigned int* counters;    // [0] groups, [1] overflow rows, [2] overflow records
+    unsigned int max_groups;
+    // overflow
+    uint32_t* ov_rows;
+    uint64_t* ov_recs;
+    unsigned int ov_rec_cap;
+};
+
+__device__ __forceinline__ uint64_t identity_of(uint8_t op)
+{
+    switch (op) {
+    case OP_MIN_I64: return 0x7fffffffffffffffull;
+    case OP_MAX_I64: return 0x8000000000000000ull;
+    case OP_MIN_U64: return ~0ull;
+    case OP_PAIR_MIN: return ~0ull;
+    case OP_PAIR_HI: return 0ull;  // set per reducer at insert (see pair_hi_identity)
+    default: return 0ull;
+    }
+}
+
+// Initial value of a new group's word.
+__device__ __forceinline__ uint64_t word_init(const WordDesc& d)
+{
+    if (d.kind == WK_PAIR_LO) return d.part ? 0ull : ~0ull;
+    if (d.kind == WK_PAIR_HI) return d.part ? 0x8000000000000000ull : 0x7fffffffffffffffull;
+    if (d.kind == WK_LOCK) return 0ull;
+    return identity_of(d.op);
+}
+
+__device__ __forceinline__ uint64_t combine(uint8_t op, uint64_t a, uint64_t b)
+{
+    switch (op) {
+    case OP_ADD: return a + b;
+    case OP_FADD: return (uint64_t)__double_as_longlong(__longlong_as_double((long long)a) + __longlong_as_double((long long)b));
+    case OP_MIN_I64: return (int64_t)a < (int64_t)b ? a : b;
+    case OP_MAX_I64: return (int64_t)a > (int64_t)b ? a : b;
+    case OP_MIN_U64: return a < b ? a : b;
+    default: return a > b ? a : b;
+    }
+}
+
+__device__ __forceinline__ void atomic_apply_lds(uint8_t op, uint64_t* p, uint64_t v)
+{
+    if (v == identity_of(op) && op != OP_FADD) return;
+    switch (op) {
+    case OP_ADD: atomicAdd((unsigned long long*)p, (unsigned long long)v); break;
+    case OP_FADD:
+        if (v != 0) atomicAdd((double*)p, __longlong_as_double((long long)v));
+        break;
+    case OP_MIN_I64: atomicMin((long long*)p, (long long)v); break;
+    case OP_MAX_I64: atomicMax((long long*)p, (long long)v); break;
+    case OP_MIN_U64: atomicMin((unsigned long long*)p, (unsigned long long)v); break;
+    default: atomicMax((unsigned long long*)p, (unsigned long long)v); break;
+    }
+}
+
+__device__ __forceinline__ void atomic_apply_global(uint8_t op, uint64_t* p, uint64_t v)
+{
+    // Same operations; device-scope atomics on HBM.
+    atomic_apply_lds(op, p, v);
+}
+
+template <int KW>
+__device__ __forceinline__ bool keys_equal(const uint64_t (&a)[KW], const uint64_t (&b)[KW])
+{
+    bool eq = true;
+#pragma unroll
+    for (int w = 0; w < KW; ++w) eq = eq && a[w] == b[w];
+    return eq;
+}
+
+// One aggregate argument of a row, with the parts the state words take.
+struct SrcVal {
+    i128 i;
+    double f;
+    int64_t limb[kLimbs];
+    bool valid;
+    bool outside;
+};
+
+// The packed key of a row.
+template <int KW>
+__device__ __forceinline__ void row_key(const AggArgs& A, int64_t row, uint64_t (&key)[KW])
+{
+#pragma unroll
+    for (int w = 0; w < KW; ++w) key[w] = 0;
+    for (int f = 0; f < A.nkeys; ++f) {
+        const KeyFieldDesc kf = A.keys[f];
+        const DevCol& c = A.cols.c[kf.slot];
+        const bool valid = col_valid(c, row);
+        if (kf.nullable && !valid) {
+#pragma unroll
+            for (int w = 0; w < KW; ++w) {
+                if (w == kf.null_word) key[w] |= 1ull << kf.null_bit;
+            }
+            continue;
+        }
+        uint64_t lo, hi = 0;
+        if (kf.kind == QDB_KIND_FLOAT) {
+            lo = c.width == 4 ? (uint64_t)__float_as_uint(((const float*)c.data)[row]) : canonical_f64_bits(((const double*)c.data)[row]);
+            if (c.width == 4) {
+                uint32_t b = (uint32_t)lo;
+                if ((b & 0x7fffffffu) == 0) b = 0;
+                else if ((b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu) != 0) b = 0x7fc00000u;
+                lo = b;
+            }
+        } else if (kf.kind == QDB_KIND_BOOL) {
+            lo = (uint64_t)bit_at((const uint8_t*)c.data, row, c.bit_off);
+        } else if (kf.width == 16) {
+            const uint64_t* p = (const uint64_t*)(c.data + 16 * row);
+            lo = p[0];
+            hi = p[1];
+        } else {
+            switch (kf.width) {
+            case 1: lo = ((const uint8_t*)c.data)[row]; break;
+            case 2: lo = ((const uint16_t*)c.data)[row]; break;
+            case 4: lo = ((const uint32_t*)c.data)[row]; break;
+            default: lo = ((const uint64_t*)c.data)[row]; break;
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < KW; ++w) {
+            if (w == kf.word) key[w] |= lo << kf.shift;
+            if (kf.width == 16 && w == kf.word + 1) key[w] |= hi;
+        }
+    }
+}
+
+// A row's contribution to every state word; false when the row does not
+// take part (not selected, or rejected by the fused predicate).
+template <int KW, int S>
+__device__ __forceinline__ bool row_contrib(const AggArgs& A, int64_t row, uint64_t (&key)[KW], uint64_t (&c)[S])
+{
+    if (A.sel && !A.sel[row]) return false;
+    if (A.pred) {
+        DVal p = eval_program<true>(A.pred, A.cols, row);
+        if (!(p.valid && p.i != 0)) return false;
+    }
+    SrcVal s[kMaxSrc];
+#pragma unroll
+    for (int q = 0; q < kMaxSrc; ++q) {
+        s[q].i = 0;
+        s[q].f = 0.0;
+        s[q].valid = false;
+        s[q].outside = false;
+#pragma unroll
+        for (int k = 0; k < kLimbs; ++k) s[q].limb[k] = 0;
+    }
+    // One copy of the column reader and of the interpreter; the results are
+    // stored through static indices so that `s` stays in registers.
+#pragma unroll 1
+    for (int j = 0; j < A.nsrc; ++j) {
+        const SrcDesc sd = A.src[j];
+        DVal v = sd.prog >= 0 ? eval_program<true>(A.progs + sd.prog, A.cols, row) : read_col(A.cols.c[sd.slot], row);
+        SrcVal sv;
+        sv.i = v.i;
+        sv.f = v.f;
+        sv.valid = v.valid;
+        sv.outside = false;
+#pragma unroll
+        for (int k = 0; k < kLimbs; ++k) sv.limb[k] = 0;
+        if (v.valid && v.cls == QDB_REG_F64) {
+#pragma unroll
+            for (int k = 0; k < kLimbs; ++k) sv.limb[k] = f64_limb(v.f, k, &sv.outside);
+            if (sv.outside) {
+#pragma unroll
+                for (int k = 0; k < kLimbs; ++k) sv.limb[k] = 0;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxSrc; ++q) {
+            if (q == j) s[q] = sv;
+        }
+    }
+    row_key<KW>(A, row, key);
+#pragma unroll
+    for (int w = 0; w < S; ++w) {
+        if (w >= A.nwords) { c[w] = 0; continue; }
+        const WordDesc d = A.words[w];
+        SrcVal v = s[0];
+#pragma unroll
+        for (int j = 1; j < kMaxSrc; ++j) {
+            if (d.src == j) v = s[j];
+        }
+        uint64_t out = identity_of(d.op);
+        switch (d.kind) {
+        case WK_ROWS: out = 1; break;
+        case WK_VALID: out = v.valid ? 1 : 0; break;
+        case WK_SUM64: if (v.valid) out = (uint64_t)(int64_t)v.i; break;
+        case WK_I128_P0: if (v.valid) out = (uint64_t)(u128)v.i & 0xffffffffull; break;
+        case WK_I128_P1: if (v.valid) out = ((uint64_t)(u128)v.i >> 32); break;
+        case WK_I128_P2: if (v.valid) out = (uint64_t)((u128)v.i >> 64); break;
+        case WK_LIMB: {
+            int64_t l = v.limb[0];
+#pragma unroll
+            for (int k = 1; k < kLimbs; ++k) {
+                if (d.part == k) l = v.limb[k];
+            }
+            out = v.valid ? (uint64_t)l : 0ull;
+            break;
+        }
+        case WK_OUTSIDE:
+            out = (v.valid && v.outside) ? (uint64_t)__double_as_longlong(v.f) : 0ull;
+            break;
+        case WK_MIN_I64: if (v.valid) out = (uint64_t)(int64_t)v.i; break;
+        case WK_MAX_I64: if (v.valid) out = (uint64_t)(int64_t)v.i; break;
+        case WK_MIN_F64: if (v.valid) out = f64_min_key(v.f); break;
+        case WK_MAX_F64: if (v.valid) out = f64_max_key(v.f); break;
+        case WK_PAIR_LO:
+            out = d.part ? 0ull : ~0ull;  // part: 1 = MAX
+            if (v.valid) out = (uint64_t)(u128)v.i;
+            break;
+        case WK_PAIR_HI:
+            out = d.part ? 0x8000000000000000ull : 0x7fffffffffffffffull;
+            if (v.valid) out = (uint64_t)((u128)v.i >> 64);
+            break;
+        case WK_LOCK: out = 0; break;
+        default: break;
+        }
+        c[w] = out;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// HBM table
+// ---------------------------------------------------------------------------
+
+// Finds or inserts `key`; returns the group id, or -1 when the table is
+// full. A new record's words are set to their identities before the slot is
+// published, so every update after the lookup is an atomic combine.
+template <int KW, int S>
+__device__ int64_t global_find_or_insert(const AggArgs& A, const uint64_t (&key)[KW])
+{
+    const uint64_t h = key_place_hash<KW>(key);
+    const unsigned long long tag = (unsigned long long)(h >> 32) << 32;
+    uint64_t slot = h & A.cmask;
+    const int R = KW + A.nwords;
+    for (;;) {
+        unsigned long long c = __hip_atomic_load(&A.ctrl[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (c == 0ull) {
+            if (atomicCAS(&A.ctrl[slot], 0ull, 1ull) == 0ull) {
+                unsigned int gid = atomicAdd(&A.counters[0], 1u);
+                if (gid >= A.max_groups) {
+                    __hip_atomic_store(&A.ctrl[slot], 0ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    return -1;
+                }
+                uint64_t* r = A.recs + (size_t)gid * R;
+#pragma unroll
+                for (int w = 0; w < KW; ++w) __hip_atomic_store(&r[w], key[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int w = 0; w < A.nwords; ++w) {
+                    __hip_atomic_store(&r[KW + w], word_init(A.words[w]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                __hip_atomic_store(&A.ctrl[slot], tag | (unsigned long long)(gid + 2), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                return gid;
+            }
+            continue;  // lost the race: look at the slot again
+        }
+        if (c == 1ull) continue;  // being inserted
+        if ((c & 0xffffffff00000000ull) == tag) {
+            const unsigned int gid = (unsigned int)(c & 0xffffffffull) - 2u;
+            const uint64_t* r = A.recs + (size_t)gid * R;
+            uint64_t k2[KW];
+#pragma unroll
+            for (int w = 0; w < KW; ++w) k2[w] = __hip_atomic_load(&r[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (keys_equal<KW>(k2, key)) return gid;
+        }
+        slot = (slot + 1) & A.cmask;
+    }
+}
+
+// i128 MIN/MAX under the group's lock word. The lane that takes the lock
+// finishes its update in the same iteration, so the loop always progresses.
+static __device__ void pair_apply(uint64_t* lo, bool is_max, uint64_t vlo, uint64_t vhi)
+{
+    uint64_t* hi = lo + 1;
+    unsigned long long* lock = (unsigned long long*)(lo + 2);
+    bool done = false;
+    while (!done) {
+        if (atomicCAS(lock, 0ull, 1ull) == 0ull) {
+            i128 cur = (i128)(((u128)__hip_atomic_load(hi, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) << 64)
+                | (u128)__hip_atomic_load(lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            i128 v = (i128)(((u128)vhi << 64) | (u128)vlo);
+            if (is_max ? v > cur : v < cur) {
+                __hip_atomic_store(lo, vlo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(hi, vhi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __hip_atomic_store(lock, 0ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            done = true;
+        }
+    }
+}
+
+template <int KW, int S>
+__device__ __forceinline__ void global_apply(const AggArgs& A, int64_t gid, const uint64_t (&c)[S])
+{
+    uint64_t* r = A.recs + (size_t)gid * (KW + A.nwords) + KW;
+#pragma unroll
+    for (int w = 0; w < S; ++w) {
+        if (w >= A.nwords) continue;
+        const uint8_t op = A.words[w].op;
+        if (op == OP_PAIR_MIN || op == OP_PAIR_MAX) {
+            if (w + 1 < S) pair_apply(&r[w], op == OP_PAIR_MAX, c[w], c[w + 1]);
+        } else if (op != OP_PAIR_HI && op != OP_LOCK) {
+            atomic_apply_global(op, &r[w], c[w]);
+        }
+    }
+}
+
+// Applies a contribution to the HBM table, or records it for the replay
+// after the table has grown.
+template <int KW, int S>
+__device__ void global_upsert_or_spill(const AggArgs& A, const uint64_t (&key)[KW], const uint64_t (&c)[S],
+    int64_t row, bool is_row)
+{
+    int64_t gid = global_find_or_insert<KW, S>(A, key);
+    if (gid >= 0) {
+        global_apply<KW, S>(A, gid, c);
+        return;
+    }
+    if (is_row) {
+        unsigned int pos = atomicAdd(&A.counters[1], 1u);
+        A.ov_rows[pos] = (uint32_t)row;
+    } else {
+        unsigned int pos = atomicAdd(&A.counters[2], 1u);
+        if (pos < A.ov_rec_cap) {
+            uint64_t* r = A.ov_recs + (size_t)pos * (KW + A.nwords);
+#pragma unroll
+            for (int w = 0; w < KW; ++w) r[w] = key[w];
+#pragma unroll
+            for (int w = 0; w < S; ++w) {
+                if (w < A.nwords) r[KW + w] = c[w];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LDS table: ctrl[lds_slots] (u32: 0 empty, 1 locked, else lgid + 2), a group
+// counter, then dense records of KW + nwords words.
+// ---------------------------------------------------------------------------
+
+struct LdsTable {
+    unsigned int* ctrl;
+    unsigned int* count;
+    uint64_t* recs;
+};
+
+template <int KW, int S>
+__device__ int lds_find_or_insert(const AggArgs& A, LdsTable& T, const uint64_t (&key)[KW])
+{
+    const uint64_t h = key_place_hash<KW>(key);
+    const unsigned int mask = (unsigned int)A.lds_slots - 1u;
+    unsigned int slot = (unsigned int)(h >> 40) & mask;
+    const int R = KW + A.nwords;
+    for (unsigned int probes = 0; probes <= mask; ) {
+        unsigned int c = __hip_atomic_load(&T.ctrl[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (c == 0u) {
+            if (__hip_atomic_load(T.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= (unsigned int)A.lds_groups) return -1;
+            if (atomicCAS(&T.ctrl[slot], 0u, 1u) == 0u) {
+                unsigned int lg = atomicAdd(T.count, 1u);
+                if (lg >= (unsigned int)A.lds_groups) {
+                    __hip_atomic_store(&T.ctrl[slot], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    return -1;
+                }
+                uint64_t* r = T.recs + (size_t)lg * R;
+#pragma unroll
+                for (int w = 0; w < KW; ++w) r[w] = key[w];
+                for (int w = 0; w < A.nwords; ++w) r[KW + w] = word_init(A.words[w]);
+                __hip_atomic_store(&T.ctrl[slot], lg + 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                return (int)lg;
+            }
+            continue;
+        }
+        if (c == 1u) continue;
+        const unsigned int lg = c - 2u;
+        const uint64_t* r = T.recs + (size_t)lg * R;
+        bool eq = true;
+#pragma unroll
+        for (int w = 0; w < KW; ++w) eq = eq && r[w] == key[w];
+        if (eq) return (int)lg;
+        slot = (slot + 1u) & mask;
+        ++probes;
+    }
+    return -1;
+}
+
+template <int KW, int S>
+__device__ __forceinline__ void apply_contrib(const AggArgs& A, LdsTable& T, bool use_lds, const uint64_t (&key)[KW],
+    const uint64_t (&c)[S], int64_t row)
+{
+    if (use_lds) {
+        int lg = lds_find_or_insert<KW, S>(A, T, key);
+        if (lg >= 0) {
+            uint64_t* r = T.recs + (size_t)lg * (KW + A.nwords) + KW;
+#pragma unroll
+            for (int w = 0; w < S; ++w) {
+                if (w < A.nwords) atomic_apply_lds(A.words[w].op, &r[w], c[w]);
+            }
+            return;
+        }
+    }
+    global_upsert_or_spill<KW, S>(A, key, c, row, true);
+}
+
+// Butterfly reduction of one word over the lanes in `m` (others give the
+// identity); every lane ends with the combined value.
+__device__ __forceinline__ uint64_t wave_reduce(uint8_t op, uint64_t v, bool in)
+{
+    uint64_t x = in ? v : identity_of(op);
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        x = combine(op, x, shfl_xor_u64(x, off));
+    }
+    return x;
+}
+
+template <int KW, int S>
+__global__ void __launch_bounds__(kBlock) k_agg_update(AggArgs A)
+{
+    extern __shared__ uint64_t lds_raw[];
+    LdsTable T;
+    const bool use_lds = A.lds_slots > 0 && A.row_index == nullptr;
+    if (use_lds) {
+        T.recs = lds_raw;
+        T.ctrl = (unsigned int*)(lds_raw + (size_t)A.lds_groups * (KW + A.nwords));
+        T.count = T.ctrl + A.lds_slots;
+        for (int i = threadIdx.x; i < A.lds_slots; i += blockDim.x) T.ctrl[i] = 0u;
+        if (threadIdx.x == 0) *T.count = 0u;
+        __syncthreads();
+    }
+    const int lane = lane_id();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t start = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    // Every lane of a wave runs the same number of iterations so that the
+    // wave-level ballots see the whole wave.
+    const int64_t wave_start = start - lane;
+    for (int64_t base = wave_start; base < A.nrows; base += stride) {
+        const int64_t idx = base + lane;
+        bool valid = idx < A.nrows;
+        int64_t row = 0;
+        if (valid) row = A.row_index ? (int64_t)A.row_index[idx] : A.row_base + idx;
+        uint64_t key[KW];
+        uint64_t c[S];
+        if (valid) valid = row_contrib<KW, S>(A, row, key, c);
+        if (!valid) {
+#pragma unroll
+            for (int w = 0; w < KW; ++w) key[w] = 0;
+#pragma unroll
+            for (int w = 0; w < S; ++w) c[w] = 0;
+        }
+        uint64_t active = __ballot(valid);
+        for (int round = 0; round < A.match_rounds && active; ++round) {
+            const int leader = __ffsll((unsigned long long)active) - 1;
+            uint64_t lk[KW];
+#pragma unroll
+            for (int w = 0; w < KW; ++w) lk[w] = shfl_u64(key[w], leader);
+            const bool same = valid && ((active >> lane) & 1ull) && keys_equal<KW>(key, lk);
+            const uint64_t m = __ballot(same);
+            if (__popcll(m) > 1) {
+                uint64_t red[S];
+#pragma unroll
+                for (int w = 0; w < S; ++w) {
+                    red[w] = w < A.nwords ? wave_reduce(A.words[w].op, c[w], same) : 0;
+                }
+                if (lane == leader) apply_contrib<KW, S>(A, T, use_lds, lk, red, row);
+            } else if (lane == leader) {
+                apply_contrib<KW, S>(A, T, use_lds, key, c, row);
+            }
+            active &= ~m;
+            if (same) valid = false;
+        }
+        if (valid) apply_contrib<KW, S>(A, T, use_lds, key, c, row);
+    }
+    if (!use_lds) return;
+    __syncthreads();
+    // Flush the workgroup's groups to the HBM table.
+    const unsigned int n = *T.count < (unsigned int)A.lds_groups ? *T.count : (unsigned int)A.lds_groups;
+    const int R = KW + A.nwords;
+    for (unsigned int lg = threadIdx.x; lg < n; lg += blockDim.x) {
+        const uint64_t* r = T.recs + (size_t)lg * R;
+        uint64_t key[KW];
+        uint64_t c[S];
+#pragma unroll
+        for (int w = 0; w < KW; ++w) key[w] = r[w];
+#pragma unroll
+        for (int w = 0; w < S; ++w) c[w] = w < A.nwords ? r[KW + w] : 0;
+        global_upsert_or_spill<KW, S>(A, key, c, 0, false);
+    }
+}
+
+// Replays spilled records (key + words) into the grown HBM table.
+template <int KW, int S>
+__global__ void __launch_bounds__(kBlock) k_agg_apply_records(AggArgs A, const uint64_t* recs, unsigned int n)
+{
+    const int R = KW + A.nwords;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint64_t* r = recs + (size_t)i * R;
+        uint64_t key[KW];
+        uint64_t c[S];
+#pragma unroll
+        for (int w = 0; w < KW; ++w) key[w] = r[w];
+#pragma unroll
+        for (int w = 0; w < S; ++w) c[w] = w < A.nwords ? r[KW + w] : 0;
+        global_upsert_or_spill<KW, S>(A, key, c, 0, false);
+    }
+}
+
+// Re-inserts every group of the old records into a grown control array.
+template <int KW>
+__global__ void __launch_bounds__(kBlock) k_agg_rehash(unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R,
+    unsigned int groups)
+{
+    for (unsigned int g = blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += gridDim.x * blockDim.x) {
+        uint64_t key[KW];
+#pragma unroll
+        for (int w = 0; w < KW; ++w) key[w] = recs[(size_t)g * R + w];
+        const uint64_t h = key_place_hash<KW>(key);
+        const unsigned long long v = ((unsigned long long)(h >> 32) << 32) | (unsigned long long)(g + 2);
+        uint64_t slot = h & cmask;
+        while (atomicCAS(&ctrl[slot], 0ull, v) != 0ull) slot = (slot + 1) & cmask;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Finalize: records -> output columns.
+// ---------------------------------------------------------------------------
+
+enum RedKind : uint8_t { RK_COUNT_ROWS, RK_COUNT_VALID, RK_SUM64, RK_SUM128, RK_SUMF64, RK_MINMAX_I64, RK_MINMAX_F64, RK_MINMAX_I128 };
+
+struct RedOut {
+    uint8_t kind;
+    uint8_t out_width;    // bytes per output value
+    uint8_t nullable_out;
+    uint8_t is_max;
+    int16_t w0;           // first state word
+    int16_t wvalid;       // valid-count word, or -1 (then the rows word)
+};
+
+struct FinArgs {
+    const uint64_t* recs;
+    int R;
+    int KW;
+    unsigned int groups;
+    int nkeys;
+    KeyFieldDesc keys[QDB_AGG_MAX_KEYS];
+    uint8_t key_out_width[QDB_AGG_MAX_KEYS];
+    int nred;
+    RedOut red[QDB_AGG_MAX_AGGS];
+    int rows_word;
+    char* out_data[QDB_AGG_MAX_KEYS + QDB_AGG_MAX_AGGS];
+    uint8_t* out_mask[QDB_AGG_MAX_KEYS + QDB_AGG_MAX_AGGS];
+};
+
+
+// Launchers, one translation unit per key width (aggregate_kw<KW>.hip).
+void launch_update(int KW, int S, const AggArgs& A, int grid, size_t lds);
+void launch_records(int KW, int S, const AggArgs& A, const uint64_t* recs, unsigned int n);
+void launch_rehash(int KW, unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups);
+
+}  // namespace agg
+}  // namespace qdbgpu

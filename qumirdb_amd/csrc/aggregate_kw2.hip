@@ -1,0 +1,34 @@
+// Aggregate kernels instantiated for packed keys of 2 64-bit word(s); one
+// translation unit per key width so that they compile in parallel.
+#include "aggregate_kernels.h"
+
+namespace qdbgpu {
+namespace agg {
+
+void launch_update_kw2(int S, const AggArgs& A, int grid, size_t lds)
+{
+    switch (S) {
+    case 8: hipLaunchKernelGGL((k_agg_update<2, 8>), dim3(grid), dim3(kBlock), lds, stream(), A); break;
+    case 16: hipLaunchKernelGGL((k_agg_update<2, 16>), dim3(grid), dim3(kBlock), lds, stream(), A); break;
+    default: hipLaunchKernelGGL((k_agg_update<2, 32>), dim3(grid), dim3(kBlock), lds, stream(), A); break;
+    }
+}
+
+void launch_records_kw2(int S, const AggArgs& A, const uint64_t* recs, unsigned int n)
+{
+    const int grid = grid_for(n, kBlock, 8);
+    switch (S) {
+    case 8: hipLaunchKernelGGL((k_agg_apply_records<2, 8>), dim3(grid), dim3(kBlock), 0, stream(), A, recs, n); break;
+    case 16: hipLaunchKernelGGL((k_agg_apply_records<2, 16>), dim3(grid), dim3(kBlock), 0, stream(), A, recs, n); break;
+    default: hipLaunchKernelGGL((k_agg_apply_records<2, 32>), dim3(grid), dim3(kBlock), 0, stream(), A, recs, n); break;
+    }
+}
+
+void launch_rehash_kw2(unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups)
+{
+    const int grid = grid_for(groups, kBlock, 8);
+    hipLaunchKernelGGL((k_agg_rehash<2>), dim3(grid), dim3(kBlock), 0, stream(), ctrl, cmask, recs, R, groups);
+}
+
+}  // namespace agg
+}  // namespace qdbgpu

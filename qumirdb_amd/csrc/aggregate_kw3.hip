@@ -1,0 +1,34 @@
+// Aggregate kernels instantiated for packed keys of 3 64-bit word(s); one
+// translation unit per key width so that they compile in parallel.
+#include "aggregate_kernels.h"
+
+namespace qdbgpu {
+namespace agg {
+
+void launch_update_kw3(int S, const AggArgs& A, int grid, size_t lds)
+{
+    switch (S) {
+    case 8: hipLaunchKernelGGL((k_agg_update<3, 8>), dim3(grid), dim3(kBlock), lds, stream(), A); break;
+    case 16: hipLaunchKernelGGL((k_agg_update<3, 16>), dim3(grid), dim3(kBlock), lds, stream(), A); break;
+    default: hipLaunchKernelGGL((k_agg_update<3, 32>), dim3(grid), dim3(kBlock), lds, stream(), A); break;
+    }
+}
+
+void launch_records_kw3(int S, const AggArgs& A, const uint64_t* recs, unsigned int n)
+{
+    const int grid = grid_for(n, kBlock, 8);
+    switch (S) {
+    case 8: hipLaunchKernelGGL((k_agg_apply_records<3, 8>), dim3(grid), dim3(kBlock), 0, stream(), A, recs, n); break;
+    case 16: hipLaunchKernelGGL((k_agg_apply_records<3, 16>), dim3(grid), dim3(kBlock), 0, stream(), A, recs, n); break;
+    default: hipLaunchKernelGGL((k_agg_apply_records<3, 32>), dim3(grid), dim3(kBlock), 0, stream(), A, recs, n); break;
+    }
+}
+
+void launch_rehash_kw3(unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups)
+{
+    const int grid = grid_for(groups, kBlock, 8);
+    hipLaunchKernelGGL((k_agg_rehash<3>), dim3(grid), dim3(kBlock), 0, stream(), ctrl, cmask, recs, R, groups);
+}
+
+}  // namespace agg
+}  // namespace qdbgpu

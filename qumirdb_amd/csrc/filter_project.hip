@@ -1,0 +1,420 @@
+// Filter, project and group-hash kernels.
+//
+// Filter: GenFilterKernelAst (qdb/kernel/gen.cpp:911-1091), called as
+// filter(rs, arena, regexes, input_sel) (qdb/kernel/compiler.cpp:2252-2260,
+// qdb/exec/filter_exec.cpp:27-70): rs->Selection[i] = the predicate IS TRUE
+// for the rows input_sel selects (all rows when NULL), 0 for the others.
+// Project: GenProjectKernelAst (gen.cpp:1373-1613), project(in, out, arena,
+// regexes, input_sel) (compiler.cpp:2329-2336): out[k] data buffers,
+// out[n + k] validity of nullable outputs; every row is evaluated when
+// input_sel is NULL, else the selected rows.
+// Group hash: CompileAggregateHash (compiler.cpp:3057-3125), rh_hash of the
+// key columns of every selected row (HashKeyValue, gen.cpp:233-379).
+#include <hip/hip_runtime.h>
+
+#include <string.h>
+
+#include <vector>
+
+#include "device.h"
+#include "runtime.h"
+#include "slots.h"
+
+namespace qdbgpu {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct ExprPlan {
+    QdbColumnSchema input;
+    int nslots = 0;
+    int slot_col[kMaxDevCols];
+    int nprogs = 0;
+    QdbExprProgram* d_progs = nullptr;
+    QdbValueType out_types[QDB_PROJECT_MAX_OUTPUTS];
+    ~ExprPlan() { dfree(d_progs); }
+};
+
+// Remaps the programs' input columns to device slots and uploads them.
+bool compile_programs(const QdbColumnSchema& input, const QdbExprProgram* progs, int n, ExprPlan* P)
+{
+    P->input = input;
+    int col_slot[64];
+    for (int c = 0; c < 64; ++c) col_slot[c] = -1;
+    std::vector<QdbExprProgram> out((size_t)n);
+    for (int p = 0; p < n; ++p) {
+        out[(size_t)p] = progs[p];
+        QdbExprProgram& q = out[(size_t)p];
+        if (q.NumInsns < 0 || q.NumInsns > QDB_EXPR_MAX_INSNS) return fail("program has %d instructions", q.NumInsns);
+        if (q.Result < 0 || q.Result >= QDB_EXPR_MAX_REGS) return fail("program result register out of range");
+        for (int k = 0; k < q.NumInsns; ++k) {
+            QdbExprInsn& in = q.Insns[k];
+            if (in.Dst >= QDB_EXPR_MAX_REGS || in.A >= QDB_EXPR_MAX_REGS || in.B >= QDB_EXPR_MAX_REGS) {
+                return fail("program register out of range");
+            }
+            if (in.Op == QDB_EXPR_TO_F64) return fail("decimal-to-double is not supported inside kernels");
+            if (in.Op != QDB_EXPR_COLUMN) continue;
+            if (in.Column < 0 || in.Column >= input.NumColumns) return fail("program column %d out of range", in.Column);
+            const QdbValueType& t = input.Types[in.Column];
+            if (t.Kind == QDB_KIND_STRING) return fail("string columns are not supported on this device");
+            if (col_slot[in.Column] < 0) {
+                if (P->nslots >= kMaxDevCols) return fail("too many columns in one program (max %d)", kMaxDevCols);
+                col_slot[in.Column] = P->nslots;
+                P->slot_col[P->nslots++] = in.Column;
+            }
+            in.Column = col_slot[in.Column];
+        }
+    }
+    P->nprogs = n;
+    if (n > 0) {
+        P->d_progs = (QdbExprProgram*)upload(out.data(), out.size() * sizeof(QdbExprProgram));
+        if (!P->d_progs) return false;
+    }
+    return true;
+}
+
+bool bind_cols(const ExprPlan* P, const TRowSet* rs, DevCols* cols)
+{
+    memset(cols, 0, sizeof(*cols));
+    for (int s = 0; s < P->nslots; ++s) {
+        const int c = P->slot_col[s];
+        if (c >= rs->ColumnCount) return fail("batch has %lld columns, the program reads column %d", (long long)rs->ColumnCount, c);
+        const TColumn& col = rs->Columns[c];
+        const QdbValueType& t = P->input.Types[c];
+        DevCol& d = cols->c[s];
+        d.data = col.Data;
+        d.mask = col.Mask;
+        d.mask_off = col.MaskBitOffset;
+        d.bit_off = col.DataBitOffset;
+        d.kind = t.Kind;
+        d.width = (uint8_t)t.Width;
+        d.nullable = t.Nullable;
+    }
+    return true;
+}
+
+__global__ void __launch_bounds__(kBlock) k_filter(DevCols cols, const QdbExprProgram* prog, int64_t n,
+    const uint8_t* in_sel, uint8_t* out_sel)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint8_t keep = 0;
+        if (!in_sel || in_sel[i]) {
+            DVal v = eval_program<true>(prog, cols, i);
+            keep = (v.valid && v.i != 0) ? 1 : 0;
+        }
+        out_sel[i] = keep;
+    }
+}
+
+struct ProjectOut {
+    char* data[QDB_PROJECT_MAX_OUTPUTS];
+    uint8_t* mask[QDB_PROJECT_MAX_OUTPUTS];
+    uint8_t kind[QDB_PROJECT_MAX_OUTPUTS];
+    uint8_t width[QDB_PROJECT_MAX_OUTPUTS];
+};
+
+// Writes 64 bits of a bitmap for the 64 rows starting at `base` (a multiple
+// of 64); the last word writes only the bytes inside the buffer.
+__device__ __forceinline__ void store_bits(uint8_t* bitmap, int64_t base, int64_t n, uint64_t bits, uint64_t covered)
+{
+    const int64_t first_byte = base >> 3;
+    const int64_t rows = n - base < 64 ? n - base : 64;
+    const int nbytes = (int)((rows + 7) >> 3);
+    for (int b = 0; b < nbytes; ++b) {
+        const uint8_t cov = (uint8_t)(covered >> (8 * b));
+        if (cov == 0) continue;
+        const uint8_t val = (uint8_t)(bits >> (8 * b));
+        if (cov == 0xff) bitmap[first_byte + b] = val;
+        else bitmap[first_byte + b] = (uint8_t)((bitmap[first_byte + b] & ~cov) | (val & cov));
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_project(DevCols cols, const QdbExprProgram* progs, int nout, ProjectOut O,
+    int64_t n, const uint8_t* in_sel)
+{
+    const int lane = lane_id();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // Waves cover 64 consecutive rows at multiples of 64 so that bitmap words
+    // are written whole by one lane.
+    for (int64_t base = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) - lane; base < n; base += stride) {
+        const int64_t i = base + lane;
+        const bool in_range = i < n;
+        const bool eval = in_range && (!in_sel || in_sel[i]);
+        const uint64_t evaluated = __ballot(eval);
+#pragma unroll 1
+        for (int k = 0; k < nout; ++k) {
+            DVal v;
+            v.valid = 0;
+            v.i = 0;
+            v.f = 0.0;
+            v.cls = 0;
+            if (eval) v = eval_program<true>(progs + k, cols, i);
+            if (O.kind[k] == QDB_KIND_BOOL) {
+                const uint64_t bits = __ballot(eval && v.valid && v.i != 0);
+                if (lane == 0) store_bits((uint8_t*)O.data[k], base, n, bits, evaluated);
+            } else if (eval) {
+                char* d = O.data[k];
+                if (O.kind[k] == QDB_KIND_FLOAT) {
+                    double f = v.cls == QDB_REG_F64 ? v.f : (double)v.i;
+                    if (!v.valid) f = 0.0;
+                    if (O.width[k] == 4) ((float*)d)[i] = (float)f;
+                    else ((double*)d)[i] = f;
+                } else {
+                    const u128 x = v.valid ? (u128)v.i : 0;
+                    switch (O.width[k]) {
+                    case 1: ((uint8_t*)d)[i] = (uint8_t)x; break;
+                    case 2: ((uint16_t*)d)[i] = (uint16_t)x; break;
+                    case 4: ((uint32_t*)d)[i] = (uint32_t)x; break;
+                    case 8: ((uint64_t*)d)[i] = (uint64_t)x; break;
+                    default: {
+                        uint64_t* p = (uint64_t*)(d + 16 * i);
+                        p[0] = (uint64_t)x;
+                        p[1] = (uint64_t)(x >> 64);
+                    }
+                    }
+                }
+            }
+            if (O.mask[k]) {
+                const uint64_t bits = __ballot(eval && v.valid);
+                if (lane == 0) store_bits(O.mask[k], base, n, bits, evaluated);
+            }
+        }
+    }
+}
+
+struct HashPlan {
+    QdbColumnSchema input;
+    int nkeys = 0;
+};
+
+struct HashCols {
+    DevCol c[QDB_AGG_MAX_KEYS];
+};
+
+__global__ void __launch_bounds__(kBlock) k_group_hash(HashCols cols, int nkeys, bool scalar, int64_t n, const uint8_t* sel,
+    uint64_t* hashes)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (sel && !sel[i]) continue;
+        uint64_t h;
+        if (scalar) {
+            h = leaf_hash(cols.c[0], i);
+        } else {
+            h = 0;
+            for (int k = 0; k < nkeys; ++k) {
+                const uint64_t fh = col_valid(cols.c[k], i) ? leaf_hash(cols.c[k], i) : 0ull;
+                h = hash_combine(h, fh);
+            }
+        }
+        hashes[i] = h;
+    }
+}
+
+SlotTable<ExprPlan>& filter_slots()
+{
+    static SlotTable<ExprPlan>* t = new SlotTable<ExprPlan>();
+    return *t;
+}
+
+SlotTable<ExprPlan>& project_slots()
+{
+    static SlotTable<ExprPlan>* t = new SlotTable<ExprPlan>();
+    return *t;
+}
+
+SlotTable<HashPlan>& hash_slots()
+{
+    static SlotTable<HashPlan>* t = new SlotTable<HashPlan>();
+    return *t;
+}
+
+void filter_impl(ExprPlan* P, TRowSet* rs, const uint8_t* input_sel)
+{
+    clear_error();
+    if (!P || !rs || rs->RowCount <= 0) return;
+    if (!rs->Selection) {
+        fail("filter: the rowset needs a device Selection buffer for the output");
+        return;
+    }
+    DevCols cols;
+    if (!bind_cols(P, rs, &cols)) return;
+    {
+        TimedRegion timer("filter");
+        hipLaunchKernelGGL(k_filter, dim3(grid_for(rs->RowCount, kBlock, 8)), dim3(kBlock), 0, stream(), cols, P->d_progs,
+            rs->RowCount, input_sel, rs->Selection);
+    }
+    if (hip_ok(hipGetLastError(), "filter launch")) (void)sync("filter");
+}
+
+void project_impl(ExprPlan* P, TRowSet* in, void** out, const uint8_t* input_sel)
+{
+    clear_error();
+    if (!P || !in || in->RowCount <= 0) return;
+    DevCols cols;
+    if (!bind_cols(P, in, &cols)) return;
+    ProjectOut O;
+    memset(&O, 0, sizeof(O));
+    for (int k = 0; k < P->nprogs; ++k) {
+        O.data[k] = (char*)out[k];
+        O.mask[k] = P->out_types[k].Nullable ? (uint8_t*)out[P->nprogs + k] : nullptr;
+        O.kind[k] = P->out_types[k].Kind;
+        O.width[k] = (uint8_t)P->out_types[k].Width;
+    }
+    {
+        TimedRegion timer("project");
+        hipLaunchKernelGGL(k_project, dim3(grid_for(in->RowCount, kBlock, 8)), dim3(kBlock), 0, stream(), cols, P->d_progs,
+            P->nprogs, O, in->RowCount, input_sel);
+    }
+    if (hip_ok(hipGetLastError(), "project launch")) (void)sync("project");
+}
+
+bool hash_impl(HashPlan* P, TRowSet* rs, uint64_t* hashes, const int64_t* key_cols)
+{
+    clear_error();
+    if (!P || !rs) return false;
+    if (rs->RowCount <= 0) return true;
+    HashCols cols;
+    memset(&cols, 0, sizeof(cols));
+    bool scalar = P->nkeys == 1;
+    for (int k = 0; k < P->nkeys; ++k) {
+        const int64_t c = key_cols[k];
+        if (c < 0 || c >= rs->ColumnCount || c >= P->input.NumColumns) return fail("hash key column %lld out of range", (long long)c);
+        const TColumn& col = rs->Columns[c];
+        const QdbValueType& t = P->input.Types[c];
+        if (t.Kind == QDB_KIND_STRING) return fail("string keys are not supported on this device");
+        DevCol& d = cols.c[k];
+        d.data = col.Data;
+        d.mask = col.Mask;
+        d.mask_off = col.MaskBitOffset;
+        d.bit_off = col.DataBitOffset;
+        d.kind = t.Kind;
+        d.width = (uint8_t)t.Width;
+        d.nullable = t.Nullable;
+        if (t.Nullable) scalar = false;
+    }
+    hipLaunchKernelGGL(k_group_hash, dim3(grid_for(rs->RowCount, kBlock, 8)), dim3(kBlock), 0, stream(), cols, P->nkeys, scalar,
+        rs->RowCount, rs->Selection, hashes);
+    return hip_ok(hipGetLastError(), "hash launch") && sync("hash");
+}
+
+template <int N>
+struct FilterTramp {
+    static void call(TRowSet* rs, void* arena, void** regexes, const uint8_t* input_sel)
+    {
+        (void)arena;
+        (void)regexes;
+        filter_impl(filter_slots().get(N), rs, input_sel);
+    }
+};
+
+template <int N>
+struct ProjectTramp {
+    static void call(TRowSet* in, void** out, void* arena, void** regexes, const uint8_t* input_sel)
+    {
+        (void)arena;
+        (void)regexes;
+        project_impl(project_slots().get(N), in, out, input_sel);
+    }
+};
+
+template <int N>
+struct HashTramp {
+    static bool call(TRowSet* rs, uint64_t* hashes, const int64_t* key_cols)
+    {
+        return hash_impl(hash_slots().get(N), rs, hashes, key_cols);
+    }
+};
+
+const auto kFilterFns = make_tramp_table<QdbFilterFn, FilterTramp>(std::make_integer_sequence<int, kSlots>{});
+const auto kProjectFns = make_tramp_table<QdbProjectFn, ProjectTramp>(std::make_integer_sequence<int, kSlots>{});
+const auto kHashFns = make_tramp_table<QdbGroupHashFn, HashTramp>(std::make_integer_sequence<int, kSlots>{});
+
+}  // namespace
+
+void release_aggregate_slot(int slot);
+void release_join_slot(int slot);
+
+}  // namespace qdbgpu
+
+using namespace qdbgpu;
+
+extern "C" int qdb_gpu_compile_filter(const QdbFilterSpec* spec, QdbFilterFn* out)
+{
+    clear_error();
+    if (!spec || !out) return fail("null argument"), -1;
+    if (!ensure_init()) return -1;
+    ExprPlan* P = new ExprPlan();
+    if (!compile_programs(spec->Input, &spec->Predicate, 1, P)) {
+        delete P;
+        return -1;
+    }
+    const int slot = filter_slots().add(P);
+    if (slot < 0) {
+        delete P;
+        return fail("all filter slots are in use"), -1;
+    }
+    *out = kFilterFns.fns[slot];
+    return slot;
+}
+
+extern "C" int qdb_gpu_compile_project(const QdbProjectSpec* spec, QdbProjectFn* out)
+{
+    clear_error();
+    if (!spec || !out) return fail("null argument"), -1;
+    if (spec->NumOutputs < 1 || spec->NumOutputs > QDB_PROJECT_MAX_OUTPUTS) return fail("bad output count %d", spec->NumOutputs), -1;
+    if (!ensure_init()) return -1;
+    ExprPlan* P = new ExprPlan();
+    for (int k = 0; k < spec->NumOutputs; ++k) {
+        P->out_types[k] = spec->OutputTypes[k];
+        const QdbValueType& t = spec->OutputTypes[k];
+        const bool ok_width = t.Kind == QDB_KIND_BOOL || t.Width == 1 || t.Width == 2 || t.Width == 4 || t.Width == 8 || t.Width == 16;
+        if (!ok_width || t.Kind == QDB_KIND_STRING) {
+            delete P;
+            return fail("project output %d: unsupported type", k), -1;
+        }
+    }
+    if (!compile_programs(spec->Input, spec->Outputs, spec->NumOutputs, P)) {
+        delete P;
+        return -1;
+    }
+    const int slot = project_slots().add(P);
+    if (slot < 0) {
+        delete P;
+        return fail("all project slots are in use"), -1;
+    }
+    *out = kProjectFns.fns[slot];
+    return slot;
+}
+
+extern "C" int qdb_gpu_compile_group_hash(const QdbColumnSchema* input, int32_t num_keys, QdbGroupHashFn* out)
+{
+    clear_error();
+    if (!input || !out) return fail("null argument"), -1;
+    if (num_keys < 1 || num_keys > QDB_AGG_MAX_KEYS) return fail("bad key count %d", num_keys), -1;
+    if (!ensure_init()) return -1;
+    HashPlan* P = new HashPlan();
+    P->input = *input;
+    P->nkeys = num_keys;
+    const int slot = hash_slots().add(P);
+    if (slot < 0) {
+        delete P;
+        return fail("all hash slots are in use"), -1;
+    }
+    *out = kHashFns.fns[slot];
+    return slot;
+}
+
+extern "C" void qdb_gpu_release_slot(int32_t kind, int32_t slot)
+{
+    switch (kind) {
+    case QDB_SLOT_FILTER: filter_slots().release(slot); break;
+    case QDB_SLOT_PROJECT: project_slots().release(slot); break;
+    case QDB_SLOT_AGGREGATE: release_aggregate_slot(slot); break;
+    case QDB_SLOT_JOIN: release_join_slot(slot); break;
+    case QDB_SLOT_HASH: hash_slots().release(slot); break;
+    default: break;
+    }
+}

@@ -1,0 +1,80 @@
+// Host-side runtime of the library: device selection, the stream every
+// entry point runs on, a caching device-memory pool, pinned staging buffers,
+// error text, and HIP-event timing of kernel families.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/qdb_gpu.h"
+
+namespace qdbgpu {
+
+// Records an error for qdb_gpu_last_error (thread-local) and returns false.
+bool fail(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void clear_error();
+
+// Lazily selects the device and creates the library stream; false on error.
+bool ensure_init();
+hipStream_t stream();
+int num_cus();
+
+// Checks a HIP status; on failure records "<what>: <hip error>".
+bool hip_ok(hipError_t e, const char* what);
+
+// Caching device allocator: freed blocks are kept per size class and
+// reused, so repeated queries do not call hipMalloc.
+void* dalloc(size_t bytes);
+void dfree(void* p);
+size_t dalloc_size(const void* p);
+// Pinned host staging memory (hipHostMalloc), also cached.
+void* halloc_pinned(size_t bytes);
+void hfree_pinned(void* p);
+
+// Synchronises the library stream; false (with the error text) on failure.
+bool sync(const char* what);
+
+// Fills `bytes` of device memory with zero through a kernel (no memset
+// node, so the call is safe under stream capture).
+bool zero_fill(void* p, size_t bytes);
+
+// Times one kernel family with HIP events on the library stream when timing
+// is enabled (qdb_gpu_timing_enable).
+class TimedRegion {
+public:
+    explicit TimedRegion(const char* family);
+    ~TimedRegion();
+    TimedRegion(const TimedRegion&) = delete;
+    TimedRegion& operator=(const TimedRegion&) = delete;
+
+private:
+    const char* family_;
+    hipEvent_t start_ = nullptr;
+};
+
+// Grid size for a grid-stride kernel over `items` with `block` threads: at
+// most `per_cu` blocks per CU and no more than the items need.
+int grid_for(int64_t items, int block, int per_cu);
+
+// Device-resident copy of a host struct (uploaded once, freed with dfree).
+void* upload(const void* host, size_t bytes);
+
+// The buffers behind a TRowSet that an entry point produced: device blocks
+// (dalloc) and host blocks (malloc), freed together by the rowset's Destroy.
+struct OwnedRowSet {
+    std::vector<TColumn> cols;
+    std::vector<void*> dev;
+    std::vector<void*> host;
+};
+
+// Fills `out` as an owned rowset of `ncols` zeroed TColumns over `rows`
+// rows; returns the owner to which buffers are added.
+OwnedRowSet* init_owned_rowset(TRowSet* out, int64_t rows, int ncols);
+// Moves device output columns to host memory (for OutputOnHost specs).
+bool move_rowset_to_host(TRowSet* rs, const QdbValueType* types);
+
+}  // namespace qdbgpu

@@ -1,0 +1,620 @@
+// Runtime: device init, stream, memory pool, pinned staging, errors, timing,
+// staging of host batches, downloads, synthetic-data generators.
+#include "runtime.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/qdb_gen.h"
+#include "device.h"
+
+namespace qdbgpu {
+
+namespace {
+
+thread_local std::string g_error;
+
+struct Runtime {
+    std::mutex mu;
+    bool ready = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 256;
+    // pool: size class -> free blocks; block -> size class
+    std::map<size_t, std::vector<void*>> free_dev;
+    std::unordered_map<void*, size_t> dev_sizes;
+    std::map<size_t, std::vector<void*>> free_host;
+    std::unordered_map<void*, size_t> host_sizes;
+    // timing
+    bool timing = false;
+    struct Pair { hipEvent_t a, b; };
+    std::map<std::string, std::vector<Pair>> timed;
+    std::vector<hipEvent_t> spare_events;
+};
+
+Runtime& R()
+{
+    static Runtime* r = new Runtime();
+    return *r;
+}
+
+size_t size_class(size_t bytes)
+{
+    if (bytes < 256) return 256;
+    if (bytes <= (1u << 20)) {
+        size_t c = 256;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    const size_t gran = 2u << 20;
+    return (bytes + gran - 1) / gran * gran;
+}
+
+}  // namespace
+
+bool fail(const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return false;
+}
+
+void clear_error()
+{
+    g_error.clear();
+}
+
+bool hip_ok(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return true;
+    return fail("%s: %s", what, hipGetErrorString(e));
+}
+
+bool ensure_init()
+{
+    Runtime& r = R();
+    std::lock_guard<std::mutex> lock(r.mu);
+    if (r.ready) return true;
+    int count = 0;
+    if (!hip_ok(hipGetDeviceCount(&count), "hipGetDeviceCount")) return false;
+    if (count <= 0) return fail("no HIP device");
+    if (!hip_ok(hipSetDevice(r.device), "hipSetDevice")) return false;
+    hipDeviceProp_t prop;
+    if (!hip_ok(hipGetDeviceProperties(&prop, r.device), "hipGetDeviceProperties")) return false;
+    r.cus = prop.multiProcessorCount;
+    // Default priority, non-blocking with respect to the legacy stream.
+    if (!hip_ok(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
+    r.ready = true;
+    return true;
+}
+
+hipStream_t stream()
+{
+    return R().stream;
+}
+
+int num_cus()
+{
+    return R().cus;
+}
+
+void* dalloc(size_t bytes)
+{
+    if (!ensure_init()) return nullptr;
+    Runtime& r = R();
+    size_t c = size_class(bytes);
+    {
+        std::lock_guard<std::mutex> lock(r.mu);
+        auto it = r.free_dev.find(c);
+        if (it != r.free_dev.end() && !it->second.empty()) {
+            void* p = it->second.back();
+            it->second.pop_back();
+            return p;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, c);
+    if (e != hipSuccess) {
+        // Return cached blocks to the driver and retry once.
+        std::lock_guard<std::mutex> lock(r.mu);
+        for (auto& kv : r.free_dev) {
+            for (void* q : kv.second) {
+                r.dev_sizes.erase(q);
+                (void)hipFree(q);
+            }
+            kv.second.clear();
+        }
+        e = hipMalloc(&p, c);
+        if (e != hipSuccess) {
+            fail("hipMalloc(%zu): %s", c, hipGetErrorString(e));
+            return nullptr;
+        }
+    }
+    std::lock_guard<std::mutex> lock(r.mu);
+    r.dev_sizes[p] = c;
+    return p;
+}
+
+void dfree(void* p)
+{
+    if (!p) return;
+    Runtime& r = R();
+    std::lock_guard<std::mutex> lock(r.mu);
+    auto it = r.dev_sizes.find(p);
+    if (it == r.dev_sizes.end()) return;
+    r.free_dev[it->second].push_back(p);
+}
+
+size_t dalloc_size(const void* p)
+{
+    Runtime& r = R();
+    std::lock_guard<std::mutex> lock(r.mu);
+    auto it = r.dev_sizes.find(const_cast<void*>(p));
+    return it == r.dev_sizes.end() ? 0 : it->second;
+}
+
+void* halloc_pinned(size_t bytes)
+{
+    if (!ensure_init()) return nullptr;
+    Runtime& r = R();
+    size_t c = size_class(bytes);
+    {
+        std::lock_guard<std::mutex> lock(r.mu);
+        auto it = r.free_host.find(c);
+        if (it != r.free_host.end() && !it->second.empty()) {
+            void* p = it->second.back();
+            it->second.pop_back();
+            return p;
+        }
+    }
+    void* p = nullptr;
+    if (!hip_ok(hipHostMalloc(&p, c, hipHostMallocDefault), "hipHostMalloc")) return nullptr;
+    std::lock_guard<std::mutex> lock(r.mu);
+    r.host_sizes[p] = c;
+    return p;
+}
+
+void hfree_pinned(void* p)
+{
+    if (!p) return;
+    Runtime& r = R();
+    std::lock_guard<std::mutex> lock(r.mu);
+    auto it = r.host_sizes.find(p);
+    if (it == r.host_sizes.end()) return;
+    r.free_host[it->second].push_back(p);
+}
+
+bool sync(const char* what)
+{
+    return hip_ok(hipStreamSynchronize(stream()), what);
+}
+
+__global__ void k_zero_fill(uint64_t* p, size_t words, uint8_t* tail, size_t tail_bytes)
+{
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t w = i; w < words; w += stride) p[w] = 0;
+    if (i < tail_bytes) tail[i] = 0;
+}
+
+bool zero_fill(void* p, size_t bytes)
+{
+    if (bytes == 0) return true;
+    size_t words = bytes / 8;
+    uint8_t* tail = (uint8_t*)p + words * 8;
+    int grid = grid_for((int64_t)(words > 64 ? words : 64), 256, 8);
+    hipLaunchKernelGGL(k_zero_fill, dim3(grid), dim3(256), 0, stream(), (uint64_t*)p, words, tail, bytes - words * 8);
+    return hip_ok(hipGetLastError(), "zero_fill launch");
+}
+
+int grid_for(int64_t items, int block, int per_cu)
+{
+    int64_t need = (items + block - 1) / block;
+    int64_t cap = (int64_t)num_cus() * per_cu;
+    if (need < 1) need = 1;
+    return (int)(need < cap ? need : cap);
+}
+
+void* upload(const void* host, size_t bytes)
+{
+    void* d = dalloc(bytes);
+    if (!d) return nullptr;
+    if (!hip_ok(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, stream()), "upload")) return nullptr;
+    if (!sync("upload")) return nullptr;
+    return d;
+}
+
+TimedRegion::TimedRegion(const char* family) : family_(family)
+{
+    Runtime& r = R();
+    if (!r.timing) return;
+    if (hipEventCreate(&start_) != hipSuccess) {
+        start_ = nullptr;
+        return;
+    }
+    (void)hipEventRecord(start_, stream());
+}
+
+TimedRegion::~TimedRegion()
+{
+    if (!start_) return;
+    Runtime& r = R();
+    hipEvent_t end = nullptr;
+    if (hipEventCreate(&end) != hipSuccess) return;
+    (void)hipEventRecord(end, stream());
+    std::lock_guard<std::mutex> lock(r.mu);
+    r.timed[family_].push_back({start_, end});
+}
+
+}  // namespace qdbgpu
+
+using namespace qdbgpu;
+
+// ---------------------------------------------------------------------------
+// C-ABI: runtime
+// ---------------------------------------------------------------------------
+
+extern "C" int qdb_gpu_init(int device)
+{
+    Runtime& r = R();
+    {
+        std::lock_guard<std::mutex> lock(r.mu);
+        if (!r.ready) r.device = device;
+    }
+    return ensure_init() ? 0 : -1;
+}
+
+extern "C" const char* qdb_gpu_last_error(void)
+{
+    return g_error.c_str();
+}
+
+extern "C" const char* qdb_gpu_version(void)
+{
+    return "qdb_gpu 0.1 gfx950";
+}
+
+extern "C" void* qdb_gpu_alloc(size_t bytes)
+{
+    return dalloc(bytes);
+}
+
+extern "C" void qdb_gpu_free(void* p)
+{
+    dfree(p);
+}
+
+extern "C" int qdb_gpu_memcpy_h2d(void* dst, const void* src, size_t bytes)
+{
+    if (!ensure_init()) return -1;
+    if (bytes == 0) return 0;
+    // Through a pinned staging buffer so that the copy runs at DMA speed.
+    const size_t chunk = 64u << 20;
+    void* pin = halloc_pinned(bytes < chunk ? bytes : chunk);
+    if (!pin) return -1;
+    for (size_t off = 0; off < bytes; off += chunk) {
+        size_t n = bytes - off < chunk ? bytes - off : chunk;
+        memcpy(pin, (const char*)src + off, n);
+        if (!hip_ok(hipMemcpyAsync((char*)dst + off, pin, n, hipMemcpyHostToDevice, stream()), "h2d") || !sync("h2d")) {
+            hfree_pinned(pin);
+            return -1;
+        }
+    }
+    hfree_pinned(pin);
+    return 0;
+}
+
+extern "C" int qdb_gpu_memcpy_d2h(void* dst, const void* src, size_t bytes)
+{
+    if (!ensure_init()) return -1;
+    if (bytes == 0) return 0;
+    if (!hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()), "d2h")) return -1;
+    return sync("d2h") ? 0 : -1;
+}
+
+extern "C" int qdb_gpu_memset(void* dst, int value, size_t bytes)
+{
+    if (!ensure_init()) return -1;
+    if (value == 0) return zero_fill(dst, bytes) && sync("memset") ? 0 : -1;
+    if (!hip_ok(hipMemsetAsync(dst, value, bytes, stream()), "memset")) return -1;
+    return sync("memset") ? 0 : -1;
+}
+
+extern "C" int qdb_gpu_synchronize(void)
+{
+    if (!ensure_init()) return -1;
+    return sync("synchronize") ? 0 : -1;
+}
+
+extern "C" void qdb_gpu_timing_enable(int on)
+{
+    R().timing = on != 0;
+}
+
+extern "C" void qdb_gpu_timing_reset(void)
+{
+    Runtime& r = R();
+    (void)hipStreamSynchronize(stream());
+    std::lock_guard<std::mutex> lock(r.mu);
+    for (auto& kv : r.timed) {
+        for (auto& p : kv.second) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+    }
+    r.timed.clear();
+}
+
+extern "C" double qdb_gpu_timing_avg_ms(const char* family, int64_t* launches)
+{
+    Runtime& r = R();
+    (void)hipStreamSynchronize(stream());
+    std::lock_guard<std::mutex> lock(r.mu);
+    auto it = r.timed.find(family);
+    if (launches) *launches = 0;
+    if (it == r.timed.end() || it->second.empty()) return 0.0;
+    double total = 0.0;
+    for (auto& p : it->second) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, p.a, p.b);
+        total += ms;
+    }
+    if (launches) *launches = (int64_t)it->second.size();
+    return total / (double)it->second.size();
+}
+
+// ---------------------------------------------------------------------------
+// Staging: host TRowSet -> device TRowSet through pinned buffers.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+void destroy_owned(TRowSet* rs)
+{
+    OwnedRowSet* o = (OwnedRowSet*)rs->Private;
+    if (!o) return;
+    for (void* p : o->dev) dfree(p);
+    for (void* p : o->host) free(p);
+    delete o;
+    rs->Private = nullptr;
+    rs->Columns = nullptr;
+}
+
+}  // namespace
+
+namespace qdbgpu {
+
+OwnedRowSet* init_owned_rowset(TRowSet* out, int64_t rows, int ncols)
+{
+    OwnedRowSet* o = new OwnedRowSet();
+    o->cols.resize((size_t)(ncols > 0 ? ncols : 1));
+    memset(o->cols.data(), 0, o->cols.size() * sizeof(TColumn));
+    memset(out, 0, sizeof(*out));
+    out->Columns = o->cols.data();
+    out->ColumnCount = ncols;
+    out->RowCount = rows;
+    out->Destroy = destroy_owned;
+    out->Private = o;
+    out->RefCount = 1;
+    return o;
+}
+
+bool move_rowset_to_host(TRowSet* rs, const QdbValueType* types)
+{
+    OwnedRowSet* o = (OwnedRowSet*)rs->Private;
+    const int64_t n = rs->RowCount;
+    std::vector<void*> host;
+    auto fetch = [&](const void* src, size_t bytes) -> void* {
+        void* h = malloc(bytes ? bytes : 1);
+        host.push_back(h);
+        if (src && bytes) (void)hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, stream());
+        return h;
+    };
+    for (int64_t c = 0; c < rs->ColumnCount; ++c) {
+        TColumn& col = o->cols[(size_t)c];
+        const QdbValueType& t = types[c];
+        size_t bytes = t.Kind == QDB_KIND_BOOL ? (size_t)((n + 7) / 8) : (size_t)n * (size_t)t.Width;
+        col.Data = (char*)fetch(col.Data, bytes);
+        if (col.Mask) col.Mask = (const uint8_t*)fetch(col.Mask, (size_t)((n + 7) / 8));
+    }
+    if (!sync("download output")) return false;
+    for (void* p : o->dev) dfree(p);
+    o->dev.clear();
+    o->host.insert(o->host.end(), host.begin(), host.end());
+    return true;
+}
+
+}  // namespace qdbgpu
+
+extern "C" int qdb_gpu_stage_rowset(const TRowSet* host, const QdbColumnSchema* types, TRowSet* out)
+{
+    clear_error();
+    if (!ensure_init()) return -1;
+    const int64_t n = host->RowCount;
+    OwnedRowSet* o = new OwnedRowSet();
+    o->cols.resize((size_t)host->ColumnCount);
+    const size_t mask_bytes = (size_t)((n + 7) / 8);
+    auto stage = [&](const void* src, size_t bytes) -> void* {
+        if (!src || bytes == 0) return nullptr;
+        void* d = dalloc(bytes);
+        if (!d) return nullptr;
+        o->dev.push_back(d);
+        void* pin = halloc_pinned(bytes);
+        if (!pin) return nullptr;
+        memcpy(pin, src, bytes);
+        bool ok = hip_ok(hipMemcpyAsync(d, pin, bytes, hipMemcpyHostToDevice, stream()), "stage copy") && sync("stage");
+        hfree_pinned(pin);
+        return ok ? d : nullptr;
+    };
+    for (int64_t c = 0; c < host->ColumnCount; ++c) {
+        const TColumn& hc = host->Columns[c];
+        const QdbValueType& t = types->Types[c];
+        TColumn& dc = o->cols[(size_t)c];
+        memset(&dc, 0, sizeof(dc));
+        if (t.Kind == QDB_KIND_STRING) {
+            destroy_owned(&*out);
+            return fail("stage: string columns are not supported"), -1;
+        }
+        size_t bytes = t.Kind == QDB_KIND_BOOL ? (size_t)((n + hc.DataBitOffset + 7) / 8) : (size_t)n * (size_t)t.Width;
+        dc.Data = (char*)stage(hc.Data, bytes);
+        dc.DataBitOffset = hc.DataBitOffset;
+        if (hc.Mask) {
+            dc.Mask = (const uint8_t*)stage(hc.Mask, (size_t)((n + hc.MaskBitOffset + 7) / 8));
+            dc.MaskBitOffset = hc.MaskBitOffset;
+        }
+        if (bytes && !dc.Data) return -1;
+    }
+    memset(out, 0, sizeof(*out));
+    out->Columns = o->cols.data();
+    out->ColumnCount = host->ColumnCount;
+    out->RowCount = n;
+    if (host->Selection) out->Selection = (uint8_t*)stage(host->Selection, (size_t)n);
+    if (host->Hash) out->Hash = (uint64_t*)stage(host->Hash, (size_t)n * 8);
+    out->Destroy = destroy_owned;
+    out->Private = o;
+    out->RefCount = 1;
+    (void)mask_bytes;
+    return 0;
+}
+
+extern "C" int qdb_gpu_download_rowset(const TRowSet* dev, const QdbValueType* types, TRowSet* out)
+{
+    clear_error();
+    if (!ensure_init()) return -1;
+    const int64_t n = dev->RowCount;
+    OwnedRowSet* o = new OwnedRowSet();
+    o->cols.resize((size_t)dev->ColumnCount);
+    auto fetch = [&](const void* src, size_t bytes) -> void* {
+        if (!src || bytes == 0) return nullptr;
+        void* h = malloc(bytes);
+        o->host.push_back(h);
+        if (!hip_ok(hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, stream()), "download")) return nullptr;
+        return h;
+    };
+    for (int64_t c = 0; c < dev->ColumnCount; ++c) {
+        const TColumn& dc = dev->Columns[c];
+        const QdbValueType& t = types[c];
+        TColumn& hc = o->cols[(size_t)c];
+        memset(&hc, 0, sizeof(hc));
+        size_t bytes = t.Kind == QDB_KIND_BOOL ? (size_t)((n + dc.DataBitOffset + 7) / 8) : (size_t)n * (size_t)t.Width;
+        hc.Data = (char*)fetch(dc.Data, bytes);
+        hc.DataBitOffset = dc.DataBitOffset;
+        if (dc.Mask) {
+            hc.Mask = (const uint8_t*)fetch(dc.Mask, (size_t)((n + dc.MaskBitOffset + 7) / 8));
+            hc.MaskBitOffset = dc.MaskBitOffset;
+        }
+    }
+    if (!sync("download")) return -1;
+    memset(out, 0, sizeof(*out));
+    out->Columns = o->cols.data();
+    out->ColumnCount = dev->ColumnCount;
+    out->RowCount = n;
+    out->Destroy = destroy_owned;
+    out->Private = o;
+    out->RefCount = 1;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic inputs in HBM (qdb_gen.h).
+// ---------------------------------------------------------------------------
+
+__global__ void k_gen_lineitem(uint64_t seed, int64_t row0, int64_t rows, int8_t* flag, int8_t* status,
+    int32_t* shipdate, int64_t* quantity, int64_t* price, int64_t* discount, int64_t* tax)
+{
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += stride) {
+        qdb_gen_lineitem_row(seed, row0 + i, &flag[i], &status[i], &shipdate[i], &quantity[i], &price[i], &discount[i], &tax[i]);
+    }
+}
+
+__global__ void k_gen_join_build(uint64_t seed, int64_t row0, int64_t rows, int64_t build_rows, int64_t* key, int32_t* segment)
+{
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += stride) {
+        qdb_gen_join_build_row(seed, row0 + i, build_rows, &key[i], &segment[i]);
+    }
+}
+
+__global__ void k_gen_join_probe(uint64_t seed, int64_t row0, int64_t rows, int64_t build_rows, int64_t* fk, int64_t* value)
+{
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += stride) {
+        qdb_gen_join_probe_row(seed, row0 + i, build_rows, &fk[i], &value[i]);
+    }
+}
+
+__global__ void k_gen_groups(uint64_t seed, int64_t row0, int64_t rows, const double* cdf, int64_t distinct,
+    int64_t* key, int64_t* vi, double* vf)
+{
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += stride) {
+        qdb_gen_groups_row(seed, row0 + i, cdf, distinct, &key[i], &vi[i], &vf[i]);
+    }
+}
+
+extern "C" int qdb_gpu_gen_lineitem(uint64_t seed, int64_t row0, int64_t rows, int8_t* flag, int8_t* status,
+    int32_t* shipdate, int64_t* quantity, int64_t* price, int64_t* discount, int64_t* tax)
+{
+    if (!ensure_init()) return -1;
+    if (rows <= 0) return 0;
+    hipLaunchKernelGGL(k_gen_lineitem, dim3(grid_for(rows, 256, 16)), dim3(256), 0, stream(), seed, row0, rows, flag,
+        status, shipdate, quantity, price, discount, tax);
+    return hip_ok(hipGetLastError(), "gen_lineitem") && sync("gen_lineitem") ? 0 : -1;
+}
+
+extern "C" int qdb_gpu_gen_join_build(uint64_t seed, int64_t row0, int64_t rows, int64_t* key, int32_t* segment)
+{
+    if (!ensure_init()) return -1;
+    if (rows <= 0) return 0;
+    // The build table is the permutation of 1..rows of the whole table.
+    hipLaunchKernelGGL(k_gen_join_build, dim3(grid_for(rows, 256, 16)), dim3(256), 0, stream(), seed, row0, rows,
+        row0 + rows, key, segment);
+    return hip_ok(hipGetLastError(), "gen_join_build") && sync("gen_join_build") ? 0 : -1;
+}
+
+extern "C" int qdb_gpu_gen_join_probe(uint64_t seed, int64_t row0, int64_t rows, int64_t build_rows, int64_t* fk, int64_t* value)
+{
+    if (!ensure_init()) return -1;
+    if (rows <= 0) return 0;
+    hipLaunchKernelGGL(k_gen_join_probe, dim3(grid_for(rows, 256, 16)), dim3(256), 0, stream(), seed, row0, rows,
+        build_rows, fk, value);
+    return hip_ok(hipGetLastError(), "gen_join_probe") && sync("gen_join_probe") ? 0 : -1;
+}
+
+extern "C" int qdb_gpu_gen_zipf_groups(uint64_t seed, int64_t row0, int64_t rows, const double* zipf_cdf,
+    int64_t distinct, int64_t* key, int64_t* vi, double* vf)
+{
+    if (!ensure_init()) return -1;
+    if (rows <= 0) return 0;
+    hipLaunchKernelGGL(k_gen_groups, dim3(grid_for(rows, 256, 16)), dim3(256), 0, stream(), seed, row0, rows, zipf_cdf,
+        distinct, key, vi, vf);
+    return hip_ok(hipGetLastError(), "gen_groups") && sync("gen_groups") ? 0 : -1;
+}
+
+// Sizes of the ABI structs, in the order the header declares them, so that
+// bindings can check their layouts.
+extern "C" int64_t qdb_gpu_struct_sizes(int64_t* out, int64_t n)
+{
+    const int64_t sizes[] = {
+        (int64_t)sizeof(TColumn), (int64_t)sizeof(TRowSet), (int64_t)sizeof(QdbValueType), (int64_t)sizeof(QdbExprInsn),
+        (int64_t)sizeof(QdbExprProgram), (int64_t)sizeof(QdbColumnSchema), (int64_t)sizeof(QdbFilterSpec),
+        (int64_t)sizeof(QdbProjectSpec), (int64_t)sizeof(QdbAggregateFn), (int64_t)sizeof(QdbAggregateSpec),
+        (int64_t)sizeof(QdbAggregateKernels), (int64_t)sizeof(QdbRowJoinSpec), (int64_t)sizeof(QdbRowJoinKernels),
+        (int64_t)sizeof(QdbPairBuffer), (int64_t)sizeof(QdbFusedResult),
+    };
+    const int64_t count = (int64_t)(sizeof(sizes) / sizeof(sizes[0]));
+    for (int64_t i = 0; i < n && i < count; ++i) out[i] = sizes[i];
+    return count;
+}

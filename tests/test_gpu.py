@@ -1,0 +1,358 @@
+"""Parity of the HIP path with the reference's goldens and the CPU oracle on
+the same seeded inputs. Integer, decimal, key and pair results are compared
+bit-exactly; f64 sums exactly against the oracle's correctly rounded sum and
+within 1e-12 relative against its Kahan-Babuska-Neumaier sum."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import cases
+from cases import (AGG_COUNT, AGG_COUNT_STAR, AGG_MAX, AGG_MIN, AGG_SUM, AggCase, Batch, GpuBackend, JoinCase,
+                   OracleBackend)
+from helpers import DECIMAL, FLOAT, SIGNED, T, result_rows, sort_key
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+F64_REL_TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def gpu_backend():
+    return GpuBackend()
+
+
+def same(got, want):
+    assert len(got) == len(want), f"{len(got)} rows vs {len(want)}\n{got[:5]}\n{want[:5]}"
+    for g, w in zip(got, want):
+        assert g == w, f"{g} vs {w}"
+
+
+@pytest.mark.parametrize("name", sorted(cases.AGG_GOLDENS))
+def test_aggregate_goldens(gpu_backend, name):
+    got, want = cases.AGG_GOLDENS[name](gpu_backend)
+    same(got, want)
+
+
+@pytest.mark.parametrize("name", sorted(cases.JOIN_GOLDENS))
+def test_join_goldens(gpu_backend, name):
+    got, want = cases.JOIN_GOLDENS[name](gpu_backend)
+    same(got, want)
+
+
+@pytest.mark.parametrize("name", sorted(cases.PROJECT_GOLDENS))
+def test_project_goldens(gpu_backend, name):
+    got, want = cases.PROJECT_GOLDENS[name](gpu_backend)
+    same(got, want)
+
+
+def random_agg_case(seed, n):
+    rng = np.random.default_rng(seed)
+    k1 = rng.integers(-3, 4, n).astype(np.int32)
+    k1v = rng.random(n) > 0.1
+    k2 = rng.integers(0, 50 + seed * 2000, n).astype(np.int64)
+    vi = rng.integers(-(1 << 62), 1 << 62, n).astype(np.int64)
+    viv = rng.random(n) > 0.2
+    dec = [int(x) * (10 ** 20) + int(y) for x, y in zip(rng.integers(-10 ** 9, 10 ** 9, n), rng.integers(0, 10 ** 9, n))]
+    decv = rng.random(n) > 0.3
+    vf = rng.normal(0, 1e3, n)
+    d64 = rng.integers(-10 ** 12, 10 ** 12, n).astype(np.int64)
+    types = [T(SIGNED, 4, nullable=True), T(SIGNED, 8), T(SIGNED, 8, nullable=True), T(DECIMAL, 16, 3, True, 38),
+             T(FLOAT, 8), T(DECIMAL, 8, 2, False, 15)]
+    case = AggCase(types, [0, 1], [(AGG_COUNT_STAR, -1), (AGG_COUNT, 2), (AGG_SUM, 2), (AGG_MIN, 2), (AGG_MAX, 2),
+                                   (AGG_SUM, 3), (AGG_MIN, 3), (AGG_MAX, 3), (AGG_SUM, 4), (AGG_MIN, 4), (AGG_MAX, 4),
+                                   (AGG_SUM, 5), (AGG_MIN, 5)])
+    sel = (rng.random(n) > 0.15).astype(np.uint8)
+    b = Batch(types, [k1, k2, vi, dec, vf, d64], [k1v, None, viv, decv, None, None], sel)
+    return case, b
+
+
+@pytest.mark.parametrize("seed,n,capacity", [(0, 20000, 8), (1, 50000, 8), (3, 200000, 1 << 14)])
+def test_aggregate_random_vs_oracle(gpu_backend, seed, n, capacity):
+    """Nullable multi-column keys (one 16-byte decimal argument), every
+    reducer; a small initial capacity forces the HBM table to grow and the
+    spilled rows and LDS groups to be replayed."""
+    case, b = random_agg_case(seed, n)
+    got = sorted(result_rows(gpu_backend.aggregate(case, [b], capacity=capacity)), key=sort_key)
+    want = sorted(result_rows(OracleBackend(fsum_mode=2).aggregate(case, [b])), key=sort_key)
+    same(got, want)
+
+
+def test_aggregate_batches_accumulate(gpu_backend):
+    case, b = random_agg_case(5, 30000)
+    half = 15000
+    b1 = Batch(b.types, [c[:half] for c in b.columns], [None if v is None else v[:half] for v in b.valid], b.selection[:half])
+    b2 = Batch(b.types, [c[half:] for c in b.columns], [None if v is None else v[half:] for v in b.valid], b.selection[half:])
+    got = sorted(result_rows(gpu_backend.aggregate(case, [b1, b2])), key=sort_key)
+    want = sorted(result_rows(OracleBackend(fsum_mode=2).aggregate(case, [b])), key=sort_key)
+    same(got, want)
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_join_random_vs_oracle(gpu_backend, seed):
+    rng = np.random.default_rng(seed)
+    nb, npr = 5000 + seed * 40000, 200000
+    types_b = [T(SIGNED, 8, nullable=True), T(SIGNED, 4), T(DECIMAL, 16, 2, True, 30)]
+    types_p = [T(SIGNED, 8, nullable=True), T(DECIMAL, 8, 2, False, 15)]
+    bk = rng.integers(0, nb // 2, nb).astype(np.int64)
+    bv = rng.random(nb) > 0.05
+    seg = rng.integers(0, 5, nb).astype(np.int32)
+    pay = [int(x) * 10 ** 18 for x in rng.integers(-1000, 1000, nb)]
+    payv = rng.random(nb) > 0.1
+    pk = rng.integers(0, nb, npr).astype(np.int64)
+    pv = rng.random(npr) > 0.05
+    val = rng.integers(90000, 10500000, npr).astype(np.int64)
+    bsel = (seg != 0).astype(np.uint8)
+    for jt in (cases.JOIN_INNER, cases.JOIN_LEFT_OUTER):
+        case = JoinCase(types_b, types_p, [0], [0], join_type=jt, probe_is_left=True)
+        build = Batch(types_b, [bk, seg, pay], [bv, None, payv], bsel)
+        probe = Batch(types_p, [pk, val], [pv, None])
+        gp, grows = gpu_backend.join(case, [build], probe)
+        op, orows = OracleBackend().join(case, [build], probe)
+        assert np.array_equal(gp[np.lexsort((gp[:, 1], gp[:, 0]))], op[np.lexsort((op[:, 1], op[:, 0]))])
+        assert sorted(grows, key=sort_key) == sorted(orows, key=sort_key)
+
+
+def test_join_multi_batch_build_and_fused(gpu_backend):
+    """Build from three batches (ordinals continue across them), duplicate
+    keys (chains in build order), fused COUNT(*) / SUM / MIN / MAX."""
+    rng = np.random.default_rng(11)
+    types_b = [T(SIGNED, 8), T(SIGNED, 8)]
+    types_p = [T(SIGNED, 8), T(SIGNED, 8, nullable=True)]
+    batches = []
+    for i in range(3):
+        k = rng.integers(0, 3000, 4000).astype(np.int64)
+        w = rng.integers(-100, 100, 4000).astype(np.int64)
+        batches.append(Batch(types_b, [k, w]))
+    pk = rng.integers(0, 3500, 100000).astype(np.int64)
+    pval = rng.integers(-10 ** 15, 10 ** 15, 100000).astype(np.int64)
+    pvv = rng.random(100000) > 0.1
+    probe = Batch(types_p, [pk, pval], [None, pvv])
+    case = JoinCase(types_b, types_p, [0], [0], fused=[(AGG_COUNT_STAR, -1), (AGG_SUM, 1), (AGG_MIN, 1), (AGG_MAX, -3),
+                                                        ])
+    kern, proc = gpu_backend.join_processor(case)
+    for b in batches:
+        proc.append(gpu_backend.dev(b))
+    proc.build()
+    pairs = proc.probe_pairs(gpu_backend.dev(probe))
+    fused = proc.fused_results()
+    oj = O.OracleJoin(case.to_c())
+    for b in batches:
+        oj.append(OracleBackend().host(b))
+    oj.build()
+    opairs = oj.probe(OracleBackend().host(probe))
+    srt = lambda p: p[np.lexsort((p[:, 1], p[:, 0]))]
+    assert np.array_equal(srt(pairs), srt(opairs))
+    rows = (opairs[:, 0] + (1 << 32)).astype(np.int64)
+    bw = np.concatenate([b.columns[1] for b in batches])
+    vals = [int(pval[r]) for r in rows if pvv[r]]
+    assert fused[0]["value"] == len(opairs)
+    assert fused[1]["value"] == sum(vals) and fused[1]["count"] == len(vals)
+    assert fused[2]["value"] == min(vals)
+    assert fused[3]["value"] == int(bw[opairs[:, 1]].max())
+    proc.close()
+
+
+def test_filter_and_project_random(gpu_backend):
+    rng = np.random.default_rng(3)
+    n = 100003
+    types = [T(SIGNED, 8, nullable=True), T(DECIMAL, 8, 2, True, 15), T(FLOAT, 8, nullable=True), T(SIGNED, 2)]
+    cols = [rng.integers(-50, 50, n).astype(np.int64), rng.integers(-10 ** 6, 10 ** 6, n).astype(np.int64),
+            rng.normal(0, 10, n), rng.integers(-100, 100, n).astype(np.int16)]
+    valid = [rng.random(n) > 0.1, rng.random(n) > 0.1, rng.random(n) > 0.1, None]
+    sel = (rng.random(n) > 0.3).astype(np.uint8)
+    E = cases._abi()
+    p = cases.Prog(types)
+    a = p.cmp(E.CMP_GT, p.col(0), p.const(3, T(SIGNED, 8)))
+    b = p.cmp(E.CMP_LE, p.col(2), p.const_f64(1.5))
+    r = p._emit(E.EXPR_AND, E.REG_BOOL, T(3, 1, nullable=True), a=a, b=b)
+    batch = Batch(types, cols, valid, sel)
+    assert np.array_equal(gpu_backend.filter(types, p, r, batch), OracleBackend().filter(types, p, r, batch))
+    # project: i128 decimal products, f64 arithmetic, a bool output
+    q1 = cases.Prog(types)
+    x = q1.op(E.EXPR_MUL, q1.col(1), q1.op(E.EXPR_SUB, q1.const(100, T(DECIMAL, 8, 2, False, 3)), q1.col(1), T(DECIMAL, 8, 2, True, 16)), T(DECIMAL, 16, 4, True, 31))
+    q2 = cases.Prog(types)
+    y = q2.op(E.EXPR_ADD, q2.col(2), q2.const_f64(0.25), T(FLOAT, 8, nullable=True), cls=E.REG_F64)
+    q3 = cases.Prog(types)
+    z = q3.cmp(E.CMP_NE, q3.col(3), q3.const(0, T(SIGNED, 8)))
+    outs = [(q1, x, T(DECIMAL, 16, 4, True, 31)), (q2, y, T(FLOAT, 8, nullable=True)), (q3, z, T(3, 1))]
+    g = gpu_backend.project(types, outs, batch)
+    o = OracleBackend().project(types, outs, batch)
+    for (gv, gm), (ov, om) in zip(g, o):
+        mask = sel.astype(bool)
+        gv, ov = np.asarray(gv, dtype=object), np.asarray(ov, dtype=object)
+        assert np.array_equal(gv[mask], ov[mask])
+        if om is not None:
+            assert np.array_equal(gm[mask], om[mask])
+
+
+def test_group_hash_matches_rh_hash(gpu_backend):
+    """The group-hash entry point computes the reference's rh_hash."""
+    q = gpu_backend.q
+    rng = np.random.default_rng(1)
+    n = 5000
+    types = [T(SIGNED, 8), T(SIGNED, 4, nullable=True), T(FLOAT, 8)]
+    cols = [rng.integers(-10 ** 12, 10 ** 12, n).astype(np.int64), rng.integers(-5, 5, n).astype(np.int32), rng.normal(0, 1, n)]
+    cols[2][:10] = [0.0, -0.0, np.nan, -np.nan, 1.0, -1.0, np.inf, 2.0, 3.0, 4.0]
+    valid = [None, rng.random(n) > 0.2, None]
+    batch = Batch(types, cols, valid)
+    d = gpu_backend.dev(batch)
+    hb = OracleBackend().host(batch)
+    for keys in ([0], [1, 0, 2], [2]):
+        fn = C.c_void_p()
+        slot = q.check(q.lib().qdb_gpu_compile_group_hash(C.byref(cases.schema_c(types)), len(keys), C.byref(fn)), "hash")
+        f = q.abi.GROUP_HASH_FN(fn.value)
+        out = gpu_backend.H.DeviceBuffer(n * 8)
+        kc = (C.c_int64 * len(keys))(*keys)
+        assert f(C.byref(d.rs), out.ptr, kc)
+        got = out.to_numpy(np.uint64, n)
+        kk = (C.c_int32 * len(keys))(*keys)
+        tt = (q.abi.QdbValueType * len(types))(*[cases.value_type_c(t) for t in types])
+        want = np.array([O.LIB.oracle_key_hash(hb.rs.Columns, tt, kk, len(keys), i) for i in range(n)], dtype=np.uint64)
+        assert np.array_equal(got, want), keys
+        q.lib().qdb_gpu_release_slot(q.abi.SLOT_HASH, slot)
+
+
+# ---------------------------------------------------------------------------
+# The three workloads
+# ---------------------------------------------------------------------------
+
+def lineitem_oracle(rows):
+    from qumirdb_amd import workloads as W
+    from qumirdb_amd.host import schema_c
+    abi = O.abi
+    cols = O.gen_lineitem(W.SEED_LINEITEM, 0, rows)
+    hrs = O.HostRowSet(W.LINEITEM_TYPES, cols)
+    fs = abi.QdbFilterSpec()
+    fs.Input = schema_c(W.LINEITEM_TYPES)
+    p, r = W.lineitem_predicate()
+    fs.Predicate = p.build(r)
+    sel = O.filter_rows(fs, hrs)
+    ps = abi.QdbProjectSpec()
+    ps.Input = schema_c(W.LINEITEM_TYPES)
+    outs = [W.lineitem_disc_price(), W.lineitem_charge()]
+    ps.NumOutputs = 2
+    for k, (pp, rr) in enumerate(outs):
+        ps.OutputTypes[k] = pp.result_type(rr).to_c()
+        ps.Outputs[k] = pp.build(rr)
+    proj = O.project_rows(ps, hrs, [pp.result_type(rr) for pp, rr in outs])
+    full = O.HostRowSet(W.lineitem_projected_types(), cols + [proj[0][0], proj[1][0]], selection=sel)
+    return W.lineitem_finalize(O.aggregate(W.lineitem_plain_spec().to_c(), [full]))
+
+
+@pytest.mark.parametrize("rows", [1000, 6_000_000])
+def test_lineitem_fused_vs_oracle(gpu_backend, rows):
+    """WHERE shipdate <= 2436 GROUP BY flag, status with the four exact
+    decimal sums, two AVGs and COUNT(*): the fused kernel equals the
+    oracle's filter -> project -> aggregate bit for bit (AVG as f64)."""
+    from qumirdb_amd import workloads as W
+    from qumirdb_amd.host import AggregateKernels, TAggregateProcessor
+    dev = W.LineitemDevice(rows)
+    proc = TAggregateProcessor(AggregateKernels(W.lineitem_fused_spec()), initial_capacity=8)
+    proc.add(dev.rowset)
+    got = W.lineitem_finalize(proc.finish())
+    assert got == lineitem_oracle(rows)
+    assert len(got) == 6
+
+
+def test_lineitem_operators_equal_fused(gpu_backend):
+    """The drop-in operator chain (filter kernel -> project kernel ->
+    aggregate) equals the fused kernel."""
+    from qumirdb_amd import workloads as W
+    from qumirdb_amd.host import (AggregateKernels, DeviceRowSet, FilterKernel, ProjectKernel, TAggregateProcessor)
+    rows = 500_000
+    dev = W.LineitemDevice(rows)
+    sel = FilterKernel(W.LINEITEM_TYPES, *W.lineitem_predicate())(dev.rowset)
+    proj = ProjectKernel(W.LINEITEM_TYPES, [W.lineitem_disc_price(), W.lineitem_charge()])
+    bufs, _ = proj(dev.rowset)
+    full = DeviceRowSet(W.lineitem_projected_types(), dev.bufs + bufs[:2], rows=rows)
+    full.set_selection_buffer(sel)
+    p1 = TAggregateProcessor(AggregateKernels(W.lineitem_plain_spec()), initial_capacity=8)
+    p1.add(full)
+    plain = W.lineitem_finalize(p1.finish())
+    p2 = TAggregateProcessor(AggregateKernels(W.lineitem_fused_spec()), initial_capacity=8)
+    p2.add(dev.rowset)
+    assert plain == W.lineitem_finalize(p2.finish())
+
+
+def run_join(build_rows, probe_rows):
+    from qumirdb_amd import workloads as W
+    from qumirdb_amd.host import FilterKernel, RowJoinKernels, TRowJoinProcessor
+    jd = W.JoinDevice(build_rows, probe_rows)
+    sel = FilterKernel(W.BUILD_TYPES, *W.join_build_predicate())(jd.build)
+    jd.build.set_selection_buffer(sel)
+    proc = TRowJoinProcessor(RowJoinKernels(W.join_spec(fused=True)), build_is_left=False)
+    proc.append(jd.build)
+    proc.build()
+    pairs = proc.probe_pairs(jd.probe)
+    return jd, sel, pairs, proc.fused_results(), proc
+
+
+def test_join_small_vs_oracle(gpu_backend):
+    from qumirdb_amd import workloads as W
+    B, P = 150_000, 2_000_000
+    jd, sel, pairs, fused, proc = run_join(B, P)
+    key, seg = O.gen_join_build(W.SEED_JOIN, 0, B, B)
+    fk, val = O.gen_join_probe(W.SEED_JOIN, 0, P, B)
+    oj = O.OracleJoin(W.join_spec().to_c())
+    oj.append(O.HostRowSet(W.BUILD_TYPES, [key, seg], selection=(seg == 1)))
+    oj.build()
+    op = oj.probe(O.HostRowSet(W.PROBE_TYPES, [fk, val]))
+    srt = lambda p: p[np.lexsort((p[:, 1], p[:, 0]))]
+    assert np.array_equal(srt(pairs), srt(op))
+    rows = (op[:, 0] + (1 << 32)).astype(np.int64)
+    assert fused[0]["value"] == len(op)
+    assert fused[1]["value"] == int(val[rows].sum(dtype=np.int64))
+    assert abs(len(op) / P - 0.2) < 0.01
+
+
+def test_join_full_size_properties(gpu_backend):
+    """The 1.5M x 60M join: every pair joins equal keys of a segment-1 build
+    row; the pair count, the fused COUNT(*) and the rows whose key is in
+    the filtered build agree; the fused exact SUM equals an independent
+    aggregate over the probe rows the pairs name."""
+    from qumirdb_amd import workloads as W
+    B, P = 1_500_000, 60_000_000
+    jd, sel, pairs, fused, proc = run_join(B, P)
+    key = jd.key.to_numpy(np.int64, B)
+    seg = jd.segment.to_numpy(np.int32, B)
+    fk = jd.fk.to_numpy(np.int64, P)
+    val = jd.value.to_numpy(np.int64, P)
+    rows = (pairs[:, 0] + (1 << 32)).astype(np.int64)
+    assert len(np.unique(rows)) == len(rows)          # build keys are unique
+    assert np.array_equal(fk[rows], key[pairs[:, 1]])
+    assert (seg[pairs[:, 1]] == 1).all()
+    in_build = np.zeros(B + 1, bool)
+    in_build[key[seg == 1]] = True
+    assert len(rows) == int(in_build[fk].sum())
+    assert fused[0]["value"] == len(rows)
+    assert fused[1]["value"] == sum(int(x) for x in np.add.reduceat(val[np.sort(rows)], np.arange(0, len(rows), 1 << 20)))
+
+
+@pytest.mark.parametrize("rows,distinct", [(1_000_000, 100_000), (20_000_000, 10_000_000)])
+def test_groups_vs_oracle(gpu_backend, rows, distinct):
+    """High-cardinality Zipf group-by: keys, counts, i64 sums, MIN/MAX bit
+    exact; the f64 SUM equals the oracle's correctly rounded exact sum and
+    is within 1e-12 relative of its Kahan-Babuska-Neumaier sum."""
+    from qumirdb_amd import workloads as W
+    from qumirdb_amd.host import AggregateKernels, TAggregateProcessor
+    cdf = W.zipf1_cdf(distinct)
+    dev = W.GroupsDevice(rows, distinct, cdf=cdf)
+    proc = TAggregateProcessor(AggregateKernels(W.groups_spec()), initial_capacity=1 << 20)
+    proc.add(dev.rowset)
+    got = proc.finish()
+    key, vi, vf = O.gen_groups(W.SEED_GROUPS, 0, rows, cdf)
+    hb = O.HostRowSet(W.GROUPS_TYPES, [key, vi, vf])
+    exact = O.aggregate(W.groups_spec().to_c(), [hb], fsum_mode=2)
+    kahan = O.aggregate(W.groups_spec().to_c(), [hb], fsum_mode=1)
+    gi, ei, ki = np.argsort(got[0][0]), np.argsort(exact[0][0]), np.argsort(kahan[0][0])
+    for c in range(5):
+        assert np.array_equal(np.asarray(got[c][0])[gi], np.asarray(exact[c][0])[ei]), c
+    gs = np.asarray(got[5][0])[gi]
+    ks = np.asarray(kahan[5][0])[ki]
+    rel = np.abs(gs - ks) / np.maximum(np.abs(ks), 1e-300)
+    assert (rel <= F64_REL_TOL).all(), float(rel.max())
+    assert np.array_equal(gs, np.asarray(exact[5][0])[ei])

@@ -1,0 +1,290 @@
+"""Pins the CPU oracle: the reference's own golden results for the path,
+closed forms (exact rational rounding), and brute force on small inputs."""
+import math
+import random
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import cases
+from cases import AGG_COUNT, AGG_COUNT_STAR, AGG_MAX, AGG_MIN, AGG_SUM, AggCase, Batch, JoinCase, OracleBackend
+from helpers import DECIMAL, FLOAT, SIGNED, T, golden_rows, result_rows, sort_key
+
+from oracle import oracle as O
+
+
+@pytest.fixture(scope="module")
+def backend():
+    return OracleBackend(fsum_mode=0)
+
+
+def same(got, want):
+    assert len(got) == len(want), f"{len(got)} rows vs {len(want)}\n{got}\n{want}"
+    for g, w in zip(got, want):
+        assert len(g) == len(w)
+        for a, b in zip(g, w):
+            if isinstance(b, float) or isinstance(a, float):
+                assert a is not None and b is not None and (a == b or math.isclose(a, b, rel_tol=0, abs_tol=0)), f"{g} vs {w}"
+            else:
+                assert a == b, f"{g} vs {w}"
+
+
+@pytest.mark.parametrize("name", sorted(cases.AGG_GOLDENS))
+def test_aggregate_goldens(backend, name):
+    got, want = cases.AGG_GOLDENS[name](backend)
+    same(got, want)
+
+
+@pytest.mark.parametrize("name", sorted(cases.JOIN_GOLDENS))
+def test_join_goldens(backend, name):
+    got, want = cases.JOIN_GOLDENS[name](backend)
+    same(got, want)
+
+
+@pytest.mark.parametrize("name", sorted(cases.PROJECT_GOLDENS))
+def test_project_goldens(backend, name):
+    got, want = cases.PROJECT_GOLDENS[name](backend)
+    same(got, want)
+
+
+def test_decimal_casts_golden():
+    """decimal_parquet/casts: CAST(d2 AS DOUBLE), CAST(d10 AS DOUBLE) are the
+    correctly rounded doubles (qdb_decimal_to_f64)."""
+    names, b = cases.table_batch("decimal_parquet.wide")
+    d2, d10 = names.index("d2"), names.index("d10")
+    golden = golden_rows(cases.FIX["goldens"]["decimal_parquet/casts"])
+    by_id = {int(i): r for r, i in enumerate(b.columns[names.index("id")])}
+    for gid, f2, f10, _ in golden:
+        r = by_id[gid]
+        if f2 is not None:
+            assert O.decimal_to_f64(b.columns[d2][r], 2) == f2
+        if f10 is not None:
+            assert O.decimal_to_f64(b.columns[d10][r], 10) == f10
+
+
+def nearest(fr: Fraction) -> float:
+    return float(fr)  # CPython rounds a rational to the nearest double, ties to even
+
+
+@pytest.mark.parametrize("scale", [0, 2, 4, 6, 10, 18, 22, 24, 30, 38])
+def test_decimal_to_f64_exact_rounding(scale):
+    rng = random.Random(scale)
+    vals = [0, 1, -1, (1 << 53) - 1, 1 << 53, (1 << 53) + 1, -(1 << 53) - 1, 10 ** 38 - 1, -(10 ** 38 - 1),
+            (1 << 63), 5 * 10 ** (scale - 1) if scale else 5, 12345678901234567890123456789012345678 % (10 ** 38)]
+    vals += [rng.randrange(-(10 ** 38) + 1, 10 ** 38) for _ in range(200)]
+    vals += [rng.randrange(-(1 << 60), 1 << 60) for _ in range(200)]
+    for v in vals:
+        assert O.decimal_to_f64(v, scale) == nearest(Fraction(v, 10 ** scale)), (v, scale)
+
+
+def test_decimal_to_f64_ties_to_even():
+    # 2^53 + 1 over 1: exactly between two doubles -> even mantissa.
+    assert O.decimal_to_f64((1 << 53) + 1, 0) == float(1 << 53)
+    assert O.decimal_to_f64((1 << 53) + 3, 0) == float((1 << 53) + 4)
+    assert O.decimal_to_f64(((1 << 53) + 1) * 100, 2) == float(1 << 53)
+
+
+def exact_fraction_sum(vals):
+    return sum((Fraction(v) for v in vals), Fraction(0))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_exact_sum_is_correctly_rounded(seed):
+    rng = np.random.default_rng(seed)
+    n = 1 + seed * 37
+    vals = rng.normal(0, 1e3, n)
+    if seed % 2:
+        vals = np.concatenate([vals, -vals[:-1], [1e-300, 1e300, -1e300]])
+    if seed == 4:
+        vals = np.array([1e16, 1.0, -1e16, 1e-20, 3.0 * 2 ** -1074])
+    want = float(exact_fraction_sum(vals.tolist()))
+    assert O.exact_sum(vals) == want
+
+
+def test_exact_sum_specials():
+    assert math.isnan(O.exact_sum([1.0, float("nan")]))
+    assert O.exact_sum([1.0, float("inf")]) == float("inf")
+    assert math.isnan(O.exact_sum([float("inf"), float("-inf")]))
+    assert O.exact_sum([]) == 0.0
+
+
+def test_kahan_close_to_exact():
+    rng = np.random.default_rng(7)
+    vals = rng.normal(0, 1e3, 100000)
+    assert abs(O.kahan_sum(vals) - O.exact_sum(vals)) <= 1e-12 * abs(O.exact_sum(vals))
+
+
+def brute_groupby(keys, vals, valid):
+    out = {}
+    for k, v, ok in zip(keys, vals, valid):
+        g = out.setdefault(k, {"n": 0, "cnt": 0, "sum": 0, "min": None, "max": None, "f": []})
+        g["n"] += 1
+        if ok:
+            g["cnt"] += 1
+            g["sum"] += v
+            g["min"] = v if g["min"] is None else min(g["min"], v)
+            g["max"] = v if g["max"] is None else max(g["max"], v)
+    return out
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_aggregate_brute_force(seed):
+    """Random nullable keys and values: counts, i64 wrap sums, i128 decimal
+    sums, min/max and exact f64 sums against dict-based brute force."""
+    rng = np.random.default_rng(seed)
+    n = 5000
+    k1 = rng.integers(-3, 4, n).astype(np.int32)
+    k1v = rng.random(n) > 0.1
+    k2 = rng.integers(0, 50, n).astype(np.int64)
+    vi = rng.integers(-(1 << 62), 1 << 62, n).astype(np.int64)
+    viv = rng.random(n) > 0.2
+    dec = [int(x) * (10 ** 20) + int(y) for x, y in zip(rng.integers(-10 ** 9, 10 ** 9, n), rng.integers(0, 10 ** 9, n))]
+    decv = rng.random(n) > 0.3
+    vf = rng.normal(0, 1e3, n)
+    types = [T(SIGNED, 4, nullable=True), T(SIGNED, 8), T(SIGNED, 8, nullable=True), T(DECIMAL, 16, 3, True, 38), T(FLOAT, 8)]
+    case = AggCase(types, [0, 1], [(AGG_COUNT_STAR, -1), (AGG_COUNT, 2), (AGG_SUM, 2), (AGG_MIN, 2), (AGG_MAX, 2),
+                                   (AGG_SUM, 3), (AGG_MIN, 3), (AGG_MAX, 3), (AGG_SUM, 4)])
+    b = Batch(types, [k1, k2, vi, dec, vf], [k1v, None, viv, decv, None])
+    res = result_rows(OracleBackend(fsum_mode=2).aggregate(case, [b]))
+    keys = [(int(a) if ok else None, int(c)) for a, ok, c in zip(k1, k1v, k2)]
+    bi = brute_groupby(keys, [int(x) for x in vi], viv)
+    bd = brute_groupby(keys, dec, decv)
+    bf = {}
+    for kk, f in zip(keys, vf):
+        bf.setdefault(kk, []).append(f)
+    assert len(res) == len(bi)
+    wrap = lambda x: ((x + (1 << 63)) % (1 << 64)) - (1 << 63)
+    for row in res:
+        kk = (row[0], row[1])
+        gi, gd = bi[kk], bd[kk]
+        assert row[2] == gi["n"]
+        assert row[3] == gi["cnt"]
+        assert row[4] == (wrap(gi["sum"]) if gi["cnt"] else None)
+        assert row[5] == gi["min"] and row[6] == gi["max"]
+        assert row[7] == (gd["sum"] if gd["cnt"] else None)
+        assert row[8] == gd["min"] and row[9] == gd["max"]
+        assert row[10] == float(exact_fraction_sum(bf[kk]))
+
+
+def test_aggregate_selection_and_batches():
+    """Rows with Selection[i] == 0 are skipped; batches accumulate."""
+    types = [T(SIGNED, 8), T(SIGNED, 8)]
+    case = AggCase(types, [0], [(AGG_COUNT_STAR, -1), (AGG_SUM, 1)])
+    b1 = Batch(types, [[1, 2, 1, 3], [10, 20, 30, 40]], selection=np.array([1, 1, 0, 1], np.uint8))
+    b2 = Batch(types, [[1, 3], [5, 6]])
+    got = sorted(result_rows(OracleBackend().aggregate(case, [b1, b2])), key=sort_key)
+    assert got == [(1, 2, 15), (2, 1, 20), (3, 2, 46)]
+
+
+def brute_join(bkeys, bvalid, pkeys, pvalid, psel, outer):
+    pairs = []
+    for i, (pk, pv) in enumerate(zip(pkeys, pvalid)):
+        if psel is not None and not psel[i]:
+            continue
+        m = [j for j, (bk, bv) in enumerate(zip(bkeys, bvalid)) if pv and bv and bk == pk]
+        for j in m:
+            pairs.append((i, j))
+        if not m and outer:
+            pairs.append((i, -1))
+    return sorted(pairs)
+
+
+@pytest.mark.parametrize("outer", [False, True])
+@pytest.mark.parametrize("seed", range(3))
+def test_join_brute_force(seed, outer):
+    rng = np.random.default_rng(seed)
+    nb, npr = 300, 2000
+    bk = rng.integers(0, 200, nb).astype(np.int64)
+    bv = rng.random(nb) > 0.1
+    bsel = rng.random(nb) > 0.3
+    pk = rng.integers(0, 260, npr).astype(np.int64)
+    pv = rng.random(npr) > 0.1
+    psel = (rng.random(npr) > 0.2).astype(np.uint8)
+    types = [T(SIGNED, 8, nullable=True)]
+    case = JoinCase(types, types, [0], [0], join_type=cases.JOIN_LEFT_OUTER if outer else cases.JOIN_INNER)
+    j = O.OracleJoin(case.to_c())
+    j.append(O.HostRowSet(types, [bk], [bv], selection=bsel.astype(np.uint8)))
+    j.build()
+    pairs = j.probe(O.HostRowSet(types, [pk], [pv], selection=psel))
+    got = sorted((int(a) + (1 << 32), int(b)) for a, b in pairs)
+    want = brute_join(bk, bv & bsel, pk, pv, psel, outer)
+    assert got == want
+
+
+def test_join_chains_in_build_order():
+    """Equal build keys are emitted in build order (jr_build_rows links the
+    rows from the last to the first)."""
+    types = [T(SIGNED, 8)]
+    case = JoinCase(types, types, [0], [0])
+    j = O.OracleJoin(case.to_c())
+    j.append(O.HostRowSet(types, [np.array([5, 7, 5, 5, 9], np.int64)]))
+    j.build()
+    pairs = j.probe(O.HostRowSet(types, [np.array([5, 9], np.int64)]))
+    assert [int(b) for _, b in pairs] == [0, 2, 3, 4]
+    assert j.keys() == 3 and j.rows() == 5
+
+
+def test_three_valued_filter():
+    """NULL is not TRUE: x > 0 OR y > 0 with NULLs (SQL three-valued logic)."""
+    E = cases._abi()
+    types = [T(SIGNED, 8, nullable=True), T(SIGNED, 8, nullable=True)]
+    x = [1, -1, 0, 1, 0, -1]
+    xv = np.array([1, 1, 0, 0, 0, 1], bool)
+    y = [0, 0, 1, 0, 0, 5]
+    yv = np.array([0, 0, 1, 0, 0, 1], bool)
+    p = cases.Prog(types)
+    zero = p.const(0, T(SIGNED, 8))
+    a = p.cmp(E.CMP_GT, p.col(0), zero)
+    b = p.cmp(E.CMP_GT, p.col(1), zero)
+    r = p._emit(E.EXPR_OR, E.REG_BOOL, T(3, 1, nullable=True), a=a, b=b)
+    sel = OracleBackend().filter(types, p, r, Batch(types, [x, y], [xv, yv]))
+    # row: x>0, y>0 -> OR
+    # 0: T, NULL -> T ; 1: F, NULL -> NULL ; 2: NULL, F(1>0 T) -> T ; 3: NULL, NULL -> NULL
+    # 4: NULL, NULL -> NULL ; 5: F, T -> T
+    assert sel.tolist() == [1, 0, 1, 0, 0, 1]
+    sel2 = OracleBackend().filter(types, p, r, Batch(types, [x, y], [xv, yv], selection=np.array([0, 1, 1, 1, 1, 1], np.uint8)))
+    assert sel2.tolist() == [0, 0, 1, 0, 0, 1]
+
+
+def test_zipf_cdf_host_matches_numpy():
+    """The host generator's CDF equals the numpy cumulative sum the device
+    workloads upload (qumirdb_amd/workloads.py zipf1_cdf)."""
+    n = 10000
+    h = np.cumsum(1.0 / np.arange(1, n + 1, dtype=np.float64))
+    cdf = h / h[-1]
+    cdf[-1] = 1.0
+    assert np.array_equal(O.zipf_cdf(n), cdf)
+
+
+def test_lineitem_generator_ranges():
+    flag, status, ship, qty, price, disc, tax = O.gen_lineitem(0xC0, 0, 100000)
+    assert set(np.unique(flag)) == {0, 1, 2} and set(np.unique(status)) == {0, 1}
+    assert ship.min() >= 0 and ship.max() <= 2525
+    assert set(np.unique(qty // 100)) == set(range(1, 51)) and (qty % 100 == 0).all()
+    assert price.min() >= 90000 and price.max() <= 10500000
+    assert disc.min() == 0 and disc.max() == 10 and tax.min() == 0 and tax.max() == 8
+    sel = (ship <= 2436).mean()
+    assert abs(sel - 2437 / 2526) < 0.005
+    # counter-based: a slice regenerates the same rows
+    again = O.gen_lineitem(0xC0, 500, 100)
+    assert np.array_equal(again[4], price[500:600])
+
+
+def test_join_generator_permutation():
+    key, seg = O.gen_join_build(0xC1, 0, 100000, 100000)
+    assert np.array_equal(np.sort(key), np.arange(1, 100001))
+    assert abs((seg == 1).mean() - 0.2) < 0.01
+    fk, val = O.gen_join_probe(0xC1, 0, 100000, 100000)
+    assert fk.min() >= 1 and fk.max() <= 100000
+
+
+def test_groups_generator_zipf():
+    cdf = O.zipf_cdf(10000)
+    key, vi, vf = O.gen_groups(0xC2, 0, 200000, cdf)
+    vals, counts = np.unique(key, return_counts=True)
+    top = counts.max() / len(key)
+    h = np.sum(1.0 / np.arange(1, 10001))
+    assert abs(top - 1.0 / h) < 0.01
+    assert vi.min() >= -1000000 and vi.max() <= 1000000
+    assert abs(vf.std() - 1000.0) < 20.0
