@@ -48,11 +48,25 @@ QDB_GEN_FN uint64_t qdb_rand(uint64_t seed, uint64_t stream, uint64_t row)
     return qdb_mix64(qdb_mix64(seed + stream * 0x9E3779B97F4A7C15ull) ^ (row * 0xD1B54A32D192ED03ull + stream));
 }
 
-/* Uniform integer in [lo, hi] (modulo reduction). */
+/* High 64 bits of the 128-bit product a * b, from 32-bit partial products
+ * (the same operations on the host and the device). */
+QDB_GEN_FN uint64_t qdb_mulhi64(uint64_t a, uint64_t b)
+{
+    const uint64_t a_lo = a & 0xffffffffu, a_hi = a >> 32;
+    const uint64_t b_lo = b & 0xffffffffu, b_hi = b >> 32;
+    const uint64_t p0 = a_lo * b_lo, p1 = a_lo * b_hi, p2 = a_hi * b_lo, p3 = a_hi * b_hi;
+    const uint64_t mid = (p0 >> 32) + (p1 & 0xffffffffu) + (p2 & 0xffffffffu);
+    return p3 + (p1 >> 32) + (p2 >> 32) + (mid >> 32);
+}
+
+/* Uniform integer in [lo, hi]: the high word of rand * span (multiply-shift
+ * range reduction, no division). A 64-bit remainder by a constant is avoided
+ * on purpose: hipcc (ROCm 7.2) has lowered `x % 50` inside the lineitem
+ * generator kernel for gfx950 to code that drops bits of the quotient. */
 QDB_GEN_FN int64_t qdb_uniform(uint64_t seed, uint64_t stream, uint64_t row, int64_t lo, int64_t hi)
 {
     uint64_t span = (uint64_t)(hi - lo) + 1u;
-    return lo + (int64_t)(qdb_rand(seed, stream, row) % span);
+    return lo + (int64_t)qdb_mulhi64(qdb_rand(seed, stream, row), span);
 }
 
 /* Uniform double in [0, 1) on the 2^-53 grid. */

@@ -21,8 +21,16 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 
 def _load_abi():
-    spec = importlib.util.spec_from_file_location("qdb_abi_for_oracle", os.path.join(ROOT, "qumirdb_amd", "abi.py"))
+    """The ABI module, registered under its package name so that the HIP
+    bindings (imported before or after) share the very same struct classes;
+    loading it by file does not load the HIP library."""
+    import sys
+    name = "qumirdb_amd.abi"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "qumirdb_amd", "abi.py"))
     mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
     spec.loader.exec_module(mod)
     return mod
 

@@ -157,8 +157,8 @@ struct AggPlan {
     AggArgs proto;             // launch arguments without per-batch fields
     int slot_col[kMaxDevCols]; // device slot -> input column
     int nslots = 0;
-    QdbExprProgram* d_pred = nullptr;
-    QdbExprProgram* d_progs = nullptr;
+    DevProgram* d_pred = nullptr;
+    DevProgram* d_progs = nullptr;
     FinArgs fin;               // finalize arguments without buffers
     int nout = 0;
     QdbValueType out_types[QDB_AGG_MAX_KEYS + QDB_AGG_MAX_AGGS];
@@ -331,15 +331,15 @@ bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P)
         KeyFieldDesc& kf = A.keys[k];
         int s = slot_of(c);
         if (s < 0) return fail("too many columns");
-        kf.slot = (int8_t)s;
+        kf.slot = s;
         kf.kind = t.Kind;
-        kf.width = (uint8_t)w;
+        kf.width = w;
         kf.nullable = t.Nullable;
-        kf.word = (uint8_t)(byte / 8);
-        kf.shift = (uint8_t)((byte % 8) * 8);
+        kf.word = byte / 8;
+        kf.shift = (byte % 8) * 8;
         if (t.Nullable) {
-            kf.null_word = (uint8_t)(nullbit / 64);
-            kf.null_bit = (uint8_t)(nullbit % 64);
+            kf.null_word = nullbit / 64;
+            kf.null_bit = nullbit % 64;
             ++nullbit;
         }
         byte += w;
@@ -361,11 +361,11 @@ bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P)
         SrcDesc& sd = A.src[A.nsrc];
         if (c >= nin) {
             sd.slot = -1;
-            sd.prog = (int8_t)(c - nin);
+            sd.prog = c - nin;
         } else {
             int s = slot_of(c);
             if (s < 0) return -1;
-            sd.slot = (int8_t)s;
+            sd.slot = s;
             sd.prog = -1;
         }
         return A.nsrc++;
@@ -376,8 +376,8 @@ bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P)
         WordDesc& d = A.words[nw];
         d.kind = kind;
         d.op = op;
-        d.src = (int8_t)src;
-        d.part = (uint8_t)part;
+        d.src = src;
+        d.part = part;
         return nw++;
     };
     const int rows_word = word(WK_ROWS, OP_ADD, 0, 0);
@@ -418,7 +418,7 @@ bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P)
         if (fn.Func == QDB_AGG_COUNT) {
             ro.kind = RK_COUNT_VALID;
             ro.out_width = 8;
-            ro.w0 = (int16_t)word(WK_VALID, OP_ADD, src, 0);
+            ro.w0 = word(WK_VALID, OP_ADD, src, 0);
             continue;
         }
         const bool is_max = fn.Func == QDB_AGG_MAX;
@@ -427,44 +427,45 @@ bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P)
             if (t.Kind == QDB_KIND_FLOAT) {
                 ro.kind = RK_SUMF64;
                 ro.out_width = 8;
-                ro.w0 = (int16_t)word(WK_LIMB, OP_ADD, src, 0);
+                ro.w0 = word(WK_LIMB, OP_ADD, src, 0);
                 for (int k = 1; k < kLimbs; ++k) word(WK_LIMB, OP_ADD, src, k);
                 word(WK_OUTSIDE, OP_FADD, src, 0);
             } else if (t.Kind == QDB_KIND_DECIMAL) {
                 ro.kind = RK_SUM128;
                 ro.out_width = 16;
-                ro.w0 = (int16_t)word(WK_I128_P0, OP_ADD, src, 0);
+                ro.w0 = word(WK_I128_P0, OP_ADD, src, 0);
                 word(WK_I128_P1, OP_ADD, src, 0);
                 word(WK_I128_P2, OP_ADD, src, 0);
             } else {
                 ro.kind = RK_SUM64;
                 ro.out_width = 8;
-                ro.w0 = (int16_t)word(WK_SUM64, OP_ADD, src, 0);
+                ro.w0 = word(WK_SUM64, OP_ADD, src, 0);
             }
         } else if (fn.Func == QDB_AGG_MIN || fn.Func == QDB_AGG_MAX) {
             if (t.Kind == QDB_KIND_FLOAT) {
                 ro.kind = RK_MINMAX_F64;
                 ro.out_width = 8;
                 ro.is_max = is_max;
-                ro.w0 = (int16_t)word(is_max ? WK_MAX_F64 : WK_MIN_F64, is_max ? OP_MAX_U64 : OP_MIN_U64, src, 0);
+                ro.w0 = word(is_max ? WK_MAX_F64 : WK_MIN_F64, is_max ? OP_MAX_U64 : OP_MIN_U64, src, 0);
             } else if (t.Kind == QDB_KIND_DECIMAL && t.Width == 16) {
                 ro.kind = RK_MINMAX_I128;
                 ro.out_width = 16;
                 ro.is_max = is_max;
-                ro.w0 = (int16_t)word(WK_PAIR_LO, is_max ? OP_PAIR_MAX : OP_PAIR_MIN, src, is_max);
+                ro.w0 = word(WK_PAIR_LO, is_max ? OP_PAIR_MAX : OP_PAIR_MIN, src, is_max);
                 word(WK_PAIR_HI, OP_PAIR_HI, src, is_max);
                 word(WK_LOCK, OP_LOCK, src, 0);
                 P->has_pairs = true;
+                A.has_pairs = 1;
             } else {
                 ro.kind = RK_MINMAX_I64;
                 ro.out_width = 8;
                 ro.is_max = is_max;
-                ro.w0 = (int16_t)word(is_max ? WK_MAX_I64 : WK_MIN_I64, is_max ? OP_MAX_I64 : OP_MIN_I64, src, 0);
+                ro.w0 = word(is_max ? WK_MAX_I64 : WK_MIN_I64, is_max ? OP_MAX_I64 : OP_MIN_I64, src, 0);
             }
         } else {
             return fail("aggregate %d: unknown function %d", a, fn.Func);
         }
-        if (t.Nullable) ro.wvalid = (int16_t)word(WK_VALID, OP_ADD, src, 0);
+        if (t.Nullable) ro.wvalid = word(WK_VALID, OP_ADD, src, 0);
         if (ro.w0 < 0 || (t.Nullable && ro.wvalid < 0)) return fail("too many state words (max %d)", kMaxWords);
     }
     A.nwords = nw;
@@ -475,15 +476,18 @@ bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P)
     if (spec.HasPredicate) {
         QdbExprProgram pred;
         if (!remap(spec.Predicate, &pred)) return false;
-        P->d_pred = (QdbExprProgram*)upload(&pred, sizeof(pred));
+        const DevProgram dp = to_dev_program(pred);
+        P->d_pred = (DevProgram*)upload(&dp, sizeof(dp));
         if (!P->d_pred) return false;
     }
     if (spec.NumComputed > 0) {
-        std::vector<QdbExprProgram> progs((size_t)spec.NumComputed);
+        std::vector<DevProgram> progs((size_t)spec.NumComputed);
         for (int k = 0; k < spec.NumComputed; ++k) {
-            if (!remap(spec.Computed[k], &progs[(size_t)k])) return false;
+            QdbExprProgram prog;
+            if (!remap(spec.Computed[k], &prog)) return false;
+            progs[(size_t)k] = to_dev_program(prog);
         }
-        P->d_progs = (QdbExprProgram*)upload(progs.data(), progs.size() * sizeof(QdbExprProgram));
+        P->d_progs = (DevProgram*)upload(progs.data(), progs.size() * sizeof(DevProgram));
         if (!P->d_progs) return false;
     }
 
@@ -493,7 +497,7 @@ bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P)
     F.nkeys = spec.NumKeys;
     for (int k = 0; k < spec.NumKeys; ++k) {
         F.keys[k] = A.keys[k];
-        F.key_out_width[k] = (uint8_t)key_bytes_of(types[spec.KeyColumns[k]]);
+        F.key_out_width[k] = key_bytes_of(types[spec.KeyColumns[k]]);
     }
     output_types(spec, types, P->out_types);
     P->nout = spec.NumKeys + spec.NumAggs;
@@ -516,7 +520,7 @@ bool bind_batch(const AggPlan* P, const TRowSet* batch, AggArgs* A)
         d.mask_off = col.MaskBitOffset;
         d.bit_off = col.DataBitOffset;
         d.kind = t.Kind;
-        d.width = (uint8_t)t.Width;
+        d.width = t.Width;
         d.nullable = t.Nullable;
     }
     A->nrows = batch->RowCount;
@@ -666,8 +670,12 @@ bool table_update(AggTable* T, const TRowSet* batch)
         if (!read_counters(T, cnt)) { release(); return false; }
         int cur = 0;
         while (cnt[1] || cnt[2]) {
+            if (cnt[2] > ov_recs_cap || cnt[1] > (uint64_t)n) {
+                release();
+                return fail("aggregate spill overflow (%u rows, %u records)", cnt[1], cnt[2]);
+            }
             const unsigned int spilled_rows = cnt[1];
-            const unsigned int spilled_recs = cnt[2] < ov_recs_cap ? cnt[2] : (unsigned int)ov_recs_cap;
+            const unsigned int spilled_recs = cnt[2];
             if (!table_grow(T, (uint64_t)T->groups + spilled_rows + spilled_recs + 1024)) { release(); return false; }
             const int nxt = cur ^ 1;
             table_args(T, &A);

@@ -10,7 +10,7 @@
 namespace qdbgpu {
 namespace agg {
 
-constexpr int kMaxSrc = 8;
+constexpr int kMaxSrc = QDB_AGG_MAX_AGGS;
 constexpr int kMaxWords = 32;
 constexpr int kBlock = 256;
 
@@ -48,26 +48,26 @@ enum WordKind : uint8_t {
 };
 
 struct WordDesc {
-    uint8_t kind;
-    uint8_t op;
-    int8_t src;
-    uint8_t part;
+    int32_t kind;
+    int32_t op;
+    int32_t src;
+    int32_t part;
 };
 
 struct KeyFieldDesc {
-    int8_t slot;      // device column slot
-    uint8_t kind;
-    uint8_t width;    // bytes of the packed value (16 for i128)
-    uint8_t nullable;
-    uint8_t word;     // packed word index
-    uint8_t shift;    // bit offset inside the word
-    uint8_t null_word;
-    uint8_t null_bit;
+    int32_t slot;      // device column slot
+    int32_t kind;
+    int32_t width;     // bytes of the packed value (16 for i128)
+    int32_t nullable;
+    int32_t word;      // packed word index
+    int32_t shift;     // bit offset inside the word
+    int32_t null_word;
+    int32_t null_bit;
 };
 
 struct SrcDesc {
-    int8_t slot;      // device column slot, or -1
-    int8_t prog;      // computed program index, or -1
+    int32_t slot;      // device column slot, or -1
+    int32_t prog;      // computed program index, or -1
 };
 
 // Everything an update launch needs, passed by value.
@@ -77,8 +77,8 @@ struct AggArgs {
     int64_t row_base;          // first row of this launch
     const uint8_t* sel;
     const uint32_t* row_index; // replay: rows to apply (row_base-relative)
-    const QdbExprProgram* pred;
-    const QdbExprProgram* progs;
+    const DevProgram* pred;
+    const DevProgram* progs;
     int nkeys;
     KeyFieldDesc keys[QDB_AGG_MAX_KEYS];
     int nsrc;
@@ -88,6 +88,7 @@ struct AggArgs {
     int lds_slots;             // power of two
     int lds_groups;            // max groups in LDS
     int match_rounds;
+    int has_pairs;             // i128 MIN/MAX words present
     // HBM table
     unsigned long long* ctrl;
     uint64_t cmask;
@@ -163,15 +164,6 @@ __device__ __forceinline__ bool keys_equal(const uint64_t (&a)[KW], const uint64
     return eq;
 }
 
-// One aggregate argument of a row, with the parts the state words take.
-struct SrcVal {
-    i128 i;
-    double f;
-    int64_t limb[kLimbs];
-    bool valid;
-    bool outside;
-};
-
 // The packed key of a row.
 template <int KW>
 __device__ __forceinline__ void row_key(const AggArgs& A, int64_t row, uint64_t (&key)[KW])
@@ -221,7 +213,10 @@ __device__ __forceinline__ void row_key(const AggArgs& A, int64_t row, uint64_t 
 }
 
 // A row's contribution to every state word; false when the row does not
-// take part (not selected, or rejected by the fused predicate).
+// take part (not selected, or rejected by the fused predicate). Each
+// argument is read (or computed) once, by one copy of the column reader and
+// the interpreter, and stored into the words that take it; the word loop is
+// unrolled so that `c` stays in registers.
 template <int KW, int S>
 __device__ __forceinline__ bool row_contrib(const AggArgs& A, int64_t row, uint64_t (&key)[KW], uint64_t (&c)[S])
 {
@@ -230,89 +225,61 @@ __device__ __forceinline__ bool row_contrib(const AggArgs& A, int64_t row, uint6
         DVal p = eval_program<true>(A.pred, A.cols, row);
         if (!(p.valid && p.i != 0)) return false;
     }
-    SrcVal s[kMaxSrc];
 #pragma unroll
-    for (int q = 0; q < kMaxSrc; ++q) {
-        s[q].i = 0;
-        s[q].f = 0.0;
-        s[q].valid = false;
-        s[q].outside = false;
-#pragma unroll
-        for (int k = 0; k < kLimbs; ++k) s[q].limb[k] = 0;
+    for (int w = 0; w < S; ++w) {
+        c[w] = 0;
+        if (w < A.nwords) c[w] = A.words[w].kind == WK_ROWS ? 1ull : word_init(A.words[w]);
     }
-    // One copy of the column reader and of the interpreter; the results are
-    // stored through static indices so that `s` stays in registers.
 #pragma unroll 1
     for (int j = 0; j < A.nsrc; ++j) {
         const SrcDesc sd = A.src[j];
-        DVal v = sd.prog >= 0 ? eval_program<true>(A.progs + sd.prog, A.cols, row) : read_col(A.cols.c[sd.slot], row);
-        SrcVal sv;
-        sv.i = v.i;
-        sv.f = v.f;
-        sv.valid = v.valid;
-        sv.outside = false;
+        const DVal v = sd.prog >= 0 ? eval_program<true>(A.progs + sd.prog, A.cols, row) : read_col(A.cols.c[sd.slot], row);
+        if (!v.valid) continue;  // the words keep their identities
+        int64_t limb[kLimbs];
+        bool outside = false;
 #pragma unroll
-        for (int k = 0; k < kLimbs; ++k) sv.limb[k] = 0;
-        if (v.valid && v.cls == QDB_REG_F64) {
+        for (int k = 0; k < kLimbs; ++k) limb[k] = 0;
+        if (v.cls == QDB_REG_F64) {
 #pragma unroll
-            for (int k = 0; k < kLimbs; ++k) sv.limb[k] = f64_limb(v.f, k, &sv.outside);
-            if (sv.outside) {
+            for (int k = 0; k < kLimbs; ++k) limb[k] = f64_limb(v.f, k, &outside);
+            if (outside) {
 #pragma unroll
-                for (int k = 0; k < kLimbs; ++k) sv.limb[k] = 0;
+                for (int k = 0; k < kLimbs; ++k) limb[k] = 0;
             }
         }
+        const u128 u = (u128)v.i;
 #pragma unroll
-        for (int q = 0; q < kMaxSrc; ++q) {
-            if (q == j) s[q] = sv;
+        for (int w = 0; w < S; ++w) {
+            if (w >= A.nwords) continue;
+            const WordDesc d = A.words[w];
+            if (d.src != j) continue;
+            switch (d.kind) {
+            case WK_VALID: c[w] = 1; break;
+            case WK_SUM64:
+            case WK_MIN_I64:
+            case WK_MAX_I64: c[w] = (uint64_t)u; break;
+            case WK_I128_P0: c[w] = (uint64_t)u & 0xffffffffull; break;
+            case WK_I128_P1: c[w] = (uint64_t)u >> 32; break;
+            case WK_I128_P2: c[w] = (uint64_t)(u >> 64); break;
+            case WK_LIMB: {
+                int64_t l = limb[0];
+#pragma unroll
+                for (int k = 1; k < kLimbs; ++k) {
+                    if (d.part == k) l = limb[k];
+                }
+                c[w] = (uint64_t)l;
+                break;
+            }
+            case WK_OUTSIDE: c[w] = outside ? (uint64_t)__double_as_longlong(v.f) : 0ull; break;
+            case WK_MIN_F64: c[w] = f64_min_key(v.f); break;
+            case WK_MAX_F64: c[w] = f64_max_key(v.f); break;
+            case WK_PAIR_LO: c[w] = (uint64_t)u; break;
+            case WK_PAIR_HI: c[w] = (uint64_t)(u >> 64); break;
+            default: break;
+            }
         }
     }
     row_key<KW>(A, row, key);
-#pragma unroll
-    for (int w = 0; w < S; ++w) {
-        if (w >= A.nwords) { c[w] = 0; continue; }
-        const WordDesc d = A.words[w];
-        SrcVal v = s[0];
-#pragma unroll
-        for (int j = 1; j < kMaxSrc; ++j) {
-            if (d.src == j) v = s[j];
-        }
-        uint64_t out = identity_of(d.op);
-        switch (d.kind) {
-        case WK_ROWS: out = 1; break;
-        case WK_VALID: out = v.valid ? 1 : 0; break;
-        case WK_SUM64: if (v.valid) out = (uint64_t)(int64_t)v.i; break;
-        case WK_I128_P0: if (v.valid) out = (uint64_t)(u128)v.i & 0xffffffffull; break;
-        case WK_I128_P1: if (v.valid) out = ((uint64_t)(u128)v.i >> 32); break;
-        case WK_I128_P2: if (v.valid) out = (uint64_t)((u128)v.i >> 64); break;
-        case WK_LIMB: {
-            int64_t l = v.limb[0];
-#pragma unroll
-            for (int k = 1; k < kLimbs; ++k) {
-                if (d.part == k) l = v.limb[k];
-            }
-            out = v.valid ? (uint64_t)l : 0ull;
-            break;
-        }
-        case WK_OUTSIDE:
-            out = (v.valid && v.outside) ? (uint64_t)__double_as_longlong(v.f) : 0ull;
-            break;
-        case WK_MIN_I64: if (v.valid) out = (uint64_t)(int64_t)v.i; break;
-        case WK_MAX_I64: if (v.valid) out = (uint64_t)(int64_t)v.i; break;
-        case WK_MIN_F64: if (v.valid) out = f64_min_key(v.f); break;
-        case WK_MAX_F64: if (v.valid) out = f64_max_key(v.f); break;
-        case WK_PAIR_LO:
-            out = d.part ? 0ull : ~0ull;  // part: 1 = MAX
-            if (v.valid) out = (uint64_t)(u128)v.i;
-            break;
-        case WK_PAIR_HI:
-            out = d.part ? 0x8000000000000000ull : 0x7fffffffffffffffull;
-            if (v.valid) out = (uint64_t)((u128)v.i >> 64);
-            break;
-        case WK_LOCK: out = 0; break;
-        default: break;
-        }
-        c[w] = out;
-    }
     return true;
 }
 
@@ -320,9 +287,19 @@ __device__ __forceinline__ bool row_contrib(const AggArgs& A, int64_t row, uint6
 // HBM table
 // ---------------------------------------------------------------------------
 
-// Finds or inserts `key`; returns the group id, or -1 when the table is
-// full. A new record's words are set to their identities before the slot is
-// published, so every update after the lookup is an atomic combine.
+// Lookups never wait for another lane. A slot that another lane is still
+// filling makes the lookup return kBusy, and the caller retries in a loop
+// that the whole wave runs together (locate, apply_pairs). A lane that spun
+// inside the lookup could wait forever on a lane of its own wave: the
+// compiler may move the code that publishes an insert after the loop in
+// which the other lanes wait.
+constexpr int64_t kNoRoom = -1;
+constexpr int64_t kBusy = -2;
+
+// Finds or inserts `key` in the HBM table: the group id, kNoRoom when the
+// table is full, or kBusy. A new record's words are set to their identities
+// before the slot is published, so every update after the lookup is an
+// atomic combine.
 template <int KW, int S>
 __device__ int64_t global_find_or_insert(const AggArgs& A, const uint64_t (&key)[KW])
 {
@@ -333,11 +310,12 @@ __device__ int64_t global_find_or_insert(const AggArgs& A, const uint64_t (&key)
     for (;;) {
         unsigned long long c = __hip_atomic_load(&A.ctrl[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         if (c == 0ull) {
-            if (atomicCAS(&A.ctrl[slot], 0ull, 1ull) == 0ull) {
+            c = atomicCAS(&A.ctrl[slot], 0ull, 1ull);
+            if (c == 0ull) {
                 unsigned int gid = atomicAdd(&A.counters[0], 1u);
                 if (gid >= A.max_groups) {
                     __hip_atomic_store(&A.ctrl[slot], 0ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                    return -1;
+                    return kNoRoom;
                 }
                 uint64_t* r = A.recs + (size_t)gid * R;
 #pragma unroll
@@ -348,9 +326,8 @@ __device__ int64_t global_find_or_insert(const AggArgs& A, const uint64_t (&key)
                 __hip_atomic_store(&A.ctrl[slot], tag | (unsigned long long)(gid + 2), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 return gid;
             }
-            continue;  // lost the race: look at the slot again
         }
-        if (c == 1ull) continue;  // being inserted
+        if (c == 1ull) return kBusy;  // being filled
         if ((c & 0xffffffff00000000ull) == tag) {
             const unsigned int gid = (unsigned int)(c & 0xffffffffull) - 2u;
             const uint64_t* r = A.recs + (size_t)gid * R;
@@ -363,75 +340,27 @@ __device__ int64_t global_find_or_insert(const AggArgs& A, const uint64_t (&key)
     }
 }
 
-// i128 MIN/MAX under the group's lock word. The lane that takes the lock
-// finishes its update in the same iteration, so the loop always progresses.
-static __device__ void pair_apply(uint64_t* lo, bool is_max, uint64_t vlo, uint64_t vhi)
+// One attempt at an i128 MIN/MAX under the group's lock word; false when
+// the lock is taken.
+__device__ __forceinline__ bool pair_try(uint64_t* lo, bool is_max, uint64_t vlo, uint64_t vhi)
 {
     uint64_t* hi = lo + 1;
     unsigned long long* lock = (unsigned long long*)(lo + 2);
-    bool done = false;
-    while (!done) {
-        if (atomicCAS(lock, 0ull, 1ull) == 0ull) {
-            i128 cur = (i128)(((u128)__hip_atomic_load(hi, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) << 64)
-                | (u128)__hip_atomic_load(lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            i128 v = (i128)(((u128)vhi << 64) | (u128)vlo);
-            if (is_max ? v > cur : v < cur) {
-                __hip_atomic_store(lo, vlo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(hi, vhi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            __hip_atomic_store(lock, 0ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            done = true;
-        }
+    if (atomicCAS(lock, 0ull, 1ull) != 0ull) return false;
+    i128 cur = (i128)(((u128)__hip_atomic_load(hi, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) << 64)
+        | (u128)__hip_atomic_load(lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    i128 v = (i128)(((u128)vhi << 64) | (u128)vlo);
+    if (is_max ? v > cur : v < cur) {
+        __hip_atomic_store(lo, vlo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(hi, vhi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-}
-
-template <int KW, int S>
-__device__ __forceinline__ void global_apply(const AggArgs& A, int64_t gid, const uint64_t (&c)[S])
-{
-    uint64_t* r = A.recs + (size_t)gid * (KW + A.nwords) + KW;
-#pragma unroll
-    for (int w = 0; w < S; ++w) {
-        if (w >= A.nwords) continue;
-        const uint8_t op = A.words[w].op;
-        if (op == OP_PAIR_MIN || op == OP_PAIR_MAX) {
-            if (w + 1 < S) pair_apply(&r[w], op == OP_PAIR_MAX, c[w], c[w + 1]);
-        } else if (op != OP_PAIR_HI && op != OP_LOCK) {
-            atomic_apply_global(op, &r[w], c[w]);
-        }
-    }
-}
-
-// Applies a contribution to the HBM table, or records it for the replay
-// after the table has grown.
-template <int KW, int S>
-__device__ void global_upsert_or_spill(const AggArgs& A, const uint64_t (&key)[KW], const uint64_t (&c)[S],
-    int64_t row, bool is_row)
-{
-    int64_t gid = global_find_or_insert<KW, S>(A, key);
-    if (gid >= 0) {
-        global_apply<KW, S>(A, gid, c);
-        return;
-    }
-    if (is_row) {
-        unsigned int pos = atomicAdd(&A.counters[1], 1u);
-        A.ov_rows[pos] = (uint32_t)row;
-    } else {
-        unsigned int pos = atomicAdd(&A.counters[2], 1u);
-        if (pos < A.ov_rec_cap) {
-            uint64_t* r = A.ov_recs + (size_t)pos * (KW + A.nwords);
-#pragma unroll
-            for (int w = 0; w < KW; ++w) r[w] = key[w];
-#pragma unroll
-            for (int w = 0; w < S; ++w) {
-                if (w < A.nwords) r[KW + w] = c[w];
-            }
-        }
-    }
+    __hip_atomic_store(lock, 0ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
 }
 
 // ---------------------------------------------------------------------------
-// LDS table: ctrl[lds_slots] (u32: 0 empty, 1 locked, else lgid + 2), a group
-// counter, then dense records of KW + nwords words.
+// LDS table: ctrl[lds_slots] (u32: 0 empty, 1 being filled, else lgid + 2),
+// a group counter, then dense records of KW + nwords words.
 // ---------------------------------------------------------------------------
 
 struct LdsTable {
@@ -440,6 +369,8 @@ struct LdsTable {
     uint64_t* recs;
 };
 
+// The LDS group of `key`, kNoRoom when it is absent and the table is full,
+// or kBusy.
 template <int KW, int S>
 __device__ int lds_find_or_insert(const AggArgs& A, LdsTable& T, const uint64_t (&key)[KW])
 {
@@ -447,15 +378,18 @@ __device__ int lds_find_or_insert(const AggArgs& A, LdsTable& T, const uint64_t 
     const unsigned int mask = (unsigned int)A.lds_slots - 1u;
     unsigned int slot = (unsigned int)(h >> 40) & mask;
     const int R = KW + A.nwords;
-    for (unsigned int probes = 0; probes <= mask; ) {
+    for (unsigned int probes = 0; probes <= mask; ++probes) {
         unsigned int c = __hip_atomic_load(&T.ctrl[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (c == 0u) {
-            if (__hip_atomic_load(T.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= (unsigned int)A.lds_groups) return -1;
-            if (atomicCAS(&T.ctrl[slot], 0u, 1u) == 0u) {
+            if (__hip_atomic_load(T.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= (unsigned int)A.lds_groups) {
+                return (int)kNoRoom;
+            }
+            c = atomicCAS(&T.ctrl[slot], 0u, 1u);
+            if (c == 0u) {
                 unsigned int lg = atomicAdd(T.count, 1u);
                 if (lg >= (unsigned int)A.lds_groups) {
                     __hip_atomic_store(&T.ctrl[slot], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    return -1;
+                    return (int)kNoRoom;
                 }
                 uint64_t* r = T.recs + (size_t)lg * R;
 #pragma unroll
@@ -464,9 +398,8 @@ __device__ int lds_find_or_insert(const AggArgs& A, LdsTable& T, const uint64_t 
                 __hip_atomic_store(&T.ctrl[slot], lg + 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                 return (int)lg;
             }
-            continue;
         }
-        if (c == 1u) continue;
+        if (c == 1u) return (int)kBusy;
         const unsigned int lg = c - 2u;
         const uint64_t* r = T.recs + (size_t)lg * R;
         bool eq = true;
@@ -474,30 +407,121 @@ __device__ int lds_find_or_insert(const AggArgs& A, LdsTable& T, const uint64_t 
         for (int w = 0; w < KW; ++w) eq = eq && r[w] == key[w];
         if (eq) return (int)lg;
         slot = (slot + 1u) & mask;
-        ++probes;
     }
-    return -1;
+    return (int)kNoRoom;
 }
 
+enum Where : int { AT_NONE = 0, AT_LDS = 1, AT_HBM = 2, AT_BUSY = 3 };
+
+// One lookup attempt: the workgroup's LDS table first, then the HBM table.
 template <int KW, int S>
-__device__ __forceinline__ void apply_contrib(const AggArgs& A, LdsTable& T, bool use_lds, const uint64_t (&key)[KW],
-    const uint64_t (&c)[S], int64_t row)
+__device__ __forceinline__ int locate_once(const AggArgs& A, LdsTable& T, bool use_lds, const uint64_t (&key)[KW],
+    int64_t* id)
 {
     if (use_lds) {
-        int lg = lds_find_or_insert<KW, S>(A, T, key);
+        const int lg = lds_find_or_insert<KW, S>(A, T, key);
         if (lg >= 0) {
-            uint64_t* r = T.recs + (size_t)lg * (KW + A.nwords) + KW;
-#pragma unroll
-            for (int w = 0; w < S; ++w) {
-                if (w < A.nwords) atomic_apply_lds(A.words[w].op, &r[w], c[w]);
-            }
-            return;
+            *id = lg;
+            return AT_LDS;
         }
+        if (lg == (int)kBusy) return AT_BUSY;
     }
-    global_upsert_or_spill<KW, S>(A, key, c, row, true);
+    const int64_t gid = global_find_or_insert<KW, S>(A, key);
+    if (gid >= 0) {
+        *id = gid;
+        return AT_HBM;
+    }
+    return gid == kBusy ? AT_BUSY : AT_NONE;
 }
 
-// Butterfly reduction of one word over the lanes in `m` (others give the
+// Wave-uniform: every lane of the wave calls it, lanes with `want` false
+// only take part. Returns where the key's group is (AT_NONE: no room).
+template <int KW, int S>
+__device__ __forceinline__ int locate(const AggArgs& A, LdsTable& T, bool use_lds, bool want, const uint64_t (&key)[KW],
+    int64_t* id)
+{
+    int where = want ? AT_BUSY : AT_NONE;
+    while (__any(where == AT_BUSY)) {
+        if (where == AT_BUSY) where = locate_once<KW, S>(A, T, use_lds, key, id);
+    }
+    return where;
+}
+
+// Combines a contribution into the group's words, except the i128 MIN/MAX
+// pairs (apply_pairs).
+template <int KW, int S>
+__device__ __forceinline__ void apply_words(const AggArgs& A, LdsTable& T, int where, int64_t id, const uint64_t (&c)[S])
+{
+    uint64_t* r = where == AT_LDS ? T.recs + (size_t)id * (KW + A.nwords) + KW : A.recs + (size_t)id * (KW + A.nwords) + KW;
+#pragma unroll
+    for (int w = 0; w < S; ++w) {
+        if (w >= A.nwords) continue;
+        const int op = A.words[w].op;
+        if (op == OP_PAIR_MIN || op == OP_PAIR_MAX || op == OP_PAIR_HI || op == OP_LOCK) continue;
+        atomic_apply_lds((uint8_t)op, &r[w], c[w]);
+    }
+}
+
+// Wave-uniform: the i128 MIN/MAX pairs of HBM group `gid` for the lanes
+// with `want`, each under the group's lock (pairs never live in LDS).
+template <int KW, int S>
+__device__ __forceinline__ void apply_pairs(const AggArgs& A, bool want, int64_t gid, const uint64_t (&c)[S])
+{
+#pragma unroll
+    for (int w = 0; w + 1 < S; ++w) {
+        if (w >= A.nwords) continue;
+        const int op = A.words[w].op;
+        if (op != OP_PAIR_MIN && op != OP_PAIR_MAX) continue;
+        bool pending = want;
+        while (__any(pending)) {
+            if (pending) {
+                uint64_t* r = A.recs + (size_t)gid * (KW + A.nwords) + KW;
+                if (pair_try(&r[w], op == OP_PAIR_MAX, c[w], c[w + 1])) pending = false;
+            }
+        }
+    }
+}
+
+// Records a row for the replay after the HBM table has grown.
+__device__ __forceinline__ void spill_row(const AggArgs& A, int64_t row)
+{
+    const unsigned int pos = atomicAdd(&A.counters[1], 1u);
+    A.ov_rows[pos] = (uint32_t)row;
+}
+
+// Wave-uniform: applies a contribution (the lanes with `want`) to the LDS
+// table or the HBM table; false for a lane whose group found no room.
+template <int KW, int S>
+__device__ __forceinline__ bool apply_contrib(const AggArgs& A, LdsTable& T, bool use_lds, bool want,
+    const uint64_t (&key)[KW], const uint64_t (&c)[S])
+{
+    int64_t id = 0;
+    const int where = locate<KW, S>(A, T, use_lds, want, key, &id);
+    if (want && where != AT_NONE) apply_words<KW, S>(A, T, where, id, c);
+    if (A.has_pairs) apply_pairs<KW, S>(A, want && where == AT_HBM, id, c);
+    return !want || where != AT_NONE;
+}
+
+// Wave-uniform: applies group records (key + state words) to the HBM table,
+// or keeps them for the replay after the table has grown.
+template <int KW, int S>
+__device__ void global_upsert_or_spill(const AggArgs& A, LdsTable& T, bool want, const uint64_t (&key)[KW],
+    const uint64_t (&c)[S])
+{
+    if (apply_contrib<KW, S>(A, T, false, want, key, c)) return;
+    unsigned int pos = atomicAdd(&A.counters[2], 1u);
+    if (pos < A.ov_rec_cap) {
+        uint64_t* r = A.ov_recs + (size_t)pos * (KW + A.nwords);
+#pragma unroll
+        for (int w = 0; w < KW; ++w) r[w] = key[w];
+#pragma unroll
+        for (int w = 0; w < S; ++w) {
+            if (w < A.nwords) r[KW + w] = c[w];
+        }
+    }
+}
+
+// Butterfly reduction of one word over the lanes in `in` (others give the
 // identity); every lane ends with the combined value.
 __device__ __forceinline__ uint64_t wave_reduce(uint8_t op, uint64_t v, bool in)
 {
@@ -514,6 +538,9 @@ __global__ void __launch_bounds__(kBlock) k_agg_update(AggArgs A)
 {
     extern __shared__ uint64_t lds_raw[];
     LdsTable T;
+    T.recs = nullptr;
+    T.ctrl = nullptr;
+    T.count = nullptr;
     const bool use_lds = A.lds_slots > 0 && A.row_index == nullptr;
     if (use_lds) {
         T.recs = lds_raw;
@@ -526,8 +553,8 @@ __global__ void __launch_bounds__(kBlock) k_agg_update(AggArgs A)
     const int lane = lane_id();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t start = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    // Every lane of a wave runs the same number of iterations so that the
-    // wave-level ballots see the whole wave.
+    // Every lane of a wave runs the same number of iterations: the ballots
+    // and the retry loops below see the whole wave.
     const int64_t wave_start = start - lane;
     for (int64_t base = wave_start; base < A.nrows; base += stride) {
         const int64_t idx = base + lane;
@@ -543,6 +570,8 @@ __global__ void __launch_bounds__(kBlock) k_agg_update(AggArgs A)
 #pragma unroll
             for (int w = 0; w < S; ++w) c[w] = 0;
         }
+        // Pre-aggregation: the lanes that share the first remaining lane's
+        // key are combined and applied once by that lane.
         uint64_t active = __ballot(valid);
         for (int round = 0; round < A.match_rounds && active; ++round) {
             const int leader = __ffsll((unsigned long long)active) - 1;
@@ -551,35 +580,42 @@ __global__ void __launch_bounds__(kBlock) k_agg_update(AggArgs A)
             for (int w = 0; w < KW; ++w) lk[w] = shfl_u64(key[w], leader);
             const bool same = valid && ((active >> lane) & 1ull) && keys_equal<KW>(key, lk);
             const uint64_t m = __ballot(same);
+            uint64_t red[S];
             if (__popcll(m) > 1) {
-                uint64_t red[S];
 #pragma unroll
                 for (int w = 0; w < S; ++w) {
-                    red[w] = w < A.nwords ? wave_reduce(A.words[w].op, c[w], same) : 0;
+                    red[w] = w < A.nwords ? wave_reduce((uint8_t)A.words[w].op, c[w], same) : 0;
                 }
-                if (lane == leader) apply_contrib<KW, S>(A, T, use_lds, lk, red, row);
-            } else if (lane == leader) {
-                apply_contrib<KW, S>(A, T, use_lds, key, c, row);
+            } else {
+#pragma unroll
+                for (int w = 0; w < S; ++w) red[w] = c[w];
             }
+            const bool ok = apply_contrib<KW, S>(A, T, use_lds, lane == leader, lk, red);
+            // The combined contribution found no room: every row of the
+            // match is replayed on its own.
+            if (!__shfl((int)ok, leader) && same) spill_row(A, row);
             active &= ~m;
             if (same) valid = false;
         }
-        if (valid) apply_contrib<KW, S>(A, T, use_lds, key, c, row);
+        if (!apply_contrib<KW, S>(A, T, use_lds, valid, key, c)) spill_row(A, row);
     }
     if (!use_lds) return;
     __syncthreads();
-    // Flush the workgroup's groups to the HBM table.
+    // Flush the workgroup's groups to the HBM table; every thread runs the
+    // same number of iterations.
     const unsigned int n = *T.count < (unsigned int)A.lds_groups ? *T.count : (unsigned int)A.lds_groups;
     const int R = KW + A.nwords;
-    for (unsigned int lg = threadIdx.x; lg < n; lg += blockDim.x) {
-        const uint64_t* r = T.recs + (size_t)lg * R;
+    for (unsigned int b = 0; b < n; b += blockDim.x) {
+        const unsigned int lg = b + threadIdx.x;
+        const bool want = lg < n;
         uint64_t key[KW];
         uint64_t c[S];
+        const uint64_t* r = T.recs + (size_t)(want ? lg : 0u) * R;
 #pragma unroll
         for (int w = 0; w < KW; ++w) key[w] = r[w];
 #pragma unroll
         for (int w = 0; w < S; ++w) c[w] = w < A.nwords ? r[KW + w] : 0;
-        global_upsert_or_spill<KW, S>(A, key, c, 0, false);
+        global_upsert_or_spill<KW, S>(A, T, want, key, c);
     }
 }
 
@@ -587,16 +623,23 @@ __global__ void __launch_bounds__(kBlock) k_agg_update(AggArgs A)
 template <int KW, int S>
 __global__ void __launch_bounds__(kBlock) k_agg_apply_records(AggArgs A, const uint64_t* recs, unsigned int n)
 {
+    LdsTable T;
+    T.recs = nullptr;
+    T.ctrl = nullptr;
+    T.count = nullptr;
     const int R = KW + A.nwords;
-    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint64_t* r = recs + (size_t)i * R;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += stride) {
+        const uint64_t i = b + threadIdx.x;
+        const bool want = i < n;
+        const uint64_t* r = recs + (size_t)(want ? i : 0) * R;
         uint64_t key[KW];
         uint64_t c[S];
 #pragma unroll
         for (int w = 0; w < KW; ++w) key[w] = r[w];
 #pragma unroll
         for (int w = 0; w < S; ++w) c[w] = w < A.nwords ? r[KW + w] : 0;
-        global_upsert_or_spill<KW, S>(A, key, c, 0, false);
+        global_upsert_or_spill<KW, S>(A, T, want, key, c);
     }
 }
 
@@ -623,12 +666,12 @@ __global__ void __launch_bounds__(kBlock) k_agg_rehash(unsigned long long* ctrl,
 enum RedKind : uint8_t { RK_COUNT_ROWS, RK_COUNT_VALID, RK_SUM64, RK_SUM128, RK_SUMF64, RK_MINMAX_I64, RK_MINMAX_F64, RK_MINMAX_I128 };
 
 struct RedOut {
-    uint8_t kind;
-    uint8_t out_width;    // bytes per output value
-    uint8_t nullable_out;
-    uint8_t is_max;
-    int16_t w0;           // first state word
-    int16_t wvalid;       // valid-count word, or -1 (then the rows word)
+    int32_t kind;
+    int32_t out_width;    // bytes per output value
+    int32_t nullable_out;
+    int32_t is_max;
+    int32_t w0;           // first state word
+    int32_t wvalid;       // valid-count word, or -1 (then the rows word)
 };
 
 struct FinArgs {
@@ -638,7 +681,7 @@ struct FinArgs {
     unsigned int groups;
     int nkeys;
     KeyFieldDesc keys[QDB_AGG_MAX_KEYS];
-    uint8_t key_out_width[QDB_AGG_MAX_KEYS];
+    int32_t key_out_width[QDB_AGG_MAX_KEYS];
     int nred;
     RedOut red[QDB_AGG_MAX_AGGS];
     int rows_word;

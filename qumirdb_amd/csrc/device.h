@@ -16,18 +16,60 @@ typedef unsigned __int128 u128;
 constexpr int kWave = 64;
 constexpr int kMaxDevCols = 16;
 
+// Every descriptor a kernel reads from its arguments has 4-byte fields: a
+// walk over an array of byte fields can make the compiler address a scalar
+// load from an odd base, and scalar loads ignore the low address bits.
+
 // One input column as a kernel sees it: the TColumn fields plus the value
-// type, packed so that a launch passes its columns by value.
+// type, so that a launch passes its columns by value.
 struct DevCol {
     const char* data;
     const uint8_t* mask;
     int32_t mask_off;
     int32_t bit_off;
-    uint8_t kind;
-    uint8_t width;
-    uint8_t nullable;
-    uint8_t pad[5];
+    int32_t kind;
+    int32_t width;
+    int32_t nullable;
+    int32_t pad;
 };
+
+// An expression instruction as the kernels read it (QdbExprInsn with 4-byte
+// fields; programs are converted when they are uploaded).
+struct DevInsn {
+    int32_t op;
+    int32_t dst;
+    int32_t a;
+    int32_t b;
+    int32_t cls;
+    int32_t column;
+    int64_t lo;
+    int64_t hi;
+};
+
+struct DevProgram {
+    int32_t n;
+    int32_t result;
+    DevInsn insns[QDB_EXPR_MAX_INSNS];
+};
+
+inline DevProgram to_dev_program(const QdbExprProgram& p)
+{
+    DevProgram d;
+    d.n = p.NumInsns;
+    d.result = p.Result;
+    for (int k = 0; k < QDB_EXPR_MAX_INSNS; ++k) {
+        const QdbExprInsn& s = p.Insns[k];
+        d.insns[k].op = s.Op;
+        d.insns[k].dst = s.Dst;
+        d.insns[k].a = s.A;
+        d.insns[k].b = s.B;
+        d.insns[k].cls = s.Class;
+        d.insns[k].column = s.Column;
+        d.insns[k].lo = s.ImmLo;
+        d.insns[k].hi = s.ImmHi;
+    }
+    return d;
+}
 
 struct DevCols {
     DevCol c[kMaxDevCols];
@@ -179,7 +221,7 @@ __device__ __forceinline__ i128 pow10_i128(int64_t k)
 }
 
 template <bool kHasCols>
-__device__ __forceinline__ DVal eval_program(const QdbExprProgram* __restrict__ prog, const DevCols& cols, int64_t row)
+__device__ __forceinline__ DVal eval_program(const DevProgram* __restrict__ prog, const DevCols& cols, int64_t row)
 {
     DVal r[QDB_EXPR_MAX_REGS];
 #pragma unroll
@@ -189,59 +231,59 @@ __device__ __forceinline__ DVal eval_program(const QdbExprProgram* __restrict__ 
         r[k].cls = 0;
         r[k].valid = 0;
     }
-    const int n = prog->NumInsns;
+    const int n = prog->n;
     for (int k = 0; k < n; ++k) {
-        const QdbExprInsn in = prog->Insns[k];
+        const DevInsn in = prog->insns[k];
         DVal a, b;
         // Registers are indexed by uniform values: select through a switch so
         // that the register file stays in VGPRs.
         a = r[0]; b = r[0];
 #pragma unroll
         for (int q = 1; q < QDB_EXPR_MAX_REGS; ++q) {
-            if (in.A == q) a = r[q];
-            if (in.B == q) b = r[q];
+            if (in.a == q) a = r[q];
+            if (in.b == q) b = r[q];
         }
         DVal o;
         o.i = 0;
         o.f = 0.0;
-        o.cls = in.Class;
+        o.cls = in.cls;
         o.valid = 0;
-        switch (in.Op) {
+        switch (in.op) {
         case QDB_EXPR_COLUMN:
             // The column index is uniform: the descriptor is read from the
             // kernel arguments with scalar loads.
-            if (kHasCols) o = read_col(cols.c[in.Column], row);
+            if (kHasCols) o = read_col(cols.c[in.column], row);
             break;
         case QDB_EXPR_CONST_INT:
             o.valid = 1;
-            o.i = (i128)(((u128)(uint64_t)in.ImmHi << 64) | (u128)(uint64_t)in.ImmLo);
+            o.i = (i128)(((u128)(uint64_t)in.hi << 64) | (u128)(uint64_t)in.lo);
             break;
         case QDB_EXPR_CONST_F64:
             o.valid = 1;
-            o.f = __longlong_as_double(in.ImmLo);
+            o.f = __longlong_as_double(in.lo);
             break;
         case QDB_EXPR_ADD:
         case QDB_EXPR_SUB:
         case QDB_EXPR_MUL:
             o.valid = a.valid && b.valid;
-            if (in.Class == QDB_REG_F64) {
+            if (in.cls == QDB_REG_F64) {
                 double x = a.cls == QDB_REG_F64 ? a.f : (double)a.i;
                 double y = b.cls == QDB_REG_F64 ? b.f : (double)b.i;
-                double z = in.Op == QDB_EXPR_ADD ? x + y : in.Op == QDB_EXPR_SUB ? x - y : x * y;
+                double z = in.op == QDB_EXPR_ADD ? x + y : in.op == QDB_EXPR_SUB ? x - y : x * y;
                 o.f = o.valid ? z : 0.0;
             } else if (o.valid) {
                 u128 x = (u128)a.i, y = (u128)b.i;
-                o.i = (i128)(in.Op == QDB_EXPR_ADD ? x + y : in.Op == QDB_EXPR_SUB ? x - y : x * y);
+                o.i = (i128)(in.op == QDB_EXPR_ADD ? x + y : in.op == QDB_EXPR_SUB ? x - y : x * y);
             }
             break;
         case QDB_EXPR_NEG:
             o.valid = a.valid;
-            if (in.Class == QDB_REG_F64) o.f = a.valid ? -a.f : 0.0;
+            if (in.cls == QDB_REG_F64) o.f = a.valid ? -a.f : 0.0;
             else if (a.valid) o.i = (i128)((u128)0 - (u128)a.i);
             break;
         case QDB_EXPR_SCALE_UP:
             o.valid = a.valid;
-            if (a.valid) o.i = (i128)((u128)a.i * (u128)pow10_i128(in.ImmLo));
+            if (a.valid) o.i = (i128)((u128)a.i * (u128)pow10_i128(in.lo));
             break;
         case QDB_EXPR_CMP: {
             o.cls = QDB_REG_BOOL;
@@ -250,7 +292,7 @@ __device__ __forceinline__ DVal eval_program(const QdbExprProgram* __restrict__ 
             if (a.cls == QDB_REG_F64 || b.cls == QDB_REG_F64) {
                 double x = a.cls == QDB_REG_F64 ? a.f : (double)a.i;
                 double y = b.cls == QDB_REG_F64 ? b.f : (double)b.i;
-                switch (in.ImmLo) {
+                switch (in.lo) {
                 case QDB_CMP_EQ: c = x == y; break;
                 case QDB_CMP_NE: c = x != y; break;
                 case QDB_CMP_LT: c = x < y; break;
@@ -260,7 +302,7 @@ __device__ __forceinline__ DVal eval_program(const QdbExprProgram* __restrict__ 
                 }
             } else {
                 i128 x = a.i, y = b.i;
-                switch (in.ImmLo) {
+                switch (in.lo) {
                 case QDB_CMP_EQ: c = x == y; break;
                 case QDB_CMP_NE: c = x != y; break;
                 case QDB_CMP_LT: c = x < y; break;
@@ -303,13 +345,13 @@ __device__ __forceinline__ DVal eval_program(const QdbExprProgram* __restrict__ 
         }
 #pragma unroll
         for (int q = 0; q < QDB_EXPR_MAX_REGS; ++q) {
-            if (in.Dst == q) r[q] = o;
+            if (in.dst == q) r[q] = o;
         }
     }
     DVal res = r[0];
 #pragma unroll
     for (int q = 1; q < QDB_EXPR_MAX_REGS; ++q) {
-        if (prog->Result == q) res = r[q];
+        if (prog->result == q) res = r[q];
     }
     return res;
 }

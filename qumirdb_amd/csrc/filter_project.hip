@@ -31,7 +31,7 @@ struct ExprPlan {
     int nslots = 0;
     int slot_col[kMaxDevCols];
     int nprogs = 0;
-    QdbExprProgram* d_progs = nullptr;
+    DevProgram* d_progs = nullptr;
     QdbValueType out_types[QDB_PROJECT_MAX_OUTPUTS];
     ~ExprPlan() { dfree(d_progs); }
 };
@@ -68,7 +68,9 @@ bool compile_programs(const QdbColumnSchema& input, const QdbExprProgram* progs,
     }
     P->nprogs = n;
     if (n > 0) {
-        P->d_progs = (QdbExprProgram*)upload(out.data(), out.size() * sizeof(QdbExprProgram));
+        std::vector<DevProgram> dev((size_t)n);
+        for (int p = 0; p < n; ++p) dev[(size_t)p] = to_dev_program(out[(size_t)p]);
+        P->d_progs = (DevProgram*)upload(dev.data(), dev.size() * sizeof(DevProgram));
         if (!P->d_progs) return false;
     }
     return true;
@@ -88,13 +90,13 @@ bool bind_cols(const ExprPlan* P, const TRowSet* rs, DevCols* cols)
         d.mask_off = col.MaskBitOffset;
         d.bit_off = col.DataBitOffset;
         d.kind = t.Kind;
-        d.width = (uint8_t)t.Width;
+        d.width = t.Width;
         d.nullable = t.Nullable;
     }
     return true;
 }
 
-__global__ void __launch_bounds__(kBlock) k_filter(DevCols cols, const QdbExprProgram* prog, int64_t n,
+__global__ void __launch_bounds__(kBlock) k_filter(DevCols cols, const DevProgram* prog, int64_t n,
     const uint8_t* in_sel, uint8_t* out_sel)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -111,8 +113,8 @@ __global__ void __launch_bounds__(kBlock) k_filter(DevCols cols, const QdbExprPr
 struct ProjectOut {
     char* data[QDB_PROJECT_MAX_OUTPUTS];
     uint8_t* mask[QDB_PROJECT_MAX_OUTPUTS];
-    uint8_t kind[QDB_PROJECT_MAX_OUTPUTS];
-    uint8_t width[QDB_PROJECT_MAX_OUTPUTS];
+    int32_t kind[QDB_PROJECT_MAX_OUTPUTS];
+    int32_t width[QDB_PROJECT_MAX_OUTPUTS];
 };
 
 // Writes 64 bits of a bitmap for the 64 rows starting at `base` (a multiple
@@ -131,7 +133,7 @@ __device__ __forceinline__ void store_bits(uint8_t* bitmap, int64_t base, int64_
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_project(DevCols cols, const QdbExprProgram* progs, int nout, ProjectOut O,
+__global__ void __launch_bounds__(kBlock) k_project(DevCols cols, const DevProgram* progs, int nout, ProjectOut O,
     int64_t n, const uint8_t* in_sel)
 {
     const int lane = lane_id();
@@ -261,7 +263,7 @@ void project_impl(ExprPlan* P, TRowSet* in, void** out, const uint8_t* input_sel
         O.data[k] = (char*)out[k];
         O.mask[k] = P->out_types[k].Nullable ? (uint8_t*)out[P->nprogs + k] : nullptr;
         O.kind[k] = P->out_types[k].Kind;
-        O.width[k] = (uint8_t)P->out_types[k].Width;
+        O.width[k] = P->out_types[k].Width;
     }
     {
         TimedRegion timer("project");
@@ -291,7 +293,7 @@ bool hash_impl(HashPlan* P, TRowSet* rs, uint64_t* hashes, const int64_t* key_co
         d.mask_off = col.MaskBitOffset;
         d.bit_off = col.DataBitOffset;
         d.kind = t.Kind;
-        d.width = (uint8_t)t.Width;
+        d.width = t.Width;
         d.nullable = t.Nullable;
         if (t.Nullable) scalar = false;
     }

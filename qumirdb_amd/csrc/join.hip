@@ -45,11 +45,11 @@ constexpr uint32_t kLockHead = 0xffffffffu;
 constexpr int kMaxFused = QDB_JOIN_MAX_FUSED;
 
 struct JKeyField {
-    int8_t slot;
-    uint8_t kind;
-    uint8_t width;
-    uint8_t word;
-    uint8_t shift;
+    int32_t slot;
+    int32_t kind;
+    int32_t width;
+    int32_t word;
+    int32_t shift;
 };
 
 struct JKeyDesc {
@@ -156,9 +156,9 @@ __global__ void __launch_bounds__(1024) k_scan_counts(unsigned int* counts, unsi
 struct ArenaCols {
     char* data[64];
     uint8_t* valid[64];
-    uint8_t width[64];
-    uint8_t kind[64];
-    int8_t src_slot[64];
+    int32_t width[64];
+    int32_t kind[64];
+    int32_t src_slot[64];
 };
 
 // The arena payload is written for every build column the plan keeps.
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(kBlock) k_append_write(DevCols cols, JKeyDesc 
         if (AC.valid[p]) AC.valid[p][dst] = v ? 1 : 0;
         char* d = AC.data[p];
         if (AC.kind[p] == QDB_KIND_BOOL) {
-            ((uint8_t*)d)[dst] = v ? (uint8_t)bit_at((const uint8_t*)c.data, i, c.bit_off) : 0;
+            ((uint8_t*)d)[dst] = v ? bit_at((const uint8_t*)c.data, i, c.bit_off) : 0;
             continue;
         }
         switch (AC.width[p]) {
@@ -232,45 +232,58 @@ __device__ __forceinline__ uint32_t bloom_bits(uint64_t h)
     return (1u << ((h >> 52) & 31u)) | (1u << ((h >> 58) & 31u));
 }
 
+// One insert attempt of arena row `r`; false when the slot it needs is
+// being filled by another lane. The caller retries in a loop that the whole
+// wave runs together: a lane that spun here could wait forever on a lane of
+// its own wave whose publishing store the compiler placed after the loop.
+template <int KW>
+__device__ __forceinline__ bool build_insert_once(Slot<KW>* table, uint64_t cmask, const uint64_t (&key)[KW], uint64_t h,
+    int64_t r, uint32_t* next)
+{
+    uint64_t s = h & cmask;
+    for (;;) {
+        Slot<KW>* sl = &table[s];
+        uint32_t head = __hip_atomic_load(&sl->head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (head == 0u) {
+            head = atomicCAS(&sl->head, 0u, kLockHead);
+            if (head == 0u) {
+#pragma unroll
+                for (int w = 0; w < KW; ++w) __hip_atomic_store(&sl->key[w], key[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                next[r] = 0u;
+                __hip_atomic_store(&sl->head, (uint32_t)(r + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                return true;
+            }
+        }
+        if (head == kLockHead) return false;
+        bool eq = true;
+#pragma unroll
+        for (int w = 0; w < KW; ++w) eq = eq && __hip_atomic_load(&sl->key[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == key[w];
+        if (eq) {
+            // Push on the chain; the order is fixed up afterwards. A
+            // published head is never the lock value again.
+            next[r] = atomicExch(&sl->head, (uint32_t)(r + 1));
+            return true;
+        }
+        s = (s + 1) & cmask;
+    }
+}
+
 template <int KW>
 __global__ void __launch_bounds__(kBlock) k_build_insert(Slot<KW>* table, uint64_t cmask, const uint64_t* arena_keys,
     int64_t rows, uint32_t* next, uint32_t* bloom, uint32_t bloom_mask)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += stride) {
+    // Every thread of the block runs the same number of iterations.
+    for (int64_t b = blockIdx.x * (int64_t)blockDim.x; b < rows; b += stride) {
+        const int64_t r = b + threadIdx.x;
+        bool pending = r < rows;
         uint64_t key[KW];
 #pragma unroll
-        for (int w = 0; w < KW; ++w) key[w] = arena_keys[r * KW + w];
+        for (int w = 0; w < KW; ++w) key[w] = pending ? arena_keys[r * KW + w] : 0ull;
         const uint64_t h = key_place_hash<KW>(key);
-        atomicOr(&bloom[bloom_word(h, bloom_mask)], bloom_bits(h));
-        uint64_t s = h & cmask;
-        bool done = false;
-        while (!done) {
-            Slot<KW>* sl = &table[s];
-            uint32_t head = __hip_atomic_load(&sl->head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            if (head == 0u) {
-                if (atomicCAS(&sl->head, 0u, kLockHead) == 0u) {
-#pragma unroll
-                    for (int w = 0; w < KW; ++w) __hip_atomic_store(&sl->key[w], key[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    next[r] = 0u;
-                    __hip_atomic_store(&sl->head, (uint32_t)(r + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                    done = true;
-                }
-                continue;
-            }
-            if (head == kLockHead) continue;
-            bool eq = true;
-#pragma unroll
-            for (int w = 0; w < KW; ++w) eq = eq && __hip_atomic_load(&sl->key[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == key[w];
-            if (eq) {
-                // Push on the chain; the order is fixed up afterwards.
-                uint32_t old = atomicExch(&sl->head, (uint32_t)(r + 1));
-                while (old == kLockHead) old = atomicExch(&sl->head, (uint32_t)(r + 1));
-                next[r] = old;
-                done = true;
-                continue;
-            }
-            s = (s + 1) & cmask;
+        if (pending) atomicOr(&bloom[bloom_word(h, bloom_mask)], bloom_bits(h));
+        while (__any(pending)) {
+            if (pending && build_insert_once<KW>(table, cmask, key, h, r, next)) pending = false;
         }
     }
 }
@@ -317,10 +330,10 @@ __global__ void __launch_bounds__(kBlock) k_build_chains(Slot<KW>* table, uint64
 enum FusedKind : uint8_t { FK_COUNT_PAIRS, FK_COUNT_VALID, FK_SUM, FK_MIN, FK_MAX };
 
 struct FusedDesc {
-    uint8_t kind;
-    uint8_t from_build;   // argument from the build arena
-    int8_t slot;          // probe column slot
-    int8_t payload;       // build payload index
+    int32_t kind;
+    int32_t from_build;   // argument from the build arena
+    int32_t slot;         // probe column slot
+    int32_t payload;      // build payload index
 };
 
 struct ProbeArgs {
@@ -344,7 +357,7 @@ struct ProbeArgs {
     FusedDesc fused[kMaxFused];
     const char* payload_data[kMaxFused];
     const uint8_t* payload_valid[kMaxFused];
-    uint8_t payload_width[kMaxFused];
+    int32_t payload_width[kMaxFused];
     unsigned long long* fused_acc;  // per fused agg: [count, p0, p1, p2, min, max] words
 };
 
@@ -556,12 +569,12 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe(ProbeArgs P)
 // ---------------------------------------------------------------------------
 
 struct GatherCol {
-    uint8_t from_build;
-    uint8_t kind;
-    uint8_t width;
-    uint8_t nullable_out;
-    int8_t slot;       // probe column slot
-    int16_t payload;   // build payload index
+    int32_t from_build;
+    int32_t kind;
+    int32_t width;
+    int32_t nullable_out;
+    int32_t slot;      // probe column slot
+    int32_t payload;   // build payload index
     const char* arena_data;
     const uint8_t* arena_valid;
     char* out_data;
@@ -631,14 +644,14 @@ __global__ void __launch_bounds__(kBlock) k_materialize(GatherArgs G)
                     }
                 }
                 const int bit = (int)(o & 7);
-                if (valid) mbits |= (uint8_t)(1u << bit);
+                if (valid) mbits |= (1u << bit);
                 if (gc.kind == QDB_KIND_BOOL) {
-                    if (valid && v) dbits |= (uint8_t)(1u << bit);
+                    if (valid && v) dbits |= (1u << bit);
                     continue;
                 }
                 char* d = gc.out_data;
                 switch (gc.width) {
-                case 1: ((uint8_t*)d)[o] = (uint8_t)v; break;
+                case 1: ((uint8_t*)d)[o] = v; break;
                 case 2: ((uint16_t*)d)[o] = (uint16_t)v; break;
                 case 4: ((uint32_t*)d)[o] = (uint32_t)v; break;
                 case 8: ((uint64_t*)d)[o] = (uint64_t)v; break;
@@ -767,14 +780,14 @@ bool compile_join(const QdbRowJoinSpec& spec, JoinPlan* P)
         byte = (byte + align - 1) / align * align;
         JKeyField f;
         f.kind = bt.Kind;
-        f.width = (uint8_t)w;
-        f.word = (uint8_t)(byte / 8);
-        f.shift = (uint8_t)((byte % 8) * 8);
-        f.slot = (int8_t)P->build_nslots;
+        f.width = w;
+        f.word = (byte / 8);
+        f.shift = ((byte % 8) * 8);
+        f.slot = P->build_nslots;
         P->build_key.f[k] = f;
         P->build_slot_col[P->build_nslots++] = bc;
         f.kind = pt.Kind;
-        f.slot = (int8_t)P->probe_nslots;
+        f.slot = P->probe_nslots;
         P->probe_key.f[k] = f;
         P->probe_slot_col[P->probe_nslots++] = pc;
         byte += w;
@@ -812,7 +825,7 @@ bool compile_join(const QdbRowJoinSpec& spec, JoinPlan* P)
         if (fn.Func == QDB_AGG_SUM && t.Kind == QDB_KIND_DATE) return fail("function sum(DATE) does not exist");
         d.from_build = from_build;
         if (from_build) {
-            d.payload = (int8_t)col;
+            d.payload = col;
         } else {
             int s = -1;
             for (int q = 0; q < P->probe_nslots; ++q) {
@@ -823,7 +836,7 @@ bool compile_join(const QdbRowJoinSpec& spec, JoinPlan* P)
                 s = P->probe_nslots;
                 P->probe_slot_col[P->probe_nslots++] = col;
             }
-            d.slot = (int8_t)s;
+            d.slot = s;
         }
     }
     // Output columns (left side first).
@@ -839,9 +852,9 @@ bool compile_join(const QdbRowJoinSpec& spec, JoinPlan* P)
             memset(&g, 0, sizeof(g));
             g.from_build = build_side;
             g.kind = t.Kind;
-            g.width = (uint8_t)(t.Kind == QDB_KIND_BOOL ? 1 : t.Width);
+            g.width = (t.Kind == QDB_KIND_BOOL ? 1 : t.Width);
             g.nullable_out = t.Nullable;
-            g.payload = (int16_t)c;
+            g.payload = c;
             g.slot = -1;
             ++P->nout;
         }
@@ -869,7 +882,7 @@ bool bind_side(const int* slot_col, int nslots, const QdbColumnSchema& sch, cons
         d.mask_off = col.MaskBitOffset;
         d.bit_off = col.DataBitOffset;
         d.kind = t.Kind;
-        d.width = (uint8_t)t.Width;
+        d.width = t.Width;
         d.nullable = t.Nullable;
     }
     return true;
@@ -962,9 +975,9 @@ bool do_append(JoinBuild* J, const TRowSet* batch)
             const QdbValueType& t = P->spec.Build.Types[p0 + q];
             AC.data[q] = J->pdata[(size_t)(p0 + q)];
             AC.valid[q] = t.Nullable ? J->pvalid[(size_t)(p0 + q)] : nullptr;
-            AC.width[q] = (uint8_t)(t.Kind == QDB_KIND_BOOL ? 1 : t.Width);
+            AC.width[q] = (t.Kind == QDB_KIND_BOOL ? 1 : t.Width);
             AC.kind[q] = t.Kind;
-            AC.src_slot[q] = (int8_t)q;
+            AC.src_slot[q] = q;
         }
         if (!bind_side(slot_col, np, P->spec.Build, batch, &pcols)) return false;
         hipLaunchKernelGGL((k_append_write<KW>), dim3(nblocks), dim3(kBlock), 0, stream(), cols, P->build_key, n,
@@ -1049,7 +1062,7 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
             const QdbValueType& t = P->spec.Build.Types[col];
             A.payload_data[f] = J->pdata[(size_t)col];
             A.payload_valid[f] = t.Nullable ? J->pvalid[(size_t)col] : nullptr;
-            A.payload_width[f] = (uint8_t)t.Width;
+            A.payload_width[f] = t.Width;
         }
     }
     A.fused_acc = J->fused_acc;
@@ -1296,7 +1309,7 @@ int64_t jr_materialize_impl(JoinPlan* P, void* pairs, TRowSet* stream_rs, int64_
             GatherCol gc = all[c];
             if (!gc.from_build) {
                 if (ns == kMaxDevCols) break;
-                gc.slot = (int8_t)ns;
+                gc.slot = ns;
                 slot_col[ns++] = gc.payload;
             }
             g.col[g.ncols++] = gc;

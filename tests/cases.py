@@ -245,11 +245,17 @@ class OracleBackend:
 class GpuBackend:
     name = "gpu"
 
-    def __init__(self):
+    def __init__(self, init=True):
         import qumirdb_amd as q
         from qumirdb_amd import host as H
-        q.init_device(0)
+        if init:
+            q.init_device(0)
         self.q, self.H = q, H
+
+    def cast(self, s):
+        """The same struct as the HIP library's binding class (the oracle loads
+        its own copy of the ABI module)."""
+        return getattr(self.q.abi, type(s).__name__).from_buffer_copy(s)
 
     def dev(self, b: Batch):
         H = self.H
@@ -261,7 +267,7 @@ class GpuBackend:
         H = self.H
         kern = H.AggregateKernels.__new__(H.AggregateKernels)
         kern.spec = None
-        kern.c_spec = case.to_c()
+        kern.c_spec = self.cast(case.to_c())
         kern.k = self.q.abi.QdbAggregateKernels()
         self.q.check(self.q.lib().qdb_gpu_compile_aggregate(C.byref(kern.c_spec), C.byref(kern.k)), "compile_aggregate")
         kern.out_types = [H.ValueType(t.Kind, t.Width, t.Scale, bool(t.Nullable)) for t in kern.k.OutputTypes[: kern.k.NumOutputColumns]]
@@ -278,7 +284,7 @@ class GpuBackend:
         H = self.H
         kern = H.RowJoinKernels.__new__(H.RowJoinKernels)
         kern.spec = case
-        kern.c_spec = case.to_c()
+        kern.c_spec = self.cast(case.to_c())
         kern.k = self.q.abi.QdbRowJoinKernels()
         self.q.check(self.q.lib().qdb_gpu_compile_row_join(C.byref(kern.c_spec), C.byref(kern.k)), "compile_row_join")
         kern.out_types = [H.ValueType(t.Kind, t.Width, t.Scale, bool(t.Nullable)) for t in kern.k.OutputTypes[: kern.k.NumOutputColumns]]
@@ -302,8 +308,8 @@ class GpuBackend:
     def filter(self, types, prog, res, batch: Batch):
         abi = self.q.abi
         spec = abi.QdbFilterSpec()
-        spec.Input = schema_c(types)
-        spec.Predicate = prog.build(res)
+        spec.Input = abi.QdbColumnSchema.from_buffer_copy(schema_c(types))
+        spec.Predicate = abi.QdbExprProgram.from_buffer_copy(prog.build(res))
         fn = C.c_void_p()
         slot = self.q.check(self.q.lib().qdb_gpu_compile_filter(C.byref(spec), C.byref(fn)), "compile_filter")
         f = abi.FILTER_FN(fn.value)
@@ -320,11 +326,11 @@ class GpuBackend:
         abi = self.q.abi
         H = self.H
         spec = abi.QdbProjectSpec()
-        spec.Input = schema_c(types)
+        spec.Input = abi.QdbColumnSchema.from_buffer_copy(schema_c(types))
         spec.NumOutputs = len(outputs)
         for k, (p, r, t) in enumerate(outputs):
-            spec.OutputTypes[k] = value_type_c(t)
-            spec.Outputs[k] = p.build(r)
+            spec.OutputTypes[k] = abi.QdbValueType.from_buffer_copy(value_type_c(t))
+            spec.Outputs[k] = abi.QdbExprProgram.from_buffer_copy(p.build(r))
         fn = C.c_void_p()
         slot = self.q.check(self.q.lib().qdb_gpu_compile_project(C.byref(spec), C.byref(fn)), "compile_project")
         f = abi.PROJECT_FN(fn.value)

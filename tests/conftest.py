@@ -21,3 +21,11 @@ def gpu():
     import qumirdb_amd
     qumirdb_amd.init_device(0)
     return qumirdb_amd
+
+
+def pytest_runtest_logreport(report):
+    """Prints a failure's report as soon as the test ends, so that it is
+    kept even when a later test stops the run."""
+    if report.failed and report.when == "call":
+        sys.stdout.write("\n--- %s failed ---\n%s\n" % (report.nodeid, report.longreprtext[-4000:]))
+        sys.stdout.flush()
