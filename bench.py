@@ -49,14 +49,16 @@ def timing(family):
     return ms, n.value
 
 
-def bench_join(steps, warmup, build_rows=1_500_000, probe_rows=60_000_000):
+def bench_join(steps, warmup, build_rows=1_500_000, probe_rows=60_000_000, table_mode=0, jd=None):
+    """table_mode 0: the build picks its table (dense direct table for these
+    keys); 1: the hash table."""
     import ctypes
     from qumirdb_amd import lib, workloads as W
     from qumirdb_amd.host import DeviceBuffer, FilterKernel, RowJoinKernels, TRowJoinProcessor
 
-    jd = W.JoinDevice(build_rows, probe_rows)
+    jd = jd or W.JoinDevice(build_rows, probe_rows)
     pred = FilterKernel(W.BUILD_TYPES, *W.join_build_predicate())
-    kern = RowJoinKernels(W.join_spec(fused=True))
+    kern = RowJoinKernels(W.join_spec(fused=True, table_mode=table_mode))
     sel = DeviceBuffer(build_rows)
     results = {}
 
@@ -86,7 +88,9 @@ def bench_join(steps, warmup, build_rows=1_500_000, probe_rows=60_000_000):
     t1 = time.perf_counter()
     lib().qdb_gpu_timing_enable(0)
     probe_ms, launches = timing("join_probe")
-    build_ms, _ = timing("join_build")
+    build_ms, nb = timing("join_build")
+    place_ms, npl = timing("join_build_place")
+    build_ms = (build_ms * nb + place_ms * npl) / steps
     ms_per_step = (t1 - t0) * 1e3 / steps
     pairs = results["pairs"]
     probe_bytes = probe_rows * 16 + pairs * 16
@@ -98,7 +102,8 @@ def bench_join(steps, warmup, build_rows=1_500_000, probe_rows=60_000_000):
         "probe_kernel_ms": probe_ms, "probe_launches": launches, "build_kernel_ms": build_ms,
         "pairs": pairs, "count": results["count"], "sum_value_unscaled": str(results["sum"]),
         "algorithmic_bytes_per_probe": probe_bytes, "achieved_gbs": achieved,
-    }
+        "table": "hash" if table_mode == 1 else "auto",
+    }, jd
 
 
 def bench_lineitem(steps, warmup, rows=6_000_000):
@@ -201,7 +206,9 @@ def main():
     qumirdb_amd.init_device(0)
     out = {}
     if args.workload in ("join", "all"):
-        out["join"] = bench_join(args.steps, args.warmup)
+        out["join"], jd = bench_join(args.steps, args.warmup)
+        out["join_hash_table"], jd = bench_join(args.steps, args.warmup, table_mode=1, jd=jd)
+        del jd
     if args.workload in ("lineitem", "all"):
         out["lineitem"] = bench_lineitem(args.steps, args.warmup)
     if args.workload in ("groups", "all"):
@@ -217,6 +224,10 @@ def main():
         roof = {"bound": "hbm", "achieved": round(j["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(j["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_probe",
                 "kernel_ms": round(j["probe_kernel_ms"], 4)}
+    import ctypes
+    from qumirdb_amd import lib
+    limits = (ctypes.c_int64 * 2)()
+    lib().qdb_gpu_device_limits(limits, 2)
     line = {
         "metric": "rows/sec + achieved HBM GB/s (% of peak) for agg/join; queries/s and p99 latency",
         "value": head["rows_per_s"], "unit": "rows/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -226,6 +237,7 @@ def main():
                                "COUNT(*), exact SUM(value), matched pairs" if name == "join" else name,
                    "parallelism": "single GPU"},
         "roofline": roof, "cpu_baseline": cpu,
+        "device": {"compute_units": limits[0], "max_lds_per_workgroup": limits[1]},
         "workloads": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in out.items()},
     }
     print(json.dumps(line), flush=True)

@@ -307,7 +307,15 @@ typedef struct QdbRowJoinSpec {
     QdbAggregateFn Fused[QDB_JOIN_MAX_FUSED];
     /* Output buffers of jr_materialize: 0 = device memory, 1 = host. */
     int32_t OutputOnHost;
+    /* Build table: QDB_JOIN_TABLE_AUTO picks a direct table indexed by
+     * key - min (a perfect hash) when the build has one integer key, distinct
+     * keys and a key range of at most 8 x its rows (or 2^16); otherwise, or
+     * with QDB_JOIN_TABLE_HASH, the open-addressing hash table. Results are
+     * the same either way. */
+    int32_t TableMode;
 } QdbRowJoinSpec;
+
+enum { QDB_JOIN_TABLE_AUTO = 0, QDB_JOIN_TABLE_HASH = 1 };
 
 typedef bool (*QdbJoinDispatchFn)(void* arena, void* table, void* scratch, void* pairs,
     TRowSet* batch, const int64_t* key_columns, int64_t op);
@@ -408,6 +416,10 @@ double qdb_gpu_timing_avg_ms(const char* family, int64_t* launches);
 /* Sizes of the structs above (TColumn ... QdbFusedResult, in declaration
  * order) so that bindings can check their layouts; returns the count. */
 int64_t qdb_gpu_struct_sizes(int64_t* out, int64_t n);
+
+/* Device limits the kernels size themselves by: [compute units, largest LDS
+ * allocation of one workgroup in bytes]. Returns the count written, or -1. */
+int64_t qdb_gpu_device_limits(int64_t* out, int64_t n);
 
 /* ------------------------------------------------------------------------ */
 /* Host processors over the entry points above, mirroring qdb/exec:         */

@@ -13,7 +13,8 @@ import os
 from . import abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libqdb_gpu.so")
+# QDB_GPU_LIB selects another build of the library (same C-ABI).
+LIB_PATH = os.environ.get("QDB_GPU_LIB") or os.path.join(_HERE, "libqdb_gpu.so")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(

@@ -30,6 +30,7 @@ AGG_COUNT_STAR, AGG_COUNT, AGG_SUM, AGG_MIN, AGG_MAX = range(5)
 AGG_OP_INIT, AGG_OP_UPDATE, AGG_OP_INIT_ROWS, AGG_OP_SWITCH_TO_ROWS_AT, AGG_OP_EXPECT_ROW_GROUPS = 0, 1, 3, 4, 5
 JR_OP_INIT, JR_OP_APPEND, JR_OP_BUILD, JR_OP_PROBE = 0, 1, 2, 3
 JOIN_INNER, JOIN_LEFT_OUTER, JOIN_RIGHT_OUTER = range(3)
+JOIN_TABLE_AUTO, JOIN_TABLE_HASH = range(2)
 SLOT_FILTER, SLOT_PROJECT, SLOT_AGGREGATE, SLOT_JOIN, SLOT_HASH = range(5)
 
 HASH_TABLE_SIZE = 304
@@ -156,6 +157,7 @@ class QdbRowJoinSpec(C.Structure):
         ("NumFused", C.c_int32),
         ("Fused", QdbAggregateFn * JOIN_MAX_FUSED),
         ("OutputOnHost", C.c_int32),
+        ("TableMode", C.c_int32),
     ]
 
 
@@ -191,6 +193,7 @@ EXPORTS = [
     "qdb_gpu_memset", "qdb_gpu_synchronize", "qdb_gpu_stage_rowset", "qdb_gpu_download_rowset",
     "qdb_gpu_gen_lineitem", "qdb_gpu_gen_join_build", "qdb_gpu_gen_join_probe", "qdb_gpu_gen_zipf_groups",
     "qdb_gpu_timing_enable", "qdb_gpu_timing_reset", "qdb_gpu_timing_avg_ms", "qdb_gpu_struct_sizes",
+    "qdb_gpu_device_limits",
     "qdb_gpu_exec_last_error", "qdb_gpu_agg_processor_new", "qdb_gpu_agg_processor_add",
     "qdb_gpu_agg_processor_finish", "qdb_gpu_agg_processor_groups", "qdb_gpu_agg_processor_free",
     "qdb_gpu_join_processor_new", "qdb_gpu_join_processor_append", "qdb_gpu_join_processor_build",
@@ -232,6 +235,7 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "qdb_gpu_timing_enable": (None, [C.c_int]),
         "qdb_gpu_timing_reset": (None, []),
         "qdb_gpu_timing_avg_ms": (C.c_double, [C.c_char_p, C.POINTER(i64)]),
+        "qdb_gpu_device_limits": (i64, [C.POINTER(i64), i64]),
         "qdb_gpu_struct_sizes": (i64, [C.POINTER(i64), i64]),
         "qdb_gpu_exec_last_error": (C.c_char_p, []),
         "qdb_gpu_agg_processor_new": (P, [C.POINTER(QdbAggregateKernels), i64, C.c_int, i64]),

@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libqdb_gpu.so")
 
 SOURCES = ["runtime.hip", "filter_project.hip", "aggregate.hip", "aggregate_kw1.hip", "aggregate_kw2.hip",
-           "aggregate_kw3.hip", "aggregate_kw4.hip", "join.hip", "exec.cpp"]
+           "aggregate_kw3.hip", "aggregate_kw4.hip", "join.hip", "primitives.hip", "exec.cpp"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = os.environ.get("QDB_GPU_ARCH", "gfx950")

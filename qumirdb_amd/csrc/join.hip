@@ -219,8 +219,14 @@ __global__ void __launch_bounds__(kBlock) k_append_write(DevCols cols, JKeyDesc 
 }
 
 // ---------------------------------------------------------------------------
-// Build: insert, order chains, Bloom filter.
+// Build: distinct keys, CSR order, the probe table, Bloom filter.
 // ---------------------------------------------------------------------------
+//
+// The insert-time slot {key, head, count} becomes the probe-time slot after
+// the build: count 0 = empty; count 1: head = the key's build ordinal;
+// count > 1: head = the first of the key's `count` rows in CSR order (the
+// rows of a key contiguous, in build order). A match costs one 16-byte slot
+// read (KW = 1).
 
 __device__ __forceinline__ uint32_t bloom_word(uint64_t h, uint32_t words_mask)
 {
@@ -232,13 +238,14 @@ __device__ __forceinline__ uint32_t bloom_bits(uint64_t h)
     return (1u << ((h >> 52) & 31u)) | (1u << ((h >> 58) & 31u));
 }
 
-// One insert attempt of arena row `r`; false when the slot it needs is
-// being filled by another lane. The caller retries in a loop that the whole
-// wave runs together: a lane that spun here could wait forever on a lane of
-// its own wave whose publishing store the compiler placed after the loop.
+// One insert attempt of arena row `r`: the slot of its key, or -1 when the
+// slot it needs is being filled by another lane. The caller retries in a
+// loop that the whole wave runs together: a lane that spun here could wait
+// forever on a lane of its own wave whose publishing store the compiler
+// placed after the loop.
 template <int KW>
-__device__ __forceinline__ bool build_insert_once(Slot<KW>* table, uint64_t cmask, const uint64_t (&key)[KW], uint64_t h,
-    int64_t r, uint32_t* next)
+__device__ __forceinline__ int64_t build_insert_once(Slot<KW>* table, uint64_t cmask, const uint64_t (&key)[KW], uint64_t h,
+    int64_t r)
 {
     uint64_t s = h & cmask;
     for (;;) {
@@ -249,28 +256,25 @@ __device__ __forceinline__ bool build_insert_once(Slot<KW>* table, uint64_t cmas
             if (head == 0u) {
 #pragma unroll
                 for (int w = 0; w < KW; ++w) __hip_atomic_store(&sl->key[w], key[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                next[r] = 0u;
                 __hip_atomic_store(&sl->head, (uint32_t)(r + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                return true;
+                return (int64_t)s;
             }
         }
-        if (head == kLockHead) return false;
+        if (head == kLockHead) return -1;
         bool eq = true;
 #pragma unroll
         for (int w = 0; w < KW; ++w) eq = eq && __hip_atomic_load(&sl->key[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == key[w];
-        if (eq) {
-            // Push on the chain; the order is fixed up afterwards. A
-            // published head is never the lock value again.
-            next[r] = atomicExch(&sl->head, (uint32_t)(r + 1));
-            return true;
-        }
+        if (eq) return (int64_t)s;
         s = (s + 1) & cmask;
     }
 }
 
+// Inserts every arena row's key (distinct keys get a slot), counts the rows
+// of each key and records each row's slot. `flags[0]` is set when a key has
+// more than one row.
 template <int KW>
 __global__ void __launch_bounds__(kBlock) k_build_insert(Slot<KW>* table, uint64_t cmask, const uint64_t* arena_keys,
-    int64_t rows, uint32_t* next, uint32_t* bloom, uint32_t bloom_mask)
+    int64_t rows, uint32_t* row_slot, unsigned int* flags)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     // Every thread of the block runs the same number of iterations.
@@ -281,53 +285,219 @@ __global__ void __launch_bounds__(kBlock) k_build_insert(Slot<KW>* table, uint64
 #pragma unroll
         for (int w = 0; w < KW; ++w) key[w] = pending ? arena_keys[r * KW + w] : 0ull;
         const uint64_t h = key_place_hash<KW>(key);
-        if (pending) atomicOr(&bloom[bloom_word(h, bloom_mask)], bloom_bits(h));
+        int64_t s = -1;
         while (__any(pending)) {
-            if (pending && build_insert_once<KW>(table, cmask, key, h, r, next)) pending = false;
+            if (pending) {
+                s = build_insert_once<KW>(table, cmask, key, h, r);
+                pending = s < 0;
+            }
+        }
+        if (r < rows) {
+            row_slot[r] = (uint32_t)s;
+            if (atomicAdd(&table[s].count, 1u) != 0u) flags[0] = 1u;
         }
     }
 }
 
-// Orders every chain by arena row (= build order) and counts it.
+// Per-slot row counts (for the scan) and the Bloom filter bits of every
+// distinct key.
+// Per-slot row counts (for the scan), the Bloom filter bits of every
+// distinct key, and per workgroup the number of distinct keys
+// (distinct_part[block]).
 template <int KW>
-__global__ void __launch_bounds__(kBlock) k_build_chains(Slot<KW>* table, uint64_t capacity, uint32_t* next,
-    unsigned long long* keys_out)
+__global__ void __launch_bounds__(kBlock) k_slot_counts(const Slot<KW>* table, uint64_t capacity, uint32_t* counts,
+    uint32_t* bloom, uint32_t bloom_mask, unsigned long long* distinct_part)
+{
+    __shared__ unsigned long long wsum[kBlock / kWave];
+    unsigned long long used = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < capacity; s += stride) {
+        const Slot<KW> sl = table[s];
+        counts[s] = sl.count;
+        used += sl.count != 0u ? 1u : 0u;
+        if (sl.count == 0u || !bloom) continue;
+        uint64_t key[KW];
+#pragma unroll
+        for (int w = 0; w < KW; ++w) key[w] = sl.key[w];
+        const uint64_t h = key_place_hash<KW>(key);
+        atomicOr(&bloom[bloom_word(h, bloom_mask)], bloom_bits(h));
+    }
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) used += shfl_xor_u64(used, o);
+    if (lane_id() == 0) wsum[threadIdx.x / kWave] = used;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < kBlock / kWave; ++w) t += wsum[w];
+        distinct_part[blockIdx.x] = t;
+    }
+}
+
+// Distinct keys: each row goes to the first position of its slot.
+__global__ void __launch_bounds__(kBlock) k_place_unique(const uint32_t* row_slot, const uint32_t* start, int64_t rows,
+    uint32_t* perm)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += stride) perm[start[row_slot[r]]] = (uint32_t)r;
+}
+
+__global__ void __launch_bounds__(kBlock) k_iota(uint32_t* v, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) v[i] = (uint32_t)i;
+}
+
+// The arena keys and ordinals in CSR order, and ord_to_arena.
+template <int KW>
+__global__ void __launch_bounds__(kBlock) k_build_csr(const uint32_t* perm, int64_t rows, const uint64_t* keys,
+    const int64_t* ordinal, uint64_t* new_keys, int64_t* new_ordinal, int32_t* ord_to_arena)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < rows; p += stride) {
+        const uint32_t r = perm[p];
+#pragma unroll
+        for (int w = 0; w < KW; ++w) new_keys[p * KW + w] = keys[(int64_t)r * KW + w];
+        const int64_t ord = ordinal[r];
+        new_ordinal[p] = ord;
+        ord_to_arena[ord] = (int32_t)p;
+    }
+}
+
+// Insert-time slots -> probe-time slots.
+template <int KW>
+__global__ void __launch_bounds__(kBlock) k_finish_slots(Slot<KW>* table, uint64_t capacity, const uint32_t* start,
+    const int64_t* csr_ordinal)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    unsigned long long keys = 0;
     for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < capacity; s += stride) {
-        uint32_t head = table[s].head;
-        if (head == 0u) continue;
-        ++keys;
-        // Insertion sort of the singly linked list by row index.
-        uint32_t sorted = 0u;
-        uint32_t count = 0;
-        uint32_t cur = head;
-        while (cur) {
-            uint32_t nxt = next[cur - 1];
-            if (sorted == 0u || cur < sorted) {
-                next[cur - 1] = sorted;
-                sorted = cur;
-            } else {
-                uint32_t p = sorted;
-                while (next[p - 1] && next[p - 1] < cur) p = next[p - 1];
-                next[cur - 1] = next[p - 1];
-                next[p - 1] = cur;
-            }
-            ++count;
-            cur = nxt;
-        }
-        table[s].head = sorted;
-        table[s].count = count;
+        const uint32_t c = table[s].count;
+        if (c == 0u) continue;
+        table[s].head = c == 1u ? (uint32_t)csr_ordinal[start[s]] : start[s];
     }
-    if (keys) atomicAdd(keys_out, keys);
+}
+
+// Smallest and largest packed key of a one-word-key arena: a partial per
+// workgroup (part[2 * block], part[2 * block + 1]), then k_reduce_partials.
+__global__ void __launch_bounds__(kBlock) k_key_minmax(const uint64_t* keys, int64_t rows, unsigned long long* part)
+{
+    __shared__ unsigned long long s_lo[kBlock / kWave], s_hi[kBlock / kWave];
+    unsigned long long lo = ~0ull, hi = 0ull;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += stride) {
+        const unsigned long long k = keys[r];
+        lo = k < lo ? k : lo;
+        hi = k > hi ? k : hi;
+    }
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const unsigned long long a = shfl_xor_u64(lo, o), b = shfl_xor_u64(hi, o);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    if (lane_id() == 0) {
+        s_lo[threadIdx.x / kWave] = lo;
+        s_hi[threadIdx.x / kWave] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / kWave; ++w) {
+            lo = s_lo[w] < lo ? s_lo[w] : lo;
+            hi = s_hi[w] > hi ? s_hi[w] : hi;
+        }
+        part[2 * blockIdx.x] = lo;
+        part[2 * blockIdx.x + 1] = hi;
+    }
+}
+
+// Reduces `n` partials of `words` u64 words each (partial p, word k at
+// part[p * stride + k]) into out[k]: op[k] 0 = sum, 1 = signed min, 2 =
+// signed max, 3 = unsigned min, 4 = unsigned max. One workgroup of kBlock
+// threads; every thread reads its share of the partials.
+struct ReduceOps {
+    int32_t op[kMaxFused * 6];
+};
+
+__global__ void __launch_bounds__(kBlock) k_reduce_partials(const unsigned long long* part, int n, int stride, int words,
+    ReduceOps ops, unsigned long long* out)
+{
+    __shared__ unsigned long long red[kBlock / kWave];
+    for (int k = 0; k < words; ++k) {
+        const int op = ops.op[k];
+        auto comb = [op](unsigned long long a, unsigned long long b) -> unsigned long long {
+            switch (op) {
+            case 0: return a + b;
+            case 1: return (int64_t)b < (int64_t)a ? b : a;
+            case 2: return (int64_t)b > (int64_t)a ? b : a;
+            case 3: return b < a ? b : a;
+            default: return b > a ? b : a;
+            }
+        };
+        const unsigned long long id = op == 0 ? 0ull : op == 1 ? 0x7fffffffffffffffull : op == 2 ? 0x8000000000000000ull
+                                                   : op == 3 ? ~0ull : 0ull;
+        unsigned long long x = id;
+        for (int p = threadIdx.x; p < n; p += blockDim.x) x = comb(x, part[(size_t)p * stride + k]);
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) x = comb(x, shfl_xor_u64(x, o));
+        if (lane_id() == 0) red[threadIdx.x / kWave] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long t = id;
+            for (int w = 0; w < kBlock / kWave; ++w) t = comb(t, red[w]);
+            out[k] = t;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* p, uint64_t n, uint32_t v)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// The dense table: a membership bit and the build ordinal of every row at
+// key - min (kAbsent where no row has the key); flags[0] = 1 when a key
+// repeats.
+__global__ void __launch_bounds__(kBlock) k_dense_fill(const uint64_t* keys, const int64_t* ordinal, int64_t rows,
+    uint64_t dmin, uint32_t* dbits, uint32_t* dtab, unsigned int* flags)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += stride) {
+        const uint64_t idx = keys[r] - dmin;
+        const uint32_t bit = 1u << (idx & 31u);
+        if (atomicOr(&dbits[idx >> 5], bit) & bit) flags[0] = 1u;
+        dtab[idx] = (uint32_t)ordinal[r];
+    }
+}
+
+// One payload column (and its validity bytes) gathered into CSR order.
+__global__ void __launch_bounds__(kBlock) k_permute_column(const uint32_t* perm, int64_t rows, const char* src, char* dst,
+    int width, const uint8_t* vsrc, uint8_t* vdst)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < rows; p += stride) {
+        const int64_t r = perm[p];
+        switch (width) {
+        case 1: ((uint8_t*)dst)[p] = ((const uint8_t*)src)[r]; break;
+        case 2: ((uint16_t*)dst)[p] = ((const uint16_t*)src)[r]; break;
+        case 4: ((uint32_t*)dst)[p] = ((const uint32_t*)src)[r]; break;
+        case 8: ((uint64_t*)dst)[p] = ((const uint64_t*)src)[r]; break;
+        default: {
+            const uint64_t* s = (const uint64_t*)(src + 16 * r);
+            uint64_t* d = (uint64_t*)(dst + 16 * p);
+            d[0] = s[0];
+            d[1] = s[1];
+        }
+        }
+        if (vsrc) vdst[p] = vsrc[r];
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Probe.
 // ---------------------------------------------------------------------------
 
-enum FusedKind : uint8_t { FK_COUNT_PAIRS, FK_COUNT_VALID, FK_SUM, FK_MIN, FK_MAX };
+enum FusedKind : int32_t { FK_COUNT_PAIRS, FK_COUNT_VALID, FK_SUM, FK_MIN, FK_MAX };
 
 struct FusedDesc {
     int32_t kind;
@@ -337,231 +507,583 @@ struct FusedDesc {
 };
 
 struct ProbeArgs {
-    DevCols cols;         // probe columns used (keys, fused args, selection none)
+    DevCols cols;         // probe columns used (keys, fused args)
     JKeyDesc K;
+    int32_t key_i64;      // one non-nullable 8-byte key in cols.c[0]: direct loads
+    int32_t pad0;
     int64_t n;
     const uint8_t* sel;
-    const void* table;
+    const void* table;          // probe-time slots
     uint64_t cmask;
-    const uint32_t* next;
-    const int64_t* ordinal;
+    const int64_t* ordinal;     // build ordinals in CSR order
+    const int32_t* ord_to_arena;
     const uint32_t* bloom;
     uint32_t bloom_words;       // power of two, or 0 (no filter)
-    uint32_t lds_table_slots;   // > 0: the whole table is copied to LDS
-    int emit_unmatched;
-    int probe_is_left;
+    uint32_t lds_words;         // dense: bitmap words staged in LDS
+    uint64_t dmin;              // dense: smallest build key
+    uint64_t drange;            // dense: max - min + 1
+    const uint32_t* dbits;      // dense: key - min membership bitmap
+    const uint32_t* dtab;       // dense: build ordinal by key - min
+    int32_t emit_unmatched;
+    int32_t probe_is_left;
+    int32_t val_slot;           // the one probe column of the fused aggregates (kMode 2)
     int64_t* pairs;             // 2 i64 per pair
     unsigned long long* pair_count;
     unsigned long long pair_cap;
-    int nfused;
+    int32_t nfused;
+    int32_t pad2;
     FusedDesc fused[kMaxFused];
     const char* payload_data[kMaxFused];
     const uint8_t* payload_valid[kMaxFused];
     int32_t payload_width[kMaxFused];
-    unsigned long long* fused_acc;  // per fused agg: [count, p0, p1, p2, min, max] words
+    // Per workgroup and fused agg: [count, p0, p1, p2, min, max] words;
+    // k_reduce_partials folds them into the join's totals.
+    unsigned long long* fused_part;
 };
 
 constexpr int kAccWords = 6;
 
-// Probe of one valid key: the chain head and its length, or 0.
-template <int KW, bool kLdsTable>
-__device__ __forceinline__ uint32_t probe_key(const ProbeArgs& P, const Slot<KW>* lds_table, const uint64_t (&key)[KW],
-    uint64_t h, uint32_t* count)
+// Probe kernel modes: kGeneric reads keys through the column descriptors;
+// kDirectKey reads one non-nullable 8-byte integer key with a streaming
+// (non-temporal) load; kDirectVal also streams the one non-nullable 8-byte
+// integer probe column that every probe-side fused aggregate reads, beside
+// the key, so that the fused aggregate needs no load of its own.
+constexpr int kGeneric = 0, kDirectKey = 1, kDirectVal = 2;
+
+// Build tables the probe reads: the hash table in HBM behind a Bloom filter
+// in LDS, the whole hash table in LDS, or the dense direct table (the
+// membership bitmap in LDS as far as it fits, the ordinal table in HBM).
+constexpr int kTabHash = 0, kTabLds = 1, kTabDense = 2;
+
+// The key of selected probe row i; false when a key field is NULL.
+template <int KW, bool kDirect>
+__device__ __forceinline__ bool probe_row_key(const ProbeArgs& P, int64_t i, uint64_t (&key)[KW])
 {
-    *count = 0;
-    if (kLdsTable) {
-        const uint32_t mask = P.lds_table_slots - 1u;
-        uint32_t s = (uint32_t)h & mask;
-        for (;;) {
-            const Slot<KW>& sl = lds_table[s];
-            if (sl.head == 0u) return 0u;
-            bool eq = true;
+    if (kDirect) {
+        key[0] = __builtin_nontemporal_load((const uint64_t*)P.cols.c[0].data + i);
 #pragma unroll
-            for (int w = 0; w < KW; ++w) eq = eq && sl.key[w] == key[w];
-            if (eq) {
-                *count = sl.count;
-                return sl.head;
-            }
-            s = (s + 1u) & mask;
-        }
+        for (int w = 1; w < KW; ++w) key[w] = 0;
+        return true;
     }
-    const Slot<KW>* table = (const Slot<KW>*)P.table;
-    uint64_t s = h & P.cmask;
+    return join_key<KW>(P.cols, P.K, i, key);
+}
+
+// The rest of a lookup after the slot `sl` at position s was read: the
+// key's row count (0 = no match) and the slot's head.
+template <int KW>
+__device__ __forceinline__ uint32_t lookup_from(const ProbeArgs& P, const Slot<KW>* T, const uint64_t (&key)[KW], uint64_t s,
+    Slot<KW> sl, uint32_t* head)
+{
     for (;;) {
-        const Slot<KW> sl = table[s];
-        if (sl.head == 0u) return 0u;
+        if (sl.count == 0u) return 0u;
         bool eq = true;
 #pragma unroll
         for (int w = 0; w < KW; ++w) eq = eq && sl.key[w] == key[w];
         if (eq) {
-            *count = sl.count;
-            return sl.head;
+            *head = sl.head;
+            return sl.count;
         }
         s = (s + 1) & P.cmask;
+        sl = T[s];
     }
 }
 
-struct FusedAcc {
+// Per-lane state of one fused aggregate: count and either the i128 sum
+// (lo, hi) or the running min / max in `lo`.
+struct FusedLane {
     unsigned long long count;
-    u128 sum;
-    int64_t mn, mx;
+    uint64_t lo, hi;
 };
 
-template <int KW, bool kLdsTable>
+__device__ __forceinline__ void fused_add(FusedLane& a, int kind, uint32_t times, i128 x)
+{
+    a.count += times;
+    if (kind == FK_SUM) {
+        const u128 s = ((((u128)a.hi) << 64) | (u128)a.lo) + (u128)x * (u128)times;
+        a.lo = (uint64_t)s;
+        a.hi = (uint64_t)(s >> 64);
+    } else if (kind == FK_MIN) {
+        a.lo = (int64_t)x < (int64_t)a.lo ? (uint64_t)(int64_t)x : a.lo;
+    } else if (kind == FK_MAX) {
+        a.lo = (int64_t)x > (int64_t)a.lo ? (uint64_t)(int64_t)x : a.lo;
+    }
+}
+
+constexpr int kSub = 4;                                  // rows per lane per load round
+constexpr int kSubRounds = 4;                            // rounds per wave tile
+constexpr int kTileRows = kSub * kSubRounds;             // rows per lane per wave tile
+constexpr int kWaveTile = kWave * kTileRows;             // rows per output reservation
+
+typedef long long ll2 __attribute__((ext_vector_type(2)));
+
+// Persistent probe. Every wave works alone on tiles of kWaveTile rows (no
+// workgroup barrier after the filter is loaded): per round of kSub rows per
+// lane, the keys, then the first slot of every row that passes the Bloom
+// filter, are loaded as independent rounds. The tile's pairs are placed with
+// one atomic per wave tile; within the tile they are ordered row-major across
+// the wave so that every store instruction writes a contiguous run. The
+// fused aggregate of a probe column is applied once per matched row (times
+// its match count) into per-lane state, combined once at the end.
+// Writes a workgroup's fused state: wave reduction (add words hold the sum
+// as bits [0,32), [32,64), [64,128)), the waves' partials through LDS, and
+// one partial per workgroup with plain stores; k_reduce_partials combines the
+// workgroups. (Atomics from every wave to the same few words serialize in
+// the memory system and cost more than the probe itself.) Every thread of
+// the workgroup calls it. kShared: the state is the pair count and the sum,
+// min and max of the one probe column (shared by every fused aggregate).
+template <int kAcc, bool kShared>
+__device__ __forceinline__ void fused_flush(const ProbeArgs& P, const FusedLane (&acc)[kAcc], unsigned long long v_pairs,
+    uint64_t v_lo, uint64_t v_hi, int64_t v_min, int64_t v_max, int lane)
+{
+    __shared__ unsigned long long part[kProbeBlock / kWave][kMaxFused][kAccWords];
+    const int wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int f = 0; f < kMaxFused; ++f) {
+        if (f >= P.nfused) continue;
+        const int kind = P.fused[f].kind;
+        unsigned long long w[kAccWords];
+        if (kShared) {
+            w[0] = v_pairs;
+            w[1] = kind == FK_SUM ? (v_lo & 0xffffffffull) : 0ull;
+            w[2] = kind == FK_SUM ? (v_lo >> 32) : 0ull;
+            w[3] = kind == FK_SUM ? v_hi : 0ull;
+            w[4] = kind == FK_MIN ? (uint64_t)v_min : 0x7fffffffffffffffull;
+            w[5] = kind == FK_MAX ? (uint64_t)v_max : 0x8000000000000000ull;
+        } else {
+            const FusedLane a = acc[f < kAcc ? f : 0];
+            w[0] = a.count;
+            w[1] = kind == FK_SUM ? (a.lo & 0xffffffffull) : 0ull;
+            w[2] = kind == FK_SUM ? (a.lo >> 32) : 0ull;
+            w[3] = kind == FK_SUM ? a.hi : 0ull;
+            w[4] = kind == FK_MIN ? a.lo : 0x7fffffffffffffffull;
+            w[5] = kind == FK_MAX ? a.lo : 0x8000000000000000ull;
+        }
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            w[0] += shfl_xor_u64(w[0], o);
+            w[1] += shfl_xor_u64(w[1], o);
+            w[2] += shfl_xor_u64(w[2], o);
+            w[3] += shfl_xor_u64(w[3], o);
+            const int64_t x4 = (int64_t)shfl_xor_u64(w[4], o);
+            const int64_t x5 = (int64_t)shfl_xor_u64(w[5], o);
+            w[4] = (unsigned long long)((int64_t)w[4] < x4 ? (int64_t)w[4] : x4);
+            w[5] = (unsigned long long)((int64_t)w[5] > x5 ? (int64_t)w[5] : x5);
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < kAccWords; ++k) part[wave][f][k] = w[k];
+        }
+    }
+    __syncthreads();
+    if (wave == 0 && lane < P.nfused * kAccWords) {
+        const int f = lane / kAccWords, k = lane % kAccWords;
+        unsigned long long x = part[0][f][k];
+        for (int v = 1; v < kProbeBlock / kWave; ++v) {
+            const unsigned long long y = part[v][f][k];
+            if (k < 4) x += y;
+            else if (k == 4) x = (int64_t)y < (int64_t)x ? y : x;
+            else x = (int64_t)y > (int64_t)x ? y : x;
+        }
+        P.fused_part[((size_t)blockIdx.x * kMaxFused + f) * kAccWords + k] = x;
+    }
+}
+
+
+constexpr uint32_t kAbsent = 0xffffffffu;   // dense table: no build row for this key
+
+// Dense direct table probe (distinct one-word integer keys), in two
+// passes over chunks of kWaveTile rows. A wave takes a chunk at a time, with
+// no workgroup barrier once the bitmap is in LDS: kTileRows rows per lane,
+// loaded as independent rounds (keys, then the bitmap: LDS for the keys it
+// covers, HBM beyond). The count pass writes each chunk's output row count;
+// after a scan, the write pass reads the ordinal table for the matches and
+// writes the chunk's pairs from its offset on, in probe row order (row i =
+// chunk start + r * 64 + lane, r-major), with the fused aggregate. A key
+// matches at most one build row, so a row's output is at most one pair (or
+// its unmatched pair in an outer join).
+template <int kMode, bool kCount>
+__global__ void __launch_bounds__(kProbeBlock) k_probe_dense(ProbeArgs P, uint32_t* chunk_cnt, const uint32_t* chunk_off)
+{
+    constexpr bool kDirect = kMode != kGeneric;
+    extern __shared__ uint64_t lds64[];
+    const uint32_t* lbits = (const uint32_t*)lds64;
+    {
+        uint32_t* b = (uint32_t*)lds64;
+        for (uint32_t i = threadIdx.x; i < P.lds_words; i += blockDim.x) b[i] = P.dbits[i];
+    }
+    __syncthreads();
+    const uint64_t lds_range = (uint64_t)P.lds_words * 32u;
+    const int lane = lane_id();
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    constexpr int kAcc = kMode == kDirectVal ? 1 : kMaxFused;
+    FusedLane acc[kAcc];
+#pragma unroll
+    for (int f = 0; f < kAcc; ++f) {
+        const int kind = f < P.nfused ? P.fused[f].kind : FK_COUNT_PAIRS;
+        acc[f].count = 0;
+        acc[f].lo = kind == FK_MIN ? 0x7fffffffffffffffull : kind == FK_MAX ? 0x8000000000000000ull : 0ull;
+        acc[f].hi = 0;
+    }
+    unsigned long long v_pairs = 0;
+    uint64_t v_lo = 0, v_hi = 0;
+    int64_t v_min = 0x7fffffffffffffffll, v_max = (int64_t)0x8000000000000000ll;
+    const int64_t nchunks = (P.n + kWaveTile - 1) / kWaveTile;
+    const int64_t nwaves = (int64_t)gridDim.x * (kProbeBlock / kWave);
+    for (int64_t c = (int64_t)blockIdx.x * (kProbeBlock / kWave) + threadIdx.x / kWave; c < nchunks; c += nwaves) {
+        const int64_t t0 = c * kWaveTile;
+        // Round 1: keys, unconditional loads at clamped rows.
+        uint64_t key[kTileRows];
+        uint32_t part = 0, live = 0;
+#pragma unroll
+        for (int r = 0; r < kTileRows; ++r) {
+            const int64_t i = t0 + (int64_t)r * kWave + lane;
+            const bool in = i < P.n;
+            const int64_t ic = in ? i : 0;
+            const uint8_t sb = P.sel ? P.sel[ic] : (uint8_t)1;
+            const bool sel = in & (sb != 0);
+            uint64_t k1[1] = {0};
+            const bool kv = probe_row_key<1, kDirect>(P, ic, k1);
+            key[r] = k1[0];
+            part |= sel ? 1u << r : 0u;
+            live |= (sel & kv) ? 1u << r : 0u;
+        }
+        // Round 2: the bitmap words of every row; writing, then the
+        // ordinals of the matches. Counting reads the HBM part of the bitmap
+        // for keys beyond the LDS part; writing reads the ordinal table for
+        // those keys instead (kAbsent there = no build row), so that its
+        // only memory reads after the keys are the ordinals. The rounds are
+        // kept apart: memory counters are waited on in issue order.
+        uint32_t ixs[kTileRows], wl[kTileRows], wg[kTileRows];
+        uint32_t inr = 0, cov = 0;
+#pragma unroll
+        for (int r = 0; r < kTileRows; ++r) {
+            const uint64_t idx = key[r] - P.dmin;
+            const bool in_range = ((live >> r) & 1u) & (idx < P.drange);
+            const uint32_t ix = in_range ? (uint32_t)idx : 0u;
+            const bool covered = ix < lds_range;
+            ixs[r] = ix;
+            inr |= in_range ? 1u << r : 0u;
+            cov |= covered ? 1u << r : 0u;
+            wl[r] = lbits[covered ? (ix >> 5) : 0u];
+            // The HBM part is read with its own intrinsic so that the
+            // compiler cannot fold the two reads into one flat load.
+            wg[r] = 0;
+            if (kCount) wg[r] = __builtin_nontemporal_load(P.dbits + (covered ? 0u : (ix >> 5)));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        uint32_t need = 0;
+#pragma unroll
+        for (int r = 0; r < kTileRows; ++r) {
+            const bool covered = (cov >> r) & 1u;
+            const uint32_t w = covered ? wl[r] : wg[r];
+            const bool bit = ((w >> (ixs[r] & 31u)) & 1u) != 0u;
+            const bool h = ((inr >> r) & 1u) & (kCount ? bit : (!covered | bit));
+            need |= h ? 1u << r : 0u;
+        }
+        // Round 3 (writing): the ordinals, and the values of the fused
+        // aggregates' probe column.
+        uint32_t head[kTileRows];
+        int64_t val[kTileRows];
+        uint32_t hit = need;
+        if (!kCount) {
+#pragma unroll
+            for (int r = 0; r < kTileRows; ++r) {
+                head[r] = P.dtab[((need >> r) & 1u) ? ixs[r] : 0u];
+                val[r] = 0;
+                if (kMode == kDirectVal) {
+                    const int64_t i = t0 + (int64_t)r * kWave + lane;
+                    val[r] = __builtin_nontemporal_load((const int64_t*)P.cols.c[P.val_slot].data + (i < P.n ? i : 0));
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int r = 0; r < kTileRows; ++r) {
+                if (head[r] == kAbsent) hit &= ~(1u << r);
+            }
+        }
+        const uint32_t emit = hit | (P.emit_unmatched ? part : 0u);
+        if (kCount) {
+            uint32_t run = 0;
+#pragma unroll
+            for (int r = 0; r < kTileRows; ++r) run += (uint32_t)__popcll(__ballot((emit >> r) & 1u));
+            if (lane == 0) chunk_cnt[c] = run;
+            continue;
+        }
+        // Pairs in probe row order and the fused aggregate.
+        unsigned long long pos0 = chunk_off[c];
+#pragma unroll
+        for (int r = 0; r < kTileRows; ++r) {
+            const uint64_t m = __ballot((emit >> r) & 1u);
+            const unsigned long long pos = pos0 + (uint32_t)__popcll(m & lt_mask);
+            pos0 += (uint32_t)__popcll(m);
+            if (!((emit >> r) & 1u)) continue;
+            const int64_t i = t0 + (int64_t)r * kWave + lane;
+            const int64_t stream_id = (int64_t)((uint64_t)-1 << 32) + i;
+            const bool h = (hit >> r) & 1u;
+            const int64_t ord = h ? (int64_t)head[r] : -1;
+            if (pos < P.pair_cap) {
+                const ll2 v = P.probe_is_left ? ll2{stream_id, ord} : ll2{ord, stream_id};
+                __builtin_nontemporal_store(v, (ll2*)(P.pairs + 2 * pos));
+            }
+            if (kMode == kDirectVal) {
+                v_pairs += 1;
+                const uint64_t x = (uint64_t)val[r];
+                const uint64_t lo = v_lo + x;
+                v_hi += (lo < v_lo ? 1ull : 0ull) + (val[r] < 0 ? ~0ull : 0ull);
+                v_lo = lo;
+                v_min = val[r] < v_min ? val[r] : v_min;
+                v_max = val[r] > v_max ? val[r] : v_max;
+                continue;
+            }
+#pragma unroll
+            for (int f = 0; f < kAcc; ++f) {
+                if (f >= P.nfused) continue;
+                const FusedDesc d = P.fused[f];
+                if (d.kind == FK_COUNT_PAIRS) {
+                    acc[f].count += 1;
+                } else if (!d.from_build) {
+                    const DevCol& col = P.cols.c[d.slot];
+                    if (col_valid(col, i)) fused_add(acc[f], d.kind, 1u, col_int(col, i));
+                } else if (h) {
+                    const int64_t b = (int64_t)P.ord_to_arena[head[r]];
+                    if (P.payload_valid[f] && !P.payload_valid[f][b]) continue;
+                    const char* pd = P.payload_data[f];
+                    i128 x;
+                    switch (P.payload_width[f]) {
+                    case 1: x = ((const int8_t*)pd)[b]; break;
+                    case 2: x = ((const int16_t*)pd)[b]; break;
+                    case 4: x = ((const int32_t*)pd)[b]; break;
+                    case 8: x = ((const int64_t*)pd)[b]; break;
+                    default: {
+                        const uint64_t* q = (const uint64_t*)(pd + 16 * (size_t)b);
+                        x = (i128)(((u128)q[1] << 64) | (u128)q[0]);
+                    }
+                    }
+                    fused_add(acc[f], d.kind, 1u, x);
+                }
+            }
+        }
+    }
+    if (!kCount && P.nfused) fused_flush<kAcc, kMode == kDirectVal>(P, acc, v_pairs, v_lo, v_hi, v_min, v_max, lane);
+}
+
+// pair_count = the last chunk's offset + its count.
+__global__ void k_chunk_total(const uint32_t* cnt, const uint32_t* off, int64_t nchunks, unsigned long long* total)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *total = nchunks ? (unsigned long long)off[nchunks - 1] + cnt[nchunks - 1] : 0ull;
+}
+
+template <int KW, int kTab, int kMode>
 __global__ void __launch_bounds__(kProbeBlock) k_probe(ProbeArgs P)
 {
-    extern __shared__ uint32_t lds[];
-    __shared__ unsigned int wave_tot[kProbeBlock / kWave];
-    __shared__ unsigned long long block_base;
-    const Slot<KW>* lds_table = (const Slot<KW>*)lds;
-    const uint32_t* lds_bloom = lds;
-    if (kLdsTable) {
-        const uint32_t* src = (const uint32_t*)P.table;
-        const uint32_t words = P.lds_table_slots * (uint32_t)(sizeof(Slot<KW>) / 4);
-        for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) lds[i] = src[i];
+    constexpr bool kDirect = kMode != kGeneric;
+    constexpr bool kLds = kTab == kTabLds;
+    extern __shared__ uint64_t lds64[];
+    const Slot<KW>* T = kLds ? (const Slot<KW>*)lds64 : (const Slot<KW>*)P.table;
+    const uint32_t* lds_bloom = (const uint32_t*)lds64;
+    if (kLds) {
+        const uint64_t words = (P.cmask + 1) * (sizeof(Slot<KW>) / 8);
+        for (uint64_t i = threadIdx.x; i < words; i += blockDim.x) lds64[i] = ((const uint64_t*)P.table)[i];
     } else if (P.bloom_words) {
-        for (uint32_t i = threadIdx.x; i < P.bloom_words; i += blockDim.x) lds[i] = P.bloom[i];
+        uint32_t* b = (uint32_t*)lds64;
+        for (uint32_t i = threadIdx.x; i < P.bloom_words; i += blockDim.x) b[i] = P.bloom[i];
     }
     __syncthreads();
     const int lane = lane_id();
-    const int wave = threadIdx.x / kWave;
-    FusedAcc acc[kMaxFused];
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    // Fused state per lane. kDirectVal: the pair count and the sum, min and
+    // max of the one probe column, shared by every fused aggregate; else one
+    // state per fused aggregate.
+    constexpr int kAcc = kMode == kDirectVal ? 1 : kMaxFused;
+    FusedLane acc[kAcc];
 #pragma unroll
-    for (int f = 0; f < kMaxFused; ++f) {
+    for (int f = 0; f < kAcc; ++f) {
+        const int kind = f < P.nfused ? P.fused[f].kind : FK_COUNT_PAIRS;
         acc[f].count = 0;
-        acc[f].sum = 0;
-        acc[f].mn = 0x7fffffffffffffffll;
-        acc[f].mx = (int64_t)0x8000000000000000ll;
+        acc[f].lo = kind == FK_MIN ? 0x7fffffffffffffffull : kind == FK_MAX ? 0x8000000000000000ull : 0ull;
+        acc[f].hi = 0;
     }
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    // Every thread of the block runs the same number of iterations (barriers).
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < P.n; base += stride) {
-        const int64_t i = base + threadIdx.x;
-        uint32_t head = 0, count = 0;
-        bool unmatched = false;
-        if (i < P.n && (!P.sel || P.sel[i])) {
-            uint64_t key[KW];
-            const bool valid = join_key<KW>(P.cols, P.K, i, key);
-            if (valid) {
-                const uint64_t h = key_place_hash<KW>(key);
-                bool maybe = true;
-                if (!kLdsTable && P.bloom_words) {
-                    const uint32_t bits = bloom_bits(h);
-                    maybe = (lds_bloom[bloom_word(h, P.bloom_words - 1u)] & bits) == bits;
-                }
-                if (maybe) head = probe_key<KW, kLdsTable>(P, lds_table, key, h, &count);
-            }
-            unmatched = head == 0u && P.emit_unmatched;
-        }
-        const unsigned int emit = count + (unmatched ? 1u : 0u);
-        // Block-wide exclusive scan of the pair counts: one atomic per block.
-        unsigned int incl = emit;
+    unsigned long long v_pairs = 0;
+    uint64_t v_lo = 0, v_hi = 0;
+    int64_t v_min = 0x7fffffffffffffffll, v_max = (int64_t)0x8000000000000000ll;
+    __shared__ unsigned int wave_run[kProbeBlock / kWave];
+    __shared__ unsigned long long block_base;
+    const int wave = threadIdx.x / kWave;
+    constexpr int64_t kBlockTile = (int64_t)(kProbeBlock / kWave) * kWaveTile;
+    // Every thread of the block runs the same number of block tiles (two
+    // barriers per tile place the block's pairs with one atomic).
+    for (int64_t b0 = blockIdx.x * kBlockTile; b0 < P.n; b0 += (int64_t)gridDim.x * kBlockTile) {
+        const int64_t t0 = b0 + (int64_t)wave * kWaveTile;
+        // Per row of the tile: match count and slot head; and whether the
+        // row is selected.
+        uint32_t cnt[kTileRows], head[kTileRows];
+        uint32_t part = 0;
 #pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-            unsigned int y = (unsigned int)__shfl_up((int)incl, off, kWave);
-            if (lane >= off) incl += y;
-        }
-        if (lane == kWave - 1) wave_tot[wave] = incl;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long run = 0;
-            for (int w = 0; w < kProbeBlock / kWave; ++w) {
-                unsigned int v = wave_tot[w];
-                wave_tot[w] = (unsigned int)run;
-                run += v;
-            }
-            block_base = run ? atomicAdd(P.pair_count, run) : 0ull;
-        }
-        __syncthreads();
-        unsigned long long pos = block_base + wave_tot[wave] + (incl - emit);
-        const int64_t stream_id = (int64_t)((uint64_t)-1 << 32) + i;
-        if (emit) {
-            if (unmatched) {
-                if (pos < P.pair_cap) {
-                    P.pairs[2 * pos] = P.probe_is_left ? stream_id : -1;
-                    P.pairs[2 * pos + 1] = P.probe_is_left ? -1 : stream_id;
-                }
-            }
-            for (uint32_t r = head; r; r = P.next[r - 1]) {
-                const int64_t ord = P.ordinal[r - 1];
-                if (pos < P.pair_cap) {
-                    P.pairs[2 * pos] = P.probe_is_left ? stream_id : ord;
-                    P.pairs[2 * pos + 1] = P.probe_is_left ? ord : stream_id;
-                }
-                ++pos;
-                // Fused global aggregate over this pair.
+        for (int q = 0; q < kSubRounds; ++q) {
+            uint64_t key[kSub][KW];
+            uint64_t s[kSub];
+            int64_t val[kSub];
+            bool live[kSub];
+            // Every load below is unconditional, from a clamped index: a
+            // load under a branch that merges with a default value makes
+            // the compiler wait for it inside the branch, one row at a time.
+            // Round 1: keys.
 #pragma unroll
-                for (int f = 0; f < kMaxFused; ++f) {
-                    if (f >= P.nfused) continue;
-                    const FusedDesc d = P.fused[f];
-                    if (d.kind == FK_COUNT_PAIRS) {
-                        acc[f].count += 1;
-                        continue;
+            for (int u = 0; u < kSub; ++u) {
+                const int r = q * kSub + u;
+                const int64_t i = t0 + (int64_t)r * kWave + lane;
+                const bool in = i < P.n;
+                const int64_t ic = in ? i : 0;
+                const uint8_t sb = P.sel ? P.sel[ic] : (uint8_t)1;
+                const bool sel = in & (sb != 0);
+                part |= sel ? 1u << r : 0u;
+#pragma unroll
+                for (int w = 0; w < KW; ++w) key[u][w] = 0;
+                const bool kv = probe_row_key<KW, kDirect>(P, ic, key[u]);
+                live[u] = sel & kv;
+                val[u] = 0;
+                if (kMode == kDirectVal) val[u] = __builtin_nontemporal_load((const int64_t*)P.cols.c[P.val_slot].data + ic);
+            }
+            // Round 2: Bloom filter and the first slot (dense: the bitmap
+            // and the ordinal).
+            Slot<KW> sl[kSub];
+            {
+#pragma unroll
+                for (int u = 0; u < kSub; ++u) {
+                    const uint64_t h = key_place_hash<KW>(key[u]);
+                    s[u] = h & P.cmask;
+                    bool maybe = live[u];
+                    if (!kLds && P.bloom_words) {
+                        const uint32_t bits = bloom_bits(h);
+                        const uint32_t bw = lds_bloom[bloom_word(h, P.bloom_words - 1u)];
+                        maybe = maybe & ((bw & bits) == bits);
                     }
-                    bool v;
-                    i128 x;
-                    if (d.from_build) {
-                        const uint8_t* vv = P.payload_valid[f];
-                        v = !vv || vv[r - 1];
-                        const char* pd = P.payload_data[f];
+                    sl[u] = T[maybe ? s[u] : 0];
+                    if (!maybe) sl[u].count = 0;
+                }
+            }
+            // Round 3: compare (the walk continues only on a key miss).
+#pragma unroll
+            for (int u = 0; u < kSub; ++u) {
+                const int r = q * kSub + u;
+                head[r] = 0;
+                cnt[r] = lookup_from<KW>(P, T, key[u], s[u], sl[u], &head[r]);
+                // An unmatched row of an outer join is one output row too
+                // (its build side NULL).
+                const uint32_t times = cnt[r] ? cnt[r] : (((part >> r) & 1u) && P.emit_unmatched ? 1u : 0u);
+                if (kMode == kDirectVal && times) {
+                    v_pairs += times;
+                    const u128 sum = ((((u128)v_hi) << 64) | (u128)v_lo) + (u128)(i128)val[u] * (u128)times;
+                    v_lo = (uint64_t)sum;
+                    v_hi = (uint64_t)(sum >> 64);
+                    v_min = val[u] < v_min ? val[u] : v_min;
+                    v_max = val[u] > v_max ? val[u] : v_max;
+                }
+            }
+        }
+        // Output positions, row-major across the wave: the tile's total
+        // first, one atomic, then each row's offset.
+        auto emits = [&](int r) -> uint32_t {
+            return cnt[r] + (((part >> r) & 1u) && cnt[r] == 0u && P.emit_unmatched ? 1u : 0u);
+        };
+        uint32_t run = 0;
+#pragma unroll
+        for (int r = 0; r < kTileRows; ++r) {
+            const uint32_t e = emits(r);
+            if (__all(e <= 1u)) {
+                run += (uint32_t)__popcll(__ballot(e != 0u));
+            } else {
+                uint32_t x = e;
+#pragma unroll
+                for (int o = 1; o < kWave; o <<= 1) x += (uint32_t)__shfl_xor((int)x, o, kWave);
+                run += x;
+            }
+        }
+        if (lane == 0) wave_run[wave] = run;
+        __syncthreads();
+        if (wave == 0) {
+            constexpr int nw = kProbeBlock / kWave;
+            const unsigned int v = lane < nw ? wave_run[lane] : 0u;
+            unsigned int incl = v;
+#pragma unroll
+            for (int o = 1; o < nw; o <<= 1) {
+                const unsigned int y = (unsigned int)__shfl_up((int)incl, o, kWave);
+                if (lane >= o) incl += y;
+            }
+            if (lane < nw) wave_run[lane] = incl - v;
+            const unsigned int total = (unsigned int)__shfl((int)incl, nw - 1, kWave);
+            if (lane == 0) block_base = total ? atomicAdd(P.pair_count, (unsigned long long)total) : 0ull;
+        }
+        __syncthreads();
+        unsigned long long pos0 = block_base + wave_run[wave];
+#pragma unroll
+        for (int r = 0; r < kTileRows; ++r) {
+            const uint32_t e = emits(r);
+            unsigned long long pos;
+            if (__all(e <= 1u)) {
+                const uint64_t m = __ballot(e != 0u);
+                pos = pos0 + (uint32_t)__popcll(m & lt_mask);
+                pos0 += (uint32_t)__popcll(m);
+            } else {
+                uint32_t incl = e;
+#pragma unroll
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const uint32_t y = (uint32_t)__shfl_up((int)incl, o, kWave);
+                    if (lane >= o) incl += y;
+                }
+                pos = pos0 + incl - e;
+                pos0 += (uint32_t)__shfl((int)incl, kWave - 1, kWave);
+            }
+            if (e == 0u) continue;
+            const int64_t i = t0 + (int64_t)r * kWave + lane;
+            const int64_t stream_id = (int64_t)((uint64_t)-1 << 32) + i;
+            const uint32_t c = cnt[r];
+            if (c == 0u && pos < P.pair_cap) {
+                const ll2 v = P.probe_is_left ? ll2{stream_id, -1} : ll2{-1, stream_id};
+                __builtin_nontemporal_store(v, (ll2*)(P.pairs + 2 * pos));
+            }
+            for (uint32_t k = 0; k < c; ++k, ++pos) {
+                const int64_t ord = c == 1u ? (int64_t)head[r] : P.ordinal[head[r] + k];
+                if (pos < P.pair_cap) {
+                    const ll2 v = P.probe_is_left ? ll2{stream_id, ord} : ll2{ord, stream_id};
+                    __builtin_nontemporal_store(v, (ll2*)(P.pairs + 2 * pos));
+                }
+            }
+            // Fused aggregates of this row's e output rows (kDirectVal:
+            // applied above); an unmatched row has a NULL build side.
+#pragma unroll
+            for (int f = 0; f < kAcc; ++f) {
+                if (kMode == kDirectVal || f >= P.nfused) continue;
+                const FusedDesc d = P.fused[f];
+                if (d.kind == FK_COUNT_PAIRS) {
+                    acc[f].count += e;
+                } else if (!d.from_build) {
+                    const DevCol& col = P.cols.c[d.slot];
+                    if (col_valid(col, i)) fused_add(acc[f], d.kind, e, col_int(col, i));
+                } else {
+                    const uint8_t* vv = P.payload_valid[f];
+                    const char* pd = P.payload_data[f];
+                    for (uint32_t k = 0; k < c; ++k) {
+                        const int64_t b = c == 1u ? (int64_t)P.ord_to_arena[head[r]] : (int64_t)(head[r] + k);
+                        if (vv && !vv[b]) continue;
+                        i128 x;
                         switch (P.payload_width[f]) {
-                        case 1: x = ((const int8_t*)pd)[r - 1]; break;
-                        case 2: x = ((const int16_t*)pd)[r - 1]; break;
-                        case 4: x = ((const int32_t*)pd)[r - 1]; break;
-                        case 8: x = ((const int64_t*)pd)[r - 1]; break;
+                        case 1: x = ((const int8_t*)pd)[b]; break;
+                        case 2: x = ((const int16_t*)pd)[b]; break;
+                        case 4: x = ((const int32_t*)pd)[b]; break;
+                        case 8: x = ((const int64_t*)pd)[b]; break;
                         default: {
-                            const uint64_t* p = (const uint64_t*)(pd + 16 * (size_t)(r - 1));
-                            x = (i128)(((u128)p[1] << 64) | (u128)p[0]);
+                            const uint64_t* q = (const uint64_t*)(pd + 16 * (size_t)b);
+                            x = (i128)(((u128)q[1] << 64) | (u128)q[0]);
                         }
                         }
-                    } else {
-                        const DevCol& c = P.cols.c[d.slot];
-                        v = col_valid(c, i);
-                        x = v ? col_int(c, i) : 0;
+                        fused_add(acc[f], d.kind, 1u, x);
                     }
-                    if (!v) continue;
-                    acc[f].count += 1;
-                    if (d.kind == FK_SUM) acc[f].sum += (u128)x;
-                    else if (d.kind == FK_MIN) acc[f].mn = (int64_t)x < acc[f].mn ? (int64_t)x : acc[f].mn;
-                    else if (d.kind == FK_MAX) acc[f].mx = (int64_t)x > acc[f].mx ? (int64_t)x : acc[f].mx;
                 }
             }
         }
-        __syncthreads();
     }
-    // Combine the fused accumulators: wave reduction, then one atomic per
-    // wave and word (add words hold the sum as bits [0,32), [32,64), [64,128)).
-    for (int f = 0; f < P.nfused; ++f) {
-        unsigned long long w[kAccWords];
-        w[0] = acc[f].count;
-        w[1] = (unsigned long long)((uint64_t)acc[f].sum & 0xffffffffull);
-        w[2] = (unsigned long long)((uint64_t)acc[f].sum >> 32);
-        w[3] = (unsigned long long)(uint64_t)(acc[f].sum >> 64);
-        w[4] = (unsigned long long)acc[f].mn;
-        w[5] = (unsigned long long)acc[f].mx;
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-            w[0] += shfl_xor_u64(w[0], off);
-            w[1] += shfl_xor_u64(w[1], off);
-            w[2] += shfl_xor_u64(w[2], off);
-            w[3] += shfl_xor_u64(w[3], off);
-            int64_t a = (int64_t)shfl_xor_u64(w[4], off);
-            int64_t b = (int64_t)shfl_xor_u64(w[5], off);
-            w[4] = (unsigned long long)((int64_t)w[4] < a ? (int64_t)w[4] : a);
-            w[5] = (unsigned long long)((int64_t)w[5] > b ? (int64_t)w[5] : b);
-        }
-        if (lane == 0) {
-            unsigned long long* dst = P.fused_acc + (size_t)f * kAccWords;
-            if (w[0]) {
-                atomicAdd(&dst[0], w[0]);
-                atomicAdd(&dst[1], w[1]);
-                atomicAdd(&dst[2], w[2]);
-                atomicAdd(&dst[3], w[3]);
-                atomicMin((long long*)&dst[4], (long long)w[4]);
-                atomicMax((long long*)&dst[5], (long long)w[5]);
-            }
-        }
-    }
+    if (P.nfused) fused_flush<kAcc, kMode == kDirectVal>(P, acc, v_pairs, v_lo, v_hi, v_min, v_max, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -680,6 +1202,7 @@ struct JoinPlan {
     int build_nslots = 0, build_slot_col[kMaxDevCols];
     int probe_nslots = 0, probe_slot_col[kMaxDevCols];
     int build_key_slots = 0;  // build slots used by keys (payload slots follow)
+    bool dense_ok = false;    // one integer key of at most 8 bytes: the dense table may serve
     int npayload = 0;         // every build column is kept in the arena
     FusedDesc fused[kMaxFused];
     int nout = 0;
@@ -697,11 +1220,15 @@ struct JoinBuild {
     std::vector<uint8_t*> pvalid;
     int64_t appended = 0, ord_cap = 0;
     int32_t* ord_to_arena = nullptr;
-    // table
+    // probe structures (after Build): the slot table and the Bloom filter,
+    // or the dense direct table
     void* table = nullptr;
     uint64_t capacity = 0;
-    uint32_t* next = nullptr;
     uint32_t* bloom = nullptr;
+    bool dense = false;
+    uint64_t dmin = 0, drange = 0;
+    uint32_t* dbits = nullptr;
+    uint32_t* dtab = nullptr;
     uint32_t bloom_words = 0;
     int64_t keys_distinct = 0;
     bool built = false;
@@ -793,6 +1320,11 @@ bool compile_join(const QdbRowJoinSpec& spec, JoinPlan* P)
         byte += w;
     }
     P->build_key.nkeys = P->probe_key.nkeys = spec.NumKeys;
+    {
+        const QdbValueType& bt = spec.Build.Types[spec.BuildKeyColumns[0]];
+        P->dense_ok = spec.NumKeys == 1 && bt.Kind != QDB_KIND_FLOAT && bt.Kind != QDB_KIND_BOOL && key_bytes(bt) <= 8
+            && spec.TableMode == QDB_JOIN_TABLE_AUTO;
+    }
     P->KW = byte <= 8 ? 1 : byte <= 16 ? 2 : 0;
     if (P->KW == 0) return fail("join key of %d bytes exceeds 16", byte);
     P->build_key_slots = P->build_nslots;
@@ -897,8 +1429,9 @@ void build_free(JoinBuild* J)
     for (uint8_t* p : J->pvalid) dfree(p);
     dfree(J->ord_to_arena);
     dfree(J->table);
-    dfree(J->next);
     dfree(J->bloom);
+    dfree(J->dbits);
+    dfree(J->dtab);
     dfree(J->fused_acc);
     dfree(J->pair_count);
     delete J;
@@ -996,39 +1529,317 @@ bool do_append(JoinBuild* J, const TRowSet* batch)
     return true;
 }
 
+// The dense direct table when the keys are distinct and their range is
+// small; false (and no error) when the hash table must serve instead.
+bool try_dense_build(JoinBuild* J)
+{
+    const int64_t rows = J->rows;
+    const int mgrid = grid_for(rows, kBlock, 2);
+    unsigned long long* mm = (unsigned long long*)dalloc(16);
+    unsigned long long* mpart = (unsigned long long*)dalloc((size_t)mgrid * 16);
+    unsigned int* flags = (unsigned int*)dalloc(16);
+    if (!mm || !mpart || !flags) {
+        dfree(mm);
+        dfree(mpart);
+        dfree(flags);
+        return false;
+    }
+    unsigned long long h_mm[2] = {0, 0};
+    bool ok = zero_fill(flags, 16);
+    if (ok) {
+        TimedRegion timer("join_build");
+        hipLaunchKernelGGL(k_key_minmax, dim3(mgrid), dim3(kBlock), 0, stream(), J->keys, rows, mpart);
+        ReduceOps ops;
+        memset(&ops, 0, sizeof(ops));
+        ops.op[0] = 3;  // unsigned min
+        ops.op[1] = 4;  // unsigned max
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, stream(), mpart, mgrid, 2, 2, ops, mm);
+    }
+    dfree(mpart);
+    ok = ok && hip_ok(hipGetLastError(), "key range") && hip_ok(hipMemcpyAsync(h_mm, mm, 16, hipMemcpyDeviceToHost, stream()), "key range")
+        && sync("key range");
+    const uint64_t range = ok ? h_mm[1] - h_mm[0] + 1 : 0;
+    const uint64_t limit = (uint64_t)rows * 8 > (1ull << 16) ? (uint64_t)rows * 8 : (1ull << 16);
+    if (!ok || h_mm[1] < h_mm[0] || range == 0 || range > limit || range >= (1ull << 32)) {
+        dfree(mm);
+        dfree(flags);
+        clear_error();
+        return false;
+    }
+    const size_t words = (size_t)((range + 31) / 32);
+    J->dbits = (uint32_t*)dalloc(words * 4);
+    J->dtab = (uint32_t*)dalloc((size_t)range * 4);
+    unsigned int h_flags[4] = {0, 0, 0, 0};
+    ok = J->dbits && J->dtab && zero_fill(J->dbits, words * 4);
+    if (ok) {
+        TimedRegion timer("join_build");
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((int64_t)range, kBlock, 8)), dim3(kBlock), 0, stream(), J->dtab, range, kAbsent);
+        hipLaunchKernelGGL(k_dense_fill, dim3(grid_for(rows, kBlock, 8)), dim3(kBlock), 0, stream(), J->keys, J->ordinal, rows,
+            (uint64_t)h_mm[0], J->dbits, J->dtab, flags);
+    }
+    ok = ok && hip_ok(hipGetLastError(), "dense fill") && hip_ok(hipMemcpyAsync(h_flags, flags, 16, hipMemcpyDeviceToHost, stream()), "dense flags")
+        && sync("dense fill");
+    dfree(mm);
+    dfree(flags);
+    if (!ok || h_flags[0]) {  // repeated keys: the hash table serves
+        dfree(J->dbits);
+        dfree(J->dtab);
+        J->dbits = nullptr;
+        J->dtab = nullptr;
+        clear_error();
+        return false;
+    }
+    J->dense = true;
+    J->dmin = h_mm[0];
+    J->drange = range;
+    J->keys_distinct = rows;
+    J->built = true;
+    return true;
+}
+
 template <int KW>
 bool do_build(JoinBuild* J)
 {
+    JoinPlan* P = J->plan;
+    const int64_t rows = J->rows;
+    if (rows >= (int64_t)0x7fffffff) return fail("join builds are limited to 2^31 rows");
+    if (J->appended > (int64_t)0xffffffff) return fail("join builds are limited to 2^32 appended rows");
+    if (KW == 1 && P->dense_ok && rows > 0 && try_dense_build(J)) return true;
     uint64_t capacity = 1024;
-    while (capacity < (uint64_t)J->rows * 2) capacity *= 2;
+    int cap_bits = 10;
+    while (capacity < (uint64_t)rows * 2) {
+        capacity *= 2;
+        ++cap_bits;
+    }
     J->capacity = capacity;
-    J->table = dalloc(capacity * sizeof(Slot<KW>));
-    J->next = (uint32_t*)dalloc((size_t)(J->rows > 0 ? J->rows : 1) * 4);
     // Bloom filter: about 16 bits per row, between 8 KiB and 128 KiB.
     uint32_t words = 2048;
-    while (words < 32768u && (uint64_t)words * 32 < (uint64_t)J->rows * 16) words *= 2;
+    while (words < 32768u && (size_t)words * 8 <= max_lds_per_block() && (uint64_t)words * 32 < (uint64_t)rows * 16) words *= 2;
     J->bloom_words = words;
+    const size_t nrows = (size_t)(rows > 0 ? rows : 1);
+    J->table = dalloc(capacity * sizeof(Slot<KW>));
     J->bloom = (uint32_t*)dalloc((size_t)words * 4);
-    unsigned long long* keys = (unsigned long long*)dalloc(16);
-    if (!J->table || !J->next || !J->bloom || !keys) return false;
-    if (!zero_fill(J->table, capacity * sizeof(Slot<KW>)) || !zero_fill(J->bloom, (size_t)words * 4) || !zero_fill(keys, 16)) {
+    Slot<KW>* table = (Slot<KW>*)J->table;
+    uint32_t* row_slot = (uint32_t*)dalloc(nrows * 4);
+    uint32_t* counts = (uint32_t*)dalloc(capacity * 4);
+    uint32_t* start = (uint32_t*)dalloc(capacity * 4);
+    uint32_t* perm = (uint32_t*)dalloc(nrows * 4);
+    unsigned long long* stats = (unsigned long long*)dalloc(16);  // distinct keys, duplicate flag
+    const int cgrid = grid_for((int64_t)capacity, kBlock, 8);
+    unsigned long long* dpart = (unsigned long long*)dalloc((size_t)cgrid * 8);
+    uint64_t* new_keys = (uint64_t*)dalloc(nrows * KW * 8);
+    int64_t* new_ord = (int64_t*)dalloc(nrows * 8);
+    uint32_t* sorted_slot = nullptr;
+    uint32_t* iota = nullptr;
+    auto release = [&]() {
+        dfree(dpart);
+        dfree(row_slot);
+        dfree(counts);
+        dfree(start);
+        dfree(perm);
+        dfree(stats);
+        dfree(sorted_slot);
+        dfree(iota);
+    };
+    if (!table || !J->bloom || !row_slot || !counts || !start || !perm || !stats || !dpart || !new_keys || !new_ord) {
+        release();
         return false;
     }
-    if (J->rows > 0) {
+    bool ok = zero_fill(table, capacity * sizeof(Slot<KW>)) && zero_fill(J->bloom, (size_t)words * 4) && zero_fill(stats, 16);
+    if (ok) {
         TimedRegion timer("join_build");
-        hipLaunchKernelGGL((k_build_insert<KW>), dim3(grid_for(J->rows, kBlock, 8)), dim3(kBlock), 0, stream(),
-            (Slot<KW>*)J->table, capacity - 1, J->keys, J->rows, J->next, J->bloom, words - 1u);
-        hipLaunchKernelGGL((k_build_chains<KW>), dim3(grid_for((int64_t)capacity, kBlock, 8)), dim3(kBlock), 0, stream(),
-            (Slot<KW>*)J->table, capacity, J->next, keys);
+        if (rows > 0) {
+            hipLaunchKernelGGL((k_build_insert<KW>), dim3(grid_for(rows, kBlock, 8)), dim3(kBlock), 0, stream(), table,
+                capacity - 1, J->keys, rows, row_slot, (unsigned int*)(stats + 1));
+        }
+        hipLaunchKernelGGL((k_slot_counts<KW>), dim3(cgrid), dim3(kBlock), 0, stream(), table, capacity, counts, J->bloom,
+            words - 1u, dpart);
+        ReduceOps ops;
+        memset(&ops, 0, sizeof(ops));
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, stream(), dpart, cgrid, 1, 1, ops, stats);
+        ok = hip_ok(hipGetLastError(), "build insert") && exclusive_scan_u32(counts, start, (int64_t)capacity);
     }
-    unsigned long long h_keys = 0;
-    bool ok = hip_ok(hipGetLastError(), "build launch")
-        && hip_ok(hipMemcpyAsync(&h_keys, keys, 8, hipMemcpyDeviceToHost, stream()), "build keys") && sync("build");
-    dfree(keys);
-    if (!ok) return false;
-    J->keys_distinct = (int64_t)h_keys;
+    unsigned long long h_stats[2] = {0, 0};
+    ok = ok && hip_ok(hipGetLastError(), "build table")
+        && hip_ok(hipMemcpyAsync(h_stats, stats, 16, hipMemcpyDeviceToHost, stream()), "build stats") && sync("build");
+    if (!ok) {
+        release();
+        dfree(new_keys);
+        dfree(new_ord);
+        return false;
+    }
+    J->keys_distinct = (int64_t)h_stats[0];
+    const bool dups = (h_stats[1] & 0xffffffffull) != 0;
+    if (rows > 0) {
+        TimedRegion timer("join_build_place");
+        const int g = grid_for(rows, kBlock, 8);
+        if (!dups) {
+            hipLaunchKernelGGL(k_place_unique, dim3(g), dim3(kBlock), 0, stream(), row_slot, start, rows, perm);
+        } else {
+            // Duplicate keys: a stable sort by slot keeps each key's rows in
+            // build order.
+            sorted_slot = (uint32_t*)dalloc(nrows * 4);
+            iota = (uint32_t*)dalloc(nrows * 4);
+            ok = sorted_slot && iota;
+            if (ok) {
+                hipLaunchKernelGGL(k_iota, dim3(g), dim3(kBlock), 0, stream(), iota, rows);
+                ok = sort_pairs_u32(row_slot, sorted_slot, iota, perm, rows, cap_bits);
+            }
+        }
+        if (ok) {
+            hipLaunchKernelGGL((k_build_csr<KW>), dim3(g), dim3(kBlock), 0, stream(), perm, rows, J->keys, J->ordinal,
+                new_keys, new_ord, J->ord_to_arena);
+            hipLaunchKernelGGL((k_finish_slots<KW>), dim3(grid_for((int64_t)capacity, kBlock, 8)), dim3(kBlock), 0, stream(),
+                table, capacity, start, new_ord);
+            // Payload columns in the same order.
+            for (int p = 0; ok && p < P->npayload; ++p) {
+                const QdbValueType& t = P->spec.Build.Types[p];
+                const int w = t.Kind == QDB_KIND_BOOL ? 1 : t.Width;
+                char* nd = (char*)dalloc((size_t)J->cap * (size_t)w);
+                uint8_t* nv = t.Nullable ? (uint8_t*)dalloc((size_t)J->cap) : nullptr;
+                if (!nd || (t.Nullable && !nv)) {
+                    ok = false;
+                    break;
+                }
+                hipLaunchKernelGGL(k_permute_column, dim3(g), dim3(kBlock), 0, stream(), perm, rows, J->pdata[(size_t)p], nd,
+                    w, t.Nullable ? J->pvalid[(size_t)p] : nullptr, nv);
+                dfree(J->pdata[(size_t)p]);
+                J->pdata[(size_t)p] = nd;
+                if (t.Nullable) {
+                    dfree(J->pvalid[(size_t)p]);
+                    J->pvalid[(size_t)p] = nv;
+                }
+            }
+            ok = ok && hip_ok(hipGetLastError(), "build place");
+        }
+    }
+    ok = ok && sync("build place");
+    release();
+    if (!ok) {
+        dfree(new_keys);
+        dfree(new_ord);
+        return false;
+    }
+    dfree(J->keys);
+    dfree(J->ordinal);
+    J->keys = new_keys;
+    J->ordinal = new_ord;
+    J->cap = rows > 0 ? rows : 1;
     J->built = true;
     return true;
+}
+
+// The kernel's own (static) LDS bytes; the dynamic allocation of a launch
+// must fit beside it within the device's per-workgroup limit.
+size_t static_lds(const void* fn)
+{
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, fn) != hipSuccess) {
+        (void)hipGetLastError();
+        return 16 * 1024;  // conservative
+    }
+    return a.sharedSizeBytes;
+}
+
+template <int KW, int kTab, int kMode>
+void launch_probe_one(int grid, size_t lds, const ProbeArgs& A)
+{
+    static bool attr_set = false;
+    if (!attr_set) {
+        // Room for the kernel's static LDS beside the dynamic allocation; a
+        // refused attribute must not leave an error for the launch check.
+        const void* fn = (const void*)k_probe<KW, kTab, kMode>;
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(max_lds_per_block() - static_lds(fn)));
+        (void)hipGetLastError();
+        attr_set = true;
+    }
+    const void* fn = (const void*)k_probe<KW, kTab, kMode>;
+    if (lds + static_lds(fn) > max_lds_per_block()) {
+        fail("probe: %zu bytes of LDS exceed the workgroup limit", lds + static_lds(fn));
+        return;
+    }
+    hipLaunchKernelGGL((k_probe<KW, kTab, kMode>), dim3(grid), dim3(kProbeBlock), lds, stream(), A);
+}
+
+template <int KW, int kTab>
+void launch_probe_tab(int mode, int grid, size_t lds, const ProbeArgs& A)
+{
+    if constexpr (KW == 1) {
+        if (mode == kDirectVal) return launch_probe_one<KW, kTab, kDirectVal>(grid, lds, A);
+        if (mode == kDirectKey) return launch_probe_one<KW, kTab, kDirectKey>(grid, lds, A);
+    }
+    launch_probe_one<KW, kTab, kGeneric>(grid, lds, A);
+}
+
+template <int kMode, bool kCount>
+void launch_dense_pass(int grid, size_t lds, const ProbeArgs& A, uint32_t* cnt, const uint32_t* off)
+{
+    static bool attr_set = false;
+    if (!attr_set) {
+        const void* fn = (const void*)k_probe_dense<kMode, kCount>;
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(max_lds_per_block() - static_lds(fn)));
+        (void)hipGetLastError();
+        attr_set = true;
+    }
+    const void* fn = (const void*)k_probe_dense<kMode, kCount>;
+    if (lds + static_lds(fn) > max_lds_per_block()) {
+        fail("dense probe: %zu bytes of LDS exceed the workgroup limit", lds + static_lds(fn));
+        return;
+    }
+    hipLaunchKernelGGL((k_probe_dense<kMode, kCount>), dim3(grid), dim3(kProbeBlock), lds, stream(), A, cnt, off);
+}
+
+// Dynamic LDS the dense passes of a mode may use: the device limit less the
+// larger static LDS of the two passes.
+template <int kMode>
+size_t dense_lds_budget()
+{
+    const size_t a = static_lds((const void*)k_probe_dense<kMode, true>);
+    const size_t b = static_lds((const void*)k_probe_dense<kMode, false>);
+    const size_t s = a > b ? a : b;
+    return max_lds_per_block() > s ? max_lds_per_block() - s : 0;
+}
+
+size_t dense_lds_budget_of(int mode)
+{
+    return mode == kDirectVal ? dense_lds_budget<kDirectVal>() : mode == kDirectKey ? dense_lds_budget<kDirectKey>()
+                                                                                       : dense_lds_budget<kGeneric>();
+}
+
+// The two dense passes with the scan between them; pair_count gets the total.
+template <int kMode>
+void launch_dense_one(int grid, size_t lds, const ProbeArgs& A)
+{
+    const int64_t nchunks = (A.n + kWaveTile - 1) / kWaveTile;
+    uint32_t* cnt = (uint32_t*)dalloc((size_t)nchunks * 4 + 16);
+    uint32_t* off = (uint32_t*)dalloc((size_t)nchunks * 4 + 16);
+    if (!cnt || !off) {
+        dfree(cnt);
+        dfree(off);
+        return;
+    }
+    launch_dense_pass<kMode, true>(grid, lds, A, cnt, off);
+    if (exclusive_scan_u32(cnt, off, nchunks)) {
+        hipLaunchKernelGGL(k_chunk_total, dim3(1), dim3(64), 0, stream(), cnt, off, nchunks, A.pair_count);
+        launch_dense_pass<kMode, false>(grid, lds, A, cnt, off);
+    }
+    dfree(cnt);
+    dfree(off);
+}
+
+// Direct modes and the dense table exist for one-word keys only.
+template <int KW>
+void launch_probe(int mode, int tab, int grid, size_t lds, const ProbeArgs& A)
+{
+    if constexpr (KW == 1) {
+        if (tab == kTabDense) {
+            if (mode == kDirectVal) return launch_dense_one<kDirectVal>(grid, lds, A);
+            if (mode == kDirectKey) return launch_dense_one<kDirectKey>(grid, lds, A);
+            return launch_dense_one<kGeneric>(grid, lds, A);
+        }
+    }
+    if (tab == kTabLds) return launch_probe_tab<KW, kTabLds>(mode, grid, lds, A);
+    launch_probe_tab<KW, kTabHash>(mode, grid, lds, A);
 }
 
 template <int KW>
@@ -1044,12 +1855,17 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
     memset(&A, 0, sizeof(A));
     if (!bind_side(P->probe_slot_col, P->probe_nslots, P->spec.Probe, batch, &A.cols)) return false;
     A.K = P->probe_key;
+    {
+        const QdbValueType& kt = P->spec.Probe.Types[P->probe_slot_col[0]];
+        A.key_i64 = KW == 1 && P->probe_key.nkeys == 1 && kt.Width == 8 && !kt.Nullable && kt.Kind != QDB_KIND_FLOAT
+            && kt.Kind != QDB_KIND_BOOL && P->probe_key.f[0].slot == 0;
+    }
     A.n = n;
     A.sel = batch->Selection;
     A.table = J->table;
     A.cmask = J->capacity - 1;
-    A.next = J->next;
     A.ordinal = J->ordinal;
+    A.ord_to_arena = J->ord_to_arena;
     A.bloom = J->bloom;
     A.bloom_words = J->bloom_words;
     A.emit_unmatched = P->spec.JoinType != QDB_JOIN_INNER;
@@ -1065,14 +1881,48 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
             A.payload_width[f] = t.Width;
         }
     }
-    A.fused_acc = J->fused_acc;
-    // Whole table in LDS when it fits in 64 KiB; else the Bloom filter.
+    A.fused_part = nullptr;
+    // Kernel mode: direct key loads, and the fused aggregates' one probe
+    // column streamed beside the key when it is a non-nullable 8-byte
+    // integer.
+    int mode = A.key_i64 ? kDirectKey : kGeneric;
+    if (mode == kDirectKey) {
+        int vslot = -1;
+        bool one = true;
+        for (int f = 0; f < A.nfused; ++f) {
+            const FusedDesc& d = P->fused[f];
+            if (d.from_build) one = false;
+            if (d.kind == FK_COUNT_PAIRS || d.from_build) continue;
+            if (vslot >= 0 && vslot != d.slot) one = false;
+            vslot = d.slot;
+        }
+        if (one && vslot >= 0) {
+            const QdbValueType& vt = P->spec.Probe.Types[P->probe_slot_col[vslot]];
+            if (vt.Width == 8 && !vt.Nullable && vt.Kind != QDB_KIND_FLOAT && vt.Kind != QDB_KIND_BOOL) {
+                mode = kDirectVal;
+                A.val_slot = vslot;
+            }
+        }
+    }
+    // The slot table in LDS when it fits in 64 KiB; else the Bloom filter.
+    // Dense: as much of the membership bitmap as LDS holds.
     const size_t table_bytes = J->capacity * sizeof(Slot<KW>);
-    const bool lds_table = table_bytes <= 64 * 1024;
-    A.lds_table_slots = lds_table ? (uint32_t)J->capacity : 0u;
-    const size_t lds = lds_table ? table_bytes : (size_t)J->bloom_words * 4;
-    // Pair capacity: the sum of the chain lengths bounds the matches; with
-    // distinct build keys every probe row matches at most once.
+    const bool lds_table = !J->dense && table_bytes <= 64 * 1024;
+    size_t lds = lds_table ? table_bytes : (size_t)J->bloom_words * 4;
+    int tab = lds_table ? kTabLds : kTabHash;
+    if (J->dense) {
+        tab = kTabDense;
+        const uint64_t words = (J->drange + 31) / 32;
+        const uint64_t fit = dense_lds_budget_of(mode) / 4;  // beside the kernels' static LDS
+        A.lds_words = (uint32_t)(words < fit ? words : fit);
+        A.dmin = J->dmin;
+        A.drange = J->drange;
+        A.dbits = J->dbits;
+        A.dtab = J->dtab;
+        lds = (size_t)A.lds_words * 4;
+    }
+    // Pair capacity: with distinct build keys every probe row matches at
+    // most once; otherwise the duplicates bound the extra matches.
     unsigned long long cap = (unsigned long long)n;
     if (J->keys_distinct != J->rows) cap = (unsigned long long)n * (unsigned long long)(J->rows - J->keys_distinct + 1);
     if (A.emit_unmatched) cap += (unsigned long long)n;
@@ -1096,18 +1946,30 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
             }
             if (!hip_ok(hipMemcpyAsync(J->fused_acc, init.data(), init.size() * 8, hipMemcpyHostToDevice, stream()), "fused init")) return false;
         }
-        const int grid = grid_for(n, kProbeBlock, 1);
-        static bool attr_set = false;
-        if (!attr_set) {
-            (void)hipFuncSetAttribute((const void*)k_probe<KW, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
-            (void)hipFuncSetAttribute((const void*)k_probe<KW, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
-            attr_set = true;
+        const int64_t tiles = (n + (int64_t)kProbeBlock / kWave * kWaveTile - 1) / ((int64_t)kProbeBlock / kWave * kWaveTile);
+        const int per_cu = lds_table ? 2 : 1;
+        const int grid = (int)(tiles < (int64_t)num_cus() * per_cu ? tiles : (int64_t)num_cus() * per_cu);
+        unsigned long long* part = nullptr;
+        if (A.nfused) {
+            part = (unsigned long long*)dalloc((size_t)grid * kMaxFused * kAccWords * 8);
+            if (!part) return false;
+            A.fused_part = part;
         }
         {
             TimedRegion timer("join_probe");
-            if (lds_table) hipLaunchKernelGGL((k_probe<KW, true>), dim3(grid), dim3(kProbeBlock), lds, stream(), A);
-            else hipLaunchKernelGGL((k_probe<KW, false>), dim3(grid), dim3(kProbeBlock), lds, stream(), A);
+            launch_probe<KW>(mode, tab, grid, lds, A);
+            if (A.nfused) {
+                ReduceOps ops;
+                for (int k = 0; k < kMaxFused * kAccWords; ++k) {
+                    const int w = k % kAccWords;
+                    ops.op[k] = w < 4 ? 0 : w == 4 ? 1 : 2;
+                }
+                hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, stream(), part, grid, kMaxFused * kAccWords,
+                    A.nfused * kAccWords, ops, J->fused_acc);
+            }
         }
+        dfree(part);
+        if (!last_error_text().empty()) return false;
         unsigned long long count = 0;
         if (!hip_ok(hipGetLastError(), "probe launch")) return false;
         if (!hip_ok(hipMemcpyAsync(&count, J->pair_count, 8, hipMemcpyDeviceToHost, stream()), "pair count") || !sync("probe")) return false;

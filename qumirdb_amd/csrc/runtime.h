@@ -17,11 +17,15 @@ namespace qdbgpu {
 // Records an error for qdb_gpu_last_error (thread-local) and returns false.
 bool fail(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 void clear_error();
+// The calling thread's current error text (empty when none).
+const std::string& last_error_text();
 
 // Lazily selects the device and creates the library stream; false on error.
 bool ensure_init();
 hipStream_t stream();
 int num_cus();
+// Largest LDS allocation of one workgroup on the device, in bytes.
+size_t max_lds_per_block();
 
 // Checks a HIP status; on failure records "<what>: <hip error>".
 bool hip_ok(hipError_t e, const char* what);
@@ -59,6 +63,13 @@ private:
 // Grid size for a grid-stride kernel over `items` with `block` threads: at
 // most `per_cu` blocks per CU and no more than the items need.
 int grid_for(int64_t items, int block, int per_cu);
+
+// Exclusive prefix sum of n u32 values (out may alias nothing in in), and a
+// stable LSD radix sort of (key, value) u32 pairs over key bits [0, end_bit).
+// Both run on the library stream.
+bool exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n);
+bool sort_pairs_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out, int64_t n,
+    int end_bit);
 
 // Device-resident copy of a host struct (uploaded once, freed with dfree).
 void* upload(const void* host, size_t bytes);

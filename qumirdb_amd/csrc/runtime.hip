@@ -27,6 +27,7 @@ struct Runtime {
     int device = 0;
     hipStream_t stream = nullptr;
     int cus = 256;
+    size_t max_lds = 64 * 1024;
     // pool: size class -> free blocks; block -> size class
     std::map<size_t, std::vector<void*>> free_dev;
     std::unordered_map<void*, size_t> dev_sizes;
@@ -75,6 +76,11 @@ void clear_error()
     g_error.clear();
 }
 
+const std::string& last_error_text()
+{
+    return g_error;
+}
+
 bool hip_ok(hipError_t e, const char* what)
 {
     if (e == hipSuccess) return true;
@@ -93,6 +99,11 @@ bool ensure_init()
     hipDeviceProp_t prop;
     if (!hip_ok(hipGetDeviceProperties(&prop, r.device), "hipGetDeviceProperties")) return false;
     r.cus = prop.multiProcessorCount;
+    // Largest dynamic LDS allocation of one workgroup (160 KiB on gfx950).
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, r.device) == hipSuccess && lds > 0) {
+        r.max_lds = (size_t)lds;
+    }
     // Default priority, non-blocking with respect to the legacy stream.
     if (!hip_ok(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
     r.ready = true;
@@ -103,6 +114,26 @@ hipStream_t stream()
 {
     return R().stream;
 }
+
+size_t max_lds_per_block()
+{
+    return R().max_lds;
+}
+
+}  // namespace qdbgpu
+
+// [compute units, largest LDS allocation per workgroup in bytes]; returns
+// how many values were written, or -1 when the device is unavailable.
+extern "C" int64_t qdb_gpu_device_limits(int64_t* out, int64_t n)
+{
+    if (!qdbgpu::ensure_init()) return -1;
+    const int64_t v[2] = {(int64_t)qdbgpu::num_cus(), (int64_t)qdbgpu::max_lds_per_block()};
+    int64_t k = 0;
+    for (; k < n && k < 2; ++k) out[k] = v[k];
+    return k;
+}
+
+namespace qdbgpu {
 
 int num_cus()
 {

@@ -523,6 +523,7 @@ class JoinSpec:
     join_type: int = abi.JOIN_INNER
     probe_is_left: bool = True
     fused: Sequence[tuple[int, int]] = field(default_factory=list)
+    table_mode: int = abi.JOIN_TABLE_AUTO
 
     def to_c(self) -> abi.QdbRowJoinSpec:
         s = abi.QdbRowJoinSpec()
@@ -537,6 +538,7 @@ class JoinSpec:
         s.NumFused = len(self.fused)
         for i, (f, c) in enumerate(self.fused):
             s.Fused[i] = abi.QdbAggregateFn(f, c)
+        s.TableMode = self.table_mode
         return s
 
 

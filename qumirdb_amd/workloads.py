@@ -138,10 +138,11 @@ def join_build_predicate():
     return p, r
 
 
-def join_spec(fused: bool = True) -> JoinSpec:
+def join_spec(fused: bool = True, table_mode: int = abi.JOIN_TABLE_AUTO) -> JoinSpec:
     return JoinSpec(build_types=BUILD_TYPES, probe_types=PROBE_TYPES, build_keys=[BUILD_KEY], probe_keys=[PROBE_FK],
                     join_type=abi.JOIN_INNER, probe_is_left=True,
-                    fused=[(abi.AGG_COUNT_STAR, -1), (abi.AGG_SUM, PROBE_VALUE)] if fused else [])
+                    fused=[(abi.AGG_COUNT_STAR, -1), (abi.AGG_SUM, PROBE_VALUE)] if fused else [],
+                    table_mode=table_mode)
 
 
 class JoinDevice:

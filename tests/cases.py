@@ -173,6 +173,7 @@ class JoinCase:
     join_type: int = JOIN_INNER
     probe_is_left: bool = True
     fused: list = field(default_factory=list)
+    table_mode: int = 0     # 0: the build picks (dense direct table or hash), 1: hash table
 
     def to_c(self):
         abi = _abi()
@@ -188,6 +189,7 @@ class JoinCase:
         s.NumFused = len(self.fused)
         for i, (f, c) in enumerate(self.fused):
             s.Fused[i] = abi.QdbAggregateFn(f, c)
+        s.TableMode = self.table_mode
         return s
 
     def out_types(self):

@@ -115,8 +115,9 @@ def test_aggregate_batches_accumulate(gpu_backend):
     same(got, want)
 
 
+@pytest.mark.parametrize("table_mode", [0, 1])
 @pytest.mark.parametrize("seed", range(2))
-def test_join_random_vs_oracle(gpu_backend, seed):
+def test_join_random_vs_oracle(gpu_backend, seed, table_mode):
     rng = np.random.default_rng(seed)
     nb, npr = 5000 + seed * 40000, 200000
     types_b = [T(SIGNED, 8, nullable=True), T(SIGNED, 4), T(DECIMAL, 16, 2, True, 30)]
@@ -131,13 +132,56 @@ def test_join_random_vs_oracle(gpu_backend, seed):
     val = rng.integers(90000, 10500000, npr).astype(np.int64)
     bsel = (seg != 0).astype(np.uint8)
     for jt in (cases.JOIN_INNER, cases.JOIN_LEFT_OUTER):
-        case = JoinCase(types_b, types_p, [0], [0], join_type=jt, probe_is_left=True)
+        case = JoinCase(types_b, types_p, [0], [0], join_type=jt, probe_is_left=True, table_mode=table_mode)
         build = Batch(types_b, [bk, seg, pay], [bv, None, payv], bsel)
         probe = Batch(types_p, [pk, val], [pv, None])
         gp, grows = gpu_backend.join(case, [build], probe)
         op, orows = OracleBackend().join(case, [build], probe)
         assert np.array_equal(gp[np.lexsort((gp[:, 1], gp[:, 0]))], op[np.lexsort((op[:, 1], op[:, 0]))])
         assert sorted(grows, key=sort_key) == sorted(orows, key=sort_key)
+
+
+@pytest.mark.parametrize("table_mode", [0, 1])
+@pytest.mark.parametrize("join_type", [cases.JOIN_INNER, cases.JOIN_LEFT_OUTER])
+def test_join_distinct_dense_keys_vs_oracle(gpu_backend, join_type, table_mode):
+    """Distinct integer build keys over a small range (the dense direct table
+    serves them in mode 0) with NULL keys, a build selection, negative keys,
+    probe keys outside the range, and fused COUNT/SUM/MIN over a build
+    column: the same results as the oracle and as the hash table."""
+    rng = np.random.default_rng(7 + join_type)
+    nb, npr = 30000, 300000
+    types_b = [T(SIGNED, 8, nullable=True), T(SIGNED, 8)]
+    types_p = [T(SIGNED, 8, nullable=True), T(SIGNED, 8)]
+    bk = rng.permutation(nb).astype(np.int64) - 5000
+    bv = rng.random(nb) > 0.02
+    bw = rng.integers(-10 ** 12, 10 ** 12, nb).astype(np.int64)
+    bsel = (rng.random(nb) > 0.3).astype(np.uint8)
+    pk = rng.integers(-9000, nb, npr).astype(np.int64)
+    pv = rng.random(npr) > 0.03
+    pval = rng.integers(-10 ** 9, 10 ** 9, npr).astype(np.int64)
+    case = JoinCase(types_b, types_p, [0], [0], join_type=join_type, probe_is_left=True, table_mode=table_mode,
+                    fused=[(AGG_COUNT_STAR, -1), (AGG_SUM, -3), (AGG_MIN, -3), (AGG_SUM, 1)])
+    build = Batch(types_b, [bk, bw], [bv, None], bsel)
+    probe = Batch(types_p, [pk, pval], [pv, None])
+    kern, proc = gpu_backend.join_processor(case)
+    proc.append(gpu_backend.dev(build))
+    proc.build()
+    pairs = proc.probe_pairs(gpu_backend.dev(probe))
+    fused = proc.fused_results()
+    oj = O.OracleJoin(case.to_c())
+    oj.append(OracleBackend().host(build))
+    oj.build()
+    opairs = oj.probe(OracleBackend().host(probe))
+    srt = lambda p: p[np.lexsort((p[:, 1], p[:, 0]))]
+    assert np.array_equal(srt(pairs), srt(opairs))
+    # The fused aggregates equal an aggregate over the join's output rows:
+    # an unmatched probe row of the outer join counts, its build side NULL.
+    m = opairs[opairs[:, 1] >= 0]
+    prow = (opairs[:, 0] + (1 << 32)).astype(np.int64)
+    assert fused[0]["value"] == len(opairs)
+    assert fused[1]["value"] == int(bw[m[:, 1]].sum(dtype=np.int64))
+    assert fused[2]["value"] == int(bw[m[:, 1]].min())
+    assert fused[3]["value"] == int(pval[prow].sum(dtype=np.int64))
 
 
 def test_join_multi_batch_build_and_fused(gpu_backend):
@@ -302,23 +346,27 @@ def test_lineitem_operators_equal_fused(gpu_backend):
     assert plain == W.lineitem_finalize(p2.finish())
 
 
-def run_join(build_rows, probe_rows):
+def run_join(build_rows, probe_rows, table_mode=0):
     from qumirdb_amd import workloads as W
     from qumirdb_amd.host import FilterKernel, RowJoinKernels, TRowJoinProcessor
     jd = W.JoinDevice(build_rows, probe_rows)
     sel = FilterKernel(W.BUILD_TYPES, *W.join_build_predicate())(jd.build)
     jd.build.set_selection_buffer(sel)
-    proc = TRowJoinProcessor(RowJoinKernels(W.join_spec(fused=True)), build_is_left=False)
+    proc = TRowJoinProcessor(RowJoinKernels(W.join_spec(fused=True, table_mode=table_mode)), build_is_left=False)
     proc.append(jd.build)
     proc.build()
     pairs = proc.probe_pairs(jd.probe)
     return jd, sel, pairs, proc.fused_results(), proc
 
 
-def test_join_small_vs_oracle(gpu_backend):
+@pytest.mark.parametrize("table_mode", [0, 1])
+def test_join_small_vs_oracle(gpu_backend, table_mode):
+    """The join workload at 150K x 2M against the oracle's row-table join:
+    the same sorted pair set, fused COUNT(*) and exact SUM; with the table
+    the build picks (dense direct table here) and with the hash table."""
     from qumirdb_amd import workloads as W
     B, P = 150_000, 2_000_000
-    jd, sel, pairs, fused, proc = run_join(B, P)
+    jd, sel, pairs, fused, proc = run_join(B, P, table_mode)
     key, seg = O.gen_join_build(W.SEED_JOIN, 0, B, B)
     fk, val = O.gen_join_probe(W.SEED_JOIN, 0, P, B)
     oj = O.OracleJoin(W.join_spec().to_c())
@@ -333,14 +381,15 @@ def test_join_small_vs_oracle(gpu_backend):
     assert abs(len(op) / P - 0.2) < 0.01
 
 
-def test_join_full_size_properties(gpu_backend):
+@pytest.mark.parametrize("table_mode", [0, 1])
+def test_join_full_size_properties(gpu_backend, table_mode):
     """The 1.5M x 60M join: every pair joins equal keys of a segment-1 build
     row; the pair count, the fused COUNT(*) and the rows whose key is in
     the filtered build agree; the fused exact SUM equals an independent
     aggregate over the probe rows the pairs name."""
     from qumirdb_amd import workloads as W
     B, P = 1_500_000, 60_000_000
-    jd, sel, pairs, fused, proc = run_join(B, P)
+    jd, sel, pairs, fused, proc = run_join(B, P, table_mode)
     key = jd.key.to_numpy(np.int64, B)
     seg = jd.segment.to_numpy(np.int32, B)
     fk = jd.fk.to_numpy(np.int64, P)
