@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["join", "lineitem", "groups", "all"], default="all")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (3 launches), the known-size kernel that "
+                         "scales rocprof FETCH_SIZE / WRITE_SIZE readings")
     return ap.parse_args()
 
 
@@ -196,6 +199,42 @@ def cpu_join_baseline(build_rows=1_500_000, sample_probe=60_000_000):
                       f"{t1 - t0:.2f} s, one thread"}
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_probe_dense__join_build1.5M_probe60M.md")
+
+
+def profiled_traffic():
+    """HBM bytes per probe launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (counter readings scaled by the copy calibration), or
+    None when the summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(mib):
+    """A plain copy of `mib` MiB through an elementwise kernel (bitwise NOT,
+    3 launches; a same-type copy_ would go to the DMA engine, which no kernel
+    counter sees): FETCH_SIZE / WRITE_SIZE of this kernel against its known
+    bytes gives the factor that turns the counters into HBM bytes."""
+    import torch
+    n = mib << 20
+    src = torch.ones(n // 8, dtype=torch.int64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        torch.bitwise_not(src, out=dst)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / 3
+    del src, dst
+    return {"bytes_read": n, "bytes_written": n, "ms": ms, "gbs": 2 * n / (ms * 1e-3) / 1e9}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -213,6 +252,8 @@ def main():
         out["lineitem"] = bench_lineitem(args.steps, args.warmup)
     if args.workload in ("groups", "all"):
         out["groups"] = bench_groups(max(1, args.steps // 2), max(1, args.warmup // 2))
+    if args.copy_calibration:
+        out["copy_calibration"] = copy_calibration(args.copy_calibration)
     head = out.get("join") or out.get("lineitem") or out.get("groups")
     name = "join" if "join" in out else list(out)[0]
     cpu = None
@@ -222,8 +263,8 @@ def main():
     if name == "join":
         j = out["join"]
         roof = {"bound": "hbm", "achieved": round(j["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(j["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_probe",
-                "kernel_ms": round(j["probe_kernel_ms"], 4)}
+                "frac": round(j["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(),
+                "kernel": "join probe (k_probe_dense passes)", "kernel_ms": round(j["probe_kernel_ms"], 4)}
     import ctypes
     from qumirdb_amd import lib
     limits = (ctypes.c_int64 * 2)()
