@@ -691,18 +691,45 @@ __device__ __forceinline__ void fused_flush(const ProbeArgs& P, const FusedLane 
 
 constexpr uint32_t kAbsent = 0xffffffffu;   // dense table: no build row for this key
 
-// Dense direct table probe (distinct one-word integer keys), in two
-// passes over chunks of kWaveTile rows. A wave takes a chunk at a time, with
-// no workgroup barrier once the bitmap is in LDS: kTileRows rows per lane,
-// loaded as independent rounds (keys, then the bitmap: LDS for the keys it
-// covers, HBM beyond). The count pass writes each chunk's output row count;
-// after a scan, the write pass reads the ordinal table for the matches and
-// writes the chunk's pairs from its offset on, in probe row order (row i =
-// chunk start + r * 64 + lane, r-major), with the fused aggregate. A key
-// matches at most one build row, so a row's output is at most one pair (or
-// its unmatched pair in an outer join).
-template <int kMode, bool kCount>
-__global__ void __launch_bounds__(kProbeBlock) k_probe_dense(ProbeArgs P, uint32_t* chunk_cnt, const uint32_t* chunk_off)
+// Places a block tile's output rows: every wave passes its row count `run`;
+// returns the wave's first output position. One atomic per block tile (the
+// waves' counts are combined through LDS between two barriers); every
+// thread of the block calls it once per tile.
+__device__ __forceinline__ unsigned long long reserve_block_tile(unsigned long long* counter, uint32_t run, int wave, int lane)
+{
+    constexpr int nw = kProbeBlock / kWave;
+    __shared__ unsigned int wave_run[nw];
+    __shared__ unsigned long long block_base;
+    if (lane == 0) wave_run[wave] = run;
+    __syncthreads();
+    if (wave == 0) {
+        const unsigned int v = lane < nw ? wave_run[lane] : 0u;
+        unsigned int incl = v;
+#pragma unroll
+        for (int o = 1; o < nw; o <<= 1) {
+            const unsigned int y = (unsigned int)__shfl_up((int)incl, o, kWave);
+            if (lane >= o) incl += y;
+        }
+        if (lane < nw) wave_run[lane] = incl - v;
+        const unsigned int total = (unsigned int)__shfl((int)incl, nw - 1, kWave);
+        if (lane == 0) block_base = total ? atomicAdd(counter, (unsigned long long)total) : 0ull;
+    }
+    __syncthreads();
+    return block_base + wave_run[wave];
+}
+
+// Dense direct table probe (distinct one-word integer keys), one pass. A
+// block takes tiles of kProbeBlock / kWave wave tiles; per wave tile, with
+// kTileRows rows per lane loaded as independent rounds: the keys; the
+// bitmap words (LDS for the keys it covers; beyond it the ordinal table is
+// read directly, kAbsent = no build row); the ordinals of the matches and
+// the fused aggregate's probe column. The block's output rows are placed
+// with one atomic per tile; within a wave tile they are in probe row order
+// (row i = tile start + r * 64 + lane, r-major). A key matches at most one
+// build row, so a row's output is at most one pair (or its unmatched pair in
+// an outer join).
+template <int kMode>
+__global__ void __launch_bounds__(kProbeBlock) k_probe_dense(ProbeArgs P)
 {
     constexpr bool kDirect = kMode != kGeneric;
     extern __shared__ uint64_t lds64[];
@@ -714,6 +741,7 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe_dense(ProbeArgs P, uint32
     __syncthreads();
     const uint64_t lds_range = (uint64_t)P.lds_words * 32u;
     const int lane = lane_id();
+    const int wave = threadIdx.x / kWave;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     constexpr int kAcc = kMode == kDirectVal ? 1 : kMaxFused;
     FusedLane acc[kAcc];
@@ -727,12 +755,15 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe_dense(ProbeArgs P, uint32
     unsigned long long v_pairs = 0;
     uint64_t v_lo = 0, v_hi = 0;
     int64_t v_min = 0x7fffffffffffffffll, v_max = (int64_t)0x8000000000000000ll;
-    const int64_t nchunks = (P.n + kWaveTile - 1) / kWaveTile;
-    const int64_t nwaves = (int64_t)gridDim.x * (kProbeBlock / kWave);
-    for (int64_t c = (int64_t)blockIdx.x * (kProbeBlock / kWave) + threadIdx.x / kWave; c < nchunks; c += nwaves) {
-        const int64_t t0 = c * kWaveTile;
-        // Round 1: keys, unconditional loads at clamped rows.
+    constexpr int64_t kBlockTile = (int64_t)(kProbeBlock / kWave) * kWaveTile;
+    // Every thread of the block runs the same number of block tiles.
+    for (int64_t b0 = blockIdx.x * kBlockTile; b0 < P.n; b0 += (int64_t)gridDim.x * kBlockTile) {
+        const int64_t t0 = b0 + (int64_t)wave * kWaveTile;
+        // Round 1: keys and the fused aggregates' probe column, unconditional
+        // loads at clamped rows (the column is streamed with the keys: a
+        // later round would wait for it after the ordinals).
         uint64_t key[kTileRows];
+        int64_t val[kTileRows];
         uint32_t part = 0, live = 0;
 #pragma unroll
         for (int r = 0; r < kTileRows; ++r) {
@@ -744,16 +775,14 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe_dense(ProbeArgs P, uint32
             uint64_t k1[1] = {0};
             const bool kv = probe_row_key<1, kDirect>(P, ic, k1);
             key[r] = k1[0];
+            val[r] = 0;
+            if (kMode == kDirectVal) val[r] = __builtin_nontemporal_load((const int64_t*)P.cols.c[P.val_slot].data + ic);
             part |= sel ? 1u << r : 0u;
             live |= (sel & kv) ? 1u << r : 0u;
         }
-        // Round 2: the bitmap words of every row; writing, then the
-        // ordinals of the matches. Counting reads the HBM part of the bitmap
-        // for keys beyond the LDS part; writing reads the ordinal table for
-        // those keys instead (kAbsent there = no build row), so that its
-        // only memory reads after the keys are the ordinals. The rounds are
-        // kept apart: memory counters are waited on in issue order.
-        uint32_t ixs[kTileRows], wl[kTileRows], wg[kTileRows];
+        // Round 2: the LDS bitmap words. The rounds are kept apart: memory
+        // counters are waited on in issue order.
+        uint32_t ixs[kTileRows], wl[kTileRows];
         uint32_t inr = 0, cov = 0;
 #pragma unroll
         for (int r = 0; r < kTileRows; ++r) {
@@ -765,52 +794,31 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe_dense(ProbeArgs P, uint32
             inr |= in_range ? 1u << r : 0u;
             cov |= covered ? 1u << r : 0u;
             wl[r] = lbits[covered ? (ix >> 5) : 0u];
-            // The HBM part is read with its own intrinsic so that the
-            // compiler cannot fold the two reads into one flat load.
-            wg[r] = 0;
-            if (kCount) wg[r] = __builtin_nontemporal_load(P.dbits + (covered ? 0u : (ix >> 5)));
         }
         __builtin_amdgcn_sched_barrier(0);
         uint32_t need = 0;
 #pragma unroll
         for (int r = 0; r < kTileRows; ++r) {
             const bool covered = (cov >> r) & 1u;
-            const uint32_t w = covered ? wl[r] : wg[r];
-            const bool bit = ((w >> (ixs[r] & 31u)) & 1u) != 0u;
-            const bool h = ((inr >> r) & 1u) & (kCount ? bit : (!covered | bit));
-            need |= h ? 1u << r : 0u;
+            const bool bit = ((wl[r] >> (ixs[r] & 31u)) & 1u) != 0u;
+            need |= (((inr >> r) & 1u) & (!covered | bit)) ? 1u << r : 0u;
         }
-        // Round 3 (writing): the ordinals, and the values of the fused
-        // aggregates' probe column.
+        // Round 3: the ordinals.
         uint32_t head[kTileRows];
-        int64_t val[kTileRows];
         uint32_t hit = need;
-        if (!kCount) {
 #pragma unroll
-            for (int r = 0; r < kTileRows; ++r) {
-                head[r] = P.dtab[((need >> r) & 1u) ? ixs[r] : 0u];
-                val[r] = 0;
-                if (kMode == kDirectVal) {
-                    const int64_t i = t0 + (int64_t)r * kWave + lane;
-                    val[r] = __builtin_nontemporal_load((const int64_t*)P.cols.c[P.val_slot].data + (i < P.n ? i : 0));
-                }
-            }
-            __builtin_amdgcn_sched_barrier(0);
+        for (int r = 0; r < kTileRows; ++r) head[r] = P.dtab[((need >> r) & 1u) ? ixs[r] : 0u];
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int r = 0; r < kTileRows; ++r) {
-                if (head[r] == kAbsent) hit &= ~(1u << r);
-            }
+        for (int r = 0; r < kTileRows; ++r) {
+            if (head[r] == kAbsent) hit &= ~(1u << r);
         }
         const uint32_t emit = hit | (P.emit_unmatched ? part : 0u);
-        if (kCount) {
-            uint32_t run = 0;
+        uint32_t run = 0;
 #pragma unroll
-            for (int r = 0; r < kTileRows; ++r) run += (uint32_t)__popcll(__ballot((emit >> r) & 1u));
-            if (lane == 0) chunk_cnt[c] = run;
-            continue;
-        }
+        for (int r = 0; r < kTileRows; ++r) run += (uint32_t)__popcll(__ballot((emit >> r) & 1u));
+        unsigned long long pos0 = reserve_block_tile(P.pair_count, run, wave, lane);
         // Pairs in probe row order and the fused aggregate.
-        unsigned long long pos0 = chunk_off[c];
 #pragma unroll
         for (int r = 0; r < kTileRows; ++r) {
             const uint64_t m = __ballot((emit >> r) & 1u);
@@ -864,13 +872,7 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe_dense(ProbeArgs P, uint32
             }
         }
     }
-    if (!kCount && P.nfused) fused_flush<kAcc, kMode == kDirectVal>(P, acc, v_pairs, v_lo, v_hi, v_min, v_max, lane);
-}
-
-// pair_count = the last chunk's offset + its count.
-__global__ void k_chunk_total(const uint32_t* cnt, const uint32_t* off, int64_t nchunks, unsigned long long* total)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) *total = nchunks ? (unsigned long long)off[nchunks - 1] + cnt[nchunks - 1] : 0ull;
+    if (P.nfused) fused_flush<kAcc, kMode == kDirectVal>(P, acc, v_pairs, v_lo, v_hi, v_min, v_max, lane);
 }
 
 template <int KW, int kTab, int kMode>
@@ -906,8 +908,6 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe(ProbeArgs P)
     unsigned long long v_pairs = 0;
     uint64_t v_lo = 0, v_hi = 0;
     int64_t v_min = 0x7fffffffffffffffll, v_max = (int64_t)0x8000000000000000ll;
-    __shared__ unsigned int wave_run[kProbeBlock / kWave];
-    __shared__ unsigned long long block_base;
     const int wave = threadIdx.x / kWave;
     constexpr int64_t kBlockTile = (int64_t)(kProbeBlock / kWave) * kWaveTile;
     // Every thread of the block runs the same number of block tiles (two
@@ -999,23 +999,7 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe(ProbeArgs P)
                 run += x;
             }
         }
-        if (lane == 0) wave_run[wave] = run;
-        __syncthreads();
-        if (wave == 0) {
-            constexpr int nw = kProbeBlock / kWave;
-            const unsigned int v = lane < nw ? wave_run[lane] : 0u;
-            unsigned int incl = v;
-#pragma unroll
-            for (int o = 1; o < nw; o <<= 1) {
-                const unsigned int y = (unsigned int)__shfl_up((int)incl, o, kWave);
-                if (lane >= o) incl += y;
-            }
-            if (lane < nw) wave_run[lane] = incl - v;
-            const unsigned int total = (unsigned int)__shfl((int)incl, nw - 1, kWave);
-            if (lane == 0) block_base = total ? atomicAdd(P.pair_count, (unsigned long long)total) : 0ull;
-        }
-        __syncthreads();
-        unsigned long long pos0 = block_base + wave_run[wave];
+        unsigned long long pos0 = reserve_block_tile(P.pair_count, run, wave, lane);
 #pragma unroll
         for (int r = 0; r < kTileRows; ++r) {
             const uint32_t e = emits(r);
@@ -1771,32 +1755,29 @@ void launch_probe_tab(int mode, int grid, size_t lds, const ProbeArgs& A)
     launch_probe_one<KW, kTab, kGeneric>(grid, lds, A);
 }
 
-template <int kMode, bool kCount>
-void launch_dense_pass(int grid, size_t lds, const ProbeArgs& A, uint32_t* cnt, const uint32_t* off)
+template <int kMode>
+void launch_dense_one(int grid, size_t lds, const ProbeArgs& A)
 {
     static bool attr_set = false;
+    const void* fn = (const void*)k_probe_dense<kMode>;
     if (!attr_set) {
-        const void* fn = (const void*)k_probe_dense<kMode, kCount>;
         (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(max_lds_per_block() - static_lds(fn)));
         (void)hipGetLastError();
         attr_set = true;
     }
-    const void* fn = (const void*)k_probe_dense<kMode, kCount>;
     if (lds + static_lds(fn) > max_lds_per_block()) {
         fail("dense probe: %zu bytes of LDS exceed the workgroup limit", lds + static_lds(fn));
         return;
     }
-    hipLaunchKernelGGL((k_probe_dense<kMode, kCount>), dim3(grid), dim3(kProbeBlock), lds, stream(), A, cnt, off);
+    hipLaunchKernelGGL((k_probe_dense<kMode>), dim3(grid), dim3(kProbeBlock), lds, stream(), A);
 }
 
-// Dynamic LDS the dense passes of a mode may use: the device limit less the
-// larger static LDS of the two passes.
+// Dynamic LDS the dense probe of a mode may use: the device limit less the
+// kernel's static LDS.
 template <int kMode>
 size_t dense_lds_budget()
 {
-    const size_t a = static_lds((const void*)k_probe_dense<kMode, true>);
-    const size_t b = static_lds((const void*)k_probe_dense<kMode, false>);
-    const size_t s = a > b ? a : b;
+    const size_t s = static_lds((const void*)k_probe_dense<kMode>);
     return max_lds_per_block() > s ? max_lds_per_block() - s : 0;
 }
 
@@ -1804,27 +1785,6 @@ size_t dense_lds_budget_of(int mode)
 {
     return mode == kDirectVal ? dense_lds_budget<kDirectVal>() : mode == kDirectKey ? dense_lds_budget<kDirectKey>()
                                                                                        : dense_lds_budget<kGeneric>();
-}
-
-// The two dense passes with the scan between them; pair_count gets the total.
-template <int kMode>
-void launch_dense_one(int grid, size_t lds, const ProbeArgs& A)
-{
-    const int64_t nchunks = (A.n + kWaveTile - 1) / kWaveTile;
-    uint32_t* cnt = (uint32_t*)dalloc((size_t)nchunks * 4 + 16);
-    uint32_t* off = (uint32_t*)dalloc((size_t)nchunks * 4 + 16);
-    if (!cnt || !off) {
-        dfree(cnt);
-        dfree(off);
-        return;
-    }
-    launch_dense_pass<kMode, true>(grid, lds, A, cnt, off);
-    if (exclusive_scan_u32(cnt, off, nchunks)) {
-        hipLaunchKernelGGL(k_chunk_total, dim3(1), dim3(64), 0, stream(), cnt, off, nchunks, A.pair_count);
-        launch_dense_pass<kMode, false>(grid, lds, A, cnt, off);
-    }
-    dfree(cnt);
-    dfree(off);
 }
 
 // Direct modes and the dense table exist for one-word keys only.
