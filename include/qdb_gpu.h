@@ -40,9 +40,22 @@
 #ifndef QDB_GPU_H
 #define QDB_GPU_H
 
+#ifndef __HIPCC_RTC__
 #include <stddef.h>
 #include <stdint.h>
 #include <stdbool.h>
+#else
+/* Kernels compiled at run time (hipRTC) get the fixed-width types from the
+ * runtime compiler's own header. */
+typedef __hip_internal::int8_t int8_t;
+typedef __hip_internal::int16_t int16_t;
+typedef __hip_internal::int32_t int32_t;
+typedef __hip_internal::int64_t int64_t;
+typedef __hip_internal::uint8_t uint8_t;
+typedef __hip_internal::uint16_t uint16_t;
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::uint64_t uint64_t;
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -420,6 +433,20 @@ int64_t qdb_gpu_struct_sizes(int64_t* out, int64_t n);
 /* Device limits the kernels size themselves by: [compute units, largest LDS
  * allocation of one workgroup in bytes]. Returns the count written, or -1. */
 int64_t qdb_gpu_device_limits(int64_t* out, int64_t n);
+
+/* Kernels generated per aggregate (no counterpart in the reference, whose
+ * compiler generates its kernels per query; for diagnostics and tests).
+ * qdb_gpu_aggregate_kernel_source writes the source of the update kernel
+ * generated for `spec` (at most cap-1 bytes and a NUL) and returns its
+ * length, or -1; it needs no GPU. qdb_gpu_jit_check compiles a generated
+ * source for `arch` without loading it: 0 when it compiles, else -1 with
+ * the compiler log in err. qdb_gpu_jit_stats returns how many kernels this
+ * process compiled at run time and their total compile time. The
+ * environment variable QDB_GPU_JIT=0 runs the precompiled generic kernels
+ * instead. */
+int64_t qdb_gpu_aggregate_kernel_source(const QdbAggregateSpec* spec, char* out, int64_t cap);
+int qdb_gpu_jit_check(const char* src, const char* arch, char* err, int64_t cap);
+int64_t qdb_gpu_jit_stats(double* total_ms);
 
 /* ------------------------------------------------------------------------ */
 /* Host processors over the entry points above, mirroring qdb/exec:         */

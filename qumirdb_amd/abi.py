@@ -193,7 +193,7 @@ EXPORTS = [
     "qdb_gpu_memset", "qdb_gpu_synchronize", "qdb_gpu_stage_rowset", "qdb_gpu_download_rowset",
     "qdb_gpu_gen_lineitem", "qdb_gpu_gen_join_build", "qdb_gpu_gen_join_probe", "qdb_gpu_gen_zipf_groups",
     "qdb_gpu_timing_enable", "qdb_gpu_timing_reset", "qdb_gpu_timing_avg_ms", "qdb_gpu_struct_sizes",
-    "qdb_gpu_device_limits",
+    "qdb_gpu_device_limits", "qdb_gpu_aggregate_kernel_source", "qdb_gpu_jit_check", "qdb_gpu_jit_stats",
     "qdb_gpu_exec_last_error", "qdb_gpu_agg_processor_new", "qdb_gpu_agg_processor_add",
     "qdb_gpu_agg_processor_finish", "qdb_gpu_agg_processor_groups", "qdb_gpu_agg_processor_free",
     "qdb_gpu_join_processor_new", "qdb_gpu_join_processor_append", "qdb_gpu_join_processor_build",
@@ -236,6 +236,9 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "qdb_gpu_timing_reset": (None, []),
         "qdb_gpu_timing_avg_ms": (C.c_double, [C.c_char_p, C.POINTER(i64)]),
         "qdb_gpu_device_limits": (i64, [C.POINTER(i64), i64]),
+        "qdb_gpu_aggregate_kernel_source": (i64, [C.POINTER(QdbAggregateSpec), C.c_char_p, i64]),
+        "qdb_gpu_jit_check": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, i64]),
+        "qdb_gpu_jit_stats": (i64, [C.POINTER(C.c_double)]),
         "qdb_gpu_struct_sizes": (i64, [C.POINTER(i64), i64]),
         "qdb_gpu_exec_last_error": (C.c_char_p, []),
         "qdb_gpu_agg_processor_new": (P, [C.POINTER(QdbAggregateKernels), i64, C.c_int, i64]),

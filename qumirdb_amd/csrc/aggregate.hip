@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "aggregate_kernels.h"
+#include "jit.h"
 #include "slots.h"
 
 namespace qdbgpu {
@@ -39,6 +40,8 @@ namespace qdbgpu {
 namespace {
 
 using namespace agg;
+
+typedef long long ll2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ void store_width(char* data, int width, unsigned int g, u128 v)
 {
@@ -63,7 +66,7 @@ __global__ void __launch_bounds__(kBlock) k_agg_finalize(FinArgs F)
         uint8_t masks[QDB_AGG_MAX_KEYS + QDB_AGG_MAX_AGGS];
         for (int c = 0; c < F.nkeys + F.nred; ++c) masks[c] = 0;
         for (unsigned int g = b * 8u; g < b * 8u + 8u && g < F.groups; ++g) {
-            const uint64_t* r = F.recs + (size_t)g * F.R;
+            const uint64_t* r = F.recs + (size_t)(F.index ? F.index[g] : g) * (size_t)F.stride;
             for (int f = 0; f < F.nkeys; ++f) {
                 const KeyFieldDesc kf = F.keys[f];
                 bool is_null = kf.nullable && ((r[kf.null_word] >> kf.null_bit) & 1ull);
@@ -145,6 +148,77 @@ __global__ void __launch_bounds__(kBlock) k_agg_finalize(FinArgs F)
     }
 }
 
+// Fills `n` records of `stride` words with the empty-table pattern (the
+// empty key marker, then every state word's identity), 16 bytes per store.
+struct RecordPattern {
+    uint64_t w[2 * kMaxWords];
+};
+
+__global__ void __launch_bounds__(kBlock) k_fill_records(uint64_t* recs, uint64_t words, int stride, RecordPattern pat)
+{
+    const uint64_t pairs = words / 2;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < pairs; i += (uint64_t)gridDim.x * blockDim.x) {
+        const int w = (int)((2 * i) % (uint64_t)stride);
+        const ll2 v = {(long long)pat.w[w], (long long)pat.w[w + 1]};
+        *(ll2*)(recs + 2 * i) = v;
+    }
+}
+
+constexpr int kOccChunk = 4096;   // slots per workgroup in the occupancy passes
+
+template <int KW>
+__device__ __forceinline__ bool occupied(const uint64_t* recs, const unsigned long long* ctrl, int stride, uint64_t slots,
+    int extra_used, uint64_t s)
+{
+    if (s >= slots) return s == slots && extra_used;
+    return slot_used<KW>(recs, ctrl, stride, s);
+}
+
+// Groups per chunk of kOccChunk slots (slots [0, slots] with the extra
+// record last).
+template <int KW>
+__global__ void __launch_bounds__(kBlock) k_occ_counts(const uint64_t* recs, const unsigned long long* ctrl, int stride,
+    uint64_t slots, int extra_used, uint32_t* counts)
+{
+    __shared__ unsigned int total;
+    if (threadIdx.x == 0) total = 0u;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kOccChunk;
+    unsigned int n = 0;
+    for (int k = threadIdx.x; k < kOccChunk; k += blockDim.x) {
+        n += occupied<KW>(recs, ctrl, stride, slots, extra_used, base + (uint64_t)k) ? 1u : 0u;
+    }
+    atomicAdd(&total, n);
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+// The slot of every group, in slot order: chunk offset + rank in the chunk.
+template <int KW>
+__global__ void __launch_bounds__(kBlock) k_occ_index(const uint64_t* recs, const unsigned long long* ctrl, int stride,
+    uint64_t slots, int extra_used, const uint32_t* offsets, uint32_t* index)
+{
+    __shared__ unsigned int wave_total[kBlock / kWave];
+    const uint64_t base = (uint64_t)blockIdx.x * kOccChunk;
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    unsigned int out = offsets[blockIdx.x];
+    for (int k0 = 0; k0 < kOccChunk; k0 += kBlock) {
+        const uint64_t s = base + (uint64_t)(k0 + threadIdx.x);
+        const bool used = occupied<KW>(recs, ctrl, stride, slots, extra_used, s);
+        const uint64_t m = __ballot(used);
+        if (lane == 0) wave_total[wave] = (unsigned int)__popcll(m);
+        __syncthreads();
+        unsigned int before = 0, all = 0;
+        for (int v = 0; v < kBlock / kWave; ++v) {
+            before += v < wave ? wave_total[v] : 0u;
+            all += wave_total[v];
+        }
+        if (used) index[out + before + (unsigned int)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)s;
+        out += all;
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host: plans, tables, entry points.
 // ---------------------------------------------------------------------------
@@ -164,6 +238,15 @@ struct AggPlan {
     QdbValueType out_types[QDB_AGG_MAX_KEYS + QDB_AGG_MAX_AGGS];
     bool has_pairs = false;
     uint64_t word_init_host[kMaxWords];
+    // Host copies of the programs and the value type of every device slot,
+    // for the kernel generated for this aggregate.
+    bool has_pred = false;
+    DevProgram h_pred;
+    std::vector<DevProgram> h_progs;
+    QdbValueType slot_type[kMaxDevCols];
+    std::string jit_src;
+    hipFunction_t jit_fn = nullptr;
+    bool jit_tried = false;
 
     ~AggPlan()
     {
@@ -175,10 +258,11 @@ struct AggPlan {
 // The device table behind a HashTable blob.
 struct AggTable {
     AggPlan* plan = nullptr;
-    unsigned long long* ctrl = nullptr;
-    uint64_t capacity = 0;
+    unsigned long long* ctrl = nullptr; // wider keys only
+    uint64_t capacity = 0;              // slots (a power of two); the extra record follows
     uint64_t* recs = nullptr;
-    unsigned int max_groups = 0;
+    int rstride = 1;                    // words per record
+    unsigned int extra_used = 0;        // the extra record holds a group
     unsigned int* counters = nullptr;   // device [4]
     unsigned int groups = 0;
     int64_t expect_groups = 0;          // ExpectRowGroups hint
@@ -262,7 +346,7 @@ void output_types(const QdbAggregateSpec& spec, const QdbValueType* col_types, Q
     }
 }
 
-bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P)
+bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P, bool on_device = true)
 {
     P->spec = spec;
     AggArgs& A = P->proto;
@@ -282,6 +366,7 @@ bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P)
         if (col_slot[c] < 0) {
             if (P->nslots >= kMaxDevCols) return -2;
             col_slot[c] = P->nslots;
+            P->slot_type[P->nslots] = spec.Input.Types[c];
             P->slot_col[P->nslots++] = c;
         }
         return col_slot[c];
@@ -477,8 +562,12 @@ bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P)
         QdbExprProgram pred;
         if (!remap(spec.Predicate, &pred)) return false;
         const DevProgram dp = to_dev_program(pred);
-        P->d_pred = (DevProgram*)upload(&dp, sizeof(dp));
-        if (!P->d_pred) return false;
+        P->has_pred = true;
+        P->h_pred = dp;
+        if (on_device) {
+            P->d_pred = (DevProgram*)upload(&dp, sizeof(dp));
+            if (!P->d_pred) return false;
+        }
     }
     if (spec.NumComputed > 0) {
         std::vector<DevProgram> progs((size_t)spec.NumComputed);
@@ -487,8 +576,11 @@ bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P)
             if (!remap(spec.Computed[k], &prog)) return false;
             progs[(size_t)k] = to_dev_program(prog);
         }
-        P->d_progs = (DevProgram*)upload(progs.data(), progs.size() * sizeof(DevProgram));
-        if (!P->d_progs) return false;
+        P->h_progs = progs;
+        if (on_device) {
+            P->d_progs = (DevProgram*)upload(progs.data(), progs.size() * sizeof(DevProgram));
+            if (!P->d_progs) return false;
+        }
     }
 
     // Finalize layout.
@@ -531,24 +623,191 @@ bool bind_batch(const AggPlan* P, const TRowSet* batch, AggArgs* A)
 }
 
 
+// Words per record: a power of two up to 16 words (a record is then one
+// 128-byte line, aligned), else a multiple of 16.
+int record_stride(int R)
+{
+    if (R > 16) return (R + 15) / 16 * 16;
+    int s = 1;
+    while (s < R) s *= 2;
+    return s;
+}
+
+bool fill_records(const AggPlan* P, uint64_t* recs, uint64_t nrecs, int stride)
+{
+    RecordPattern pat;
+    memset(&pat, 0, sizeof(pat));
+    const int R = P->KW + P->proto.nwords;
+    for (int w = 0; w < 2 * kMaxWords; ++w) {
+        const int x = w % stride;
+        if (x >= R) pat.w[w] = 0;
+        else if (x < P->KW) pat.w[w] = (P->KW == 1 && x == 0) ? kEmptyKey : 0ull;
+        else pat.w[w] = P->word_init_host[x - P->KW];
+    }
+    const uint64_t words = nrecs * (uint64_t)stride;
+    const int grid = grid_for((int64_t)(words / 2), kBlock, 8);
+    hipLaunchKernelGGL(k_fill_records, dim3(grid), dim3(kBlock), 0, stream(), recs, words, stride, pat);
+    return hip_ok(hipGetLastError(), "record fill");
+}
+
+// A table of `capacity` slots plus the extra record, every slot empty.
 bool table_alloc(AggTable* T, uint64_t capacity)
 {
     const AggPlan* P = T->plan;
-    const int R = P->KW + P->proto.nwords;
     T->capacity = capacity;
-    T->max_groups = (unsigned int)(capacity / 2 < 0xfffffff0ull ? capacity / 2 : 0xfffffff0ull);
-    T->ctrl = (unsigned long long*)dalloc(capacity * 8);
-    T->recs = (uint64_t*)dalloc((size_t)T->max_groups * R * 8);
-    if (!T->ctrl || !T->recs) return false;
-    return zero_fill(T->ctrl, capacity * 8);
+    T->rstride = record_stride(P->KW + P->proto.nwords);
+    T->ctrl = nullptr;
+    if (P->KW > 1) {
+        T->ctrl = (unsigned long long*)dalloc(capacity * 8);
+        if (!T->ctrl || !zero_fill(T->ctrl, capacity * 8)) return false;
+    }
+    T->recs = (uint64_t*)dalloc((size_t)(capacity + 1) * T->rstride * 8);
+    if (!T->recs) return false;
+    return fill_records(P, T->recs, capacity + 1, T->rstride);
 }
 
-AggTable* table_new(AggPlan* P, int64_t groups_hint)
+// ---------------------------------------------------------------------------
+// The update kernel generated for one aggregate: the state-word layout as
+// constants and the row's contribution (selection, predicate, arguments,
+// key) as straight-line code over the shared helpers, so that the compiler
+// folds every descriptor switch; the tables and their protocol are the
+// shared agg_update_body.
+// ---------------------------------------------------------------------------
+
+std::string i64_lit(int64_t v)
+{
+    char b[40];
+    snprintf(b, sizeof(b), "(int64_t)0x%llxull", (unsigned long long)v);
+    return b;
+}
+
+std::string u64_lit(uint64_t v)
+{
+    char b[40];
+    snprintf(b, sizeof(b), "0x%llxull", (unsigned long long)v);
+    return b;
+}
+
+// Emits the program as one DVal per register and returns the name holding
+// its result.
+std::string emit_program(std::string& o, const std::string& name, const DevProgram& prog, const AggPlan* P)
+{
+    for (int q = 0; q < QDB_EXPR_MAX_REGS; ++q) o += "    DVal " + name + "_r" + std::to_string(q) + " = dval_zero();\n";
+    auto reg = [&](int q) { return name + "_r" + std::to_string(q >= 0 && q < QDB_EXPR_MAX_REGS ? q : 0); };
+    for (int k = 0; k < prog.n && k < QDB_EXPR_MAX_INSNS; ++k) {
+        const DevInsn& in = prog.insns[k];
+        if (in.dst < 0 || in.dst >= QDB_EXPR_MAX_REGS) continue;
+        if (in.op == QDB_EXPR_COLUMN) {
+            const QdbValueType& t = P->slot_type[in.column];
+            o += "    " + reg(in.dst) + " = read_col_t<" + std::to_string(t.Kind) + ", " + std::to_string(t.Width) + ">(C.c[" +
+                std::to_string(in.column) + "], row);\n";
+        } else {
+            char b[400];
+            snprintf(b, sizeof(b), "    %s = exec_insn<false>(DevInsn{%d, %d, %d, %d, %d, %d, %s, %s}, %s, %s, C, row);\n",
+                reg(in.dst).c_str(), in.op, in.dst, in.a, in.b, in.cls, in.column, i64_lit(in.lo).c_str(), i64_lit(in.hi).c_str(),
+                reg(in.a).c_str(), reg(in.b).c_str());
+            o += b;
+        }
+    }
+    return reg(prog.result);
+}
+
+std::string agg_jit_source(const AggPlan* P)
+{
+    const AggArgs& A = P->proto;
+    const int nw = A.nwords;
+    std::string o;
+    o += "#define QDB_SPECIALIZED 1\n#include \"aggregate_kernels.h\"\n";
+    o += "namespace qdbgpu {\nnamespace agg {\n";
+    o += "__device__ __forceinline__ DVal dval_zero()\n{\n    DVal v;\n    v.i = 0;\n    v.f = 0.0;\n    v.cls = 0;\n    v.valid = 0;\n    return v;\n}\n";
+    o += "struct JitPolicy {\n";
+    o += "    static constexpr int kNW = " + std::to_string(nw) + ";\n";
+    o += "    __device__ static __forceinline__ int nwords(const AggArgs&) { return kNW; }\n";
+    o += "    __device__ static __forceinline__ uint8_t op(const AggArgs&, int w)\n    {\n        constexpr uint8_t ops[] = {";
+    for (int w = 0; w < nw; ++w) o += std::to_string(A.words[w].op) + ", ";
+    o += "0};\n        return w < kNW ? ops[w] : 0;\n    }\n";
+    o += "    __device__ static __forceinline__ uint64_t init(const AggArgs&, int w)\n    {\n        constexpr uint64_t v[] = {";
+    for (int w = 0; w < nw; ++w) o += u64_lit(P->word_init_host[w]) + ", ";
+    o += "0};\n        return w < kNW ? v[w] : 0;\n    }\n";
+    o += std::string("    __device__ static __forceinline__ bool has_pairs(const AggArgs&) { return ") + (P->has_pairs ? "true" : "false") + "; }\n";
+    o += "    __device__ static __forceinline__ int match_rounds(const AggArgs& A) { return A.match_rounds; }\n";
+    o += "    template <int KW, int S>\n";
+    o += "    __device__ static __forceinline__ bool contrib(const AggArgs& A, int64_t row, uint64_t (&key)[KW], uint64_t (&c)[S])\n    {\n";
+    o += "        const DevCols& C = A.cols;\n";
+    o += "        if (A.sel && !A.sel[row]) return false;\n";
+    if (P->has_pred) {
+        o += "        {\n";
+        const std::string r = emit_program(o, "p", P->h_pred, P);
+        o += "    if (!(" + r + ".valid && " + r + ".i != 0)) return false;\n        }\n";
+    }
+    o += "#pragma unroll\n        for (int w = 0; w < S; ++w) c[w] = 0;\n";
+    for (int w = 0; w < nw; ++w) {
+        const WordDesc& d = A.words[w];
+        o += "        c[" + std::to_string(w) + "] = " + (d.kind == WK_ROWS ? std::string("1ull") : u64_lit(P->word_init_host[w])) + ";\n";
+    }
+    for (int j = 0; j < A.nsrc; ++j) {
+        const SrcDesc& sd = A.src[j];
+        const std::string v = "s" + std::to_string(j);
+        o += "        {\n";
+        if (sd.prog >= 0) {
+            const std::string r = emit_program(o, v + "p", P->h_progs[(size_t)sd.prog], P);
+            o += "        const DVal " + v + " = " + r + ";\n";
+        } else {
+            const QdbValueType& t = P->slot_type[sd.slot];
+            o += "        const DVal " + v + " = read_col_t<" + std::to_string(t.Kind) + ", " + std::to_string(t.Width) + ">(C.c[" +
+                std::to_string(sd.slot) + "], row);\n";
+        }
+        o += "        if (" + v + ".valid) {\n";
+        o += "            int64_t limb[kLimbs] = {};\n            bool outside = false;\n";
+        bool limbs = false;
+        for (int w = 0; w < nw; ++w) limbs |= A.words[w].src == j && (A.words[w].kind == WK_LIMB || A.words[w].kind == WK_OUTSIDE);
+        if (limbs) o += "            if (" + v + ".cls == QDB_REG_F64) f64_limbs(" + v + ".f, limb, &outside);\n";
+        for (int w = 0; w < nw; ++w) {
+            const WordDesc& d = A.words[w];
+            if (d.src != j || d.kind == WK_ROWS || d.kind == WK_LOCK) continue;
+            char b[200];
+            snprintf(b, sizeof(b), "            c[%d] = word_contrib(WordDesc{%d, %d, %d, %d}, %s, limb, outside);\n", w, d.kind, d.op, d.src,
+                d.part, v.c_str());
+            o += b;
+        }
+        o += "        }\n        }\n";
+    }
+    o += "#pragma unroll\n        for (int w = 0; w < KW; ++w) key[w] = 0;\n";
+    for (int f = 0; f < A.nkeys; ++f) {
+        const KeyFieldDesc& k = A.keys[f];
+        char b[300];
+        snprintf(b, sizeof(b), "        pack_key_field<KW>(KeyFieldDesc{%d, %d, %d, %d, %d, %d, %d, %d}, C.c[%d], row, key);\n", k.slot, k.kind,
+            k.width, k.nullable, k.word, k.shift, k.null_word, k.null_bit, k.slot);
+        o += b;
+    }
+    o += "        return true;\n    }\n};\n}  // namespace agg\n}  // namespace qdbgpu\n";
+    o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kBlock) + ") qdb_agg_update(qdbgpu::agg::AggArgs A)\n{\n";
+    o += "    qdbgpu::agg::agg_update_body<" + std::to_string(P->KW) + ", " + std::to_string(P->S) + ", qdbgpu::agg::JitPolicy>(A);\n}\n";
+    return o;
+}
+
+// The generated update kernel of the plan, compiled on first use; nullptr
+// when run-time kernels are off or the compile failed (the generic kernel
+// runs then).
+hipFunction_t plan_jit_update(AggPlan* P)
+{
+    if (!jit_enabled()) return nullptr;
+    if (!P->jit_tried) {
+        P->jit_tried = true;
+        if (P->jit_src.empty()) P->jit_src = agg_jit_source(P);
+        std::string err;
+        P->jit_fn = jit_function(P->jit_src, "qdb_agg_update", &err);
+    }
+    return P->jit_fn;
+}
+
+AggTable* table_new(AggPlan* P, int64_t slots_hint)
 {
     AggTable* T = new AggTable();
     T->plan = P;
     uint64_t cap = 1024;
-    while (cap < (uint64_t)groups_hint * 2) cap *= 2;
+    while (cap < (uint64_t)slots_hint) cap *= 2;
+    const int64_t groups_hint = slots_hint / 2;
     T->counters = (unsigned int*)dalloc(16);
     if (!T->counters || !zero_fill(T->counters, 16) || !table_alloc(T, cap)) {
         dfree(T->counters);
@@ -572,21 +831,36 @@ void table_free(AggTable* T)
     delete T;
 }
 
-// Doubles the HBM table until it holds `need` groups; re-inserts the groups.
+// Doubles the HBM table until `need` groups fill at most half of it; moves
+// the groups over.
 bool table_grow(AggTable* T, uint64_t need)
 {
     const AggPlan* P = T->plan;
-    const int R = P->KW + P->proto.nwords;
     uint64_t cap = T->capacity * 2;
     while (cap / 2 < need) cap *= 2;
     unsigned long long* old_ctrl = T->ctrl;
     uint64_t* old_recs = T->recs;
-    if (!table_alloc(T, cap)) return false;
-    if (T->groups) {
-        if (!hip_ok(hipMemcpyAsync(T->recs, old_recs, (size_t)T->groups * R * 8, hipMemcpyDeviceToDevice, stream()), "grow copy")) {
-            return false;
-        }
-        agg::launch_rehash(P->KW, T->ctrl, T->capacity - 1, T->recs, R, T->groups);
+    const uint64_t old_cap = T->capacity;
+    if (!table_alloc(T, cap)) {
+        dfree(T->ctrl);
+        dfree(T->recs);
+        T->ctrl = old_ctrl;
+        T->recs = old_recs;
+        T->capacity = old_cap;
+        return false;
+    }
+    AggArgs A = P->proto;
+    A.ctrl = T->ctrl;
+    A.cmask = T->capacity - 1;
+    A.recs = T->recs;
+    A.rstride = T->rstride;
+    agg::launch_rehash(P->KW, A, old_recs, old_ctrl, old_cap);
+    if (!hip_ok(hipGetLastError(), "rehash launch")) return false;
+    // The extra record keeps its contents at its new place.
+    const size_t rb = (size_t)T->rstride * 8;
+    if (!hip_ok(hipMemcpyAsync(T->recs + cap * (uint64_t)T->rstride, old_recs + old_cap * (uint64_t)T->rstride, rb,
+            hipMemcpyDeviceToDevice, stream()), "grow copy")) {
+        return false;
     }
     if (!sync("grow")) return false;
     dfree(old_ctrl);
@@ -599,8 +873,8 @@ void table_args(const AggTable* T, AggArgs* A)
     A->ctrl = T->ctrl;
     A->cmask = T->capacity - 1;
     A->recs = T->recs;
+    A->rstride = T->rstride;
     A->counters = T->counters;
-    A->max_groups = T->max_groups;
     A->match_rounds = T->match_rounds;
 }
 
@@ -608,20 +882,21 @@ bool read_counters(AggTable* T, unsigned int out[4])
 {
     if (!hip_ok(hipMemcpyAsync(out, T->counters, 16, hipMemcpyDeviceToHost, stream()), "counters")) return false;
     if (!sync("aggregate update")) return false;
-    T->groups = out[0] < T->max_groups ? out[0] : T->max_groups;
+    T->groups = out[0];
+    T->extra_used = out[3];
     return true;
 }
 
 bool set_counters(AggTable* T)
 {
-    unsigned int h[4] = {T->groups, 0u, 0u, 0u};
+    unsigned int h[4] = {T->groups, 0u, 0u, T->extra_used};
     return hip_ok(hipMemcpyAsync(T->counters, h, 16, hipMemcpyHostToDevice, stream()), "counters");
 }
 
 // agg_dispatch(Update): every selected row of the batch, exactly once.
 bool table_update(AggTable* T, const TRowSet* batch)
 {
-    const AggPlan* P = T->plan;
+    AggPlan* P = T->plan;
     if (batch->RowCount <= 0) return true;
     AggArgs A;
     if (!bind_batch(P, batch, &A)) return false;
@@ -661,11 +936,18 @@ bool table_update(AggTable* T, const TRowSet* batch)
         A.ov_recs = ov_recs[0];
         A.ov_rec_cap = (unsigned int)ov_recs_cap;
         if (!set_counters(T)) { release(); return false; }
+        hipError_t launched = hipSuccess;
         {
+            hipFunction_t fn = plan_jit_update(T->plan);
             TimedRegion timer("agg_update");
-            agg::launch_update(KW, S, A, grid, lds);
+            if (fn) {
+                void* args[] = {&A};
+                launched = hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, kBlock, 1, 1, (unsigned)lds, stream(), args, nullptr);
+            } else {
+                agg::launch_update(KW, S, A, grid, lds);
+            }
         }
-        if (!hip_ok(hipGetLastError(), "agg update launch")) { release(); return false; }
+        if (!hip_ok(launched, "agg update launch") || !hip_ok(hipGetLastError(), "agg update launch")) { release(); return false; }
         unsigned int cnt[4];
         if (!read_counters(T, cnt)) { release(); return false; }
         int cur = 0;
@@ -701,6 +983,8 @@ bool table_update(AggTable* T, const TRowSet* batch)
         }
         release();
     }
+    // Keep the load at most one half for the next batch.
+    if ((uint64_t)T->groups > T->capacity / 2 && !table_grow(T, (uint64_t)T->groups)) return false;
     return true;
 }
 
@@ -708,33 +992,84 @@ bool table_update(AggTable* T, const TRowSet* batch)
 int64_t table_finish(AggTable* T, TRowSet* out)
 {
     AggPlan* P = T->plan;
-    const int R = P->KW + P->proto.nwords;
+    FinArgs F = P->fin;
+    F.recs = T->recs;
+    F.stride = T->rstride;
+    F.index = nullptr;
+    uint32_t* index = nullptr;
+    uint32_t* counts = nullptr;
+    uint32_t* offsets = nullptr;
+    auto release = [&]() {
+        dfree(index);
+        dfree(counts);
+        dfree(offsets);
+    };
     if (P->spec.Global && T->groups == 0) {
-        // The one group of a global aggregate over an empty input.
-        std::vector<uint64_t> rec((size_t)R, 0ull);
-        for (int w = 0; w < P->proto.nwords; ++w) rec[(size_t)(P->KW + w)] = P->word_init_host[w];
-        if (!hip_ok(hipMemcpyAsync(T->recs, rec.data(), (size_t)R * 8, hipMemcpyHostToDevice, stream()), "global group")) return -1;
+        // The one group of a global aggregate over an empty input: slot 0,
+        // still at its identities (the key word of a keyless aggregate is 0).
+        F.index = nullptr;
         T->groups = 1;
+    } else if (T->groups > 0) {
+        // The slots that hold groups, in slot order.
+        const uint64_t slots = T->capacity + 1;
+        const int64_t chunks = (int64_t)((slots + kOccChunk - 1) / kOccChunk);
+        index = (uint32_t*)dalloc((size_t)T->groups * 4 + 16);
+        counts = (uint32_t*)dalloc((size_t)chunks * 4 + 16);
+        offsets = (uint32_t*)dalloc((size_t)chunks * 4 + 16);
+        if (!index || !counts || !offsets) {
+            release();
+            return -1;
+        }
+        const int extra = (int)T->extra_used;
+        TimedRegion timer("agg_finalize");
+        switch (P->KW) {
+        case 1: hipLaunchKernelGGL(k_occ_counts<1>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, counts); break;
+        case 2: hipLaunchKernelGGL(k_occ_counts<2>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, counts); break;
+        case 3: hipLaunchKernelGGL(k_occ_counts<3>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, counts); break;
+        default: hipLaunchKernelGGL(k_occ_counts<4>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, counts); break;
+        }
+        if (!hip_ok(hipGetLastError(), "occupancy launch") || !exclusive_scan_u32(counts, offsets, chunks)) {
+            release();
+            return -1;
+        }
+        switch (P->KW) {
+        case 1: hipLaunchKernelGGL(k_occ_index<1>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, offsets, index); break;
+        case 2: hipLaunchKernelGGL(k_occ_index<2>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, offsets, index); break;
+        case 3: hipLaunchKernelGGL(k_occ_index<3>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, offsets, index); break;
+        default: hipLaunchKernelGGL(k_occ_index<4>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, offsets, index); break;
+        }
+        if (!hip_ok(hipGetLastError(), "index launch")) {
+            release();
+            return -1;
+        }
+        F.index = index;
     }
     const unsigned int n = T->groups;
     OwnedRowSet* o = init_owned_rowset(out, n, P->nout);
-    FinArgs F = P->fin;
-    F.recs = T->recs;
     F.groups = n;
     const size_t mask_bytes = (size_t)((n + 7) / 8) + 8;
     for (int c = 0; c < P->nout; ++c) {
         const QdbValueType& t = P->out_types[c];
         const size_t bytes = t.Kind == QDB_KIND_BOOL ? mask_bytes : (size_t)n * (size_t)t.Width + 8;
         char* data = (char*)dalloc(bytes);
-        if (!data) return -1;
+        if (!data) {
+            release();
+            return -1;
+        }
         o->dev.push_back(data);
-        if (t.Kind == QDB_KIND_BOOL && !zero_fill(data, bytes)) return -1;
+        if (t.Kind == QDB_KIND_BOOL && !zero_fill(data, bytes)) {
+            release();
+            return -1;
+        }
         F.out_data[c] = data;
         o->cols[(size_t)c].Data = data;
         uint8_t* mask = nullptr;
         if (t.Nullable) {
             mask = (uint8_t*)dalloc(mask_bytes);
-            if (!mask) return -1;
+            if (!mask) {
+                release();
+                return -1;
+            }
             o->dev.push_back(mask);
         }
         F.out_mask[c] = mask;
@@ -744,9 +1079,14 @@ int64_t table_finish(AggTable* T, TRowSet* out)
         TimedRegion timer("agg_finalize");
         const int grid = grid_for((n + 7) / 8, kBlock, 8);
         hipLaunchKernelGGL(k_agg_finalize, dim3(grid), dim3(kBlock), 0, stream(), F);
-        if (!hip_ok(hipGetLastError(), "agg finalize launch")) return -1;
+        if (!hip_ok(hipGetLastError(), "agg finalize launch")) {
+            release();
+            return -1;
+        }
     }
-    if (!sync("agg finalize")) return -1;
+    const bool synced = sync("agg finalize");
+    release();
+    if (!synced) return -1;
     if (P->spec.OutputOnHost && !move_rowset_to_host(out, P->out_types)) return -1;
     return (int64_t)n;
 }
@@ -765,9 +1105,10 @@ int64_t agg_dispatch_impl(AggPlan* P, void* ht, TRowSet* batch, int64_t arg, int
         }
         if (B->table) table_free(B->table);
         memset(B, 0, sizeof(*B));
-        // Init's arg is swiss slots (load <= 7/8); InitRows' arg is groups.
-        const int64_t groups = op == QDB_AGG_OP_INIT ? arg - arg / 8 : arg;
-        AggTable* T = table_new(P, groups);
+        // Init's arg is the reference's swiss slot count; InitRows' arg is
+        // groups, which want twice as many slots here (load at most 1/2).
+        const int64_t slots = op == QDB_AGG_OP_INIT ? arg : 2 * arg;
+        AggTable* T = table_new(P, slots);
         if (!T) return 0;
         B->table = T;
         B->Capacity = (int64_t)T->capacity;
@@ -871,6 +1212,39 @@ extern "C" int qdb_gpu_compile_aggregate(const QdbAggregateSpec* spec, QdbAggreg
     return 0;
 }
 
+extern "C" int64_t qdb_gpu_aggregate_kernel_source(const QdbAggregateSpec* spec, char* out, int64_t cap)
+{
+    clear_error();
+    if (!spec) return fail("null argument"), -1;
+    AggPlan* P = new AggPlan();
+    if (!compile_plan(*spec, P, false)) {
+        delete P;
+        return -1;
+    }
+    const std::string src = agg_jit_source(P);
+    delete P;
+    if (out && cap > 0) {
+        const size_t n = src.size() < (size_t)cap - 1 ? src.size() : (size_t)cap - 1;
+        memcpy(out, src.data(), n);
+        out[n] = '\0';
+    }
+    return (int64_t)src.size();
+}
+
+extern "C" int qdb_gpu_jit_check(const char* src, const char* arch, char* err, int64_t cap)
+{
+    clear_error();
+    if (!src || !arch) return fail("null argument"), -1;
+    std::string e;
+    const bool ok = jit_compile_only(src, arch, &e);
+    if (err && cap > 0) {
+        const size_t n = e.size() < (size_t)cap - 1 ? e.size() : (size_t)cap - 1;
+        memcpy(err, e.data(), n);
+        err[n] = '\0';
+    }
+    return ok ? 0 : -1;
+}
+
 extern "C" void qdb_gpu_aggregate_destroy(void* ht)
 {
     HtBlob* B = (HtBlob*)ht;
@@ -897,10 +1271,10 @@ void launch_records_kw1(int S, const AggArgs& A, const uint64_t* recs, unsigned 
 void launch_records_kw2(int S, const AggArgs& A, const uint64_t* recs, unsigned int n);
 void launch_records_kw3(int S, const AggArgs& A, const uint64_t* recs, unsigned int n);
 void launch_records_kw4(int S, const AggArgs& A, const uint64_t* recs, unsigned int n);
-void launch_rehash_kw1(unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups);
-void launch_rehash_kw2(unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups);
-void launch_rehash_kw3(unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups);
-void launch_rehash_kw4(unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups);
+void launch_rehash_kw1(const AggArgs& A, const uint64_t* old_recs, const unsigned long long* old_ctrl, uint64_t old_slots);
+void launch_rehash_kw2(const AggArgs& A, const uint64_t* old_recs, const unsigned long long* old_ctrl, uint64_t old_slots);
+void launch_rehash_kw3(const AggArgs& A, const uint64_t* old_recs, const unsigned long long* old_ctrl, uint64_t old_slots);
+void launch_rehash_kw4(const AggArgs& A, const uint64_t* old_recs, const unsigned long long* old_ctrl, uint64_t old_slots);
 
 void launch_update(int KW, int S, const AggArgs& A, int grid, size_t lds)
 {
@@ -922,13 +1296,13 @@ void launch_records(int KW, int S, const AggArgs& A, const uint64_t* recs, unsig
     }
 }
 
-void launch_rehash(int KW, unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups)
+void launch_rehash(int KW, const AggArgs& A, const uint64_t* old_recs, const unsigned long long* old_ctrl, uint64_t old_slots)
 {
     switch (KW) {
-    case 1: launch_rehash_kw1(ctrl, cmask, recs, R, groups); break;
-    case 2: launch_rehash_kw2(ctrl, cmask, recs, R, groups); break;
-    case 3: launch_rehash_kw3(ctrl, cmask, recs, R, groups); break;
-    default: launch_rehash_kw4(ctrl, cmask, recs, R, groups); break;
+    case 1: launch_rehash_kw1(A, old_recs, old_ctrl, old_slots); break;
+    case 2: launch_rehash_kw2(A, old_recs, old_ctrl, old_slots); break;
+    case 3: launch_rehash_kw3(A, old_recs, old_ctrl, old_slots); break;
+    default: launch_rehash_kw4(A, old_recs, old_ctrl, old_slots); break;
     }
 }
 

@@ -2,10 +2,14 @@
 // translation units (aggregate_kw*.hip) and the host code (aggregate.hip).
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 
 #include "device.h"
 #include "runtime.h"
+#else
+#include "device.h"
+#endif
 
 namespace qdbgpu {
 namespace agg {
@@ -89,12 +93,16 @@ struct AggArgs {
     int lds_groups;            // max groups in LDS
     int match_rounds;
     int has_pairs;             // i128 MIN/MAX words present
-    // HBM table
-    unsigned long long* ctrl;
+    // HBM table: one record per slot, `rstride` words apart (KW key words,
+    // then the state words, set to their identities when the table is
+    // made). One-word keys are published by a 64-bit CAS on the key word
+    // (kEmptyKey = empty slot; the key equal to kEmptyKey has the extra
+    // record at slot cmask + 1); wider keys through a control word per slot.
+    unsigned long long* ctrl;  // KW > 1: 0 empty, 1 being filled, tag | 2 ready
     uint64_t cmask;
-    uint64_t* recs;            // dense records: KW key words, then words
-    unsigned int* counters;    // [0] groups, [1] overflow rows, [2] overflow records
-    unsigned int max_groups;
+    uint64_t* recs;
+    int rstride;
+    unsigned int* counters;    // [0] groups, [1] overflow rows, [2] overflow records, [3] extra record used
     // overflow
     uint32_t* ov_rows;
     uint64_t* ov_recs;
@@ -164,52 +172,124 @@ __device__ __forceinline__ bool keys_equal(const uint64_t (&a)[KW], const uint64
     return eq;
 }
 
+// ORs key field `kf` of a row into the packed key.
+template <int KW>
+__device__ __forceinline__ void pack_key_field(const KeyFieldDesc& kf, const DevCol& c, int64_t row, uint64_t (&key)[KW])
+{
+    const bool valid = col_valid(c, row);
+    if (kf.nullable && !valid) {
+#pragma unroll
+        for (int w = 0; w < KW; ++w) {
+            if (w == kf.null_word) key[w] |= 1ull << kf.null_bit;
+        }
+        return;
+    }
+    uint64_t lo, hi = 0;
+    if (kf.kind == QDB_KIND_FLOAT) {
+        lo = kf.width == 4 ? (uint64_t)__float_as_uint(((const float*)c.data)[row]) : canonical_f64_bits(((const double*)c.data)[row]);
+        if (kf.width == 4) {
+            uint32_t b = (uint32_t)lo;
+            if ((b & 0x7fffffffu) == 0) b = 0;
+            else if ((b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu) != 0) b = 0x7fc00000u;
+            lo = b;
+        }
+    } else if (kf.kind == QDB_KIND_BOOL) {
+        lo = (uint64_t)bit_at((const uint8_t*)c.data, row, c.bit_off);
+    } else if (kf.width == 16) {
+        const uint64_t* p = (const uint64_t*)(c.data + 16 * row);
+        lo = p[0];
+        hi = p[1];
+    } else {
+        switch (kf.width) {
+        case 1: lo = ((const uint8_t*)c.data)[row]; break;
+        case 2: lo = ((const uint16_t*)c.data)[row]; break;
+        case 4: lo = ((const uint32_t*)c.data)[row]; break;
+        default: lo = ((const uint64_t*)c.data)[row]; break;
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < KW; ++w) {
+        if (w == kf.word) key[w] |= lo << kf.shift;
+        if (kf.width == 16 && w == kf.word + 1) key[w] |= hi;
+    }
+}
+
 // The packed key of a row.
 template <int KW>
 __device__ __forceinline__ void row_key(const AggArgs& A, int64_t row, uint64_t (&key)[KW])
 {
 #pragma unroll
     for (int w = 0; w < KW; ++w) key[w] = 0;
-    for (int f = 0; f < A.nkeys; ++f) {
+    QDB_DESC_LOOP
+    for (int f = 0; f < QDB_AGG_MAX_KEYS; ++f) {
+        if (f >= A.nkeys) break;
         const KeyFieldDesc kf = A.keys[f];
-        const DevCol& c = A.cols.c[kf.slot];
-        const bool valid = col_valid(c, row);
-        if (kf.nullable && !valid) {
-#pragma unroll
-            for (int w = 0; w < KW; ++w) {
-                if (w == kf.null_word) key[w] |= 1ull << kf.null_bit;
-            }
-            continue;
-        }
-        uint64_t lo, hi = 0;
-        if (kf.kind == QDB_KIND_FLOAT) {
-            lo = c.width == 4 ? (uint64_t)__float_as_uint(((const float*)c.data)[row]) : canonical_f64_bits(((const double*)c.data)[row]);
-            if (c.width == 4) {
-                uint32_t b = (uint32_t)lo;
-                if ((b & 0x7fffffffu) == 0) b = 0;
-                else if ((b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu) != 0) b = 0x7fc00000u;
-                lo = b;
-            }
-        } else if (kf.kind == QDB_KIND_BOOL) {
-            lo = (uint64_t)bit_at((const uint8_t*)c.data, row, c.bit_off);
-        } else if (kf.width == 16) {
-            const uint64_t* p = (const uint64_t*)(c.data + 16 * row);
-            lo = p[0];
-            hi = p[1];
-        } else {
-            switch (kf.width) {
-            case 1: lo = ((const uint8_t*)c.data)[row]; break;
-            case 2: lo = ((const uint16_t*)c.data)[row]; break;
-            case 4: lo = ((const uint32_t*)c.data)[row]; break;
-            default: lo = ((const uint64_t*)c.data)[row]; break;
-            }
-        }
-#pragma unroll
-        for (int w = 0; w < KW; ++w) {
-            if (w == kf.word) key[w] |= lo << kf.shift;
-            if (kf.width == 16 && w == kf.word + 1) key[w] |= hi;
-        }
+        pack_key_field<KW>(kf, A.cols.c[kf.slot], row, key);
     }
+}
+
+// The exact-summation limbs of an f64 value (all 0 and *outside set when it
+// falls outside the limb window).
+__device__ __forceinline__ void f64_limbs(double f, int64_t (&limb)[kLimbs], bool* outside)
+{
+#pragma unroll
+    for (int k = 0; k < kLimbs; ++k) limb[k] = f64_limb(f, k, outside);
+    if (*outside) {
+#pragma unroll
+        for (int k = 0; k < kLimbs; ++k) limb[k] = 0;
+    }
+}
+
+// The contribution of a valid argument value to state word `d`.
+__device__ __forceinline__ uint64_t word_contrib(const WordDesc& d, const DVal& v, const int64_t (&limb)[kLimbs], bool outside)
+{
+    const u128 u = (u128)v.i;
+    switch (d.kind) {
+    case WK_VALID: return 1;
+    case WK_SUM64:
+    case WK_MIN_I64:
+    case WK_MAX_I64: return (uint64_t)u;
+    case WK_I128_P0: return (uint64_t)u & 0xffffffffull;
+    case WK_I128_P1: return (uint64_t)u >> 32;
+    case WK_I128_P2: return (uint64_t)(u >> 64);
+    case WK_LIMB: {
+        int64_t l = limb[0];
+#pragma unroll
+        for (int k = 1; k < kLimbs; ++k) {
+            if (d.part == k) l = limb[k];
+        }
+        return (uint64_t)l;
+    }
+    case WK_OUTSIDE: return outside ? (uint64_t)__double_as_longlong(v.f) : 0ull;
+    case WK_MIN_F64: return f64_min_key(v.f);
+    case WK_MAX_F64: return f64_max_key(v.f);
+    case WK_PAIR_LO: return (uint64_t)u;
+    case WK_PAIR_HI: return (uint64_t)(u >> 64);
+    default: return 0;
+    }
+}
+
+// A column value whose kind and width are constants (kernels specialized at
+// run time). The load is unconditional: a NULL row's slot is read and not
+// used, so no wait sits inside a branch.
+template <int kKind, int kWidth>
+__device__ __forceinline__ DVal read_col_t(const DevCol& c, int64_t row)
+{
+    DevCol k = c;
+    k.kind = kKind;
+    k.width = kWidth;
+    DVal v;
+    v.valid = col_valid(k, row);
+    v.i = 0;
+    v.f = 0.0;
+    if (kKind == QDB_KIND_FLOAT) {
+        v.cls = QDB_REG_F64;
+        v.f = col_f64(k, row);
+    } else {
+        v.cls = kKind == QDB_KIND_BOOL ? QDB_REG_BOOL : QDB_REG_INT;
+        v.i = col_int(k, row);
+    }
+    return v;
 }
 
 // A row's contribution to every state word; false when the row does not
@@ -230,8 +310,9 @@ __device__ __forceinline__ bool row_contrib(const AggArgs& A, int64_t row, uint6
         c[w] = 0;
         if (w < A.nwords) c[w] = A.words[w].kind == WK_ROWS ? 1ull : word_init(A.words[w]);
     }
-#pragma unroll 1
-    for (int j = 0; j < A.nsrc; ++j) {
+    QDB_DESC_LOOP
+    for (int j = 0; j < kMaxSrc; ++j) {
+        if (j >= A.nsrc) break;
         const SrcDesc sd = A.src[j];
         const DVal v = sd.prog >= 0 ? eval_program<true>(A.progs + sd.prog, A.cols, row) : read_col(A.cols.c[sd.slot], row);
         if (!v.valid) continue;  // the words keep their identities
@@ -239,49 +320,36 @@ __device__ __forceinline__ bool row_contrib(const AggArgs& A, int64_t row, uint6
         bool outside = false;
 #pragma unroll
         for (int k = 0; k < kLimbs; ++k) limb[k] = 0;
-        if (v.cls == QDB_REG_F64) {
-#pragma unroll
-            for (int k = 0; k < kLimbs; ++k) limb[k] = f64_limb(v.f, k, &outside);
-            if (outside) {
-#pragma unroll
-                for (int k = 0; k < kLimbs; ++k) limb[k] = 0;
-            }
-        }
-        const u128 u = (u128)v.i;
+        if (v.cls == QDB_REG_F64) f64_limbs(v.f, limb, &outside);
 #pragma unroll
         for (int w = 0; w < S; ++w) {
             if (w >= A.nwords) continue;
             const WordDesc d = A.words[w];
             if (d.src != j) continue;
-            switch (d.kind) {
-            case WK_VALID: c[w] = 1; break;
-            case WK_SUM64:
-            case WK_MIN_I64:
-            case WK_MAX_I64: c[w] = (uint64_t)u; break;
-            case WK_I128_P0: c[w] = (uint64_t)u & 0xffffffffull; break;
-            case WK_I128_P1: c[w] = (uint64_t)u >> 32; break;
-            case WK_I128_P2: c[w] = (uint64_t)(u >> 64); break;
-            case WK_LIMB: {
-                int64_t l = limb[0];
-#pragma unroll
-                for (int k = 1; k < kLimbs; ++k) {
-                    if (d.part == k) l = limb[k];
-                }
-                c[w] = (uint64_t)l;
-                break;
-            }
-            case WK_OUTSIDE: c[w] = outside ? (uint64_t)__double_as_longlong(v.f) : 0ull; break;
-            case WK_MIN_F64: c[w] = f64_min_key(v.f); break;
-            case WK_MAX_F64: c[w] = f64_max_key(v.f); break;
-            case WK_PAIR_LO: c[w] = (uint64_t)u; break;
-            case WK_PAIR_HI: c[w] = (uint64_t)(u >> 64); break;
-            default: break;
-            }
+            if (d.kind == WK_ROWS || d.kind == WK_LOCK) continue;
+            c[w] = word_contrib(d, v, limb, outside);
         }
     }
     row_key<KW>(A, row, key);
     return true;
 }
+
+// The update path reads the state-word layout and the row's contribution
+// through a policy: GenericPolicy reads the descriptors in the launch
+// arguments; a policy generated for one aggregate (at run time) gives them
+// as constants and computes the contribution with straight-line code.
+struct GenericPolicy {
+    __device__ static __forceinline__ int nwords(const AggArgs& A) { return A.nwords; }
+    __device__ static __forceinline__ uint8_t op(const AggArgs& A, int w) { return (uint8_t)A.words[w].op; }
+    __device__ static __forceinline__ uint64_t init(const AggArgs& A, int w) { return word_init(A.words[w]); }
+    __device__ static __forceinline__ bool has_pairs(const AggArgs& A) { return A.has_pairs != 0; }
+    __device__ static __forceinline__ int match_rounds(const AggArgs& A) { return A.match_rounds; }
+    template <int KW, int S>
+    __device__ static __forceinline__ bool contrib(const AggArgs& A, int64_t row, uint64_t (&key)[KW], uint64_t (&c)[S])
+    {
+        return row_contrib<KW, S>(A, row, key, c);
+    }
+};
 
 // ---------------------------------------------------------------------------
 // HBM table
@@ -296,47 +364,77 @@ __device__ __forceinline__ bool row_contrib(const AggArgs& A, int64_t row, uint6
 constexpr int64_t kNoRoom = -1;
 constexpr int64_t kBusy = -2;
 
-// Finds or inserts `key` in the HBM table: the group id, kNoRoom when the
-// table is full, or kBusy. A new record's words are set to their identities
-// before the slot is published, so every update after the lookup is an
-// atomic combine.
+// The empty marker of a one-word key slot.
+constexpr unsigned long long kEmptyKey = 0x8000000000000000ull;
+// Slots a lookup walks before it gives up (the row or record is then kept
+// for the replay after the table has grown).
+constexpr int kMaxProbe = 256;
+
+struct LdsTable {
+    unsigned int* ctrl;
+    unsigned int* count;
+    uint64_t* recs;
+    unsigned int* inserted;   // HBM groups this workgroup inserted
+};
+
+// Finds or inserts `key` in the HBM table: the slot of its record, kNoRoom
+// when kMaxProbe slots hold other keys, or kBusy. Every state word starts at
+// its identity, so an inserted key needs no further publication: every
+// update is an atomic combine. One-word keys: the key word itself is the
+// CAS target, and keys never change once set, so relaxed loads suffice (a
+// stale empty marker only sends the lane to the CAS, which sees the truth).
 template <int KW, int S>
-__device__ int64_t global_find_or_insert(const AggArgs& A, const uint64_t (&key)[KW])
+__device__ __forceinline__ int64_t global_find_or_insert(const AggArgs& A, LdsTable& T, const uint64_t (&key)[KW])
 {
     const uint64_t h = key_place_hash<KW>(key);
-    const unsigned long long tag = (unsigned long long)(h >> 32) << 32;
     uint64_t slot = h & A.cmask;
-    const int R = KW + A.nwords;
-    for (;;) {
-        unsigned long long c = __hip_atomic_load(&A.ctrl[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    if constexpr (KW == 1) {
+        if (key[0] == kEmptyKey) {
+            if (atomicCAS(&A.counters[3], 0u, 1u) == 0u) atomicAdd(T.inserted, 1u);
+            return (int64_t)(A.cmask + 1);
+        }
+        for (int p = 0; p < kMaxProbe; ++p) {
+            unsigned long long* kp = (unsigned long long*)(A.recs + slot * (uint64_t)A.rstride);
+            unsigned long long c = __hip_atomic_load(kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (c == kEmptyKey) {
+                c = atomicCAS(kp, kEmptyKey, (unsigned long long)key[0]);
+                if (c == kEmptyKey) {
+                    atomicAdd(T.inserted, 1u);
+                    return (int64_t)slot;
+                }
+            }
+            if (c == (unsigned long long)key[0]) return (int64_t)slot;
+            slot = (slot + 1) & A.cmask;
+        }
+        return kNoRoom;
+    } else {
+    const unsigned long long tag = (unsigned long long)(h >> 32) << 32;
+    for (int p = 0; p < kMaxProbe; ++p) {
+        unsigned long long c = __hip_atomic_load(&A.ctrl[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t* r = A.recs + slot * (uint64_t)A.rstride;
         if (c == 0ull) {
             c = atomicCAS(&A.ctrl[slot], 0ull, 1ull);
             if (c == 0ull) {
-                unsigned int gid = atomicAdd(&A.counters[0], 1u);
-                if (gid >= A.max_groups) {
-                    __hip_atomic_store(&A.ctrl[slot], 0ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                    return kNoRoom;
-                }
-                uint64_t* r = A.recs + (size_t)gid * R;
 #pragma unroll
                 for (int w = 0; w < KW; ++w) __hip_atomic_store(&r[w], key[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                for (int w = 0; w < A.nwords; ++w) {
-                    __hip_atomic_store(&r[KW + w], word_init(A.words[w]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                __hip_atomic_store(&A.ctrl[slot], tag | (unsigned long long)(gid + 2), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                return gid;
+                __hip_atomic_store(&A.ctrl[slot], tag | 2ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                atomicAdd(T.inserted, 1u);
+                return (int64_t)slot;
             }
         }
         if (c == 1ull) return kBusy;  // being filled
         if ((c & 0xffffffff00000000ull) == tag) {
-            const unsigned int gid = (unsigned int)(c & 0xffffffffull) - 2u;
-            const uint64_t* r = A.recs + (size_t)gid * R;
+            // The key words were written before the control word: read them
+            // after an acquire of it (only on a tag match).
+            (void)__hip_atomic_load(&A.ctrl[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
             uint64_t k2[KW];
 #pragma unroll
             for (int w = 0; w < KW; ++w) k2[w] = __hip_atomic_load(&r[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (keys_equal<KW>(k2, key)) return gid;
+            if (keys_equal<KW>(k2, key)) return (int64_t)slot;
         }
         slot = (slot + 1) & A.cmask;
+    }
+    return kNoRoom;
     }
 }
 
@@ -363,21 +461,15 @@ __device__ __forceinline__ bool pair_try(uint64_t* lo, bool is_max, uint64_t vlo
 // a group counter, then dense records of KW + nwords words.
 // ---------------------------------------------------------------------------
 
-struct LdsTable {
-    unsigned int* ctrl;
-    unsigned int* count;
-    uint64_t* recs;
-};
-
 // The LDS group of `key`, kNoRoom when it is absent and the table is full,
 // or kBusy.
-template <int KW, int S>
-__device__ int lds_find_or_insert(const AggArgs& A, LdsTable& T, const uint64_t (&key)[KW])
+template <int KW, int S, class P>
+__device__ __forceinline__ int lds_find_or_insert(const AggArgs& A, LdsTable& T, const uint64_t (&key)[KW])
 {
     const uint64_t h = key_place_hash<KW>(key);
     const unsigned int mask = (unsigned int)A.lds_slots - 1u;
     unsigned int slot = (unsigned int)(h >> 40) & mask;
-    const int R = KW + A.nwords;
+    const int R = KW + P::nwords(A);
     for (unsigned int probes = 0; probes <= mask; ++probes) {
         unsigned int c = __hip_atomic_load(&T.ctrl[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (c == 0u) {
@@ -394,7 +486,7 @@ __device__ int lds_find_or_insert(const AggArgs& A, LdsTable& T, const uint64_t 
                 uint64_t* r = T.recs + (size_t)lg * R;
 #pragma unroll
                 for (int w = 0; w < KW; ++w) r[w] = key[w];
-                for (int w = 0; w < A.nwords; ++w) r[KW + w] = word_init(A.words[w]);
+                for (int w = 0; w < P::nwords(A); ++w) r[KW + w] = P::init(A, w);
                 __hip_atomic_store(&T.ctrl[slot], lg + 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                 return (int)lg;
             }
@@ -414,19 +506,19 @@ __device__ int lds_find_or_insert(const AggArgs& A, LdsTable& T, const uint64_t 
 enum Where : int { AT_NONE = 0, AT_LDS = 1, AT_HBM = 2, AT_BUSY = 3 };
 
 // One lookup attempt: the workgroup's LDS table first, then the HBM table.
-template <int KW, int S>
+template <int KW, int S, class P>
 __device__ __forceinline__ int locate_once(const AggArgs& A, LdsTable& T, bool use_lds, const uint64_t (&key)[KW],
     int64_t* id)
 {
     if (use_lds) {
-        const int lg = lds_find_or_insert<KW, S>(A, T, key);
+        const int lg = lds_find_or_insert<KW, S, P>(A, T, key);
         if (lg >= 0) {
             *id = lg;
             return AT_LDS;
         }
         if (lg == (int)kBusy) return AT_BUSY;
     }
-    const int64_t gid = global_find_or_insert<KW, S>(A, key);
+    const int64_t gid = global_find_or_insert<KW, S>(A, T, key);
     if (gid >= 0) {
         *id = gid;
         return AT_HBM;
@@ -436,46 +528,68 @@ __device__ __forceinline__ int locate_once(const AggArgs& A, LdsTable& T, bool u
 
 // Wave-uniform: every lane of the wave calls it, lanes with `want` false
 // only take part. Returns where the key's group is (AT_NONE: no room).
-template <int KW, int S>
+template <int KW, int S, class P>
 __device__ __forceinline__ int locate(const AggArgs& A, LdsTable& T, bool use_lds, bool want, const uint64_t (&key)[KW],
     int64_t* id)
 {
     int where = want ? AT_BUSY : AT_NONE;
+    if constexpr (KW == 1) {
+        // One-word keys never see a slot being filled in HBM; the LDS table
+        // may still report one.
+        if (!use_lds) {
+            if (want) where = locate_once<KW, S, P>(A, T, false, key, id);
+            return where;
+        }
+    }
     while (__any(where == AT_BUSY)) {
-        if (where == AT_BUSY) where = locate_once<KW, S>(A, T, use_lds, key, id);
+        if (where == AT_BUSY) where = locate_once<KW, S, P>(A, T, use_lds, key, id);
     }
     return where;
 }
 
 // Combines a contribution into the group's words, except the i128 MIN/MAX
 // pairs (apply_pairs).
-template <int KW, int S>
+template <int KW, int S, class P>
 __device__ __forceinline__ void apply_words(const AggArgs& A, LdsTable& T, int where, int64_t id, const uint64_t (&c)[S])
 {
-    uint64_t* r = where == AT_LDS ? T.recs + (size_t)id * (KW + A.nwords) + KW : A.recs + (size_t)id * (KW + A.nwords) + KW;
+    // Separate paths for the two tables: LDS atomics for the workgroup's
+    // table, global atomics for the HBM record (a pointer that could be
+    // either would make every access a flat one).
+    if (where == AT_LDS) {
+        uint64_t* r = T.recs + (size_t)id * (KW + P::nwords(A)) + KW;
 #pragma unroll
-    for (int w = 0; w < S; ++w) {
-        if (w >= A.nwords) continue;
-        const int op = A.words[w].op;
-        if (op == OP_PAIR_MIN || op == OP_PAIR_MAX || op == OP_PAIR_HI || op == OP_LOCK) continue;
-        atomic_apply_lds((uint8_t)op, &r[w], c[w]);
+        for (int w = 0; w < S; ++w) {
+            if (w >= P::nwords(A)) continue;
+            const int op = P::op(A, w);
+            if (op == OP_PAIR_MIN || op == OP_PAIR_MAX || op == OP_PAIR_HI || op == OP_LOCK) continue;
+            atomic_apply_lds((uint8_t)op, &r[w], c[w]);
+        }
+    } else {
+        uint64_t* r = A.recs + (uint64_t)id * (uint64_t)A.rstride + KW;
+#pragma unroll
+        for (int w = 0; w < S; ++w) {
+            if (w >= P::nwords(A)) continue;
+            const int op = P::op(A, w);
+            if (op == OP_PAIR_MIN || op == OP_PAIR_MAX || op == OP_PAIR_HI || op == OP_LOCK) continue;
+            atomic_apply_global((uint8_t)op, &r[w], c[w]);
+        }
     }
 }
 
 // Wave-uniform: the i128 MIN/MAX pairs of HBM group `gid` for the lanes
 // with `want`, each under the group's lock (pairs never live in LDS).
-template <int KW, int S>
+template <int KW, int S, class P>
 __device__ __forceinline__ void apply_pairs(const AggArgs& A, bool want, int64_t gid, const uint64_t (&c)[S])
 {
 #pragma unroll
     for (int w = 0; w + 1 < S; ++w) {
-        if (w >= A.nwords) continue;
-        const int op = A.words[w].op;
+        if (w >= P::nwords(A)) continue;
+        const int op = P::op(A, w);
         if (op != OP_PAIR_MIN && op != OP_PAIR_MAX) continue;
         bool pending = want;
         while (__any(pending)) {
             if (pending) {
-                uint64_t* r = A.recs + (size_t)gid * (KW + A.nwords) + KW;
+                uint64_t* r = A.recs + (size_t)gid * (uint64_t)A.rstride + KW;
                 if (pair_try(&r[w], op == OP_PAIR_MAX, c[w], c[w + 1])) pending = false;
             }
         }
@@ -491,32 +605,32 @@ __device__ __forceinline__ void spill_row(const AggArgs& A, int64_t row)
 
 // Wave-uniform: applies a contribution (the lanes with `want`) to the LDS
 // table or the HBM table; false for a lane whose group found no room.
-template <int KW, int S>
+template <int KW, int S, class P>
 __device__ __forceinline__ bool apply_contrib(const AggArgs& A, LdsTable& T, bool use_lds, bool want,
     const uint64_t (&key)[KW], const uint64_t (&c)[S])
 {
     int64_t id = 0;
-    const int where = locate<KW, S>(A, T, use_lds, want, key, &id);
-    if (want && where != AT_NONE) apply_words<KW, S>(A, T, where, id, c);
-    if (A.has_pairs) apply_pairs<KW, S>(A, want && where == AT_HBM, id, c);
+    const int where = locate<KW, S, P>(A, T, use_lds, want, key, &id);
+    if (want && where != AT_NONE) apply_words<KW, S, P>(A, T, where, id, c);
+    if (P::has_pairs(A)) apply_pairs<KW, S, P>(A, want && where == AT_HBM, id, c);
     return !want || where != AT_NONE;
 }
 
 // Wave-uniform: applies group records (key + state words) to the HBM table,
 // or keeps them for the replay after the table has grown.
-template <int KW, int S>
-__device__ void global_upsert_or_spill(const AggArgs& A, LdsTable& T, bool want, const uint64_t (&key)[KW],
+template <int KW, int S, class P>
+__device__ __forceinline__ void global_upsert_or_spill(const AggArgs& A, LdsTable& T, bool want, const uint64_t (&key)[KW],
     const uint64_t (&c)[S])
 {
-    if (apply_contrib<KW, S>(A, T, false, want, key, c)) return;
+    if (apply_contrib<KW, S, P>(A, T, false, want, key, c)) return;
     unsigned int pos = atomicAdd(&A.counters[2], 1u);
     if (pos < A.ov_rec_cap) {
-        uint64_t* r = A.ov_recs + (size_t)pos * (KW + A.nwords);
+        uint64_t* r = A.ov_recs + (size_t)pos * (KW + P::nwords(A));
 #pragma unroll
         for (int w = 0; w < KW; ++w) r[w] = key[w];
 #pragma unroll
         for (int w = 0; w < S; ++w) {
-            if (w < A.nwords) r[KW + w] = c[w];
+            if (w < P::nwords(A)) r[KW + w] = c[w];
         }
     }
 }
@@ -533,23 +647,31 @@ __device__ __forceinline__ uint64_t wave_reduce(uint8_t op, uint64_t v, bool in)
     return x;
 }
 
-template <int KW, int S>
-__global__ void __launch_bounds__(kBlock) k_agg_update(AggArgs A)
+// The update of one launch: every selected row of [row_base, row_base +
+// nrows) (or of row_index), pre-aggregated in the workgroup's LDS table and
+// then in the HBM table. The generic kernel below reads the descriptors
+// from its arguments; kernels specialized at run time call it with the
+// descriptors set to constants.
+template <int KW, int S, class P>
+__device__ __forceinline__ void agg_update_body(const AggArgs& A)
 {
     extern __shared__ uint64_t lds_raw[];
+    __shared__ unsigned int s_inserted;
     LdsTable T;
     T.recs = nullptr;
     T.ctrl = nullptr;
     T.count = nullptr;
+    T.inserted = &s_inserted;
+    if (threadIdx.x == 0) s_inserted = 0u;
     const bool use_lds = A.lds_slots > 0 && A.row_index == nullptr;
     if (use_lds) {
         T.recs = lds_raw;
-        T.ctrl = (unsigned int*)(lds_raw + (size_t)A.lds_groups * (KW + A.nwords));
+        T.ctrl = (unsigned int*)(lds_raw + (size_t)A.lds_groups * (KW + P::nwords(A)));
         T.count = T.ctrl + A.lds_slots;
         for (int i = threadIdx.x; i < A.lds_slots; i += blockDim.x) T.ctrl[i] = 0u;
         if (threadIdx.x == 0) *T.count = 0u;
-        __syncthreads();
     }
+    __syncthreads();
     const int lane = lane_id();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t start = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -563,7 +685,7 @@ __global__ void __launch_bounds__(kBlock) k_agg_update(AggArgs A)
         if (valid) row = A.row_index ? (int64_t)A.row_index[idx] : A.row_base + idx;
         uint64_t key[KW];
         uint64_t c[S];
-        if (valid) valid = row_contrib<KW, S>(A, row, key, c);
+        if (valid) valid = P::template contrib<KW, S>(A, row, key, c);
         if (!valid) {
 #pragma unroll
             for (int w = 0; w < KW; ++w) key[w] = 0;
@@ -573,7 +695,7 @@ __global__ void __launch_bounds__(kBlock) k_agg_update(AggArgs A)
         // Pre-aggregation: the lanes that share the first remaining lane's
         // key are combined and applied once by that lane.
         uint64_t active = __ballot(valid);
-        for (int round = 0; round < A.match_rounds && active; ++round) {
+        for (int round = 0; round < P::match_rounds(A) && active; ++round) {
             const int leader = __ffsll((unsigned long long)active) - 1;
             uint64_t lk[KW];
 #pragma unroll
@@ -584,50 +706,65 @@ __global__ void __launch_bounds__(kBlock) k_agg_update(AggArgs A)
             if (__popcll(m) > 1) {
 #pragma unroll
                 for (int w = 0; w < S; ++w) {
-                    red[w] = w < A.nwords ? wave_reduce((uint8_t)A.words[w].op, c[w], same) : 0;
+                    red[w] = w < P::nwords(A) ? wave_reduce(P::op(A, w), c[w], same) : 0;
                 }
             } else {
 #pragma unroll
                 for (int w = 0; w < S; ++w) red[w] = c[w];
             }
-            const bool ok = apply_contrib<KW, S>(A, T, use_lds, lane == leader, lk, red);
+            const bool ok = apply_contrib<KW, S, P>(A, T, use_lds, lane == leader, lk, red);
             // The combined contribution found no room: every row of the
             // match is replayed on its own.
             if (!__shfl((int)ok, leader) && same) spill_row(A, row);
             active &= ~m;
             if (same) valid = false;
         }
-        if (!apply_contrib<KW, S>(A, T, use_lds, valid, key, c)) spill_row(A, row);
+        if (!apply_contrib<KW, S, P>(A, T, use_lds, valid, key, c)) spill_row(A, row);
     }
-    if (!use_lds) return;
     __syncthreads();
-    // Flush the workgroup's groups to the HBM table; every thread runs the
-    // same number of iterations.
-    const unsigned int n = *T.count < (unsigned int)A.lds_groups ? *T.count : (unsigned int)A.lds_groups;
-    const int R = KW + A.nwords;
-    for (unsigned int b = 0; b < n; b += blockDim.x) {
-        const unsigned int lg = b + threadIdx.x;
-        const bool want = lg < n;
-        uint64_t key[KW];
-        uint64_t c[S];
-        const uint64_t* r = T.recs + (size_t)(want ? lg : 0u) * R;
+    if (use_lds) {
+        // Flush the workgroup's groups to the HBM table; every thread runs
+        // the same number of iterations.
+        const unsigned int n = *T.count < (unsigned int)A.lds_groups ? *T.count : (unsigned int)A.lds_groups;
+        const int R = KW + P::nwords(A);
+        for (unsigned int b = 0; b < n; b += blockDim.x) {
+            const unsigned int lg = b + threadIdx.x;
+            const bool want = lg < n;
+            uint64_t key[KW];
+            uint64_t c[S];
+            const uint64_t* r = T.recs + (size_t)(want ? lg : 0u) * R;
 #pragma unroll
-        for (int w = 0; w < KW; ++w) key[w] = r[w];
+            for (int w = 0; w < KW; ++w) key[w] = r[w];
 #pragma unroll
-        for (int w = 0; w < S; ++w) c[w] = w < A.nwords ? r[KW + w] : 0;
-        global_upsert_or_spill<KW, S>(A, T, want, key, c);
+            for (int w = 0; w < S; ++w) c[w] = w < P::nwords(A) ? r[KW + w] : 0;
+            global_upsert_or_spill<KW, S, P>(A, T, want, key, c);
+        }
+        __syncthreads();
     }
+    // One atomic per workgroup for the groups it inserted.
+    if (threadIdx.x == 0 && s_inserted) atomicAdd(&A.counters[0], s_inserted);
+}
+
+template <int KW, int S>
+__global__ void __launch_bounds__(kBlock) k_agg_update(AggArgs A)
+{
+    agg_update_body<KW, S, GenericPolicy>(A);
 }
 
 // Replays spilled records (key + words) into the grown HBM table.
 template <int KW, int S>
 __global__ void __launch_bounds__(kBlock) k_agg_apply_records(AggArgs A, const uint64_t* recs, unsigned int n)
 {
+    using P = GenericPolicy;
+    __shared__ unsigned int s_inserted;
     LdsTable T;
     T.recs = nullptr;
     T.ctrl = nullptr;
     T.count = nullptr;
-    const int R = KW + A.nwords;
+    T.inserted = &s_inserted;
+    if (threadIdx.x == 0) s_inserted = 0u;
+    __syncthreads();
+    const int R = KW + P::nwords(A);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += stride) {
         const uint64_t i = b + threadIdx.x;
@@ -638,24 +775,48 @@ __global__ void __launch_bounds__(kBlock) k_agg_apply_records(AggArgs A, const u
 #pragma unroll
         for (int w = 0; w < KW; ++w) key[w] = r[w];
 #pragma unroll
-        for (int w = 0; w < S; ++w) c[w] = w < A.nwords ? r[KW + w] : 0;
-        global_upsert_or_spill<KW, S>(A, T, want, key, c);
+        for (int w = 0; w < S; ++w) c[w] = w < P::nwords(A) ? r[KW + w] : 0;
+        global_upsert_or_spill<KW, S, P>(A, T, want, key, c);
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_inserted) atomicAdd(&A.counters[0], s_inserted);
 }
 
-// Re-inserts every group of the old records into a grown control array.
+// Whether slot s of a table holds a group (the extra record excluded).
 template <int KW>
-__global__ void __launch_bounds__(kBlock) k_agg_rehash(unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R,
-    unsigned int groups)
+__device__ __forceinline__ bool slot_used(const uint64_t* recs, const unsigned long long* ctrl, int rstride, uint64_t s)
 {
-    for (unsigned int g = blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += gridDim.x * blockDim.x) {
+    if (KW == 1) return recs[s * (uint64_t)rstride] != kEmptyKey;
+    return ctrl[s] >= 2ull;
+}
+
+// Moves every group of the old table (slots [0, old_slots)) into the grown
+// table of A: keys are distinct, so a slot is claimed with one CAS and the
+// record copied with plain stores. The extra record is copied by the host.
+template <int KW>
+__global__ void __launch_bounds__(kBlock) k_agg_rehash(AggArgs A, const uint64_t* old_recs, const unsigned long long* old_ctrl,
+    uint64_t old_slots)
+{
+    const int R = KW + A.nwords;
+    for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < old_slots; s += (uint64_t)gridDim.x * blockDim.x) {
+        if (!slot_used<KW>(old_recs, old_ctrl, A.rstride, s)) continue;
+        const uint64_t* src = old_recs + s * (uint64_t)A.rstride;
         uint64_t key[KW];
 #pragma unroll
-        for (int w = 0; w < KW; ++w) key[w] = recs[(size_t)g * R + w];
+        for (int w = 0; w < KW; ++w) key[w] = src[w];
         const uint64_t h = key_place_hash<KW>(key);
-        const unsigned long long v = ((unsigned long long)(h >> 32) << 32) | (unsigned long long)(g + 2);
-        uint64_t slot = h & cmask;
-        while (atomicCAS(&ctrl[slot], 0ull, v) != 0ull) slot = (slot + 1) & cmask;
+        uint64_t slot = h & A.cmask;
+        if (KW == 1) {
+            while (atomicCAS((unsigned long long*)(A.recs + slot * (uint64_t)A.rstride), kEmptyKey, (unsigned long long)key[0]) != kEmptyKey) {
+                slot = (slot + 1) & A.cmask;
+            }
+        } else {
+            const unsigned long long v = ((unsigned long long)(h >> 32) << 32) | 2ull;
+            while (atomicCAS(&A.ctrl[slot], 0ull, v) != 0ull) slot = (slot + 1) & A.cmask;
+        }
+        uint64_t* dst = A.recs + slot * (uint64_t)A.rstride;
+        for (int w = 1; w < R; ++w) dst[w] = src[w];
+        if (KW > 1) dst[0] = src[0];
     }
 }
 
@@ -676,6 +837,8 @@ struct RedOut {
 
 struct FinArgs {
     const uint64_t* recs;
+    const uint32_t* index;    // slot of output group g (null: g itself)
+    int stride;               // words between records
     int R;
     int KW;
     unsigned int groups;
@@ -690,10 +853,12 @@ struct FinArgs {
 };
 
 
+#ifndef __HIPCC_RTC__
 // Launchers, one translation unit per key width (aggregate_kw<KW>.hip).
 void launch_update(int KW, int S, const AggArgs& A, int grid, size_t lds);
 void launch_records(int KW, int S, const AggArgs& A, const uint64_t* recs, unsigned int n);
-void launch_rehash(int KW, unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups);
+void launch_rehash(int KW, const AggArgs& A, const uint64_t* old_recs, const unsigned long long* old_ctrl, uint64_t old_slots);
+#endif
 
 }  // namespace agg
 }  // namespace qdbgpu

@@ -24,10 +24,10 @@ void launch_records_kw4(int S, const AggArgs& A, const uint64_t* recs, unsigned 
     }
 }
 
-void launch_rehash_kw4(unsigned long long* ctrl, uint64_t cmask, const uint64_t* recs, int R, unsigned int groups)
+void launch_rehash_kw4(const AggArgs& A, const uint64_t* old_recs, const unsigned long long* old_ctrl, uint64_t old_slots)
 {
-    const int grid = grid_for(groups, kBlock, 8);
-    hipLaunchKernelGGL((k_agg_rehash<4>), dim3(grid), dim3(kBlock), 0, stream(), ctrl, cmask, recs, R, groups);
+    const int grid = grid_for((int64_t)old_slots, kBlock, 8);
+    hipLaunchKernelGGL((k_agg_rehash<4>), dim3(grid), dim3(kBlock), 0, stream(), A, old_recs, old_ctrl, old_slots);
 }
 
 }  // namespace agg

@@ -3,10 +3,23 @@
 // ordered-float encodings, wave64 primitives.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../include/qdb_gpu.h"
+#else
+#include "qdb_gpu.h"
+#endif
+
+// Loops over descriptors (key fields, sources, program instructions) are
+// kept rolled in the precompiled generic kernels and fully unrolled in the
+// kernels specialized at run time, where the descriptors are constants.
+#ifdef QDB_SPECIALIZED
+#define QDB_DESC_LOOP _Pragma("unroll")
+#else
+#define QDB_DESC_LOOP _Pragma("unroll 1")
+#endif
 
 namespace qdbgpu {
 
@@ -220,6 +233,115 @@ __device__ __forceinline__ i128 pow10_i128(int64_t k)
     return p;
 }
 
+// One expression instruction on its operand values: the semantics shared
+// by the interpreter below and by kernels specialized at run time (which
+// call it with a constant instruction, so that the switch folds away).
+template <bool kHasCols>
+__device__ __forceinline__ DVal exec_insn(const DevInsn& in, const DVal& a, const DVal& b, const DevCols& cols, int64_t row)
+{
+    DVal o;
+    o.i = 0;
+    o.f = 0.0;
+    o.cls = in.cls;
+    o.valid = 0;
+    switch (in.op) {
+    case QDB_EXPR_COLUMN:
+        // The column index is uniform: the descriptor is read from the
+        // kernel arguments with scalar loads.
+        if (kHasCols) o = read_col(cols.c[in.column], row);
+        break;
+    case QDB_EXPR_CONST_INT:
+        o.valid = 1;
+        o.i = (i128)(((u128)(uint64_t)in.hi << 64) | (u128)(uint64_t)in.lo);
+        break;
+    case QDB_EXPR_CONST_F64:
+        o.valid = 1;
+        o.f = __longlong_as_double(in.lo);
+        break;
+    case QDB_EXPR_ADD:
+    case QDB_EXPR_SUB:
+    case QDB_EXPR_MUL:
+        o.valid = a.valid && b.valid;
+        if (in.cls == QDB_REG_F64) {
+            double x = a.cls == QDB_REG_F64 ? a.f : (double)a.i;
+            double y = b.cls == QDB_REG_F64 ? b.f : (double)b.i;
+            double z = in.op == QDB_EXPR_ADD ? x + y : in.op == QDB_EXPR_SUB ? x - y : x * y;
+            o.f = o.valid ? z : 0.0;
+        } else if (o.valid) {
+            u128 x = (u128)a.i, y = (u128)b.i;
+            o.i = (i128)(in.op == QDB_EXPR_ADD ? x + y : in.op == QDB_EXPR_SUB ? x - y : x * y);
+        }
+        break;
+    case QDB_EXPR_NEG:
+        o.valid = a.valid;
+        if (in.cls == QDB_REG_F64) o.f = a.valid ? -a.f : 0.0;
+        else if (a.valid) o.i = (i128)((u128)0 - (u128)a.i);
+        break;
+    case QDB_EXPR_SCALE_UP:
+        o.valid = a.valid;
+        if (a.valid) o.i = (i128)((u128)a.i * (u128)pow10_i128(in.lo));
+        break;
+    case QDB_EXPR_CMP: {
+        o.cls = QDB_REG_BOOL;
+        o.valid = a.valid && b.valid;
+        bool c = false;
+        if (a.cls == QDB_REG_F64 || b.cls == QDB_REG_F64) {
+            double x = a.cls == QDB_REG_F64 ? a.f : (double)a.i;
+            double y = b.cls == QDB_REG_F64 ? b.f : (double)b.i;
+            switch (in.lo) {
+            case QDB_CMP_EQ: c = x == y; break;
+            case QDB_CMP_NE: c = x != y; break;
+            case QDB_CMP_LT: c = x < y; break;
+            case QDB_CMP_LE: c = x <= y; break;
+            case QDB_CMP_GT: c = x > y; break;
+            default: c = x >= y; break;
+            }
+        } else {
+            i128 x = a.i, y = b.i;
+            switch (in.lo) {
+            case QDB_CMP_EQ: c = x == y; break;
+            case QDB_CMP_NE: c = x != y; break;
+            case QDB_CMP_LT: c = x < y; break;
+            case QDB_CMP_LE: c = x <= y; break;
+            case QDB_CMP_GT: c = x > y; break;
+            default: c = x >= y; break;
+            }
+        }
+        o.i = (o.valid && c) ? 1 : 0;
+        break;
+    }
+    case QDB_EXPR_AND: {
+        bool af = a.valid && a.i == 0, bf = b.valid && b.i == 0;
+        o.cls = QDB_REG_BOOL;
+        if (af || bf) { o.valid = 1; o.i = 0; }
+        else if (!a.valid || !b.valid) { o.valid = 0; }
+        else { o.valid = 1; o.i = 1; }
+        break;
+    }
+    case QDB_EXPR_OR: {
+        bool at = a.valid && a.i != 0, bt = b.valid && b.i != 0;
+        o.cls = QDB_REG_BOOL;
+        if (at || bt) { o.valid = 1; o.i = 1; }
+        else if (!a.valid || !b.valid) { o.valid = 0; }
+        else { o.valid = 1; o.i = 0; }
+        break;
+    }
+    case QDB_EXPR_NOT:
+        o.cls = QDB_REG_BOOL;
+        o.valid = a.valid;
+        o.i = a.valid ? (a.i == 0) : 0;
+        break;
+    case QDB_EXPR_IS_NULL:
+        o.cls = QDB_REG_BOOL;
+        o.valid = 1;
+        o.i = a.valid ? 0 : 1;
+        break;
+    default:
+        break;
+    }
+    return o;
+}
+
 template <bool kHasCols>
 __device__ __forceinline__ DVal eval_program(const DevProgram* __restrict__ prog, const DevCols& cols, int64_t row)
 {
@@ -232,7 +354,9 @@ __device__ __forceinline__ DVal eval_program(const DevProgram* __restrict__ prog
         r[k].valid = 0;
     }
     const int n = prog->n;
-    for (int k = 0; k < n; ++k) {
+    QDB_DESC_LOOP
+    for (int k = 0; k < QDB_EXPR_MAX_INSNS; ++k) {
+        if (k >= n) break;
         const DevInsn in = prog->insns[k];
         DVal a, b;
         // Registers are indexed by uniform values: select through a switch so
@@ -243,106 +367,7 @@ __device__ __forceinline__ DVal eval_program(const DevProgram* __restrict__ prog
             if (in.a == q) a = r[q];
             if (in.b == q) b = r[q];
         }
-        DVal o;
-        o.i = 0;
-        o.f = 0.0;
-        o.cls = in.cls;
-        o.valid = 0;
-        switch (in.op) {
-        case QDB_EXPR_COLUMN:
-            // The column index is uniform: the descriptor is read from the
-            // kernel arguments with scalar loads.
-            if (kHasCols) o = read_col(cols.c[in.column], row);
-            break;
-        case QDB_EXPR_CONST_INT:
-            o.valid = 1;
-            o.i = (i128)(((u128)(uint64_t)in.hi << 64) | (u128)(uint64_t)in.lo);
-            break;
-        case QDB_EXPR_CONST_F64:
-            o.valid = 1;
-            o.f = __longlong_as_double(in.lo);
-            break;
-        case QDB_EXPR_ADD:
-        case QDB_EXPR_SUB:
-        case QDB_EXPR_MUL:
-            o.valid = a.valid && b.valid;
-            if (in.cls == QDB_REG_F64) {
-                double x = a.cls == QDB_REG_F64 ? a.f : (double)a.i;
-                double y = b.cls == QDB_REG_F64 ? b.f : (double)b.i;
-                double z = in.op == QDB_EXPR_ADD ? x + y : in.op == QDB_EXPR_SUB ? x - y : x * y;
-                o.f = o.valid ? z : 0.0;
-            } else if (o.valid) {
-                u128 x = (u128)a.i, y = (u128)b.i;
-                o.i = (i128)(in.op == QDB_EXPR_ADD ? x + y : in.op == QDB_EXPR_SUB ? x - y : x * y);
-            }
-            break;
-        case QDB_EXPR_NEG:
-            o.valid = a.valid;
-            if (in.cls == QDB_REG_F64) o.f = a.valid ? -a.f : 0.0;
-            else if (a.valid) o.i = (i128)((u128)0 - (u128)a.i);
-            break;
-        case QDB_EXPR_SCALE_UP:
-            o.valid = a.valid;
-            if (a.valid) o.i = (i128)((u128)a.i * (u128)pow10_i128(in.lo));
-            break;
-        case QDB_EXPR_CMP: {
-            o.cls = QDB_REG_BOOL;
-            o.valid = a.valid && b.valid;
-            bool c = false;
-            if (a.cls == QDB_REG_F64 || b.cls == QDB_REG_F64) {
-                double x = a.cls == QDB_REG_F64 ? a.f : (double)a.i;
-                double y = b.cls == QDB_REG_F64 ? b.f : (double)b.i;
-                switch (in.lo) {
-                case QDB_CMP_EQ: c = x == y; break;
-                case QDB_CMP_NE: c = x != y; break;
-                case QDB_CMP_LT: c = x < y; break;
-                case QDB_CMP_LE: c = x <= y; break;
-                case QDB_CMP_GT: c = x > y; break;
-                default: c = x >= y; break;
-                }
-            } else {
-                i128 x = a.i, y = b.i;
-                switch (in.lo) {
-                case QDB_CMP_EQ: c = x == y; break;
-                case QDB_CMP_NE: c = x != y; break;
-                case QDB_CMP_LT: c = x < y; break;
-                case QDB_CMP_LE: c = x <= y; break;
-                case QDB_CMP_GT: c = x > y; break;
-                default: c = x >= y; break;
-                }
-            }
-            o.i = (o.valid && c) ? 1 : 0;
-            break;
-        }
-        case QDB_EXPR_AND: {
-            bool af = a.valid && a.i == 0, bf = b.valid && b.i == 0;
-            o.cls = QDB_REG_BOOL;
-            if (af || bf) { o.valid = 1; o.i = 0; }
-            else if (!a.valid || !b.valid) { o.valid = 0; }
-            else { o.valid = 1; o.i = 1; }
-            break;
-        }
-        case QDB_EXPR_OR: {
-            bool at = a.valid && a.i != 0, bt = b.valid && b.i != 0;
-            o.cls = QDB_REG_BOOL;
-            if (at || bt) { o.valid = 1; o.i = 1; }
-            else if (!a.valid || !b.valid) { o.valid = 0; }
-            else { o.valid = 1; o.i = 0; }
-            break;
-        }
-        case QDB_EXPR_NOT:
-            o.cls = QDB_REG_BOOL;
-            o.valid = a.valid;
-            o.i = a.valid ? (a.i == 0) : 0;
-            break;
-        case QDB_EXPR_IS_NULL:
-            o.cls = QDB_REG_BOOL;
-            o.valid = 1;
-            o.i = a.valid ? 0 : 1;
-            break;
-        default:
-            break;
-        }
+        const DVal o = exec_insn<kHasCols>(in, a, b, cols, row);
 #pragma unroll
         for (int q = 0; q < QDB_EXPR_MAX_REGS; ++q) {
             if (in.dst == q) r[q] = o;

@@ -1,0 +1,160 @@
+// Kernels specialized at run time. An operator whose layout is fixed when it
+// is compiled (the aggregate's keys, state words, predicate and computed
+// arguments) gets a kernel generated for it, as the reference's compiler
+// generates one per query (qdb/kernel/compiler.cpp): the generated source
+// includes the library's own device headers, which are built into the
+// library, and is compiled with hipRTC for the device's architecture. Each
+// distinct source is compiled once per process.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <chrono>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "jit.h"
+#include "runtime.h"
+
+namespace qdbgpu {
+
+namespace {
+
+// The device headers, embedded by the build (jit_headers.inc defines
+// kJitHeaderNames / kJitHeaderTexts / kJitHeaderCount).
+#include "jit_headers.inc"
+
+struct JitEntry {
+    hipModule_t module = nullptr;
+    hipFunction_t fn = nullptr;
+    std::string error;
+    double compile_ms = 0.0;
+};
+
+struct JitCache {
+    std::mutex mu;
+    std::unordered_map<std::string, JitEntry> map;
+    int compiles = 0;
+    double total_ms = 0.0;
+};
+
+JitCache& cache()
+{
+    static JitCache* c = new JitCache();
+    return *c;
+}
+
+std::string device_arch()
+{
+    hipDeviceProp_t p;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return "gfx950";
+    }
+    std::string a = p.gcnArchName;
+    const size_t colon = a.find(':');
+    return colon == std::string::npos ? a : a.substr(0, colon);
+}
+
+// Compiles `src` to a code object; false with the log in *err.
+bool compile_source(const std::string& src, const std::string& arch, std::vector<char>* code, std::string* err)
+{
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "qdb_specialized.hip", kJitHeaderCount, kJitHeaderTexts, kJitHeaderNames) !=
+        HIPRTC_SUCCESS) {
+        *err = "hiprtcCreateProgram failed";
+        return false;
+    }
+    const std::string arch_opt = "--offload-arch=" + arch;
+    // The packed-FP32 feature stays off in run-time kernels too.
+    const char* opts[] = {arch_opt.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    size_t log_size = 0;
+    (void)hiprtcGetProgramLogSize(prog, &log_size);
+    std::string log(log_size, '\0');
+    if (log_size) (void)hiprtcGetProgramLog(prog, &log[0]);
+    if (r != HIPRTC_SUCCESS) {
+        *err = std::string("hipRTC: ") + hiprtcGetErrorString(r) + "\n" + log;
+        (void)hiprtcDestroyProgram(&prog);
+        return false;
+    }
+    size_t size = 0;
+    (void)hiprtcGetCodeSize(prog, &size);
+    code->resize(size);
+    (void)hiprtcGetCode(prog, code->data());
+    (void)hiprtcDestroyProgram(&prog);
+    return size > 0;
+}
+
+}  // namespace
+
+bool jit_enabled()
+{
+    const char* e = getenv("QDB_GPU_JIT");
+    return !(e && e[0] == '0');
+}
+
+hipFunction_t jit_function(const std::string& src, const char* name, std::string* err)
+{
+    JitCache& c = cache();
+    std::lock_guard<std::mutex> lock(c.mu);
+    auto it = c.map.find(src);
+    if (it != c.map.end()) {
+        if (err) *err = it->second.error;
+        return it->second.fn;
+    }
+    JitEntry e;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<char> code;
+    if (compile_source(src, device_arch(), &code, &e.error)) {
+        hipError_t h = hipModuleLoadData(&e.module, code.data());
+        if (h == hipSuccess) h = hipModuleGetFunction(&e.fn, e.module, name);
+        if (h != hipSuccess) {
+            e.error = std::string("module load: ") + hipGetErrorString(h);
+            e.fn = nullptr;
+            (void)hipGetLastError();
+        }
+    }
+    e.compile_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c.compiles += 1;
+    c.total_ms += e.compile_ms;
+    const char* dump = getenv("QDB_GPU_JIT_LOG");
+    if (dump && dump[0] == '1') {
+        fprintf(stderr, "[qdb_gpu jit] %s: %.0f ms%s%s\n", name, e.compile_ms, e.fn ? "" : ", failed: ", e.fn ? "" : e.error.c_str());
+    }
+    if (err) *err = e.error;
+    hipFunction_t fn = e.fn;
+    c.map.emplace(src, std::move(e));
+    return fn;
+}
+
+bool jit_compile_only(const std::string& src, const std::string& arch, std::string* err)
+{
+    std::vector<char> code;
+    return compile_source(src, arch, &code, err);
+}
+
+void jit_stats(int* compiles, double* total_ms)
+{
+    JitCache& c = cache();
+    std::lock_guard<std::mutex> lock(c.mu);
+    *compiles = c.compiles;
+    *total_ms = c.total_ms;
+}
+
+}  // namespace qdbgpu
+
+extern "C" int64_t qdb_gpu_jit_stats(double* total_ms)
+{
+    int n = 0;
+    double ms = 0.0;
+    qdbgpu::jit_stats(&n, &ms);
+    if (total_ms) *total_ms = ms;
+    return n;
+}
