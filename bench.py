@@ -153,8 +153,7 @@ def bench_groups(steps, warmup, rows=100_000_000, distinct=10_000_000):
     def step():
         p = TAggregateProcessor(kern, initial_capacity=1 << 24)
         p.add(dev.rowset)
-        res = p.finish()
-        out["groups"] = len(res[0][0])
+        out["groups"] = p.finish_in_hbm()
         p.close()
 
     for _ in range(warmup):

@@ -266,7 +266,7 @@ struct AggTable {
     unsigned int* counters = nullptr;   // device [4]
     unsigned int groups = 0;
     int64_t expect_groups = 0;          // ExpectRowGroups hint
-    int match_rounds = 8;
+    int match_rounds = 0;
 };
 
 // The HashTable blob (qdb/exec/aggregate_exec.cpp:18-39). The processor reads
@@ -817,8 +817,13 @@ AggTable* table_new(AggPlan* P, int64_t slots_hint)
         return nullptr;
     }
     // Few expected groups: wave-level matching pays; many: it does not.
-    T->match_rounds = groups_hint > (1 << 16) ? 1 : 8;
-    if (P->has_pairs) T->match_rounds = 0;
+    // Wave-level matching of equal keys before the table updates: measured
+    // slower than letting every lane apply its own row for both few groups
+    // (the per-round reduction of every state word costs more than the LDS
+    // atomics it saves) and many (most keys differ within a wave), so it is
+    // off; the generic kernels keep the machinery.
+    T->match_rounds = 0;
+    (void)groups_hint;
     return T;
 }
 
@@ -1126,7 +1131,7 @@ int64_t agg_dispatch_impl(AggPlan* P, void* ht, TRowSet* batch, int64_t arg, int
         B->RowExpectedGroups = arg;
         if (B->table) {
             B->table->expect_groups = arg;
-            if (arg > (1 << 16)) B->table->match_rounds = P->has_pairs ? 0 : 1;
+            (void)arg;
         }
         return 1;
     case QDB_AGG_OP_UPDATE: {

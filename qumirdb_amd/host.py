@@ -502,6 +502,16 @@ class TAggregateProcessor:
         finally:
             lib().qdb_gpu_rowset_release(C.byref(out))
 
+    def finish_in_hbm(self) -> int:
+        """Finishes into a device-resident rowset, as a downstream GPU
+        operator would take it, releases it and returns the group count."""
+        out = abi.TRowSet()
+        if lib().qdb_gpu_agg_processor_finish(self.p, C.byref(out)) < 0:
+            raise RuntimeError((lib().qdb_gpu_exec_last_error() or b"").decode())
+        n = int(out.RowCount)
+        lib().qdb_gpu_rowset_release(C.byref(out))
+        return n
+
     def close(self):
         if self.p:
             lib().qdb_gpu_agg_processor_free(self.p)

@@ -705,3 +705,24 @@ JOIN_GOLDENS = {
 }
 
 PROJECT_GOLDENS = {"multiply": golden_multiply_project}
+
+
+def random_agg_case(seed, n):
+    rng = np.random.default_rng(seed)
+    k1 = rng.integers(-3, 4, n).astype(np.int32)
+    k1v = rng.random(n) > 0.1
+    k2 = rng.integers(0, 50 + seed * 2000, n).astype(np.int64)
+    vi = rng.integers(-(1 << 62), 1 << 62, n).astype(np.int64)
+    viv = rng.random(n) > 0.2
+    dec = [int(x) * (10 ** 20) + int(y) for x, y in zip(rng.integers(-10 ** 9, 10 ** 9, n), rng.integers(0, 10 ** 9, n))]
+    decv = rng.random(n) > 0.3
+    vf = rng.normal(0, 1e3, n)
+    d64 = rng.integers(-10 ** 12, 10 ** 12, n).astype(np.int64)
+    types = [T(SIGNED, 4, nullable=True), T(SIGNED, 8), T(SIGNED, 8, nullable=True), T(DECIMAL, 16, 3, True, 38),
+             T(FLOAT, 8), T(DECIMAL, 8, 2, False, 15)]
+    case = AggCase(types, [0, 1], [(AGG_COUNT_STAR, -1), (AGG_COUNT, 2), (AGG_SUM, 2), (AGG_MIN, 2), (AGG_MAX, 2),
+                                   (AGG_SUM, 3), (AGG_MIN, 3), (AGG_MAX, 3), (AGG_SUM, 4), (AGG_MIN, 4), (AGG_MAX, 4),
+                                   (AGG_SUM, 5), (AGG_MIN, 5)])
+    sel = (rng.random(n) > 0.15).astype(np.uint8)
+    b = Batch(types, [k1, k2, vi, dec, vf, d64], [k1v, None, viv, decv, None, None], sel)
+    return case, b

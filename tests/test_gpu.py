@@ -10,7 +10,7 @@ import pytest
 
 import cases
 from cases import (AGG_COUNT, AGG_COUNT_STAR, AGG_MAX, AGG_MIN, AGG_SUM, AggCase, Batch, GpuBackend, JoinCase,
-                   OracleBackend)
+                   OracleBackend, random_agg_case)
 from helpers import DECIMAL, FLOAT, SIGNED, T, result_rows, sort_key
 
 from oracle import oracle as O
@@ -73,27 +73,6 @@ def test_project_goldens(gpu_backend, name):
     same(got, want)
 
 
-def random_agg_case(seed, n):
-    rng = np.random.default_rng(seed)
-    k1 = rng.integers(-3, 4, n).astype(np.int32)
-    k1v = rng.random(n) > 0.1
-    k2 = rng.integers(0, 50 + seed * 2000, n).astype(np.int64)
-    vi = rng.integers(-(1 << 62), 1 << 62, n).astype(np.int64)
-    viv = rng.random(n) > 0.2
-    dec = [int(x) * (10 ** 20) + int(y) for x, y in zip(rng.integers(-10 ** 9, 10 ** 9, n), rng.integers(0, 10 ** 9, n))]
-    decv = rng.random(n) > 0.3
-    vf = rng.normal(0, 1e3, n)
-    d64 = rng.integers(-10 ** 12, 10 ** 12, n).astype(np.int64)
-    types = [T(SIGNED, 4, nullable=True), T(SIGNED, 8), T(SIGNED, 8, nullable=True), T(DECIMAL, 16, 3, True, 38),
-             T(FLOAT, 8), T(DECIMAL, 8, 2, False, 15)]
-    case = AggCase(types, [0, 1], [(AGG_COUNT_STAR, -1), (AGG_COUNT, 2), (AGG_SUM, 2), (AGG_MIN, 2), (AGG_MAX, 2),
-                                   (AGG_SUM, 3), (AGG_MIN, 3), (AGG_MAX, 3), (AGG_SUM, 4), (AGG_MIN, 4), (AGG_MAX, 4),
-                                   (AGG_SUM, 5), (AGG_MIN, 5)])
-    sel = (rng.random(n) > 0.15).astype(np.uint8)
-    b = Batch(types, [k1, k2, vi, dec, vf, d64], [k1v, None, viv, decv, None, None], sel)
-    return case, b
-
-
 @pytest.mark.parametrize("seed,n,capacity", [(0, 20000, 8), (1, 50000, 8), (3, 200000, 1 << 14)])
 def test_aggregate_random_vs_oracle(gpu_backend, seed, n, capacity):
     """Nullable multi-column keys (one 16-byte decimal argument), every
@@ -103,6 +82,32 @@ def test_aggregate_random_vs_oracle(gpu_backend, seed, n, capacity):
     got = sorted(result_rows(gpu_backend.aggregate(case, [b], capacity=capacity)), key=sort_key)
     want = sorted(result_rows(OracleBackend(fsum_mode=2).aggregate(case, [b])), key=sort_key)
     same(got, want)
+
+
+def test_generated_kernel_equals_generic(gpu_backend, monkeypatch):
+    """The update kernel generated for an aggregate and the precompiled
+    generic kernel give identical results (f64 sums included: both sum
+    exactly). QDB_GPU_JIT=0 selects the generic kernel for aggregates
+    compiled while it is set."""
+    import qumirdb_amd as q
+    from qumirdb_amd import workloads as W
+    case, b = random_agg_case(7, 40000)
+    got_jit = sorted(result_rows(gpu_backend.aggregate(case, [b], capacity=8)), key=sort_key)
+    assert q.lib().qdb_gpu_jit_stats(None) > 0, "no kernel was generated"
+    monkeypatch.setenv("QDB_GPU_JIT", "0")
+    got_generic = sorted(result_rows(gpu_backend.aggregate(case, [b], capacity=8)), key=sort_key)
+    same(got_jit, got_generic)
+    # the fused lineitem aggregate (predicate and computed i128 arguments)
+    outs = []
+    for jit in ("1", "0"):
+        monkeypatch.setenv("QDB_GPU_JIT", jit)
+        dev = W.LineitemDevice(200_000)
+        kern = q.AggregateKernels(W.lineitem_fused_spec())
+        p = q.TAggregateProcessor(kern, initial_capacity=8)
+        p.add(dev.rowset)
+        outs.append(W.lineitem_finalize(p.finish()))
+        p.close()
+    assert outs[0] == outs[1]
 
 
 def test_aggregate_batches_accumulate(gpu_backend):
