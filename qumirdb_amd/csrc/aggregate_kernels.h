@@ -159,7 +159,14 @@ __device__ __forceinline__ void atomic_apply_lds(uint8_t op, uint64_t* p, uint64
 
 __device__ __forceinline__ void atomic_apply_global(uint8_t op, uint64_t* p, uint64_t v)
 {
-    // Same operations; device-scope atomics on HBM.
+    // Same operations; device-scope atomics on HBM. A MIN/MAX word only
+    // moves one way, so a plain read that already beats v (even a stale one,
+    // which can only be further from the final value) makes the atomic
+    // unnecessary; the record's line is in cache from the key lookup.
+    if (op == OP_MIN_I64 || op == OP_MAX_I64 || op == OP_MIN_U64 || op == OP_MAX_U64) {
+        const uint64_t cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (combine(op, cur, v) == cur) return;
+    }
     atomic_apply_lds(op, p, v);
 }
 
