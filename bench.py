@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["join", "lineitem", "groups", "all"], default="all")
+    ap.add_argument("--workload", choices=["join", "lineitem", "groups", "batched", "all"], default="all")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (3 launches), the known-size kernel that "
@@ -177,6 +177,72 @@ def bench_groups(steps, warmup, rows=100_000_000, distinct=10_000_000):
             "achieved_gbs": nbytes / ((k_ms + f_ms) * 1e-3) / 1e9 if k_ms else 0.0}
 
 
+def bench_batched(n_instances=512):
+    """The batched form: 512 group-by instances of one aggregate (COUNT(*),
+    SUM(i64), MIN/MAX/SUM(f64) by an int64 key; rows log-uniform 10K..20M,
+    cardinality log-uniform 4..1M, uniform or Zipf(1.1) keys), whole queries
+    (table set-up, update, finish into HBM) one after another versus the
+    same instances with all updates in one persistent batched launch."""
+    from qumirdb_amd import lib, workloads as W
+    from qumirdb_amd.host import AggregateKernels, TAggregateProcessor
+
+    inst = W.batched_instances(n_instances)
+    dev = W.BatchedDevice(inst)
+    kern = AggregateKernels(W.groups_spec())
+
+    def capacity(rows, card):
+        c = 8
+        while c < 2 * min(rows, card):
+            c *= 2
+        return c
+
+    caps = [capacity(r, c) for r, c, _ in inst]
+
+    def plain():
+        groups = []
+        for d, cap in zip(dev.devs, caps):
+            p = TAggregateProcessor(kern, initial_capacity=cap)
+            p.add(d.rowset)
+            groups.append(p.finish_in_hbm())
+            p.close()
+        return groups
+
+    def batched():
+        procs = [TAggregateProcessor(kern, initial_capacity=cap) for cap in caps]
+        TAggregateProcessor.add_batched(procs, [d.rowset for d in dev.devs])
+        groups = [p.finish_in_hbm() for p in procs]
+        for p in procs:
+            p.close()
+        return groups
+
+    plain()
+    batched()  # warm-up: kernels compiled, allocator primed
+    lib().qdb_gpu_synchronize()
+    lib().qdb_gpu_timing_reset()
+    lib().qdb_gpu_timing_enable(1)
+    t0 = time.perf_counter()
+    g_plain = plain()
+    lib().qdb_gpu_synchronize()
+    t1 = time.perf_counter()
+    upd_ms, upd_n = timing("agg_update")
+    lib().qdb_gpu_timing_reset()
+    t2 = time.perf_counter()
+    g_batch = batched()
+    lib().qdb_gpu_synchronize()
+    t3 = time.perf_counter()
+    bat_ms, _ = timing("agg_batch")
+    lib().qdb_gpu_timing_enable(0)
+    lib().qdb_gpu_timing_reset()
+    if g_plain != g_batch:
+        raise RuntimeError("batched and per-instance group counts differ")
+    rows = sum(r for r, _, _ in inst)
+    return {"instances": n_instances, "rows": rows, "groups": sum(g_batch),
+            "plain_s": t1 - t0, "batched_s": t3 - t2,
+            "plain_queries_per_s": n_instances / (t1 - t0), "batched_queries_per_s": n_instances / (t3 - t2),
+            "speedup": (t1 - t0) / (t3 - t2), "rows_per_s": rows / (t3 - t2), "ms_per_step": (t3 - t2) * 1e3,
+            "plain_update_kernels_ms": upd_ms * upd_n, "batched_update_kernel_ms": bat_ms}
+
+
 def cpu_join_baseline(build_rows=1_500_000, sample_probe=60_000_000):
     """The oracle's row-table join (one thread) over the filtered build and
     `sample_probe` probe rows (by default the whole workload)."""
@@ -251,9 +317,11 @@ def main():
         out["lineitem"] = bench_lineitem(args.steps, args.warmup)
     if args.workload in ("groups", "all"):
         out["groups"] = bench_groups(max(1, args.steps // 2), max(1, args.warmup // 2))
+    if args.workload in ("batched", "all"):
+        out["batched"] = bench_batched()
     if args.copy_calibration:
         out["copy_calibration"] = copy_calibration(args.copy_calibration)
-    head = out.get("join") or out.get("lineitem") or out.get("groups")
+    head = out.get("join") or out.get("lineitem") or out.get("groups") or out.get("batched")
     name = "join" if "join" in out else list(out)[0]
     cpu = None
     if not args.no_cpu and name == "join":

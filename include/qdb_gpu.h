@@ -445,6 +445,15 @@ int64_t qdb_gpu_device_limits(int64_t* out, int64_t n);
  * environment variable QDB_GPU_JIT=0 runs the precompiled generic kernels
  * instead. */
 int64_t qdb_gpu_aggregate_kernel_source(const QdbAggregateSpec* spec, char* out, int64_t cap);
+
+/* Batched form (no counterpart in the reference, whose processors update
+ * one table per call): applies batches[i] to the initialized HashTable blob
+ * hts[i], for every i < n, in ONE persistent launch whose workgroups take
+ * (table, row range) items from a device work queue. Every table must come
+ * from `kernels` (one compiled aggregate). Results are exactly those of n
+ * agg_dispatch(Update) calls. Returns 0, or -1 with the error text. */
+int qdb_gpu_agg_update_batched(const QdbAggregateKernels* kernels, void* const* hts, const TRowSet* const* batches,
+                               int64_t n);
 int qdb_gpu_jit_check(const char* src, const char* arch, char* err, int64_t cap);
 int64_t qdb_gpu_jit_stats(double* total_ms);
 
@@ -457,6 +466,9 @@ int64_t qdb_gpu_jit_stats(double* total_ms);
 /* ------------------------------------------------------------------------ */
 
 const char* qdb_gpu_exec_last_error(void);
+/* processors[i] (all of one compiled aggregate) take batches[i], in one
+ * batched launch (qdb_gpu_agg_update_batched). 0 or -1. */
+int qdb_gpu_agg_processor_add_batched(void* const* procs, const TRowSet* const* batches, int64_t n);
 /* row_layout != 0 starts with InitRows(initial_groups), else Init. */
 void* qdb_gpu_agg_processor_new(const QdbAggregateKernels* kernels, int64_t initial_capacity, int row_layout,
     int64_t initial_groups);

@@ -245,7 +245,6 @@ struct AggPlan {
     std::vector<DevProgram> h_progs;
     QdbValueType slot_type[kMaxDevCols];
     std::string jit_src;
-    hipFunction_t jit_fn = nullptr;
     bool jit_tried = false;
 
     ~AggPlan()
@@ -783,22 +782,25 @@ std::string agg_jit_source(const AggPlan* P)
     o += "        return true;\n    }\n};\n}  // namespace agg\n}  // namespace qdbgpu\n";
     o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kBlock) + ") qdb_agg_update(qdbgpu::agg::AggArgs A)\n{\n";
     o += "    qdbgpu::agg::agg_update_body<" + std::to_string(P->KW) + ", " + std::to_string(P->S) + ", qdbgpu::agg::JitPolicy>(A);\n}\n";
+    o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kBlock) +
+        ") qdb_agg_batch(const qdbgpu::agg::AggArgs* insts, const qdbgpu::agg::BatchItem* items, int nitems, unsigned int* head)\n{\n";
+    o += "    qdbgpu::agg::agg_batch_body<" + std::to_string(P->KW) + ", " + std::to_string(P->S) +
+        ", qdbgpu::agg::JitPolicy>(insts, items, nitems, head);\n}\n";
     return o;
 }
 
-// The generated update kernel of the plan, compiled on first use; nullptr
-// when run-time kernels are off or the compile failed (the generic kernel
-// runs then).
-hipFunction_t plan_jit_update(AggPlan* P)
+// A kernel of the plan's generated source ("qdb_agg_update" or
+// "qdb_agg_batch"), compiled on first use; nullptr when run-time kernels are
+// off or the compile failed (the generic kernels run then).
+hipFunction_t plan_jit_function(AggPlan* P, const char* name)
 {
     if (!jit_enabled()) return nullptr;
     if (!P->jit_tried) {
         P->jit_tried = true;
-        if (P->jit_src.empty()) P->jit_src = agg_jit_source(P);
-        std::string err;
-        P->jit_fn = jit_function(P->jit_src, "qdb_agg_update", &err);
+        P->jit_src = agg_jit_source(P);
     }
-    return P->jit_fn;
+    std::string err;
+    return jit_function(P->jit_src, name, &err);
 }
 
 AggTable* table_new(AggPlan* P, int64_t slots_hint)
@@ -898,99 +900,250 @@ bool set_counters(AggTable* T)
     return hip_ok(hipMemcpyAsync(T->counters, h, 16, hipMemcpyHostToDevice, stream()), "counters");
 }
 
+// LDS table of an update launch: as many groups as fit in 64 KiB, a power
+// of two (none for i128 MIN/MAX, which live in HBM only).
+void lds_layout(const AggPlan* P, AggArgs* A, size_t* lds_bytes)
+{
+    const int R = P->KW + A->nwords;
+    int lds_groups = 1024;
+    while (lds_groups > 16 && (size_t)lds_groups * R * 8 + (size_t)lds_groups * 2 * 4 + 16 > 64 * 1024) lds_groups /= 2;
+    if (P->has_pairs) lds_groups = 0;
+    A->lds_groups = lds_groups;
+    A->lds_slots = lds_groups * 2;
+    *lds_bytes = lds_groups ? (size_t)lds_groups * R * 8 + (size_t)A->lds_slots * 4 + 16 : 0;
+}
+
+// One update launch over rows [base, base + n) of a batch into table T: its
+// arguments and the buffers for what it cannot insert (every row of the
+// launch, and `recs_cap` group records). The second pair of buffers, for a
+// replay that spills again, is made only when a replay runs.
+struct UpdateLaunch {
+    AggTable* T = nullptr;
+    AggArgs A;
+    int64_t n = 0;
+    size_t recs_cap = 0;
+    uint32_t* ov_rows[2] = {nullptr, nullptr};
+    uint64_t* ov_recs[2] = {nullptr, nullptr};
+
+    void release()
+    {
+        for (int k = 0; k < 2; ++k) {
+            dfree(ov_rows[k]);
+            dfree(ov_recs[k]);
+            ov_rows[k] = nullptr;
+            ov_recs[k] = nullptr;
+        }
+    }
+};
+
+bool prepare_launch(AggTable* T, const AggArgs& bound, int64_t base, int64_t n, size_t recs_cap, UpdateLaunch* L)
+{
+    const AggPlan* P = T->plan;
+    const int R = P->KW + bound.nwords;
+    L->T = T;
+    L->A = bound;
+    L->n = n;
+    L->recs_cap = recs_cap;
+    L->ov_rows[0] = (uint32_t*)dalloc((size_t)n * 4 + 16);
+    L->ov_recs[0] = (uint64_t*)dalloc(recs_cap * R * 8);
+    if (!L->ov_rows[0] || !L->ov_recs[0]) {
+        L->release();
+        return false;
+    }
+    table_args(T, &L->A);
+    L->A.row_base = base;
+    L->A.nrows = n;
+    L->A.row_index = nullptr;
+    L->A.ov_rows = L->ov_rows[0];
+    L->A.ov_recs = L->ov_recs[0];
+    L->A.ov_rec_cap = (unsigned int)recs_cap;
+    return true;
+}
+
+// After the launch: while rows or records were left over, grows the table
+// and replays exactly those (generic kernels), so every row is applied
+// once; then keeps the table at most half full for the next batch.
+bool settle_launch(UpdateLaunch* L, unsigned int cnt[4])
+{
+    AggTable* T = L->T;
+    const AggPlan* P = T->plan;
+    const int KW = P->KW, S = P->S;
+    const int R = KW + L->A.nwords;
+    AggArgs A = L->A;
+    int cur = 0;
+    while (cnt[1] || cnt[2]) {
+        if (cnt[2] > L->recs_cap || cnt[1] > (uint64_t)L->n) {
+            return fail("aggregate spill overflow (%u rows, %u records)", cnt[1], cnt[2]);
+        }
+        const unsigned int spilled_rows = cnt[1];
+        const unsigned int spilled_recs = cnt[2];
+        const int nxt = cur ^ 1;
+        if (!L->ov_rows[nxt]) {
+            L->ov_rows[nxt] = (uint32_t*)dalloc((size_t)L->n * 4 + 16);
+            L->ov_recs[nxt] = (uint64_t*)dalloc(L->recs_cap * R * 8);
+            if (!L->ov_rows[nxt] || !L->ov_recs[nxt]) return false;
+        }
+        if (!table_grow(T, (uint64_t)T->groups + spilled_rows + spilled_recs + 1024)) return false;
+        table_args(T, &A);
+        A.ov_rows = L->ov_rows[nxt];
+        A.ov_recs = L->ov_recs[nxt];
+        if (!set_counters(T)) return false;
+        if (spilled_rows) {
+            AggArgs B = A;
+            B.row_index = L->ov_rows[cur];
+            B.nrows = spilled_rows;
+            B.row_base = L->A.row_base;
+            B.lds_slots = 0;
+            B.lds_groups = 0;
+            const int g2 = grid_for(spilled_rows, kBlock, 2);
+            agg::launch_update(KW, S, B, g2, 0);
+        }
+        if (spilled_recs) agg::launch_records(KW, S, A, L->ov_recs[cur], spilled_recs);
+        if (!hip_ok(hipGetLastError(), "agg replay launch")) return false;
+        if (!read_counters(T, cnt)) return false;
+        cur = nxt;
+    }
+    // Keep the load at most one half for the next batch.
+    if ((uint64_t)T->groups > T->capacity / 2 && !table_grow(T, (uint64_t)T->groups)) return false;
+    return true;
+}
+
 // agg_dispatch(Update): every selected row of the batch, exactly once.
 bool table_update(AggTable* T, const TRowSet* batch)
 {
     AggPlan* P = T->plan;
     if (batch->RowCount <= 0) return true;
-    AggArgs A;
-    if (!bind_batch(P, batch, &A)) return false;
+    AggArgs bound;
+    if (!bind_batch(P, batch, &bound)) return false;
     const int KW = P->KW, S = P->S;
-    const int R = KW + A.nwords;
-    // LDS table: as many groups as fit in 64 KiB, a power of two.
-    int lds_groups = 1024;
-    while (lds_groups > 16 && (size_t)lds_groups * R * 8 + (size_t)lds_groups * 2 * 4 + 16 > 64 * 1024) lds_groups /= 2;
-    if (P->has_pairs) lds_groups = 0;
-    A.lds_groups = lds_groups;
-    A.lds_slots = lds_groups * 2;
-    const size_t lds = lds_groups ? (size_t)lds_groups * R * 8 + (size_t)A.lds_slots * 4 + 16 : 0;
+    size_t lds = 0;
+    lds_layout(P, &bound, &lds);
     const int64_t chunk = (int64_t)1 << 30;
     for (int64_t base = 0; base < batch->RowCount; base += chunk) {
         const int64_t n = batch->RowCount - base < chunk ? batch->RowCount - base : chunk;
         const int grid = grid_for(n, kBlock, 2);
-        // Spill buffers: every row of the launch, every LDS group of every
-        // workgroup, twice (a replay may spill again).
-        const size_t ov_recs_cap = (size_t)grid * (lds_groups ? lds_groups : 1) + 1024;
-        uint32_t* ov_rows[2] = {(uint32_t*)dalloc((size_t)n * 4), (uint32_t*)dalloc((size_t)n * 4)};
-        uint64_t* ov_recs[2] = {(uint64_t*)dalloc(ov_recs_cap * R * 8), (uint64_t*)dalloc(ov_recs_cap * R * 8)};
-        auto release = [&]() {
-            dfree(ov_rows[0]);
-            dfree(ov_rows[1]);
-            dfree(ov_recs[0]);
-            dfree(ov_recs[1]);
-        };
-        if (!ov_rows[0] || !ov_rows[1] || !ov_recs[0] || !ov_recs[1]) {
-            release();
+        UpdateLaunch L;
+        if (!prepare_launch(T, bound, base, n, (size_t)grid * (bound.lds_groups ? bound.lds_groups : 1) + 1024, &L)) return false;
+        if (!set_counters(T)) {
+            L.release();
             return false;
         }
-        table_args(T, &A);
-        A.row_base = base;
-        A.nrows = n;
-        A.row_index = nullptr;
-        A.ov_rows = ov_rows[0];
-        A.ov_recs = ov_recs[0];
-        A.ov_rec_cap = (unsigned int)ov_recs_cap;
-        if (!set_counters(T)) { release(); return false; }
         hipError_t launched = hipSuccess;
         {
-            hipFunction_t fn = plan_jit_update(T->plan);
+            hipFunction_t fn = plan_jit_function(P, "qdb_agg_update");
             TimedRegion timer("agg_update");
             if (fn) {
-                void* args[] = {&A};
+                void* args[] = {&L.A};
                 launched = hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, kBlock, 1, 1, (unsigned)lds, stream(), args, nullptr);
             } else {
-                agg::launch_update(KW, S, A, grid, lds);
+                agg::launch_update(KW, S, L.A, grid, lds);
             }
         }
-        if (!hip_ok(launched, "agg update launch") || !hip_ok(hipGetLastError(), "agg update launch")) { release(); return false; }
         unsigned int cnt[4];
-        if (!read_counters(T, cnt)) { release(); return false; }
-        int cur = 0;
-        while (cnt[1] || cnt[2]) {
-            if (cnt[2] > ov_recs_cap || cnt[1] > (uint64_t)n) {
-                release();
-                return fail("aggregate spill overflow (%u rows, %u records)", cnt[1], cnt[2]);
-            }
-            const unsigned int spilled_rows = cnt[1];
-            const unsigned int spilled_recs = cnt[2];
-            if (!table_grow(T, (uint64_t)T->groups + spilled_rows + spilled_recs + 1024)) { release(); return false; }
-            const int nxt = cur ^ 1;
-            table_args(T, &A);
-            A.ov_rows = ov_rows[nxt];
-            A.ov_recs = ov_recs[nxt];
-            if (!set_counters(T)) { release(); return false; }
-            if (spilled_rows) {
-                AggArgs B = A;
-                B.row_index = ov_rows[cur];
-                B.nrows = spilled_rows;
-                B.row_base = 0;
-                B.lds_slots = 0;
-                B.lds_groups = 0;
-                const int g2 = grid_for(spilled_rows, kBlock, 2);
-                agg::launch_update(KW, S, B, g2, 0);
-            }
-            if (spilled_recs) {
-                agg::launch_records(KW, S, A, ov_recs[cur], spilled_recs);
-            }
-            if (!hip_ok(hipGetLastError(), "agg replay launch")) { release(); return false; }
-            if (!read_counters(T, cnt)) { release(); return false; }
-            cur = nxt;
-        }
-        release();
+        const bool ok = hip_ok(launched, "agg update launch") && hip_ok(hipGetLastError(), "agg update launch") &&
+            read_counters(T, cnt) && settle_launch(&L, cnt);
+        L.release();
+        if (!ok) return false;
     }
-    // Keep the load at most one half for the next batch.
-    if ((uint64_t)T->groups > T->capacity / 2 && !table_grow(T, (uint64_t)T->groups)) return false;
     return true;
+}
+
+// Rows per work item of a batched update.
+constexpr int64_t kBatchItemRows = 32768;
+
+// Updates tables[i] with batches[i] for every i in ONE persistent launch:
+// workgroups take (instance, row range) items from a device queue. Leftover
+// rows and records are replayed per table afterwards, as after a plain
+// update. Batches above 2^30 rows take the plain path.
+bool batched_update(AggPlan* P, AggTable* const* tables, const TRowSet* const* batches, int64_t n)
+{
+    const int KW = P->KW, S = P->S;
+    std::vector<UpdateLaunch> launches;
+    std::vector<AggArgs> insts;
+    std::vector<BatchItem> items;
+    std::vector<int64_t> which;
+    launches.reserve((size_t)n);
+    size_t lds = 0;
+    bool ok = true;
+    auto release_all = [&]() {
+        for (auto& L : launches) L.release();
+    };
+    for (int64_t i = 0; i < n && ok; ++i) {
+        AggTable* T = tables[i];
+        const TRowSet* b = batches[i];
+        if (b->RowCount <= 0) continue;
+        if (b->RowCount > ((int64_t)1 << 30)) {
+            ok = table_update(T, b);
+            continue;
+        }
+        AggArgs bound;
+        if (!bind_batch(P, b, &bound)) {
+            ok = false;
+            break;
+        }
+        lds_layout(P, &bound, &lds);
+        const int64_t nitems = (b->RowCount + kBatchItemRows - 1) / kBatchItemRows;
+        launches.emplace_back();
+        if (!prepare_launch(T, bound, 0, b->RowCount, (size_t)nitems * (bound.lds_groups ? bound.lds_groups : 1) + 1024,
+                &launches.back()) ||
+            !set_counters(T)) {
+            launches.pop_back();
+            ok = false;
+            break;
+        }
+        const int inst = (int)insts.size();
+        insts.push_back(launches.back().A);
+        which.push_back(i);
+        for (int64_t r = 0; r < b->RowCount; r += kBatchItemRows) {
+            BatchItem it;
+            it.inst = inst;
+            it.pad = 0;
+            it.row0 = r;
+            it.row1 = r + kBatchItemRows < b->RowCount ? r + kBatchItemRows : b->RowCount;
+            items.push_back(it);
+        }
+    }
+    if (!ok || insts.empty()) {
+        release_all();
+        return ok;
+    }
+    AggArgs* d_insts = (AggArgs*)upload(insts.data(), insts.size() * sizeof(AggArgs));
+    BatchItem* d_items = (BatchItem*)upload(items.data(), items.size() * sizeof(BatchItem));
+    unsigned int* d_head = (unsigned int*)dalloc(16);
+    ok = d_insts && d_items && d_head && zero_fill(d_head, 16);
+    if (ok) {
+        const int nitems = (int)items.size();
+        const int grid = nitems < num_cus() * 2 ? nitems : num_cus() * 2;
+        hipFunction_t fn = plan_jit_function(P, "qdb_agg_batch");
+        hipError_t launched = hipSuccess;
+        {
+            TimedRegion timer("agg_batch");
+            if (fn) {
+                void* args[] = {&d_insts, &d_items, (void*)&nitems, &d_head};
+                launched = hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, kBlock, 1, 1, (unsigned)lds, stream(), args, nullptr);
+            } else {
+                agg::launch_batch(KW, S, d_insts, d_items, nitems, d_head, grid, lds);
+            }
+        }
+        ok = hip_ok(launched, "agg batch launch") && hip_ok(hipGetLastError(), "agg batch launch");
+    }
+    // Every table's counters in one synchronisation.
+    std::vector<unsigned int> cnt(launches.size() * 4, 0u);
+    for (size_t k = 0; ok && k < launches.size(); ++k) {
+        ok = hip_ok(hipMemcpyAsync(&cnt[k * 4], launches[k].T->counters, 16, hipMemcpyDeviceToHost, stream()), "counters");
+    }
+    ok = ok && sync("agg batch");
+    for (size_t k = 0; ok && k < launches.size(); ++k) {
+        AggTable* T = launches[k].T;
+        T->groups = cnt[k * 4];
+        T->extra_used = cnt[k * 4 + 3];
+        ok = settle_launch(&launches[k], &cnt[k * 4]);
+    }
+    release_all();
+    dfree(d_insts);
+    dfree(d_items);
+    dfree(d_head);
+    return ok;
 }
 
 // agg_finish_rowset: groups -> columnar TRowSet; destroys the table.
@@ -1217,6 +1370,31 @@ extern "C" int qdb_gpu_compile_aggregate(const QdbAggregateSpec* spec, QdbAggreg
     return 0;
 }
 
+extern "C" int qdb_gpu_agg_update_batched(const QdbAggregateKernels* kernels, void* const* hts, const TRowSet* const* batches,
+    int64_t n)
+{
+    clear_error();
+    if (!kernels || (n > 0 && (!hts || !batches))) return fail("null argument"), -1;
+    if (kernels->Slot < 0 || kernels->Slot >= kSlots) return fail("bad aggregate slot %lld", (long long)kernels->Slot), -1;
+    AggPlan* P = agg_slots().get((int)kernels->Slot);
+    if (!P) return fail("aggregate kernels are not compiled"), -1;
+    std::vector<AggTable*> tables((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        HtBlob* B = (HtBlob*)hts[i];
+        if (!B || !B->table) return fail("batched update of table %lld before Init", (long long)i), -1;
+        if (B->table->plan != P) return fail("table %lld belongs to another aggregate", (long long)i), -1;
+        if (!batches[i]) return fail("null batch %lld", (long long)i), -1;
+        tables[(size_t)i] = B->table;
+    }
+    if (!batched_update(P, tables.data(), batches, n)) return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        HtBlob* B = (HtBlob*)hts[i];
+        B->Size = B->table->groups;
+        B->Capacity = (int64_t)B->table->capacity;
+    }
+    return 0;
+}
+
 extern "C" int64_t qdb_gpu_aggregate_kernel_source(const QdbAggregateSpec* spec, char* out, int64_t cap)
 {
     clear_error();
@@ -1272,6 +1450,10 @@ void launch_update_kw1(int S, const AggArgs& A, int grid, size_t lds);
 void launch_update_kw2(int S, const AggArgs& A, int grid, size_t lds);
 void launch_update_kw3(int S, const AggArgs& A, int grid, size_t lds);
 void launch_update_kw4(int S, const AggArgs& A, int grid, size_t lds);
+void launch_batch_kw1(int S, const AggArgs* insts, const BatchItem* items, int nitems, unsigned int* head, int grid, size_t lds);
+void launch_batch_kw2(int S, const AggArgs* insts, const BatchItem* items, int nitems, unsigned int* head, int grid, size_t lds);
+void launch_batch_kw3(int S, const AggArgs* insts, const BatchItem* items, int nitems, unsigned int* head, int grid, size_t lds);
+void launch_batch_kw4(int S, const AggArgs* insts, const BatchItem* items, int nitems, unsigned int* head, int grid, size_t lds);
 void launch_records_kw1(int S, const AggArgs& A, const uint64_t* recs, unsigned int n);
 void launch_records_kw2(int S, const AggArgs& A, const uint64_t* recs, unsigned int n);
 void launch_records_kw3(int S, const AggArgs& A, const uint64_t* recs, unsigned int n);
@@ -1288,6 +1470,16 @@ void launch_update(int KW, int S, const AggArgs& A, int grid, size_t lds)
     case 2: launch_update_kw2(S, A, grid, lds); break;
     case 3: launch_update_kw3(S, A, grid, lds); break;
     default: launch_update_kw4(S, A, grid, lds); break;
+    }
+}
+
+void launch_batch(int KW, int S, const AggArgs* insts, const BatchItem* items, int nitems, unsigned int* head, int grid, size_t lds)
+{
+    switch (KW) {
+    case 1: launch_batch_kw1(S, insts, items, nitems, head, grid, lds); break;
+    case 2: launch_batch_kw2(S, insts, items, nitems, head, grid, lds); break;
+    case 3: launch_batch_kw3(S, insts, items, nitems, head, grid, lds); break;
+    default: launch_batch_kw4(S, insts, items, nitems, head, grid, lds); break;
     }
 }
 

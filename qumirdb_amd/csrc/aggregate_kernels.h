@@ -260,11 +260,10 @@ __device__ __forceinline__ uint64_t word_contrib(const WordDesc& d, const DVal& 
     case WK_I128_P1: return (uint64_t)u >> 32;
     case WK_I128_P2: return (uint64_t)(u >> 64);
     case WK_LIMB: {
-        int64_t l = limb[0];
-#pragma unroll
-        for (int k = 1; k < kLimbs; ++k) {
-            if (d.part == k) l = limb[k];
-        }
+        // Constant indices only: a limb picked by a run-time index would put
+        // the array in scratch memory.
+        static_assert(kLimbs == 4, "limb selection below");
+        const int64_t l = d.part == 0 ? limb[0] : d.part == 1 ? limb[1] : d.part == 2 ? limb[2] : limb[3];
         return (uint64_t)l;
     }
     case WK_OUTSIDE: return outside ? (uint64_t)__double_as_longlong(v.f) : 0ull;
@@ -654,40 +653,43 @@ __device__ __forceinline__ uint64_t wave_reduce(uint8_t op, uint64_t v, bool in)
     return x;
 }
 
-// The update of one launch: every selected row of [row_base, row_base +
-// nrows) (or of row_index), pre-aggregated in the workgroup's LDS table and
-// then in the HBM table. The generic kernel below reads the descriptors
-// from its arguments; kernels specialized at run time call it with the
-// descriptors set to constants.
-template <int KW, int S, class P>
-__device__ __forceinline__ void agg_update_body(const AggArgs& A)
+// The workgroup's LDS table laid out for A (records, then the control
+// words, then the group count).
+template <int KW, class P>
+__device__ __forceinline__ LdsTable lds_table_at(const AggArgs& A, uint64_t* lds_raw, unsigned int* inserted)
 {
-    extern __shared__ uint64_t lds_raw[];
-    __shared__ unsigned int s_inserted;
     LdsTable T;
-    T.recs = nullptr;
-    T.ctrl = nullptr;
-    T.count = nullptr;
-    T.inserted = &s_inserted;
-    if (threadIdx.x == 0) s_inserted = 0u;
-    const bool use_lds = A.lds_slots > 0 && A.row_index == nullptr;
+    T.recs = lds_raw;
+    T.ctrl = (unsigned int*)(lds_raw + (size_t)A.lds_groups * (KW + P::nwords(A)));
+    T.count = T.ctrl + A.lds_slots;
+    T.inserted = inserted;
+    return T;
+}
+
+// Clears the LDS table and the inserted-group counter (every thread; ends
+// with a barrier).
+template <int KW, class P>
+__device__ __forceinline__ void lds_table_reset(const AggArgs& A, const LdsTable& T, bool use_lds)
+{
+    if (threadIdx.x == 0) *T.inserted = 0u;
     if (use_lds) {
-        T.recs = lds_raw;
-        T.ctrl = (unsigned int*)(lds_raw + (size_t)A.lds_groups * (KW + P::nwords(A)));
-        T.count = T.ctrl + A.lds_slots;
         for (int i = threadIdx.x; i < A.lds_slots; i += blockDim.x) T.ctrl[i] = 0u;
         if (threadIdx.x == 0) *T.count = 0u;
     }
     __syncthreads();
+}
+
+// Rows idx in [wave_first + lane, end) stepping by `step` (row = row_base +
+// idx, or row_index[idx]), pre-aggregated in the LDS table and then in the
+// HBM table. Every lane of a wave runs the same number of iterations: the
+// ballots and the retry loops inside see the whole wave.
+template <int KW, int S, class P>
+__device__ __forceinline__ void agg_rows(const AggArgs& A, LdsTable& T, bool use_lds, int64_t wave_first, int64_t end, int64_t step)
+{
     const int lane = lane_id();
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t start = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    // Every lane of a wave runs the same number of iterations: the ballots
-    // and the retry loops below see the whole wave.
-    const int64_t wave_start = start - lane;
-    for (int64_t base = wave_start; base < A.nrows; base += stride) {
+    for (int64_t base = wave_first; base < end; base += step) {
         const int64_t idx = base + lane;
-        bool valid = idx < A.nrows;
+        bool valid = idx < end;
         int64_t row = 0;
         if (valid) row = A.row_index ? (int64_t)A.row_index[idx] : A.row_base + idx;
         uint64_t key[KW];
@@ -728,10 +730,17 @@ __device__ __forceinline__ void agg_update_body(const AggArgs& A)
         }
         if (!apply_contrib<KW, S, P>(A, T, use_lds, valid, key, c)) spill_row(A, row);
     }
+}
+
+// Moves the workgroup's LDS groups into the HBM table and adds the groups
+// the workgroup inserted to the table's count (every thread; barriers
+// inside, the last one before the table may be reused).
+template <int KW, int S, class P>
+__device__ __forceinline__ void lds_table_flush(const AggArgs& A, LdsTable& T, bool use_lds)
+{
     __syncthreads();
     if (use_lds) {
-        // Flush the workgroup's groups to the HBM table; every thread runs
-        // the same number of iterations.
+        // Every thread runs the same number of iterations.
         const unsigned int n = *T.count < (unsigned int)A.lds_groups ? *T.count : (unsigned int)A.lds_groups;
         const int R = KW + P::nwords(A);
         for (unsigned int b = 0; b < n; b += blockDim.x) {
@@ -749,13 +758,81 @@ __device__ __forceinline__ void agg_update_body(const AggArgs& A)
         __syncthreads();
     }
     // One atomic per workgroup for the groups it inserted.
-    if (threadIdx.x == 0 && s_inserted) atomicAdd(&A.counters[0], s_inserted);
+    if (threadIdx.x == 0 && *T.inserted) atomicAdd(&A.counters[0], *T.inserted);
+    __syncthreads();
+}
+
+// The update of one launch: every selected row of [row_base, row_base +
+// nrows) (or of row_index), pre-aggregated in the workgroup's LDS table and
+// then in the HBM table. The generic kernel below reads the descriptors
+// from its arguments; kernels specialized at run time call it with the
+// descriptors set to constants.
+template <int KW, int S, class P>
+__device__ __forceinline__ void agg_update_body(const AggArgs& A)
+{
+    extern __shared__ uint64_t lds_raw[];
+    __shared__ unsigned int s_inserted;
+    const bool use_lds = A.lds_slots > 0 && A.row_index == nullptr;
+    LdsTable T = lds_table_at<KW, P>(A, lds_raw, &s_inserted);
+    lds_table_reset<KW, P>(A, T, use_lds);
+    const int64_t start = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    agg_rows<KW, S, P>(A, T, use_lds, start - lane_id(), A.nrows, (int64_t)gridDim.x * blockDim.x);
+    lds_table_flush<KW, S, P>(A, T, use_lds);
+}
+
+// One work item of a batched update: rows [row0, row1) of instance `inst`.
+struct BatchItem {
+    int32_t inst;
+    int32_t pad;
+    int64_t row0;
+    int64_t row1;
+};
+
+// Many aggregate instances of one layout in one launch: persistent
+// workgroups take work items from a device queue (one atomic per item);
+// a workgroup keeps one instance's groups in its LDS table while its items
+// stay on that instance and moves them to that instance's HBM table when
+// it changes instance and at the end. Items come instance by instance.
+template <int KW, int S, class P>
+__device__ __forceinline__ void agg_batch_body(const AggArgs* __restrict__ insts, const BatchItem* __restrict__ items, int nitems,
+    unsigned int* head)
+{
+    extern __shared__ uint64_t lds_raw[];
+    __shared__ unsigned int s_inserted;
+    __shared__ int s_item;
+    LdsTable T = lds_table_at<KW, P>(insts[0], lds_raw, &s_inserted);
+    int cur = -1;
+    bool use_lds = false;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(head, 1u);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();  // every thread has read s_item before it is written again
+        if (it >= nitems) break;
+        const BatchItem w = items[it];
+        const int inst = __builtin_amdgcn_readfirstlane(w.inst);
+        if (inst != cur) {
+            if (cur >= 0) lds_table_flush<KW, S, P>(insts[cur], T, use_lds);
+            cur = inst;
+            use_lds = insts[cur].lds_slots > 0;
+            T = lds_table_at<KW, P>(insts[cur], lds_raw, &s_inserted);
+            lds_table_reset<KW, P>(insts[cur], T, use_lds);
+        }
+        agg_rows<KW, S, P>(insts[cur], T, use_lds, w.row0 + (int64_t)threadIdx.x - lane_id(), w.row1, (int64_t)blockDim.x);
+    }
+    if (cur >= 0) lds_table_flush<KW, S, P>(insts[cur], T, use_lds);
 }
 
 template <int KW, int S>
 __global__ void __launch_bounds__(kBlock) k_agg_update(AggArgs A)
 {
     agg_update_body<KW, S, GenericPolicy>(A);
+}
+
+template <int KW, int S>
+__global__ void __launch_bounds__(kBlock) k_agg_batch(const AggArgs* insts, const BatchItem* items, int nitems, unsigned int* head)
+{
+    agg_batch_body<KW, S, GenericPolicy>(insts, items, nitems, head);
 }
 
 // Replays spilled records (key + words) into the grown HBM table.
@@ -863,6 +940,7 @@ struct FinArgs {
 #ifndef __HIPCC_RTC__
 // Launchers, one translation unit per key width (aggregate_kw<KW>.hip).
 void launch_update(int KW, int S, const AggArgs& A, int grid, size_t lds);
+void launch_batch(int KW, int S, const AggArgs* insts, const BatchItem* items, int nitems, unsigned int* head, int grid, size_t lds);
 void launch_records(int KW, int S, const AggArgs& A, const uint64_t* recs, unsigned int n);
 void launch_rehash(int KW, const AggArgs& A, const uint64_t* old_recs, const unsigned long long* old_ctrl, uint64_t old_slots);
 #endif

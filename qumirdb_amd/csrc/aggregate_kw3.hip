@@ -14,6 +14,15 @@ void launch_update_kw3(int S, const AggArgs& A, int grid, size_t lds)
     }
 }
 
+void launch_batch_kw3(int S, const AggArgs* insts, const BatchItem* items, int nitems, unsigned int* head, int grid, size_t lds)
+{
+    switch (S) {
+    case 8: hipLaunchKernelGGL((k_agg_batch<3, 8>), dim3(grid), dim3(kBlock), lds, stream(), insts, items, nitems, head); break;
+    case 16: hipLaunchKernelGGL((k_agg_batch<3, 16>), dim3(grid), dim3(kBlock), lds, stream(), insts, items, nitems, head); break;
+    default: hipLaunchKernelGGL((k_agg_batch<3, 32>), dim3(grid), dim3(kBlock), lds, stream(), insts, items, nitems, head); break;
+    }
+}
+
 void launch_records_kw3(int S, const AggArgs& A, const uint64_t* recs, unsigned int n)
 {
     const int grid = grid_for(n, kBlock, 8);

@@ -78,6 +78,20 @@ void TAggregateProcessor::Add(TRowSet& rowSet)
     if (Kernels_.Dispatch(HashTable_.data(), &rowSet, 0, QDB_AGG_OP_UPDATE) < 0) throw Error("aggregate update failed");
 }
 
+void TAggregateProcessor::AddBatched(TAggregateProcessor* const* processors, TRowSet* const* rowSets, int64_t n)
+{
+    if (n <= 0) return;
+    std::vector<void*> tables((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        TAggregateProcessor* p = processors[i];
+        if (p->Finished_) throw std::runtime_error("aggregate processor is already finished");
+        if (p->Kernels_.Slot != processors[0]->Kernels_.Slot) throw std::invalid_argument("batched processors of different aggregates");
+        p->EnsureInit();
+        tables[(size_t)i] = p->HashTable_.data();
+    }
+    if (qdb_gpu_agg_update_batched(&processors[0]->Kernels_, tables.data(), rowSets, n) != 0) throw Error("batched aggregate update failed");
+}
+
 bool TAggregateProcessor::Finish(TRowSet& rowSet)
 {
     if (Finished_) return false;
@@ -331,6 +345,19 @@ int qdb_gpu_agg_processor_add(void* p, const TRowSet* batch)
     return Guard([&] {
         TRowSet v = View(batch);
         static_cast<NQdbGpu::TAggregateProcessor*>(p)->Add(v);
+    });
+}
+
+int qdb_gpu_agg_processor_add_batched(void* const* procs, const TRowSet* const* batches, int64_t n)
+{
+    return Guard([&] {
+        std::vector<TRowSet> views((size_t)(n > 0 ? n : 0));
+        std::vector<TRowSet*> ptrs((size_t)(n > 0 ? n : 0));
+        for (int64_t i = 0; i < n; ++i) {
+            views[(size_t)i] = View(batches[i]);
+            ptrs[(size_t)i] = &views[(size_t)i];
+        }
+        NQdbGpu::TAggregateProcessor::AddBatched(reinterpret_cast<NQdbGpu::TAggregateProcessor* const*>(procs), ptrs.data(), n);
     });
 }
 

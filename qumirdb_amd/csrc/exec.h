@@ -48,6 +48,9 @@ public:
     TAggregateProcessor& operator=(const TAggregateProcessor&) = delete;
 
     void Add(TRowSet& rowSet);
+    // Updates many processors of one compiled aggregate, processors[i] with
+    // rowSets[i], in one batched launch (qdb_gpu_agg_update_batched).
+    static void AddBatched(TAggregateProcessor* const* processors, TRowSet* const* rowSets, int64_t n);
     bool Finish(TRowSet& rowSet);
     int64_t Groups() const;
     bool Flush(TRowSet& rowSet);

@@ -31,9 +31,9 @@ namespace {
 
 struct JitEntry {
     hipModule_t module = nullptr;
-    hipFunction_t fn = nullptr;
     std::string error;
     double compile_ms = 0.0;
+    std::unordered_map<std::string, hipFunction_t> fns;
 };
 
 struct JitCache {
@@ -105,32 +105,43 @@ hipFunction_t jit_function(const std::string& src, const char* name, std::string
     JitCache& c = cache();
     std::lock_guard<std::mutex> lock(c.mu);
     auto it = c.map.find(src);
-    if (it != c.map.end()) {
-        if (err) *err = it->second.error;
-        return it->second.fn;
-    }
-    JitEntry e;
-    const auto t0 = std::chrono::steady_clock::now();
-    std::vector<char> code;
-    if (compile_source(src, device_arch(), &code, &e.error)) {
-        hipError_t h = hipModuleLoadData(&e.module, code.data());
-        if (h == hipSuccess) h = hipModuleGetFunction(&e.fn, e.module, name);
-        if (h != hipSuccess) {
-            e.error = std::string("module load: ") + hipGetErrorString(h);
-            e.fn = nullptr;
-            (void)hipGetLastError();
+    if (it == c.map.end()) {
+        JitEntry e;
+        const auto t0 = std::chrono::steady_clock::now();
+        std::vector<char> code;
+        if (compile_source(src, device_arch(), &code, &e.error)) {
+            const hipError_t h = hipModuleLoadData(&e.module, code.data());
+            if (h != hipSuccess) {
+                e.error = std::string("module load: ") + hipGetErrorString(h);
+                e.module = nullptr;
+                (void)hipGetLastError();
+            }
         }
+        e.compile_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        c.compiles += 1;
+        c.total_ms += e.compile_ms;
+        const char* log = getenv("QDB_GPU_JIT_LOG");
+        if (log && log[0] == '1') {
+            fprintf(stderr, "[qdb_gpu jit] compiled in %.0f ms%s%s\n", e.compile_ms, e.module ? "" : ", failed: ",
+                e.module ? "" : e.error.c_str());
+        }
+        it = c.map.emplace(src, std::move(e)).first;
     }
-    e.compile_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    c.compiles += 1;
-    c.total_ms += e.compile_ms;
-    const char* dump = getenv("QDB_GPU_JIT_LOG");
-    if (dump && dump[0] == '1') {
-        fprintf(stderr, "[qdb_gpu jit] %s: %.0f ms%s%s\n", name, e.compile_ms, e.fn ? "" : ", failed: ", e.fn ? "" : e.error.c_str());
+    JitEntry& e = it->second;
+    if (!e.module) {
+        if (err) *err = e.error;
+        return nullptr;
     }
-    if (err) *err = e.error;
-    hipFunction_t fn = e.fn;
-    c.map.emplace(src, std::move(e));
+    auto f = e.fns.find(name);
+    if (f != e.fns.end()) return f->second;
+    hipFunction_t fn = nullptr;
+    const hipError_t h = hipModuleGetFunction(&fn, e.module, name);
+    if (h != hipSuccess) {
+        (void)hipGetLastError();
+        if (err) *err = std::string("no kernel ") + name;
+        fn = nullptr;
+    }
+    e.fns.emplace(name, fn);
     return fn;
 }
 

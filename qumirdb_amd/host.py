@@ -490,6 +490,18 @@ class TAggregateProcessor:
         if lib().qdb_gpu_agg_processor_add(self.p, batch.ptr) != 0:
             raise RuntimeError((lib().qdb_gpu_exec_last_error() or b"").decode())
 
+    @staticmethod
+    def add_batched(processors: Sequence["TAggregateProcessor"], batches: Sequence[DeviceRowSet]) -> None:
+        """processors[i].add(batches[i]) for every i, as ONE batched launch
+        (all processors of one compiled aggregate)."""
+        n = len(processors)
+        if n != len(batches):
+            raise ValueError("one batch per processor")
+        procs = (C.c_void_p * max(n, 1))(*[p.p for p in processors])
+        rows = (C.POINTER(abi.TRowSet) * max(n, 1))(*[C.pointer(b.rs) for b in batches])
+        if lib().qdb_gpu_agg_processor_add_batched(procs, rows, n) != 0:
+            raise RuntimeError((lib().qdb_gpu_exec_last_error() or b"").decode())
+
     def groups(self) -> int:
         return lib().qdb_gpu_agg_processor_groups(self.p)
 

@@ -181,6 +181,15 @@ def zipf1_cdf(n: int) -> np.ndarray:
     return cdf
 
 
+def zipf_cdf(n: int, s: float) -> np.ndarray:
+    """Zipf(s) CDF over ranks 1..n (s = 0: uniform)."""
+    w = np.arange(1, n + 1, dtype=np.float64) ** (-s)
+    h = np.cumsum(w)
+    cdf = h / h[-1]
+    cdf[-1] = 1.0
+    return cdf
+
+
 class GroupsDevice:
     def __init__(self, rows: int, distinct: int, seed: int = SEED_GROUPS, cdf: np.ndarray | None = None):
         self.rows, self.distinct = rows, distinct
@@ -192,3 +201,32 @@ class GroupsDevice:
         check(lib().qdb_gpu_gen_zipf_groups(seed, 0, rows, self.cdf.ptr, distinct, self.key.ptr, self.vi.ptr, self.vf.ptr),
               "gen_groups")
         self.rowset = DeviceRowSet(GROUPS_TYPES, [self.key, self.vi, self.vf], rows=rows)
+
+
+# --------------------------------------------------------------------------
+# batched form: many group-by instances of one aggregate
+# --------------------------------------------------------------------------
+
+SEED_BATCHED = 0xC3
+
+
+def batched_instances(n: int = 512, seed: int = SEED_BATCHED, rows_lo: int = 10_000, rows_hi: int = 20_000_000,
+                      card_lo: int = 4, card_hi: int = 1_000_000):
+    """(rows, cardinality, zipf exponent) per instance: rows and cardinality
+    log-uniform, keys uniform or Zipf(1.1) over the cardinality."""
+    rng = np.random.default_rng(seed)
+    rows = np.exp(rng.uniform(np.log(rows_lo), np.log(rows_hi), n)).astype(np.int64)
+    card = np.exp(rng.uniform(np.log(card_lo), np.log(card_hi), n)).astype(np.int64)
+    s = np.where(rng.random(n) < 0.5, 0.0, 1.1)
+    return [(int(r), int(c), float(z)) for r, c, z in zip(rows, card, s)]
+
+
+class BatchedDevice:
+    """Every instance's group-by columns (int64 key, int64 and f64 values,
+    as the high-cardinality shape) generated in HBM."""
+
+    def __init__(self, instances, seed: int = SEED_BATCHED):
+        self.instances = list(instances)
+        self.devs = []
+        for i, (rows, card, s) in enumerate(self.instances):
+            self.devs.append(GroupsDevice(rows, card, seed=seed * 1000003 + i, cdf=zipf_cdf(card, s)))

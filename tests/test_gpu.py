@@ -110,6 +110,48 @@ def test_generated_kernel_equals_generic(gpu_backend, monkeypatch):
     assert outs[0] == outs[1]
 
 
+def test_batched_update_equals_plain(gpu_backend):
+    """Many tables of one aggregate updated in one persistent launch give
+    exactly the results of one update per table: instances from empty to
+    300K rows, few to many groups, nullable multi-column keys, every
+    reducer; the small initial capacities make several tables grow and
+    replay their leftover rows inside the batched call."""
+    import qumirdb_amd as q
+    from cases import random_agg_case
+    sizes = [0, 1, 7, 100, 2000, 5000, 33000, 65536, 100000, 300000, 50, 40000]
+    insts = [random_agg_case(11 + i, n) for i, n in enumerate(sizes)]
+    case = insts[0][0]
+    kern = q.AggregateKernels.__new__(q.AggregateKernels)
+    kern.spec = None
+    kern.c_spec = gpu_backend.cast(case.to_c())
+    kern.k = q.abi.QdbAggregateKernels()
+    q.check(q.lib().qdb_gpu_compile_aggregate(C.byref(kern.c_spec), C.byref(kern.k)), "compile_aggregate")
+    kern.out_types = [gpu_backend.H.ValueType(t.Kind, t.Width, t.Scale, bool(t.Nullable))
+                      for t in kern.k.OutputTypes[: kern.k.NumOutputColumns]]
+    devs = [gpu_backend.dev(b) for _, b in insts]
+    caps = [8 if i % 3 == 0 else 1 << 16 for i in range(len(insts))]
+    plain = []
+    for d, cap in zip(devs, caps):
+        p = q.TAggregateProcessor(kern, initial_capacity=cap)
+        p.add(d)
+        plain.append(sorted(result_rows(p.finish()), key=sort_key))
+        p.close()
+    procs = [q.TAggregateProcessor(kern, initial_capacity=cap) for cap in caps]
+    q.TAggregateProcessor.add_batched(procs, devs)
+    # a second batched round on the same tables accumulates
+    q.TAggregateProcessor.add_batched(procs[:3], devs[:3])
+    for i, p in enumerate(procs):
+        got = sorted(result_rows(p.finish()), key=sort_key)
+        p.close()
+        if i < 3:
+            _, b = insts[i]
+            want = sorted(result_rows(OracleBackend(fsum_mode=2).aggregate(case, [b, b])), key=sort_key)
+        else:
+            want = plain[i]
+        same(got, want)
+    q.lib().qdb_gpu_release_slot(q.abi.SLOT_AGGREGATE, kern.k.Slot)
+
+
 def test_aggregate_batches_accumulate(gpu_backend):
     case, b = random_agg_case(5, 30000)
     half = 15000
