@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["join", "lineitem", "groups", "batched", "all"], default="all")
+    ap.add_argument("--workload", choices=["join", "lineitem", "groups", "batched", "served", "all"], default="all")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (3 launches), the known-size kernel that "
@@ -243,6 +243,32 @@ def bench_batched(n_instances=512):
             "plain_update_kernels_ms": upd_ms * upd_n, "batched_update_kernel_ms": bat_ms}
 
 
+def bench_served(qps_levels=(50, 100, 200, 500, 1000, 2000), n_requests=120):
+    """The served form: Poisson request streams at each offered QPS over the
+    resident tables (lineitem 6M rows, join build 1.5M / probe 60M rows,
+    groups 100M rows; each request one shape over a log-uniform 1%..100%
+    row range), continuous batching versus one request at a time, with the
+    same results checked request by request."""
+    from qumirdb_amd import serve as SV
+    res = SV.Resident()
+    srv = SV.Server(res)
+    warm = SV.make_stream(res, 12, 1e6, seed=1)
+    srv.serve(warm, batched=True)
+    srv.serve(SV.fresh(warm), batched=False)
+    levels = []
+    for k, qps in enumerate(qps_levels):
+        stream = SV.make_stream(res, n_requests, qps, seed=0xC4 + k)
+        plain = srv.serve(SV.fresh(stream), batched=False)
+        batched = srv.serve(SV.fresh(stream), batched=True)
+        for a, b in zip(plain, batched):
+            if a.result != b.result:
+                raise RuntimeError(f"request {a.rid} ({a.shape}) differs between plain and batched serving")
+        levels.append({"offered_qps": qps, "plain": SV.summarize(plain), "batched": SV.summarize(batched)})
+    res.close()
+    return {"levels": levels, "requests_per_level": n_requests,
+            "resident_bytes": res.table_bytes()}
+
+
 def cpu_join_baseline(build_rows=1_500_000, sample_probe=60_000_000):
     """The oracle's row-table join (one thread) over the filtered build and
     `sample_probe` probe rows (by default the whole workload)."""
@@ -319,9 +345,14 @@ def main():
         out["groups"] = bench_groups(max(1, args.steps // 2), max(1, args.warmup // 2))
     if args.workload in ("batched", "all"):
         out["batched"] = bench_batched()
+    if args.workload in ("served", "all"):
+        out["served"] = bench_served()
     if args.copy_calibration:
         out["copy_calibration"] = copy_calibration(args.copy_calibration)
     head = out.get("join") or out.get("lineitem") or out.get("groups") or out.get("batched")
+    if head is None:  # served only: its best sustained throughput
+        best = max(out["served"]["levels"], key=lambda l: l["batched"]["qps"])["batched"]
+        head = {"rows_per_s": best["qps"], "ms_per_step": best["p50_ms"]}
     name = "join" if "join" in out else list(out)[0]
     cpu = None
     if not args.no_cpu and name == "join":

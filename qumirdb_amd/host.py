@@ -311,6 +311,37 @@ class DeviceRowSet:
             self.sel_buf = DeviceBuffer.from_numpy(np.asarray(selection, dtype=np.uint8))
             self.rs.Selection = self.sel_buf.ptr
 
+    def view(self, r0: int, r1: int) -> "DeviceRowSet":
+        """Rows [r0, r1) of this batch without a copy: data pointers
+        advanced, bit-packed data and validity by their bit offsets."""
+        if not 0 <= r0 <= r1 <= self.rows:
+            raise ValueError(f"rows [{r0}, {r1}) outside [0, {self.rows})")
+        v = DeviceRowSet.__new__(DeviceRowSet)
+        v.types = self.types
+        v.buffers = [self]  # keeps the parent's buffers alive
+        cols = (abi.TColumn * max(1, len(self.types)))()
+        for i, t in enumerate(self.types):
+            c = self.cols[i]
+            if t.kind == abi.BOOL:
+                cols[i].Data = c.Data
+                cols[i].DataBitOffset = c.DataBitOffset + r0
+            else:
+                cols[i].Data = (c.Data or 0) + r0 * t.width
+            if c.Mask:
+                cols[i].Mask = c.Mask
+                cols[i].MaskBitOffset = c.MaskBitOffset + r0
+        v.rows = r1 - r0
+        v.cols = cols
+        v.rs = abi.TRowSet()
+        v.rs.Columns = C.cast(cols, C.POINTER(abi.TColumn))
+        v.rs.ColumnCount = len(self.types)
+        v.rs.RowCount = v.rows
+        v.rs.RefCount = 1
+        v.sel_buf = None
+        if self.rs.Selection:
+            v.rs.Selection = self.rs.Selection + r0
+        return v
+
     def set_selection_buffer(self, buf: Optional[DeviceBuffer]) -> None:
         self.sel_buf = buf
         self.rs.Selection = buf.ptr if buf is not None else None

@@ -476,3 +476,30 @@ def test_groups_vs_oracle(gpu_backend, rows, distinct):
     rel = np.abs(gs - ks) / np.maximum(np.abs(ks), 1e-300)
     assert (rel <= F64_REL_TOL).all(), float(rel.max())
     assert np.array_equal(gs, np.asarray(exact[5][0])[ei])
+
+
+def test_served_results_same_alone_and_in_the_mix(gpu_backend):
+    """Requests served with continuous batching (every request that has
+    arrived runs in the next round; group-bys of a shape share one batched
+    launch) return exactly what each returns alone, one at a time."""
+    from qumirdb_amd import serve as SV
+    res = SV.Resident(lineitem_rows=200_000, build_rows=50_000, probe_rows=400_000, group_rows=300_000, distinct=40_000)
+    try:
+        srv = SV.Server(res, full_results=True)
+        stream = SV.make_stream(res, 36, 20_000.0, seed=7)
+        mixed = srv.serve(SV.fresh(stream), batched=True)
+        alone = srv.serve(SV.fresh(stream), batched=False)
+        assert {r.shape for r in stream} == set(SV.SHAPES)
+        for a, b in zip(alone, mixed):
+            assert a.result == b.result, f"request {a.rid} ({a.shape}, rows {a.r0}..{a.r1})"
+        # and a join request's fused aggregate against the oracle's pairs
+        j = next(r for r in alone if r.shape == "join")
+        fk = res.jd.fk.to_numpy(np.int64, res.jd.probe_rows)[j.r0:j.r1]
+        key = res.jd.key.to_numpy(np.int64, res.jd.build_rows)
+        seg = res.jd.segment.to_numpy(np.int32, res.jd.build_rows)
+        val = res.jd.value.to_numpy(np.int64, res.jd.probe_rows)[j.r0:j.r1]
+        build = set(key[seg == 1].tolist())
+        hit = np.array([k in build for k in fk.tolist()], dtype=bool)
+        assert j.result == (int(hit.sum()), int(hit.sum()), int(val[hit].sum()))
+    finally:
+        res.close()
