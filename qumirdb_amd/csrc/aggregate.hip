@@ -29,6 +29,7 @@
 #include <memory>
 #include <mutex>
 #include <utility>
+#include <algorithm>
 #include <vector>
 
 #include "aggregate_kernels.h"
@@ -42,6 +43,17 @@ namespace {
 using namespace agg;
 
 typedef long long ll2 __attribute__((ext_vector_type(2)));
+
+constexpr int kMaxStageWords = 8;     // staged row words the bucket sort moves
+constexpr int kSortedBlock = 1024;    // threads of a bucket-aggregation workgroup
+
+struct StageLayout {
+    int words = 0;                    // 0: the plan does not stage
+    int valid_word = -1;
+    int off[kMaxSrc] = {};
+    int cls[kMaxSrc] = {};
+    bool two[kMaxSrc] = {};
+};
 
 __device__ __forceinline__ void store_width(char* data, int width, unsigned int g, u128 v)
 {
@@ -217,6 +229,21 @@ __global__ void __launch_bounds__(kBlock) k_occ_index(const uint64_t* recs, cons
         out += all;
         __syncthreads();
     }
+}
+
+// starts[b] = first index of bucket b in the sorted bucket keys (entries of
+// empty buckets keep their fill value and are fixed on the host).
+__global__ void __launch_bounds__(kBlock) k_bucket_starts(const unsigned int* keys, int64_t n, uint32_t* starts)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned int k = keys[i];
+        if (i == 0 || keys[i - 1] != k) starts[k] = (uint32_t)i;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -711,6 +738,169 @@ std::string emit_program(std::string& o, const std::string& name, const DevProgr
     return reg(prog.result);
 }
 
+// Class (QDB_REG_*) of a plan's argument j as a kernel computes it.
+int source_class(const AggPlan* P, int j)
+{
+    const SrcDesc& sd = P->proto.src[j];
+    auto col_cls = [](const QdbValueType& t) {
+        return t.Kind == QDB_KIND_FLOAT ? QDB_REG_F64 : t.Kind == QDB_KIND_BOOL ? QDB_REG_BOOL : QDB_REG_INT;
+    };
+    if (sd.prog < 0) return col_cls(P->slot_type[sd.slot]);
+    const DevProgram& prog = P->h_progs[(size_t)sd.prog];
+    int cls[QDB_EXPR_MAX_REGS] = {0};
+    for (int k = 0; k < prog.n && k < QDB_EXPR_MAX_INSNS; ++k) {
+        const DevInsn& in = prog.insns[k];
+        if (in.dst < 0 || in.dst >= QDB_EXPR_MAX_REGS) continue;
+        if (in.op == QDB_EXPR_COLUMN) cls[in.dst] = col_cls(P->slot_type[in.column]);
+        else if (in.op == QDB_EXPR_CMP || in.op == QDB_EXPR_AND || in.op == QDB_EXPR_OR || in.op == QDB_EXPR_NOT || in.op == QDB_EXPR_IS_NULL)
+            cls[in.dst] = QDB_REG_BOOL;
+        else cls[in.dst] = in.cls;
+    }
+    return cls[prog.result >= 0 && prog.result < QDB_EXPR_MAX_REGS ? prog.result : 0];
+}
+
+// Where a row's key and arguments sit in its staged words (the bucketed
+// high-cardinality path): key words, then per argument one word (f64 bits,
+// or an integer that fits 64 bits) or two (an i128), then a word of
+// argument validity bits when an argument can be NULL. words = 0: the plan
+// does not stage (i128 MIN/MAX, or more than kMaxStageWords words).
+StageLayout stage_layout(const AggPlan* P)
+{
+    StageLayout L;
+    const AggArgs& A = P->proto;
+    int w = P->KW;
+    bool nullable = false;
+    for (int j = 0; j < A.nsrc; ++j) {
+        const SrcDesc& sd = A.src[j];
+        L.cls[j] = source_class(P, j);
+        L.off[j] = w;
+        bool two = false;
+        if (sd.prog >= 0) {
+            two = L.cls[j] == QDB_REG_INT;
+            nullable = true;
+        } else {
+            two = P->slot_type[sd.slot].Width == 16;
+            nullable |= P->slot_type[sd.slot].Nullable != 0;
+        }
+        L.two[j] = two;
+        w += two ? 2 : 1;
+    }
+    L.valid_word = nullable ? w++ : -1;
+    // The sort moves 2..6 or 8 words: pad to the next such count.
+    if (w < 2) w = 2;
+    if (w == 7) w = 8;
+    L.words = (P->has_pairs || w > kMaxStageWords) ? 0 : w;
+    return L;
+}
+
+// `const DVal v = <argument j of the row>;`
+void emit_source(std::string& o, const AggPlan* P, int j, const std::string& v)
+{
+    const SrcDesc& sd = P->proto.src[j];
+    if (sd.prog >= 0) {
+        const std::string r = emit_program(o, v + "p", P->h_progs[(size_t)sd.prog], P);
+        o += "        const DVal " + v + " = " + r + ";\n";
+    } else {
+        const QdbValueType& t = P->slot_type[sd.slot];
+        o += "        const DVal " + v + " = read_col_t<" + std::to_string(t.Kind) + ", " + std::to_string(t.Width) + ">(C.c[" +
+            std::to_string(sd.slot) + "], row);\n";
+    }
+}
+
+// The state words argument j contributes to, from its value v.
+void emit_contribs(std::string& o, const AggPlan* P, int j, const std::string& v)
+{
+    const AggArgs& A = P->proto;
+    o += "        if (" + v + ".valid) {\n";
+    o += "            int64_t limb[kLimbs] = {};\n            bool outside = false;\n";
+    bool limbs = false;
+    for (int w = 0; w < A.nwords; ++w) limbs |= A.words[w].src == j && (A.words[w].kind == WK_LIMB || A.words[w].kind == WK_OUTSIDE);
+    if (limbs) o += "            if (" + v + ".cls == QDB_REG_F64) f64_limbs(" + v + ".f, limb, &outside);\n";
+    for (int w = 0; w < A.nwords; ++w) {
+        const WordDesc& d = A.words[w];
+        if (d.src != j || d.kind == WK_ROWS || d.kind == WK_LOCK) continue;
+        char b[200];
+        snprintf(b, sizeof(b), "            c[%d] = word_contrib(WordDesc{%d, %d, %d, %d}, %s, limb, outside);\n", w, d.kind, d.op, d.src, d.part,
+            v.c_str());
+        o += b;
+    }
+    o += "        }\n";
+}
+
+// stage(): the row's staged words (false: the row does not take part);
+// unstage(): the key and contributions back from them.
+void emit_stage(std::string& o, const AggPlan* P)
+{
+    const StageLayout L = stage_layout(P);
+    const AggArgs& A = P->proto;
+    const int nw = A.nwords;
+    o += "    static constexpr int kStage = " + std::to_string(L.words > 0 ? L.words : 1) + ";\n";
+    if (L.words == 0) return;
+    o += "    template <int KW>\n";
+    o += "    __device__ static __forceinline__ bool stage(const AggArgs& A, int64_t row, uint64_t (&st)[kStage])\n    {\n";
+    o += "        const DevCols& C = A.cols;\n";
+    o += "#pragma unroll\n        for (int w = 0; w < kStage; ++w) st[w] = 0;\n";
+    o += "        if (A.sel && !A.sel[row]) return false;\n";
+    if (P->has_pred) {
+        o += "        {\n";
+        const std::string r = emit_program(o, "p", P->h_pred, P);
+        o += "    if (!(" + r + ".valid && " + r + ".i != 0)) return false;\n        }\n";
+    }
+    o += "        uint64_t key[KW];\n#pragma unroll\n        for (int w = 0; w < KW; ++w) key[w] = 0;\n";
+    for (int f = 0; f < A.nkeys; ++f) {
+        const KeyFieldDesc& k = A.keys[f];
+        char b[300];
+        snprintf(b, sizeof(b), "        pack_key_field<KW>(KeyFieldDesc{%d, %d, %d, %d, %d, %d, %d, %d}, C.c[%d], row, key);\n", k.slot, k.kind,
+            k.width, k.nullable, k.word, k.shift, k.null_word, k.null_bit, k.slot);
+        o += b;
+    }
+    o += "#pragma unroll\n        for (int w = 0; w < KW; ++w) st[w] = key[w];\n";
+    o += "        uint64_t vb = 0;\n";
+    for (int j = 0; j < A.nsrc; ++j) {
+        const std::string v = "s" + std::to_string(j), off = std::to_string(L.off[j]);
+        o += "        {\n";
+        emit_source(o, P, j, v);
+        o += "        if (" + v + ".valid) vb |= 1ull << " + std::to_string(j) + ";\n";
+        if (L.cls[j] == QDB_REG_F64) {
+            o += "        st[" + off + "] = (uint64_t)__double_as_longlong(" + v + ".f);\n";
+        } else if (L.two[j]) {
+            o += "        st[" + off + "] = (uint64_t)(u128)" + v + ".i;\n";
+            o += "        st[" + std::to_string(L.off[j] + 1) + "] = (uint64_t)((u128)" + v + ".i >> 64);\n";
+        } else {
+            o += "        st[" + off + "] = (uint64_t)(int64_t)" + v + ".i;\n";
+        }
+        o += "        }\n";
+    }
+    if (L.valid_word >= 0) o += "        st[" + std::to_string(L.valid_word) + "] = vb;\n";
+    o += "        (void)vb;\n        return true;\n    }\n";
+    o += "    template <int KW, int S>\n";
+    o += "    __device__ static __forceinline__ void unstage(const AggArgs& A, const uint64_t (&st)[kStage], uint64_t (&key)[KW], uint64_t (&c)[S])\n    {\n";
+    o += "#pragma unroll\n        for (int w = 0; w < KW; ++w) key[w] = st[w];\n";
+    o += "#pragma unroll\n        for (int w = 0; w < S; ++w) c[w] = 0;\n";
+    for (int w = 0; w < nw; ++w) {
+        const WordDesc& d = A.words[w];
+        o += "        c[" + std::to_string(w) + "] = " + (d.kind == WK_ROWS ? std::string("1ull") : u64_lit(P->word_init_host[w])) + ";\n";
+    }
+    for (int j = 0; j < A.nsrc; ++j) {
+        const std::string v = "s" + std::to_string(j), off = std::to_string(L.off[j]);
+        o += "        {\n        DVal " + v + " = dval_zero();\n";
+        o += "        " + v + ".cls = " + std::to_string(L.cls[j]) + ";\n";
+        o += "        " + v + ".valid = " +
+            (L.valid_word >= 0 ? "(uint8_t)((st[" + std::to_string(L.valid_word) + "] >> " + std::to_string(j) + ") & 1ull)" : std::string("1")) +
+            ";\n";
+        if (L.cls[j] == QDB_REG_F64) {
+            o += "        " + v + ".f = __longlong_as_double((long long)st[" + off + "]);\n";
+        } else if (L.two[j]) {
+            o += "        " + v + ".i = (i128)(((u128)st[" + std::to_string(L.off[j] + 1) + "] << 64) | (u128)st[" + off + "]);\n";
+        } else {
+            o += "        " + v + ".i = (i128)(int64_t)st[" + off + "];\n";
+        }
+        emit_contribs(o, P, j, v);
+        o += "        }\n";
+    }
+    o += "    }\n";
+}
+
 std::string agg_jit_source(const AggPlan* P)
 {
     const AggArgs& A = P->proto;
@@ -745,31 +935,11 @@ std::string agg_jit_source(const AggPlan* P)
         o += "        c[" + std::to_string(w) + "] = " + (d.kind == WK_ROWS ? std::string("1ull") : u64_lit(P->word_init_host[w])) + ";\n";
     }
     for (int j = 0; j < A.nsrc; ++j) {
-        const SrcDesc& sd = A.src[j];
         const std::string v = "s" + std::to_string(j);
         o += "        {\n";
-        if (sd.prog >= 0) {
-            const std::string r = emit_program(o, v + "p", P->h_progs[(size_t)sd.prog], P);
-            o += "        const DVal " + v + " = " + r + ";\n";
-        } else {
-            const QdbValueType& t = P->slot_type[sd.slot];
-            o += "        const DVal " + v + " = read_col_t<" + std::to_string(t.Kind) + ", " + std::to_string(t.Width) + ">(C.c[" +
-                std::to_string(sd.slot) + "], row);\n";
-        }
-        o += "        if (" + v + ".valid) {\n";
-        o += "            int64_t limb[kLimbs] = {};\n            bool outside = false;\n";
-        bool limbs = false;
-        for (int w = 0; w < nw; ++w) limbs |= A.words[w].src == j && (A.words[w].kind == WK_LIMB || A.words[w].kind == WK_OUTSIDE);
-        if (limbs) o += "            if (" + v + ".cls == QDB_REG_F64) f64_limbs(" + v + ".f, limb, &outside);\n";
-        for (int w = 0; w < nw; ++w) {
-            const WordDesc& d = A.words[w];
-            if (d.src != j || d.kind == WK_ROWS || d.kind == WK_LOCK) continue;
-            char b[200];
-            snprintf(b, sizeof(b), "            c[%d] = word_contrib(WordDesc{%d, %d, %d, %d}, %s, limb, outside);\n", w, d.kind, d.op, d.src,
-                d.part, v.c_str());
-            o += b;
-        }
-        o += "        }\n        }\n";
+        emit_source(o, P, j, v);
+        emit_contribs(o, P, j, v);
+        o += "        }\n";
     }
     o += "#pragma unroll\n        for (int w = 0; w < KW; ++w) key[w] = 0;\n";
     for (int f = 0; f < A.nkeys; ++f) {
@@ -779,13 +949,31 @@ std::string agg_jit_source(const AggPlan* P)
             k.width, k.nullable, k.word, k.shift, k.null_word, k.null_bit, k.slot);
         o += b;
     }
-    o += "        return true;\n    }\n};\n}  // namespace agg\n}  // namespace qdbgpu\n";
+    o += "        return true;\n    }\n";
+    emit_stage(o, P);
+    o += "};\n}  // namespace agg\n}  // namespace qdbgpu\n";
     o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kBlock) + ") qdb_agg_update(qdbgpu::agg::AggArgs A)\n{\n";
     o += "    qdbgpu::agg::agg_update_body<" + std::to_string(P->KW) + ", " + std::to_string(P->S) + ", qdbgpu::agg::JitPolicy>(A);\n}\n";
     o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kBlock) +
         ") qdb_agg_batch(const qdbgpu::agg::AggArgs* insts, const qdbgpu::agg::BatchItem* items, int nitems, unsigned int* head)\n{\n";
     o += "    qdbgpu::agg::agg_batch_body<" + std::to_string(P->KW) + ", " + std::to_string(P->S) +
         ", qdbgpu::agg::JitPolicy>(insts, items, nitems, head);\n}\n";
+    const StageLayout L = stage_layout(P);
+    if (L.words > 0) {
+        const std::string kw = std::to_string(P->KW), sw = std::to_string(P->S), w = std::to_string(L.words);
+        o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kBlock) +
+            ") qdb_agg_stage(qdbgpu::agg::AggArgs A, unsigned long long* staged, unsigned int* bucket, int shift, unsigned int discard)\n{\n";
+        o += "    qdbgpu::agg::agg_stage_body<" + kw + ", " + w + ", qdbgpu::agg::JitPolicy>(A, staged, bucket, shift, discard);\n}\n";
+        o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kSortedBlock) +
+            ") qdb_agg_sorted(qdbgpu::agg::AggArgs A, const unsigned long long* staged, const qdbgpu::agg::SortedItem* items, int nitems, "
+            "unsigned int* head, unsigned long long* recs, unsigned int* counts)\n{\n";
+        o += "    qdbgpu::agg::agg_sorted_body<" + kw + ", " + sw + ", " + w +
+            ", qdbgpu::agg::JitPolicy>(A, staged, items, nitems, head, recs, counts);\n}\n";
+        o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kBlock) +
+            ") qdb_agg_merge(qdbgpu::agg::AggArgs A, const unsigned long long* recs, const qdbgpu::agg::SortedItem* items, "
+            "const unsigned int* counts, int nitems, unsigned int* head)\n{\n";
+        o += "    qdbgpu::agg::agg_merge_body<" + kw + ", " + sw + ", qdbgpu::agg::JitPolicy>(A, recs, items, counts, nitems, head);\n}\n";
+    }
     return o;
 }
 
@@ -1008,11 +1196,174 @@ bool settle_launch(UpdateLaunch* L, unsigned int cnt[4])
     return true;
 }
 
+// Rows of a batch from which the bucketed path pays, and table capacity
+// (slots, from Init / ExpectRowGroups or growth) that selects it.
+constexpr int64_t kSortedMinRows = (int64_t)1 << 20;
+constexpr uint64_t kSortedMinSlots = (uint64_t)1 << 20;
+constexpr int64_t kSortedItemRows = (int64_t)1 << 16;
+
+bool sorted_path_wanted(AggPlan* P, const AggTable* T, int64_t rows)
+{
+    return rows >= kSortedMinRows && T->capacity >= kSortedMinSlots && stage_layout(P).words > 0 &&
+        plan_jit_function(P, "qdb_agg_sorted") != nullptr;
+}
+
+// The bucketed update of rows [0, n) of a bound batch (n <= 2^30): stage,
+// sort by bucket, aggregate every bucket in LDS into records, grow the
+// table for them, merge them into the HBM table.
+bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n)
+{
+    AggPlan* P = T->plan;
+    const StageLayout SL = stage_layout(P);
+    const int W = SL.words, KW = P->KW, R = KW + bound.nwords;
+    // LDS table of the bucket workgroups: the most groups (a power of two)
+    // that fit next to the kernel's own LDS, and buckets sized to fill
+    // about 60% of it.
+    AggArgs A = bound;
+    table_args(T, &A);
+    A.row_base = 0;
+    A.nrows = n;
+    A.row_index = nullptr;
+    int g = 4096;
+    while (g > 64 && (size_t)g * R * 8 + (size_t)g * 2 * 4 + 16 > max_lds_per_block() - 4096) g /= 2;
+    A.lds_groups = g;
+    A.lds_slots = 2 * g;
+    const size_t lds = (size_t)g * R * 8 + (size_t)A.lds_slots * 4 + 16;
+    const uint64_t expect = (uint64_t)n < T->capacity / 2 ? (uint64_t)n : T->capacity / 2;
+    int bits = 8;
+    while (bits < 20 && ((uint64_t)1 << bits) * (uint64_t)(g * 6 / 10) < expect) ++bits;
+    const uint32_t B = 1u << bits;
+    hipFunction_t f_stage = plan_jit_function(P, "qdb_agg_stage");
+    hipFunction_t f_sorted = plan_jit_function(P, "qdb_agg_sorted");
+    hipFunction_t f_merge = plan_jit_function(P, "qdb_agg_merge");
+    if (!f_stage || !f_sorted || !f_merge) return fail("bucketed aggregate kernels unavailable");
+
+    unsigned int* bkey[2] = {(unsigned int*)dalloc((size_t)n * 4 + 16), (unsigned int*)dalloc((size_t)n * 4 + 16)};
+    unsigned long long* st[2] = {(unsigned long long*)dalloc((size_t)n * W * 8 + 16), (unsigned long long*)dalloc((size_t)n * W * 8 + 16)};
+    uint32_t* starts = (uint32_t*)dalloc(((size_t)B + 2) * 4);
+    unsigned int* head = (unsigned int*)dalloc(16);
+    unsigned long long* recs = nullptr;
+    SortedItem* d_items = nullptr;
+    unsigned int* counts = nullptr;
+    auto release = [&]() {
+        dfree(bkey[0]);
+        dfree(bkey[1]);
+        dfree(st[0]);
+        dfree(st[1]);
+        dfree(starts);
+        dfree(head);
+        dfree(recs);
+        dfree(d_items);
+        dfree(counts);
+    };
+    if (!bkey[0] || !bkey[1] || !st[0] || !st[1] || !starts || !head) {
+        release();
+        return false;
+    }
+    TimedRegion timer("agg_update");
+    {
+        int shift = 64 - bits;
+        unsigned int discard = B;
+        void* args[] = {&A, &st[0], &bkey[0], &shift, &discard};
+        const int grid = grid_for(n, kBlock, 8);
+        if (!hip_ok(hipModuleLaunchKernel(f_stage, (unsigned)grid, 1, 1, kBlock, 1, 1, 0, stream(), args, nullptr), "stage launch")) {
+            release();
+            return false;
+        }
+    }
+    if (!sort_pairs_u32_words(W, bkey[0], bkey[1], st[0], st[1], n, bits + 1)) {
+        release();
+        return false;
+    }
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(B + 2, kBlock, 1)), dim3(kBlock), 0, stream(), starts, (int64_t)B + 2, (uint32_t)n);
+    hipLaunchKernelGGL(k_bucket_starts, dim3(grid_for(n, kBlock, 8)), dim3(kBlock), 0, stream(), bkey[1], n, starts);
+    std::vector<uint32_t> h_starts((size_t)B + 2);
+    if (!hip_ok(hipGetLastError(), "bucket launch") ||
+        !hip_ok(hipMemcpyAsync(h_starts.data(), starts, h_starts.size() * 4, hipMemcpyDeviceToHost, stream()), "starts") || !sync("starts")) {
+        release();
+        return false;
+    }
+    h_starts[B + 1] = (uint32_t)n;
+    for (int64_t b = (int64_t)B; b >= 0; --b) h_starts[(size_t)b] = std::min(h_starts[(size_t)b], h_starts[(size_t)b + 1]);
+    std::vector<SortedItem> items;
+    for (uint32_t b = 0; b < B; ++b) {
+        const int64_t r0 = h_starts[b], r1 = h_starts[b + 1];
+        if (r1 <= r0) continue;
+        const bool owned = r1 - r0 <= kSortedItemRows;
+        for (int64_t r = r0; r < r1; r += kSortedItemRows) {
+            SortedItem it;
+            it.row0 = r;
+            it.row1 = r + kSortedItemRows < r1 ? r + kSortedItemRows : r1;
+            it.owned = owned ? 1 : 0;
+            it.pad = 0;
+            items.push_back(it);
+        }
+    }
+    int nitems = (int)items.size();
+    if (nitems == 0) {
+        release();
+        return true;
+    }
+    recs = (unsigned long long*)dalloc((size_t)n * R * 8 + 16);
+    d_items = (SortedItem*)upload(items.data(), items.size() * sizeof(SortedItem));
+    counts = (unsigned int*)dalloc((size_t)nitems * 4 + 16);
+    if (!recs || !d_items || !counts || !zero_fill(head, 16)) {
+        release();
+        return false;
+    }
+    {
+        const int grid = nitems < num_cus() ? nitems : num_cus();
+        void* args[] = {&A, &st[1], &d_items, &nitems, &head, &recs, &counts};
+        if (!hip_ok(hipModuleLaunchKernel(f_sorted, (unsigned)grid, 1, 1, kSortedBlock, 1, 1, (unsigned)lds, stream(), args, nullptr),
+                "bucket aggregate launch")) {
+            release();
+            return false;
+        }
+    }
+    std::vector<uint32_t> h_counts((size_t)nitems);
+    if (!hip_ok(hipMemcpyAsync(h_counts.data(), counts, h_counts.size() * 4, hipMemcpyDeviceToHost, stream()), "counts") || !sync("bucket aggregate")) {
+        release();
+        return false;
+    }
+    uint64_t total = 0;
+    for (uint32_t c : h_counts) total += c & ~kSharedItem;
+    // Room for every record as a new group keeps the merge free of spills.
+    if ((uint64_t)T->groups + total > T->capacity / 2 && !table_grow(T, (uint64_t)T->groups + total)) {
+        release();
+        return false;
+    }
+    UpdateLaunch L;
+    if (!prepare_launch(T, bound, 0, 1, (size_t)total + 1024, &L) || !set_counters(T) || !zero_fill(head, 16)) {
+        L.release();
+        release();
+        return false;
+    }
+    {
+        const int grid = nitems < num_cus() * 4 ? nitems : num_cus() * 4;
+        void* args[] = {&L.A, &recs, &d_items, &counts, &nitems, &head};
+        if (!hip_ok(hipModuleLaunchKernel(f_merge, (unsigned)grid, 1, 1, kBlock, 1, 1, 0, stream(), args, nullptr), "merge launch")) {
+            L.release();
+            release();
+            return false;
+        }
+    }
+    unsigned int cnt[4];
+    const bool ok = read_counters(T, cnt) && settle_launch(&L, cnt);
+    L.release();
+    release();
+    return ok;
+}
+
 // agg_dispatch(Update): every selected row of the batch, exactly once.
 bool table_update(AggTable* T, const TRowSet* batch)
 {
     AggPlan* P = T->plan;
     if (batch->RowCount <= 0) return true;
+    if (batch->RowCount <= ((int64_t)1 << 30) && sorted_path_wanted(P, T, batch->RowCount)) {
+        AggArgs bound;
+        if (!bind_batch(P, batch, &bound)) return false;
+        return table_update_sorted(T, bound, batch->RowCount);
+    }
     AggArgs bound;
     if (!bind_batch(P, batch, &bound)) return false;
     const int KW = P->KW, S = P->S;

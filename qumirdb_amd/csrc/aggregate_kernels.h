@@ -823,6 +823,206 @@ __device__ __forceinline__ void agg_batch_body(const AggArgs* __restrict__ insts
     if (cur >= 0) lds_table_flush<KW, S, P>(insts[cur], T, use_lds);
 }
 
+// ---------------------------------------------------------------------------
+// Bucketed path for many groups: rows are staged (key and argument words),
+// sorted by bucket (top bits of the key hash), each bucket is aggregated
+// in one workgroup's LDS table, and the groups are merged into the HBM
+// table -- with plain stores when one workgroup holds the whole bucket, as
+// no other workgroup can hold one of its keys.
+// ---------------------------------------------------------------------------
+
+// A run of bucket-sorted staged rows aggregated by one workgroup; `owned`:
+// the run is its bucket's only run.
+struct SortedItem {
+    int64_t row0;
+    int64_t row1;
+    int32_t owned;
+    int32_t pad;
+};
+
+constexpr unsigned int kSharedItem = 0x80000000u;  // count flag: an item's records may repeat a key
+
+// Stages rows [0, nrows) of the launch: words of row i at staged[i * W],
+// its bucket at bucket[i] (`discard` for a row that does not take part).
+template <int KW, int W, class P>
+__device__ __forceinline__ void agg_stage_body(const AggArgs& A, unsigned long long* staged, unsigned int* bucket, int shift,
+    unsigned int discard)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < A.nrows; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t st[W];
+        const bool ok = P::template stage<KW>(A, A.row_base + i, st);
+        unsigned int b = discard;
+        if (ok) {
+            uint64_t key[KW];
+#pragma unroll
+            for (int w = 0; w < KW; ++w) key[w] = st[w];
+            b = (unsigned int)(key_place_hash<KW>(key) >> shift);
+        }
+#pragma unroll
+        for (int w = 0; w < W; ++w) staged[(uint64_t)i * W + w] = st[w];
+        bucket[i] = b;
+    }
+}
+
+// Aggregates items of bucket-sorted staged rows in the workgroup's LDS table,
+// one item at a time (from a device queue); writes the item's groups as
+// records (key words, state words) from recs[row0 * R] on and their count to
+// counts[item]. A row whose group finds no room in the LDS table becomes a
+// record of its own, and the count carries kSharedItem.
+template <int KW, int S, int W, class P>
+__device__ __forceinline__ void agg_sorted_body(const AggArgs& A, const unsigned long long* staged, const SortedItem* items, int nitems,
+    unsigned int* head, unsigned long long* recs, unsigned int* counts)
+{
+    extern __shared__ uint64_t lds_raw[];
+    __shared__ unsigned int s_unused;
+    __shared__ unsigned int s_nrec;
+    __shared__ unsigned int s_shared;
+    __shared__ int s_item;
+    LdsTable T = lds_table_at<KW, P>(A, lds_raw, &s_unused);
+    const int R = KW + P::nwords(A);
+    const int lane = lane_id();
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(head, 1u);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (it >= nitems) break;
+        const SortedItem w = items[it];
+        if (threadIdx.x == 0) {
+            s_nrec = 0u;
+            s_shared = 0u;
+        }
+        lds_table_reset<KW, P>(A, T, true);
+        unsigned long long* out = recs + (uint64_t)w.row0 * (uint64_t)R;
+        for (int64_t base = w.row0 + (int64_t)threadIdx.x - lane; base < w.row1; base += blockDim.x) {
+            const int64_t i = base + lane;
+            const bool valid = i < w.row1;
+            uint64_t st[W];
+#pragma unroll
+            for (int k = 0; k < W; ++k) st[k] = staged[(uint64_t)(valid ? i : w.row0) * W + k];
+            uint64_t key[KW];
+            uint64_t c[S];
+            P::template unstage<KW, S>(A, st, key, c);
+            int where = valid ? AT_BUSY : AT_NONE;
+            int lg = 0;
+            while (__any(where == AT_BUSY)) {
+                if (where == AT_BUSY) {
+                    const int r = lds_find_or_insert<KW, S, P>(A, T, key);
+                    where = r >= 0 ? AT_LDS : r == (int)kBusy ? AT_BUSY : AT_NONE;
+                    if (r >= 0) lg = r;
+                }
+            }
+            if (valid && where == AT_LDS) {
+                apply_words<KW, S, P>(A, T, AT_LDS, lg, c);
+            } else if (valid) {
+                // No room in LDS: the row is a record of its own.
+                const unsigned int pos = atomicAdd(&s_nrec, 1u);
+                unsigned long long* r = out + (uint64_t)pos * R;
+#pragma unroll
+                for (int k = 0; k < KW; ++k) r[k] = key[k];
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    if (k < P::nwords(A)) r[KW + k] = c[k];
+                }
+                s_shared = 1u;
+            }
+        }
+        __syncthreads();
+        const unsigned int n = *T.count < (unsigned int)A.lds_groups ? *T.count : (unsigned int)A.lds_groups;
+        const unsigned int first = s_nrec;
+        for (unsigned int g = threadIdx.x; g < n; g += blockDim.x) {
+            const uint64_t* src = T.recs + (size_t)g * R;
+            unsigned long long* r = out + (uint64_t)(first + g) * R;
+            for (int k = 0; k < R; ++k) r[k] = src[k];
+        }
+        if (threadIdx.x == 0) counts[it] = (first + n) | (s_shared ? kSharedItem : 0u);
+        __syncthreads();
+    }
+}
+
+// Merges one record into the HBM table. Owned: its key occurs once in the
+// item and in no other item, so once the slot is found or claimed the words
+// are combined with plain (coherent) loads and stores; else with atomics. A
+// record that finds no room is kept for the replay after growth. One lookup
+// serves both cases.
+template <int KW, int S, class P>
+__device__ __forceinline__ void merge_record(const AggArgs& A, LdsTable& T, bool want, bool owned, const uint64_t (&key)[KW],
+    const uint64_t (&c)[S])
+{
+    int64_t id = 0;
+    const int where = locate<KW, S, P>(A, T, false, want, key, &id);
+    if (want && where == AT_HBM) {
+        uint64_t* r = A.recs + (uint64_t)id * (uint64_t)A.rstride + KW;
+#pragma unroll
+        for (int w = 0; w < S; ++w) {
+            if (w >= P::nwords(A)) continue;
+            const uint8_t op = P::op(A, w);
+            if (c[w] == identity_of(op) && op != OP_FADD) continue;
+            if (owned) {
+                const uint64_t cur = __hip_atomic_load(&r[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&r[w], combine(op, cur, c[w]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                atomic_apply_global(op, &r[w], c[w]);
+            }
+        }
+    } else if (want) {
+        const unsigned int pos = atomicAdd(&A.counters[2], 1u);
+        if (pos < A.ov_rec_cap) {
+            uint64_t* r = A.ov_recs + (size_t)pos * (KW + P::nwords(A));
+#pragma unroll
+            for (int w = 0; w < KW; ++w) r[w] = key[w];
+#pragma unroll
+            for (int w = 0; w < S; ++w) {
+                if (w < P::nwords(A)) r[KW + w] = c[w];
+            }
+        }
+    }
+}
+
+// Merges every item's records into the HBM table (items from a device
+// queue), with plain stores for owned items and atomics for the others.
+template <int KW, int S, class P>
+__device__ __forceinline__ void agg_merge_body(const AggArgs& A, const unsigned long long* recs, const SortedItem* items,
+    const unsigned int* counts, int nitems, unsigned int* head)
+{
+    __shared__ unsigned int s_inserted;
+    __shared__ int s_item;
+    LdsTable T;
+    T.recs = nullptr;
+    T.ctrl = nullptr;
+    T.count = nullptr;
+    T.inserted = &s_inserted;
+    if (threadIdx.x == 0) s_inserted = 0u;
+    __syncthreads();
+    const int R = KW + P::nwords(A);
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(head, 1u);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (it >= nitems) break;
+        const SortedItem w = items[it];
+        const unsigned int cw = counts[it];
+        const unsigned int n = cw & ~kSharedItem;
+        const bool owned = w.owned && !(cw & kSharedItem);
+        const unsigned long long* in = recs + (uint64_t)w.row0 * (uint64_t)R;
+        for (unsigned int b = 0; b < n; b += blockDim.x) {
+            const unsigned int g = b + threadIdx.x;
+            const bool want = g < n;
+            const unsigned long long* r = in + (uint64_t)(want ? g : 0u) * R;
+            uint64_t key[KW];
+            uint64_t c[S];
+#pragma unroll
+            for (int k = 0; k < KW; ++k) key[k] = r[k];
+#pragma unroll
+            for (int k = 0; k < S; ++k) c[k] = k < P::nwords(A) ? r[KW + k] : 0;
+            merge_record<KW, S, P>(A, T, want, owned, key, c);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_inserted) atomicAdd(&A.counters[0], s_inserted);
+}
+
 template <int KW, int S>
 __global__ void __launch_bounds__(kBlock) k_agg_update(AggArgs A)
 {

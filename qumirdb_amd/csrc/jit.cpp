@@ -8,10 +8,14 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <signal.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/wait.h>
+#include <unistd.h>
 
+#include <cerrno>
 #include <chrono>
 #include <mutex>
 #include <string>
@@ -62,8 +66,9 @@ std::string device_arch()
     return colon == std::string::npos ? a : a.substr(0, colon);
 }
 
-// Compiles `src` to a code object; false with the log in *err.
-bool compile_source(const std::string& src, const std::string& arch, std::vector<char>* code, std::string* err)
+// Compiles `src` to a code object in this process; false with the log in
+// *err.
+bool compile_here(const std::string& src, const std::string& arch, std::vector<char>* code, std::string* err)
 {
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "qdb_specialized.hip", kJitHeaderCount, kJitHeaderTexts, kJitHeaderNames) !=
@@ -90,6 +95,82 @@ bool compile_source(const std::string& src, const std::string& arch, std::vector
     (void)hiprtcGetCode(prog, code->data());
     (void)hiprtcDestroyProgram(&prog);
     return size > 0;
+}
+
+bool write_all(int fd, const void* p, size_t n)
+{
+    const char* c = (const char*)p;
+    while (n) {
+        const ssize_t k = write(fd, c, n);
+        if (k <= 0) return false;
+        c += k;
+        n -= (size_t)k;
+    }
+    return true;
+}
+
+bool read_all(int fd, void* p, size_t n)
+{
+    char* c = (char*)p;
+    while (n) {
+        const ssize_t k = read(fd, c, n);
+        if (k <= 0) return false;
+        c += k;
+        n -= (size_t)k;
+    }
+    return true;
+}
+
+// Compiles `src` in a forked child that runs only the compiler and never
+// touches the GPU: a fault inside the compiler then ends the child, and
+// the caller gets a failed compile (and runs the generic kernels) instead
+// of a crash. QDB_GPU_JIT_INPROCESS=1 compiles in this process.
+bool compile_source(const std::string& src, const std::string& arch, std::vector<char>* code, std::string* err)
+{
+    const char* inproc = getenv("QDB_GPU_JIT_INPROCESS");
+    if (inproc && inproc[0] == '1') return compile_here(src, arch, code, err);
+    int fds[2];
+    if (pipe(fds) != 0) return compile_here(src, arch, code, err);
+    fflush(nullptr);
+    const pid_t pid = fork();
+    if (pid < 0) {
+        close(fds[0]);
+        close(fds[1]);
+        return compile_here(src, arch, code, err);
+    }
+    if (pid == 0) {
+        close(fds[0]);
+        std::vector<char> c;
+        std::string e;
+        const uint64_t ok = compile_here(src, arch, &c, &e) ? 1 : 0;
+        const uint64_t hdr[3] = {ok, (uint64_t)c.size(), (uint64_t)e.size()};
+        const bool sent = write_all(fds[1], hdr, sizeof(hdr)) && write_all(fds[1], c.data(), c.size()) &&
+            write_all(fds[1], e.data(), e.size());
+        close(fds[1]);
+        _exit(sent ? 0 : 3);
+    }
+    close(fds[1]);
+    uint64_t hdr[3] = {0, 0, 0};
+    bool got = read_all(fds[0], hdr, sizeof(hdr));
+    if (got) {
+        code->resize(hdr[1]);
+        std::string e(hdr[2], '\0');
+        got = read_all(fds[0], code->data(), hdr[1]) && read_all(fds[0], &e[0], hdr[2]);
+        *err = e;
+    }
+    close(fds[0]);
+    int status = 0;
+    while (waitpid(pid, &status, 0) < 0 && errno == EINTR) {
+    }
+    if (!got || !WIFEXITED(status) || WEXITSTATUS(status) != 0) {
+        char b[96];
+        if (WIFSIGNALED(status)) snprintf(b, sizeof(b), "the run-time compiler ended with signal %d", WTERMSIG(status));
+        else snprintf(b, sizeof(b), "the run-time compiler process failed (status %d)", status);
+        *err = b;
+        code->clear();
+        return false;
+    }
+    return hdr[0] == 1 && !code->empty();
 }
 
 }  // namespace

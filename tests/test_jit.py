@@ -65,12 +65,16 @@ def test_generated_update_kernels_compile():
 
 def test_generated_source_fixes_the_layout():
     """The generated policy carries the plan's state-word operations as
-    constants (no descriptor is read at run time) and one straight-line
-    read per argument column."""
+    constants (no descriptor is read at run time), one straight-line read
+    per argument column in the row contribution and in the staging of the
+    bucketed path, and the bucketed path's kernels."""
     import qumirdb_amd as q
     from qumirdb_amd import workloads as W
     src = _source(q.lib(), q, W.groups_spec().to_c()).decode()
     # COUNT(*), SUM(i64), MIN/MAX(f64), SUM(f64) = rows, sum, min, max, 4 limbs, side word
     assert "static constexpr int kNW = 9;" in src
-    assert src.count("read_col_t<") == 2
+    assert src.count("read_col_t<") == 4
     assert "A.words[" not in src and "eval_program" not in src
+    for k in ("qdb_agg_update", "qdb_agg_batch", "qdb_agg_stage", "qdb_agg_sorted", "qdb_agg_merge"):
+        assert k in src, k
+    assert "static constexpr int kStage = 3;" in src  # key, SUM(i64) argument, f64 argument
