@@ -231,20 +231,15 @@ __global__ void __launch_bounds__(kBlock) k_occ_index(const uint64_t* recs, cons
     }
 }
 
-// starts[b] = first index of bucket b in the sorted bucket keys (entries of
-// empty buckets keep their fill value and are fixed on the host).
-__global__ void __launch_bounds__(kBlock) k_bucket_starts(const unsigned int* keys, int64_t n, uint32_t* starts)
+// starts[b] = offs[b * nblk]: where bucket b's run of the first row range
+// starts in the partitioned rows.
+__global__ void __launch_bounds__(kBlock) k_gather_starts(const unsigned int* offs, unsigned int nb, unsigned int nblk, uint32_t* starts)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const unsigned int k = keys[i];
-        if (i == 0 || keys[i - 1] != k) starts[k] = (uint32_t)i;
+    for (unsigned int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+        starts[b] = offs[(uint64_t)b * nblk];
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
-{
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
-}
 
 // ---------------------------------------------------------------------------
 // Host: plans, tables, entry points.
@@ -961,9 +956,13 @@ std::string agg_jit_source(const AggPlan* P)
     const StageLayout L = stage_layout(P);
     if (L.words > 0) {
         const std::string kw = std::to_string(P->KW), sw = std::to_string(P->S), w = std::to_string(L.words);
-        o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kBlock) +
-            ") qdb_agg_stage(qdbgpu::agg::AggArgs A, unsigned long long* staged, unsigned int* bucket, int shift, unsigned int discard)\n{\n";
-        o += "    qdbgpu::agg::agg_stage_body<" + kw + ", " + w + ", qdbgpu::agg::JitPolicy>(A, staged, bucket, shift, discard);\n}\n";
+        o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kSortedBlock) +
+            ") qdb_agg_count(qdbgpu::agg::AggArgs A, unsigned int* hist, int shift, unsigned int nb, long long chunk)\n{\n";
+        o += "    qdbgpu::agg::agg_count_body<" + kw + ", " + w + ", qdbgpu::agg::JitPolicy>(A, hist, shift, nb, chunk);\n}\n";
+        o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kSortedBlock) +
+            ") qdb_agg_scatter(qdbgpu::agg::AggArgs A, const unsigned int* offs, unsigned long long* staged, int shift, unsigned int nb, "
+            "long long chunk)\n{\n";
+        o += "    qdbgpu::agg::agg_scatter_body<" + kw + ", " + w + ", qdbgpu::agg::JitPolicy>(A, offs, staged, shift, nb, chunk);\n}\n";
         o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kSortedBlock) +
             ") qdb_agg_sorted(qdbgpu::agg::AggArgs A, const unsigned long long* staged, const qdbgpu::agg::SortedItem* items, int nitems, "
             "unsigned int* head, unsigned long long* recs, unsigned int* counts)\n{\n";
@@ -1231,60 +1230,82 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n)
     const size_t lds = (size_t)g * R * 8 + (size_t)A.lds_slots * 4 + 16;
     const uint64_t expect = (uint64_t)n < T->capacity / 2 ? (uint64_t)n : T->capacity / 2;
     int bits = 8;
-    while (bits < 20 && ((uint64_t)1 << bits) * (uint64_t)(g * 6 / 10) < expect) ++bits;
+    while (bits < 15 && ((uint64_t)1 << bits) * (uint64_t)(g * 6 / 10) < expect) ++bits;
     const uint32_t B = 1u << bits;
-    hipFunction_t f_stage = plan_jit_function(P, "qdb_agg_stage");
     hipFunction_t f_sorted = plan_jit_function(P, "qdb_agg_sorted");
     hipFunction_t f_merge = plan_jit_function(P, "qdb_agg_merge");
-    if (!f_stage || !f_sorted || !f_merge) return fail("bucketed aggregate kernels unavailable");
+    if (!f_sorted || !f_merge) return fail("bucketed aggregate kernels unavailable");
 
-    unsigned int* bkey[2] = {(unsigned int*)dalloc((size_t)n * 4 + 16), (unsigned int*)dalloc((size_t)n * 4 + 16)};
-    unsigned long long* st[2] = {(unsigned long long*)dalloc((size_t)n * W * 8 + 16), (unsigned long long*)dalloc((size_t)n * W * 8 + 16)};
-    uint32_t* starts = (uint32_t*)dalloc(((size_t)B + 2) * 4);
+    // Partition: per-workgroup bucket counts over contiguous row ranges, a
+    // scan, then every row's staged words at its bucket run's next slot.
+    const unsigned int nb = B + 1;  // the last bucket takes rows that do not take part
+    int64_t nblk = (n + 16383) / 16384;
+    if (nblk > 2048) nblk = 2048;
+    long long chunk = (long long)((n + nblk - 1) / nblk);
+    nblk = (n + chunk - 1) / chunk;
+    const size_t nhist = (size_t)nb * (size_t)nblk;
+    unsigned int* hist = (unsigned int*)dalloc(nhist * 4 + 16);
+    unsigned int* offs = (unsigned int*)dalloc(nhist * 4 + 16);
+    unsigned long long* staged = (unsigned long long*)dalloc((size_t)n * W * 8 + 16);
+    uint32_t* starts = (uint32_t*)dalloc(((size_t)nb + 1) * 4);
     unsigned int* head = (unsigned int*)dalloc(16);
     unsigned long long* recs = nullptr;
     SortedItem* d_items = nullptr;
     unsigned int* counts = nullptr;
     auto release = [&]() {
-        dfree(bkey[0]);
-        dfree(bkey[1]);
-        dfree(st[0]);
-        dfree(st[1]);
+        dfree(hist);
+        dfree(offs);
+        dfree(staged);
         dfree(starts);
         dfree(head);
         dfree(recs);
         dfree(d_items);
         dfree(counts);
     };
-    if (!bkey[0] || !bkey[1] || !st[0] || !st[1] || !starts || !head) {
+    if (!hist || !offs || !staged || !starts || !head) {
         release();
         return false;
     }
+    hipFunction_t f_count = plan_jit_function(P, "qdb_agg_count");
+    hipFunction_t f_scatter = plan_jit_function(P, "qdb_agg_scatter");
+    if (!f_count || !f_scatter) {
+        release();
+        return fail("bucketed aggregate kernels unavailable");
+    }
     TimedRegion timer("agg_update");
+    int shift = 64 - bits;
+    const unsigned int hist_lds = nb * 4;
     {
-        int shift = 64 - bits;
-        unsigned int discard = B;
-        void* args[] = {&A, &st[0], &bkey[0], &shift, &discard};
-        const int grid = grid_for(n, kBlock, 8);
-        if (!hip_ok(hipModuleLaunchKernel(f_stage, (unsigned)grid, 1, 1, kBlock, 1, 1, 0, stream(), args, nullptr), "stage launch")) {
+        unsigned int nbv = nb;
+        void* args[] = {&A, &hist, &shift, &nbv, &chunk};
+        if (!hip_ok(hipModuleLaunchKernel(f_count, (unsigned)nblk, 1, 1, kSortedBlock, 1, 1, hist_lds, stream(), args, nullptr),
+                "count launch")) {
             release();
             return false;
         }
     }
-    if (!sort_pairs_u32_words(W, bkey[0], bkey[1], st[0], st[1], n, bits + 1)) {
+    if (!exclusive_scan_u32(hist, offs, (int64_t)nhist)) {
         release();
         return false;
     }
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(B + 2, kBlock, 1)), dim3(kBlock), 0, stream(), starts, (int64_t)B + 2, (uint32_t)n);
-    hipLaunchKernelGGL(k_bucket_starts, dim3(grid_for(n, kBlock, 8)), dim3(kBlock), 0, stream(), bkey[1], n, starts);
-    std::vector<uint32_t> h_starts((size_t)B + 2);
+    {
+        unsigned int nbv = nb;
+        void* args[] = {&A, &offs, &staged, &shift, &nbv, &chunk};
+        if (!hip_ok(hipModuleLaunchKernel(f_scatter, (unsigned)nblk, 1, 1, kSortedBlock, 1, 1, hist_lds, stream(), args, nullptr),
+                "scatter launch")) {
+            release();
+            return false;
+        }
+    }
+    // Bucket b starts where the first range's run of it starts.
+    hipLaunchKernelGGL(k_gather_starts, dim3(grid_for(nb, kBlock, 1)), dim3(kBlock), 0, stream(), offs, nb, (unsigned int)nblk, starts);
+    std::vector<uint32_t> h_starts((size_t)nb + 1);
     if (!hip_ok(hipGetLastError(), "bucket launch") ||
-        !hip_ok(hipMemcpyAsync(h_starts.data(), starts, h_starts.size() * 4, hipMemcpyDeviceToHost, stream()), "starts") || !sync("starts")) {
+        !hip_ok(hipMemcpyAsync(h_starts.data(), starts, (size_t)nb * 4, hipMemcpyDeviceToHost, stream()), "starts") || !sync("starts")) {
         release();
         return false;
     }
-    h_starts[B + 1] = (uint32_t)n;
-    for (int64_t b = (int64_t)B; b >= 0; --b) h_starts[(size_t)b] = std::min(h_starts[(size_t)b], h_starts[(size_t)b + 1]);
+    h_starts[nb] = (uint32_t)n;
     std::vector<SortedItem> items;
     for (uint32_t b = 0; b < B; ++b) {
         const int64_t r0 = h_starts[b], r1 = h_starts[b + 1];
@@ -1313,7 +1334,7 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n)
     }
     {
         const int grid = nitems < num_cus() ? nitems : num_cus();
-        void* args[] = {&A, &st[1], &d_items, &nitems, &head, &recs, &counts};
+        void* args[] = {&A, &staged, &d_items, &nitems, &head, &recs, &counts};
         if (!hip_ok(hipModuleLaunchKernel(f_sorted, (unsigned)grid, 1, 1, kSortedBlock, 1, 1, (unsigned)lds, stream(), args, nullptr),
                 "bucket aggregate launch")) {
             release();
@@ -1423,7 +1444,9 @@ bool batched_update(AggPlan* P, AggTable* const* tables, const TRowSet* const* b
         AggTable* T = tables[i];
         const TRowSet* b = batches[i];
         if (b->RowCount <= 0) continue;
-        if (b->RowCount > ((int64_t)1 << 30)) {
+        // Very large batches, and large ones into large tables (the bucketed
+        // path), are updated on their own.
+        if (b->RowCount > ((int64_t)1 << 30) || sorted_path_wanted(P, T, b->RowCount)) {
             ok = table_update(T, b);
             continue;
         }

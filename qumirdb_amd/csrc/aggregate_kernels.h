@@ -824,8 +824,8 @@ __device__ __forceinline__ void agg_batch_body(const AggArgs* __restrict__ insts
 }
 
 // ---------------------------------------------------------------------------
-// Bucketed path for many groups: rows are staged (key and argument words),
-// sorted by bucket (top bits of the key hash), each bucket is aggregated
+// Bucketed path for many groups: rows are staged (key and argument words)
+// and partitioned by bucket (top bits of the key hash), each bucket is aggregated
 // in one workgroup's LDS table, and the groups are merged into the HBM
 // table -- with plain stores when one workgroup holds the whole bucket, as
 // no other workgroup can hold one of its keys.
@@ -842,25 +842,54 @@ struct SortedItem {
 
 constexpr unsigned int kSharedItem = 0x80000000u;  // count flag: an item's records may repeat a key
 
-// Stages rows [0, nrows) of the launch: words of row i at staged[i * W],
-// its bucket at bucket[i] (`discard` for a row that does not take part).
+// Bucket partition of the staged rows in two passes over the same
+// contiguous row ranges (workgroup k: rows [k * chunk, (k + 1) * chunk)).
+// Counting: per-bucket counts of the range into hist[bucket * gridDim.x +
+// k]. After an exclusive scan of hist, scattering: each row's staged words
+// at the next position of its bucket's run for the range. A row that does
+// not take part counts in the last bucket and is not written.
 template <int KW, int W, class P>
-__device__ __forceinline__ void agg_stage_body(const AggArgs& A, unsigned long long* staged, unsigned int* bucket, int shift,
-    unsigned int discard)
+__device__ __forceinline__ void agg_count_body(const AggArgs& A, unsigned int* hist, int shift, unsigned int nb, int64_t chunk)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < A.nrows; i += (int64_t)gridDim.x * blockDim.x) {
+    extern __shared__ unsigned int lh[];
+    for (unsigned int b = threadIdx.x; b < nb; b += blockDim.x) lh[b] = 0u;
+    __syncthreads();
+    const int64_t r0 = blockIdx.x * chunk;
+    const int64_t r1 = r0 + chunk < A.nrows ? r0 + chunk : A.nrows;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
         uint64_t st[W];
-        const bool ok = P::template stage<KW>(A, A.row_base + i, st);
-        unsigned int b = discard;
-        if (ok) {
+        unsigned int b = nb - 1u;
+        if (P::template stage<KW>(A, A.row_base + i, st)) {
             uint64_t key[KW];
 #pragma unroll
             for (int w = 0; w < KW; ++w) key[w] = st[w];
             b = (unsigned int)(key_place_hash<KW>(key) >> shift);
         }
+        atomicAdd(&lh[b], 1u);
+    }
+    __syncthreads();
+    for (unsigned int b = threadIdx.x; b < nb; b += blockDim.x) hist[(uint64_t)b * gridDim.x + blockIdx.x] = lh[b];
+}
+
+template <int KW, int W, class P>
+__device__ __forceinline__ void agg_scatter_body(const AggArgs& A, const unsigned int* offs, unsigned long long* staged, int shift,
+    unsigned int nb, int64_t chunk)
+{
+    extern __shared__ unsigned int cur[];
+    for (unsigned int b = threadIdx.x; b < nb; b += blockDim.x) cur[b] = offs[(uint64_t)b * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const int64_t r0 = blockIdx.x * chunk;
+    const int64_t r1 = r0 + chunk < A.nrows ? r0 + chunk : A.nrows;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+        uint64_t st[W];
+        if (!P::template stage<KW>(A, A.row_base + i, st)) continue;
+        uint64_t key[KW];
 #pragma unroll
-        for (int w = 0; w < W; ++w) staged[(uint64_t)i * W + w] = st[w];
-        bucket[i] = b;
+        for (int w = 0; w < KW; ++w) key[w] = st[w];
+        const unsigned int b = (unsigned int)(key_place_hash<KW>(key) >> shift);
+        const unsigned int pos = atomicAdd(&cur[b], 1u);
+#pragma unroll
+        for (int w = 0; w < W; ++w) staged[(uint64_t)pos * W + w] = st[w];
     }
 }
 

@@ -1,4 +1,4 @@
-// Device-wide scan and stable radix sorts (hipCUB over rocPRIM) used by the
+// Device-wide scan and stable radix sort (hipCUB over rocPRIM) used by the
 // join build and the bucketed aggregate. Temporary storage comes from the library's caching pool.
 #include <hip/hip_runtime.h>
 
@@ -38,48 +38,6 @@ bool sort_pairs_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t*
         end_bit, stream()), "sort");
     dfree(tmp);
     return ok;
-}
-
-namespace {
-
-// A staged row of W 64-bit words, moved as one value by the sort.
-template <int W>
-struct Words {
-    unsigned long long w[W];
-};
-
-template <int W>
-bool sort_words(const uint32_t* keys_in, uint32_t* keys_out, const void* vals_in, void* vals_out, int n, int end_bit)
-{
-    const Words<W>* vi = (const Words<W>*)vals_in;
-    Words<W>* vo = (Words<W>*)vals_out;
-    size_t bytes = 0;
-    if (!hip_ok(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_in, keys_out, vi, vo, n, 0, end_bit, stream()), "sort size")) {
-        return false;
-    }
-    void* tmp = dalloc(bytes ? bytes : 16);
-    if (!tmp) return false;
-    const bool ok = hip_ok(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys_in, keys_out, vi, vo, n, 0, end_bit, stream()), "sort");
-    dfree(tmp);
-    return ok;
-}
-
-}  // namespace
-
-bool sort_pairs_u32_words(int words, const uint32_t* keys_in, uint32_t* keys_out, const void* vals_in, void* vals_out, int64_t n,
-    int end_bit)
-{
-    if (n <= 0) return true;
-    if (n > (int64_t)0x7fffffff) return fail("sort of %lld items exceeds 2^31", (long long)n);
-    switch (words) {
-    case 2: return sort_words<2>(keys_in, keys_out, vals_in, vals_out, (int)n, end_bit);
-    case 3: return sort_words<3>(keys_in, keys_out, vals_in, vals_out, (int)n, end_bit);
-    case 4: return sort_words<4>(keys_in, keys_out, vals_in, vals_out, (int)n, end_bit);
-    case 5: return sort_words<5>(keys_in, keys_out, vals_in, vals_out, (int)n, end_bit);
-    case 6: return sort_words<6>(keys_in, keys_out, vals_in, vals_out, (int)n, end_bit);
-    case 8: return sort_words<8>(keys_in, keys_out, vals_in, vals_out, (int)n, end_bit);
-    default: return fail("no sort for staged rows of %d words", words);
-    }
 }
 
 }  // namespace qdbgpu

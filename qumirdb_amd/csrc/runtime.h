@@ -70,9 +70,6 @@ int grid_for(int64_t items, int block, int per_cu);
 bool exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n);
 bool sort_pairs_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out, int64_t n,
     int end_bit);
-// The same sort with values of `words` 64-bit words (2..6 or 8) per item.
-bool sort_pairs_u32_words(int words, const uint32_t* keys_in, uint32_t* keys_out, const void* vals_in, void* vals_out, int64_t n,
-    int end_bit);
 
 // Device-resident copy of a host struct (uploaded once, freed with dfree).
 void* upload(const void* host, size_t bytes);

@@ -73,8 +73,8 @@ def test_generated_source_fixes_the_layout():
     src = _source(q.lib(), q, W.groups_spec().to_c()).decode()
     # COUNT(*), SUM(i64), MIN/MAX(f64), SUM(f64) = rows, sum, min, max, 4 limbs, side word
     assert "static constexpr int kNW = 9;" in src
-    assert src.count("read_col_t<") == 4
+    assert src.count("read_col_t<") == 4  # contribution and staging
     assert "A.words[" not in src and "eval_program" not in src
-    for k in ("qdb_agg_update", "qdb_agg_batch", "qdb_agg_stage", "qdb_agg_sorted", "qdb_agg_merge"):
+    for k in ("qdb_agg_update", "qdb_agg_batch", "qdb_agg_count", "qdb_agg_scatter", "qdb_agg_sorted", "qdb_agg_merge"):
         assert k in src, k
     assert "static constexpr int kStage = 3;" in src  # key, SUM(i64) argument, f64 argument
