@@ -381,6 +381,7 @@ struct LdsTable {
     unsigned int* count;
     uint64_t* recs;
     unsigned int* inserted;   // HBM groups this workgroup inserted
+    int fresh;                // this thread's last HBM lookup inserted its key
 };
 
 // Finds or inserts `key` in the HBM table: the slot of its record, kNoRoom
@@ -406,6 +407,7 @@ __device__ __forceinline__ int64_t global_find_or_insert(const AggArgs& A, LdsTa
                 c = atomicCAS(kp, kEmptyKey, (unsigned long long)key[0]);
                 if (c == kEmptyKey) {
                     atomicAdd(T.inserted, 1u);
+                    T.fresh = 1;
                     return (int64_t)slot;
                 }
             }
@@ -425,6 +427,7 @@ __device__ __forceinline__ int64_t global_find_or_insert(const AggArgs& A, LdsTa
                 for (int w = 0; w < KW; ++w) __hip_atomic_store(&r[w], key[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(&A.ctrl[slot], tag | 2ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 atomicAdd(T.inserted, 1u);
+                T.fresh = 1;
                 return (int64_t)slot;
             }
         }
@@ -663,6 +666,7 @@ __device__ __forceinline__ LdsTable lds_table_at(const AggArgs& A, uint64_t* lds
     T.ctrl = (unsigned int*)(lds_raw + (size_t)A.lds_groups * (KW + P::nwords(A)));
     T.count = T.ctrl + A.lds_slots;
     T.inserted = inserted;
+    T.fresh = 0;
     return T;
 }
 
@@ -959,40 +963,40 @@ __device__ __forceinline__ void agg_sorted_body(const AggArgs& A, const unsigned
         __syncthreads();
         const unsigned int n = *T.count < (unsigned int)A.lds_groups ? *T.count : (unsigned int)A.lds_groups;
         const unsigned int first = s_nrec;
-        for (unsigned int g = threadIdx.x; g < n; g += blockDim.x) {
-            const uint64_t* src = T.recs + (size_t)g * R;
-            unsigned long long* r = out + (uint64_t)(first + g) * R;
-            for (int k = 0; k < R; ++k) r[k] = src[k];
-        }
+        // The LDS records are dense in the output layout: one flat copy.
+        unsigned long long* dst = out + (uint64_t)first * R;
+        for (unsigned int k = threadIdx.x; k < n * (unsigned int)R; k += blockDim.x) dst[k] = T.recs[k];
         if (threadIdx.x == 0) counts[it] = (first + n) | (s_shared ? kSharedItem : 0u);
         __syncthreads();
     }
 }
 
 // Merges one record into the HBM table. Owned: its key occurs once in the
-// item and in no other item, so once the slot is found or claimed the words
-// are combined with plain (coherent) loads and stores; else with atomics. A
-// record that finds no room is kept for the replay after growth. One lookup
-// serves both cases.
+// item and in no other item, so a key this thread inserts takes the
+// record's words with plain (coherent) stores; otherwise (a key already in
+// the table, or a bucket split over items) the words are combined with
+// atomics. A record that finds no room is kept for the replay after growth.
+// One lookup serves every case.
 template <int KW, int S, class P>
 __device__ __forceinline__ void merge_record(const AggArgs& A, LdsTable& T, bool want, bool owned, const uint64_t (&key)[KW],
     const uint64_t (&c)[S])
 {
     int64_t id = 0;
+    T.fresh = 0;
     const int where = locate<KW, S, P>(A, T, false, want, key, &id);
     if (want && where == AT_HBM) {
+        // A key this thread just inserted into an owned bucket's slot: its
+        // words still hold their identities and no one else can update
+        // them, so the record's words are stored as they are.
+        const bool store = owned && T.fresh;
         uint64_t* r = A.recs + (uint64_t)id * (uint64_t)A.rstride + KW;
 #pragma unroll
         for (int w = 0; w < S; ++w) {
             if (w >= P::nwords(A)) continue;
             const uint8_t op = P::op(A, w);
             if (c[w] == identity_of(op) && op != OP_FADD) continue;
-            if (owned) {
-                const uint64_t cur = __hip_atomic_load(&r[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&r[w], combine(op, cur, c[w]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                atomic_apply_global(op, &r[w], c[w]);
-            }
+            if (store) __hip_atomic_store(&r[w], c[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else atomic_apply_global(op, &r[w], c[w]);
         }
     } else if (want) {
         const unsigned int pos = atomicAdd(&A.counters[2], 1u);
@@ -1020,6 +1024,7 @@ __device__ __forceinline__ void agg_merge_body(const AggArgs& A, const unsigned 
     T.recs = nullptr;
     T.ctrl = nullptr;
     T.count = nullptr;
+    T.fresh = 0;
     T.inserted = &s_inserted;
     if (threadIdx.x == 0) s_inserted = 0u;
     __syncthreads();
@@ -1074,6 +1079,7 @@ __global__ void __launch_bounds__(kBlock) k_agg_apply_records(AggArgs A, const u
     T.recs = nullptr;
     T.ctrl = nullptr;
     T.count = nullptr;
+    T.fresh = 0;
     T.inserted = &s_inserted;
     if (threadIdx.x == 0) s_inserted = 0u;
     __syncthreads();
