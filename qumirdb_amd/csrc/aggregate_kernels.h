@@ -845,6 +845,7 @@ struct SortedItem {
 };
 
 constexpr unsigned int kSharedItem = 0x80000000u;  // count flag: an item's records may repeat a key
+constexpr int kStageUnroll = 4;                    // rows per thread per step in the bucketed kernels
 
 // Bucket partition of the staged rows in two passes over the same
 // contiguous row ranges (workgroup k: rows [k * chunk, (k + 1) * chunk)).
@@ -860,16 +861,28 @@ __device__ __forceinline__ void agg_count_body(const AggArgs& A, unsigned int* h
     __syncthreads();
     const int64_t r0 = blockIdx.x * chunk;
     const int64_t r1 = r0 + chunk < A.nrows ? r0 + chunk : A.nrows;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-        uint64_t st[W];
-        unsigned int b = nb - 1u;
-        if (P::template stage<KW>(A, A.row_base + i, st)) {
-            uint64_t key[KW];
+    // kStageUnroll rows per thread per step, staged (loads issued) before
+    // any is used, so that their memory latencies overlap.
+    for (int64_t i0 = r0 + threadIdx.x; i0 < r1; i0 += (int64_t)blockDim.x * kStageUnroll) {
+        uint64_t st[kStageUnroll][W];
+        bool ok[kStageUnroll];
 #pragma unroll
-            for (int w = 0; w < KW; ++w) key[w] = st[w];
-            b = (unsigned int)(key_place_hash<KW>(key) >> shift);
+        for (int u = 0; u < kStageUnroll; ++u) {
+            const int64_t i = i0 + (int64_t)u * blockDim.x;
+            ok[u] = i < r1 && P::template stage<KW>(A, A.row_base + (i < r1 ? i : r0), st[u]);
         }
-        atomicAdd(&lh[b], 1u);
+#pragma unroll
+        for (int u = 0; u < kStageUnroll; ++u) {
+            if (i0 + (int64_t)u * blockDim.x >= r1) continue;
+            unsigned int b = nb - 1u;
+            if (ok[u]) {
+                uint64_t key[KW];
+#pragma unroll
+                for (int w = 0; w < KW; ++w) key[w] = st[u][w];
+                b = (unsigned int)(key_place_hash<KW>(key) >> shift);
+            }
+            atomicAdd(&lh[b], 1u);
+        }
     }
     __syncthreads();
     for (unsigned int b = threadIdx.x; b < nb; b += blockDim.x) hist[(uint64_t)b * gridDim.x + blockIdx.x] = lh[b];
@@ -884,16 +897,25 @@ __device__ __forceinline__ void agg_scatter_body(const AggArgs& A, const unsigne
     __syncthreads();
     const int64_t r0 = blockIdx.x * chunk;
     const int64_t r1 = r0 + chunk < A.nrows ? r0 + chunk : A.nrows;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-        uint64_t st[W];
-        if (!P::template stage<KW>(A, A.row_base + i, st)) continue;
-        uint64_t key[KW];
+    for (int64_t i0 = r0 + threadIdx.x; i0 < r1; i0 += (int64_t)blockDim.x * kStageUnroll) {
+        uint64_t st[kStageUnroll][W];
+        bool ok[kStageUnroll];
 #pragma unroll
-        for (int w = 0; w < KW; ++w) key[w] = st[w];
-        const unsigned int b = (unsigned int)(key_place_hash<KW>(key) >> shift);
-        const unsigned int pos = atomicAdd(&cur[b], 1u);
+        for (int u = 0; u < kStageUnroll; ++u) {
+            const int64_t i = i0 + (int64_t)u * blockDim.x;
+            ok[u] = i < r1 && P::template stage<KW>(A, A.row_base + (i < r1 ? i : r0), st[u]);
+        }
 #pragma unroll
-        for (int w = 0; w < W; ++w) staged[(uint64_t)pos * W + w] = st[w];
+        for (int u = 0; u < kStageUnroll; ++u) {
+            if (!ok[u]) continue;
+            uint64_t key[KW];
+#pragma unroll
+            for (int w = 0; w < KW; ++w) key[w] = st[u][w];
+            const unsigned int b = (unsigned int)(key_place_hash<KW>(key) >> shift);
+            const unsigned int pos = atomicAdd(&cur[b], 1u);
+#pragma unroll
+            for (int w = 0; w < W; ++w) staged[(uint64_t)pos * W + w] = st[u][w];
+        }
     }
 }
 
@@ -927,15 +949,25 @@ __device__ __forceinline__ void agg_sorted_body(const AggArgs& A, const unsigned
         }
         lds_table_reset<KW, P>(A, T, true);
         unsigned long long* out = recs + (uint64_t)w.row0 * (uint64_t)R;
-        for (int64_t base = w.row0 + (int64_t)threadIdx.x - lane; base < w.row1; base += blockDim.x) {
+        for (int64_t base0 = w.row0 + (int64_t)threadIdx.x - lane; base0 < w.row1; base0 += (int64_t)blockDim.x * kStageUnroll) {
+          // kStageUnroll rows per lane: every load issued before the first
+          // row is applied.
+          uint64_t stu[kStageUnroll][W];
+#pragma unroll
+          for (int u = 0; u < kStageUnroll; ++u) {
+              const int64_t i = base0 + (int64_t)u * blockDim.x + lane;
+#pragma unroll
+              for (int k = 0; k < W; ++k) stu[u][k] = staged[(uint64_t)(i < w.row1 ? i : w.row0) * W + k];
+          }
+#pragma unroll
+          for (int u = 0; u < kStageUnroll; ++u) {
+            const int64_t base = base0 + (int64_t)u * blockDim.x;
+            if (base >= w.row1) break;  // wave-uniform
             const int64_t i = base + lane;
             const bool valid = i < w.row1;
-            uint64_t st[W];
-#pragma unroll
-            for (int k = 0; k < W; ++k) st[k] = staged[(uint64_t)(valid ? i : w.row0) * W + k];
             uint64_t key[KW];
             uint64_t c[S];
-            P::template unstage<KW, S>(A, st, key, c);
+            P::template unstage<KW, S>(A, stu[u], key, c);
             int where = valid ? AT_BUSY : AT_NONE;
             int lg = 0;
             while (__any(where == AT_BUSY)) {
@@ -959,6 +991,7 @@ __device__ __forceinline__ void agg_sorted_body(const AggArgs& A, const unsigned
                 }
                 s_shared = 1u;
             }
+          }
         }
         __syncthreads();
         const unsigned int n = *T.count < (unsigned int)A.lds_groups ? *T.count : (unsigned int)A.lds_groups;
