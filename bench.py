@@ -362,7 +362,7 @@ def main():
         j = out["join"]
         roof = {"bound": "hbm", "achieved": round(j["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(j["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(),
-                "kernel": "join probe (k_probe_dense passes)", "kernel_ms": round(j["probe_kernel_ms"], 4)}
+                "kernel": "k_probe_dense (one pass) + k_reduce_partials", "kernel_ms": round(j["probe_kernel_ms"], 4)}
     import ctypes
     from qumirdb_amd import lib
     limits = (ctypes.c_int64 * 2)()
