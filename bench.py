@@ -335,16 +335,28 @@ def main():
     import qumirdb_amd
     qumirdb_amd.init_device(0)
     out = {}
+    import gc
+    from qumirdb_amd import lib as _lib
+
+    def release():
+        """Between legs: the previous leg's buffers back to the driver."""
+        gc.collect()
+        _lib().qdb_gpu_trim_cache()
+
     if args.workload in ("join", "all"):
         out["join"], jd = bench_join(args.steps, args.warmup)
         out["join_hash_table"], jd = bench_join(args.steps, args.warmup, table_mode=1, jd=jd)
         del jd
+        release()
     if args.workload in ("lineitem", "all"):
         out["lineitem"] = bench_lineitem(args.steps, args.warmup)
+        release()
     if args.workload in ("groups", "all"):
         out["groups"] = bench_groups(max(1, args.steps // 2), max(1, args.warmup // 2))
+        release()
     if args.workload in ("batched", "all"):
         out["batched"] = bench_batched()
+        release()
     if args.workload in ("served", "all"):
         out["served"] = bench_served()
     if args.copy_calibration:

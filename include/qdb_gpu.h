@@ -395,6 +395,10 @@ void qdb_gpu_aggregate_destroy(void* ht);
 /* Device memory (from the library's caching pool) and copies. */
 void* qdb_gpu_alloc(size_t bytes);
 void qdb_gpu_free(void* ptr);
+/* Returns the device blocks the library's allocator keeps for reuse to the
+ * driver (it keeps at most 1/8 of the device memory otherwise). No
+ * counterpart in the reference, whose kernels allocate from host heaps. */
+void qdb_gpu_trim_cache(void);
 int qdb_gpu_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int qdb_gpu_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int qdb_gpu_memset(void* dst, int value, size_t bytes);

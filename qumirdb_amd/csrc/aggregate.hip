@@ -24,6 +24,7 @@
 // replays exactly those rows and records, so every row is applied once.
 #include <hip/hip_runtime.h>
 
+#include <stdlib.h>
 #include <string.h>
 
 #include <memory>
@@ -1195,15 +1196,23 @@ bool settle_launch(UpdateLaunch* L, unsigned int cnt[4])
     return true;
 }
 
-// Rows of a batch from which the bucketed path pays, and table capacity
+// Rows of a batch from which the bucketed path pays (below, its fixed
+// costs -- four launches, three host synchronisations -- exceed the atomic
+// path's time), and table capacity
 // (slots, from Init / ExpectRowGroups or growth) that selects it.
-constexpr int64_t kSortedMinRows = (int64_t)1 << 20;
+constexpr int64_t kSortedMinRows = (int64_t)1 << 22;
 constexpr uint64_t kSortedMinSlots = (uint64_t)1 << 20;
 constexpr int64_t kSortedItemRows = (int64_t)1 << 16;
 
 bool sorted_path_wanted(AggPlan* P, const AggTable* T, int64_t rows)
 {
-    return rows >= kSortedMinRows && T->capacity >= kSortedMinSlots && stage_layout(P).words > 0 &&
+    // QDB_GPU_BUCKETED_MIN_ROWS moves the row threshold (0: never).
+    int64_t min_rows = kSortedMinRows;
+    if (const char* e = getenv("QDB_GPU_BUCKETED_MIN_ROWS")) {
+        min_rows = atoll(e);
+        if (min_rows <= 0) return false;
+    }
+    return rows >= min_rows && T->capacity >= kSortedMinSlots && stage_layout(P).words > 0 &&
         plan_jit_function(P, "qdb_agg_sorted") != nullptr;
 }
 
@@ -1437,6 +1446,19 @@ bool batched_update(AggPlan* P, AggTable* const* tables, const TRowSet* const* b
     launches.reserve((size_t)n);
     size_t lds = 0;
     bool ok = true;
+    // One instance (after those updated on their own) gains nothing from
+    // the queue: it takes the plain update.
+    int64_t queued = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t rows = batches[i]->RowCount;
+        queued += rows > 0 && rows <= ((int64_t)1 << 30) && !sorted_path_wanted(P, tables[i], rows);
+    }
+    if (queued <= 1) {
+        for (int64_t i = 0; i < n && ok; ++i) {
+            if (batches[i]->RowCount > 0) ok = table_update(tables[i], batches[i]);
+        }
+        return ok;
+    }
     auto release_all = [&]() {
         for (auto& L : launches) L.release();
     };

@@ -35,6 +35,8 @@ bool hip_ok(hipError_t e, const char* what);
 void* dalloc(size_t bytes);
 void dfree(void* p);
 size_t dalloc_size(const void* p);
+// Returns every cached device block to the driver.
+void trim_device_cache();
 // Pinned host staging memory (hipHostMalloc), also cached.
 void* halloc_pinned(size_t bytes);
 void hfree_pinned(void* p);

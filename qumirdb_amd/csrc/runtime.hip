@@ -28,9 +28,17 @@ struct Runtime {
     hipStream_t stream = nullptr;
     int cus = 256;
     size_t max_lds = 64 * 1024;
-    // pool: size class -> free blocks; block -> size class
-    std::map<size_t, std::vector<void*>> free_dev;
+    // pool: block size -> free blocks (with the order they were freed in);
+    // block -> size
+    struct FreeBlock {
+        void* p;
+        uint64_t freed;
+    };
+    std::multimap<size_t, FreeBlock> free_dev;
     std::unordered_map<void*, size_t> dev_sizes;
+    uint64_t frees = 0;
+    size_t cached = 0;          // bytes in free_dev
+    size_t reserve = 0;         // device memory the pool leaves free for the HIP runtime and other allocators
     std::map<size_t, std::vector<void*>> free_host;
     std::unordered_map<void*, size_t> host_sizes;
     // timing
@@ -54,8 +62,20 @@ size_t size_class(size_t bytes)
         while (c < bytes) c <<= 1;
         return c;
     }
-    const size_t gran = 2u << 20;
-    return (bytes + gran - 1) / gran * gran;
+    if (bytes <= ((size_t)64 << 20)) {
+        const size_t gran = 2u << 20;
+        return (bytes + gran - 1) / gran * gran;
+    }
+    // Above 64 MiB: four classes per power of two (1, 1.25, 1.5, 1.75 x
+    // 2^k), so that buffers of varying size (per-query tables and staging)
+    // reuse cached blocks, at most 25% larger than asked.
+    size_t p2 = (size_t)64 << 20;
+    while (p2 * 2 <= bytes) p2 *= 2;
+    for (int q = 1; q <= 4; ++q) {
+        const size_t c = p2 + p2 / 4 * (size_t)q;
+        if (bytes <= c) return bytes <= p2 ? p2 : c;
+    }
+    return p2 * 2;
 }
 
 }  // namespace
@@ -99,6 +119,8 @@ bool ensure_init()
     hipDeviceProp_t prop;
     if (!hip_ok(hipGetDeviceProperties(&prop, r.device), "hipGetDeviceProperties")) return false;
     r.cus = prop.multiProcessorCount;
+    // The pool leaves 1/8 of the device memory (36 GB of 288 GB) free.
+    r.reserve = prop.totalGlobalMem / 8;
     // Largest dynamic LDS allocation of one workgroup (160 KiB on gfx950).
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, r.device) == hipSuccess && lds > 0) {
@@ -140,32 +162,81 @@ int num_cus()
     return R().cus;
 }
 
+// Largest cached block a request of class `c` may take: blocks above
+// 64 MiB serve requests down to 2/3 of their size (per-query tables and
+// staging vary in size from call to call); smaller blocks serve their own
+// class only.
+static size_t reuse_limit(size_t c)
+{
+    return c > ((size_t)64 << 20) ? c + c / 2 : c;
+}
+
+static void evict_oldest_locked(Runtime& r)
+{
+    auto oldest = r.free_dev.begin();
+    for (auto f = r.free_dev.begin(); f != r.free_dev.end(); ++f) {
+        if (f->second.freed < oldest->second.freed) oldest = f;
+    }
+    r.cached -= oldest->first;
+    r.dev_sizes.erase(oldest->second.p);
+    (void)hipFree(oldest->second.p);
+    r.free_dev.erase(oldest);
+}
+
+// Returns the blocks cached longest to the driver until the device has
+// `need` bytes free beyond the reserve (or nothing is cached).
+static void make_room_locked(Runtime& r, size_t need)
+{
+    while (!r.free_dev.empty()) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            (void)hipGetLastError();
+            return;
+        }
+        if (free_b >= need + r.reserve) return;
+        evict_oldest_locked(r);
+    }
+}
+
+static void release_cached_locked(Runtime& r)
+{
+    for (auto& kv : r.free_dev) {
+        r.dev_sizes.erase(kv.second.p);
+        (void)hipFree(kv.second.p);
+    }
+    r.free_dev.clear();
+    r.cached = 0;
+}
+
 void* dalloc(size_t bytes)
 {
     if (!ensure_init()) return nullptr;
     Runtime& r = R();
-    size_t c = size_class(bytes);
+    const size_t c = size_class(bytes);
     {
         std::lock_guard<std::mutex> lock(r.mu);
-        auto it = r.free_dev.find(c);
-        if (it != r.free_dev.end() && !it->second.empty()) {
-            void* p = it->second.back();
-            it->second.pop_back();
+        // Best fit: the smallest cached block of at least class c.
+        auto it = r.free_dev.lower_bound(c);
+        if (it != r.free_dev.end() && it->first <= reuse_limit(c)) {
+            void* p = it->second.p;
+            r.cached -= it->first;
+            r.free_dev.erase(it);
             return p;
         }
     }
     void* p = nullptr;
+    {
+        // A new block leaves the reserve free, if cached blocks can make
+        // room for it.
+        std::lock_guard<std::mutex> lock(r.mu);
+        if (r.cached) make_room_locked(r, c);
+    }
     hipError_t e = hipMalloc(&p, c);
     if (e != hipSuccess) {
         // Return cached blocks to the driver and retry once.
         std::lock_guard<std::mutex> lock(r.mu);
-        for (auto& kv : r.free_dev) {
-            for (void* q : kv.second) {
-                r.dev_sizes.erase(q);
-                (void)hipFree(q);
-            }
-            kv.second.clear();
-        }
+        release_cached_locked(r);
+        (void)hipGetLastError();
         e = hipMalloc(&p, c);
         if (e != hipSuccess) {
             fail("hipMalloc(%zu): %s", c, hipGetErrorString(e));
@@ -184,7 +255,23 @@ void dfree(void* p)
     std::lock_guard<std::mutex> lock(r.mu);
     auto it = r.dev_sizes.find(p);
     if (it == r.dev_sizes.end()) return;
-    r.free_dev[it->second].push_back(p);
+    const size_t sz = it->second;
+    r.free_dev.emplace(sz, Runtime::FreeBlock{p, ++r.frees});
+    r.cached += sz;
+    // Freed blocks stay cached while the device keeps the reserve free (the
+    // HIP runtime and other allocators need memory of their own); past
+    // that, the blocks cached longest go back to the driver first, so the
+    // pool follows the sizes the current work asks for. hipFree waits for
+    // the device, so a block still in use by queued work is safe to
+    // release. Below 1/8 of the device cached, free memory is not checked.
+    if (r.cached > r.reserve) make_room_locked(r, 0);
+}
+
+void trim_device_cache()
+{
+    Runtime& r = R();
+    std::lock_guard<std::mutex> lock(r.mu);
+    release_cached_locked(r);
 }
 
 size_t dalloc_size(const void* p)
@@ -319,6 +406,13 @@ extern "C" const char* qdb_gpu_version(void)
 extern "C" void* qdb_gpu_alloc(size_t bytes)
 {
     return dalloc(bytes);
+}
+
+extern "C" void qdb_gpu_trim_cache(void)
+{
+    if (!qdbgpu::ensure_init()) return;
+    (void)hipStreamSynchronize(qdbgpu::stream());
+    qdbgpu::trim_device_cache();
 }
 
 extern "C" void qdb_gpu_free(void* p)

@@ -152,12 +152,13 @@ def test_batched_update_equals_plain(gpu_backend):
     q.lib().qdb_gpu_release_slot(q.abi.SLOT_AGGREGATE, kern.k.Slot)
 
 
-def test_bucketed_aggregate_random_vs_oracle(gpu_backend):
-    """The bucketed path (rows staged, sorted by key-hash bucket, each bucket
-    aggregated in LDS, merged into the HBM table) taken by a batch of at
-    least 2^20 rows into a table of at least 2^20 slots: nullable two-column
-    keys, an i128 argument, a selection, every reducer; then a second batch
-    into the same table through the same path."""
+def test_bucketed_aggregate_random_vs_oracle(gpu_backend, monkeypatch):
+    """The bucketed path (rows staged and partitioned by key-hash bucket,
+    each bucket aggregated in LDS, merged into the HBM table), taken here
+    from 2^20 rows (QDB_GPU_BUCKETED_MIN_ROWS) into a table of at least 2^20
+    slots: nullable two-column keys, an i128 argument, a selection, every
+    reducer; then a second batch into the same table through the same path."""
+    monkeypatch.setenv("QDB_GPU_BUCKETED_MIN_ROWS", str(1 << 20))
     case, b = random_agg_case(21, 1_200_000)
     got = sorted(result_rows(gpu_backend.aggregate(case, [b, b], capacity=1 << 21)), key=sort_key)
     want = sorted(result_rows(OracleBackend(fsum_mode=2).aggregate(case, [b, b])), key=sort_key)
