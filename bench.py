@@ -15,7 +15,8 @@ project + aggregate) and the high-cardinality Zipf group-by over 100M rows.
 roofline: algorithmic bytes of the dominant kernel (the probe: 16 B per
 probe row read + 16 B per pair written) over its average duration, timed
 with HIP events on the library's stream. cpu_baseline: the CPU oracle (a
-restatement of the reference algorithm, one thread) on a bounded sample.
+restatement of the reference algorithm) over the whole join, its probe in
+262,144-row morsels on the host's threads, and on one thread beside it.
 """
 from __future__ import annotations
 
@@ -269,25 +270,45 @@ def bench_served(qps_levels=(50, 100, 200, 500, 1000, 2000), n_requests=120):
             "resident_bytes": res.table_bytes()}
 
 
+def cpu_threads():
+    """Host threads for the CPU baseline: the process's CPU share, at most
+    OMP_NUM_THREADS (16 per GPU here)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    cap = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(n, cap))
+
+
 def cpu_join_baseline(build_rows=1_500_000, sample_probe=60_000_000):
-    """The oracle's row-table join (one thread) over the filtered build and
-    `sample_probe` probe rows (by default the whole workload)."""
+    """The oracle's row-table join over the filtered build and `sample_probe`
+    probe rows (by default the whole workload): build on one thread, then the
+    probe over 262,144-row morsels on the host's threads (the value), and the
+    same join on one thread."""
     import numpy as np
     from oracle import oracle as O
     from qumirdb_amd import workloads as W
     key, seg = O.gen_join_build(W.SEED_JOIN, 0, build_rows, build_rows)
     fk, val = O.gen_join_probe(W.SEED_JOIN, 0, sample_probe, build_rows)
-    t0 = time.perf_counter()
-    oj = O.OracleJoin(W.join_spec().to_c())
-    oj.append(O.HostRowSet(W.BUILD_TYPES, [key, seg], selection=(seg == 1).astype(np.uint8)))
-    oj.build()
-    probe = O.HostRowSet(W.PROBE_TYPES, [fk, val])  # kept alive across the call
-    n = O.LIB.oracle_join_probe(oj.h, probe.ptr)
-    t1 = time.perf_counter()
+    threads = cpu_threads()
     rows = build_rows + sample_probe
-    return {"value": rows / (t1 - t0), "unit": "rows/s", "cores": 1, "kind": "port",
-            "sample": f"build {build_rows} rows (filter segment = 1) + probe {sample_probe} of 60M rows, {n} pairs, "
-                      f"{t1 - t0:.2f} s, one thread"}
+    probe = O.HostRowSet(W.PROBE_TYPES, [fk, val])  # kept alive across the calls
+    build = O.HostRowSet(W.BUILD_TYPES, [key, seg], selection=(seg == 1).astype(np.uint8))
+    times = {}
+    for t in (1, threads):
+        t0 = time.perf_counter()
+        oj = O.OracleJoin(W.join_spec().to_c())
+        oj.append(build)
+        oj.build()
+        n = O.LIB.oracle_join_probe_parallel(oj.h, probe.ptr, t) if t > 1 else O.LIB.oracle_join_probe(oj.h, probe.ptr)
+        times[t] = time.perf_counter() - t0
+        del oj
+    return {"value": rows / times[threads], "unit": "rows/s", "cores": threads, "kind": "port",
+            "one_thread_value": rows / times[1],
+            "sample": f"build {build_rows} rows (filter segment = 1) + probe {sample_probe} of 60M rows, {n} pairs: "
+                      f"{times[threads]:.2f} s on {threads} threads (probe in 262,144-row morsels), "
+                      f"{times[1]:.2f} s on one thread"}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_probe_dense__join_build1.5M_probe60M.md")

@@ -1260,38 +1260,42 @@ void oracle_join_build(OracleJoin* j)
     }
 }
 
-static void push_pair(OracleJoin* j, int64_t probe_row, int64_t build)
+typedef struct PairBuf {
+    int64_t* pairs;     /* 2 i64 each */
+    int64_t n, cap;
+} PairBuf;
+
+/* jr_push_pair (qdb/kernel/join/join_rows.oz:117-125), pb_push x1.5 growth
+ * (qdb/kernel/join/join_table.oz:51-74). */
+static void push_pair(const OracleJoin* j, PairBuf* b, int64_t probe_row, int64_t build)
 {
-    if (j->npairs == j->pair_cap) {
-        j->pair_cap = j->pair_cap ? j->pair_cap + j->pair_cap / 2 : 4;
-        j->pairs = (int64_t*)realloc(j->pairs, (size_t)j->pair_cap * 16);
+    if (b->n == b->cap) {
+        b->cap = b->cap ? b->cap + b->cap / 2 : 4;
+        b->pairs = (int64_t*)realloc(b->pairs, (size_t)b->cap * 16);
     }
     int64_t stream_id = (int64_t)((uint64_t)-1 << 32) + probe_row;
     if (j->spec.ProbeIsLeft) {
-        j->pairs[2 * j->npairs] = stream_id;
-        j->pairs[2 * j->npairs + 1] = build;
+        b->pairs[2 * b->n] = stream_id;
+        b->pairs[2 * b->n + 1] = build;
     } else {
-        j->pairs[2 * j->npairs] = build;
-        j->pairs[2 * j->npairs + 1] = stream_id;
+        b->pairs[2 * b->n] = build;
+        b->pairs[2 * b->n + 1] = stream_id;
     }
-    j->npairs++;
+    b->n++;
 }
 
-/* jr_probe_batch: pairs of (stream row id, build ordinal) in probe order;
- * a chain emits its rows in build order. Returns the pair count. The pair
- * buffer is reset by every probe (the processor drains it per batch). */
-int64_t oracle_join_probe(OracleJoin* j, const TRowSet* rs)
+/* Probe rows [r0, r1) of rs into b (the table is only read). */
+static void probe_rows(const OracleJoin* j, const TRowSet* rs, int64_t r0, int64_t r1, PairBuf* b)
 {
     const QdbRowJoinSpec* s = &j->spec;
     int outer = s->JoinType != QDB_JOIN_INNER;
-    j->npairs = 0;
-    for (int64_t i = 0; i < rs->RowCount; ++i) {
+    for (int64_t i = r0; i < r1; ++i) {
         if (rs->Selection && !rs->Selection[i]) continue;
         GroupKey key;
         int valid;
         join_read_key(&s->Probe, s->ProbeKeyColumns, s->NumKeys, rs, i, &key, &valid);
         if (!valid) {
-            if (outer) push_pair(j, i, -1);
+            if (outer) push_pair(j, b, i, -1);
             continue;
         }
         int matched = 0;
@@ -1303,7 +1307,7 @@ int64_t oracle_join_probe(OracleJoin* j, const TRowSet* rs)
                 int64_t head = (int64_t)(e & 0xffffffffffffull) - 1;
                 if (key_equal(&j->key[head], &key, s->NumKeys)) {
                     for (int64_t cur = head + 1; cur != 0; cur = j->next[cur - 1]) {
-                        push_pair(j, i, j->ordinal[cur - 1]);
+                        push_pair(j, b, i, j->ordinal[cur - 1]);
                     }
                     matched = 1;
                     break;
@@ -1312,9 +1316,80 @@ int64_t oracle_join_probe(OracleJoin* j, const TRowSet* rs)
                 e = rt_seek(j, off, salt, &off);
             }
         }
-        if (!matched && outer) push_pair(j, i, -1);
+        if (!matched && outer) push_pair(j, b, i, -1);
     }
+}
+
+/* jr_probe_batch: pairs of (stream row id, build ordinal) in probe order;
+ * a chain emits its rows in build order. Returns the pair count. The pair
+ * buffer is reset by every probe (the processor drains it per batch). */
+int64_t oracle_join_probe(OracleJoin* j, const TRowSet* rs)
+{
+    PairBuf b = {j->pairs, 0, j->pair_cap};
+    probe_rows(j, rs, 0, rs->RowCount, &b);
+    j->pairs = b.pairs;
+    j->pair_cap = b.cap;
+    j->npairs = b.n;
     return j->npairs;
+}
+
+/* The same probe over morsels of 262,144 rows (the reference's batch size,
+ * qdb/io/parquet/source.cpp:333) taken by `threads` threads from a shared
+ * counter, each into its own pair buffer; the buffers are joined in morsel
+ * order, so the pairs equal oracle_join_probe's. For the CPU baseline. */
+typedef struct ProbeWork {
+    const OracleJoin* j;
+    const TRowSet* rs;
+    int64_t morsels, next;
+    PairBuf* out;       /* one per morsel */
+} ProbeWork;
+
+enum { kMorselRows = 262144 };
+
+static void* probe_worker(void* arg)
+{
+    ProbeWork* w = (ProbeWork*)arg;
+    for (;;) {
+        int64_t m = __atomic_fetch_add(&w->next, 1, __ATOMIC_RELAXED);
+        if (m >= w->morsels) return NULL;
+        int64_t r0 = m * kMorselRows, r1 = r0 + kMorselRows;
+        if (r1 > w->rs->RowCount) r1 = w->rs->RowCount;
+        probe_rows(w->j, w->rs, r0, r1, &w->out[m]);
+    }
+}
+
+int64_t oracle_join_probe_parallel(OracleJoin* j, const TRowSet* rs, int threads)
+{
+    ProbeWork w;
+    w.j = j;
+    w.rs = rs;
+    w.morsels = (rs->RowCount + kMorselRows - 1) / kMorselRows;
+    w.next = 0;
+    w.out = (PairBuf*)calloc((size_t)(w.morsels ? w.morsels : 1), sizeof(PairBuf));
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t tid[256];
+    int started = 0;
+    for (int t = 1; t < threads; ++t) {
+        if (pthread_create(&tid[started], NULL, probe_worker, &w) == 0) started++;
+    }
+    probe_worker(&w);
+    for (int t = 0; t < started; ++t) pthread_join(tid[t], NULL);
+    int64_t total = 0;
+    for (int64_t m = 0; m < w.morsels; ++m) total += w.out[m].n;
+    if (total > j->pair_cap) {
+        j->pair_cap = total;
+        j->pairs = (int64_t*)realloc(j->pairs, (size_t)total * 16);
+    }
+    int64_t at = 0;
+    for (int64_t m = 0; m < w.morsels; ++m) {
+        if (w.out[m].n) memcpy(j->pairs + 2 * at, w.out[m].pairs, (size_t)w.out[m].n * 16);
+        at += w.out[m].n;
+        free(w.out[m].pairs);
+    }
+    free(w.out);
+    j->npairs = total;
+    return total;
 }
 
 const int64_t* oracle_join_pairs(const OracleJoin* j)

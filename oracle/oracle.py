@@ -70,6 +70,7 @@ def _lib() -> C.CDLL:
         "oracle_join_append": (None, [P, C.POINTER(abi.TRowSet)]),
         "oracle_join_build": (None, [P]),
         "oracle_join_probe": (i64, [P, C.POINTER(abi.TRowSet)]),
+        "oracle_join_probe_parallel": (i64, [P, C.POINTER(abi.TRowSet), C.c_int]),
         "oracle_join_pairs": (C.POINTER(C.c_int64), [P]),
         "oracle_join_keys": (i64, [P]),
         "oracle_join_rows": (i64, [P]),

@@ -225,6 +225,27 @@ def test_join_chains_in_build_order():
     assert j.keys() == 3 and j.rows() == 5
 
 
+@pytest.mark.parametrize("outer", [False, True])
+def test_join_parallel_probe_equals_serial(outer):
+    """The morsel-parallel probe (the CPU baseline's multi-thread leg) emits
+    the serial probe's pairs in the same order: 700K probe rows over three
+    morsels, duplicate build keys, NULL keys and a selection."""
+    rng = np.random.default_rng(11)
+    nb, npr = 5000, 700_000
+    types = [T(SIGNED, 8, nullable=True)]
+    case = JoinCase(types, types, [0], [0], join_type=cases.JOIN_LEFT_OUTER if outer else cases.JOIN_INNER)
+    j = O.OracleJoin(case.to_c())
+    j.append(O.HostRowSet(types, [rng.integers(0, 4000, nb).astype(np.int64)], [rng.random(nb) > 0.05]))
+    j.build()
+    probe = O.HostRowSet(types, [rng.integers(0, 5000, npr).astype(np.int64)], [rng.random(npr) > 0.05],
+                         selection=(rng.random(npr) > 0.1).astype(np.uint8))
+    serial = j.probe(probe)
+    n = O.LIB.oracle_join_probe_parallel(j.h, probe.ptr, 4)
+    assert n == len(serial) > 0
+    par = np.ctypeslib.as_array(O.LIB.oracle_join_pairs(j.h), shape=(2 * n,)).reshape(n, 2)
+    assert np.array_equal(par, serial)
+
+
 def test_three_valued_filter():
     """NULL is not TRUE: x > 0 OR y > 0 with NULLs (SQL three-valued logic)."""
     E = cases._abi()
