@@ -25,6 +25,7 @@
 //   Materialize gathers probe and build columns for a range of pairs.
 #include <hip/hip_runtime.h>
 
+#include <stdlib.h>
 #include <string.h>
 
 #include <mutex>
@@ -873,6 +874,187 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe_dense(ProbeArgs P)
         }
     }
     if (P.nfused) fused_flush<kAcc, kMode == kDirectVal>(P, acc, v_pairs, v_lo, v_hi, v_min, v_max, lane);
+}
+
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t x)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// The dense probe's common case, pipelined: a non-nullable 8-byte integer
+// key, no probe selection, an inner join, and fused aggregates that count
+// pairs or read one non-nullable 8-byte probe column (kVal); MIN/MAX words
+// are kept only when a fused aggregate asks (kMinMax); kLeft: the probe
+// side is the left one. A wave tile is kLeanRows rows per lane. While a
+// tile's ordinals are gathered and its pairs placed and stored, the keys
+// (and probe column) of the wave's next tile are already loading: they are
+// issued right after the gathers, and memory returns in issue order, so
+// the wait for the gathers does not wait for them. A tile loads through
+// one address per lane with immediate offsets (the batch holds at least
+// one wave tile: a tile that would run past the end is moved back to end
+// at the last row, its rows before the tile start left out); pairs are
+// stored at 32-bit offsets from the wave's output base; the pair count is
+// per wave. Output order within a wave tile: row-major over r, as in
+// k_probe_dense.
+constexpr int kLeanRows = 8;
+constexpr int kLeanWaveTile = kWave * kLeanRows;
+constexpr uint64_t kLeanBlockTile = (uint64_t)(kProbeBlock / kWave) * kLeanWaveTile;
+
+struct LeanState {
+    unsigned long long pairs;   // this wave's pairs
+    uint64_t lo, hi;            // SUM of the probe column (two's complement, 128 bits)
+    int64_t mn, mx;
+};
+
+// The start of the wave tile at t0, moved back to end at the last row when
+// it would run past the end.
+__device__ __forceinline__ uint64_t lean_base(uint64_t t0, uint64_t n)
+{
+    return t0 + kLeanWaveTile <= n ? t0 : n - kLeanWaveTile;
+}
+
+template <bool kVal>
+__device__ __forceinline__ void lean_load(const ProbeArgs& P, uint64_t tl, int lane, uint64_t (&key)[kLeanRows],
+    int64_t (&val)[kLeanRows])
+{
+    const uint64_t* kp = (const uint64_t*)P.cols.c[0].data + tl + lane;
+    const int64_t* vp = kVal ? (const int64_t*)P.cols.c[P.val_slot].data + tl + lane : nullptr;
+#pragma unroll
+    for (int r = 0; r < kLeanRows; ++r) {
+        key[r] = __builtin_nontemporal_load(kp + r * kWave);
+        val[r] = kVal ? __builtin_nontemporal_load(vp + r * kWave) : 0;
+    }
+}
+
+// One wave tile (rows tl + r * 64 + lane; those before t0 left out) whose
+// keys and probe column are in key/val; loads the tile at ntl into
+// nkey/nval on the way.
+template <bool kVal, bool kMinMax, bool kLeft>
+__device__ __forceinline__ void lean_tile(const ProbeArgs& P, const uint32_t* lbits, uint32_t lds_range, int wave, int lane,
+    uint64_t t0, uint64_t tl, const uint64_t (&key)[kLeanRows], const int64_t (&val)[kLeanRows], uint64_t ntl,
+    uint64_t (&nkey)[kLeanRows], int64_t (&nval)[kLeanRows], LeanState& st)
+{
+    const uint64_t n = (uint64_t)P.n;
+    const uint32_t skip = (uint32_t)(t0 < n ? t0 - tl : kLeanWaveTile);
+    // Round 2: the LDS bitmap words.
+    uint32_t ixs[kLeanRows], wl[kLeanRows];
+    bool inr[kLeanRows], cov[kLeanRows];
+#pragma unroll
+    for (int r = 0; r < kLeanRows; ++r) {
+        const uint64_t idx = key[r] - P.dmin;
+        inr[r] = (lane >= (int)skip - r * kWave) & (idx < P.drange);
+        const uint32_t ix = inr[r] ? (uint32_t)idx : 0u;
+        cov[r] = ix < lds_range;
+        ixs[r] = ix;
+        wl[r] = lbits[cov[r] ? (ix >> 5) : 0u];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // Round 3: the ordinals of the rows that may match; then the next
+    // tile's loads.
+    uint32_t head[kLeanRows];
+    bool need[kLeanRows];
+#pragma unroll
+    for (int r = 0; r < kLeanRows; ++r) {
+        const bool bit = ((wl[r] >> (ixs[r] & 31u)) & 1u) != 0u;
+        need[r] = inr[r] & (!cov[r] | bit);
+        // Only the lanes that may match issue the gather.
+        head[r] = kAbsent;
+        if (need[r]) head[r] = P.dtab[ixs[r]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    lean_load<kVal>(P, ntl, lane, nkey, nval);
+    __builtin_amdgcn_sched_barrier(0);
+    bool hit[kLeanRows];
+#pragma unroll
+    for (int r = 0; r < kLeanRows; ++r) hit[r] = need[r] & (head[r] != kAbsent);
+    if (kVal) {
+#pragma unroll
+        for (int r = 0; r < kLeanRows; ++r) {
+            const int64_t x = hit[r] ? val[r] : 0;
+            const uint64_t lo = st.lo + (uint64_t)x;
+            st.hi += (lo < st.lo ? 1ull : 0ull) + (x < 0 ? ~0ull : 0ull);
+            st.lo = lo;
+            if (kMinMax) {
+                st.mn = hit[r] && x < st.mn ? x : st.mn;
+                st.mx = hit[r] && x > st.mx ? x : st.mx;
+            }
+        }
+    }
+    uint64_t m[kLeanRows];
+    uint32_t run = 0;
+#pragma unroll
+    for (int r = 0; r < kLeanRows; ++r) {
+        m[r] = __ballot(hit[r]);
+        run += (uint32_t)__popcll(m[r]);
+    }
+    const uint64_t pos0 = uniform_u64(reserve_block_tile(P.pair_count, run, wave, lane));
+    st.pairs += run;
+    // Pairs in probe row order.
+    ll2* out = (ll2*)(P.pairs + 2 * pos0);
+    const bool fits = pos0 + run <= P.pair_cap;
+    const int64_t sid0 = (int64_t)(tl - (1ull << 32)) + lane;  // stream id of row tl + lane
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t at = 0;
+#pragma unroll
+    for (int r = 0; r < kLeanRows; ++r) {
+        if (hit[r]) {
+            const uint32_t o = at + (uint32_t)__popcll(m[r] & below);
+            if (fits || pos0 + o < P.pair_cap) {
+                const int64_t sid = sid0 + (int64_t)r * kWave;
+                const int64_t ord = (int64_t)head[r];
+                const ll2 v = kLeft ? ll2{sid, ord} : ll2{ord, sid};
+                __builtin_nontemporal_store(v, out + o);
+            }
+        }
+        at += (uint32_t)__popcll(m[r]);
+    }
+}
+
+template <bool kVal, bool kMinMax, bool kLeft>
+__global__ void __launch_bounds__(kProbeBlock) k_probe_dense_lean(ProbeArgs P)
+{
+    extern __shared__ uint64_t lds64[];
+    const uint32_t* lbits = (const uint32_t*)lds64;
+    {
+        uint32_t* b = (uint32_t*)lds64;
+        for (uint32_t i = threadIdx.x; i < P.lds_words; i += blockDim.x) b[i] = P.dbits[i];
+    }
+    __syncthreads();
+    const uint32_t lds_range = P.lds_words * 32u;
+    const int lane = lane_id();
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const uint64_t n = (uint64_t)P.n;
+    const uint64_t stride = (uint64_t)gridDim.x * kLeanBlockTile;
+    LeanState st;
+    st.pairs = 0;
+    st.lo = 0;
+    st.hi = 0;
+    st.mn = 0x7fffffffffffffffll;
+    st.mx = (int64_t)0x8000000000000000ll;
+    uint64_t ka[kLeanRows], kb[kLeanRows];
+    int64_t va[kLeanRows], vb[kLeanRows];
+    // Every wave of the block runs the same number of tiles (b0 is the
+    // block's); two per trip, the buffers alternating.
+    uint64_t b0 = blockIdx.x * kLeanBlockTile;
+    const uint64_t woff = (uint64_t)wave * kLeanWaveTile;
+    lean_load<kVal>(P, lean_base(b0 + woff, n), lane, ka, va);
+    while (b0 < n) {
+        const uint64_t t0 = b0 + woff, t1 = b0 + stride + woff, t2 = b0 + 2 * stride + woff;
+        lean_tile<kVal, kMinMax, kLeft>(P, lbits, lds_range, wave, lane, t0, lean_base(t0, n), ka, va, lean_base(t1, n), kb, vb, st);
+        b0 += stride;
+        if (b0 >= n) break;
+        lean_tile<kVal, kMinMax, kLeft>(P, lbits, lds_range, wave, lane, t1, lean_base(t1, n), kb, vb, lean_base(t2, n), ka, va, st);
+        b0 += stride;
+    }
+    if (P.nfused) {
+        FusedLane none[1];
+        none[0].count = 0;
+        none[0].lo = 0;
+        none[0].hi = 0;
+        fused_flush<1, true>(P, none, lane == 0 ? st.pairs : 0ull, st.lo, st.hi, st.mn, st.mx, lane);
+    }
 }
 
 template <int KW, int kTab, int kMode>
@@ -1772,13 +1954,55 @@ void launch_dense_one(int grid, size_t lds, const ProbeArgs& A)
     hipLaunchKernelGGL((k_probe_dense<kMode>), dim3(grid), dim3(kProbeBlock), lds, stream(), A);
 }
 
+template <bool kVal, bool kMinMax, bool kLeft>
+void launch_lean_one(int grid, size_t lds, const ProbeArgs& A)
+{
+    static bool attr_set = false;
+    const void* fn = (const void*)k_probe_dense_lean<kVal, kMinMax, kLeft>;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(max_lds_per_block() - static_lds(fn)));
+        (void)hipGetLastError();
+        attr_set = true;
+    }
+    if (lds + static_lds(fn) > max_lds_per_block()) {
+        fail("dense probe: %zu bytes of LDS exceed the workgroup limit", lds + static_lds(fn));
+        return;
+    }
+    hipLaunchKernelGGL((k_probe_dense_lean<kVal, kMinMax, kLeft>), dim3(grid), dim3(kProbeBlock), lds, stream(), A);
+}
+
+template <bool kVal, bool kMinMax>
+void launch_lean_left(bool left, int grid, size_t lds, const ProbeArgs& A)
+{
+    if (left) launch_lean_one<kVal, kMinMax, true>(grid, lds, A);
+    else launch_lean_one<kVal, kMinMax, false>(grid, lds, A);
+}
+
+void launch_dense_lean(bool val, bool minmax, bool left, int grid, size_t lds, const ProbeArgs& A)
+{
+    if (val) {
+        if (minmax) launch_lean_left<true, true>(left, grid, lds, A);
+        else launch_lean_left<true, false>(left, grid, lds, A);
+    } else {
+        launch_lean_left<false, false>(left, grid, lds, A);
+    }
+}
+
 // Dynamic LDS the dense probe of a mode may use: the device limit less the
 // kernel's static LDS.
 template <int kMode>
 size_t dense_lds_budget()
 {
-    const size_t s = static_lds((const void*)k_probe_dense<kMode>);
-    return max_lds_per_block() > s ? max_lds_per_block() - s : 0;
+    static size_t cached = 0;
+    if (cached) return cached;
+    size_t s = static_lds((const void*)k_probe_dense<kMode>);
+    // The lean forms (same LDS use) fit the same budget.
+    const void* lean[] = {(const void*)k_probe_dense_lean<true, true, true>, (const void*)k_probe_dense_lean<true, true, false>,
+        (const void*)k_probe_dense_lean<true, false, true>, (const void*)k_probe_dense_lean<true, false, false>,
+        (const void*)k_probe_dense_lean<false, false, true>, (const void*)k_probe_dense_lean<false, false, false>};
+    for (const void* f : lean) s = s > static_lds(f) ? s : static_lds(f);
+    cached = max_lds_per_block() > s ? max_lds_per_block() - s : 0;
+    return cached;
 }
 
 size_t dense_lds_budget_of(int mode)
@@ -1864,6 +2088,18 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
             }
         }
     }
+    // The dense probe's lean form: direct key, no probe selection, inner
+    // join, fused aggregates that count pairs or read the one direct probe
+    // column (QDB_GPU_JOIN_LEAN=0 keeps the general form).
+    bool lean = J->dense && A.sel == nullptr && !A.emit_unmatched && (mode == kDirectVal || mode == kDirectKey);
+    bool minmax = false;
+    for (int f = 0; f < A.nfused; ++f) {
+        const int kind = A.fused[f].kind;
+        if (mode == kDirectKey && kind != FK_COUNT_PAIRS) lean = false;
+        if (kind == FK_MIN || kind == FK_MAX) minmax = true;
+    }
+    lean = lean && n >= kLeanWaveTile;  // every wave tile lies inside the batch
+    if (const char* e = getenv("QDB_GPU_JOIN_LEAN")) lean = lean && e[0] != '0';
     // The slot table in LDS when it fits in 64 KiB; else the Bloom filter.
     // Dense: as much of the membership bitmap as LDS holds.
     const size_t table_bytes = J->capacity * sizeof(Slot<KW>);
@@ -1917,7 +2153,8 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
         }
         {
             TimedRegion timer("join_probe");
-            launch_probe<KW>(mode, tab, grid, lds, A);
+            if (lean) launch_dense_lean(mode == kDirectVal, minmax, A.probe_is_left != 0, grid, lds, A);
+            else launch_probe<KW>(mode, tab, grid, lds, A);
             if (A.nfused) {
                 ReduceOps ops;
                 for (int k = 0; k < kMaxFused * kAccWords; ++k) {

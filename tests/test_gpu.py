@@ -244,6 +244,51 @@ def test_join_distinct_dense_keys_vs_oracle(gpu_backend, join_type, table_mode):
     assert fused[3]["value"] == int(pval[prow].sum(dtype=np.int64))
 
 
+@pytest.mark.parametrize("probe_is_left", [True, False])
+@pytest.mark.parametrize("fused", ["count", "sum", "minmax"])
+@pytest.mark.parametrize("npr", [1024, 300_007])
+def test_join_dense_lean_equals_general(gpu_backend, monkeypatch, probe_is_left, fused, npr):
+    """The dense probe's lean form (non-nullable 8-byte key, no selection,
+    inner join, fused aggregates over one probe column) against its general
+    form (QDB_GPU_JOIN_LEAN=0) and the oracle: both pair orders, with and
+    without MIN/MAX, a probe of exactly one wave tile and a ragged one."""
+    rng = np.random.default_rng(npr + (3 if probe_is_left else 5))
+    nb = 20000
+    types_b = [T(SIGNED, 8), T(SIGNED, 4)]
+    types_p = [T(SIGNED, 8), T(SIGNED, 8)]
+    bk = rng.permutation(nb).astype(np.int64) + 17
+    bsel = (rng.random(nb) > 0.6).astype(np.uint8)
+    pk = rng.integers(0, nb + 40, npr).astype(np.int64)
+    pval = rng.integers(-10 ** 14, 10 ** 14, npr).astype(np.int64)
+    specs = {"count": [(AGG_COUNT_STAR, -1)], "sum": [(AGG_COUNT_STAR, -1), (AGG_SUM, 1)],
+             "minmax": [(AGG_COUNT_STAR, -1), (AGG_SUM, 1), (AGG_MIN, 1), (AGG_MAX, 1)]}[fused]
+    case = JoinCase(types_b, types_p, [0], [0], probe_is_left=probe_is_left, fused=specs)
+    build = Batch(types_b, [bk, rng.integers(0, 5, nb).astype(np.int32)], None, bsel)
+    probe = Batch(types_p, [pk, pval])
+    got = {}
+    for lean in ("1", "0"):
+        monkeypatch.setenv("QDB_GPU_JOIN_LEAN", lean)
+        kern, proc = gpu_backend.join_processor(case)
+        proc.append(gpu_backend.dev(build))
+        proc.build()
+        pairs = proc.probe_pairs(gpu_backend.dev(probe))
+        got[lean] = (pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))], [(f["value"], f.get("count")) for f in proc.fused_results()])
+        proc.close()
+    oj = O.OracleJoin(case.to_c())
+    oj.append(OracleBackend().host(build))
+    oj.build()
+    opairs = oj.probe(OracleBackend().host(probe))
+    opairs = opairs[np.lexsort((opairs[:, 1], opairs[:, 0]))]
+    assert np.array_equal(got["1"][0], opairs) and np.array_equal(got["0"][0], opairs)
+    assert got["1"][1] == got["0"][1]
+    prow = (opairs[:, 0 if probe_is_left else 1] + (1 << 32)).astype(np.int64)
+    assert got["1"][1][0][0] == len(opairs)
+    if fused != "count":
+        assert got["1"][1][1][0] == int(sum(int(v) for v in pval[prow]))
+    if fused == "minmax":
+        assert got["1"][1][2][0] == int(pval[prow].min()) and got["1"][1][3][0] == int(pval[prow].max())
+
+
 def test_join_multi_batch_build_and_fused(gpu_backend):
     """Build from three batches (ordinals continue across them), duplicate
     keys (chains in build order), fused COUNT(*) / SUM / MIN / MAX."""
