@@ -690,6 +690,31 @@ __device__ __forceinline__ void fused_flush(const ProbeArgs& P, const FusedLane 
 }
 
 
+// Copies `words` 32-bit words from global memory (16-byte aligned) to LDS:
+// each thread's 16-byte loads of a pass are in flight together, then
+// stored (a load-store loop would wait one memory round trip per word).
+__device__ __forceinline__ void fill_lds_words(uint32_t* dst, const uint32_t* src, uint64_t words)
+{
+    constexpr int kU = 10;  // 10 x 16 B x 1024 threads = 160 KiB in one pass
+    const uint64_t v4 = words / 4;
+    const uint4* s4 = (const uint4*)src;
+    uint4* d4 = (uint4*)dst;
+    for (uint64_t base = 0; base < v4; base += (uint64_t)kU * blockDim.x) {
+        uint4 t[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint64_t i = base + (uint64_t)u * blockDim.x + threadIdx.x;
+            t[u] = s4[i < v4 ? i : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint64_t i = base + (uint64_t)u * blockDim.x + threadIdx.x;
+            if (i < v4) d4[i] = t[u];
+        }
+    }
+    for (uint64_t i = v4 * 4 + threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+}
+
 constexpr uint32_t kAbsent = 0xffffffffu;   // dense table: no build row for this key
 
 // Places a block tile's output rows: every wave passes its row count `run`;
@@ -733,11 +758,11 @@ template <int kMode>
 __global__ void __launch_bounds__(kProbeBlock) k_probe_dense(ProbeArgs P)
 {
     constexpr bool kDirect = kMode != kGeneric;
-    extern __shared__ uint64_t lds64[];
+    extern __shared__ __align__(16) uint64_t lds64[];
     const uint32_t* lbits = (const uint32_t*)lds64;
     {
         uint32_t* b = (uint32_t*)lds64;
-        for (uint32_t i = threadIdx.x; i < P.lds_words; i += blockDim.x) b[i] = P.dbits[i];
+        fill_lds_words(b, P.dbits, P.lds_words);
     }
     __syncthreads();
     const uint64_t lds_range = (uint64_t)P.lds_words * 32u;
@@ -1015,11 +1040,11 @@ __device__ __forceinline__ void lean_tile(const ProbeArgs& P, const uint32_t* lb
 template <bool kVal, bool kMinMax, bool kLeft>
 __global__ void __launch_bounds__(kProbeBlock) k_probe_dense_lean(ProbeArgs P)
 {
-    extern __shared__ uint64_t lds64[];
+    extern __shared__ __align__(16) uint64_t lds64[];
     const uint32_t* lbits = (const uint32_t*)lds64;
     {
         uint32_t* b = (uint32_t*)lds64;
-        for (uint32_t i = threadIdx.x; i < P.lds_words; i += blockDim.x) b[i] = P.dbits[i];
+        fill_lds_words(b, P.dbits, P.lds_words);
     }
     __syncthreads();
     const uint32_t lds_range = P.lds_words * 32u;
@@ -1062,15 +1087,15 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe(ProbeArgs P)
 {
     constexpr bool kDirect = kMode != kGeneric;
     constexpr bool kLds = kTab == kTabLds;
-    extern __shared__ uint64_t lds64[];
+    extern __shared__ __align__(16) uint64_t lds64[];
     const Slot<KW>* T = kLds ? (const Slot<KW>*)lds64 : (const Slot<KW>*)P.table;
     const uint32_t* lds_bloom = (const uint32_t*)lds64;
     if (kLds) {
         const uint64_t words = (P.cmask + 1) * (sizeof(Slot<KW>) / 8);
-        for (uint64_t i = threadIdx.x; i < words; i += blockDim.x) lds64[i] = ((const uint64_t*)P.table)[i];
+        fill_lds_words((uint32_t*)lds64, (const uint32_t*)P.table, words * 2);
     } else if (P.bloom_words) {
         uint32_t* b = (uint32_t*)lds64;
-        for (uint32_t i = threadIdx.x; i < P.bloom_words; i += blockDim.x) b[i] = P.bloom[i];
+        fill_lds_words(b, P.bloom, P.bloom_words);
     }
     __syncthreads();
     const int lane = lane_id();
