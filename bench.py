@@ -311,7 +311,7 @@ def cpu_join_baseline(build_rows=1_500_000, sample_probe=60_000_000):
                       f"{times[1]:.2f} s on one thread"}
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_probe_dense__join_build1.5M_probe60M.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_probe_dense_lean__join_build1.5M_probe60M.md")
 
 
 def profiled_traffic():
@@ -395,7 +395,7 @@ def main():
         j = out["join"]
         roof = {"bound": "hbm", "achieved": round(j["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(j["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(),
-                "kernel": "k_probe_dense (one pass) + k_reduce_partials", "kernel_ms": round(j["probe_kernel_ms"], 4)}
+                "kernel": "k_probe_dense_lean (one pass) + k_reduce_partials", "kernel_ms": round(j["probe_kernel_ms"], 4)}
     import ctypes
     from qumirdb_amd import lib
     limits = (ctypes.c_int64 * 2)()
