@@ -154,6 +154,12 @@ __global__ void __launch_bounds__(1024) k_scan_counts(unsigned int* counts, unsi
     }
 }
 
+// The arena's row count grows by an append's kept rows (after its writes).
+__global__ void k_add_rows(unsigned long long* arena_rows, const unsigned long long* added)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *arena_rows += *added;
+}
+
 struct ArenaCols {
     char* data[64];
     uint8_t* valid[64];
@@ -165,10 +171,11 @@ struct ArenaCols {
 // The arena payload is written for every build column the plan keeps.
 template <int KW>
 __global__ void __launch_bounds__(kBlock) k_append_write(DevCols cols, JKeyDesc K, int64_t n, const uint8_t* sel,
-    const unsigned int* block_offsets, int64_t arena_base, int64_t ordinal_base, uint64_t* arena_keys,
+    const unsigned int* block_offsets, const unsigned long long* arena_rows, int64_t ordinal_base, uint64_t* arena_keys,
     int64_t* arena_ordinal, int32_t* ord_to_arena, DevCols payload_cols, int npayload, ArenaCols AC)
 {
     __shared__ unsigned int wave_base[kBlock / kWave];
+    const int64_t arena_base = (int64_t)*arena_rows;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int lane = lane_id();
     const int wave = threadIdx.x / kWave;
@@ -379,9 +386,11 @@ __global__ void __launch_bounds__(kBlock) k_finish_slots(Slot<KW>* table, uint64
 
 // Smallest and largest packed key of a one-word-key arena: a partial per
 // workgroup (part[2 * block], part[2 * block + 1]), then k_reduce_partials.
-__global__ void __launch_bounds__(kBlock) k_key_minmax(const uint64_t* keys, int64_t rows, unsigned long long* part)
+__global__ void __launch_bounds__(kBlock) k_key_minmax(const uint64_t* keys, const unsigned long long* rows_dev,
+    unsigned long long* part)
 {
     __shared__ unsigned long long s_lo[kBlock / kWave], s_hi[kBlock / kWave];
+    const int64_t rows = (int64_t)*rows_dev;
     unsigned long long lo = ~0ull, hi = 0ull;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += stride) {
@@ -450,18 +459,50 @@ __global__ void __launch_bounds__(kBlock) k_reduce_partials(const unsigned long 
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* p, uint64_t n, uint32_t v)
+constexpr uint32_t kAbsent = 0xffffffffu;   // dense table: no build row for this key
+
+// The key range [info[0], info[1]] of the dense table, when it holds at
+// most dense_limit(rows) keys.
+__device__ __forceinline__ bool dense_range(const unsigned long long* info, uint64_t rows, uint64_t* range)
 {
+    const uint64_t lo = info[0], hi = info[1];
+    *range = hi - lo + 1;
+    return rows > 0 && hi >= lo && *range != 0 && *range <= (rows * 8 > (1ull << 16) ? rows * 8 : (1ull << 16));
+}
+
+// Clears the dense table over the key range (no membership bits, kAbsent
+// ordinals), or flags a range too wide (info word 2, high half); copies the
+// row count to info[4].
+__global__ void __launch_bounds__(kBlock) k_dense_prepare(const unsigned long long* rows_dev, unsigned long long* info,
+    uint32_t* dbits, uint32_t* dtab)
+{
+    const uint64_t rows = *rows_dev;
+    uint64_t range = 0;
+    const bool ok = dense_range(info, rows, &range);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        info[4] = rows;
+        if (!ok) ((unsigned int*)(info + 2))[1] = 1u;
+    }
+    if (!ok) return;
+    const uint64_t words = (range + 31) / 32;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < range; i += stride) {
+        dtab[i] = kAbsent;
+        if (i < words) dbits[i] = 0u;
+    }
 }
 
 // The dense table: a membership bit and the build ordinal of every row at
 // key - min (kAbsent where no row has the key); flags[0] = 1 when a key
 // repeats.
-__global__ void __launch_bounds__(kBlock) k_dense_fill(const uint64_t* keys, const int64_t* ordinal, int64_t rows,
-    uint64_t dmin, uint32_t* dbits, uint32_t* dtab, unsigned int* flags)
+__global__ void __launch_bounds__(kBlock) k_dense_fill(const uint64_t* keys, const int64_t* ordinal,
+    const unsigned long long* rows_dev, unsigned long long* info, uint32_t* dbits, uint32_t* dtab)
 {
+    const int64_t rows = (int64_t)*rows_dev;
+    uint64_t range = 0;
+    if (!dense_range(info, (uint64_t)rows, &range)) return;
+    unsigned int* flags = (unsigned int*)(info + 2);
+    const uint64_t dmin = info[0];
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += stride) {
         const uint64_t idx = keys[r] - dmin;
@@ -715,7 +756,6 @@ __device__ __forceinline__ void fill_lds_words(uint32_t* dst, const uint32_t* sr
     for (uint64_t i = v4 * 4 + threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
 }
 
-constexpr uint32_t kAbsent = 0xffffffffu;   // dense table: no build row for this key
 
 // Places a block tile's output rows: every wave passes its row count `run`;
 // returns the wave's first output position. One atomic per block tile (the
@@ -1423,9 +1463,20 @@ struct JoinBuild {
     uint32_t bloom_words = 0;
     int64_t keys_distinct = 0;
     bool built = false;
-    // fused aggregate words, device
-    unsigned long long* fused_acc = nullptr;
-    unsigned long long* pair_count = nullptr;   // device counter
+    // The arena's row count stays on the device while batches are appended
+    // (rows_dev); `rows` is exact once read back (rows_known), `rows_bound`
+    // bounds it meanwhile (the arena is sized for it).
+    unsigned long long* rows_dev = nullptr;
+    int64_t rows_bound = 0;
+    bool rows_known = true;
+    // device: the pair counter (word 0) and then the fused aggregate words
+    // of the last probe, read back together
+    unsigned long long* pair_count = nullptr;
+    unsigned long long* fused_acc = nullptr;    // pair_count + 2
+    // host: that readback; fused_ready when it holds a probe's fused words
+    // not yet added to the totals
+    unsigned long long readback[2 + kMaxFused * kAccWords];
+    bool fused_ready = false;
     // the probe batch the pair buffer refers to (for materialize)
     TRowSet last_probe;
 };
@@ -1623,8 +1674,8 @@ void build_free(JoinBuild* J)
     dfree(J->bloom);
     dfree(J->dbits);
     dfree(J->dtab);
-    dfree(J->fused_acc);
     dfree(J->pair_count);
+    dfree(J->rows_dev);
     delete J;
 }
 
@@ -1645,16 +1696,19 @@ bool grow_buf(T** p, int64_t old_elems, int64_t new_elems, size_t elem_bytes)
 bool arena_reserve(JoinBuild* J, int64_t more_rows, int64_t more_ord)
 {
     JoinPlan* P = J->plan;
-    if (J->rows + more_rows > J->cap) {
+    // Sized for the bound on the rows (the exact count may still be on the
+    // device); growing copies the bound's rows.
+    const int64_t have = J->rows_bound;
+    if (have + more_rows > J->cap) {
         int64_t cap = J->cap ? J->cap : 1024;
-        while (cap < J->rows + more_rows) cap *= 2;
-        if (!grow_buf(&J->keys, J->rows * P->KW, cap * P->KW, 8)) return false;
-        if (!grow_buf(&J->ordinal, J->rows, cap, 8)) return false;
+        while (cap < have + more_rows) cap *= 2;
+        if (!grow_buf(&J->keys, have * P->KW, cap * P->KW, 8)) return false;
+        if (!grow_buf(&J->ordinal, have, cap, 8)) return false;
         for (int p = 0; p < P->npayload; ++p) {
             const QdbValueType& t = P->spec.Build.Types[p];
             const size_t w = t.Kind == QDB_KIND_BOOL ? 1 : (size_t)t.Width;
-            if (!grow_buf(&J->pdata[(size_t)p], J->rows, cap, w)) return false;
-            if (t.Nullable && !grow_buf(&J->pvalid[(size_t)p], J->rows, cap, 1)) return false;
+            if (!grow_buf(&J->pdata[(size_t)p], have, cap, w)) return false;
+            if (t.Nullable && !grow_buf(&J->pvalid[(size_t)p], have, cap, 1)) return false;
         }
         J->cap = cap;
     }
@@ -1705,85 +1759,111 @@ bool do_append(JoinBuild* J, const TRowSet* batch)
         }
         if (!bind_side(slot_col, np, P->spec.Build, batch, &pcols)) return false;
         hipLaunchKernelGGL((k_append_write<KW>), dim3(nblocks), dim3(kBlock), 0, stream(), cols, P->build_key, n,
-            batch->Selection, counts, J->rows, J->appended, J->keys, J->ordinal, J->ord_to_arena, pcols, np, AC);
+            batch->Selection, counts, (const unsigned long long*)J->rows_dev, J->appended, J->keys, J->ordinal, J->ord_to_arena,
+            pcols, np, AC);
         ok = hip_ok(hipGetLastError(), "append write");
         if (P->npayload == 0) break;
     }
-    unsigned long long h_total = 0;
-    if (ok) ok = hip_ok(hipMemcpyAsync(&h_total, total, 8, hipMemcpyDeviceToHost, stream()), "append total") && sync("append");
+    // The kept rows are counted on the device; the host reads the count
+    // back at Build (with the dense table's key range).
+    if (ok) {
+        hipLaunchKernelGGL(k_add_rows, dim3(1), dim3(64), 0, stream(), J->rows_dev, (const unsigned long long*)total);
+        ok = hip_ok(hipGetLastError(), "append rows");
+    }
     dfree(counts);
     dfree(total);
     if (!ok) return false;
-    added = (int64_t)h_total;
-    J->rows += added;
+    (void)added;
+    J->rows_bound += n;
+    J->rows_known = false;
     J->appended += n;
     return true;
 }
 
 // The dense direct table when the keys are distinct and their range is
 // small; false (and no error) when the hash table must serve instead.
+// The arena's exact row count on the host (one readback when appends left
+// it on the device).
+bool read_rows(JoinBuild* J)
+{
+    if (J->rows_known) return true;
+    unsigned long long r = 0;
+    if (!hip_ok(hipMemcpyAsync(&r, J->rows_dev, 8, hipMemcpyDeviceToHost, stream()), "arena rows") || !sync("arena rows")) return false;
+    J->rows = (int64_t)r;
+    J->rows_known = true;
+    return true;
+}
+
+// The widest key range the dense table takes for `rows` build rows.
+__host__ __device__ inline uint64_t dense_limit(uint64_t rows)
+{
+    return rows * 8 > (1ull << 16) ? rows * 8 : (1ull << 16);
+}
+
 bool try_dense_build(JoinBuild* J)
 {
-    const int64_t rows = J->rows;
-    const int mgrid = grid_for(rows, kBlock, 2);
-    unsigned long long* mm = (unsigned long long*)dalloc(16);
+    // Sized for the bound on the rows: the key range, its validity, the
+    // repeated-key flag and the exact row count come back in one readback.
+    const int64_t bound = J->rows_known ? J->rows : J->rows_bound;
+    const uint64_t limit_cap = dense_limit((uint64_t)bound);
+    if (limit_cap >= (1ull << 32)) return false;
+    const int mgrid = grid_for(bound, kBlock, 2);
+    // info: [min, max] of the keys, the flags (repeated key, range too
+    // wide), the row count
+    unsigned long long* info = (unsigned long long*)dalloc(48);
     unsigned long long* mpart = (unsigned long long*)dalloc((size_t)mgrid * 16);
-    unsigned int* flags = (unsigned int*)dalloc(16);
-    if (!mm || !mpart || !flags) {
-        dfree(mm);
-        dfree(mpart);
-        dfree(flags);
-        return false;
-    }
-    unsigned long long h_mm[2] = {0, 0};
-    bool ok = zero_fill(flags, 16);
-    if (ok) {
-        TimedRegion timer("join_build");
-        hipLaunchKernelGGL(k_key_minmax, dim3(mgrid), dim3(kBlock), 0, stream(), J->keys, rows, mpart);
-        ReduceOps ops;
-        memset(&ops, 0, sizeof(ops));
-        ops.op[0] = 3;  // unsigned min
-        ops.op[1] = 4;  // unsigned max
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, stream(), mpart, mgrid, 2, 2, ops, mm);
-    }
-    dfree(mpart);
-    ok = ok && hip_ok(hipGetLastError(), "key range") && hip_ok(hipMemcpyAsync(h_mm, mm, 16, hipMemcpyDeviceToHost, stream()), "key range")
-        && sync("key range");
-    const uint64_t range = ok ? h_mm[1] - h_mm[0] + 1 : 0;
-    const uint64_t limit = (uint64_t)rows * 8 > (1ull << 16) ? (uint64_t)rows * 8 : (1ull << 16);
-    if (!ok || h_mm[1] < h_mm[0] || range == 0 || range > limit || range >= (1ull << 32)) {
-        dfree(mm);
-        dfree(flags);
-        clear_error();
-        return false;
-    }
-    const size_t words = (size_t)((range + 31) / 32);
-    J->dbits = (uint32_t*)dalloc(words * 4);
-    J->dtab = (uint32_t*)dalloc((size_t)range * 4);
-    unsigned int h_flags[4] = {0, 0, 0, 0};
-    ok = J->dbits && J->dtab && zero_fill(J->dbits, words * 4);
-    if (ok) {
-        TimedRegion timer("join_build");
-        hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((int64_t)range, kBlock, 8)), dim3(kBlock), 0, stream(), J->dtab, range, kAbsent);
-        hipLaunchKernelGGL(k_dense_fill, dim3(grid_for(rows, kBlock, 8)), dim3(kBlock), 0, stream(), J->keys, J->ordinal, rows,
-            (uint64_t)h_mm[0], J->dbits, J->dtab, flags);
-    }
-    ok = ok && hip_ok(hipGetLastError(), "dense fill") && hip_ok(hipMemcpyAsync(h_flags, flags, 16, hipMemcpyDeviceToHost, stream()), "dense flags")
-        && sync("dense fill");
-    dfree(mm);
-    dfree(flags);
-    if (!ok || h_flags[0]) {  // repeated keys: the hash table serves
+    const size_t words_cap = (size_t)((limit_cap + 31) / 32);
+    J->dbits = (uint32_t*)dalloc(words_cap * 4);
+    J->dtab = (uint32_t*)dalloc((size_t)limit_cap * 4);
+    auto drop = [&]() {
         dfree(J->dbits);
         dfree(J->dtab);
         J->dbits = nullptr;
         J->dtab = nullptr;
+    };
+    if (!info || !mpart || !J->dbits || !J->dtab) {
+        dfree(info);
+        dfree(mpart);
+        drop();
+        return false;
+    }
+    unsigned long long h_info[5] = {0, 0, 0, 0, 0};
+    unsigned int* flags = (unsigned int*)(info + 2);
+    bool ok = zero_fill(info, 48);
+    if (ok) {
+        TimedRegion timer("join_build");
+        hipLaunchKernelGGL(k_key_minmax, dim3(mgrid), dim3(kBlock), 0, stream(), J->keys, (const unsigned long long*)J->rows_dev, mpart);
+        ReduceOps ops;
+        memset(&ops, 0, sizeof(ops));
+        ops.op[0] = 3;  // unsigned min
+        ops.op[1] = 4;  // unsigned max
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, stream(), mpart, mgrid, 2, 2, ops, info);
+        hipLaunchKernelGGL(k_dense_prepare, dim3(grid_for((int64_t)limit_cap, kBlock, 8)), dim3(kBlock), 0, stream(),
+            (const unsigned long long*)J->rows_dev, info, J->dbits, J->dtab);
+        hipLaunchKernelGGL(k_dense_fill, dim3(grid_for(bound, kBlock, 8)), dim3(kBlock), 0, stream(), J->keys, J->ordinal,
+            (const unsigned long long*)J->rows_dev, info, J->dbits, J->dtab);
+    }
+    ok = ok && hip_ok(hipGetLastError(), "dense fill") &&
+        hip_ok(hipMemcpyAsync(h_info, info, 40, hipMemcpyDeviceToHost, stream()), "dense info") && sync("dense fill");
+    dfree(info);
+    dfree(mpart);
+    if (ok) {
+        J->rows = (int64_t)h_info[4];
+        J->rows_known = true;
+    }
+    const unsigned int* h_flags = (const unsigned int*)(h_info + 2);
+    const uint64_t range = h_info[1] - h_info[0] + 1;
+    // Repeated keys or a key range too wide: the hash table serves.
+    if (!ok || J->rows <= 0 || h_flags[0] || h_flags[1] || h_info[1] < h_info[0] || range == 0 ||
+        range > dense_limit((uint64_t)J->rows)) {
+        drop();
         clear_error();
         return false;
     }
     J->dense = true;
-    J->dmin = h_mm[0];
+    J->dmin = h_info[0];
     J->drange = range;
-    J->keys_distinct = rows;
+    J->keys_distinct = J->rows;
     J->built = true;
     return true;
 }
@@ -1792,10 +1872,12 @@ template <int KW>
 bool do_build(JoinBuild* J)
 {
     JoinPlan* P = J->plan;
+    if (J->appended > (int64_t)0xffffffff) return fail("join builds are limited to 2^32 appended rows");
+    const int64_t bound = J->rows_known ? J->rows : J->rows_bound;
+    if (KW == 1 && P->dense_ok && bound > 0 && bound < (int64_t)0x7fffffff && try_dense_build(J)) return true;
+    if (!read_rows(J)) return false;
     const int64_t rows = J->rows;
     if (rows >= (int64_t)0x7fffffff) return fail("join builds are limited to 2^31 rows");
-    if (J->appended > (int64_t)0xffffffff) return fail("join builds are limited to 2^32 appended rows");
-    if (KW == 1 && P->dense_ok && rows > 0 && try_dense_build(J)) return true;
     uint64_t capacity = 1024;
     int cap_bits = 10;
     while (capacity < (uint64_t)rows * 2) {
@@ -2058,6 +2140,7 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
     const int64_t n = batch->RowCount;
     pairs->Count = 0;
     J->last_probe = *batch;
+    J->fused_ready = false;
     if (n <= 0) return true;
     if (n >= (int64_t)1 << 32) return fail("probe batches are limited to 2^32 rows");
     ProbeArgs A;
@@ -2158,15 +2241,8 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
         A.pair_cap = (unsigned long long)pairs->Capacity;
         A.pair_count = J->pair_count;
         if (!zero_fill(J->pair_count, 8)) return false;
-        // Fused accumulators are per probe call; they are summed on the host.
-        if (A.nfused) {
-            std::vector<unsigned long long> init((size_t)A.nfused * kAccWords, 0ull);
-            for (int f = 0; f < A.nfused; ++f) {
-                init[(size_t)f * kAccWords + 4] = 0x7fffffffffffffffull;
-                init[(size_t)f * kAccWords + 5] = 0x8000000000000000ull;
-            }
-            if (!hip_ok(hipMemcpyAsync(J->fused_acc, init.data(), init.size() * 8, hipMemcpyHostToDevice, stream()), "fused init")) return false;
-        }
+        // Fused accumulators are per probe call (k_reduce_partials writes
+        // every word); they are summed on the host.
         const int64_t tiles = (n + (int64_t)kProbeBlock / kWave * kWaveTile - 1) / ((int64_t)kProbeBlock / kWave * kWaveTile);
         const int per_cu = lds_table ? 2 : 1;
         const int grid = (int)(tiles < (int64_t)num_cus() * per_cu ? tiles : (int64_t)num_cus() * per_cu);
@@ -2192,11 +2268,15 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
         }
         dfree(part);
         if (!last_error_text().empty()) return false;
-        unsigned long long count = 0;
         if (!hip_ok(hipGetLastError(), "probe launch")) return false;
-        if (!hip_ok(hipMemcpyAsync(&count, J->pair_count, 8, hipMemcpyDeviceToHost, stream()), "pair count") || !sync("probe")) return false;
+        // The pair count and the fused words in one readback.
+        const size_t rb = 16 + (size_t)A.nfused * kAccWords * 8;
+        if (!hip_ok(hipMemcpyAsync(J->readback, J->pair_count, rb, hipMemcpyDeviceToHost, stream()), "probe readback") || !sync("probe"))
+            return false;
+        const unsigned long long count = J->readback[0];
         if (count <= A.pair_cap) {
             pairs->Count = (int64_t)count;
+            J->fused_ready = A.nfused > 0;
             return true;
         }
         cap = count;  // cannot happen with the bound above; retry once
@@ -2220,9 +2300,9 @@ bool collect_fused(JoinBuild* J)
 {
     JoinPlan* P = J->plan;
     const int nf = P->spec.NumFused;
-    if (!nf) return true;
-    std::vector<unsigned long long> w((size_t)nf * kAccWords);
-    if (!hip_ok(hipMemcpyAsync(w.data(), J->fused_acc, w.size() * 8, hipMemcpyDeviceToHost, stream()), "fused read") || !sync("fused")) return false;
+    if (!nf || !J->fused_ready) return true;
+    J->fused_ready = false;
+    const unsigned long long* w = J->readback + 2;
     std::lock_guard<std::mutex> lock(g_fused_mu);
     FusedTotals& T = fused_totals()[J];
     if (T.r.empty()) {
@@ -2283,9 +2363,10 @@ bool jr_dispatch_impl(JoinPlan* P, void* arena, void* table, void* scratch, void
         J->plan = P;
         J->pdata.assign((size_t)P->npayload, nullptr);
         J->pvalid.assign((size_t)P->npayload, nullptr);
-        J->fused_acc = (unsigned long long*)dalloc((size_t)kMaxFused * kAccWords * 8);
-        J->pair_count = (unsigned long long*)dalloc(16);
-        if (!J->fused_acc || !J->pair_count) {
+        J->pair_count = (unsigned long long*)dalloc(16 + (size_t)kMaxFused * kAccWords * 8);
+        J->fused_acc = J->pair_count ? J->pair_count + 2 : nullptr;
+        J->rows_dev = (unsigned long long*)dalloc(16);
+        if (!J->pair_count || !J->rows_dev || !zero_fill(J->rows_dev, 8)) {
             build_free(J);
             return false;
         }
@@ -2308,6 +2389,7 @@ bool jr_dispatch_impl(JoinPlan* P, void* arena, void* table, void* scratch, void
         if (!J) return fail("join build before Init");
         bool ok = P->KW == 1 ? do_build<1>(J) : do_build<2>(J);
         if (!ok) return false;
+        AB->Rows = J->rows;
         TB->entries = J->table;
         TB->Capacity = (int64_t)J->capacity;
         TB->Mask = J->capacity - 1;

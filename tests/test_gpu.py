@@ -289,6 +289,35 @@ def test_join_dense_lean_equals_general(gpu_backend, monkeypatch, probe_is_left,
         assert got["1"][1][2][0] == int(pval[prow].min()) and got["1"][1][3][0] == int(pval[prow].max())
 
 
+@pytest.mark.parametrize("table_mode", [0, 1])
+def test_join_empty_probe_adds_nothing_to_fused(gpu_backend, table_mode):
+    """An empty probe batch, before and after a real one, leaves the fused
+    COUNT/SUM/MIN equal to the real batch's alone (an empty batch runs no
+    kernel and has no fused words to add)."""
+    rng = np.random.default_rng(23)
+    types_b = [T(SIGNED, 8)]
+    types_p = [T(SIGNED, 8), T(SIGNED, 8)]
+    bk = rng.permutation(5000).astype(np.int64)
+    pk = rng.integers(0, 6000, 40000).astype(np.int64)
+    pval = rng.integers(-10 ** 9, 10 ** 9, 40000).astype(np.int64)
+    case = JoinCase(types_b, types_p, [0], [0], table_mode=table_mode,
+                    fused=[(AGG_COUNT_STAR, -1), (AGG_SUM, 1), (AGG_MIN, 1)])
+    kern, proc = gpu_backend.join_processor(case)
+    proc.append(gpu_backend.dev(Batch(types_b, [bk])))
+    proc.build()
+    empty = Batch(types_p, [np.zeros(0, np.int64), np.zeros(0, np.int64)])
+    assert len(proc.probe_pairs(gpu_backend.dev(empty))) == 0
+    pairs = proc.probe_pairs(gpu_backend.dev(Batch(types_p, [pk, pval])))
+    assert len(proc.probe_pairs(gpu_backend.dev(empty))) == 0
+    fused = proc.fused_results()
+    proc.close()
+    hit = pk < 5000
+    assert len(pairs) == int(hit.sum())
+    assert fused[0]["value"] == int(hit.sum())
+    assert fused[1]["value"] == int(sum(int(v) for v in pval[hit]))
+    assert fused[2]["value"] == int(pval[hit].min())
+
+
 def test_join_multi_batch_build_and_fused(gpu_backend):
     """Build from three batches (ordinals continue across them), duplicate
     keys (chains in build order), fused COUNT(*) / SUM / MIN / MAX."""
