@@ -396,9 +396,15 @@ void qdb_gpu_aggregate_destroy(void* ht);
 void* qdb_gpu_alloc(size_t bytes);
 void qdb_gpu_free(void* ptr);
 /* Returns the device blocks the library's allocator keeps for reuse to the
- * driver (it keeps at most 1/8 of the device memory otherwise). No
+ * driver (otherwise it keeps freed blocks while 1/8 of the device memory
+ * stays free). No
  * counterpart in the reference, whose kernels allocate from host heaps. */
 void qdb_gpu_trim_cache(void);
+
+/* Device pool counters, for diagnostics (no reference counterpart):
+ * [driver allocations, driver frees, free-memory queries, cached bytes,
+ * bytes in blocks handed out]; returns how many values were written. */
+int64_t qdb_gpu_pool_stats(int64_t* out, int64_t n);
 int qdb_gpu_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int qdb_gpu_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int qdb_gpu_memset(void* dst, int value, size_t bytes);

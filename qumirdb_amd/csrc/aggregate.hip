@@ -1224,18 +1224,28 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n)
     AggPlan* P = T->plan;
     const StageLayout SL = stage_layout(P);
     const int W = SL.words, KW = P->KW, R = KW + bound.nwords;
-    // LDS table of the bucket workgroups: the most groups (a power of two)
-    // that fit next to the kernel's own LDS, and buckets sized to fill
-    // about 60% of it.
+    // LDS table of the bucket workgroups: the most groups that fit next to
+    // the kernel's own LDS with the hash slots (a power of two) at most 45%
+    // full, and buckets sized to fill about 60% of it (fewer buckets: longer
+    // runs in the partition's writes).
     AggArgs A = bound;
     table_args(T, &A);
     A.row_base = 0;
     A.nrows = n;
     A.row_index = nullptr;
-    int g = 4096;
-    while (g > 64 && (size_t)g * R * 8 + (size_t)g * 2 * 4 + 16 > max_lds_per_block() - 4096) g /= 2;
+    const size_t budget = max_lds_per_block() - 4096;
+    int g = 0, slots = 0;
+    for (int s = 16384; s >= 128; s /= 2) {
+        if ((size_t)s * 4 + 16 >= budget) continue;
+        const size_t fit = (budget - (size_t)s * 4 - 16) / ((size_t)R * 8);
+        const int cap = (int)((size_t)s * 45 / 100 < fit ? (size_t)s * 45 / 100 : fit);
+        if (cap > g) {
+            g = cap;
+            slots = s;
+        }
+    }
     A.lds_groups = g;
-    A.lds_slots = 2 * g;
+    A.lds_slots = slots;
     const size_t lds = (size_t)g * R * 8 + (size_t)A.lds_slots * 4 + 16;
     const uint64_t expect = (uint64_t)n < T->capacity / 2 ? (uint64_t)n : T->capacity / 2;
     int bits = 8;

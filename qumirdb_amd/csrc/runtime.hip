@@ -37,6 +37,8 @@ struct Runtime {
     std::multimap<size_t, FreeBlock> free_dev;
     std::unordered_map<void*, size_t> dev_sizes;
     uint64_t frees = 0;
+    // counters: driver allocations and frees, free-memory queries
+    int64_t n_malloc = 0, n_free = 0, n_meminfo = 0;
     size_t cached = 0;          // bytes in free_dev
     size_t reserve = 0;         // device memory the pool leaves free for the HIP runtime and other allocators
     std::map<size_t, std::vector<void*>> free_host;
@@ -180,6 +182,7 @@ static void evict_oldest_locked(Runtime& r)
     r.cached -= oldest->first;
     r.dev_sizes.erase(oldest->second.p);
     (void)hipFree(oldest->second.p);
+    r.n_free += 1;
     r.free_dev.erase(oldest);
 }
 
@@ -189,6 +192,7 @@ static void make_room_locked(Runtime& r, size_t need)
 {
     while (!r.free_dev.empty()) {
         size_t free_b = 0, total_b = 0;
+        r.n_meminfo += 1;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
             (void)hipGetLastError();
             return;
@@ -203,6 +207,7 @@ static void release_cached_locked(Runtime& r)
     for (auto& kv : r.free_dev) {
         r.dev_sizes.erase(kv.second.p);
         (void)hipFree(kv.second.p);
+        r.n_free += 1;
     }
     r.free_dev.clear();
     r.cached = 0;
@@ -245,6 +250,7 @@ void* dalloc(size_t bytes)
     }
     std::lock_guard<std::mutex> lock(r.mu);
     r.dev_sizes[p] = c;
+    r.n_malloc += 1;
     return p;
 }
 
@@ -406,6 +412,21 @@ extern "C" const char* qdb_gpu_version(void)
 extern "C" void* qdb_gpu_alloc(size_t bytes)
 {
     return dalloc(bytes);
+}
+
+// Device pool counters: [driver allocations, driver frees, free-memory
+// queries, cached bytes, bytes in blocks handed out]; returns how many
+// values were written.
+extern "C" int64_t qdb_gpu_pool_stats(int64_t* out, int64_t n)
+{
+    qdbgpu::Runtime& r = qdbgpu::R();
+    std::lock_guard<std::mutex> lock(r.mu);
+    size_t all = 0;
+    for (const auto& kv : r.dev_sizes) all += kv.second;
+    const int64_t v[5] = {r.n_malloc, r.n_free, r.n_meminfo, (int64_t)r.cached, (int64_t)(all - r.cached)};
+    int64_t k = 0;
+    for (; k < n && k < 5; ++k) out[k] = v[k];
+    return k;
 }
 
 extern "C" void qdb_gpu_trim_cache(void)
