@@ -152,14 +152,42 @@ def test_batched_update_equals_plain(gpu_backend):
     q.lib().qdb_gpu_release_slot(q.abi.SLOT_AGGREGATE, kern.k.Slot)
 
 
+def _bucketed_case(case):
+    """The aggregate without i128 MIN/MAX (those words are applied under a
+    lock and keep an update on the plain path)."""
+    keep = [(a, c) for a, c in case.aggs if not (a in (AGG_MIN, AGG_MAX) and c == 3)]
+    return AggCase(case.types, case.keys, keep)
+
+
 def test_bucketed_aggregate_random_vs_oracle(gpu_backend, monkeypatch):
     """The bucketed path (rows staged and partitioned by key-hash bucket,
     each bucket aggregated in LDS, merged into the HBM table), taken here
     from 2^20 rows (QDB_GPU_BUCKETED_MIN_ROWS) into a table of at least 2^20
-    slots: nullable two-column keys, an i128 argument, a selection, every
-    reducer; then a second batch into the same table through the same path."""
+    slots: nullable two-column keys, an i128 SUM, a selection, COUNT(*),
+    COUNT, SUM/MIN/MAX of i64, f64 and decimals; then a second batch into
+    the same table through the same path."""
     monkeypatch.setenv("QDB_GPU_BUCKETED_MIN_ROWS", str(1 << 20))
     case, b = random_agg_case(21, 1_200_000)
+    case = _bucketed_case(case)
+    got = sorted(result_rows(gpu_backend.aggregate(case, [b, b], capacity=1 << 21)), key=sort_key)
+    want = sorted(result_rows(OracleBackend(fsum_mode=2).aggregate(case, [b, b])), key=sort_key)
+    same(got, want)
+
+
+def test_bucketed_aggregate_skewed_keys_vs_oracle(gpu_backend, monkeypatch):
+    """The bucketed path with hot keys (40% and 20% of the rows on two keys,
+    the rest over 200K keys): whole waves of a bucket's rows share a key
+    and are combined across the wave before they reach the LDS record."""
+    monkeypatch.setenv("QDB_GPU_BUCKETED_MIN_ROWS", str(1 << 20))
+    rng = np.random.default_rng(77)
+    case, b = random_agg_case(22, 1_100_000)
+    case = _bucketed_case(case)
+    n = len(b.columns[1])
+    u = rng.random(n)
+    k2 = np.where(u < 0.4, 7, np.where(u < 0.6, -3, rng.integers(0, 200_000, n))).astype(np.int64)
+    k1 = np.where(u < 0.6, 1, b.columns[0]).astype(np.int32)
+    k1v = np.where(u < 0.6, True, b.valid[0])
+    b = Batch(b.types, [k1, k2] + list(b.columns[2:]), [k1v] + list(b.valid[1:]), b.selection)
     got = sorted(result_rows(gpu_backend.aggregate(case, [b, b], capacity=1 << 21)), key=sort_key)
     want = sorted(result_rows(OracleBackend(fsum_mode=2).aggregate(case, [b, b])), key=sort_key)
     same(got, want)
