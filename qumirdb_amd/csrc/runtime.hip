@@ -526,6 +526,83 @@ extern "C" double qdb_gpu_timing_avg_ms(const char* family, int64_t* launches)
 
 namespace {
 
+// Up to kSegs device byte ranges packed into one device buffer by one
+// launch (blockIdx.y = segment).
+constexpr int kSegs = 32;
+struct CopySegs {
+    const char* src[kSegs];
+    uint64_t off[kSegs];
+    uint64_t bytes[kSegs];
+};
+
+__global__ void k_gather_segments(CopySegs S, char* dst)
+{
+    const int g = blockIdx.y;
+    const uint64_t n = S.bytes[g];
+    const char* src = S.src[g];
+    char* d = dst + S.off[g];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) d[i] = src[i];
+}
+
+// Copies device byte ranges to new host (malloc) buffers, one per range
+// (nullptr for an empty range); false on a failed copy. Small totals are
+// packed on the device and come back in one copy through pinned memory
+// (one round trip instead of one per range); larger ones are copied range
+// by range.
+bool download_ranges(const std::vector<const void*>& src, const std::vector<size_t>& bytes, std::vector<void*>* host)
+{
+    const size_t k = src.size();
+    host->assign(k, nullptr);
+    std::vector<uint64_t> off(k, 0);
+    size_t total = 0;
+    for (size_t i = 0; i < k; ++i) {
+        if (!src[i] || !bytes[i]) continue;
+        off[i] = total;
+        total += (bytes[i] + 15) / 16 * 16;
+        (*host)[i] = malloc(bytes[i]);
+        if (!(*host)[i]) return fail("download: out of host memory");
+    }
+    if (total == 0) return true;
+    if (total > ((size_t)16 << 20)) {
+        for (size_t i = 0; i < k; ++i) {
+            if (!(*host)[i]) continue;
+            if (!hip_ok(hipMemcpyAsync((*host)[i], src[i], bytes[i], hipMemcpyDeviceToHost, stream()), "download")) return false;
+        }
+        return sync("download");
+    }
+    char* packed = (char*)dalloc(total);
+    char* pin = (char*)halloc_pinned(total);
+    bool ok = packed && pin;
+    for (size_t i0 = 0; ok && i0 < k; i0 += kSegs) {
+        CopySegs S;
+        memset(&S, 0, sizeof(S));
+        int n = 0;
+        uint64_t most = 0;
+        for (size_t i = i0; i < k && i < i0 + kSegs; ++i) {
+            if (!(*host)[i]) continue;
+            S.src[n] = (const char*)src[i];
+            S.off[n] = off[i];
+            S.bytes[n] = bytes[i];
+            most = bytes[i] > most ? bytes[i] : most;
+            ++n;
+        }
+        if (n == 0) continue;
+        const int gx = (int)((most + 255) / 256 < 64 ? (most + 255) / 256 : 64);
+        hipLaunchKernelGGL(k_gather_segments, dim3(gx > 0 ? gx : 1, n), dim3(256), 0, stream(), S, packed);
+        ok = hip_ok(hipGetLastError(), "download gather");
+    }
+    ok = ok && hip_ok(hipMemcpyAsync(pin, packed, total, hipMemcpyDeviceToHost, stream()), "download") && sync("download");
+    if (ok) {
+        for (size_t i = 0; i < k; ++i) {
+            if ((*host)[i]) memcpy((*host)[i], pin + off[i], bytes[i]);
+        }
+    }
+    dfree(packed);
+    hfree_pinned(pin);
+    return ok;
+}
+
 void destroy_owned(TRowSet* rs)
 {
     OwnedRowSet* o = (OwnedRowSet*)rs->Private;
@@ -560,24 +637,41 @@ bool move_rowset_to_host(TRowSet* rs, const QdbValueType* types)
 {
     OwnedRowSet* o = (OwnedRowSet*)rs->Private;
     const int64_t n = rs->RowCount;
+    // Every data and mask buffer, then one download for all of them.
+    std::vector<const void*> src;
+    std::vector<size_t> bytes;
+    for (int64_t c = 0; c < rs->ColumnCount; ++c) {
+        const TColumn& col = o->cols[(size_t)c];
+        const QdbValueType& t = types[c];
+        src.push_back(col.Data);
+        bytes.push_back(t.Kind == QDB_KIND_BOOL ? (size_t)((n + 7) / 8) : (size_t)n * (size_t)t.Width);
+        src.push_back(col.Mask);
+        bytes.push_back(col.Mask ? (size_t)((n + 7) / 8) : 0);
+    }
     std::vector<void*> host;
-    auto fetch = [&](const void* src, size_t bytes) -> void* {
-        void* h = malloc(bytes ? bytes : 1);
-        host.push_back(h);
-        if (src && bytes) (void)hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, stream());
-        return h;
-    };
+    const bool ok = download_ranges(src, bytes, &host);
+    for (size_t i = 0; i < host.size(); ++i) {
+        if (!host[i]) host[i] = malloc(1);  // a non-null buffer for every column and mask present
+    }
+    if (!ok) {
+        for (void* p : host) free(p);
+        return false;
+    }
     for (int64_t c = 0; c < rs->ColumnCount; ++c) {
         TColumn& col = o->cols[(size_t)c];
-        const QdbValueType& t = types[c];
-        size_t bytes = t.Kind == QDB_KIND_BOOL ? (size_t)((n + 7) / 8) : (size_t)n * (size_t)t.Width;
-        col.Data = (char*)fetch(col.Data, bytes);
-        if (col.Mask) col.Mask = (const uint8_t*)fetch(col.Mask, (size_t)((n + 7) / 8));
+        col.Data = (char*)host[2 * (size_t)c];
+        if (col.Mask) {
+            col.Mask = (const uint8_t*)host[2 * (size_t)c + 1];
+        } else {
+            free(host[2 * (size_t)c + 1]);
+            host[2 * (size_t)c + 1] = nullptr;
+        }
     }
-    if (!sync("download output")) return false;
     for (void* p : o->dev) dfree(p);
     o->dev.clear();
-    o->host.insert(o->host.end(), host.begin(), host.end());
+    for (void* p : host) {
+        if (p) o->host.push_back(p);
+    }
     return true;
 }
 
@@ -641,27 +735,38 @@ extern "C" int qdb_gpu_download_rowset(const TRowSet* dev, const QdbValueType* t
     const int64_t n = dev->RowCount;
     OwnedRowSet* o = new OwnedRowSet();
     o->cols.resize((size_t)dev->ColumnCount);
-    auto fetch = [&](const void* src, size_t bytes) -> void* {
-        if (!src || bytes == 0) return nullptr;
-        void* h = malloc(bytes);
-        o->host.push_back(h);
-        if (!hip_ok(hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, stream()), "download")) return nullptr;
-        return h;
-    };
+    // Every data and mask buffer, then one download for all of them.
+    std::vector<const void*> src;
+    std::vector<size_t> bytes;
     for (int64_t c = 0; c < dev->ColumnCount; ++c) {
         const TColumn& dc = dev->Columns[c];
         const QdbValueType& t = types[c];
+        src.push_back(dc.Data);
+        bytes.push_back(t.Kind == QDB_KIND_BOOL ? (size_t)((n + dc.DataBitOffset + 7) / 8) : (size_t)n * (size_t)t.Width);
+        src.push_back(dc.Mask);
+        bytes.push_back(dc.Mask ? (size_t)((n + dc.MaskBitOffset + 7) / 8) : 0);
+    }
+    std::vector<void*> host;
+    const bool fetched = download_ranges(src, bytes, &host);
+    for (void* p : host) {
+        if (p) o->host.push_back(p);
+    }
+    if (!fetched) {
+        for (void* p : o->host) free(p);
+        delete o;
+        return -1;
+    }
+    for (int64_t c = 0; c < dev->ColumnCount; ++c) {
+        const TColumn& dc = dev->Columns[c];
         TColumn& hc = o->cols[(size_t)c];
         memset(&hc, 0, sizeof(hc));
-        size_t bytes = t.Kind == QDB_KIND_BOOL ? (size_t)((n + dc.DataBitOffset + 7) / 8) : (size_t)n * (size_t)t.Width;
-        hc.Data = (char*)fetch(dc.Data, bytes);
+        hc.Data = (char*)host[2 * (size_t)c];
         hc.DataBitOffset = dc.DataBitOffset;
         if (dc.Mask) {
-            hc.Mask = (const uint8_t*)fetch(dc.Mask, (size_t)((n + dc.MaskBitOffset + 7) / 8));
+            hc.Mask = (const uint8_t*)host[2 * (size_t)c + 1];
             hc.MaskBitOffset = dc.MaskBitOffset;
         }
     }
-    if (!sync("download")) return -1;
     memset(out, 0, sizeof(*out));
     out->Columns = o->cols.data();
     out->ColumnCount = dev->ColumnCount;
