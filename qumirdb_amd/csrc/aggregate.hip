@@ -1082,10 +1082,22 @@ bool read_counters(AggTable* T, unsigned int out[4])
     return true;
 }
 
+__global__ void k_set_counters(unsigned int* c, unsigned int groups, unsigned int extra_used)
+{
+    if (threadIdx.x == 0) {
+        c[0] = groups;
+        c[1] = 0u;
+        c[2] = 0u;
+        c[3] = extra_used;
+    }
+}
+
+// The table's counters before a launch (a kernel, not an upload from
+// pageable memory, which would wait for the copy).
 bool set_counters(AggTable* T)
 {
-    unsigned int h[4] = {T->groups, 0u, 0u, T->extra_used};
-    return hip_ok(hipMemcpyAsync(T->counters, h, 16, hipMemcpyHostToDevice, stream()), "counters");
+    hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, stream(), T->counters, T->groups, T->extra_used);
+    return hip_ok(hipGetLastError(), "counters");
 }
 
 // LDS table of an update launch: as many groups as fit in 64 KiB, a power
