@@ -179,19 +179,22 @@ __global__ void __launch_bounds__(kBlock) k_fill_records(uint64_t* recs, uint64_
 
 constexpr int kOccChunk = 4096;   // slots per workgroup in the occupancy passes
 
+// One-word keys: the table's occupancy bitmap (4 bytes per 32 slots
+// rather than one record line per slot); wider keys: their control words.
 template <int KW>
-__device__ __forceinline__ bool occupied(const uint64_t* recs, const unsigned long long* ctrl, int stride, uint64_t slots,
-    int extra_used, uint64_t s)
+__device__ __forceinline__ bool occupied(const uint64_t* recs, const unsigned long long* ctrl, const uint32_t* occ, int stride,
+    uint64_t slots, int extra_used, uint64_t s)
 {
     if (s >= slots) return s == slots && extra_used;
+    if (KW == 1) return ((occ[s >> 5] >> (s & 31u)) & 1u) != 0u;
     return slot_used<KW>(recs, ctrl, stride, s);
 }
 
 // Groups per chunk of kOccChunk slots (slots [0, slots] with the extra
 // record last).
 template <int KW>
-__global__ void __launch_bounds__(kBlock) k_occ_counts(const uint64_t* recs, const unsigned long long* ctrl, int stride,
-    uint64_t slots, int extra_used, uint32_t* counts)
+__global__ void __launch_bounds__(kBlock) k_occ_counts(const uint64_t* recs, const unsigned long long* ctrl, const uint32_t* occ,
+    int stride, uint64_t slots, int extra_used, uint32_t* counts)
 {
     __shared__ unsigned int total;
     if (threadIdx.x == 0) total = 0u;
@@ -199,7 +202,7 @@ __global__ void __launch_bounds__(kBlock) k_occ_counts(const uint64_t* recs, con
     const uint64_t base = (uint64_t)blockIdx.x * kOccChunk;
     unsigned int n = 0;
     for (int k = threadIdx.x; k < kOccChunk; k += blockDim.x) {
-        n += occupied<KW>(recs, ctrl, stride, slots, extra_used, base + (uint64_t)k) ? 1u : 0u;
+        n += occupied<KW>(recs, ctrl, occ, stride, slots, extra_used, base + (uint64_t)k) ? 1u : 0u;
     }
     atomicAdd(&total, n);
     __syncthreads();
@@ -208,8 +211,8 @@ __global__ void __launch_bounds__(kBlock) k_occ_counts(const uint64_t* recs, con
 
 // The slot of every group, in slot order: chunk offset + rank in the chunk.
 template <int KW>
-__global__ void __launch_bounds__(kBlock) k_occ_index(const uint64_t* recs, const unsigned long long* ctrl, int stride,
-    uint64_t slots, int extra_used, const uint32_t* offsets, uint32_t* index)
+__global__ void __launch_bounds__(kBlock) k_occ_index(const uint64_t* recs, const unsigned long long* ctrl, const uint32_t* occ,
+    int stride, uint64_t slots, int extra_used, const uint32_t* offsets, uint32_t* index)
 {
     __shared__ unsigned int wave_total[kBlock / kWave];
     const uint64_t base = (uint64_t)blockIdx.x * kOccChunk;
@@ -217,7 +220,7 @@ __global__ void __launch_bounds__(kBlock) k_occ_index(const uint64_t* recs, cons
     unsigned int out = offsets[blockIdx.x];
     for (int k0 = 0; k0 < kOccChunk; k0 += kBlock) {
         const uint64_t s = base + (uint64_t)(k0 + threadIdx.x);
-        const bool used = occupied<KW>(recs, ctrl, stride, slots, extra_used, s);
+        const bool used = occupied<KW>(recs, ctrl, occ, stride, slots, extra_used, s);
         const uint64_t m = __ballot(used);
         if (lane == 0) wave_total[wave] = (unsigned int)__popcll(m);
         __syncthreads();
@@ -281,6 +284,7 @@ struct AggPlan {
 struct AggTable {
     AggPlan* plan = nullptr;
     unsigned long long* ctrl = nullptr; // wider keys only
+    uint32_t* occ = nullptr;            // one-word keys: a bit per slot that holds a group
     uint64_t capacity = 0;              // slots (a power of two); the extra record follows
     uint64_t* recs = nullptr;
     int rstride = 1;                    // words per record
@@ -679,9 +683,14 @@ bool table_alloc(AggTable* T, uint64_t capacity)
     T->capacity = capacity;
     T->rstride = record_stride(P->KW + P->proto.nwords);
     T->ctrl = nullptr;
+    T->occ = nullptr;
     if (P->KW > 1) {
         T->ctrl = (unsigned long long*)dalloc(capacity * 8);
         if (!T->ctrl || !zero_fill(T->ctrl, capacity * 8)) return false;
+    } else {
+        const size_t ob = (size_t)((capacity + 31) / 32) * 4;
+        T->occ = (uint32_t*)dalloc(ob);
+        if (!T->occ || !zero_fill(T->occ, ob)) return false;
     }
     T->recs = (uint64_t*)dalloc((size_t)(capacity + 1) * T->rstride * 8);
     if (!T->recs) return false;
@@ -1021,6 +1030,7 @@ void table_free(AggTable* T)
 {
     if (!T) return;
     dfree(T->ctrl);
+    dfree(T->occ);
     dfree(T->recs);
     dfree(T->counters);
     delete T;
@@ -1034,18 +1044,22 @@ bool table_grow(AggTable* T, uint64_t need)
     uint64_t cap = T->capacity * 2;
     while (cap / 2 < need) cap *= 2;
     unsigned long long* old_ctrl = T->ctrl;
+    uint32_t* old_occ = T->occ;
     uint64_t* old_recs = T->recs;
     const uint64_t old_cap = T->capacity;
     if (!table_alloc(T, cap)) {
         dfree(T->ctrl);
+        dfree(T->occ);
         dfree(T->recs);
         T->ctrl = old_ctrl;
+        T->occ = old_occ;
         T->recs = old_recs;
         T->capacity = old_cap;
         return false;
     }
     AggArgs A = P->proto;
     A.ctrl = T->ctrl;
+    A.occ = T->occ;
     A.cmask = T->capacity - 1;
     A.recs = T->recs;
     A.rstride = T->rstride;
@@ -1059,6 +1073,7 @@ bool table_grow(AggTable* T, uint64_t need)
     }
     if (!sync("grow")) return false;
     dfree(old_ctrl);
+    dfree(old_occ);
     dfree(old_recs);
     return true;
 }
@@ -1066,6 +1081,7 @@ bool table_grow(AggTable* T, uint64_t need)
 void table_args(const AggTable* T, AggArgs* A)
 {
     A->ctrl = T->ctrl;
+    A->occ = T->occ;
     A->cmask = T->capacity - 1;
     A->recs = T->recs;
     A->rstride = T->rstride;
@@ -1599,20 +1615,20 @@ int64_t table_finish(AggTable* T, TRowSet* out)
         const int extra = (int)T->extra_used;
         TimedRegion timer("agg_finalize");
         switch (P->KW) {
-        case 1: hipLaunchKernelGGL(k_occ_counts<1>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, counts); break;
-        case 2: hipLaunchKernelGGL(k_occ_counts<2>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, counts); break;
-        case 3: hipLaunchKernelGGL(k_occ_counts<3>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, counts); break;
-        default: hipLaunchKernelGGL(k_occ_counts<4>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, counts); break;
+        case 1: hipLaunchKernelGGL(k_occ_counts<1>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->occ, T->rstride, T->capacity, extra, counts); break;
+        case 2: hipLaunchKernelGGL(k_occ_counts<2>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->occ, T->rstride, T->capacity, extra, counts); break;
+        case 3: hipLaunchKernelGGL(k_occ_counts<3>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->occ, T->rstride, T->capacity, extra, counts); break;
+        default: hipLaunchKernelGGL(k_occ_counts<4>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->occ, T->rstride, T->capacity, extra, counts); break;
         }
         if (!hip_ok(hipGetLastError(), "occupancy launch") || !exclusive_scan_u32(counts, offsets, chunks)) {
             release();
             return -1;
         }
         switch (P->KW) {
-        case 1: hipLaunchKernelGGL(k_occ_index<1>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, offsets, index); break;
-        case 2: hipLaunchKernelGGL(k_occ_index<2>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, offsets, index); break;
-        case 3: hipLaunchKernelGGL(k_occ_index<3>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, offsets, index); break;
-        default: hipLaunchKernelGGL(k_occ_index<4>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->rstride, T->capacity, extra, offsets, index); break;
+        case 1: hipLaunchKernelGGL(k_occ_index<1>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->occ, T->rstride, T->capacity, extra, offsets, index); break;
+        case 2: hipLaunchKernelGGL(k_occ_index<2>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->occ, T->rstride, T->capacity, extra, offsets, index); break;
+        case 3: hipLaunchKernelGGL(k_occ_index<3>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->occ, T->rstride, T->capacity, extra, offsets, index); break;
+        default: hipLaunchKernelGGL(k_occ_index<4>, dim3(chunks), dim3(kBlock), 0, stream(), T->recs, T->ctrl, T->occ, T->rstride, T->capacity, extra, offsets, index); break;
         }
         if (!hip_ok(hipGetLastError(), "index launch")) {
             release();

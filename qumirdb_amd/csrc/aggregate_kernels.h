@@ -99,6 +99,7 @@ struct AggArgs {
     // (kEmptyKey = empty slot; the key equal to kEmptyKey has the extra
     // record at slot cmask + 1); wider keys through a control word per slot.
     unsigned long long* ctrl;  // KW > 1: 0 empty, 1 being filled, tag | 2 ready
+    unsigned int* occ;         // KW == 1: one bit per slot that holds a group (the finalize scans it)
     uint64_t cmask;
     uint64_t* recs;
     int rstride;
@@ -407,6 +408,7 @@ __device__ __forceinline__ int64_t global_find_or_insert(const AggArgs& A, LdsTa
                 c = atomicCAS(kp, kEmptyKey, (unsigned long long)key[0]);
                 if (c == kEmptyKey) {
                     atomicAdd(T.inserted, 1u);
+                    atomicOr(&A.occ[slot >> 5], 1u << (slot & 31u));
                     T.fresh = 1;
                     return (int64_t)slot;
                 }
@@ -1162,6 +1164,7 @@ __global__ void __launch_bounds__(kBlock) k_agg_rehash(AggArgs A, const uint64_t
             while (atomicCAS((unsigned long long*)(A.recs + slot * (uint64_t)A.rstride), kEmptyKey, (unsigned long long)key[0]) != kEmptyKey) {
                 slot = (slot + 1) & A.cmask;
             }
+            atomicOr(&A.occ[slot >> 5], 1u << (slot & 31u));
         } else {
             const unsigned long long v = ((unsigned long long)(h >> 32) << 32) | 2ull;
             while (atomicCAS(&A.ctrl[slot], 0ull, v) != 0ull) slot = (slot + 1) & A.cmask;
