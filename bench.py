@@ -190,7 +190,8 @@ def bench_batched(n_instances=512):
     SUM(i64), MIN/MAX/SUM(f64) by an int64 key; rows log-uniform 10K..20M,
     cardinality log-uniform 4..1M, uniform or Zipf(1.1) keys), whole queries
     (table set-up, update, finish into HBM) one after another versus the
-    same instances with all updates in one persistent batched launch."""
+    same instances with all updates in one persistent batched launch and
+    all finishes behind one synchronization."""
     from qumirdb_amd import lib, workloads as W
     from qumirdb_amd.host import AggregateKernels, TAggregateProcessor
 
@@ -218,7 +219,7 @@ def bench_batched(n_instances=512):
     def batched():
         procs = [TAggregateProcessor(kern, initial_capacity=cap) for cap in caps]
         TAggregateProcessor.add_batched(procs, [d.rowset for d in dev.devs])
-        groups = [p.finish_in_hbm() for p in procs]
+        groups = TAggregateProcessor.finish_in_hbm_batched(procs)
         for p in procs:
             p.close()
         return groups

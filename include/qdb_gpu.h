@@ -464,6 +464,13 @@ int64_t qdb_gpu_aggregate_kernel_source(const QdbAggregateSpec* spec, char* out,
  * agg_dispatch(Update) calls. Returns 0, or -1 with the error text. */
 int qdb_gpu_agg_update_batched(const QdbAggregateKernels* kernels, void* const* hts, const TRowSet* const* batches,
                                int64_t n);
+/* Batched finish (no counterpart in the reference): agg_finish_rowset of
+ * hts[i] into outs[i] for every i < n, every table's finalize launched
+ * before ONE synchronization; rows[i] gets outs[i]'s row count and each
+ * table is destroyed, as by agg_finish_rowset. Returns 0, or -1 with the
+ * error text (outputs already made are released). */
+int qdb_gpu_agg_finish_batched(const QdbAggregateKernels* kernels, void* const* hts, TRowSet* outs, int64_t* rows,
+                               int64_t n);
 int qdb_gpu_jit_check(const char* src, const char* arch, char* err, int64_t cap);
 int64_t qdb_gpu_jit_stats(double* total_ms);
 
@@ -479,6 +486,9 @@ const char* qdb_gpu_exec_last_error(void);
 /* processors[i] (all of one compiled aggregate) take batches[i], in one
  * batched launch (qdb_gpu_agg_update_batched). 0 or -1. */
 int qdb_gpu_agg_processor_add_batched(void* const* procs, const TRowSet* const* batches, int64_t n);
+/* processors[i] (all of one compiled aggregate) finish into outs[i], with
+ * one synchronization (qdb_gpu_agg_finish_batched). 0 or -1. */
+int qdb_gpu_agg_processor_finish_batched(void* const* procs, TRowSet* outs, int64_t n);
 /* row_layout != 0 starts with InitRows(initial_groups), else Init. */
 void* qdb_gpu_agg_processor_new(const QdbAggregateKernels* kernels, int64_t initial_capacity, int row_layout,
     int64_t initial_groups);

@@ -194,7 +194,8 @@ EXPORTS = [
     "qdb_gpu_gen_lineitem", "qdb_gpu_gen_join_build", "qdb_gpu_gen_join_probe", "qdb_gpu_gen_zipf_groups",
     "qdb_gpu_timing_enable", "qdb_gpu_timing_reset", "qdb_gpu_timing_avg_ms", "qdb_gpu_struct_sizes",
     "qdb_gpu_device_limits", "qdb_gpu_aggregate_kernel_source", "qdb_gpu_jit_check", "qdb_gpu_jit_stats",
-    "qdb_gpu_agg_update_batched", "qdb_gpu_agg_processor_add_batched", "qdb_gpu_trim_cache", "qdb_gpu_pool_stats",
+    "qdb_gpu_agg_update_batched", "qdb_gpu_agg_processor_add_batched", "qdb_gpu_agg_finish_batched",
+    "qdb_gpu_agg_processor_finish_batched", "qdb_gpu_trim_cache", "qdb_gpu_pool_stats",
     "qdb_gpu_exec_last_error", "qdb_gpu_agg_processor_new", "qdb_gpu_agg_processor_add",
     "qdb_gpu_agg_processor_finish", "qdb_gpu_agg_processor_groups", "qdb_gpu_agg_processor_free",
     "qdb_gpu_join_processor_new", "qdb_gpu_join_processor_append", "qdb_gpu_join_processor_build",
@@ -242,6 +243,9 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "qdb_gpu_jit_stats": (i64, [C.POINTER(C.c_double)]),
         "qdb_gpu_agg_update_batched": (C.c_int, [C.POINTER(QdbAggregateKernels), C.POINTER(P), C.POINTER(C.POINTER(TRowSet)), i64]),
         "qdb_gpu_agg_processor_add_batched": (C.c_int, [C.POINTER(P), C.POINTER(C.POINTER(TRowSet)), i64]),
+        "qdb_gpu_agg_finish_batched": (C.c_int, [C.POINTER(QdbAggregateKernels), C.POINTER(P), C.POINTER(TRowSet),
+                                                 C.POINTER(i64), i64]),
+        "qdb_gpu_agg_processor_finish_batched": (C.c_int, [C.POINTER(P), C.POINTER(TRowSet), i64]),
         "qdb_gpu_trim_cache": (None, []),
         "qdb_gpu_pool_stats": (i64, [P, i64]),
         "qdb_gpu_struct_sizes": (i64, [C.POINTER(i64), i64]),

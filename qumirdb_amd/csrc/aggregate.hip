@@ -1580,22 +1580,33 @@ bool batched_update(AggPlan* P, AggTable* const* tables, const TRowSet* const* b
     return ok;
 }
 
-// agg_finish_rowset: groups -> columnar TRowSet; destroys the table.
-int64_t table_finish(AggTable* T, TRowSet* out)
+// The finalize's device temporaries, released once its kernels are done.
+struct FinishTemps {
+    uint32_t* index = nullptr;
+    uint32_t* counts = nullptr;
+    uint32_t* offsets = nullptr;
+    void release()
+    {
+        dfree(index);
+        dfree(counts);
+        dfree(offsets);
+        index = counts = offsets = nullptr;
+    }
+};
+
+// agg_finish_rowset, first half: groups -> columnar TRowSet, every kernel
+// launched, nothing waited for; returns the row count or -1.
+int64_t table_finish_launch(AggTable* T, TRowSet* out, FinishTemps* tmp)
 {
     AggPlan* P = T->plan;
     FinArgs F = P->fin;
     F.recs = T->recs;
     F.stride = T->rstride;
     F.index = nullptr;
-    uint32_t* index = nullptr;
-    uint32_t* counts = nullptr;
-    uint32_t* offsets = nullptr;
-    auto release = [&]() {
-        dfree(index);
-        dfree(counts);
-        dfree(offsets);
-    };
+    uint32_t*& index = tmp->index;
+    uint32_t*& counts = tmp->counts;
+    uint32_t*& offsets = tmp->offsets;
+    auto release = [&]() { tmp->release(); };
     if (P->spec.Global && T->groups == 0) {
         // The one group of a global aggregate over an empty input: slot 0,
         // still at its identities (the key word of a keyless aggregate is 0).
@@ -1676,11 +1687,29 @@ int64_t table_finish(AggTable* T, TRowSet* out)
             return -1;
         }
     }
-    const bool synced = sync("agg finalize");
-    release();
-    if (!synced) return -1;
-    if (P->spec.OutputOnHost && !move_rowset_to_host(out, P->out_types)) return -1;
     return (int64_t)n;
+}
+
+// Second half, once the stream has been synchronized: the temporaries
+// released, the output moved to the host when the spec asks.
+bool table_finish_done(AggTable* T, TRowSet* out, FinishTemps* tmp)
+{
+    tmp->release();
+    return !T->plan->spec.OutputOnHost || move_rowset_to_host(out, T->plan->out_types);
+}
+
+// agg_finish_rowset: groups -> columnar TRowSet; destroys the table.
+int64_t table_finish(AggTable* T, TRowSet* out)
+{
+    FinishTemps tmp;
+    const int64_t n = table_finish_launch(T, out, &tmp);
+    if (n < 0) {
+        tmp.release();
+        return -1;
+    }
+    const bool synced = sync("agg finalize");
+    if (!table_finish_done(T, out, &tmp) || !synced) return -1;
+    return n;
 }
 
 int64_t agg_dispatch_impl(AggPlan* P, void* ht, TRowSet* batch, int64_t arg, int64_t op)
@@ -1825,6 +1854,52 @@ extern "C" int qdb_gpu_agg_update_batched(const QdbAggregateKernels* kernels, vo
         HtBlob* B = (HtBlob*)hts[i];
         B->Size = B->table->groups;
         B->Capacity = (int64_t)B->table->capacity;
+    }
+    return 0;
+}
+
+// agg_finish_rowset of hts[i] into outs[i] for every i: every table's
+// finalize kernels are launched before one synchronization; each table is
+// destroyed as by agg_finish_rowset. rows[i] gets the row count. Returns 0,
+// or -1 (then every output that was made is released).
+extern "C" int qdb_gpu_agg_finish_batched(const QdbAggregateKernels* kernels, void* const* hts, TRowSet* outs, int64_t* rows,
+    int64_t n)
+{
+    clear_error();
+    if (!kernels || (n > 0 && (!hts || !outs || !rows))) return fail("null argument"), -1;
+    if (kernels->Slot < 0 || kernels->Slot >= kSlots) return fail("bad aggregate slot %lld", (long long)kernels->Slot), -1;
+    AggPlan* P = agg_slots().get((int)kernels->Slot);
+    if (!P) return fail("aggregate kernels are not compiled"), -1;
+    for (int64_t i = 0; i < n; ++i) {
+        HtBlob* B = (HtBlob*)hts[i];
+        if (!B || !B->table) return fail("batched finish of table %lld before Init", (long long)i), -1;
+        if (B->table->plan != P) return fail("table %lld belongs to another aggregate", (long long)i), -1;
+    }
+    std::vector<FinishTemps> tmp((size_t)n);
+    bool ok = true;
+    int64_t made = 0;
+    for (; made < n && ok; ++made) {
+        memset(&outs[made], 0, sizeof(TRowSet));
+        rows[made] = table_finish_launch(((HtBlob*)hts[made])->table, &outs[made], &tmp[(size_t)made]);
+        ok = rows[made] >= 0;
+    }
+    ok = sync("agg finalize") && ok;
+    for (int64_t i = 0; i < made; ++i) {
+        HtBlob* B = (HtBlob*)hts[i];
+        if (rows[i] >= 0 && !table_finish_done(B->table, &outs[i], &tmp[(size_t)i])) ok = false;
+        tmp[(size_t)i].release();
+    }
+    if (!ok) {
+        for (int64_t i = 0; i < made; ++i) {
+            if (outs[i].Destroy) outs[i].Destroy(&outs[i]);
+            memset(&outs[i], 0, sizeof(TRowSet));
+        }
+        return -1;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        HtBlob* B = (HtBlob*)hts[i];
+        table_free(B->table);
+        memset(B, 0, sizeof(*B));
     }
     return 0;
 }

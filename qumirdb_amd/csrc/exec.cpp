@@ -92,6 +92,34 @@ void TAggregateProcessor::AddBatched(TAggregateProcessor* const* processors, TRo
     if (qdb_gpu_agg_update_batched(&processors[0]->Kernels_, tables.data(), rowSets, n) != 0) throw Error("batched aggregate update failed");
 }
 
+void TAggregateProcessor::FinishBatched(TAggregateProcessor* const* processors, TRowSet* rowSets, int64_t n)
+{
+    if (n <= 0) return;
+    std::vector<void*> tables((size_t)n);
+    std::vector<int64_t> sizes((size_t)n), rows((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        TAggregateProcessor* p = processors[i];
+        if (p->Finished_) throw std::runtime_error("aggregate processor is already finished");
+        if (p->Kernels_.Slot != processors[0]->Kernels_.Slot) throw std::invalid_argument("batched processors of different aggregates");
+        p->EnsureInit();
+        tables[(size_t)i] = p->HashTable_.data();
+        sizes[(size_t)i] = reinterpret_cast<THashTableHead*>(p->HashTable_.data())->Size;
+    }
+    for (int64_t i = 0; i < n; ++i) processors[i]->Finished_ = true;
+    if (qdb_gpu_agg_finish_batched(&processors[0]->Kernels_, tables.data(), rowSets, rows.data(), n) != 0) {
+        throw Error("batched aggregate finalize failed");
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        // A global aggregate over an empty input emits its one group.
+        if (rows[(size_t)i] != sizes[(size_t)i] && !(sizes[(size_t)i] == 0 && rows[(size_t)i] == 1)) {
+            for (int64_t k = 0; k < n; ++k) {
+                if (rowSets[k].Destroy) rowSets[k].Destroy(&rowSets[k]);
+            }
+            throw std::runtime_error("aggregate finalize returned an unexpected row count");
+        }
+    }
+}
+
 bool TAggregateProcessor::Finish(TRowSet& rowSet)
 {
     if (Finished_) return false;
@@ -372,6 +400,13 @@ int64_t qdb_gpu_agg_processor_finish(void* p, TRowSet* out)
         return -1;
     }
     return rows;
+}
+
+int qdb_gpu_agg_processor_finish_batched(void* const* procs, TRowSet* outs, int64_t n)
+{
+    return Guard([&] {
+        NQdbGpu::TAggregateProcessor::FinishBatched(reinterpret_cast<NQdbGpu::TAggregateProcessor* const*>(procs), outs, n);
+    });
 }
 
 int64_t qdb_gpu_agg_processor_groups(void* p)

@@ -52,6 +52,9 @@ public:
     // rowSets[i], in one batched launch (qdb_gpu_agg_update_batched).
     static void AddBatched(TAggregateProcessor* const* processors, TRowSet* const* rowSets, int64_t n);
     bool Finish(TRowSet& rowSet);
+    // Finishes many processors of one compiled aggregate, processors[i] into
+    // rowSets[i], with one synchronization (qdb_gpu_agg_finish_batched).
+    static void FinishBatched(TAggregateProcessor* const* processors, TRowSet* rowSets, int64_t n);
     int64_t Groups() const;
     bool Flush(TRowSet& rowSet);
 

@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass, field
-from typing import Optional, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -532,6 +532,39 @@ class TAggregateProcessor:
         rows = (C.POINTER(abi.TRowSet) * max(n, 1))(*[C.pointer(b.rs) for b in batches])
         if lib().qdb_gpu_agg_processor_add_batched(procs, rows, n) != 0:
             raise RuntimeError((lib().qdb_gpu_exec_last_error() or b"").decode())
+
+    @staticmethod
+    def _finish_batched_raw(processors: Sequence["TAggregateProcessor"]):
+        n = len(processors)
+        procs = (C.c_void_p * max(n, 1))(*[p.p for p in processors])
+        outs = (abi.TRowSet * max(n, 1))()
+        if lib().qdb_gpu_agg_processor_finish_batched(procs, outs, n) != 0:
+            raise RuntimeError((lib().qdb_gpu_exec_last_error() or b"").decode())
+        return outs
+
+    @staticmethod
+    def finish_batched(processors: Sequence["TAggregateProcessor"]):
+        """processors[i].finish() for every i, with one synchronization for
+        all of them (all processors of one compiled aggregate)."""
+        outs = TAggregateProcessor._finish_batched_raw(processors)
+        res = []
+        for i, p in enumerate(processors):
+            try:
+                res.append(download_rowset(outs[i], p.kernels.out_types))
+            finally:
+                lib().qdb_gpu_rowset_release(C.byref(outs[i]))
+        return res
+
+    @staticmethod
+    def finish_in_hbm_batched(processors: Sequence["TAggregateProcessor"]) -> List[int]:
+        """finish_in_hbm() of every processor with one synchronization;
+        returns the group counts."""
+        outs = TAggregateProcessor._finish_batched_raw(processors)
+        counts = []
+        for i in range(len(processors)):
+            counts.append(int(outs[i].RowCount))
+            lib().qdb_gpu_rowset_release(C.byref(outs[i]))
+        return counts
 
     def groups(self) -> int:
         return lib().qdb_gpu_agg_processor_groups(self.p)

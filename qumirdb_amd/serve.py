@@ -8,7 +8,8 @@ group-by (100M rows). A request is one shape over a row range of its table.
 Requests arrive on their own clock; the server admits every request that
 has arrived while the estimated hash-table and intermediate bytes of the
 admitted work stay under the HBM budget, and runs them together: group-bys
-of one shape share ONE batched launch (qdb_gpu_agg_update_batched), joins
+of one shape share ONE batched launch (qdb_gpu_agg_update_batched) and one
+synchronization for their finishes (qdb_gpu_agg_finish_batched), joins
 probe the resident build table in turn. Requests that arrive while a round
 runs are admitted in the next round, as slots and memory free up. The plain
 loop runs one request per round, in arrival order. Integer and decimal
@@ -135,6 +136,14 @@ class Server:
             return sorted(zip(*[c[0].tolist() for c in cols]))
         return proc.finish_in_hbm()
 
+    def _agg_results_batched(self, shape: str, procs: List[TAggregateProcessor]):
+        """The results of every processor, finished with one synchronization."""
+        if shape == "lineitem":
+            return [W.lineitem_finalize(cols) for cols in TAggregateProcessor.finish_batched(procs)]
+        if self.full:
+            return [sorted(zip(*[c[0].tolist() for c in cols])) for cols in TAggregateProcessor.finish_batched(procs)]
+        return TAggregateProcessor.finish_in_hbm_batched(procs)
+
     def _agg_procs(self, reqs: List[Request]):
         shape = reqs[0].shape
         kern = self.res.li_kern if shape == "lineitem" else self.res.g_kern
@@ -165,9 +174,15 @@ class Server:
             procs, views = self._agg_procs(group)
             if batched:
                 TAggregateProcessor.add_batched(procs, views)
-            else:
-                for p, v in zip(procs, views):
-                    p.add(v)
+                results = self._agg_results_batched(shape, procs)
+                done = clock()
+                for r, p, res in zip(group, procs, results):
+                    r.result = res
+                    p.close()
+                    r.done = done
+                continue
+            for p, v in zip(procs, views):
+                p.add(v)
             for r, p in zip(group, procs):
                 r.result = self._agg_result(shape, p)
                 p.close()

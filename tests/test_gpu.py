@@ -111,11 +111,12 @@ def test_generated_kernel_equals_generic(gpu_backend, monkeypatch):
 
 
 def test_batched_update_equals_plain(gpu_backend):
-    """Many tables of one aggregate updated in one persistent launch give
-    exactly the results of one update per table: instances from empty to
-    300K rows, few to many groups, nullable multi-column keys, every
-    reducer; the small initial capacities make several tables grow and
-    replay their leftover rows inside the batched call."""
+    """Many tables of one aggregate updated in one persistent launch, and
+    finished with one synchronization, give exactly the results of one
+    update and one finish per table: instances from empty to 300K rows, few
+    to many groups, nullable multi-column keys, every reducer; the small
+    initial capacities make several tables grow and replay their leftover
+    rows inside the batched call."""
     import qumirdb_amd as q
     from cases import random_agg_case
     sizes = [0, 1, 7, 100, 2000, 5000, 33000, 65536, 100000, 300000, 50, 40000]
@@ -140,8 +141,9 @@ def test_batched_update_equals_plain(gpu_backend):
     q.TAggregateProcessor.add_batched(procs, devs)
     # a second batched round on the same tables accumulates
     q.TAggregateProcessor.add_batched(procs[:3], devs[:3])
+    finished = q.TAggregateProcessor.finish_batched(procs)
     for i, p in enumerate(procs):
-        got = sorted(result_rows(p.finish()), key=sort_key)
+        got = sorted(result_rows(finished[i]), key=sort_key)
         p.close()
         if i < 3:
             _, b = insts[i]
