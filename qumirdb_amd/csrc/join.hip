@@ -652,7 +652,7 @@ __device__ __forceinline__ void fused_add(FusedLane& a, int kind, uint32_t times
 }
 
 constexpr int kSub = 4;                                  // rows per lane per load round
-constexpr int kSubRounds = 4;                            // rounds per wave tile
+constexpr int kSubRounds = 3;                            // rounds per wave tile
 constexpr int kTileRows = kSub * kSubRounds;             // rows per lane per wave tile
 constexpr int kWaveTile = kWave * kTileRows;             // rows per output reservation
 
@@ -1743,7 +1743,6 @@ bool do_append(JoinBuild* J, const TRowSet* batch)
     if (!counts || !total) return false;
     hipLaunchKernelGGL((k_append_count<KW>), dim3(nblocks), dim3(kBlock), 0, stream(), cols, P->build_key, n, batch->Selection, counts);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, stream(), counts, (unsigned int)nblocks, total);
-    int64_t added = 0;
     bool ok = hip_ok(hipGetLastError(), "append count");
     for (int p0 = 0; ok && (p0 < P->npayload || p0 == 0); p0 += kMaxDevCols) {
         const int np = P->npayload - p0 < kMaxDevCols ? P->npayload - p0 : kMaxDevCols;
@@ -1773,7 +1772,6 @@ bool do_append(JoinBuild* J, const TRowSet* batch)
     dfree(counts);
     dfree(total);
     if (!ok) return false;
-    (void)added;
     J->rows_bound += n;
     J->rows_known = false;
     J->appended += n;
