@@ -277,12 +277,40 @@ __device__ __forceinline__ int64_t build_insert_once(Slot<KW>* table, uint64_t c
     }
 }
 
+// One-word keys when a value no build key takes is known (`empty`, the
+// table's key words start there): a slot is claimed by one 64-bit CAS of its
+// key word, which also tells a later row of the same key where it went, so
+// no acquire or release is needed (on this part those write back or
+// invalidate L2 on every insert). Emptiness is read from the row count by
+// every later kernel.
+__device__ __forceinline__ int64_t build_insert_cas(Slot<1>* table, uint64_t cmask, uint64_t key, uint64_t h, uint64_t empty)
+{
+    uint64_t s = h & cmask;
+    for (;;) {
+        const unsigned long long old = atomicCAS((unsigned long long*)&table[s].key[0], (unsigned long long)empty,
+            (unsigned long long)key);
+        if (old == empty || old == key) return (int64_t)s;
+        s = (s + 1) & cmask;
+    }
+}
+
+// Key words of a one-word-key table to `empty`, heads and counts to 0.
+__global__ void __launch_bounds__(kBlock) k_fill_slots(Slot<1>* table, uint64_t capacity, uint64_t empty)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < capacity; s += stride) {
+        table[s].key[0] = empty;
+        table[s].head = 0u;
+        table[s].count = 0u;
+    }
+}
+
 // Inserts every arena row's key (distinct keys get a slot), counts the rows
 // of each key and records each row's slot. `flags[0]` is set when a key has
-// more than one row.
+// more than one row. use_empty (KW = 1): claim slots with build_insert_cas.
 template <int KW>
 __global__ void __launch_bounds__(kBlock) k_build_insert(Slot<KW>* table, uint64_t cmask, const uint64_t* arena_keys,
-    int64_t rows, uint32_t* row_slot, unsigned int* flags)
+    int64_t rows, uint32_t* row_slot, unsigned int* flags, int use_empty, uint64_t empty)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     // Every thread of the block runs the same number of iterations.
@@ -294,6 +322,12 @@ __global__ void __launch_bounds__(kBlock) k_build_insert(Slot<KW>* table, uint64
         for (int w = 0; w < KW; ++w) key[w] = pending ? arena_keys[r * KW + w] : 0ull;
         const uint64_t h = key_place_hash<KW>(key);
         int64_t s = -1;
+        if constexpr (KW == 1) {
+            if (use_empty) {
+                if (pending) s = build_insert_cas((Slot<1>*)table, cmask, key[0], h, empty);
+                pending = false;
+            }
+        }
         while (__any(pending)) {
             if (pending) {
                 s = build_insert_once<KW>(table, cmask, key, h, r);
@@ -1469,6 +1503,9 @@ struct JoinBuild {
     unsigned long long* rows_dev = nullptr;
     int64_t rows_bound = 0;
     bool rows_known = true;
+    // One-word keys: the smallest and largest build key, when read back.
+    bool have_range = false;
+    uint64_t kmin = 0, kmax = 0;
     // device: the pair counter (word 0) and then the fused aggregate words
     // of the last probe, read back together
     unsigned long long* pair_count = nullptr;
@@ -1848,6 +1885,9 @@ bool try_dense_build(JoinBuild* J)
     if (ok) {
         J->rows = (int64_t)h_info[4];
         J->rows_known = true;
+        J->have_range = J->rows > 0;
+        J->kmin = h_info[0];
+        J->kmax = h_info[1];
     }
     const unsigned int* h_flags = (const unsigned int*)(h_info + 2);
     const uint64_t range = h_info[1] - h_info[0] + 1;
@@ -1916,12 +1956,58 @@ bool do_build(JoinBuild* J)
         release();
         return false;
     }
-    bool ok = zero_fill(table, capacity * sizeof(Slot<KW>)) && zero_fill(J->bloom, (size_t)words * 4) && zero_fill(stats, 16);
+    // One-word keys: their range (read back here when no dense attempt
+    // read it), and an empty key word outside it.
+    if (KW == 1 && !J->have_range && rows > 0) {
+        const int mgrid = grid_for(rows, kBlock, 2);
+        unsigned long long* mm = (unsigned long long*)dalloc(16);
+        unsigned long long* mpart = (unsigned long long*)dalloc((size_t)mgrid * 16);
+        unsigned long long h_mm[2] = {0, 0};
+        bool got = mm && mpart;
+        if (got) {
+            hipLaunchKernelGGL(k_key_minmax, dim3(mgrid), dim3(kBlock), 0, stream(), J->keys, (const unsigned long long*)J->rows_dev,
+                mpart);
+            ReduceOps mops;
+            memset(&mops, 0, sizeof(mops));
+            mops.op[0] = 3;  // unsigned min
+            mops.op[1] = 4;  // unsigned max
+            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, stream(), mpart, mgrid, 2, 2, mops, mm);
+            got = hip_ok(hipGetLastError(), "key range") && hip_ok(hipMemcpyAsync(h_mm, mm, 16, hipMemcpyDeviceToHost, stream()), "key range")
+                && sync("key range");
+        }
+        dfree(mm);
+        dfree(mpart);
+        if (got) {
+            J->have_range = true;
+            J->kmin = h_mm[0];
+            J->kmax = h_mm[1];
+        }
+        clear_error();
+    }
+    int use_empty = 0;
+    uint64_t empty = 0;
+    if (KW == 1 && J->have_range) {
+        if (J->kmax != ~0ull) {
+            use_empty = 1;
+            empty = J->kmax + 1;
+        } else if (J->kmin != 0ull) {
+            use_empty = 1;
+            empty = J->kmin - 1;
+        }
+    }
+    bool ok = zero_fill(J->bloom, (size_t)words * 4) && zero_fill(stats, 16);
+    if (ok && use_empty) {
+        hipLaunchKernelGGL(k_fill_slots, dim3(grid_for((int64_t)capacity, kBlock, 8)), dim3(kBlock), 0, stream(),
+            (Slot<1>*)table, capacity, empty);
+        ok = hip_ok(hipGetLastError(), "slot fill");
+    } else if (ok) {
+        ok = zero_fill(table, capacity * sizeof(Slot<KW>));
+    }
     if (ok) {
         TimedRegion timer("join_build");
         if (rows > 0) {
             hipLaunchKernelGGL((k_build_insert<KW>), dim3(grid_for(rows, kBlock, 8)), dim3(kBlock), 0, stream(), table,
-                capacity - 1, J->keys, rows, row_slot, (unsigned int*)(stats + 1));
+                capacity - 1, J->keys, rows, row_slot, (unsigned int*)(stats + 1), use_empty, empty);
         }
         hipLaunchKernelGGL((k_slot_counts<KW>), dim3(cgrid), dim3(kBlock), 0, stream(), table, capacity, counts, J->bloom,
             words - 1u, dpart);
