@@ -319,6 +319,31 @@ def test_join_dense_lean_equals_general(gpu_backend, monkeypatch, probe_is_left,
         assert got["1"][1][2][0] == int(pval[prow].min()) and got["1"][1][3][0] == int(pval[prow].max())
 
 
+@pytest.mark.parametrize("extremes", ["none", "max", "both"])
+@pytest.mark.parametrize("table_mode", [0, 1])
+def test_join_hash_build_key_extremes(gpu_backend, extremes, table_mode):
+    """The hash build's slot claim with one-word keys takes an empty key
+    word outside the build keys' range (above the largest, else below the
+    smallest) and falls back to the locked protocol when the keys span the
+    whole range: keys with INT64_MAX and/or INT64_MIN, duplicates (chains),
+    probed with the same extremes, against the oracle."""
+    rng = np.random.default_rng(31 + len(extremes))
+    types = [T(SIGNED, 8)]
+    bk = rng.integers(-2000, 2000, 6000).astype(np.int64)
+    ext = {"none": [], "max": [np.iinfo(np.int64).max], "both": [np.iinfo(np.int64).max, np.iinfo(np.int64).min]}[extremes]
+    bk = np.concatenate([bk, np.array(ext * 3, np.int64)])
+    pk = np.concatenate([rng.integers(-2500, 2500, 50000).astype(np.int64), np.array(ext * 5 + [0, -1], np.int64)])
+    case = JoinCase(types, types, [0], [0], table_mode=table_mode)
+    pairs, _ = gpu_backend.join(case, [Batch(types, [bk])], Batch(types, [pk]))
+    oj = O.OracleJoin(case.to_c())
+    oj.append(OracleBackend().host(Batch(types, [bk])))
+    oj.build()
+    opairs = oj.probe(OracleBackend().host(Batch(types, [pk])))
+    srt = lambda p: p[np.lexsort((p[:, 1], p[:, 0]))]
+    assert len(pairs) == len(opairs) > 0
+    assert np.array_equal(srt(pairs), srt(opairs))
+
+
 @pytest.mark.parametrize("table_mode", [0, 1])
 def test_join_empty_probe_adds_nothing_to_fused(gpu_backend, table_mode):
     """An empty probe batch, before and after a real one, leaves the fused
