@@ -721,7 +721,9 @@ std::string u64_lit(uint64_t v)
 
 // Emits the program as one DVal per register and returns the name holding
 // its result.
-std::string emit_program(std::string& o, const std::string& name, const DevProgram& prog, const AggPlan* P)
+// Straight-line code for a program over columns of the given slot types;
+// returns the name of its result register.
+std::string emit_program_typed(std::string& o, const std::string& name, const DevProgram& prog, const QdbValueType* slot_type)
 {
     for (int q = 0; q < QDB_EXPR_MAX_REGS; ++q) o += "    DVal " + name + "_r" + std::to_string(q) + " = dval_zero();\n";
     auto reg = [&](int q) { return name + "_r" + std::to_string(q >= 0 && q < QDB_EXPR_MAX_REGS ? q : 0); };
@@ -729,7 +731,7 @@ std::string emit_program(std::string& o, const std::string& name, const DevProgr
         const DevInsn& in = prog.insns[k];
         if (in.dst < 0 || in.dst >= QDB_EXPR_MAX_REGS) continue;
         if (in.op == QDB_EXPR_COLUMN) {
-            const QdbValueType& t = P->slot_type[in.column];
+            const QdbValueType& t = slot_type[in.column];
             o += "    " + reg(in.dst) + " = read_col_t<" + std::to_string(t.Kind) + ", " + std::to_string(t.Width) + ">(C.c[" +
                 std::to_string(in.column) + "], row);\n";
         } else {
@@ -741,6 +743,11 @@ std::string emit_program(std::string& o, const std::string& name, const DevProgr
         }
     }
     return reg(prog.result);
+}
+
+std::string emit_program(std::string& o, const std::string& name, const DevProgram& prog, const AggPlan* P)
+{
+    return emit_program_typed(o, name, prog, P->slot_type);
 }
 
 // Class (QDB_REG_*) of a plan's argument j as a kernel computes it.
@@ -1950,6 +1957,33 @@ void release_aggregate_slot(int slot)
 {
     agg_slots().release(slot);
 }
+}  // namespace qdbgpu
+
+namespace qdbgpu {
+
+// The filter kernel generated for one predicate (the reference's compiler
+// generates one per filter, qdb/kernel/gen.cpp:911-1091): the program as
+// straight-line code over columns of the given slot types; a row is kept
+// when its input selection keeps it and the predicate is TRUE (NULL is not
+// TRUE).
+std::string filter_jit_source(const DevProgram& prog, const QdbValueType* slot_type)
+{
+    std::string o;
+    o += "#define QDB_SPECIALIZED 1\n#include \"aggregate_kernels.h\"\n";
+    o += "namespace qdbgpu {\nnamespace agg {\n";
+    o += "__device__ __forceinline__ DVal dval_zero()\n{\n    DVal v;\n    v.i = 0;\n    v.f = 0.0;\n    v.cls = 0;\n    v.valid = 0;\n    return v;\n}\n";
+    o += "}  // namespace agg\n}  // namespace qdbgpu\n";
+    o += "using namespace qdbgpu;\nusing namespace qdbgpu::agg;\n";
+    o += "extern \"C\" __global__ void __launch_bounds__(256) qdb_filter(DevCols C, int64_t n, const uint8_t* in_sel, uint8_t* out_sel)\n{\n";
+    o += "    const int64_t stride = (int64_t)gridDim.x * blockDim.x;\n";
+    o += "    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < n; row += stride) {\n";
+    o += "        uint8_t keep = 0;\n        if (!in_sel || in_sel[row]) {\n";
+    const std::string r = emit_program_typed(o, "p", prog, slot_type);
+    o += "            keep = (" + r + ".valid && " + r + ".i != 0) ? 1 : 0;\n        }\n";
+    o += "        out_sel[row] = keep;\n    }\n}\n";
+    return o;
+}
+
 }  // namespace qdbgpu
 
 namespace qdbgpu {

@@ -14,13 +14,19 @@
 
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "device.h"
+#include "jit.h"
 #include "runtime.h"
 #include "slots.h"
 
 namespace qdbgpu {
+
+// The filter kernel's source for one predicate (generated beside the
+// aggregate kernels, aggregate.hip).
+std::string filter_jit_source(const DevProgram& prog, const QdbValueType* slot_type);
 
 namespace {
 
@@ -30,9 +36,15 @@ struct ExprPlan {
     QdbColumnSchema input;
     int nslots = 0;
     int slot_col[kMaxDevCols];
+    QdbValueType slot_type[kMaxDevCols];
     int nprogs = 0;
     DevProgram* d_progs = nullptr;
+    std::vector<DevProgram> h_progs;
     QdbValueType out_types[QDB_PROJECT_MAX_OUTPUTS];
+    // A filter's kernel generated for its predicate (made at its first call;
+    // null: the generic interpreter kernel runs).
+    std::once_flag jit_once;
+    hipFunction_t jit_filter = nullptr;
     ~ExprPlan() { dfree(d_progs); }
 };
 
@@ -61,6 +73,7 @@ bool compile_programs(const QdbColumnSchema& input, const QdbExprProgram* progs,
             if (col_slot[in.Column] < 0) {
                 if (P->nslots >= kMaxDevCols) return fail("too many columns in one program (max %d)", kMaxDevCols);
                 col_slot[in.Column] = P->nslots;
+                P->slot_type[P->nslots] = t;
                 P->slot_col[P->nslots++] = in.Column;
             }
             in.Column = col_slot[in.Column];
@@ -72,6 +85,7 @@ bool compile_programs(const QdbColumnSchema& input, const QdbExprProgram* progs,
         for (int p = 0; p < n; ++p) dev[(size_t)p] = to_dev_program(out[(size_t)p]);
         P->d_progs = (DevProgram*)upload(dev.data(), dev.size() * sizeof(DevProgram));
         if (!P->d_progs) return false;
+        P->h_progs = dev;
     }
     return true;
 }
@@ -243,10 +257,27 @@ void filter_impl(ExprPlan* P, TRowSet* rs, const uint8_t* input_sel)
     }
     DevCols cols;
     if (!bind_cols(P, rs, &cols)) return;
+    std::call_once(P->jit_once, [P]() {
+        if (!jit_enabled() || P->nprogs != 1) return;
+        std::string err;
+        P->jit_filter = jit_function(filter_jit_source(P->h_progs[0], P->slot_type), "qdb_filter", &err);
+    });
     {
         TimedRegion timer("filter");
-        hipLaunchKernelGGL(k_filter, dim3(grid_for(rs->RowCount, kBlock, 8)), dim3(kBlock), 0, stream(), cols, P->d_progs,
-            rs->RowCount, input_sel, rs->Selection);
+        const int grid = grid_for(rs->RowCount, kBlock, 8);
+        if (P->jit_filter) {
+            int64_t n = rs->RowCount;
+            const uint8_t* in_sel = input_sel;
+            uint8_t* out_sel = rs->Selection;
+            void* args[] = {&cols, &n, &in_sel, &out_sel};
+            if (!hip_ok(hipModuleLaunchKernel(P->jit_filter, (unsigned)grid, 1, 1, kBlock, 1, 1, 0, stream(), args, nullptr),
+                    "filter launch")) {
+                return;
+            }
+        } else {
+            hipLaunchKernelGGL(k_filter, dim3(grid), dim3(kBlock), 0, stream(), cols, P->d_progs, rs->RowCount, input_sel,
+                rs->Selection);
+        }
     }
     if (hip_ok(hipGetLastError(), "filter launch")) (void)sync("filter");
 }

@@ -134,19 +134,29 @@ __global__ void __launch_bounds__(1024) k_scan_counts(unsigned int* counts, unsi
     const unsigned int hi = lo + per < n ? lo + per : n;
     unsigned long long s = 0;
     for (unsigned int i = lo; i < hi; ++i) s += counts[i];
-    partial[threadIdx.x] = s;
+    // Exclusive scan of the threads' sums: within each wave by shuffles,
+    // then over the waves' totals.
+    const int lane = lane_id(), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    unsigned long long incl = s;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const unsigned int ylo = (unsigned int)__shfl_up((int)(unsigned int)incl, o, kWave);
+        const unsigned int yhi = (unsigned int)__shfl_up((int)(unsigned int)(incl >> 32), o, kWave);
+        if (lane >= o) incl += ((unsigned long long)yhi << 32) | ylo;
+    }
+    if (lane == kWave - 1) partial[wave] = incl;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long run = 0;
-        for (unsigned int t = 0; t < blockDim.x; ++t) {
-            unsigned long long v = partial[t];
-            partial[t] = run;
+        for (int w = 0; w < nw; ++w) {
+            const unsigned long long v = partial[w];
+            partial[w] = run;
             run += v;
         }
         *total = run;
     }
     __syncthreads();
-    unsigned long long run = partial[threadIdx.x];
+    unsigned long long run = partial[wave] + incl - s;
     for (unsigned int i = lo; i < hi; ++i) {
         unsigned int v = counts[i];
         counts[i] = (unsigned int)run;
