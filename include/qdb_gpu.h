@@ -455,6 +455,10 @@ int64_t qdb_gpu_device_limits(int64_t* out, int64_t n);
  * environment variable QDB_GPU_JIT=0 runs the precompiled generic kernels
  * instead. */
 int64_t qdb_gpu_aggregate_kernel_source(const QdbAggregateSpec* spec, char* out, int64_t cap);
+/* The same for a filter: the kernel generated for its predicate (the
+ * reference's compiler generates one per filter, qdb/kernel/gen.cpp:
+ * 911-1091); its length, or -1. */
+int64_t qdb_gpu_filter_kernel_source(const QdbFilterSpec* spec, char* out, int64_t cap);
 
 /* Batched form (no counterpart in the reference, whose processors update
  * one table per call): applies batches[i] to the initialized HashTable blob

@@ -49,7 +49,7 @@ struct ExprPlan {
 };
 
 // Remaps the programs' input columns to device slots and uploads them.
-bool compile_programs(const QdbColumnSchema& input, const QdbExprProgram* progs, int n, ExprPlan* P)
+bool compile_programs(const QdbColumnSchema& input, const QdbExprProgram* progs, int n, ExprPlan* P, bool on_device = true)
 {
     P->input = input;
     int col_slot[64];
@@ -83,9 +83,10 @@ bool compile_programs(const QdbColumnSchema& input, const QdbExprProgram* progs,
     if (n > 0) {
         std::vector<DevProgram> dev((size_t)n);
         for (int p = 0; p < n; ++p) dev[(size_t)p] = to_dev_program(out[(size_t)p]);
+        P->h_progs = dev;
+        if (!on_device) return true;
         P->d_progs = (DevProgram*)upload(dev.data(), dev.size() * sizeof(DevProgram));
         if (!P->d_progs) return false;
-        P->h_progs = dev;
     }
     return true;
 }
@@ -391,6 +392,24 @@ extern "C" int qdb_gpu_compile_filter(const QdbFilterSpec* spec, QdbFilterFn* ou
     }
     *out = kFilterFns.fns[slot];
     return slot;
+}
+
+// The source of the kernel generated for a filter's predicate (no GPU
+// needed); returns its length, writing at most cap - 1 bytes and a NUL to
+// out; -1 on error.
+extern "C" int64_t qdb_gpu_filter_kernel_source(const QdbFilterSpec* spec, char* out, int64_t cap)
+{
+    clear_error();
+    if (!spec) return fail("null argument"), -1;
+    ExprPlan P;
+    if (!compile_programs(spec->Input, &spec->Predicate, 1, &P, false)) return -1;
+    const std::string src = filter_jit_source(P.h_progs[0], P.slot_type);
+    if (out && cap > 0) {
+        const size_t n = src.size() < (size_t)cap - 1 ? src.size() : (size_t)cap - 1;
+        memcpy(out, src.data(), n);
+        out[n] = '\0';
+    }
+    return (int64_t)src.size();
 }
 
 extern "C" int qdb_gpu_compile_project(const QdbProjectSpec* spec, QdbProjectFn* out)

@@ -78,3 +78,47 @@ def test_generated_source_fixes_the_layout():
     for k in ("qdb_agg_update", "qdb_agg_batch", "qdb_agg_count", "qdb_agg_scatter", "qdb_agg_sorted", "qdb_agg_merge"):
         assert k in src, k
     assert "static constexpr int kStage = 3;" in src  # key, SUM(i64) argument, f64 argument
+
+
+def _filter_specs():
+    """Filter predicates of the join workload and a three-valued OR over
+    nullable columns (NULL is not TRUE)."""
+    import qumirdb_amd as q
+    from qumirdb_amd import workloads as W
+    from qumirdb_amd.host import schema_c
+    out = []
+    prog, res = W.join_build_predicate()
+    spec = q.abi.QdbFilterSpec()
+    spec.Input = schema_c(W.BUILD_TYPES)
+    spec.Predicate = prog.build(res)
+    out.append(("join build", spec))
+    E = cases._abi()
+    from helpers import SIGNED, T
+    types = [T(SIGNED, 8, nullable=True), T(SIGNED, 8, nullable=True)]
+    p = cases.Prog(types)
+    zero = p.const(0, T(SIGNED, 8))
+    a = p.cmp(E.CMP_GT, p.col(0), zero)
+    b = p.cmp(E.CMP_GT, p.col(1), zero)
+    r = p._emit(E.EXPR_OR, E.REG_BOOL, T(3, 1, nullable=True), a=a, b=b)
+    spec2 = q.abi.QdbFilterSpec()
+    spec2.Input = q.abi.QdbColumnSchema.from_buffer_copy(cases.schema_c(types))
+    spec2.Predicate = q.abi.QdbExprProgram.from_buffer_copy(p.build(r))
+    out.append(("three-valued or", spec2))
+    return out
+
+
+def test_generated_filter_kernels_compile():
+    """The kernel generated for a filter's predicate: straight-line code,
+    one kernel qdb_filter, compiled for gfx950 on the host."""
+    import qumirdb_amd as q
+    lib = q.lib()
+    for name, spec in _filter_specs():
+        n = lib.qdb_gpu_filter_kernel_source(C.byref(spec), None, 0)
+        assert n > 0, q.last_error()
+        buf = C.create_string_buffer(n + 1)
+        assert lib.qdb_gpu_filter_kernel_source(C.byref(spec), buf, n + 1) == n
+        src = buf.value
+        assert b"qdb_filter" in src and b"eval_program" not in src
+        err = C.create_string_buffer(1 << 16)
+        rc = lib.qdb_gpu_jit_check(src, b"gfx950", err, len(err))
+        assert rc == 0, f"{name}: {err.value.decode(errors='replace')[:3000]}"
