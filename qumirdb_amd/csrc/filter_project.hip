@@ -280,7 +280,9 @@ void filter_impl(ExprPlan* P, TRowSet* rs, const uint8_t* input_sel)
                 rs->Selection);
         }
     }
-    if (hip_ok(hipGetLastError(), "filter launch")) (void)sync("filter");
+    // Stream-ordered: the selection is ready for the library's next call
+    // on its stream, and for the host after a copy or qdb_gpu_synchronize.
+    (void)hip_ok(hipGetLastError(), "filter launch");
 }
 
 void project_impl(ExprPlan* P, TRowSet* in, void** out, const uint8_t* input_sel)
@@ -302,7 +304,8 @@ void project_impl(ExprPlan* P, TRowSet* in, void** out, const uint8_t* input_sel
         hipLaunchKernelGGL(k_project, dim3(grid_for(in->RowCount, kBlock, 8)), dim3(kBlock), 0, stream(), cols, P->d_progs,
             P->nprogs, O, in->RowCount, input_sel);
     }
-    if (hip_ok(hipGetLastError(), "project launch")) (void)sync("project");
+    // Stream-ordered, as the filter.
+    (void)hip_ok(hipGetLastError(), "project launch");
 }
 
 bool hash_impl(HashPlan* P, TRowSet* rs, uint64_t* hashes, const int64_t* key_cols)

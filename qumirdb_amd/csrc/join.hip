@@ -2112,17 +2112,17 @@ template <int KW, int kTab, int kMode>
 void launch_probe_one(int grid, size_t lds, const ProbeArgs& A)
 {
     static bool attr_set = false;
+    const void* fn = (const void*)k_probe<KW, kTab, kMode>;
+    static const size_t s_lds = static_lds(fn);  // the kernel's static LDS, read once
     if (!attr_set) {
         // Room for the kernel's static LDS beside the dynamic allocation; a
         // refused attribute must not leave an error for the launch check.
-        const void* fn = (const void*)k_probe<KW, kTab, kMode>;
-        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(max_lds_per_block() - static_lds(fn)));
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(max_lds_per_block() - s_lds));
         (void)hipGetLastError();
         attr_set = true;
     }
-    const void* fn = (const void*)k_probe<KW, kTab, kMode>;
-    if (lds + static_lds(fn) > max_lds_per_block()) {
-        fail("probe: %zu bytes of LDS exceed the workgroup limit", lds + static_lds(fn));
+    if (lds + s_lds > max_lds_per_block()) {
+        fail("probe: %zu bytes of LDS exceed the workgroup limit", lds + s_lds);
         return;
     }
     hipLaunchKernelGGL((k_probe<KW, kTab, kMode>), dim3(grid), dim3(kProbeBlock), lds, stream(), A);
@@ -2143,13 +2143,14 @@ void launch_dense_one(int grid, size_t lds, const ProbeArgs& A)
 {
     static bool attr_set = false;
     const void* fn = (const void*)k_probe_dense<kMode>;
+    static const size_t s_lds = static_lds(fn);  // the kernel's static LDS, read once
     if (!attr_set) {
-        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(max_lds_per_block() - static_lds(fn)));
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(max_lds_per_block() - s_lds));
         (void)hipGetLastError();
         attr_set = true;
     }
-    if (lds + static_lds(fn) > max_lds_per_block()) {
-        fail("dense probe: %zu bytes of LDS exceed the workgroup limit", lds + static_lds(fn));
+    if (lds + s_lds > max_lds_per_block()) {
+        fail("dense probe: %zu bytes of LDS exceed the workgroup limit", lds + s_lds);
         return;
     }
     hipLaunchKernelGGL((k_probe_dense<kMode>), dim3(grid), dim3(kProbeBlock), lds, stream(), A);
@@ -2160,13 +2161,14 @@ void launch_lean_one(int grid, size_t lds, const ProbeArgs& A)
 {
     static bool attr_set = false;
     const void* fn = (const void*)k_probe_dense_lean<kVal, kMinMax, kLeft>;
+    static const size_t s_lds = static_lds(fn);  // the kernel's static LDS, read once
     if (!attr_set) {
-        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(max_lds_per_block() - static_lds(fn)));
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(max_lds_per_block() - s_lds));
         (void)hipGetLastError();
         attr_set = true;
     }
-    if (lds + static_lds(fn) > max_lds_per_block()) {
-        fail("dense probe: %zu bytes of LDS exceed the workgroup limit", lds + static_lds(fn));
+    if (lds + s_lds > max_lds_per_block()) {
+        fail("dense probe: %zu bytes of LDS exceed the workgroup limit", lds + s_lds);
         return;
     }
     hipLaunchKernelGGL((k_probe_dense_lean<kVal, kMinMax, kLeft>), dim3(grid), dim3(kProbeBlock), lds, stream(), A);
