@@ -359,7 +359,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
-              "(the path runs on a single MI355X; see DESIGN.md for how it would shard).", file=sys.stderr)
+              "(the path runs on a single MI355X).", file=sys.stderr)
         sys.exit(2)
     import qumirdb_amd
     qumirdb_amd.init_device(0)

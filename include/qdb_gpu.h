@@ -410,7 +410,7 @@ int qdb_gpu_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int qdb_gpu_memset(void* dst, int value, size_t bytes);
 int qdb_gpu_synchronize(void);
 
-/* Staging (qdb/io): copies a host TRowSet's fixed-width
+/* Staging (qdb/io; decimal storage qdb/io/parquet/decimal_storage.cpp:220-247): copies a host TRowSet's fixed-width
  * columns, masks and selection into HBM through pinned buffers with
  * hipMemcpyAsync, producing a device TRowSet whose Destroy frees it.
  * `types` gives each column's width. Returns 0 on success. */
