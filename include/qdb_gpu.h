@@ -459,6 +459,10 @@ int64_t qdb_gpu_aggregate_kernel_source(const QdbAggregateSpec* spec, char* out,
  * reference's compiler generates one per filter, qdb/kernel/gen.cpp:
  * 911-1091); its length, or -1. */
 int64_t qdb_gpu_filter_kernel_source(const QdbFilterSpec* spec, char* out, int64_t cap);
+/* The same for a projection: the kernel generated for its outputs (the
+ * reference's compiler generates one per project, qdb/kernel/gen.cpp:
+ * 1373-1613); its length, or -1. */
+int64_t qdb_gpu_project_kernel_source(const QdbProjectSpec* spec, char* out, int64_t cap);
 
 /* Batched form (no counterpart in the reference, whose processors update
  * one table per call): applies batches[i] to the initialized HashTable blob

@@ -195,7 +195,7 @@ EXPORTS = [
     "qdb_gpu_timing_enable", "qdb_gpu_timing_reset", "qdb_gpu_timing_avg_ms", "qdb_gpu_struct_sizes",
     "qdb_gpu_device_limits", "qdb_gpu_aggregate_kernel_source", "qdb_gpu_jit_check", "qdb_gpu_jit_stats",
     "qdb_gpu_agg_update_batched", "qdb_gpu_agg_processor_add_batched", "qdb_gpu_agg_finish_batched",
-    "qdb_gpu_filter_kernel_source",
+    "qdb_gpu_filter_kernel_source", "qdb_gpu_project_kernel_source",
     "qdb_gpu_agg_processor_finish_batched", "qdb_gpu_trim_cache", "qdb_gpu_pool_stats",
     "qdb_gpu_exec_last_error", "qdb_gpu_agg_processor_new", "qdb_gpu_agg_processor_add",
     "qdb_gpu_agg_processor_finish", "qdb_gpu_agg_processor_groups", "qdb_gpu_agg_processor_free",
@@ -248,6 +248,7 @@ def declare(lib: C.CDLL) -> C.CDLL:
                                                  C.POINTER(i64), i64]),
         "qdb_gpu_agg_processor_finish_batched": (C.c_int, [C.POINTER(P), C.POINTER(TRowSet), i64]),
         "qdb_gpu_filter_kernel_source": (i64, [C.POINTER(QdbFilterSpec), C.c_char_p, i64]),
+        "qdb_gpu_project_kernel_source": (i64, [C.POINTER(QdbProjectSpec), C.c_char_p, i64]),
         "qdb_gpu_trim_cache": (None, []),
         "qdb_gpu_pool_stats": (i64, [P, i64]),
         "qdb_gpu_struct_sizes": (i64, [C.POINTER(i64), i64]),

@@ -1984,6 +1984,65 @@ std::string filter_jit_source(const DevProgram& prog, const QdbValueType* slot_t
     return o;
 }
 
+// The project kernel generated for one projection (the reference's compiler
+// generates one per project, qdb/kernel/gen.cpp:1373-1613): each output's
+// program as straight-line code and its store specialized to the output's
+// kind and width. Waves cover 64 consecutive rows from a multiple of 64, so
+// that one lane writes each word of a BOOL column or a validity bitmap.
+// Rows the input selection drops keep their previous output values.
+std::string project_jit_source(const DevProgram* progs, int nout, const QdbValueType* slot_type, const QdbValueType* out_types)
+{
+    std::string o;
+    o += "#define QDB_SPECIALIZED 1\n#include \"aggregate_kernels.h\"\n";
+    o += "namespace qdbgpu {\nnamespace agg {\n";
+    o += "__device__ __forceinline__ DVal dval_zero()\n{\n    DVal v;\n    v.i = 0;\n    v.f = 0.0;\n    v.cls = 0;\n    v.valid = 0;\n    return v;\n}\n";
+    o += "}  // namespace agg\n}  // namespace qdbgpu\n";
+    o += "using namespace qdbgpu;\nusing namespace qdbgpu::agg;\n";
+    o += "extern \"C\" __global__ void __launch_bounds__(256) qdb_project(DevCols C, ProjectOut O, int64_t n, const uint8_t* in_sel)\n{\n";
+    o += "    const int lane = lane_id();\n";
+    o += "    const int64_t stride = (int64_t)gridDim.x * blockDim.x;\n";
+    o += "    for (int64_t base = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) - lane; base < n; base += stride) {\n";
+    o += "        const int64_t row = base + lane;\n";
+    o += "        const bool eval = row < n && (!in_sel || in_sel[row]);\n";
+    o += "        const uint64_t evaluated = __ballot(eval);\n";
+    for (int k = 0; k < nout; ++k) {
+        const std::string ks = std::to_string(k);
+        const QdbValueType& t = out_types[k];
+        o += "        {\n            DVal v = dval_zero();\n            if (eval) {\n";
+        const std::string r = emit_program_typed(o, "o" + ks, progs[k], slot_type);
+        o += "                v = " + r + ";\n            }\n";
+        if (t.Kind == QDB_KIND_BOOL) {
+            o += "            {\n                const uint64_t bits = __ballot(eval && v.valid && v.i != 0);\n";
+            o += "                if (lane == 0) store_bits((uint8_t*)O.data[" + ks + "], base, n, bits, evaluated);\n            }\n";
+        } else if (t.Kind == QDB_KIND_FLOAT) {
+            o += "            if (eval) {\n                double f = v.cls == QDB_REG_F64 ? v.f : (double)v.i;\n";
+            o += "                if (!v.valid) f = 0.0;\n";
+            if (t.Width == 4) o += "                ((float*)O.data[" + ks + "])[row] = (float)f;\n";
+            else o += "                ((double*)O.data[" + ks + "])[row] = f;\n";
+            o += "            }\n";
+        } else {
+            o += "            if (eval) {\n                const u128 x = v.valid ? (u128)v.i : 0;\n";
+            switch (t.Width) {
+            case 1: o += "                ((uint8_t*)O.data[" + ks + "])[row] = (uint8_t)x;\n"; break;
+            case 2: o += "                ((uint16_t*)O.data[" + ks + "])[row] = (uint16_t)x;\n"; break;
+            case 4: o += "                ((uint32_t*)O.data[" + ks + "])[row] = (uint32_t)x;\n"; break;
+            case 8: o += "                ((uint64_t*)O.data[" + ks + "])[row] = (uint64_t)x;\n"; break;
+            default:
+                o += "                uint64_t* p = (uint64_t*)(O.data[" + ks + "] + 16 * row);\n";
+                o += "                p[0] = (uint64_t)x;\n                p[1] = (uint64_t)(x >> 64);\n";
+            }
+            o += "            }\n";
+        }
+        if (t.Nullable) {
+            o += "            {\n                const uint64_t bits = __ballot(eval && v.valid);\n";
+            o += "                if (lane == 0) store_bits(O.mask[" + ks + "], base, n, bits, evaluated);\n            }\n";
+        }
+        o += "        }\n";
+    }
+    o += "    }\n}\n";
+    return o;
+}
+
 }  // namespace qdbgpu
 
 namespace qdbgpu {

@@ -542,4 +542,28 @@ __device__ __forceinline__ double limbs_to_f64(const int64_t (&limb)[kLimbs])
     return neg ? -r : r;
 }
 
+// Output columns of a project launch.
+struct ProjectOut {
+    char* data[QDB_PROJECT_MAX_OUTPUTS];
+    uint8_t* mask[QDB_PROJECT_MAX_OUTPUTS];
+    int32_t kind[QDB_PROJECT_MAX_OUTPUTS];
+    int32_t width[QDB_PROJECT_MAX_OUTPUTS];
+};
+
+// Writes 64 bits of a bitmap for the 64 rows starting at `base` (a multiple
+// of 64); the last word writes only the bytes inside the buffer.
+__device__ __forceinline__ void store_bits(uint8_t* bitmap, int64_t base, int64_t n, uint64_t bits, uint64_t covered)
+{
+    const int64_t first_byte = base >> 3;
+    const int64_t rows = n - base < 64 ? n - base : 64;
+    const int nbytes = (int)((rows + 7) >> 3);
+    for (int b = 0; b < nbytes; ++b) {
+        const uint8_t cov = (uint8_t)(covered >> (8 * b));
+        if (cov == 0) continue;
+        const uint8_t val = (uint8_t)(bits >> (8 * b));
+        if (cov == 0xff) bitmap[first_byte + b] = val;
+        else bitmap[first_byte + b] = (uint8_t)((bitmap[first_byte + b] & ~cov) | (val & cov));
+    }
+}
+
 }  // namespace qdbgpu

@@ -27,6 +27,7 @@ namespace qdbgpu {
 // The filter kernel's source for one predicate (generated beside the
 // aggregate kernels, aggregate.hip).
 std::string filter_jit_source(const DevProgram& prog, const QdbValueType* slot_type);
+std::string project_jit_source(const DevProgram* progs, int nout, const QdbValueType* slot_type, const QdbValueType* out_types);
 
 namespace {
 
@@ -41,10 +42,12 @@ struct ExprPlan {
     DevProgram* d_progs = nullptr;
     std::vector<DevProgram> h_progs;
     QdbValueType out_types[QDB_PROJECT_MAX_OUTPUTS];
-    // A filter's kernel generated for its predicate (made at its first call;
-    // null: the generic interpreter kernel runs).
+    // The kernel generated for a filter's predicate or a projection's
+    // outputs (made at its first call; null: the generic interpreter kernel
+    // runs).
     std::once_flag jit_once;
     hipFunction_t jit_filter = nullptr;
+    hipFunction_t jit_project = nullptr;
     ~ExprPlan() { dfree(d_progs); }
 };
 
@@ -122,29 +125,6 @@ __global__ void __launch_bounds__(kBlock) k_filter(DevCols cols, const DevProgra
             keep = (v.valid && v.i != 0) ? 1 : 0;
         }
         out_sel[i] = keep;
-    }
-}
-
-struct ProjectOut {
-    char* data[QDB_PROJECT_MAX_OUTPUTS];
-    uint8_t* mask[QDB_PROJECT_MAX_OUTPUTS];
-    int32_t kind[QDB_PROJECT_MAX_OUTPUTS];
-    int32_t width[QDB_PROJECT_MAX_OUTPUTS];
-};
-
-// Writes 64 bits of a bitmap for the 64 rows starting at `base` (a multiple
-// of 64); the last word writes only the bytes inside the buffer.
-__device__ __forceinline__ void store_bits(uint8_t* bitmap, int64_t base, int64_t n, uint64_t bits, uint64_t covered)
-{
-    const int64_t first_byte = base >> 3;
-    const int64_t rows = n - base < 64 ? n - base : 64;
-    const int nbytes = (int)((rows + 7) >> 3);
-    for (int b = 0; b < nbytes; ++b) {
-        const uint8_t cov = (uint8_t)(covered >> (8 * b));
-        if (cov == 0) continue;
-        const uint8_t val = (uint8_t)(bits >> (8 * b));
-        if (cov == 0xff) bitmap[first_byte + b] = val;
-        else bitmap[first_byte + b] = (uint8_t)((bitmap[first_byte + b] & ~cov) | (val & cov));
     }
 }
 
@@ -299,10 +279,27 @@ void project_impl(ExprPlan* P, TRowSet* in, void** out, const uint8_t* input_sel
         O.kind[k] = P->out_types[k].Kind;
         O.width[k] = P->out_types[k].Width;
     }
+    std::call_once(P->jit_once, [P]() {
+        if (!jit_enabled() || P->nprogs < 1) return;
+        std::string err;
+        P->jit_project = jit_function(project_jit_source(P->h_progs.data(), P->nprogs, P->slot_type, P->out_types),
+            "qdb_project", &err);
+    });
     {
         TimedRegion timer("project");
-        hipLaunchKernelGGL(k_project, dim3(grid_for(in->RowCount, kBlock, 8)), dim3(kBlock), 0, stream(), cols, P->d_progs,
-            P->nprogs, O, in->RowCount, input_sel);
+        const int grid = grid_for(in->RowCount, kBlock, 8);
+        if (P->jit_project) {
+            int64_t n = in->RowCount;
+            const uint8_t* in_sel = input_sel;
+            void* args[] = {&cols, &O, &n, &in_sel};
+            if (!hip_ok(hipModuleLaunchKernel(P->jit_project, (unsigned)grid, 1, 1, kBlock, 1, 1, 0, stream(), args, nullptr),
+                    "project launch")) {
+                return;
+            }
+        } else {
+            hipLaunchKernelGGL(k_project, dim3(grid), dim3(kBlock), 0, stream(), cols, P->d_progs, P->nprogs, O, in->RowCount,
+                input_sel);
+        }
     }
     // Stream-ordered, as the filter.
     (void)hip_ok(hipGetLastError(), "project launch");
@@ -415,6 +412,41 @@ extern "C" int64_t qdb_gpu_filter_kernel_source(const QdbFilterSpec* spec, char*
     return (int64_t)src.size();
 }
 
+namespace {
+
+// Checks a project spec's output types and records them in the plan.
+bool set_project_outputs(const QdbProjectSpec* spec, ExprPlan* P)
+{
+    if (spec->NumOutputs < 1 || spec->NumOutputs > QDB_PROJECT_MAX_OUTPUTS) return fail("bad output count %d", spec->NumOutputs);
+    for (int k = 0; k < spec->NumOutputs; ++k) {
+        P->out_types[k] = spec->OutputTypes[k];
+        const QdbValueType& t = spec->OutputTypes[k];
+        const bool ok_width = t.Kind == QDB_KIND_BOOL || t.Width == 1 || t.Width == 2 || t.Width == 4 || t.Width == 8 || t.Width == 16;
+        if (!ok_width || t.Kind == QDB_KIND_STRING) return fail("project output %d: unsupported type", k);
+    }
+    return true;
+}
+
+}  // namespace
+
+// The source of the kernel generated for a projection (no GPU needed), as
+// qdb_gpu_filter_kernel_source.
+extern "C" int64_t qdb_gpu_project_kernel_source(const QdbProjectSpec* spec, char* out, int64_t cap)
+{
+    clear_error();
+    if (!spec) return fail("null argument"), -1;
+    ExprPlan P;
+    if (!set_project_outputs(spec, &P)) return -1;
+    if (!compile_programs(spec->Input, spec->Outputs, spec->NumOutputs, &P, false)) return -1;
+    const std::string src = project_jit_source(P.h_progs.data(), P.nprogs, P.slot_type, P.out_types);
+    if (out && cap > 0) {
+        const size_t n = src.size() < (size_t)cap - 1 ? src.size() : (size_t)cap - 1;
+        memcpy(out, src.data(), n);
+        out[n] = '\0';
+    }
+    return (int64_t)src.size();
+}
+
 extern "C" int qdb_gpu_compile_project(const QdbProjectSpec* spec, QdbProjectFn* out)
 {
     clear_error();
@@ -422,14 +454,9 @@ extern "C" int qdb_gpu_compile_project(const QdbProjectSpec* spec, QdbProjectFn*
     if (spec->NumOutputs < 1 || spec->NumOutputs > QDB_PROJECT_MAX_OUTPUTS) return fail("bad output count %d", spec->NumOutputs), -1;
     if (!ensure_init()) return -1;
     ExprPlan* P = new ExprPlan();
-    for (int k = 0; k < spec->NumOutputs; ++k) {
-        P->out_types[k] = spec->OutputTypes[k];
-        const QdbValueType& t = spec->OutputTypes[k];
-        const bool ok_width = t.Kind == QDB_KIND_BOOL || t.Width == 1 || t.Width == 2 || t.Width == 4 || t.Width == 8 || t.Width == 16;
-        if (!ok_width || t.Kind == QDB_KIND_STRING) {
-            delete P;
-            return fail("project output %d: unsupported type", k), -1;
-        }
+    if (!set_project_outputs(spec, P)) {
+        delete P;
+        return -1;
     }
     if (!compile_programs(spec->Input, spec->Outputs, spec->NumOutputs, P)) {
         delete P;

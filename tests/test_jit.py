@@ -122,3 +122,78 @@ def test_generated_filter_kernels_compile():
         err = C.create_string_buffer(1 << 16)
         rc = lib.qdb_gpu_jit_check(src, b"gfx950", err, len(err))
         assert rc == 0, f"{name}: {err.value.decode(errors='replace')[:3000]}"
+
+
+class ProjectCapture(OracleBackend):
+    """The oracle, recording the project specs it is asked to run."""
+
+    def __init__(self):
+        super().__init__()
+        self.specs = []
+
+    def project(self, types, outputs, batch):
+        self.specs.append(_project_spec(types, outputs))
+        return super().project(types, outputs, batch)
+
+
+def _project_spec(types, outputs):
+    abi = cases._abi()
+    spec = abi.QdbProjectSpec()
+    spec.Input = cases.schema_c(types)
+    spec.NumOutputs = len(outputs)
+    for k, (p, r, t) in enumerate(outputs):
+        spec.OutputTypes[k] = cases.value_type_c(t)
+        spec.Outputs[k] = p.build(r)
+    return spec
+
+
+def _project_specs():
+    """The golden projections, the lineitem products, and outputs of every
+    store form: i128 decimal, f64, f32, a BOOL bitmap, a 2-byte integer."""
+    import qumirdb_amd as q
+    from qumirdb_amd import workloads as W
+    from qumirdb_amd.host import schema_c
+    from helpers import DECIMAL, FLOAT, SIGNED, T
+    cap = ProjectCapture()
+    for name in sorted(cases.PROJECT_GOLDENS):
+        cases.PROJECT_GOLDENS[name](cap)
+    out = [("golden", q.abi.QdbProjectSpec.from_buffer_copy(s)) for s in cap.specs]
+    spec = q.abi.QdbProjectSpec()
+    spec.Input = schema_c(W.LINEITEM_TYPES)
+    spec.NumOutputs = 2
+    for k, (prog, res) in enumerate([W.lineitem_disc_price(), W.lineitem_charge()]):
+        spec.OutputTypes[k] = prog.result_type(res).to_c()
+        spec.Outputs[k] = prog.build(res)
+    out.append(("lineitem", spec))
+    E = cases._abi()
+    types = [T(SIGNED, 8, nullable=True), T(DECIMAL, 8, 2, True, 15), T(FLOAT, 8, nullable=True), T(SIGNED, 2)]
+    q1 = cases.Prog(types)
+    x = q1.op(E.EXPR_MUL, q1.col(1), q1.const(100, T(DECIMAL, 8, 2, False, 3)), T(DECIMAL, 16, 4, True, 31))
+    q2 = cases.Prog(types)
+    y = q2.op(E.EXPR_ADD, q2.col(2), q2.const_f64(0.25), T(FLOAT, 8, nullable=True), cls=E.REG_F64)
+    q3 = cases.Prog(types)
+    z = q3.cmp(E.CMP_NE, q3.col(3), q3.const(0, T(SIGNED, 8)))
+    q4 = cases.Prog(types)
+    w = q4.col(3)
+    outs = [(q1, x, T(DECIMAL, 16, 4, True, 31)), (q2, y, T(FLOAT, 8, nullable=True)), (q2, y, T(FLOAT, 4, nullable=True)),
+            (q3, z, T(3, 1)), (q4, w, T(SIGNED, 2))]
+    out.append(("store forms", q.abi.QdbProjectSpec.from_buffer_copy(_project_spec(types, outs))))
+    return out
+
+
+def test_generated_project_kernels_compile():
+    """The kernel generated for a projection: each output's program as
+    straight-line code with its store specialized to the output type, one
+    kernel qdb_project, compiled for gfx950 on the host."""
+    import qumirdb_amd as q
+    lib = q.lib()
+    for name, spec in _project_specs():
+        n = lib.qdb_gpu_project_kernel_source(C.byref(spec), None, 0)
+        assert n > 0, q.last_error()
+        buf = C.create_string_buffer(n + 1)
+        assert lib.qdb_gpu_project_kernel_source(C.byref(spec), buf, n + 1) == n
+        src = buf.value
+        assert b"qdb_project" in src and b"eval_program" not in src
+        err = C.create_string_buffer(1 << 16)
+        rc = lib.qdb_gpu_jit_check(src, b"gfx950", err, len(err))
+        assert rc == 0, f"{name}: {err.value.decode(errors='replace')[:3000]}"
