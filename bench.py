@@ -8,9 +8,11 @@ DECIMAL(15,2) value); outputs COUNT(*), exact SUM(value) and the matched
 already resident in HBM: build filter -> append -> table build -> probe
 (pairs + fused COUNT/SUM). `value` is (build rows + probe rows) / second.
 
-The other two shapes are measured in the same run (unless --workload join):
+The other shapes are measured in the same run (unless --workload join):
 the filtered low-cardinality group-by over 6M lineitem rows (fused filter +
-project + aggregate) and the high-cardinality Zipf group-by over 100M rows.
+project + aggregate), the lineitem projection alone over 60M rows, the
+high-cardinality Zipf group-by over 100M rows, and the batched and served
+forms.
 
 roofline: algorithmic bytes of the dominant kernel (the probe: 16 B per
 probe row read + 16 B per pair written) over its average duration, timed
@@ -37,7 +39,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["join", "lineitem", "groups", "batched", "served", "all"], default="all")
+    ap.add_argument("--workload", choices=["join", "lineitem", "project", "groups", "batched", "served", "all"],
+                    default="all")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (3 launches), the known-size kernel that "
@@ -148,6 +151,45 @@ def bench_lineitem(steps, warmup, rows=6_000_000):
     return {"rows": rows, "ms_per_step": ms, "rows_per_s": rows / (ms * 1e-3), "kernel_ms": k_ms,
             "algorithmic_bytes": nbytes, "achieved_gbs": nbytes / (k_ms * 1e-3) / 1e9 if k_ms else 0.0,
             "groups": len(out["rows"])}
+
+
+def bench_project(steps, warmup, rows=60_000_000):
+    """The lineitem projection price * (1 - discount) and price * (1 -
+    discount) * (1 + tax) (two i128 DECIMAL outputs) over rows in HBM, into
+    output columns allocated once; 56 algorithmic bytes per row (three
+    8-byte inputs, two 16-byte outputs)."""
+    import ctypes
+    from qumirdb_amd import lib, workloads as W
+    from qumirdb_amd.host import ProjectKernel
+
+    dev = W.LineitemDevice(rows)
+    kern = ProjectKernel(W.LINEITEM_TYPES, [W.lineitem_disc_price(), W.lineitem_charge()])
+    bufs, ptrs = kern(dev.rowset)
+
+    def step():
+        kern.fn(ctypes.byref(dev.rowset.rs), ptrs, None, None, None)
+
+    for _ in range(warmup):
+        step()
+    lib().qdb_gpu_synchronize()
+    lib().qdb_gpu_timing_reset()
+    lib().qdb_gpu_timing_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    lib().qdb_gpu_synchronize()
+    t1 = time.perf_counter()
+    lib().qdb_gpu_timing_enable(0)
+    err = lib().qdb_gpu_last_error()
+    if err:
+        raise RuntimeError(f"project: {err.decode()}")
+    k_ms = timing_per_step("project", steps)
+    lib().qdb_gpu_timing_reset()
+    del bufs
+    ms = (t1 - t0) * 1e3 / steps
+    nbytes = rows * (8 * 3 + 16 * 2)
+    return {"rows": rows, "ms_per_step": ms, "rows_per_s": rows / (ms * 1e-3), "kernel_ms": k_ms,
+            "algorithmic_bytes": nbytes, "achieved_gbs": nbytes / (k_ms * 1e-3) / 1e9 if k_ms else 0.0}
 
 
 def bench_groups(steps, warmup, rows=100_000_000, distinct=10_000_000):
@@ -380,6 +422,9 @@ def main():
     if args.workload in ("lineitem", "all"):
         out["lineitem"] = bench_lineitem(args.steps, args.warmup)
         release()
+    if args.workload in ("project", "all"):
+        out["project"] = bench_project(args.steps, args.warmup)
+        release()
     if args.workload in ("groups", "all"):
         # Five untimed queries at least: the first compiles the generated
         # kernels, and the next few still run up to ~1.6x the steady-state
@@ -393,7 +438,7 @@ def main():
         out["served"] = bench_served()
     if args.copy_calibration:
         out["copy_calibration"] = copy_calibration(args.copy_calibration)
-    head = out.get("join") or out.get("lineitem") or out.get("groups") or out.get("batched")
+    head = out.get("join") or out.get("lineitem") or out.get("project") or out.get("groups") or out.get("batched")
     if head is None:  # served only: its best sustained throughput
         best = max(out["served"]["levels"], key=lambda l: l["batched"]["qps"])["batched"]
         head = {"rows_per_s": best["qps"], "ms_per_step": best["p50_ms"]}
