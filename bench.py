@@ -410,7 +410,7 @@ def main():
     from qumirdb_amd import lib as _lib
 
     def release():
-        """Between legs: the previous leg's buffers back to the driver."""
+        """Between legs: the previous leg's buffers back to the HIP runtime."""
         gc.collect()
         _lib().qdb_gpu_trim_cache()
 

@@ -35,7 +35,7 @@ bool hip_ok(hipError_t e, const char* what);
 void* dalloc(size_t bytes);
 void dfree(void* p);
 size_t dalloc_size(const void* p);
-// Returns every cached device block to the driver.
+// Returns every cached device block to the HIP runtime (hipFree).
 void trim_device_cache();
 // Pinned host staging memory (hipHostMalloc), also cached.
 void* halloc_pinned(size_t bytes);

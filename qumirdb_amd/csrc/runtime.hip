@@ -186,7 +186,7 @@ static void evict_oldest_locked(Runtime& r)
     r.free_dev.erase(oldest);
 }
 
-// Returns the blocks cached longest to the driver until the device has
+// Returns the blocks cached longest to the HIP runtime until the device has
 // `need` bytes free beyond the reserve (or nothing is cached).
 static void make_room_locked(Runtime& r, size_t need)
 {
@@ -238,7 +238,7 @@ void* dalloc(size_t bytes)
     }
     hipError_t e = hipMalloc(&p, c);
     if (e != hipSuccess) {
-        // Return cached blocks to the driver and retry once.
+        // Return cached blocks to the HIP runtime and retry once.
         std::lock_guard<std::mutex> lock(r.mu);
         release_cached_locked(r);
         (void)hipGetLastError();
@@ -266,7 +266,7 @@ void dfree(void* p)
     r.cached += sz;
     // Freed blocks stay cached while the device keeps the reserve free (the
     // HIP runtime and other allocators need memory of their own); past
-    // that, the blocks cached longest go back to the driver first, so the
+    // that, the blocks cached longest go back to the HIP runtime first, so the
     // pool follows the sizes the current work asks for. hipFree waits for
     // the device, so a block still in use by queued work is safe to
     // release. Below 1/8 of the device cached, free memory is not checked.
