@@ -104,6 +104,11 @@ typedef struct QdbValueType {
     int32_t Width;
     int32_t Scale;
     bool Nullable;
+    /* DECIMAL precision (digits); 0 = the storage's widest: 18 for Width 8,
+     * 38 for Width 16. A project output of DECIMAL(p <= 18) whose value has
+     * more than p digits is a row error (qdb_decimal64_narrow_checked,
+     * qdb/modules/qumirdb.oz:1236-1245). */
+    uint8_t Precision;
 } QdbValueType;
 
 /* ------------------------------------------------------------------------ */
@@ -168,7 +173,12 @@ enum {
     QDB_EXPR_OR = 11,
     QDB_EXPR_NOT = 12,
     QDB_EXPR_IS_NULL = 13, /* r[dst] = r[a] IS NULL (never NULL)                   */
-    QDB_EXPR_TO_F64 = 14   /* r[dst] = decimal r[a] at scale imm_lo as nearest f64 */
+    QDB_EXPR_TO_F64 = 14,  /* r[dst] = decimal r[a] at scale imm_lo as nearest f64,
+                            * ties to even (qdb_decimal_to_f64, qdb/modules/
+                            * qumirdb.oz:983-1046); an f64 operand is kept */
+    QDB_EXPR_DIV = 15      /* r[dst] = r[a] / r[b], IEEE f64 division (class F64
+                            * only): AVG = SUM / COUNT of the lowering,
+                            * qdb/plan/sql_lowering.cpp:799-814 */
 };
 
 enum { QDB_CMP_EQ = 0, QDB_CMP_NE = 1, QDB_CMP_LT = 2, QDB_CMP_LE = 3, QDB_CMP_GT = 4, QDB_CMP_GE = 5 };
@@ -251,6 +261,7 @@ enum {
 
 #define QDB_AGG_MAX_KEYS 4
 #define QDB_AGG_MAX_AGGS 16
+#define QDB_AGG_MAX_EMPTY_INPUT_GROUPS 4
 
 typedef struct QdbAggregateFn {
     int32_t Func;       /* QDB_AGG_* */
@@ -263,9 +274,19 @@ typedef struct QdbAggregateSpec {
     int32_t KeyColumns[QDB_AGG_MAX_KEYS];
     int32_t NumAggs;
     QdbAggregateFn Aggs[QDB_AGG_MAX_AGGS];
-    /* 1 = a global aggregate (no keys): one group that exists over an empty
-     * input, COUNT 0 and SUM/MIN/MAX NULL (qdb/kernel/spec.h:85-89). */
-    int32_t Global;
+    /* Groups that exist over an empty input (TOperatorKernelSpec::
+     * EmptyInputGroups, qdb/kernel/spec.h:85-89; seeded by the generated
+     * Init, qdb/kernel/gen.cpp:2218-2269): Init creates one group per entry
+     * whose leading key column (a non-nullable integer) holds the entry and
+     * whose other key columns are NULL (0 when not nullable), with every
+     * reducer at its identity (COUNT 0, SUM/MIN/MAX NULL), so the table's
+     * Size counts them from Init on. A SQL global aggregate groups by the
+     * constant i64 column `__group__` = 1 and seeds {1}
+     * (qdb/plan/build.cpp:2461-2524); its SUM/MIN/MAX results are nullable
+     * (qdb/plan/ops/aggregate.cpp:93-97), as they are here whenever a group
+     * is seeded. */
+    int32_t NumEmptyInputGroups;
+    int64_t EmptyInputGroups[QDB_AGG_MAX_EMPTY_INPUT_GROUPS];
     /* Output buffers of agg_finish_rowset: 0 = device memory, 1 = host. */
     int32_t OutputOnHost;
     /* Optional fused stages, each equal to the separate operator: a
@@ -378,6 +399,21 @@ int qdb_gpu_compile_row_join(const QdbRowJoinSpec* spec, QdbRowJoinKernels* out)
  * row into `hashes` (device), qdb/kernel/compiler.cpp:3057-3125. */
 typedef bool (*QdbGroupHashFn)(TRowSet* rs, uint64_t* hashes, const int64_t* key_cols);
 int qdb_gpu_compile_group_hash(const QdbColumnSchema* input, int32_t num_keys, QdbGroupHashFn* out);
+/* Row errors of the project kernel: an output of DECIMAL(p <= 18) whose
+ * value has more than p digits (qdb_decimal64_narrow_checked,
+ * qdb/modules/qumirdb.oz:1236-1245). As in the reference, the failing value
+ * is stored as 0 and the row error goes to the runtime's overflow routine,
+ * qdb_decimal64_overflow(p) (qdb/modules/qumirdb_runtime.cpp:662-667), which
+ * raises or, while the caller defers row errors, returns 0
+ * (qdb/exec/project_exec.cpp:71-81: every row evaluated deferred, then the
+ * selected rows raising). Here the project entry point calls the registered
+ * handler once per call, with the precision of the lowest-numbered failing
+ * output, after its kernel; qdb_gpu_take_row_error returns that precision
+ * (0: no row failed) and clears it, and the error text is
+ * "value does not fit DECIMAL precision p". */
+typedef int64_t (*QdbRowErrorFn)(int64_t precision);
+void qdb_gpu_set_row_error_handler(QdbRowErrorFn fn);
+int64_t qdb_gpu_take_row_error(void);
 /* Frees the slot of a compiled kernel (its Slot field / the fn pointer). */
 void qdb_gpu_release_slot(int32_t kind, int32_t slot);
 enum { QDB_SLOT_FILTER = 0, QDB_SLOT_PROJECT = 1, QDB_SLOT_AGGREGATE = 2, QDB_SLOT_JOIN = 3, QDB_SLOT_HASH = 4 };
@@ -410,13 +446,40 @@ int qdb_gpu_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int qdb_gpu_memset(void* dst, int value, size_t bytes);
 int qdb_gpu_synchronize(void);
 
+/* The host heap of the reference's kernels: qdb_alloc / qdb_free
+ * (qdb/modules/qumirdb_runtime.h:52-53), and TKernelHeap::Detach
+ * (qdb/kernel/execution_context.h:39), which moves a block out of the task's
+ * heap when an output is adopted (qdb/exec/kernel_rowset.h:30-38). Every
+ * output rowset of agg_finish_rowset and jr_materialize is kernel-owned as
+ * the reference's are: TColumns and host buffers come from `alloc`, Private
+ * is the owners list (owners[0] = count, then the blocks; the list is an
+ * `alloc` block too) and Destroy is NULL, for the host to adopt
+ * (AdoptKernelOwnedRowSet) and release (DestroyKernelOwnedRowSet frees every
+ * owner with `free`). An output left in device memory (OutputOnHost = 0) has
+ * one more owner, a host block listing its device buffers, which only
+ * qdb_gpu_destroy_kernel_rowset returns to the device pool: such outputs
+ * feed device-side consumers, which release them with that call. Without a
+ * registered heap, malloc / free are used and detach is a no-op. */
+typedef void* (*QdbHeapAllocFn)(int64_t size);
+typedef void (*QdbHeapFreeFn)(void* ptr);
+typedef void (*QdbHeapDetachFn)(void* ptr);
+void qdb_gpu_set_host_heap(QdbHeapAllocFn alloc, QdbHeapFreeFn free_fn, QdbHeapDetachFn detach);
+/* DestroyKernelOwnedRowSet for this library's outputs (device buffers
+ * included); callable as a TRowSet Destroy callback. */
+void qdb_gpu_destroy_kernel_rowset(TRowSet* rs);
+/* AdoptKernelOwnedRowSet (qdb/exec/kernel_rowset.h:30-38) for callers without
+ * the reference's: detaches every owner (and the list) through the heap's
+ * detach and installs qdb_gpu_destroy_kernel_rowset as Destroy. */
+void qdb_gpu_adopt_kernel_rowset(TRowSet* rs);
+
 /* Staging (qdb/io; decimal storage qdb/io/parquet/decimal_storage.cpp:220-247): copies a host TRowSet's fixed-width
- * columns, masks and selection into HBM through pinned buffers with
- * hipMemcpyAsync, producing a device TRowSet whose Destroy frees it.
- * `types` gives each column's width. Returns 0 on success. */
+ * columns, masks, selection and hashes into HBM through one pinned buffer
+ * with hipMemcpyAsync (one synchronization), producing a device TRowSet
+ * whose Destroy (qdb_gpu_destroy_kernel_rowset) frees it. `types` gives each
+ * column's width. Returns 0 on success. */
 int qdb_gpu_stage_rowset(const TRowSet* host, const QdbColumnSchema* types, TRowSet* device_out);
 /* Copies a device TRowSet (as produced by agg_finish_rowset/jr_materialize)
- * to host memory; the result's Destroy frees the host buffers. */
+ * to host memory (the host heap); the result's Destroy frees it. */
 int qdb_gpu_download_rowset(const TRowSet* device, const QdbValueType* types, TRowSet* host_out);
 
 /* Synthetic inputs generated in HBM from a seed (counter-based, identical to
@@ -483,11 +546,15 @@ int qdb_gpu_jit_check(const char* src, const char* arch, char* err, int64_t cap)
 int64_t qdb_gpu_jit_stats(double* total_ms);
 
 /* ------------------------------------------------------------------------ */
-/* Host processors over the entry points above, mirroring qdb/exec:         */
-/* TAggregateProcessor (qdb/exec/aggregate_exec.cpp:52-156) and             */
-/* TRowJoinProcessor (qdb/exec/row_join_exec.cpp:22-276). Batches passed in */
-/* stay owned by the caller. Errors: -1 / NULL, text in                      */
-/* qdb_gpu_exec_last_error.                                                  */
+/* Host drivers over the entry points above, for callers without the       */
+/* reference's processors: they drive the same op-code protocol and check   */
+/* the same invariants as TAggregateProcessor (qdb/exec/aggregate_exec.cpp: */
+/* 67-136: Init once, Update < 0 fails, finalized rows == Size read before  */
+/* the finish, the output adopted) and TRowJoinProcessor                    */
+/* (qdb/exec/row_join_exec.cpp:167-190: pairs materialized before the probe */
+/* batch is released, pair count reset per drain). Batches passed in stay   */
+/* owned by the caller; outputs are adopted (Destroy set). Errors: -1 / NULL,*/
+/* text in qdb_gpu_exec_last_error.                                          */
 /* ------------------------------------------------------------------------ */
 
 const char* qdb_gpu_exec_last_error(void);
@@ -509,7 +576,8 @@ void qdb_gpu_agg_processor_free(void* processor);
 void* qdb_gpu_join_processor_new(const QdbRowJoinKernels* kernels, int build_is_left, int64_t output_batch_rows);
 int qdb_gpu_join_processor_append(void* processor, const TRowSet* batch);
 int qdb_gpu_join_processor_build(void* processor);
-/* Probes and materializes the batch's matches into queued output batches. */
+/* Probes and materializes the batch's matches into queued output batches;
+ * returns the pair count. */
 int64_t qdb_gpu_join_processor_probe(void* processor, const TRowSet* batch);
 /* Probes only; returns the pair count, pairs readable until the next probe. */
 int64_t qdb_gpu_join_processor_probe_only(void* processor, const TRowSet* batch);
@@ -519,7 +587,7 @@ void* qdb_gpu_join_processor_arena(void* processor);
 /* Next queued output batch: 1 when one was returned, 0 when none is left. */
 int qdb_gpu_join_processor_next(void* processor, TRowSet* out);
 void qdb_gpu_join_processor_free(void* processor);
-/* Calls rs->Destroy when set. */
+/* Calls rs->Destroy when set, else qdb_gpu_destroy_kernel_rowset. */
 void qdb_gpu_rowset_release(TRowSet* rs);
 
 #ifdef __cplusplus

@@ -281,6 +281,16 @@ static void eval_program(const QdbExprProgram* p, const QdbColumnSchema* schema,
             out.valid = 1;
             out.i = !a.valid;
             break;
+        case QDB_EXPR_DIV:
+            /* IEEE f64 division: AVG = SUM / COUNT (qdb/plan/sql_lowering.cpp:799-814). */
+            out.cls = QDB_REG_F64;
+            out.valid = a.valid && b.valid;
+            if (out.valid) {
+                double x = a.cls == QDB_REG_F64 ? a.f : (double)a.i;
+                double y = b.cls == QDB_REG_F64 ? b.f : (double)b.i;
+                out.f = x / y;
+            }
+            break;
         case QDB_EXPR_TO_F64:
             out.cls = QDB_REG_F64;
             out.valid = a.valid;
@@ -317,6 +327,18 @@ static void set_bit(uint8_t* bitmap, int64_t i, int v)
     else bitmap[i >> 3] &= (uint8_t)~(1u << (i & 7));
 }
 
+/* The row-error check of a DECIMAL(p <= 18) store: qdb_decimal64_narrow_checked
+ * (qdb/modules/qumirdb.oz:1236-1245) calls qdb_decimal64_overflow(p) when
+ * |value| >= 10^p, which raises, or returns 0 while row errors are deferred
+ * (qdb/modules/qumirdb_runtime.cpp:662-667). Returns p on failure, else 0. */
+static int narrow_fails(const QdbValueType* t, const Reg* v)
+{
+    if (t->Kind != QDB_KIND_DECIMAL || t->Width != 8 || !v->valid || v->cls == QDB_REG_F64) return 0;
+    const int p = t->Precision ? t->Precision : 18;
+    const i128 lim = pow10_i128(p);
+    return (v->i >= lim || v->i <= -lim) ? p : 0;
+}
+
 static void store_value(void* data, const QdbValueType* t, int64_t row, const Reg* v)
 {
     if (t->Kind == QDB_KIND_BOOL) {
@@ -339,22 +361,31 @@ static void store_value(void* data, const QdbValueType* t, int64_t row, const Re
     }
 }
 
-/* Project: out[k] data, out[NumOutputs + k] validity of nullable outputs. */
-void oracle_project(const QdbProjectSpec* spec, const TRowSet* in, void** out, const uint8_t* input_sel)
+/* Project: out[k] data, out[NumOutputs + k] validity of nullable outputs.
+ * Returns 0, or the precision of the first row error (the failing value is
+ * stored as 0, as under deferral). */
+int oracle_project(const QdbProjectSpec* spec, const TRowSet* in, void** out, const uint8_t* input_sel)
 {
     Reg r[QDB_EXPR_MAX_REGS];
+    int failed = 0;
     for (int64_t i = 0; i < in->RowCount; ++i) {
         if (input_sel && !input_sel[i]) continue;
         for (int k = 0; k < spec->NumOutputs; ++k) {
             memset(r, 0, sizeof(r));
             eval_program(&spec->Outputs[k], &spec->Input, in->Columns, in->ColumnCount, i, r);
-            const Reg* v = &r[spec->Outputs[k].Result];
+            Reg* v = &r[spec->Outputs[k].Result];
+            const int p = narrow_fails(&spec->OutputTypes[k], v);
+            if (p) {
+                if (!failed) failed = p;
+                v->i = 0;
+            }
             store_value(out[k], &spec->OutputTypes[k], i, v);
             if (spec->OutputTypes[k].Nullable) {
                 set_bit((uint8_t*)out[spec->NumOutputs + k], i, v->valid);
             }
         }
     }
+    return failed;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -414,17 +445,27 @@ static uint64_t leaf_hash(const TColumn* c, const QdbValueType* t, int64_t row)
     return oracle_mix_u64(bits);
 }
 
-/* rh_hash of a key: a scalar non-nullable key hashes its leaf; otherwise the
- * boost-style ordered combine over fields with NULL fields hashed as 0. */
+/* rh_hash of a key (qdb/kernel/gen.cpp:344-373 over the key struct built by
+ * BuildAggregateKeyDescriptor, qdb/kernel/aggregate_key.cpp:246-272). The key
+ * is always a struct, even with one field: for each key column a nullable
+ * column first contributes its `valid_i` bool field (hashed as a bool leaf,
+ * valid * 2685821657736338717, gen.cpp:319-325), then its `key_i` field
+ * (the leaf hash, or 0 when the row's value is NULL, gen.cpp:357-364); each
+ * field hash enters the boost-style ordered combine
+ * h ^= fh + 0x9E3779B97F4A7C15 + (h << 6) + (h >> 2), starting from h = 0. */
 uint64_t oracle_key_hash(const TColumn* cols, const QdbValueType* types, const int32_t* key_cols, int nkeys, int64_t row)
 {
-    if (nkeys == 1 && !types[key_cols[0]].Nullable) {
-        return leaf_hash(&cols[key_cols[0]], &types[key_cols[0]], row);
-    }
     uint64_t h = 0;
     for (int k = 0; k < nkeys; ++k) {
         const TColumn* c = &cols[key_cols[k]];
-        uint64_t fh = col_valid(c, row) ? leaf_hash(c, &types[key_cols[k]], row) : 0;
+        const QdbValueType* t = &types[key_cols[k]];
+        int valid = 1;
+        if (t->Nullable) {
+            valid = col_valid(c, row);
+            const uint64_t vh = valid ? 2685821657736338717ull : 0ull;
+            h ^= vh + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+        }
+        const uint64_t fh = valid ? leaf_hash(c, t, row) : 0;
         h ^= fh + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
     }
     return h;
@@ -761,6 +802,36 @@ OracleAgg* oracle_agg_new(const QdbAggregateSpec* spec, int64_t capacity, int fs
     int64_t c = 8;
     while (c < capacity) c *= 2;
     swiss_alloc(ag, c);
+    /* Empty-input groups (qdb/kernel/gen.cpp:2218-2269): the leading key
+     * holds the id, the other fields their zero value (NULL when nullable);
+     * upserted with rh_hash of that key, reducers untouched. */
+    for (int g = 0; g < spec->NumEmptyInputGroups; ++g) {
+        GroupKey key;
+        memset(&key, 0, sizeof(key));
+        int64_t vals[QDB_AGG_MAX_KEYS][2];
+        uint8_t masks[QDB_AGG_MAX_KEYS][8];
+        TColumn cols[64];
+        memset(vals, 0, sizeof(vals));
+        memset(masks, 0, sizeof(masks));
+        memset(cols, 0, sizeof(cols));
+        for (int k = 0; k < spec->NumKeys; ++k) {
+            const int kc = spec->KeyColumns[k];
+            const int nullable = ag->spec.Input.Types[kc].Nullable;
+            key.f[k].valid = (uint8_t)(k == 0 || !nullable);
+            key.f[k].v = k == 0 ? (i128)spec->EmptyInputGroups[g] : 0;
+            /* One-row columns of the seed key, for its hash. */
+            const int64_t v = k == 0 ? spec->EmptyInputGroups[g] : 0;
+            const int w = ag->spec.Input.Types[kc].Width;
+            memcpy(vals[k], &v, w < 8 ? (size_t)w : 8);
+            cols[kc].Data = (char*)vals[k];
+            if (nullable) {
+                masks[k][0] = key.f[k].valid;
+                cols[kc].Mask = masks[k];
+            }
+        }
+        const uint64_t h = oracle_key_hash(cols, ag->spec.Input.Types, spec->KeyColumns, spec->NumKeys, 0);
+        (void)swiss_upsert(ag, &key, h);
+    }
     return ag;
 }
 
@@ -947,7 +1018,7 @@ void oracle_agg_output_types(const QdbAggregateSpec* spec, QdbValueType* out)
             t.Width = 8;
         } else {
             const QdbValueType* in = spec_col_type(spec, fn->Column);
-            t.Nullable = in->Nullable || spec->Global;
+            t.Nullable = in->Nullable || spec->NumEmptyInputGroups > 0;
             if (in->Kind == QDB_KIND_FLOAT) {
                 t.Kind = QDB_KIND_FLOAT;
                 t.Width = 8;
@@ -986,13 +1057,6 @@ int64_t oracle_agg_finish(OracleAgg* ag, TRowSet* out)
 {
     QdbAggregateSpec full = ag->spec;
     const QdbAggregateSpec* spec = &full;
-    if (spec->Global && ag->size == 0) {
-        /* A global aggregate has one group over an empty input. */
-        GroupKey none;
-        memset(&none, 0, sizeof(none));
-        int64_t d = swiss_upsert(ag, &none, 0);
-        (void)d;
-    }
     const int64_t n = ag->size;
     const int ncols = spec->NumKeys + spec->NumAggs;
     QdbValueType types[QDB_AGG_MAX_KEYS + QDB_AGG_MAX_AGGS];

@@ -55,7 +55,7 @@ def _lib() -> C.CDLL:
         "oracle_exact_sum": (C.c_double, [P, i64]),
         "oracle_kahan_sum": (C.c_double, [P, i64]),
         "oracle_filter": (None, [C.POINTER(abi.QdbFilterSpec), C.POINTER(abi.TRowSet), P, P]),
-        "oracle_project": (None, [C.POINTER(abi.QdbProjectSpec), C.POINTER(abi.TRowSet), C.POINTER(P), P]),
+        "oracle_project": (C.c_int, [C.POINTER(abi.QdbProjectSpec), C.POINTER(abi.TRowSet), C.POINTER(P), P]),
         "oracle_key_hash": (C.c_uint64, [C.POINTER(abi.TColumn), C.POINTER(abi.QdbValueType), C.POINTER(C.c_int32), C.c_int, i64]),
         "oracle_mix_u64": (C.c_uint64, [C.c_uint64]),
         "oracle_agg_new": (P, [C.POINTER(abi.QdbAggregateSpec), i64, C.c_int]),
@@ -207,6 +207,10 @@ def filter_rows(spec, rowset: HostRowSet, input_sel=None) -> np.ndarray:
     return out
 
 
+class RowError(RuntimeError):
+    """A row of a project failed (qdb_decimal64_overflow)."""
+
+
 def project_rows(spec, rowset: HostRowSet, out_types, input_sel=None):
     n = max(rowset.rows, 1)
     k = len(out_types)
@@ -221,7 +225,12 @@ def project_rows(spec, rowset: HostRowSet, out_types, input_sel=None):
             bufs.append(m)
             ptrs[k + i] = m.ctypes.data
     sel = None if input_sel is None else np.ascontiguousarray(np.asarray(input_sel, dtype=np.uint8))
-    LIB.oracle_project(C.byref(spec), rowset.ptr, ptrs, sel.ctypes.data if sel is not None else None)
+    failed = LIB.oracle_project(C.byref(spec), rowset.ptr, ptrs, sel.ctypes.data if sel is not None else None)
+    if failed:
+        # The reference's project evaluates every row with row errors
+        # deferred, then the selected rows raising (qdb/exec/project_exec.cpp:
+        # 71-81): only a failing row that the selection keeps is an error.
+        raise RowError(f"value does not fit DECIMAL precision {failed}")
     out = []
     j = 0
     for i, t in enumerate(out_types):

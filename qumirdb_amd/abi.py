@@ -13,6 +13,7 @@ MAX_COLS = 64
 PROJECT_MAX_OUTPUTS = 8
 AGG_MAX_KEYS = 4
 AGG_MAX_AGGS = 16
+AGG_MAX_EMPTY_INPUT_GROUPS = 4
 JOIN_MAX_KEYS = 2
 JOIN_MAX_FUSED = 4
 
@@ -21,7 +22,7 @@ SIGNED, UNSIGNED, FLOAT, BOOL, STRING, DATE, DECIMAL = range(7)
 
 # Expression ops
 (EXPR_COLUMN, EXPR_CONST_INT, EXPR_CONST_F64, EXPR_ADD, EXPR_SUB, EXPR_MUL, EXPR_NEG, EXPR_SCALE_UP,
- EXPR_CMP, EXPR_AND, EXPR_OR, EXPR_NOT, EXPR_IS_NULL, EXPR_TO_F64) = range(1, 15)
+ EXPR_CMP, EXPR_AND, EXPR_OR, EXPR_NOT, EXPR_IS_NULL, EXPR_TO_F64, EXPR_DIV) = range(1, 16)
 CMP_EQ, CMP_NE, CMP_LT, CMP_LE, CMP_GT, CMP_GE = range(6)
 REG_INT, REG_F64, REG_BOOL = range(3)
 
@@ -70,7 +71,8 @@ TRowSet._fields_ = [
 
 
 class QdbValueType(C.Structure):
-    _fields_ = [("Kind", C.c_uint8), ("Width", C.c_int32), ("Scale", C.c_int32), ("Nullable", C.c_bool)]
+    _fields_ = [("Kind", C.c_uint8), ("Width", C.c_int32), ("Scale", C.c_int32), ("Nullable", C.c_bool),
+                ("Precision", C.c_uint8)]
 
 
 class QdbExprInsn(C.Structure):
@@ -112,7 +114,8 @@ class QdbAggregateSpec(C.Structure):
         ("KeyColumns", C.c_int32 * AGG_MAX_KEYS),
         ("NumAggs", C.c_int32),
         ("Aggs", QdbAggregateFn * AGG_MAX_AGGS),
-        ("Global", C.c_int32),
+        ("NumEmptyInputGroups", C.c_int32),
+        ("EmptyInputGroups", C.c_int64 * AGG_MAX_EMPTY_INPUT_GROUPS),
         ("OutputOnHost", C.c_int32),
         ("HasPredicate", C.c_int32),
         ("Predicate", QdbExprProgram),
@@ -130,6 +133,10 @@ JOIN_DISPATCH_FN = C.CFUNCTYPE(C.c_bool, C.c_void_p, C.c_void_p, C.c_void_p, C.c
                                C.POINTER(C.c_int64), C.c_int64)
 JOIN_MATERIALIZE_FN = C.CFUNCTYPE(C.c_int64, C.c_void_p, C.POINTER(TRowSet), C.c_int64, C.c_int64, C.POINTER(TRowSet))
 GROUP_HASH_FN = C.CFUNCTYPE(C.c_bool, C.POINTER(TRowSet), C.c_void_p, C.POINTER(C.c_int64))
+HEAP_ALLOC_FN = C.CFUNCTYPE(C.c_void_p, C.c_int64)
+HEAP_FREE_FN = C.CFUNCTYPE(None, C.c_void_p)
+HEAP_DETACH_FN = C.CFUNCTYPE(None, C.c_void_p)
+ROW_ERROR_FN = C.CFUNCTYPE(C.c_int64, C.c_int64)
 
 
 class QdbAggregateKernels(C.Structure):
@@ -202,7 +209,8 @@ EXPORTS = [
     "qdb_gpu_join_processor_new", "qdb_gpu_join_processor_append", "qdb_gpu_join_processor_build",
     "qdb_gpu_join_processor_probe", "qdb_gpu_join_processor_probe_only", "qdb_gpu_join_processor_pairs",
     "qdb_gpu_join_processor_arena", "qdb_gpu_join_processor_next", "qdb_gpu_join_processor_free",
-    "qdb_gpu_rowset_release",
+    "qdb_gpu_rowset_release", "qdb_gpu_set_host_heap", "qdb_gpu_destroy_kernel_rowset", "qdb_gpu_adopt_kernel_rowset",
+    "qdb_gpu_set_row_error_handler", "qdb_gpu_take_row_error",
 ]
 
 
@@ -268,6 +276,11 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "qdb_gpu_join_processor_next": (C.c_int, [P, C.POINTER(TRowSet)]),
         "qdb_gpu_join_processor_free": (None, [P]),
         "qdb_gpu_rowset_release": (None, [C.POINTER(TRowSet)]),
+        "qdb_gpu_set_host_heap": (None, [HEAP_ALLOC_FN, HEAP_FREE_FN, HEAP_DETACH_FN]),
+        "qdb_gpu_destroy_kernel_rowset": (None, [C.POINTER(TRowSet)]),
+        "qdb_gpu_adopt_kernel_rowset": (None, [C.POINTER(TRowSet)]),
+        "qdb_gpu_set_row_error_handler": (None, [ROW_ERROR_FN]),
+        "qdb_gpu_take_row_error": (i64, []),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -352,7 +352,7 @@ void output_types(const QdbAggregateSpec& spec, const QdbValueType* col_types, Q
             t.Width = 8;
         } else {
             const QdbValueType& in = col_types[fn.Column];
-            t.Nullable = in.Nullable || spec.Global;
+            t.Nullable = in.Nullable || spec.NumEmptyInputGroups > 0;
             if (in.Kind == QDB_KIND_FLOAT) {
                 t.Kind = QDB_KIND_FLOAT;
                 t.Width = 8;
@@ -406,7 +406,7 @@ bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P, bool on_device = tru
             if (ins.Dst >= QDB_EXPR_MAX_REGS || ins.A >= QDB_EXPR_MAX_REGS || ins.B >= QDB_EXPR_MAX_REGS) {
                 return fail("program register out of range");
             }
-            if (ins.Op == QDB_EXPR_TO_F64) return fail("decimal-to-double is not supported inside kernels");
+            if (ins.Op == QDB_EXPR_DIV && ins.Class != QDB_REG_F64) return fail("DIV is an f64 division (register class F64)");
             if (ins.Op == QDB_EXPR_COLUMN) {
                 if (ins.Column < 0 || ins.Column >= nin) return fail("program column %d is not an input column", ins.Column);
                 int s = slot_of(ins.Column);
@@ -418,7 +418,18 @@ bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P, bool on_device = tru
     };
     if (spec.NumKeys < 0 || spec.NumKeys > QDB_AGG_MAX_KEYS) return fail("bad key count %d", spec.NumKeys);
     if (spec.NumAggs < 0 || spec.NumAggs > QDB_AGG_MAX_AGGS) return fail("bad aggregate count %d", spec.NumAggs);
-    if (spec.Global && spec.NumKeys != 0) return fail("a global aggregate has no keys");
+    if (spec.NumEmptyInputGroups < 0 || spec.NumEmptyInputGroups > QDB_AGG_MAX_EMPTY_INPUT_GROUPS) {
+        return fail("bad empty-input group count %d", spec.NumEmptyInputGroups);
+    }
+    if (spec.NumEmptyInputGroups > 0) {
+        // gen.cpp:2226-2232: the seeded groups need a leading integer key.
+        const int c0 = spec.NumKeys > 0 ? spec.KeyColumns[0] : -1;
+        const QdbValueType* t0 = c0 >= 0 && c0 < nin ? &spec.Input.Types[c0] : nullptr;
+        if (!t0 || t0->Nullable || t0->Width > 8
+            || (t0->Kind != QDB_KIND_SIGNED && t0->Kind != QDB_KIND_UNSIGNED && t0->Kind != QDB_KIND_DATE)) {
+            return fail("an aggregate with empty-input groups requires a leading integer key");
+        }
+    }
     if (spec.NumComputed < 0 || spec.NumComputed > QDB_PROJECT_MAX_OUTPUTS) return fail("bad computed count");
 
     // Keys: the null bits of nullable fields first, then every field at its
@@ -676,25 +687,57 @@ bool fill_records(const AggPlan* P, uint64_t* recs, uint64_t nrecs, int stride)
     return hip_ok(hipGetLastError(), "record fill");
 }
 
-// A table of `capacity` slots plus the extra record, every slot empty.
-bool table_alloc(AggTable* T, uint64_t capacity)
+// The buffers of a table of `capacity` slots plus the extra record, every
+// slot empty. Nothing of T changes; on failure whatever was allocated here is
+// freed again.
+struct TableBufs {
+    unsigned long long* ctrl = nullptr;
+    uint32_t* occ = nullptr;
+    uint64_t* recs = nullptr;
+    int rstride = 1;
+    void release()
+    {
+        dfree(ctrl);
+        dfree(occ);
+        dfree(recs);
+        ctrl = nullptr;
+        occ = nullptr;
+        recs = nullptr;
+    }
+};
+
+bool table_bufs_alloc(const AggPlan* P, uint64_t capacity, TableBufs* B)
 {
-    const AggPlan* P = T->plan;
-    T->capacity = capacity;
-    T->rstride = record_stride(P->KW + P->proto.nwords);
-    T->ctrl = nullptr;
-    T->occ = nullptr;
+    *B = TableBufs();
+    B->rstride = record_stride(P->KW + P->proto.nwords);
+    bool ok = true;
     if (P->KW > 1) {
-        T->ctrl = (unsigned long long*)dalloc(capacity * 8);
-        if (!T->ctrl || !zero_fill(T->ctrl, capacity * 8)) return false;
+        B->ctrl = (unsigned long long*)dalloc(capacity * 8);
+        ok = B->ctrl && zero_fill(B->ctrl, capacity * 8);
     } else {
         const size_t ob = (size_t)((capacity + 31) / 32) * 4;
-        T->occ = (uint32_t*)dalloc(ob);
-        if (!T->occ || !zero_fill(T->occ, ob)) return false;
+        B->occ = (uint32_t*)dalloc(ob);
+        ok = B->occ && zero_fill(B->occ, ob);
     }
-    T->recs = (uint64_t*)dalloc((size_t)(capacity + 1) * T->rstride * 8);
-    if (!T->recs) return false;
-    return fill_records(P, T->recs, capacity + 1, T->rstride);
+    if (ok) {
+        B->recs = (uint64_t*)dalloc((size_t)(capacity + 1) * B->rstride * 8);
+        ok = B->recs && fill_records(P, B->recs, capacity + 1, B->rstride);
+    }
+    if (!ok) B->release();
+    return ok;
+}
+
+// T gets a new empty table of `capacity` slots (T's buffers must be free).
+bool table_alloc(AggTable* T, uint64_t capacity)
+{
+    TableBufs B;
+    if (!table_bufs_alloc(T->plan, capacity, &B)) return false;
+    T->capacity = capacity;
+    T->rstride = B.rstride;
+    T->ctrl = B.ctrl;
+    T->occ = B.occ;
+    T->recs = B.recs;
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -1050,20 +1093,19 @@ bool table_grow(AggTable* T, uint64_t need)
     const AggPlan* P = T->plan;
     uint64_t cap = T->capacity * 2;
     while (cap / 2 < need) cap *= 2;
+    // The new buffers are made first; the table keeps its own until every
+    // allocation has succeeded.
+    TableBufs B;
+    if (!table_bufs_alloc(P, cap, &B)) return false;
     unsigned long long* old_ctrl = T->ctrl;
     uint32_t* old_occ = T->occ;
     uint64_t* old_recs = T->recs;
     const uint64_t old_cap = T->capacity;
-    if (!table_alloc(T, cap)) {
-        dfree(T->ctrl);
-        dfree(T->occ);
-        dfree(T->recs);
-        T->ctrl = old_ctrl;
-        T->occ = old_occ;
-        T->recs = old_recs;
-        T->capacity = old_cap;
-        return false;
-    }
+    T->capacity = cap;
+    T->rstride = B.rstride;
+    T->ctrl = B.ctrl;
+    T->occ = B.occ;
+    T->recs = B.recs;
     AggArgs A = P->proto;
     A.ctrl = T->ctrl;
     A.occ = T->occ;
@@ -1113,6 +1155,76 @@ __global__ void k_set_counters(unsigned int* c, unsigned int groups, unsigned in
         c[2] = 0u;
         c[3] = extra_used;
     }
+}
+
+// The groups an empty input has (QdbAggregateSpec.EmptyInputGroups).
+struct SeedGroups {
+    int n;
+    int64_t id[QDB_AGG_MAX_EMPTY_INPUT_GROUPS];
+};
+
+// Inserts the seeded groups into a fresh HBM table, as the reference's Init
+// upserts its seed keys (qdb/kernel/gen.cpp:2233-2266): the leading key field
+// holds the id (the zero-extension of its width, as pack_key_field packs an
+// integer key), every other field is NULL when nullable and 0 otherwise; the
+// records keep their identities. One thread does it all: a handful of keys,
+// no contention, no wait on another lane.
+template <int KW>
+__global__ void k_seed_groups(AggArgs A, SeedGroups G)
+{
+    __shared__ unsigned int s_inserted;
+    if (threadIdx.x != 0) return;
+    s_inserted = 0u;
+    agg::LdsTable T;
+    T.ctrl = nullptr;
+    T.count = nullptr;
+    T.recs = nullptr;
+    T.inserted = &s_inserted;
+    T.fresh = 0;
+    for (int i = 0; i < G.n; ++i) {
+        uint64_t key[KW];
+        for (int w = 0; w < KW; ++w) key[w] = 0;
+        for (int f = 0; f < A.nkeys; ++f) {
+            const agg::KeyFieldDesc kf = A.keys[f];
+            if (f == 0) {
+                const uint64_t id = (uint64_t)G.id[i];
+                const uint64_t lo = kf.width >= 8 ? id : (id & ((1ull << (8 * kf.width)) - 1));
+                for (int w = 0; w < KW; ++w) {
+                    if (w == kf.word) key[w] |= lo << kf.shift;
+                }
+            } else if (kf.nullable) {
+                for (int w = 0; w < KW; ++w) {
+                    if (w == kf.null_word) key[w] |= 1ull << kf.null_bit;
+                }
+            }
+        }
+        if (agg::global_find_or_insert<KW, 1>(A, T, key) < 0) atomicAdd(&A.counters[1], 1u);
+    }
+    atomicAdd(&A.counters[0], s_inserted);
+}
+
+// Seeds the empty-input groups of a new table; T->groups counts them.
+bool table_seed(AggTable* T)
+{
+    const AggPlan* P = T->plan;
+    SeedGroups G;
+    memset(&G, 0, sizeof(G));
+    G.n = P->spec.NumEmptyInputGroups;
+    for (int i = 0; i < G.n; ++i) G.id[i] = P->spec.EmptyInputGroups[i];
+    if (G.n == 0) return true;
+    AggArgs A = P->proto;
+    table_args(T, &A);
+    switch (P->KW) {
+    case 1: hipLaunchKernelGGL(k_seed_groups<1>, dim3(1), dim3(64), 0, stream(), A, G); break;
+    case 2: hipLaunchKernelGGL(k_seed_groups<2>, dim3(1), dim3(64), 0, stream(), A, G); break;
+    case 3: hipLaunchKernelGGL(k_seed_groups<3>, dim3(1), dim3(64), 0, stream(), A, G); break;
+    default: hipLaunchKernelGGL(k_seed_groups<4>, dim3(1), dim3(64), 0, stream(), A, G); break;
+    }
+    if (!hip_ok(hipGetLastError(), "seed launch")) return false;
+    unsigned int cnt[4];
+    if (!read_counters(T, cnt)) return false;
+    if (cnt[1] != 0u) return fail("seeding the empty-input groups found no room");
+    return true;
 }
 
 // The table's counters before a launch (a kernel, not an upload from
@@ -1614,12 +1726,7 @@ int64_t table_finish_launch(AggTable* T, TRowSet* out, FinishTemps* tmp)
     uint32_t*& counts = tmp->counts;
     uint32_t*& offsets = tmp->offsets;
     auto release = [&]() { tmp->release(); };
-    if (P->spec.Global && T->groups == 0) {
-        // The one group of a global aggregate over an empty input: slot 0,
-        // still at its identities (the key word of a keyless aggregate is 0).
-        F.index = nullptr;
-        T->groups = 1;
-    } else if (T->groups > 0) {
+    if (T->groups > 0) {
         // The slots that hold groups, in slot order.
         const uint64_t slots = T->capacity + 1;
         const int64_t chunks = (int64_t)((slots + kOccChunk - 1) / kOccChunk);
@@ -1702,7 +1809,8 @@ int64_t table_finish_launch(AggTable* T, TRowSet* out, FinishTemps* tmp)
 bool table_finish_done(AggTable* T, TRowSet* out, FinishTemps* tmp)
 {
     tmp->release();
-    return !T->plan->spec.OutputOnHost || move_rowset_to_host(out, T->plan->out_types);
+    if (T->plan->spec.OutputOnHost && !move_rowset_to_host(out, T->plan->out_types)) return false;
+    return publish_rowset(out);
 }
 
 // agg_finish_rowset: groups -> columnar TRowSet; destroys the table.
@@ -1738,9 +1846,13 @@ int64_t agg_dispatch_impl(AggPlan* P, void* ht, TRowSet* batch, int64_t arg, int
         const int64_t slots = op == QDB_AGG_OP_INIT ? arg : 2 * arg;
         AggTable* T = table_new(P, slots);
         if (!T) return 0;
+        if (!table_seed(T)) {
+            table_free(T);
+            return 0;
+        }
         B->table = T;
         B->Capacity = (int64_t)T->capacity;
-        B->Size = 0;
+        B->Size = T->groups;
         B->NumAggs = P->spec.NumAggs;
         B->NumKeys = P->spec.NumKeys;
         B->KeySize = P->KW * 8;
@@ -1898,7 +2010,7 @@ extern "C" int qdb_gpu_agg_finish_batched(const QdbAggregateKernels* kernels, vo
     }
     if (!ok) {
         for (int64_t i = 0; i < made; ++i) {
-            if (outs[i].Destroy) outs[i].Destroy(&outs[i]);
+            destroy_kernel_rowset(&outs[i]);
             memset(&outs[i], 0, sizeof(TRowSet));
         }
         return -1;
@@ -2021,7 +2133,11 @@ std::string project_jit_source(const DevProgram* progs, int nout, const QdbValue
             else o += "                ((double*)O.data[" + ks + "])[row] = f;\n";
             o += "            }\n";
         } else {
-            o += "            if (eval) {\n                const u128 x = v.valid ? (u128)v.i : 0;\n";
+            o += "            if (eval) {\n                u128 x = v.valid ? (u128)v.i : 0;\n";
+            if (t.Kind == QDB_KIND_DECIMAL && t.Width == 8) {
+                o += "                if (v.valid && narrow_overflows(v.i, O.prec[" + ks + "])) {\n";
+                o += "                    x = 0;\n                    atomicOr(O.err, 1u << " + ks + ");\n                }\n";
+            }
             switch (t.Width) {
             case 1: o += "                ((uint8_t*)O.data[" + ks + "])[row] = (uint8_t)x;\n"; break;
             case 2: o += "                ((uint16_t*)O.data[" + ks + "])[row] = (uint16_t)x;\n"; break;

@@ -205,6 +205,30 @@ __device__ __forceinline__ uint64_t hash_combine(uint64_t h, uint64_t fh)
     return h ^ (fh + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2));
 }
 
+// rh_hash of a row's key: the reference always hashes the key STRUCT
+// (qdb/kernel/gen.cpp:344-373), whose fields are, per key column, a
+// `valid_i` bool when the column is nullable and then `key_i`
+// (qdb/kernel/aggregate_key.cpp:246-272). A bool field hashes as
+// value * 2685821657736338717 (gen.cpp:319-325) and a NULL key field as 0
+// (gen.cpp:357-364); every field hash enters the ordered combine from h = 0,
+// so even one non-null BIGINT k hashes to mix(k) + 0x9E3779B97F4A7C15.
+template <int N>
+__device__ __forceinline__ uint64_t rh_key_hash(const DevCol (&c)[N], int nkeys, int64_t row)
+{
+    uint64_t h = 0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if (k >= nkeys) break;
+        bool valid = true;
+        if (c[k].nullable) {
+            valid = col_valid(c[k], row);
+            h = hash_combine(h, valid ? 2685821657736338717ull : 0ull);
+        }
+        h = hash_combine(h, valid ? leaf_hash(c[k], row) : 0ull);
+    }
+    return h;
+}
+
 // Placement hash of a packed key (not result-visible): a strong 64-bit
 // finalizer over the key words.
 template <int KW>
@@ -231,6 +255,66 @@ __device__ __forceinline__ i128 pow10_i128(int64_t k)
     i128 p = 1;
     for (int64_t i = 0; i < k; ++i) p *= 10;
     return p;
+}
+
+// |n| / d for u128 operands by shift and subtract (the rare exact path of
+// the decimal conversion below; no 128-bit division routine needed).
+__device__ __forceinline__ u128 udiv128(u128 n, u128 d, u128* rem)
+{
+    u128 q = 0, r = 0;
+    for (int i = 127; i >= 0; --i) {
+        r = (r << 1) | ((n >> i) & 1);
+        if (r >= d) {
+            r -= d;
+            q |= (u128)1 << i;
+        }
+    }
+    *rem = r;
+    return q;
+}
+
+// v / 10^scale as the nearest double, ties to even: the contract of
+// qdb_decimal_to_f64 (qdb/modules/qumirdb.oz:983-1046). Below 2^53 with
+// scale <= 22 both operands are exact doubles and one IEEE division rounds
+// correctly; otherwise the integer quotient gets fraction bits by long
+// division until it holds at least 55 bits, and is rounded to 53 bits with a
+// guard bit and a sticky bit (the dropped bits and the remainder).
+__device__ __noinline__ double decimal_to_f64_dev(i128 v, int64_t scale)
+{
+    const bool neg = v < 0;
+    const u128 mag = neg ? (u128)0 - (u128)v : (u128)v;
+    const u128 p = (u128)pow10_i128(scale);
+    double r;
+    if (mag < ((u128)1 << 53) && scale <= 22) {
+        r = (double)(uint64_t)mag / (double)(uint64_t)p;
+    } else if (mag == 0) {
+        r = 0.0;
+    } else {
+        u128 rem;
+        u128 q = udiv128(mag, p, &rem);
+        int e = 0;
+        while (q < ((u128)1 << 54)) {
+            // r < p, so compare r with p - r instead of forming 2r.
+            q <<= 1;
+            if (rem >= p - rem) {
+                q |= 1;
+                rem -= p - rem;
+            } else {
+                rem += rem;
+            }
+            --e;
+        }
+        int guard = 0, sticky = rem != 0;
+        while (q >= ((u128)1 << 53)) {
+            sticky |= guard;
+            guard = (int)(q & 1);
+            q >>= 1;
+            ++e;
+        }
+        if (guard && (sticky || (q & 1))) q += 1;
+        r = ldexp((double)(uint64_t)q, e);
+    }
+    return neg ? -r : r;
 }
 
 // One expression instruction on its operand values: the semantics shared
@@ -336,6 +420,19 @@ __device__ __forceinline__ DVal exec_insn(const DevInsn& in, const DVal& a, cons
         o.valid = 1;
         o.i = a.valid ? 0 : 1;
         break;
+    case QDB_EXPR_TO_F64:
+        o.cls = QDB_REG_F64;
+        o.valid = a.valid;
+        if (a.cls == QDB_REG_F64) o.f = a.f;
+        else if (a.valid) o.f = decimal_to_f64_dev(a.i, in.lo);
+        break;
+    case QDB_EXPR_DIV: {
+        o.valid = a.valid && b.valid;
+        const double x = a.cls == QDB_REG_F64 ? a.f : (double)a.i;
+        const double y = b.cls == QDB_REG_F64 ? b.f : (double)b.i;
+        o.f = o.valid ? x / y : 0.0;
+        break;
+    }
     default:
         break;
     }
@@ -548,7 +645,20 @@ struct ProjectOut {
     uint8_t* mask[QDB_PROJECT_MAX_OUTPUTS];
     int32_t kind[QDB_PROJECT_MAX_OUTPUTS];
     int32_t width[QDB_PROJECT_MAX_OUTPUTS];
+    // DECIMAL(p <= 18) outputs: p, the digits a stored value may have
+    // (0: no check). A value with more is a row error: 0 is stored and bit k
+    // of *err is set.
+    int32_t prec[QDB_PROJECT_MAX_OUTPUTS];
+    unsigned int* err;
 };
+
+// The row-error check of a narrowing store (qdb_decimal64_narrow_checked,
+// qdb/modules/qumirdb.oz:1236-1245): true when |x| >= 10^prec.
+__device__ __forceinline__ bool narrow_overflows(i128 x, int prec)
+{
+    const i128 lim = pow10_i128(prec);
+    return x >= lim || x <= -lim;
+}
 
 // Writes 64 bits of a bitmap for the 64 rows starting at `base` (a multiple
 // of 64); the last word writes only the bytes inside the buffer.

@@ -1,17 +1,31 @@
-// Host operator processors over the C-ABI entry points, mirroring the
-// reference's qdb/exec processors: TAggregateProcessor
-// (qdb/exec/aggregate_exec.h:15-44, aggregate_exec.cpp:52-156) and
-// TRowJoinProcessor (qdb/exec/row_join_exec.h:40-152,
-// row_join_exec.cpp:22-276). They own the fixed-size state blobs, drive the
-// op-code protocol (EnsureInit -> Update per batch -> Finish;
-// Init -> Append* -> Build -> Probe* -> Materialize) and turn the kernels'
-// return codes into std::runtime_error, as the reference does.
+// Host drivers of the device operators: they own the fixed-size state blobs
+// of one aggregate or one row join, drive the kernels' op-code protocol
+// (qdb/kernel/compiler.h:217-230, 363-383) and check the invariants the
+// reference's callers check, so that a caller holding only the C-ABI gets the
+// same behaviour the reference's processors give its kernels:
+//
+//   aggregate  Init (or InitRows) once, before the first batch, with the
+//              plan's capacity and its row-layout hints
+//              (qdb/exec/aggregate_exec.cpp:67-95); Update per batch (< 0 is
+//              an error, :107-113); at the end the table's Size is read from
+//              the blob BEFORE agg_finish_rowset, the finalized row count
+//              must equal it (:128-132), and the kernel-owned output is
+//              adopted (qdb/exec/kernel_rowset.h:30-38).
+//   row join   Init, Append per build batch, Build once, then Probe per probe
+//              batch; the probe's pairs are materialized in output batches
+//              of a fixed row count before the probe batch is released, and
+//              the pair count is reset after each drain
+//              (qdb/exec/row_join_exec.cpp:167-190).
+//
+// The drivers are push-style: a batch is handed in by the caller and stays
+// the caller's (borrowed, never released here); outputs queue until the
+// caller takes them. The reference's pull loop (fetch callbacks, held probe
+// batches, the build-side choice) is scheduler work and has no counterpart.
 #pragma once
 
 #include <array>
 #include <cstdint>
 #include <deque>
-#include <functional>
 #include <stdexcept>
 #include <vector>
 
@@ -19,112 +33,86 @@
 
 namespace NQdbGpu {
 
-inline void Retain(TRowSet* rs)
-{
-    ++rs->RefCount;
-}
+// A driver error carries the library's error text.
+class TDriverError : public std::runtime_error {
+public:
+    explicit TDriverError(const char* what);
+};
 
-inline void Release(TRowSet* rs)
-{
-    if (--rs->RefCount == 0 && rs->Destroy) rs->Destroy(rs);
-}
+// Takes ownership of a kernel-owned output: every owner is detached from the
+// host heap (qdb_gpu_set_host_heap's detach, TKernelHeap::Detach in the
+// reference) and the destroy callback is installed.
+void AdoptOutput(TRowSet& rowSet);
 
-enum class EAggregateTableLayout { Swiss, Row };
-
-// TAggregateTablePlan (qdb/exec/plan_data.h:226-250).
-struct TAggregateTablePlan {
-    EAggregateTableLayout Layout = EAggregateTableLayout::Swiss;
-    int64_t InitialCapacity = 8;
-    int64_t InitialGroups = 0;
+// The table plan the reference's planner hands to its processor
+// (TAggregateTablePlan, qdb/exec/plan_data.h:226-250).
+struct TAggregatePlan {
+    bool RowLayout = false;
+    int64_t InitialCapacity = 8;   // Init's argument: a power of two, at least 8
+    int64_t InitialGroups = 0;     // InitRows' argument
     int64_t RowSwitchGroups = 0;
     int64_t RowExpectedGroups = 0;
 };
 
-class TAggregateProcessor {
+class TAggregateDriver {
 public:
-    explicit TAggregateProcessor(QdbAggregateKernels kernels, TAggregateTablePlan plan = {});
-    ~TAggregateProcessor();
-    TAggregateProcessor(const TAggregateProcessor&) = delete;
-    TAggregateProcessor& operator=(const TAggregateProcessor&) = delete;
+    TAggregateDriver(const QdbAggregateKernels& kernels, const TAggregatePlan& plan);
+    ~TAggregateDriver();
+    TAggregateDriver(const TAggregateDriver&) = delete;
+    TAggregateDriver& operator=(const TAggregateDriver&) = delete;
 
-    void Add(TRowSet& rowSet);
-    // Updates many processors of one compiled aggregate, processors[i] with
-    // rowSets[i], in one batched launch (qdb_gpu_agg_update_batched).
-    static void AddBatched(TAggregateProcessor* const* processors, TRowSet* const* rowSets, int64_t n);
-    bool Finish(TRowSet& rowSet);
-    // Finishes many processors of one compiled aggregate, processors[i] into
-    // rowSets[i], with one synchronization (qdb_gpu_agg_finish_batched).
-    static void FinishBatched(TAggregateProcessor* const* processors, TRowSet* rowSets, int64_t n);
+    void Consume(const TRowSet& batch);
+    // drivers[i] consumes batches[i], every driver of one compiled
+    // aggregate, in one batched launch.
+    static void ConsumeMany(TAggregateDriver* const* drivers, const TRowSet* const* batches, int64_t n);
+    // Finalizes into `out` (adopted); returns the row count.
+    int64_t Emit(TRowSet& out);
+    // drivers[i] finalizes into outs[i], with one synchronization.
+    static void EmitMany(TAggregateDriver* const* drivers, TRowSet* outs, int64_t n);
+    // Groups so far (the table's Size); 0 before the first batch and after Emit.
     int64_t Groups() const;
-    bool Flush(TRowSet& rowSet);
 
 private:
-    void EnsureInit();
+    enum class EState { Fresh, Open, Emitted };
+    void Open();
+    int64_t TableSize() const;
+    void* Blob() { return Table_.data(); }
 
     QdbAggregateKernels Kernels_;
-    TAggregateTablePlan Plan_;
-    std::array<uint8_t, QDB_HASH_TABLE_SIZE> HashTable_ = {};
-    bool Initialized_ = false;
-    bool Finished_ = false;
+    TAggregatePlan Plan_;
+    std::array<uint8_t, QDB_HASH_TABLE_SIZE> Table_ = {};
+    EState State_ = EState::Fresh;
 };
 
-enum class EJoinBuildSide { Auto, Left, Right };
-enum class EJoinFetchResult { OK, NO_DATA, FINISHED };
-enum class EJoinProcessorResult { OK, NEED_DATA, FINISHED };
-
-class TRowJoinProcessor {
+class TJoinDriver {
 public:
-    using TFetch = std::function<EJoinFetchResult(TRowSet&)>;
+    TJoinDriver(const QdbRowJoinKernels& kernels, int64_t outputRows);
+    ~TJoinDriver();
+    TJoinDriver(const TJoinDriver&) = delete;
+    TJoinDriver& operator=(const TJoinDriver&) = delete;
 
-    // outputBatchRows: rows per materialized output batch (the reference uses
-    // 16384, row_join_exec.cpp:14).
-    TRowJoinProcessor(QdbRowJoinKernels kernels, EJoinBuildSide buildSide, size_t probeHoldBytes,
-        int64_t outputBatchRows = 16384);
-    ~TRowJoinProcessor();
-    TRowJoinProcessor(const TRowJoinProcessor&) = delete;
-    TRowJoinProcessor& operator=(const TRowJoinProcessor&) = delete;
-
-    EJoinProcessorResult Process(const TFetch& left, const TFetch& right, TRowSet& output);
-    EJoinBuildSide RequiredInputSide() const;
-
-    // The explicit steps Process is made of.
-    void AppendBuild(TRowSet& batch);
-    void FinishBuild();
-    // Probes the batch and queues its materialized output; releases it.
-    void ProbeBatch(TRowSet& batch);
-    bool DrainReadyOutput(TRowSet& rowSet);
-    // Probes without materializing: the pairs stay in the pair buffer until
-    // the next probe (for fused aggregates and pair checks).
-    int64_t ProbeOnly(TRowSet& batch);
-    const QdbPairBuffer& Pairs() const { return PairBuffer_; }
-    void* ArenaBlob() { return Arena_.data(); }
+    void AddBuild(const TRowSet& batch);
+    void Seal();
+    // Probes the batch; with `materialize` its matches are gathered into
+    // queued output batches and the pairs are drained, else the pairs stay
+    // readable (Pairs()) until the next probe. Returns the pair count.
+    int64_t Probe(const TRowSet& batch, bool materialize);
+    // The next queued output batch (adopted); false when none is left.
+    bool Next(TRowSet& out);
+    const QdbPairBuffer& Pairs() const { return Pairs_; }
+    void* Arena() { return Arena_.data(); }
 
 private:
-    void EnsureInit();
-    bool Dispatch(int64_t op, TRowSet* batch);
-    bool PullBuildBatch(const TFetch& build);
-    bool PullProbeBatch(const TFetch& probe);
-    bool CanHoldProbe() const;
-    uint64_t BatchBytes(const TRowSet& batch) const;
+    bool Op(int64_t op, const TRowSet* batch);
 
     QdbRowJoinKernels Kernels_;
-    EJoinBuildSide BuildSide_;
-    size_t ProbeHoldBytes_;
-    int64_t OutputBatchRows_;
+    int64_t OutputRows_;
     std::array<uint8_t, QDB_ROW_ARENA_SIZE> Arena_ = {};
     std::array<uint8_t, QDB_ROW_TABLE_SIZE> Table_ = {};
     std::array<uint8_t, QDB_ROW_SCRATCH_SIZE> Scratch_ = {};
-    QdbPairBuffer PairBuffer_ = {0, 0, nullptr};
-    struct THeldProbe {
-        TRowSet Batch;
-        uint64_t Bytes;
-    };
-    std::deque<THeldProbe> PendingProbe_;
-    uint64_t HeldProbeBytes_ = 0;
-    std::deque<TRowSet> ReadyOutput_;
-    bool Initialized_ = false;
-    bool BuildDone_ = false;
-    bool ProbeDone_ = false;
+    QdbPairBuffer Pairs_ = {0, 0, nullptr};
+    std::deque<TRowSet> Outputs_;
+    bool Sealed_ = false;
 };
 
 }  // namespace NQdbGpu

@@ -42,13 +42,22 @@ struct ExprPlan {
     DevProgram* d_progs = nullptr;
     std::vector<DevProgram> h_progs;
     QdbValueType out_types[QDB_PROJECT_MAX_OUTPUTS];
+    // Digits each output may hold (DECIMAL(p <= 18); 0 = no check) and the
+    // device word the kernel flags row errors in.
+    int32_t out_prec[QDB_PROJECT_MAX_OUTPUTS] = {};
+    bool checks = false;
+    unsigned int* d_err = nullptr;
     // The kernel generated for a filter's predicate or a projection's
     // outputs (made at its first call; null: the generic interpreter kernel
     // runs).
     std::once_flag jit_once;
     hipFunction_t jit_filter = nullptr;
     hipFunction_t jit_project = nullptr;
-    ~ExprPlan() { dfree(d_progs); }
+    ~ExprPlan()
+    {
+        dfree(d_progs);
+        dfree(d_err);
+    }
 };
 
 // Remaps the programs' input columns to device slots and uploads them.
@@ -68,7 +77,7 @@ bool compile_programs(const QdbColumnSchema& input, const QdbExprProgram* progs,
             if (in.Dst >= QDB_EXPR_MAX_REGS || in.A >= QDB_EXPR_MAX_REGS || in.B >= QDB_EXPR_MAX_REGS) {
                 return fail("program register out of range");
             }
-            if (in.Op == QDB_EXPR_TO_F64) return fail("decimal-to-double is not supported inside kernels");
+            if (in.Op == QDB_EXPR_DIV && in.Class != QDB_REG_F64) return fail("DIV is an f64 division (register class F64)");
             if (in.Op != QDB_EXPR_COLUMN) continue;
             if (in.Column < 0 || in.Column >= input.NumColumns) return fail("program column %d out of range", in.Column);
             const QdbValueType& t = input.Types[in.Column];
@@ -159,7 +168,11 @@ __global__ void __launch_bounds__(kBlock) k_project(DevCols cols, const DevProgr
                     if (O.width[k] == 4) ((float*)d)[i] = (float)f;
                     else ((double*)d)[i] = f;
                 } else {
-                    const u128 x = v.valid ? (u128)v.i : 0;
+                    u128 x = v.valid ? (u128)v.i : 0;
+                    if (O.prec[k] && v.valid && narrow_overflows(v.i, O.prec[k])) {
+                        x = 0;
+                        atomicOr(O.err, 1u << k);
+                    }
                     switch (O.width[k]) {
                     case 1: ((uint8_t*)d)[i] = (uint8_t)x; break;
                     case 2: ((uint16_t*)d)[i] = (uint16_t)x; break;
@@ -190,23 +203,15 @@ struct HashCols {
     DevCol c[QDB_AGG_MAX_KEYS];
 };
 
-__global__ void __launch_bounds__(kBlock) k_group_hash(HashCols cols, int nkeys, bool scalar, int64_t n, const uint8_t* sel,
+__global__ void __launch_bounds__(kBlock) k_group_hash(HashCols cols, int nkeys, int64_t n, const uint8_t* sel,
     uint64_t* hashes)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+        // A rejected row gets no hash: its slot keeps what the caller left
+        // there (qdb/kernel/join_gen.cpp:1557-1565).
         if (sel && !sel[i]) continue;
-        uint64_t h;
-        if (scalar) {
-            h = leaf_hash(cols.c[0], i);
-        } else {
-            h = 0;
-            for (int k = 0; k < nkeys; ++k) {
-                const uint64_t fh = col_valid(cols.c[k], i) ? leaf_hash(cols.c[k], i) : 0ull;
-                h = hash_combine(h, fh);
-            }
-        }
-        hashes[i] = h;
+        hashes[i] = rh_key_hash(cols.c, nkeys, i);
     }
 }
 
@@ -265,9 +270,24 @@ void filter_impl(ExprPlan* P, TRowSet* rs, const uint8_t* input_sel)
     (void)hip_ok(hipGetLastError(), "filter launch");
 }
 
+thread_local int64_t g_row_error = 0;
+QdbRowErrorFn g_row_error_handler = nullptr;
+
+// A row of the last project call failed: its precision is kept for
+// qdb_gpu_take_row_error, the error text set, and the registered handler
+// called (the reference's qdb_decimal64_overflow raises, or returns while
+// the caller defers row errors).
+void report_row_error(int precision)
+{
+    g_row_error = precision;
+    fail("value does not fit DECIMAL precision %d", precision);
+    if (g_row_error_handler) g_row_error_handler(precision);
+}
+
 void project_impl(ExprPlan* P, TRowSet* in, void** out, const uint8_t* input_sel)
 {
     clear_error();
+    g_row_error = 0;
     if (!P || !in || in->RowCount <= 0) return;
     DevCols cols;
     if (!bind_cols(P, in, &cols)) return;
@@ -278,6 +298,12 @@ void project_impl(ExprPlan* P, TRowSet* in, void** out, const uint8_t* input_sel
         O.mask[k] = P->out_types[k].Nullable ? (uint8_t*)out[P->nprogs + k] : nullptr;
         O.kind[k] = P->out_types[k].Kind;
         O.width[k] = P->out_types[k].Width;
+        O.prec[k] = P->out_prec[k];
+    }
+    if (P->checks) {
+        if (!P->d_err) P->d_err = (unsigned int*)dalloc(256);
+        if (!P->d_err || !zero_fill(P->d_err, 4)) return;
+        O.err = P->d_err;
     }
     std::call_once(P->jit_once, [P]() {
         if (!jit_enabled() || P->nprogs < 1) return;
@@ -301,8 +327,17 @@ void project_impl(ExprPlan* P, TRowSet* in, void** out, const uint8_t* input_sel
                 input_sel);
         }
     }
-    // Stream-ordered, as the filter.
-    (void)hip_ok(hipGetLastError(), "project launch");
+    if (!hip_ok(hipGetLastError(), "project launch")) return;
+    if (P->checks) {
+        // A possible row error is known only after the kernel: one readback.
+        unsigned int err = 0;
+        if (!hip_ok(hipMemcpyAsync(&err, P->d_err, 4, hipMemcpyDeviceToHost, stream()), "row error readback")
+            || !sync("project")) {
+            return;
+        }
+        if (err) report_row_error(P->out_prec[__builtin_ctz(err)]);
+    }
+    // Otherwise stream-ordered, as the filter.
 }
 
 bool hash_impl(HashPlan* P, TRowSet* rs, uint64_t* hashes, const int64_t* key_cols)
@@ -312,7 +347,6 @@ bool hash_impl(HashPlan* P, TRowSet* rs, uint64_t* hashes, const int64_t* key_co
     if (rs->RowCount <= 0) return true;
     HashCols cols;
     memset(&cols, 0, sizeof(cols));
-    bool scalar = P->nkeys == 1;
     for (int k = 0; k < P->nkeys; ++k) {
         const int64_t c = key_cols[k];
         if (c < 0 || c >= rs->ColumnCount || c >= P->input.NumColumns) return fail("hash key column %lld out of range", (long long)c);
@@ -327,9 +361,8 @@ bool hash_impl(HashPlan* P, TRowSet* rs, uint64_t* hashes, const int64_t* key_co
         d.kind = t.Kind;
         d.width = t.Width;
         d.nullable = t.Nullable;
-        if (t.Nullable) scalar = false;
     }
-    hipLaunchKernelGGL(k_group_hash, dim3(grid_for(rs->RowCount, kBlock, 8)), dim3(kBlock), 0, stream(), cols, P->nkeys, scalar,
+    hipLaunchKernelGGL(k_group_hash, dim3(grid_for(rs->RowCount, kBlock, 8)), dim3(kBlock), 0, stream(), cols, P->nkeys,
         rs->RowCount, rs->Selection, hashes);
     return hip_ok(hipGetLastError(), "hash launch") && sync("hash");
 }
@@ -423,6 +456,12 @@ bool set_project_outputs(const QdbProjectSpec* spec, ExprPlan* P)
         const QdbValueType& t = spec->OutputTypes[k];
         const bool ok_width = t.Kind == QDB_KIND_BOOL || t.Width == 1 || t.Width == 2 || t.Width == 4 || t.Width == 8 || t.Width == 16;
         if (!ok_width || t.Kind == QDB_KIND_STRING) return fail("project output %d: unsupported type", k);
+        if (t.Kind == QDB_KIND_DECIMAL && t.Width == 8) {
+            const int p = t.Precision ? t.Precision : 18;
+            if (p > 18) return fail("project output %d: DECIMAL(%d) does not fit 8 bytes", k, p);
+            P->out_prec[k] = p;
+            P->checks = true;
+        }
     }
     return true;
 }
@@ -469,6 +508,18 @@ extern "C" int qdb_gpu_compile_project(const QdbProjectSpec* spec, QdbProjectFn*
     }
     *out = kProjectFns.fns[slot];
     return slot;
+}
+
+extern "C" void qdb_gpu_set_row_error_handler(QdbRowErrorFn fn)
+{
+    g_row_error_handler = fn;
+}
+
+extern "C" int64_t qdb_gpu_take_row_error(void)
+{
+    const int64_t p = g_row_error;
+    g_row_error = 0;
+    return p;
 }
 
 extern "C" int qdb_gpu_compile_group_hash(const QdbColumnSchema* input, int32_t num_keys, QdbGroupHashFn* out)

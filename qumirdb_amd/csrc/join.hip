@@ -2584,6 +2584,7 @@ int64_t jr_materialize_impl(JoinPlan* P, void* pairs, TRowSet* stream_rs, int64_
     }
     if (!sync("materialize")) return 0;
     if (P->spec.OutputOnHost && !move_rowset_to_host(out, P->out_types)) return 0;
+    if (!publish_rowset(out)) return 0;
     return n;
 }
 

@@ -76,18 +76,32 @@ bool sort_pairs_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t*
 // Device-resident copy of a host struct (uploaded once, freed with dfree).
 void* upload(const void* host, size_t bytes);
 
-// The buffers behind a TRowSet that an entry point produced: device blocks
-// (dalloc) and host blocks (malloc), freed together by the rowset's Destroy.
+// The host heap output rowsets are allocated from (qdb_gpu_set_host_heap:
+// the reference's qdb_alloc / qdb_free; malloc / free by default).
+void* heap_alloc(size_t bytes);
+void heap_free(void* p);
+void heap_detach(void* p);
+
+// The buffers of an output rowset while an entry point builds it: its
+// TColumns, device blocks (dalloc) and host blocks (heap_alloc).
 struct OwnedRowSet {
     std::vector<TColumn> cols;
     std::vector<void*> dev;
     std::vector<void*> host;
 };
 
-// Fills `out` as an owned rowset of `ncols` zeroed TColumns over `rows`
-// rows; returns the owner to which buffers are added.
+// Starts `out` as a rowset under construction of `ncols` zeroed TColumns over
+// `rows` rows; returns the builder to which buffers are added. Until it is
+// published, out->Destroy frees everything the builder holds.
 OwnedRowSet* init_owned_rowset(TRowSet* out, int64_t rows, int ncols);
-// Moves device output columns to host memory (for OutputOnHost specs).
+// Moves the device output columns of a rowset under construction to the
+// host heap (for OutputOnHost specs).
 bool move_rowset_to_host(TRowSet* rs, const QdbValueType* types);
+// Turns a rowset under construction into the kernel-owned form of the
+// reference (qdb/exec/kernel_rowset.h:14-38): Columns from the host heap,
+// Private the owners list, Destroy NULL. False (everything freed) on failure.
+bool publish_rowset(TRowSet* rs);
+// Frees a kernel-owned rowset (or one under construction).
+void destroy_kernel_rowset(TRowSet* rs);
 
 }  // namespace qdbgpu

@@ -10,6 +10,7 @@
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/qdb_gen.h"
@@ -36,6 +37,9 @@ struct Runtime {
     };
     std::multimap<size_t, FreeBlock> free_dev;
     std::unordered_map<void*, size_t> dev_sizes;
+    // Blocks in free_dev: a second dfree of one of them is refused (two
+    // later dallocs would otherwise receive the same memory).
+    std::unordered_set<void*> cached_set;
     uint64_t frees = 0;
     // counters: driver allocations and frees, free-memory queries
     int64_t n_malloc = 0, n_free = 0, n_meminfo = 0;
@@ -181,6 +185,7 @@ static void evict_oldest_locked(Runtime& r)
     }
     r.cached -= oldest->first;
     r.dev_sizes.erase(oldest->second.p);
+    r.cached_set.erase(oldest->second.p);
     (void)hipFree(oldest->second.p);
     r.n_free += 1;
     r.free_dev.erase(oldest);
@@ -210,6 +215,7 @@ static void release_cached_locked(Runtime& r)
         r.n_free += 1;
     }
     r.free_dev.clear();
+    r.cached_set.clear();
     r.cached = 0;
 }
 
@@ -226,6 +232,7 @@ void* dalloc(size_t bytes)
             void* p = it->second.p;
             r.cached -= it->first;
             r.free_dev.erase(it);
+            r.cached_set.erase(p);
             return p;
         }
     }
@@ -261,6 +268,11 @@ void dfree(void* p)
     std::lock_guard<std::mutex> lock(r.mu);
     auto it = r.dev_sizes.find(p);
     if (it == r.dev_sizes.end()) return;
+    if (!r.cached_set.insert(p).second) {
+        // Already free: keep the pool consistent and say so.
+        fail("device pool: block %p freed twice", p);
+        return;
+    }
     const size_t sz = it->second;
     r.free_dev.emplace(sz, Runtime::FreeBlock{p, ++r.frees});
     r.cached += sz;
@@ -560,7 +572,7 @@ bool download_ranges(const std::vector<const void*>& src, const std::vector<size
         if (!src[i] || !bytes[i]) continue;
         off[i] = total;
         total += (bytes[i] + 15) / 16 * 16;
-        (*host)[i] = malloc(bytes[i]);
+        (*host)[i] = heap_alloc(bytes[i]);
         if (!(*host)[i]) return fail("download: out of host memory");
     }
     if (total == 0) return true;
@@ -603,20 +615,73 @@ bool download_ranges(const std::vector<const void*>& src, const std::vector<size
     return ok;
 }
 
-void destroy_owned(TRowSet* rs)
+// The builder of an output rowset keeps its TColumns and buffers until the
+// rowset is published; a rowset still being built carries the builder in
+// Private and destroy_builder as Destroy, so that a failure on the way frees
+// everything.
+void destroy_builder(TRowSet* rs)
 {
     OwnedRowSet* o = (OwnedRowSet*)rs->Private;
     if (!o) return;
     for (void* p : o->dev) dfree(p);
-    for (void* p : o->host) free(p);
+    for (void* p : o->host) heap_free(p);
     delete o;
     rs->Private = nullptr;
     rs->Columns = nullptr;
 }
 
+// A published rowset's device buffers are listed in one host block of its
+// owners list: [kDeviceLease, count, pointers...].
+constexpr int64_t kDeviceLease = 0x5145444c45415345ll;
+
 }  // namespace
 
 namespace qdbgpu {
+
+// ---------------------------------------------------------------------------
+// Host heap and kernel-owned output rowsets.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+void* default_alloc(int64_t size)
+{
+    return malloc(size > 0 ? (size_t)size : 1);
+}
+
+void default_free(void* p)
+{
+    free(p);
+}
+
+struct HostHeap {
+    QdbHeapAllocFn alloc = default_alloc;
+    QdbHeapFreeFn free = default_free;
+    QdbHeapDetachFn detach = nullptr;
+};
+
+HostHeap& heap()
+{
+    static HostHeap h;
+    return h;
+}
+
+}  // namespace
+
+void* heap_alloc(size_t bytes)
+{
+    return heap().alloc((int64_t)bytes);
+}
+
+void heap_free(void* p)
+{
+    if (p) heap().free(p);
+}
+
+void heap_detach(void* p)
+{
+    if (p && heap().detach) heap().detach(p);
+}
 
 OwnedRowSet* init_owned_rowset(TRowSet* out, int64_t rows, int ncols)
 {
@@ -627,10 +692,75 @@ OwnedRowSet* init_owned_rowset(TRowSet* out, int64_t rows, int ncols)
     out->Columns = o->cols.data();
     out->ColumnCount = ncols;
     out->RowCount = rows;
-    out->Destroy = destroy_owned;
+    out->Destroy = destroy_builder;
     out->Private = o;
     out->RefCount = 1;
     return o;
+}
+
+// The kernel-owned form of qdb/exec/kernel_rowset.h:14-38: Columns and every
+// host buffer come from the host heap, Private is the owners list
+// (owners[0] = count, then the blocks, the list itself allocated from the
+// heap too) and Destroy is NULL: the caller adopts the rowset and installs its
+// destroy callback, as the reference's processors do after
+// agg_finish_rowset. Device buffers (an output left in HBM) are listed in one
+// lease block of the owners list, which qdb_gpu_destroy_kernel_rowset
+// returns to the device pool.
+bool publish_rowset(TRowSet* rs)
+{
+    OwnedRowSet* o = (OwnedRowSet*)rs->Private;
+    if (!o) return fail("publish: not a rowset under construction");
+    const size_t ncols = (size_t)(rs->ColumnCount > 0 ? rs->ColumnCount : 0);
+    TColumn* cols = (TColumn*)heap_alloc(std::max<size_t>(ncols, 1) * sizeof(TColumn));
+    int64_t* lease = nullptr;
+    if (!o->dev.empty()) lease = (int64_t*)heap_alloc((o->dev.size() + 2) * sizeof(int64_t));
+    const size_t nowners = 1 + (lease ? 1 : 0) + o->host.size();
+    int64_t* owners = (int64_t*)heap_alloc((nowners + 1) * sizeof(int64_t));
+    if (!cols || (!o->dev.empty() && !lease) || !owners) {
+        heap_free(cols);
+        heap_free(lease);
+        heap_free(owners);
+        destroy_builder(rs);
+        return fail("publish: out of host memory");
+    }
+    if (ncols) memcpy(cols, o->cols.data(), ncols * sizeof(TColumn));
+    size_t k = 0;
+    owners[++k] = (int64_t)(intptr_t)cols;
+    if (lease) {
+        lease[0] = kDeviceLease;
+        lease[1] = (int64_t)o->dev.size();
+        for (size_t i = 0; i < o->dev.size(); ++i) lease[2 + i] = (int64_t)(intptr_t)o->dev[i];
+        owners[++k] = (int64_t)(intptr_t)lease;
+    }
+    for (void* p : o->host) owners[++k] = (int64_t)(intptr_t)p;
+    owners[0] = (int64_t)k;
+    delete o;
+    rs->Columns = cols;
+    rs->Private = owners;
+    rs->Destroy = nullptr;
+    return true;
+}
+
+void destroy_kernel_rowset(TRowSet* rs)
+{
+    if (!rs) return;
+    if (rs->Destroy == destroy_builder) {
+        destroy_builder(rs);
+        return;
+    }
+    int64_t* owners = (int64_t*)rs->Private;
+    if (!owners) return;
+    for (int64_t i = 1; i <= owners[0]; ++i) {
+        int64_t* block = (int64_t*)(intptr_t)owners[i];
+        if (block && block[0] == kDeviceLease) {
+            for (int64_t d = 0; d < block[1]; ++d) dfree((void*)(intptr_t)block[2 + d]);
+        }
+        heap_free(block);
+    }
+    heap_free(owners);
+    rs->Private = nullptr;
+    rs->Columns = nullptr;
+    rs->Destroy = nullptr;
 }
 
 bool move_rowset_to_host(TRowSet* rs, const QdbValueType* types)
@@ -651,21 +781,16 @@ bool move_rowset_to_host(TRowSet* rs, const QdbValueType* types)
     std::vector<void*> host;
     const bool ok = download_ranges(src, bytes, &host);
     for (size_t i = 0; i < host.size(); ++i) {
-        if (!host[i]) host[i] = malloc(1);  // a non-null buffer for every column and mask present
+        if (!host[i] && (i % 2 == 0 || src[i])) host[i] = heap_alloc(1);  // a buffer for every column and mask present
     }
     if (!ok) {
-        for (void* p : host) free(p);
+        for (void* p : host) heap_free(p);
         return false;
     }
     for (int64_t c = 0; c < rs->ColumnCount; ++c) {
         TColumn& col = o->cols[(size_t)c];
         col.Data = (char*)host[2 * (size_t)c];
-        if (col.Mask) {
-            col.Mask = (const uint8_t*)host[2 * (size_t)c + 1];
-        } else {
-            free(host[2 * (size_t)c + 1]);
-            host[2 * (size_t)c + 1] = nullptr;
-        }
+        col.Mask = col.Mask ? (const uint8_t*)host[2 * (size_t)c + 1] : nullptr;
     }
     for (void* p : o->dev) dfree(p);
     o->dev.clear();
@@ -677,64 +802,103 @@ bool move_rowset_to_host(TRowSet* rs, const QdbValueType* types)
 
 }  // namespace qdbgpu
 
+extern "C" void qdb_gpu_set_host_heap(QdbHeapAllocFn alloc, QdbHeapFreeFn free_fn, QdbHeapDetachFn detach)
+{
+    HostHeap& h = heap();
+    h.alloc = alloc ? alloc : default_alloc;
+    h.free = free_fn ? free_fn : default_free;
+    h.detach = detach;
+}
+
+extern "C" void qdb_gpu_destroy_kernel_rowset(TRowSet* rs)
+{
+    destroy_kernel_rowset(rs);
+}
+
+extern "C" void qdb_gpu_adopt_kernel_rowset(TRowSet* rs)
+{
+    if (!rs) return;
+    int64_t* owners = (int64_t*)rs->Private;
+    if (owners) {
+        for (int64_t i = 1; i <= owners[0]; ++i) heap_detach((void*)(intptr_t)owners[i]);
+        heap_detach(owners);
+    }
+    rs->Destroy = qdb_gpu_destroy_kernel_rowset;
+}
+
 extern "C" int qdb_gpu_stage_rowset(const TRowSet* host, const QdbColumnSchema* types, TRowSet* out)
 {
     clear_error();
+    memset(out, 0, sizeof(*out));
     if (!ensure_init()) return -1;
+    if (host->ColumnCount < 0 || host->ColumnCount > types->NumColumns) return fail("stage: %lld columns, schema has %d", (long long)host->ColumnCount, types->NumColumns), -1;
     const int64_t n = host->RowCount;
-    OwnedRowSet* o = new OwnedRowSet();
-    o->cols.resize((size_t)host->ColumnCount);
-    const size_t mask_bytes = (size_t)((n + 7) / 8);
-    auto stage = [&](const void* src, size_t bytes) -> void* {
-        if (!src || bytes == 0) return nullptr;
-        void* d = dalloc(bytes);
-        if (!d) return nullptr;
-        o->dev.push_back(d);
-        void* pin = halloc_pinned(bytes);
-        if (!pin) return nullptr;
-        memcpy(pin, src, bytes);
-        bool ok = hip_ok(hipMemcpyAsync(d, pin, bytes, hipMemcpyHostToDevice, stream()), "stage copy") && sync("stage");
-        hfree_pinned(pin);
-        return ok ? d : nullptr;
-    };
+    // Every buffer's bytes: data and mask per column, then selection and hash.
+    std::vector<const void*> src;
+    std::vector<size_t> bytes;
     for (int64_t c = 0; c < host->ColumnCount; ++c) {
         const TColumn& hc = host->Columns[c];
         const QdbValueType& t = types->Types[c];
-        TColumn& dc = o->cols[(size_t)c];
-        memset(&dc, 0, sizeof(dc));
-        if (t.Kind == QDB_KIND_STRING) {
-            destroy_owned(&*out);
-            return fail("stage: string columns are not supported"), -1;
-        }
-        size_t bytes = t.Kind == QDB_KIND_BOOL ? (size_t)((n + hc.DataBitOffset + 7) / 8) : (size_t)n * (size_t)t.Width;
-        dc.Data = (char*)stage(hc.Data, bytes);
-        dc.DataBitOffset = hc.DataBitOffset;
-        if (hc.Mask) {
-            dc.Mask = (const uint8_t*)stage(hc.Mask, (size_t)((n + hc.MaskBitOffset + 7) / 8));
-            dc.MaskBitOffset = hc.MaskBitOffset;
-        }
-        if (bytes && !dc.Data) return -1;
+        if (t.Kind == QDB_KIND_STRING) return fail("stage: string columns are not supported"), -1;
+        src.push_back(hc.Data);
+        bytes.push_back(t.Kind == QDB_KIND_BOOL ? (size_t)((n + hc.DataBitOffset + 7) / 8) : (size_t)n * (size_t)t.Width);
+        src.push_back(hc.Mask);
+        bytes.push_back(hc.Mask ? (size_t)((n + hc.MaskBitOffset + 7) / 8) : 0);
     }
-    memset(out, 0, sizeof(*out));
-    out->Columns = o->cols.data();
-    out->ColumnCount = host->ColumnCount;
-    out->RowCount = n;
-    if (host->Selection) out->Selection = (uint8_t*)stage(host->Selection, (size_t)n);
-    if (host->Hash) out->Hash = (uint64_t*)stage(host->Hash, (size_t)n * 8);
-    out->Destroy = destroy_owned;
-    out->Private = o;
-    out->RefCount = 1;
-    (void)mask_bytes;
+    src.push_back(host->Selection);
+    bytes.push_back(host->Selection ? (size_t)n : 0);
+    src.push_back(host->Hash);
+    bytes.push_back(host->Hash ? (size_t)n * 8 : 0);
+    // One pinned staging buffer, one device block per buffer, every copy
+    // queued before one synchronization.
+    size_t total = 0;
+    std::vector<size_t> off(src.size(), 0);
+    for (size_t i = 0; i < src.size(); ++i) {
+        off[i] = total;
+        if (src[i] && bytes[i]) total += (bytes[i] + 255) / 256 * 256;
+    }
+    OwnedRowSet* o = init_owned_rowset(out, n, (int)host->ColumnCount);
+    std::vector<char*> dev(src.size(), nullptr);
+    char* pin = total ? (char*)halloc_pinned(total) : nullptr;
+    bool ok = total == 0 || pin;
+    for (size_t i = 0; ok && i < src.size(); ++i) {
+        if (!src[i] || !bytes[i]) continue;
+        dev[i] = (char*)dalloc(bytes[i]);
+        ok = dev[i] != nullptr;
+        if (!ok) break;
+        o->dev.push_back(dev[i]);
+        memcpy(pin + off[i], src[i], bytes[i]);
+        ok = hip_ok(hipMemcpyAsync(dev[i], pin + off[i], bytes[i], hipMemcpyHostToDevice, stream()), "stage copy");
+    }
+    ok = ok && sync("stage");
+    hfree_pinned(pin);
+    if (!ok) {
+        destroy_builder(out);
+        memset(out, 0, sizeof(*out));
+        return -1;
+    }
+    for (int64_t c = 0; c < host->ColumnCount; ++c) {
+        const TColumn& hc = host->Columns[c];
+        TColumn& dc = o->cols[(size_t)c];
+        dc.Data = dev[2 * (size_t)c];
+        dc.DataBitOffset = hc.DataBitOffset;
+        dc.Mask = (const uint8_t*)dev[2 * (size_t)c + 1];
+        dc.MaskBitOffset = hc.Mask ? hc.MaskBitOffset : 0;
+    }
+    const size_t k = 2 * (size_t)host->ColumnCount;
+    out->Selection = (uint8_t*)dev[k];
+    out->Hash = (uint64_t*)dev[k + 1];
+    if (!publish_rowset(out)) return -1;
+    out->Destroy = qdb_gpu_destroy_kernel_rowset;
     return 0;
 }
 
 extern "C" int qdb_gpu_download_rowset(const TRowSet* dev, const QdbValueType* types, TRowSet* out)
 {
     clear_error();
+    memset(out, 0, sizeof(*out));
     if (!ensure_init()) return -1;
     const int64_t n = dev->RowCount;
-    OwnedRowSet* o = new OwnedRowSet();
-    o->cols.resize((size_t)dev->ColumnCount);
     // Every data and mask buffer, then one download for all of them.
     std::vector<const void*> src;
     std::vector<size_t> bytes;
@@ -746,20 +910,20 @@ extern "C" int qdb_gpu_download_rowset(const TRowSet* dev, const QdbValueType* t
         src.push_back(dc.Mask);
         bytes.push_back(dc.Mask ? (size_t)((n + dc.MaskBitOffset + 7) / 8) : 0);
     }
+    OwnedRowSet* o = init_owned_rowset(out, n, (int)dev->ColumnCount);
     std::vector<void*> host;
     const bool fetched = download_ranges(src, bytes, &host);
     for (void* p : host) {
         if (p) o->host.push_back(p);
     }
     if (!fetched) {
-        for (void* p : o->host) free(p);
-        delete o;
+        destroy_builder(out);
+        memset(out, 0, sizeof(*out));
         return -1;
     }
     for (int64_t c = 0; c < dev->ColumnCount; ++c) {
         const TColumn& dc = dev->Columns[c];
         TColumn& hc = o->cols[(size_t)c];
-        memset(&hc, 0, sizeof(hc));
         hc.Data = (char*)host[2 * (size_t)c];
         hc.DataBitOffset = dc.DataBitOffset;
         if (dc.Mask) {
@@ -767,13 +931,8 @@ extern "C" int qdb_gpu_download_rowset(const TRowSet* dev, const QdbValueType* t
             hc.MaskBitOffset = dc.MaskBitOffset;
         }
     }
-    memset(out, 0, sizeof(*out));
-    out->Columns = o->cols.data();
-    out->ColumnCount = dev->ColumnCount;
-    out->RowCount = n;
-    out->Destroy = destroy_owned;
-    out->Private = o;
-    out->RefCount = 1;
+    if (!publish_rowset(out)) return -1;
+    out->Destroy = qdb_gpu_destroy_kernel_rowset;
     return 0;
 }
 

@@ -36,7 +36,8 @@ class ValueType:
     precision: int = 0  # decimals only
 
     def to_c(self) -> abi.QdbValueType:
-        return abi.QdbValueType(self.kind, self.width, self.scale, self.nullable)
+        return abi.QdbValueType(self.kind, self.width, self.scale, self.nullable,
+                                self.precision if self.kind == abi.DECIMAL else 0)
 
     def with_nullable(self, nullable: bool) -> "ValueType":
         return ValueType(self.kind, self.width, self.scale, nullable, self.precision)
@@ -461,7 +462,7 @@ class AggSpec:
     types: Sequence[ValueType]
     keys: Sequence[int]
     aggs: Sequence[tuple[int, int]]
-    global_agg: bool = False
+    empty_input_groups: Sequence[int] = ()
     output_on_host: bool = False
     predicate: Optional[tuple[Program, int]] = None
     computed: Sequence[tuple[Program, int]] = field(default_factory=list)
@@ -475,7 +476,9 @@ class AggSpec:
         s.NumAggs = len(self.aggs)
         for i, (f, c) in enumerate(self.aggs):
             s.Aggs[i] = abi.QdbAggregateFn(f, c)
-        s.Global = int(self.global_agg)
+        s.NumEmptyInputGroups = len(self.empty_input_groups)
+        for i, g in enumerate(self.empty_input_groups):
+            s.EmptyInputGroups[i] = g
         s.OutputOnHost = int(self.output_on_host)
         if self.predicate is not None:
             s.HasPredicate = 1

@@ -45,7 +45,7 @@ def _abi():
 
 
 def value_type_c(t):
-    return _abi().QdbValueType(t.kind, t.width, t.scale, bool(t.nullable))
+    return _abi().QdbValueType(t.kind, t.width, t.scale, bool(t.nullable), t.precision if t.kind == DECIMAL else 0)
 
 
 def schema_c(types):
@@ -122,6 +122,7 @@ class AggCase:
     global_agg: bool = False
     computed: list = field(default_factory=list)  # [(Prog, result, T)]
     predicate: Optional[tuple] = None             # (Prog, result)
+    empty_groups: list = field(default_factory=list)
 
     def to_c(self):
         abi = _abi()
@@ -133,7 +134,9 @@ class AggCase:
         s.NumAggs = len(self.aggs)
         for i, (f, c) in enumerate(self.aggs):
             s.Aggs[i] = abi.QdbAggregateFn(f, c)
-        s.Global = int(self.global_agg)
+        s.NumEmptyInputGroups = len(self.empty_groups)
+        for i, g in enumerate(self.empty_groups):
+            s.EmptyInputGroups[i] = g
         if self.predicate is not None:
             s.HasPredicate = 1
             s.Predicate = self.predicate[0].build(self.predicate[1])
@@ -154,7 +157,7 @@ class AggCase:
                 out.append(T(SIGNED, 8))
                 continue
             t = types[c]
-            nullable = t.nullable or self.global_agg
+            nullable = t.nullable or self.global_agg or bool(self.empty_groups)
             if t.kind == FLOAT:
                 out.append(T(FLOAT, 8, nullable=nullable))
             elif t.kind == DECIMAL:
@@ -162,6 +165,27 @@ class AggCase:
             else:
                 out.append(T(SIGNED, 8, nullable=nullable))
         return out
+
+
+def lower_global(case: AggCase, batches):
+    """A global aggregate (no GROUP BY) as the reference plans it
+    (qdb/plan/build.cpp:2461-2524): the input gains the constant i64 column
+    `__group__` = 1 after its own columns, the aggregate groups by it and seeds
+    the group 1 that exists over an empty input. Returns the lowered case,
+    the extended batches and whether the leading key column is to be dropped
+    from the result."""
+    if not case.global_agg:
+        return case, batches, False
+    nin = len(case.types)
+    aggs = [(f, c + 1 if c >= nin else c) for f, c in case.aggs]
+    lowered = AggCase(list(case.types) + [T(SIGNED, 8)], [nin], aggs, computed=case.computed, predicate=case.predicate,
+                      empty_groups=[1])
+    out = []
+    for b in batches:
+        n = len(b.columns[0]) if b.columns else 0
+        valid = None if b.valid is None else list(b.valid) + [None]
+        out.append(Batch(list(b.types) + [T(SIGNED, 8)], list(b.columns) + [np.ones(n, np.int64)], valid, b.selection))
+    return lowered, out, True
 
 
 @dataclass
@@ -214,7 +238,9 @@ class OracleBackend:
         return self.O.HostRowSet(b.types, [np_column(t, c) for t, c in zip(b.types, b.columns)], b.valid, b.selection)
 
     def aggregate(self, case: AggCase, batches):
-        return self.O.aggregate(case.to_c(), [self.host(b) for b in batches], fsum_mode=self.fsum_mode)
+        case, batches, strip = lower_global(case, batches)
+        res = self.O.aggregate(case.to_c(), [self.host(b) for b in batches], fsum_mode=self.fsum_mode)
+        return res[1:] if strip else res
 
     def join(self, case: JoinCase, build_batches, probe: Batch):
         """(pairs, output rows) of a join; output rows gathered on the host."""
@@ -266,6 +292,11 @@ class GpuBackend:
         return H.DeviceRowSet(types, cols, b.valid, b.selection, rows=len(b.columns[0]) if b.columns else 0)
 
     def aggregate(self, case: AggCase, batches, capacity=8):
+        case, batches, strip = lower_global(case, batches)
+        res = self._aggregate(case, batches, capacity)
+        return res[1:] if strip else res
+
+    def _aggregate(self, case: AggCase, batches, capacity=8):
         H = self.H
         kern = H.AggregateKernels.__new__(H.AggregateKernels)
         kern.spec = None

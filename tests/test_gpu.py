@@ -519,6 +519,54 @@ def test_group_hash_matches_rh_hash(gpu_backend):
         q.lib().qdb_gpu_release_slot(q.abi.SLOT_HASH, slot)
 
 
+def test_group_hash_known_answers(gpu_backend):
+    """The group-hash entry point returns the reference's rh_hash values,
+    worked out by hand from qdb/kernel/gen.cpp:233-379 over the key struct
+    (a NOT NULL BIGINT 1; a nullable BIGINT 5 and NULL)."""
+    q = gpu_backend.q
+    for types, cols, valid, want in (
+            ([T(SIGNED, 8)], [np.array([1], np.int64)], None, [0xE61C480508B75932]),
+            ([T(SIGNED, 8, nullable=True)], [np.array([5, 5], np.int64)], [np.array([True, False])],
+             [0x33D914A28E354A67, 0xCD94BF3ECEF65C4F])):
+        d = gpu_backend.dev(Batch(types, cols, valid))
+        fn = C.c_void_p()
+        slot = q.check(q.lib().qdb_gpu_compile_group_hash(C.byref(gpu_backend.cast(cases.schema_c(types))), 1, C.byref(fn)), "hash")
+        out = gpu_backend.H.DeviceBuffer(len(want) * 8)
+        assert q.abi.GROUP_HASH_FN(fn.value)(C.byref(d.rs), out.ptr, (C.c_int64 * 1)(0))
+        assert [int(x) for x in out.to_numpy(np.uint64, len(want))] == want
+        q.lib().qdb_gpu_release_slot(q.abi.SLOT_HASH, slot)
+
+
+def test_group_by_float_keys_zero_and_nan(gpu_backend):
+    """GROUP BY a DOUBLE key holding +0, -0 and NaNs of several bit patterns:
+    +0/-0 form one group and every NaN one group (canonical float key bits,
+    qdb/kernel/gen.cpp:192-231), as in the oracle; keys are compared by
+    their canonical bits, since the group's stored key is whichever
+    representative arrived first."""
+    rng = np.random.default_rng(7)
+    n = 20_000
+    nan2 = np.frombuffer(np.uint64(0x7FF0000000000123).tobytes(), np.float64)[0]
+    nan3 = np.frombuffer(np.uint64(0xFFF8000000000000).tobytes(), np.float64)[0]
+    pool = np.array([0.0, -0.0, np.nan, nan2, nan3, 1.5, -2.25, np.inf, -np.inf])
+    key = pool[rng.integers(0, len(pool), n)]
+    v = rng.integers(-1000, 1000, n).astype(np.int64)
+    types = [T(FLOAT, 8), T(SIGNED, 8)]
+    case = AggCase(types, [0], [(AGG_COUNT_STAR, -1), (AGG_SUM, 1)])
+    b = Batch(types, [key, v])
+
+    def canon(rows):
+        out = []
+        for k, c, s in rows:
+            k = 0.0 if k == 0 else k
+            out.append(("nan" if k != k else k, c, s))
+        return sorted(out, key=lambda r: (str(r[0]), r[1]))
+
+    got = canon(result_rows(gpu_backend.aggregate(case, [b])))
+    want = canon(result_rows(OracleBackend().aggregate(case, [b])))
+    assert got == want
+    assert len(got) == 6
+
+
 # ---------------------------------------------------------------------------
 # The three workloads
 # ---------------------------------------------------------------------------

@@ -309,3 +309,78 @@ def test_groups_generator_zipf():
     assert abs(top - 1.0 / h) < 0.01
     assert vi.min() >= -1000000 and vi.max() <= 1000000
     assert abs(vf.std() - 1000.0) < 20.0
+
+
+# ---------------------------------------------------------------------------
+# rh_hash: known answers worked out by hand from qdb/kernel/gen.cpp:233-379
+# (HashKeyValue) over the key struct of qdb/kernel/aggregate_key.cpp:246-272.
+# ---------------------------------------------------------------------------
+
+def _key_hash(types, cols, valid, keys, rows=None):
+    import ctypes as C
+    from cases import value_type_c
+    hb = OracleBackend().host(Batch(types, cols, valid))
+    tt = (O.abi.QdbValueType * len(types))(*[value_type_c(t) for t in types])
+    kk = (C.c_int32 * len(keys))(*keys)
+    n = len(cols[0]) if rows is None else rows
+    return [int(O.LIB.oracle_key_hash(hb.rs.Columns, tt, kk, len(keys), i)) for i in range(n)]
+
+
+def test_key_hash_known_answers():
+    """Values derived from the generator's text, not from the oracle:
+    - BIGINT NOT NULL k = 1: struct {key_0}; h = 0 ^ (mix(1) + 0x9E3779B97F4A7C15)
+      = 0xe61c480508b75932;
+    - nullable BIGINT 5: struct {valid_0, key_0}; valid_0 = true hashes as
+      1 * 2685821657736338717, then key_0 = mix(5): 0x33d914a28e354a67;
+    - nullable BIGINT NULL: both field hashes 0: 0xcd94bf3ecef65c4f."""
+    assert _key_hash([T(SIGNED, 8)], [np.array([1], np.int64)], None, [0]) == [0xE61C480508B75932]
+    got = _key_hash([T(SIGNED, 8, nullable=True)], [np.array([5, 5], np.int64)], [np.array([True, False])], [0])
+    assert got == [0x33D914A28E354A67, 0xCD94BF3ECEF65C4F]
+
+
+def _py_mix(h):
+    M = (1 << 64) - 1
+    h ^= h >> 12
+    h ^= (h << 25) & M
+    h ^= h >> 27
+    return (h * 2685821657736338717) & M
+
+
+def _py_combine(h, f):
+    M = (1 << 64) - 1
+    return (h ^ ((f + 0x9E3779B97F4A7C15 + ((h << 6) & M) + (h >> 2)) & M)) & M
+
+
+def test_key_hash_struct_rules():
+    """Each rule of the generator checked on its own: the zero-extension of a
+    narrow signed key (i8 -1 hashes the bits 0xff), the i128 leaf's lo/hi
+    combine (gen.cpp:281-306), the bool leaf without a mix, and the ordered
+    combine of two fields."""
+    assert _key_hash([T(SIGNED, 1)], [np.array([-1], np.int8)], None, [0]) == [_py_combine(0, _py_mix(0xFF))]
+    v = -(1 << 100) + 12345
+    u = v & ((1 << 128) - 1)
+    lo, hi = _py_mix(u & ((1 << 64) - 1)), _py_mix(u >> 64)
+    leaf = lo ^ ((hi + 0x9E3779B97F4A7C15 + ((lo << 6) & ((1 << 64) - 1)) + (lo >> 2)) & ((1 << 64) - 1))
+    assert _key_hash([T(DECIMAL, 16, 2)], [[v]], None, [0]) == [_py_combine(0, leaf)]
+    assert _key_hash([T(3, 1)], [np.array([True, False])], None, [0]) == [
+        _py_combine(0, 2685821657736338717), _py_combine(0, 0)]
+    two = _key_hash([T(SIGNED, 4), T(SIGNED, 8)], [np.array([7], np.int32), np.array([9], np.int64)], None, [0, 1])
+    assert two == [_py_combine(_py_combine(0, _py_mix(7)), _py_mix(9))]
+    # Field order matters (the combine is ordered).
+    rev = _key_hash([T(SIGNED, 4), T(SIGNED, 8)], [np.array([7], np.int32), np.array([9], np.int64)], None, [1, 0])
+    assert rev != two
+
+
+def test_key_hash_invariants():
+    """NULL differs from every value of a nullable key (the valid_ field
+    enters the combine); +0 and -0 hash alike, as does every NaN bit pattern
+    (canonical float bits, gen.cpp:192-231); a non-null value of a nullable
+    key differs from the same value of a NOT NULL key."""
+    vals = np.array([0, 1, -1, 5], np.int64)
+    nul = _key_hash([T(SIGNED, 8, nullable=True)], [vals], [np.array([False] * 4)], [0])
+    val = _key_hash([T(SIGNED, 8, nullable=True)], [vals], [np.array([True] * 4)], [0])
+    assert len(set(nul)) == 1 and not set(nul) & set(val)
+    assert val != _key_hash([T(SIGNED, 8)], [vals], None, [0])
+    nans = np.array([0.0, -0.0, np.nan, -np.nan, np.frombuffer(np.uint64(0x7FF0000000000123).tobytes(), np.float64)[0], 1.5])
+    h = _key_hash([T(FLOAT, 8)], [nans], None, [0])
+    assert h[0] == h[1] and h[2] == h[3] == h[4] and len({h[0], h[2], h[5]}) == 3
