@@ -131,7 +131,7 @@ def bench_lineitem(steps, warmup, rows=6_000_000):
     def step():
         p = TAggregateProcessor(kern, initial_capacity=8)
         p.add(dev.rowset)
-        out["rows"] = W.lineitem_finalize(p.finish())
+        out["rows"] = W.lineitem_finalize_on_device(p, kern)
         p.close()
 
     for _ in range(warmup):

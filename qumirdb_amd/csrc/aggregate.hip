@@ -431,6 +431,16 @@ bool compile_plan(const QdbAggregateSpec& spec, AggPlan* P, bool on_device = tru
         }
     }
     if (spec.NumComputed < 0 || spec.NumComputed > QDB_PROJECT_MAX_OUTPUTS) return fail("bad computed count");
+    for (int k = 0; k < spec.NumComputed; ++k) {
+        // A computed DECIMAL(p <= 18) argument would be stored by the
+        // project operator with a row-error check (qdb_decimal64_narrow_
+        // checked); the fused update has no row-error channel, so such an
+        // argument runs through the project kernel instead.
+        const QdbValueType& t = spec.ComputedTypes[k];
+        if (t.Kind == QDB_KIND_DECIMAL && t.Width == 8) {
+            return fail("computed column %d is DECIMAL(p <= 18): run it through the project kernel", k);
+        }
+    }
 
     // Keys: the null bits of nullable fields first, then every field at its
     // natural alignment (a 16-byte field on a word boundary).

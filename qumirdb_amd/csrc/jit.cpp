@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <poll.h>
 #include <signal.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -16,6 +17,7 @@
 #include <unistd.h>
 
 #include <cerrno>
+#include <algorithm>
 #include <chrono>
 #include <mutex>
 #include <string>
@@ -109,16 +111,38 @@ bool write_all(int fd, const void* p, size_t n)
     return true;
 }
 
-bool read_all(int fd, void* p, size_t n)
+// Reads n bytes before `deadline`; false on end of file, error or time out
+// (*timed_out set).
+bool read_all(int fd, void* p, size_t n, std::chrono::steady_clock::time_point deadline, bool* timed_out)
 {
     char* c = (char*)p;
     while (n) {
+        const auto left = std::chrono::duration_cast<std::chrono::milliseconds>(deadline - std::chrono::steady_clock::now());
+        if (left.count() <= 0) {
+            *timed_out = true;
+            return false;
+        }
+        struct pollfd pfd = {fd, POLLIN, 0};
+        const int r = poll(&pfd, 1, (int)std::min<long long>(left.count(), 1000));
+        if (r < 0 && errno == EINTR) continue;
+        if (r < 0) return false;
+        if (r == 0) continue;
         const ssize_t k = read(fd, c, n);
+        if (k < 0 && errno == EINTR) continue;
         if (k <= 0) return false;
         c += k;
         n -= (size_t)k;
     }
     return true;
+}
+
+// Seconds a run-time compile may take (QDB_GPU_JIT_TIMEOUT_S, default 120);
+// past it the compiler process is killed and the generic kernels run.
+int jit_timeout_s()
+{
+    const char* e = getenv("QDB_GPU_JIT_TIMEOUT_S");
+    const int v = e ? atoi(e) : 0;
+    return v > 0 ? v : 120;
 }
 
 // Compiles `src` in a forked child that runs only the compiler and never
@@ -150,21 +174,28 @@ bool compile_source(const std::string& src, const std::string& arch, std::vector
         _exit(sent ? 0 : 3);
     }
     close(fds[1]);
+    const auto deadline = std::chrono::steady_clock::now() + std::chrono::seconds(jit_timeout_s());
+    bool timed_out = false;
     uint64_t hdr[3] = {0, 0, 0};
-    bool got = read_all(fds[0], hdr, sizeof(hdr));
+    bool got = read_all(fds[0], hdr, sizeof(hdr), deadline, &timed_out);
     if (got) {
         code->resize(hdr[1]);
         std::string e(hdr[2], '\0');
-        got = read_all(fds[0], code->data(), hdr[1]) && read_all(fds[0], &e[0], hdr[2]);
+        got = read_all(fds[0], code->data(), hdr[1], deadline, &timed_out) && read_all(fds[0], &e[0], hdr[2], deadline, &timed_out);
         *err = e;
     }
     close(fds[0]);
+    // A child that neither answered nor exited in time (a lock held by
+    // another thread when the process forked, a compiler that does not end)
+    // is killed, so the caller never waits on it for good.
+    if (timed_out) kill(pid, SIGKILL);
     int status = 0;
     while (waitpid(pid, &status, 0) < 0 && errno == EINTR) {
     }
     if (!got || !WIFEXITED(status) || WEXITSTATUS(status) != 0) {
         char b[96];
-        if (WIFSIGNALED(status)) snprintf(b, sizeof(b), "the run-time compiler ended with signal %d", WTERMSIG(status));
+        if (timed_out) snprintf(b, sizeof(b), "the run-time compiler took longer than %d s", jit_timeout_s());
+        else if (WIFSIGNALED(status)) snprintf(b, sizeof(b), "the run-time compiler ended with signal %d", WTERMSIG(status));
         else snprintf(b, sizeof(b), "the run-time compiler process failed (status %d)", status);
         *err = b;
         code->clear();

@@ -354,19 +354,20 @@ __global__ void __launch_bounds__(kBlock) k_build_insert(Slot<KW>* table, uint64
 // Per-slot row counts (for the scan) and the Bloom filter bits of every
 // distinct key.
 // Per-slot row counts (for the scan), the Bloom filter bits of every
-// distinct key, and per workgroup the number of distinct keys
-// (distinct_part[block]).
+// distinct key, and per workgroup the number of distinct keys and the most
+// rows one key has (part[2 * block], part[2 * block + 1]).
 template <int KW>
 __global__ void __launch_bounds__(kBlock) k_slot_counts(const Slot<KW>* table, uint64_t capacity, uint32_t* counts,
     uint32_t* bloom, uint32_t bloom_mask, unsigned long long* distinct_part)
 {
-    __shared__ unsigned long long wsum[kBlock / kWave];
-    unsigned long long used = 0;
+    __shared__ unsigned long long wsum[kBlock / kWave], wmax[kBlock / kWave];
+    unsigned long long used = 0, most = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < capacity; s += stride) {
         const Slot<KW> sl = table[s];
         counts[s] = sl.count;
         used += sl.count != 0u ? 1u : 0u;
+        most = sl.count > most ? sl.count : most;
         if (sl.count == 0u || !bloom) continue;
         uint64_t key[KW];
 #pragma unroll
@@ -375,13 +376,24 @@ __global__ void __launch_bounds__(kBlock) k_slot_counts(const Slot<KW>* table, u
         atomicOr(&bloom[bloom_word(h, bloom_mask)], bloom_bits(h));
     }
 #pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) used += shfl_xor_u64(used, o);
-    if (lane_id() == 0) wsum[threadIdx.x / kWave] = used;
+    for (int o = 1; o < kWave; o <<= 1) {
+        used += shfl_xor_u64(used, o);
+        const unsigned long long m = shfl_xor_u64(most, o);
+        most = m > most ? m : most;
+    }
+    if (lane_id() == 0) {
+        wsum[threadIdx.x / kWave] = used;
+        wmax[threadIdx.x / kWave] = most;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int w = 0; w < kBlock / kWave; ++w) t += wsum[w];
-        distinct_part[blockIdx.x] = t;
+        unsigned long long t = 0, m = 0;
+        for (int w = 0; w < kBlock / kWave; ++w) {
+            t += wsum[w];
+            m = wmax[w] > m ? wmax[w] : m;
+        }
+        distinct_part[2 * blockIdx.x] = t;
+        distinct_part[2 * blockIdx.x + 1] = m;
     }
 }
 
@@ -640,6 +652,8 @@ constexpr int kGeneric = 0, kDirectKey = 1, kDirectVal = 2;
 // in LDS, the whole hash table in LDS, or the dense direct table (the
 // membership bitmap in LDS as far as it fits, the ordinal table in HBM).
 constexpr int kTabHash = 0, kTabLds = 1, kTabDense = 2;
+// Pairs the first probe attempt allocates room for at most (2 GiB of pairs).
+constexpr unsigned long long kFirstPairBudget = 1ull << 27;
 
 // The key of selected probe row i; false when a key field is NULL.
 template <int KW, bool kDirect>
@@ -1506,6 +1520,7 @@ struct JoinBuild {
     uint32_t* dtab = nullptr;
     uint32_t bloom_words = 0;
     int64_t keys_distinct = 0;
+    int64_t max_key_rows = 0;   // most build rows of one key (a probe row's most matches)
     bool built = false;
     // The arena's row count stays on the device while batches are appended
     // (rows_dev); `rows` is exact once read back (rows_known), `rows_bound`
@@ -1912,6 +1927,7 @@ bool try_dense_build(JoinBuild* J)
     J->dmin = h_info[0];
     J->drange = range;
     J->keys_distinct = J->rows;
+    J->max_key_rows = J->rows > 0 ? 1 : 0;
     J->built = true;
     return true;
 }
@@ -1945,9 +1961,10 @@ bool do_build(JoinBuild* J)
     uint32_t* counts = (uint32_t*)dalloc(capacity * 4);
     uint32_t* start = (uint32_t*)dalloc(capacity * 4);
     uint32_t* perm = (uint32_t*)dalloc(nrows * 4);
-    unsigned long long* stats = (unsigned long long*)dalloc(16);  // distinct keys, duplicate flag
+    // [0] unused, [1] duplicate flag, [2] distinct keys, [3] most rows of one key
+    unsigned long long* stats = (unsigned long long*)dalloc(32);
     const int cgrid = grid_for((int64_t)capacity, kBlock, 8);
-    unsigned long long* dpart = (unsigned long long*)dalloc((size_t)cgrid * 8);
+    unsigned long long* dpart = (unsigned long long*)dalloc((size_t)cgrid * 16);
     uint64_t* new_keys = (uint64_t*)dalloc(nrows * KW * 8);
     int64_t* new_ord = (int64_t*)dalloc(nrows * 8);
     uint32_t* sorted_slot = nullptr;
@@ -2005,7 +2022,7 @@ bool do_build(JoinBuild* J)
             empty = J->kmin - 1;
         }
     }
-    bool ok = zero_fill(J->bloom, (size_t)words * 4) && zero_fill(stats, 16);
+    bool ok = zero_fill(J->bloom, (size_t)words * 4) && zero_fill(stats, 32);
     if (ok && use_empty) {
         hipLaunchKernelGGL(k_fill_slots, dim3(grid_for((int64_t)capacity, kBlock, 8)), dim3(kBlock), 0, stream(),
             (Slot<1>*)table, capacity, empty);
@@ -2023,19 +2040,21 @@ bool do_build(JoinBuild* J)
             words - 1u, dpart);
         ReduceOps ops;
         memset(&ops, 0, sizeof(ops));
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, stream(), dpart, cgrid, 1, 1, ops, stats);
+        ops.op[1] = 4;  // unsigned max
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, stream(), dpart, cgrid, 2, 2, ops, stats + 2);
         ok = hip_ok(hipGetLastError(), "build insert") && exclusive_scan_u32(counts, start, (int64_t)capacity);
     }
-    unsigned long long h_stats[2] = {0, 0};
+    unsigned long long h_stats[4] = {0, 0, 0, 0};
     ok = ok && hip_ok(hipGetLastError(), "build table")
-        && hip_ok(hipMemcpyAsync(h_stats, stats, 16, hipMemcpyDeviceToHost, stream()), "build stats") && sync("build");
+        && hip_ok(hipMemcpyAsync(h_stats, stats, 32, hipMemcpyDeviceToHost, stream()), "build stats") && sync("build");
     if (!ok) {
         release();
         dfree(new_keys);
         dfree(new_ord);
         return false;
     }
-    J->keys_distinct = (int64_t)h_stats[0];
+    J->keys_distinct = (int64_t)h_stats[2];
+    J->max_key_rows = (int64_t)h_stats[3];
     const bool dups = (h_stats[1] & 0xffffffffull) != 0;
     if (rows > 0) {
         TimedRegion timer("join_build_place");
@@ -2321,14 +2340,20 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
         A.dtab = J->dtab;
         lds = (size_t)A.lds_words * 4;
     }
-    // Pair capacity: with distinct build keys every probe row matches at
-    // most once; otherwise the duplicates bound the extra matches.
-    unsigned long long cap = (unsigned long long)n;
-    if (J->keys_distinct != J->rows) cap = (unsigned long long)n * (unsigned long long)(J->rows - J->keys_distinct + 1);
-    if (A.emit_unmatched) cap += (unsigned long long)n;
+    // Pair capacity: a probe row has at most max_key_rows matches (one
+    // pair when unmatched in an outer join). The first attempt takes that
+    // bound up to kFirstPairBudget pairs; the kernels count every pair
+    // whatever the capacity, so a larger result is probed again into a
+    // buffer of its exact size.
+    const u128 bound = (u128)(uint64_t)n * (u128)(uint64_t)(J->max_key_rows > 1 ? J->max_key_rows : 1)
+        + (A.emit_unmatched ? (u128)(uint64_t)n : (u128)0);
+    const unsigned long long budget = (unsigned long long)n > kFirstPairBudget ? (unsigned long long)n : kFirstPairBudget;
+    unsigned long long cap = bound < (u128)budget ? (unsigned long long)bound : budget;
     for (int attempt = 0; attempt < 2; ++attempt) {
         if ((unsigned long long)pairs->Capacity < cap) {
+            if (cap > (unsigned long long)(SIZE_MAX / 16)) return fail("probe: %llu pairs do not fit in memory", cap);
             dfree(pairs->Data);
+            pairs->Capacity = 0;
             pairs->Data = (int64_t*)dalloc((size_t)cap * 16);
             if (!pairs->Data) return false;
             pairs->Capacity = (int64_t)cap;
@@ -2375,7 +2400,7 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
             J->fused_ready = A.nfused > 0;
             return true;
         }
-        cap = count;  // cannot happen with the bound above; retry once
+        cap = count;  // more pairs than the first attempt's buffer: again, at the exact size
     }
     return fail("probe pair buffer overflow");
 }

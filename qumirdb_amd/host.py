@@ -196,6 +196,24 @@ class Program:
     def not_(self, a): return self._emit(abi.EXPR_NOT, abi.REG_BOOL, ValueType(abi.BOOL, 1, nullable=True), a=a)
     def is_null(self, a): return self._emit(abi.EXPR_IS_NULL, abi.REG_BOOL, ValueType(abi.BOOL, 1), a=a)
 
+    def to_f64(self, a) -> int:
+        """CAST(decimal or integer AS DOUBLE): the nearest double, ties to
+        even (qdb_decimal_to_f64, qdb/modules/qumirdb.oz:983-1046)."""
+        t = self.types[a]
+        return self._emit(abi.EXPR_TO_F64, abi.REG_F64, ValueType(abi.FLOAT, 8, nullable=t.nullable), a=a,
+                          imm_lo=t.scale if t.kind == abi.DECIMAL else 0)
+
+    def div(self, a, b) -> int:
+        """DOUBLE division (IEEE)."""
+        nullable = self.types[a].nullable or self.types[b].nullable
+        return self._emit(abi.EXPR_DIV, abi.REG_F64, ValueType(abi.FLOAT, 8, nullable=nullable), a=a, b=b)
+
+    def avg(self, sum_col: int, count_col: int) -> int:
+        """AVG as the reference lowers it (qdb/plan/sql_lowering.cpp:799-814):
+        SUM / COUNT with both converted to DOUBLE. A group whose COUNT is 0
+        has a NULL SUM, so its AVG is NULL."""
+        return self.div(self.to_f64(self.col(sum_col)), self.to_f64(self.col(count_col)))
+
     def build(self, result: int) -> abi.QdbExprProgram:
         p = abi.QdbExprProgram()
         if len(self.insns) > abi.MAX_INSNS:
@@ -352,6 +370,20 @@ class DeviceRowSet:
         return C.byref(self.rs)
 
 
+class RowSetView:
+    """A device TRowSet made by the library (an aggregate's or a join's
+    output), seen as a kernel input; the owner releases it."""
+
+    def __init__(self, rs: abi.TRowSet, types: Sequence[ValueType]):
+        self.rs = rs
+        self.types = list(types)
+        self.rows = int(rs.RowCount)
+
+    @property
+    def ptr(self):
+        return C.byref(self.rs)
+
+
 def i128_array(values) -> np.ndarray:
     dt = np.dtype([("lo", "<u8"), ("hi", "<i8")])
     out = np.empty(len(values), dtype=dt)
@@ -448,7 +480,19 @@ class ProjectKernel:
                 m.zero()
                 bufs.append(m)
                 ptrs[k + i] = m.ptr
-        self.fn(C.byref(rowset.rs), ptrs, None, None, input_sel.ptr if input_sel is not None else None)
+        sel = input_sel.ptr if input_sel is not None else None
+        self.fn(C.byref(rowset.rs), ptrs, None, None, sel)
+        if lib().qdb_gpu_take_row_error():
+            # A row failed (qdb_decimal64_overflow): as the reference's
+            # project operator does (qdb/exec/project_exec.cpp:71-81), the
+            # rows of the selection are evaluated again and only a failure
+            # among them is an error. Here the first pass already covered
+            # the selection when one was given.
+            if sel is None and rowset.rs.Selection:
+                self.fn(C.byref(rowset.rs), ptrs, None, None, rowset.rs.Selection)
+                if not lib().qdb_gpu_take_row_error():
+                    return bufs, ptrs
+            raise RuntimeError(f"project: {lib().qdb_gpu_last_error().decode()}")
         err = lib().qdb_gpu_last_error()
         if err:
             raise RuntimeError(f"project: {err.decode()}")
@@ -580,6 +624,14 @@ class TAggregateProcessor:
             return download_rowset(out, self.kernels.out_types)
         finally:
             lib().qdb_gpu_rowset_release(C.byref(out))
+
+    def finish_raw(self) -> abi.TRowSet:
+        """Finishes into a device-resident rowset (adopted: release it with
+        lib().qdb_gpu_rowset_release)."""
+        out = abi.TRowSet()
+        if lib().qdb_gpu_agg_processor_finish(self.p, C.byref(out)) < 0:
+            raise RuntimeError((lib().qdb_gpu_exec_last_error() or b"").decode())
+        return out
 
     def finish_in_hbm(self) -> int:
         """Finishes into a device-resident rowset, as a downstream GPU

@@ -130,7 +130,7 @@ class Server:
 
     def _agg_result(self, shape: str, proc: TAggregateProcessor):
         if shape == "lineitem":
-            return W.lineitem_finalize(proc.finish())
+            return W.lineitem_finalize_on_device(proc, proc.kernels)
         if self.full:
             cols = proc.finish()
             return sorted(zip(*[c[0].tolist() for c in cols]))

@@ -17,6 +17,7 @@ decimal_to_f64(SUM) / f64(COUNT) over the result.
 """
 from __future__ import annotations
 
+import ctypes as C
 from fractions import Fraction
 
 import numpy as np
@@ -94,6 +95,49 @@ def lineitem_plain_spec() -> AggSpec:
     their own operators, like the reference's plan filter -> project ->
     aggregate, test/regtest/goldens/tpch/q1.plan)."""
     return AggSpec(types=lineitem_projected_types(), keys=[FLAG, STATUS], aggs=LINEITEM_AGGS)
+
+
+def lineitem_finalize_on_device(proc, kernels):
+    """Finishes the lineitem aggregate in HBM and computes AVG(quantity) and
+    AVG(discount) there with the project kernel over the aggregate's output
+    (SUM / COUNT converted to DOUBLE, as the reference lowers AVG); returns
+    the rows of lineitem_finalize."""
+    from .host import ProjectKernel, RowSetView, download_rowset
+    types = kernels.out_types
+    out = proc.finish_raw()
+    try:
+        view = RowSetView(out, types)
+        avg_q, avg_d = Program(types), Program(types)
+        ra = avg_q.avg(6, 7)
+        rd = avg_d.avg(8, 9)
+        kern = _avg_kernel(types, [(avg_q, ra), (avg_d, rd)])
+        bufs, _ = kern(view)
+        n = view.rows
+        aq = bufs[0].to_numpy(np.float64, n)
+        ad = bufs[2 if kern.out_types[0].nullable else 1].to_numpy(np.float64, n)
+        cols = download_rowset(out, types)
+    finally:
+        lib().qdb_gpu_rowset_release(C.byref(out))
+    (flag, _), (status, _) = cols[0], cols[1]
+    vals = [c[0] for c in cols[2:]]
+    rows = []
+    for g in range(len(flag)):
+        sq, sp, sd, sc, _, _, _, _, cnt = [int(v[g]) for v in vals]
+        rows.append((int(flag[g]), int(status[g]), sq, sp, sd, sc, float(aq[g]), float(ad[g]), cnt))
+    rows.sort()
+    return rows
+
+
+_AVG_KERNELS: dict = {}
+
+
+def _avg_kernel(types, outputs):
+    """The AVG projection of the lineitem output, compiled once per process."""
+    from .host import ProjectKernel
+    key = tuple((t.kind, t.width, t.scale, t.nullable) for t in types)
+    if key not in _AVG_KERNELS:
+        _AVG_KERNELS[key] = ProjectKernel(types, outputs)
+    return _AVG_KERNELS[key]
 
 
 def lineitem_finalize(columns):

@@ -383,8 +383,20 @@ class GpuBackend:
                 masks[i] = m
                 ptrs[k + i] = m.ptr
         insel = H.DeviceBuffer.from_numpy(np.asarray(batch.selection, np.uint8)) if batch.selection is not None else None
-        f(C.byref(d.rs), ptrs, None, None, insel.ptr if insel is not None else None)
-        self.q.lib().qdb_gpu_release_slot(abi.SLOT_PROJECT, slot)
+        lib = self.q.lib()
+        try:
+            # The reference's project operator (qdb/exec/project_exec.cpp:
+            # 71-81): every row with row errors deferred; after a row error,
+            # the selected rows again, now raising.
+            f(C.byref(d.rs), ptrs, None, None, None)
+            if lib.qdb_gpu_take_row_error():
+                f(C.byref(d.rs), ptrs, None, None, insel.ptr if insel is not None else None)
+                if lib.qdb_gpu_take_row_error():
+                    raise RuntimeError(self.q.last_error())
+            elif insel is not None:
+                f(C.byref(d.rs), ptrs, None, None, insel.ptr)
+        finally:
+            lib.qdb_gpu_release_slot(abi.SLOT_PROJECT, slot)
         out = []
         for i, (_, _, t) in enumerate(outputs):
             if t.kind == 3:
@@ -397,6 +409,64 @@ class GpuBackend:
             valid = H.unpack_bitmap(masks[i].to_numpy(np.uint8, (d.rows + 7) // 8), d.rows) if i in masks else None
             out.append((vals, valid))
         return out
+
+
+class ProtocolBackend(GpuBackend):
+    """The HIP library's aggregate kernels driven by the reference's own
+    protocol (tests/native/ref_protocol.c: a task heap with owner headers,
+    TAggregateProcessor's Init / Update / Size == finalized check, and the
+    AdoptKernelOwnedRowSet / DestroyKernelOwnedRowSet sequence), with the
+    outputs in host memory from that heap (OutputOnHost = 1)."""
+
+    def __init__(self):
+        import os
+        super().__init__()
+        self.h = C.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "native", "libref_protocol.so"))
+        self.h.harness_install.argtypes = [C.c_void_p]
+        self.h.harness_counters.argtypes = [C.c_void_p]
+        self.h.harness_aggregate.restype = C.c_int64
+        self.h.harness_aggregate.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]
+        self.h.harness_install(C.cast(self.q.lib().qdb_gpu_set_host_heap, C.c_void_p))
+
+    def counters(self):
+        out = (C.c_int64 * 3)()
+        self.h.harness_counters(out)
+        return tuple(out)
+
+    def uninstall(self):
+        """Back to the library's default heap, once no block of this one is
+        alive (a block must go back to the heap it came from)."""
+        if self.counters()[0] != 0:
+            return
+        abi = self.q.abi
+        self.q.lib().qdb_gpu_set_host_heap(abi.HEAP_ALLOC_FN(0), abi.HEAP_FREE_FN(0), abi.HEAP_DETACH_FN(0))
+
+    def _aggregate(self, case: AggCase, batches, capacity=8):
+        from oracle.oracle import read_host_rowset
+        H, abi, lib = self.H, self.q.abi, self.q.lib()
+        spec = self.cast(case.to_c())
+        spec.OutputOnHost = 1
+        k = abi.QdbAggregateKernels()
+        self.q.check(lib.qdb_gpu_compile_aggregate(C.byref(spec), C.byref(k)), "compile_aggregate")
+        types = [T(t.Kind, t.Width, t.Scale, bool(t.Nullable)) for t in k.OutputTypes[: k.NumOutputColumns]]
+        devs = [self.dev(b) for b in batches]
+        arr = (C.POINTER(abi.TRowSet) * max(1, len(devs)))(*[C.pointer(d.rs) for d in devs])
+        before = self.counters()
+        out = abi.TRowSet()
+        rows = self.h.harness_aggregate(C.byref(k), arr, len(devs), capacity, C.byref(out))
+        lib.qdb_gpu_release_slot(abi.SLOT_AGGREGATE, k.Slot)
+        assert rows != -2, "agg_finish_rowset returned a row count other than the table's Size"
+        assert rows >= 0, f"aggregate failed: {self.q.last_error()}"
+        adopted = self.counters()
+        # Adopted: every owner (and the list) detached from the task heap,
+        # each one a block of that heap.
+        assert adopted[1] == before[1] and adopted[2] == 0, (before, adopted)
+        assert out.Destroy, "the adoption installs the destroy callback"
+        res = read_host_rowset(out, types)
+        lib.qdb_gpu_rowset_release(C.byref(out))
+        after = self.counters()
+        assert after[0] == before[0] and after[2] == 0, f"blocks left or foreign frees: {before} -> {after}"
+        return res
 
 
 def gather_rows(case: JoinCase, build_batches, probe: Batch, pairs):
@@ -438,9 +508,27 @@ def table_batch(name, selection=None):
     return names, Batch(types, values, valid, selection)
 
 
-def avg_f64(sum_unscaled, scale, count):
-    from oracle.oracle import decimal_to_f64
-    return None if sum_unscaled is None or not count else decimal_to_f64(sum_unscaled, scale) / float(count)
+def backend_avg(backend, sums, counts, scale, kind=DECIMAL, width=16):
+    """AVG over finished groups as the reference lowers it
+    (qdb/plan/sql_lowering.cpp:799-814): SUM / COUNT, both converted to DOUBLE
+    (the decimal correctly rounded, qdb_decimal_to_f64) and divided, as a
+    projection that the backend runs (on the GPU: the project kernel)."""
+    E = _abi()
+    st = T(kind, width, scale, True, 38 if width == 16 else 18)
+    ct = T(SIGNED, 8)
+    types = [st, ct]
+    p = Prog(types)
+    num = p._emit(E.EXPR_TO_F64, E.REG_F64, T(FLOAT, 8, nullable=True), a=p.col(0), lo=scale)
+    den = p._emit(E.EXPR_TO_F64, E.REG_F64, T(FLOAT, 8), a=p.col(1), lo=0)
+    q = p.op(E.EXPR_DIV, num, den, T(FLOAT, 8, nullable=True), cls=E.REG_F64)
+    out_t = T(FLOAT, 8, nullable=True)
+    valid = np.array([s_ is not None and bool(c) for s_, c in zip(sums, counts)])
+    svals = [int(s_ or 0) for s_ in sums]
+    if width != 16:
+        svals = np.array(svals, dtype=np.int64)
+    batch = Batch(types, [svals, np.array([int(c) for c in counts], np.int64)], [valid, None])
+    (vals, ok), = backend.project(types, [(p, q, out_t)], batch)
+    return [float(v) if ok[i] else None for i, v in enumerate(vals)]
 
 
 def golden_decimal_aggregates(backend):
@@ -453,7 +541,9 @@ def golden_decimal_aggregates(backend):
     r = result_rows(backend.aggregate(case, [b]))
     assert len(r) == 1
     n, c2, s2, lo2, hi2, c10, s10, lo10, hi10 = r[0]
-    got = [(n, c2, s2, avg_f64(s2, 2, c2), lo2, hi2, c10, s10, avg_f64(s10, 10, c10), lo10, hi10)]
+    (a2,) = backend_avg(backend, [s2], [c2], 2)
+    (a10,) = backend_avg(backend, [s10], [c10], 10)
+    got = [(n, c2, s2, a2, lo2, hi2, c10, s10, a10, lo10, hi10)]
     return got, golden_rows(FIX["goldens"]["decimal_parquet/aggregates"])
 
 
@@ -509,7 +599,8 @@ def golden_multiply_sum(backend):
     case = AggCase(types2, [G], [(AGG_SUM, nin), (AGG_SUM, nin + 1), (AGG_SUM, nin + 2), (AGG_COUNT, nin + 2),
                                 (AGG_COUNT_STAR, -1)], computed=computed)
     r = result_rows(backend.aggregate(case, [Batch(types2, values2, valid2)]))
-    got = sorted([(gk, s, dsc, avg_f64(sab, 2, cab), n) for gk, s, dsc, sab, cab, n in r], key=sort_key)
+    avgs = backend_avg(backend, [row[3] for row in r], [row[4] for row in r], 2)
+    got = sorted([(gk, s, dsc, a, n) for (gk, s, dsc, sab, cab, n), a in zip(r, avgs)], key=sort_key)
     return got, golden_rows(FIX["goldens"]["decimal_result/multiply_sum"])
 
 
@@ -630,7 +721,8 @@ def golden_aggregate_empty(backend, table, golden):
     case = AggCase(b.types, [], [(AGG_COUNT_STAR, -1), (AGG_COUNT, 0), (AGG_SUM, 0), (AGG_MIN, 0), (AGG_MAX, 0)], global_agg=True)
     r = result_rows(backend.aggregate(case, [b]))
     c, cn, s, mn, mx = r[0]
-    got = [(c, cn, s, mn, mx, None if s is None or not cn else float(s) / float(cn))]
+    (avg,) = backend_avg(backend, [s], [cn], 0, kind=SIGNED, width=8)
+    got = [(c, cn, s, mn, mx, avg)]
     return got, golden_rows(FIX["goldens"][golden])
 
 
@@ -713,6 +805,60 @@ def golden_multiply_project(backend):
     return got, golden_rows(FIX["goldens"]["decimal_result/multiply"])
 
 
+def golden_storage_error(backend, golden, table, column, scale, nullable):
+    """decimal_result/storage_overflow (CAST(b AS DECIMAL(18,2)) over t) and
+    storage_unsigned (CAST(u AS DECIMAL(18,0)) over u): a value with more
+    than 18 digits cannot be stored as DECIMAL(18, s), an 8-byte decimal, so
+    the query fails with the reference's runtime error."""
+    names, types, values, valid = table_columns(FIX["tables"][table])
+    c = names.index(column)
+    out_t = T(DECIMAL, 8, scale, nullable, 18)
+    p = Prog(types)
+    r = p.scale_up(p.col(c), scale, out_t) if scale else p.col(c)
+    try:
+        backend.project(types, [(p, r, out_t)], Batch(types, values, valid))
+        got = None
+    except RuntimeError as e:
+        got = "Runtime error: " + str(e)
+    return got, FIX["errors"][golden]["text"]
+
+
+def golden_storage_boundary(backend):
+    """decimal_result/storage_boundary: CAST(a AS DECIMAL(9,2)) squared is a
+    DECIMAL(18,4) held in 8 bytes and fits; the constant 999999999 *
+    9999999999 is DECIMAL(19,0) in 16 bytes; CAST(a AS DECIMAL(10,2)) < b
+    compares at scale 2. (The CASE column of the query is not an operator
+    of this path and is not compared.)"""
+    names, types, values, valid = _t_columns()
+    A, B = names.index("a"), names.index("b")
+    E = _abi()
+    a2 = T(DECIMAL, 8, 2, True, 9)
+    p18 = T(DECIMAL, 8, 4, True, 18)
+    p19 = T(DECIMAL, 16, 0, False, 19)
+
+    def build(fn, t):
+        p = Prog(types)
+        return (p, fn(p), t)
+
+    outs = [
+        build(lambda p: p.op(E.EXPR_MUL, p.scale_up(p.col(A), 2, a2), p.scale_up(p.col(A), 2, a2), p18), p18),
+        build(lambda p: p.op(E.EXPR_MUL, p.const(999999999, T(DECIMAL, 8, 0, False, 9)),
+                             p.const(9999999999, T(DECIMAL, 8, 0, False, 10)), p19), p19),
+        build(lambda p: p.cmp(E.CMP_LT, p.scale_up(p.col(A), 2, T(DECIMAL, 8, 2, True, 10)), p.scale_up(p.col(B), 2, T(DECIMAL, 16, 2, True, 21))),
+              T(3, 1, nullable=True)),
+    ]
+    res = backend.project(types, outs, Batch(types, values, valid))
+    ids = values[names.index("id")]
+    got = []
+    for r in range(len(ids)):
+        row = [int(ids[r])]
+        for vals, v in res:
+            row.append(None if (v is not None and not v[r]) else (bool(vals[r]) if isinstance(vals[r], (bool, np.bool_)) else int(vals[r])))
+        got.append(tuple(row))
+    want = [tuple(w[:4]) for w in golden_rows(FIX["goldens"]["decimal_result/storage_boundary"])]
+    return got, want
+
+
 AGG_GOLDENS = {
     "decimal_aggregates": golden_decimal_aggregates,
     "decimal_group_by": golden_decimal_group_by,
@@ -735,7 +881,12 @@ JOIN_GOLDENS = {
     "decimal_join": golden_decimal_join,
 }
 
-PROJECT_GOLDENS = {"multiply": golden_multiply_project}
+PROJECT_GOLDENS = {
+    "multiply": golden_multiply_project,
+    "storage_boundary": golden_storage_boundary,
+    "storage_overflow": lambda b: golden_storage_error(b, "decimal_result/storage_overflow", "decimal_result.t", "b", 2, True),
+    "storage_unsigned": lambda b: golden_storage_error(b, "decimal_result/storage_unsigned", "decimal_result.u", "u", 0, False),
+}
 
 
 def random_agg_case(seed, n):

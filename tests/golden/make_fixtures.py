@@ -16,6 +16,7 @@ TABLES = {
     "decimal_parquet.wide": "cases/decimal_parquet/tables/wide.json",
     "decimal_parquet.wide_dim": "cases/decimal_parquet/tables/wide_dim.json",
     "decimal_result.t": "cases/decimal_result/tables/t.json",
+    "decimal_result.u": "cases/decimal_result/tables/u.json",
     "aggregate_row_layout.t": "cases/aggregate_row_layout/tables/t.json",
     "aggregate_empty.empty": "cases/aggregate_empty/tables/empty.json",
     "aggregate_empty.values": "cases/aggregate_empty/tables/values.json",
@@ -27,12 +28,16 @@ TABLES = {
 GOLDENS = [
     "decimal_parquet/aggregates", "decimal_parquet/group_by", "decimal_parquet/casts", "decimal_parquet/join",
     "decimal_result/multiply_sum", "decimal_result/storage_sum", "decimal_result/multiply",
+    "decimal_result/storage_boundary",
     "aggregate_row_layout/checksum", "aggregate_row_layout/estimated_groups",
     "aggregate_empty/empty_mixed", "aggregate_empty/nonempty_mixed", "aggregate_empty/all_null_mixed",
     "aggregate_empty/grouped_empty",
     "outer_join/duplicates", "outer_join/null_keys", "outer_join/all_matched", "outer_join/none_matched",
     "outer_join/right_join",
 ]
+
+# Goldens whose expected outcome is a runtime error (the error text).
+ERRORS = ["decimal_result/storage_overflow", "decimal_result/storage_unsigned"]
 
 
 def main():
@@ -46,6 +51,11 @@ def main():
         with open(os.path.join(REF, rel)) as f:
             r = json.load(f)
         out["goldens"][g] = {"schema": r["schema"], "rows": r["rows"], "from": rel}
+    out["errors"] = {}
+    for g in ERRORS:
+        rel = f"goldens/{g}.error"
+        with open(os.path.join(REF, rel)) as f:
+            out["errors"][g] = {"text": f.read().strip(), "from": rel}
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1)
     print(f"wrote {OUT}")

@@ -42,6 +42,7 @@ def _leaf(text: str, nullable: bool) -> T:
     return {
         "i8": T(SIGNED, 1, nullable=nullable), "i16": T(SIGNED, 2, nullable=nullable),
         "i32": T(SIGNED, 4, nullable=nullable), "i64": T(SIGNED, 8, nullable=nullable),
+        "u64": T(UNSIGNED, 8, nullable=nullable),
         "f64": T(FLOAT, 8, nullable=nullable), "string": T(STRING, 0, nullable=nullable),
         "bool": T(BOOL, 1, nullable=nullable),
     }[text]
@@ -122,6 +123,8 @@ def np_column(t: T, vals):
         return vals
     if t.kind == FLOAT:
         return np.array(vals, dtype=np.float64)
+    if t.kind == UNSIGNED:
+        return np.array(vals, dtype={1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[t.width])
     return np.array(vals, dtype={1: np.int8, 2: np.int16, 4: np.int32, 8: np.int64}[t.width])
 
 

@@ -55,6 +55,24 @@ def test_device_generators_match_host(gpu_backend):
     assert np.array_equal(gd.vf.to_numpy(np.float64, 300_000).view(np.int64), gvf.view(np.int64))
 
 
+@pytest.fixture(scope="module")
+def protocol_backend(gpu_backend):
+    b = cases.ProtocolBackend()
+    yield b
+    b.uninstall()
+
+
+@pytest.mark.parametrize("name", sorted(cases.AGG_GOLDENS))
+def test_aggregate_goldens_through_reference_protocol(protocol_backend, name):
+    """Every aggregate golden through the reference's unchanged caller
+    protocol: Init / Update, Size read before agg_finish_rowset and equal to
+    the finalized row count, the kernel-owned output adopted from a task heap
+    of owner-headed blocks and released by DestroyKernelOwnedRowSet with
+    nothing left behind (AVG columns through the project kernel)."""
+    got, want = cases.AGG_GOLDENS[name](protocol_backend)
+    same(got, want)
+
+
 @pytest.mark.parametrize("name", sorted(cases.AGG_GOLDENS))
 def test_aggregate_goldens(gpu_backend, name):
     got, want = cases.AGG_GOLDENS[name](gpu_backend)
@@ -105,7 +123,7 @@ def test_generated_kernel_equals_generic(gpu_backend, monkeypatch):
         kern = q.AggregateKernels(W.lineitem_fused_spec())
         p = q.TAggregateProcessor(kern, initial_capacity=8)
         p.add(dev.rowset)
-        outs.append(W.lineitem_finalize(p.finish()))
+        outs.append(W.lineitem_finalize_on_device(p, kern))
         p.close()
     assert outs[0] == outs[1]
 
@@ -229,6 +247,41 @@ def test_join_random_vs_oracle(gpu_backend, seed, table_mode):
         op, orows = OracleBackend().join(case, [build], probe)
         assert np.array_equal(gp[np.lexsort((gp[:, 1], gp[:, 0]))], op[np.lexsort((op[:, 1], op[:, 0]))])
         assert sorted(grows, key=sort_key) == sorted(orows, key=sort_key)
+
+
+def test_join_duplicate_build_sizes_pairs_by_key_multiplicity(gpu_backend):
+    """A one-to-many build (200K rows over 50K keys: most keys 4 rows, some
+    9) against a 3M-row probe: the pair buffer is sized by the most rows of
+    one key, not by the build's duplicate total (which would ask for ~4.5e11
+    pairs), and the pairs (about 12M) equal the oracle's."""
+    rng = np.random.default_rng(11)
+    nkeys, nb, npr = 50_000, 200_000, 3_000_000
+    bk = np.concatenate([np.repeat(np.arange(nkeys, dtype=np.int64), 4), np.repeat(np.arange(5000, dtype=np.int64), 5)])
+    bk = bk[rng.permutation(len(bk))][:nb]
+    pk = rng.integers(-1000, nkeys + 1000, npr).astype(np.int64)
+    types_b = [T(SIGNED, 8)]
+    types_p = [T(SIGNED, 8), T(SIGNED, 8)]
+    pv = np.arange(npr, dtype=np.int64)
+    case = JoinCase(types_b, types_p, [0], [0], join_type=cases.JOIN_INNER, probe_is_left=True)
+    kern, proc = gpu_backend.join_processor(case)
+    proc.append(gpu_backend.dev(Batch(types_b, [bk])))
+    proc.build()
+    got = proc.probe_pairs(gpu_backend.dev(Batch(types_p, [pk, pv])))
+    cap = int(q_pairs_capacity(gpu_backend, proc))
+    per_key = np.bincount(bk, minlength=nkeys + 1)
+    assert per_key.max() == 9
+    assert cap <= max(npr * 9, 1), "pair buffer sized by the most rows of one key"
+    oj = O.OracleJoin(case.to_c())
+    oj.append(O.HostRowSet(types_b, [bk]))
+    oj.build()
+    want = oj.probe(O.HostRowSet(types_p, [pk, pv]))
+    assert len(got) == len(want) > 10_000_000
+    assert np.array_equal(got[np.lexsort((got[:, 1], got[:, 0]))], want[np.lexsort((want[:, 1], want[:, 0]))])
+    proc.close()
+
+
+def q_pairs_capacity(gpu_backend, proc):
+    return gpu_backend.q.lib().qdb_gpu_join_processor_pairs(proc.p).contents.Capacity
 
 
 @pytest.mark.parametrize("table_mode", [0, 1])
@@ -519,6 +572,57 @@ def test_group_hash_matches_rh_hash(gpu_backend):
         q.lib().qdb_gpu_release_slot(q.abi.SLOT_HASH, slot)
 
 
+def test_stage_rowset_round_trip_and_aggregate(gpu_backend):
+    """qdb_gpu_stage_rowset copies a host batch (8-, 4-, 1-byte and 16-byte
+    columns, a bit-packed bool, validity masks, a selection and hashes) into
+    HBM through pinned memory; the staged batch downloads to the same bytes
+    and aggregates to the oracle's result on the host batch."""
+    q = gpu_backend.q
+    rng = np.random.default_rng(3)
+    n = 100_003
+    types = [T(SIGNED, 8), T(SIGNED, 4, nullable=True), T(SIGNED, 1), T(DECIMAL, 16, 2, True, 30), T(3, 1)]
+    cols = [rng.integers(-10 ** 12, 10 ** 12, n).astype(np.int64), rng.integers(-50, 50, n).astype(np.int32),
+            rng.integers(-3, 4, n).astype(np.int8), [int(x) * 10 ** 20 + 7 for x in rng.integers(-10 ** 6, 10 ** 6, n)],
+            rng.random(n) > 0.5]
+    valid = [None, rng.random(n) > 0.2, None, rng.random(n) > 0.3, None]
+    sel = (rng.random(n) > 0.25).astype(np.uint8)
+    hashes = rng.integers(0, 1 << 62, n).astype(np.uint64)
+    hb = O.HostRowSet(types, cols, valid, selection=sel, hashes=hashes)
+    schema = gpu_backend.cast(cases.schema_c(types))
+    rs = q.abi.TRowSet.from_buffer_copy(hb.rs)
+    staged = q.abi.TRowSet()
+    q.check(q.lib().qdb_gpu_stage_rowset(C.byref(rs), C.byref(schema), C.byref(staged)), "stage")
+    assert staged.RowCount == n and staged.Selection and staged.Hash and staged.Destroy
+    vts = [gpu_backend.H.ValueType(t.kind, t.width, t.scale, t.nullable) for t in types]
+    back = gpu_backend.H.download_rowset(staged, vts)
+    want = O.read_host_rowset(hb.rs, types)
+    for c, ((gv, gm), (wv, wm)) in enumerate(zip(back, want)):
+        assert list(gv) == list(wv), f"column {c}"
+        if wm is not None:
+            assert np.array_equal(gm, wm), f"mask {c}"
+    got_sel = np.empty(n, np.uint8)
+    q.check(q.lib().qdb_gpu_memcpy_d2h(got_sel.ctypes.data, staged.Selection, n), "sel")
+    got_hash = np.empty(n, np.uint64)
+    q.check(q.lib().qdb_gpu_memcpy_d2h(got_hash.ctypes.data, staged.Hash, n * 8), "hash")
+    assert np.array_equal(got_sel, sel) and np.array_equal(got_hash, hashes)
+    # The staged batch is a kernel input like any device batch.
+    case = AggCase(types, [2], [(AGG_COUNT_STAR, -1), (AGG_SUM, 0), (AGG_SUM, 3), (AGG_MIN, 1)])
+    kern = q.abi.QdbAggregateKernels()
+    q.check(q.lib().qdb_gpu_compile_aggregate(C.byref(gpu_backend.cast(case.to_c())), C.byref(kern)), "compile")
+    p = q.lib().qdb_gpu_agg_processor_new(C.byref(kern), 8, 0, 0)
+    assert q.lib().qdb_gpu_agg_processor_add(p, C.byref(staged)) == 0
+    out = q.abi.TRowSet()
+    assert q.lib().qdb_gpu_agg_processor_finish(p, C.byref(out)) >= 0
+    ot = [gpu_backend.H.ValueType(t.Kind, t.Width, t.Scale, bool(t.Nullable)) for t in kern.OutputTypes[: kern.NumOutputColumns]]
+    got = sorted(result_rows(gpu_backend.H.download_rowset(out, ot)), key=sort_key)
+    q.lib().qdb_gpu_rowset_release(C.byref(out))
+    q.lib().qdb_gpu_agg_processor_free(p)
+    q.lib().qdb_gpu_release_slot(q.abi.SLOT_AGGREGATE, kern.Slot)
+    q.lib().qdb_gpu_rowset_release(C.byref(staged))
+    wantagg = sorted(result_rows(OracleBackend().aggregate(case, [Batch(types, cols, valid, sel)])), key=sort_key)
+    assert got == wantagg
+
+
 def test_group_hash_known_answers(gpu_backend):
     """The group-hash entry point returns the reference's rh_hash values,
     worked out by hand from qdb/kernel/gen.cpp:233-379 over the key struct
@@ -602,9 +706,10 @@ def test_lineitem_fused_vs_oracle(gpu_backend, rows):
     from qumirdb_amd import workloads as W
     from qumirdb_amd.host import AggregateKernels, TAggregateProcessor
     dev = W.LineitemDevice(rows)
-    proc = TAggregateProcessor(AggregateKernels(W.lineitem_fused_spec()), initial_capacity=8)
+    ak = AggregateKernels(W.lineitem_fused_spec())
+    proc = TAggregateProcessor(ak, initial_capacity=8)
     proc.add(dev.rowset)
-    got = W.lineitem_finalize(proc.finish())
+    got = W.lineitem_finalize_on_device(proc, ak)
     assert got == lineitem_oracle(rows)
     assert len(got) == 6
 
@@ -621,12 +726,14 @@ def test_lineitem_operators_equal_fused(gpu_backend):
     bufs, _ = proj(dev.rowset)
     full = DeviceRowSet(W.lineitem_projected_types(), dev.bufs + bufs[:2], rows=rows)
     full.set_selection_buffer(sel)
-    p1 = TAggregateProcessor(AggregateKernels(W.lineitem_plain_spec()), initial_capacity=8)
+    k1 = AggregateKernels(W.lineitem_plain_spec())
+    p1 = TAggregateProcessor(k1, initial_capacity=8)
     p1.add(full)
-    plain = W.lineitem_finalize(p1.finish())
-    p2 = TAggregateProcessor(AggregateKernels(W.lineitem_fused_spec()), initial_capacity=8)
+    plain = W.lineitem_finalize_on_device(p1, k1)
+    k2 = AggregateKernels(W.lineitem_fused_spec())
+    p2 = TAggregateProcessor(k2, initial_capacity=8)
     p2.add(dev.rowset)
-    assert plain == W.lineitem_finalize(p2.finish())
+    assert plain == W.lineitem_finalize_on_device(p2, k2)
 
 
 def run_join(build_rows, probe_rows, table_mode=0):
@@ -712,6 +819,96 @@ def test_groups_vs_oracle(gpu_backend, rows, distinct):
     rel = np.abs(gs - ks) / np.maximum(np.abs(ks), 1e-300)
     assert (rel <= F64_REL_TOL).all(), float(rel.max())
     assert np.array_equal(gs, np.asarray(exact[5][0])[ei])
+
+
+def test_groups_full_size_sampled_vs_oracle(gpu_backend):
+    """The high-cardinality group-by at its full size (100M rows, Zipf(1)
+    keys over 10M values, 7.2M groups; the bucketed path) against the oracle:
+    every group whose key falls in a 1/32 sample (by a multiplicative hash of
+    the key) is compared bit for bit (key, COUNT, SUM(vi), MIN/MAX(vf), and
+    the f64 SUM against the correctly rounded exact sum, within 1e-12 of
+    Kahan), the oracle aggregating exactly the sampled keys' rows; and over
+    all groups the counts add up to the row count and the i64 sums to the
+    column's total."""
+    from qumirdb_amd import workloads as W
+    from qumirdb_amd.host import AggregateKernels, TAggregateProcessor
+    rows, distinct = 100_000_000, 10_000_000
+    cdf = W.zipf1_cdf(distinct)
+    dev = W.GroupsDevice(rows, distinct, cdf=cdf)
+    proc = TAggregateProcessor(AggregateKernels(W.groups_spec()), initial_capacity=1 << 24)
+    proc.add(dev.rowset)
+    got = proc.finish()
+    del dev
+    key, vi, vf = O.gen_groups(W.SEED_GROUPS, 0, rows, cdf)
+    gk = np.asarray(got[0][0])
+    assert int(np.asarray(got[1][0]).sum()) == rows
+    assert int(np.asarray(got[2][0]).astype(np.int64).sum()) == int(vi.sum())
+    assert len(gk) == len(np.unique(key))
+
+    def sampled(k):
+        return ((k.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)) >> np.uint64(59)) == 0
+
+    rs = sampled(key)
+    hb = O.HostRowSet(W.GROUPS_TYPES, [key[rs], vi[rs], vf[rs]])
+    del key, vi, vf
+    exact = O.aggregate(W.groups_spec().to_c(), [hb], fsum_mode=2)
+    kahan = O.aggregate(W.groups_spec().to_c(), [hb], fsum_mode=1)
+    gs = sampled(gk)
+    gi = np.argsort(gk[gs])
+    ei, ki = np.argsort(exact[0][0]), np.argsort(kahan[0][0])
+    assert gs.sum() == len(exact[0][0]) > 100_000
+    for c in range(6):
+        g = np.asarray(got[c][0])[gs][gi]
+        e = np.asarray(exact[c][0])[ei]
+        assert np.array_equal(g.view(np.int64) if g.dtype == np.float64 else g,
+                              e.view(np.int64) if e.dtype == np.float64 else e), c
+    ks = np.asarray(kahan[5][0])[ki]
+    g5 = np.asarray(got[5][0])[gs][gi]
+    assert (np.abs(g5 - ks) <= F64_REL_TOL * np.maximum(np.abs(ks), 1e-300)).all()
+
+
+def test_f64_sum_edge_values(gpu_backend):
+    """f64 SUM beyond the usual range, per group:
+    0: heavy cancellation inside the exact window, whose values have every
+       bit in [2^-64, 2^64) (pairs of +-(2^60 + k) and small dyadic terms
+       k * 2^-40): equal to the correctly rounded exact sum, and the same bits
+       on a second run;
+    1: values in [1e20, 1e25] (above the exact window), 2: values in
+       [1e-25, 1e-22] (below it) and 6: N(0, 1e-6) values (bits below
+       2^-64): within 1e-12 relative of Kahan;
+    3: a +inf among finite values: +inf; 4: a NaN: NaN; 5: +inf and -inf: NaN.
+    MIN/MAX of every group equal the oracle's (no NaN or signed zero in the
+    groups that have a finite result)."""
+    rng = np.random.default_rng(5)
+    m = 20_000
+    g0a = (rng.integers(0, 1 << 20, m).astype(np.float64) + 2.0 ** 60)
+    g0 = np.concatenate([g0a, -g0a, rng.integers(-(1 << 30), 1 << 30, m).astype(np.float64) * 2.0 ** -40])
+    g6 = rng.normal(0, 1e-6, m) + 1e-3
+    g1 = 10.0 ** rng.uniform(20, 25, m)
+    g2 = 10.0 ** rng.uniform(-25, -22, m)
+    g3 = np.concatenate([rng.normal(0, 1, 1000), [np.inf]])
+    g4 = np.concatenate([rng.normal(0, 1, 1000), [np.nan]])
+    g5 = np.concatenate([rng.normal(0, 1, 1000), [np.inf, -np.inf]])
+    groups = [g0, g1, g2, g3, g4, g5, g6]
+    keys = np.concatenate([np.full(len(g), i, np.int64) for i, g in enumerate(groups)])
+    vals = np.concatenate(groups)
+    perm = rng.permutation(len(vals))
+    keys, vals = keys[perm], vals[perm]
+    types = [T(SIGNED, 8), T(FLOAT, 8)]
+    case = AggCase(types, [0], [(AGG_SUM, 1), (AGG_MIN, 1), (AGG_MAX, 1), (AGG_COUNT_STAR, -1)])
+    b = Batch(types, [keys, vals])
+    got = sorted(result_rows(gpu_backend.aggregate(case, [b])), key=sort_key)
+    again = sorted(result_rows(gpu_backend.aggregate(case, [b])), key=sort_key)
+    exact = sorted(result_rows(OracleBackend(fsum_mode=2).aggregate(case, [b])), key=sort_key)
+    kahan = sorted(result_rows(OracleBackend(fsum_mode=1).aggregate(case, [b])), key=sort_key)
+    assert [r[0] for r in got] == list(range(7))
+    assert got[0][1] == exact[0][1] and np.float64(got[0][1]).tobytes() == np.float64(again[0][1]).tobytes()
+    for g in (1, 2, 6):
+        assert abs(got[g][1] - kahan[g][1]) <= F64_REL_TOL * abs(kahan[g][1]), g
+    assert got[3][1] == np.inf
+    assert math.isnan(got[4][1]) and math.isnan(got[5][1])
+    for g in (0, 1, 2, 3, 5, 6):
+        assert got[g][2:] == exact[g][2:], g
 
 
 def test_served_results_same_alone_and_in_the_mix(gpu_backend):
