@@ -422,6 +422,23 @@ enum { QDB_SLOT_FILTER = 0, QDB_SLOT_PROJECT = 1, QDB_SLOT_AGGREGATE = 2, QDB_SL
  * join's RowArena blob. Returns the number of results written. */
 int64_t qdb_gpu_join_fused_results(void* arena, QdbFusedResult* out, int64_t max_out);
 
+/* The fused-aggregate probe that allocates nothing. It has no counterpart
+ * in the reference; it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It does the work of jr_dispatch(Probe) followed by
+ * qdb_gpu_join_fused_results for that one batch, with two differences: no
+ * pair is written (the pair count goes to *pairs_out), and out[f] holds
+ * fused aggregate f of this batch alone (not accumulated since Init, and not
+ * added to what qdb_gpu_join_fused_results reports). `arena` is the RowArena
+ * blob of a join after Build; max_out must be at least its NumFused. The
+ * per-workgroup partials live in a static device array of 128 KiB per
+ * device (room for 512 workgroups); a launch that would need more is refused
+ * with an error. Every call on one device shares that array and the join's
+ * own result words, so calls that overlap on one device (or on one join)
+ * race. Calls no hipMalloc / hipFree. Returns 0, or -1 with the error text. */
+int qdb_gpu_join_probe_fused_noalloc(void* arena, const TRowSet* batch, QdbFusedResult* out, int64_t max_out,
+    int64_t* pairs_out);
+
 /* Frees everything behind a join's arena/table/scratch/pairs blobs (the
  * reference frees them with its allocation domain). */
 void qdb_gpu_join_destroy(void* arena, void* table, void* scratch, void* pairs);

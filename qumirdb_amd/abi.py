@@ -210,7 +210,7 @@ EXPORTS = [
     "qdb_gpu_join_processor_probe", "qdb_gpu_join_processor_probe_only", "qdb_gpu_join_processor_pairs",
     "qdb_gpu_join_processor_arena", "qdb_gpu_join_processor_next", "qdb_gpu_join_processor_free",
     "qdb_gpu_rowset_release", "qdb_gpu_set_host_heap", "qdb_gpu_destroy_kernel_rowset", "qdb_gpu_adopt_kernel_rowset",
-    "qdb_gpu_set_row_error_handler", "qdb_gpu_take_row_error",
+    "qdb_gpu_set_row_error_handler", "qdb_gpu_take_row_error", "qdb_gpu_join_probe_fused_noalloc",
 ]
 
 
@@ -281,6 +281,8 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "qdb_gpu_adopt_kernel_rowset": (None, [C.POINTER(TRowSet)]),
         "qdb_gpu_set_row_error_handler": (None, [ROW_ERROR_FN]),
         "qdb_gpu_take_row_error": (i64, []),
+        "qdb_gpu_join_probe_fused_noalloc": (C.c_int, [P, C.POINTER(TRowSet), C.POINTER(QdbFusedResult), i64,
+                                                       C.POINTER(i64)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -652,6 +652,9 @@ constexpr int kGeneric = 0, kDirectKey = 1, kDirectVal = 2;
 // in LDS, the whole hash table in LDS, or the dense direct table (the
 // membership bitmap in LDS as far as it fits, the ordinal table in HBM).
 constexpr int kTabHash = 0, kTabLds = 1, kTabDense = 2;
+// Device scratch of the allocation-free probe (qdb_gpu_join_probe_fused_
+// noalloc): the fused partials of at most 512 workgroups.
+constexpr size_t kNoAllocScratchBytes = 128 * 1024;
 // Pairs the first probe attempt allocates room for at most (2 GiB of pairs).
 constexpr unsigned long long kFirstPairBudget = 1ull << 27;
 
@@ -2249,7 +2252,11 @@ void launch_probe(int mode, int tab, int grid, size_t lds, const ProbeArgs& A)
 }
 
 template <int KW>
-bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
+// Probes one batch. With `scratch_part` (the allocation-free variant) no
+// pair is written and nothing is allocated: the fused aggregates' partials
+// go to that device scratch, the pair count to *pairs_seen.
+bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs, unsigned long long* scratch_part = nullptr,
+    unsigned long long* pairs_seen = nullptr)
 {
     JoinPlan* P = J->plan;
     const int64_t n = batch->RowCount;
@@ -2350,7 +2357,7 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
     const unsigned long long budget = (unsigned long long)n > kFirstPairBudget ? (unsigned long long)n : kFirstPairBudget;
     unsigned long long cap = bound < (u128)budget ? (unsigned long long)bound : budget;
     for (int attempt = 0; attempt < 2; ++attempt) {
-        if ((unsigned long long)pairs->Capacity < cap) {
+        if (!scratch_part && (unsigned long long)pairs->Capacity < cap) {
             if (cap > (unsigned long long)(SIZE_MAX / 16)) return fail("probe: %llu pairs do not fit in memory", cap);
             dfree(pairs->Data);
             pairs->Capacity = 0;
@@ -2358,8 +2365,8 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
             if (!pairs->Data) return false;
             pairs->Capacity = (int64_t)cap;
         }
-        A.pairs = pairs->Data;
-        A.pair_cap = (unsigned long long)pairs->Capacity;
+        A.pairs = scratch_part ? nullptr : pairs->Data;
+        A.pair_cap = scratch_part ? 0ull : (unsigned long long)pairs->Capacity;
         A.pair_count = J->pair_count;
         if (!zero_fill(J->pair_count, 8)) return false;
         // Fused accumulators are per probe call (k_reduce_partials writes
@@ -2369,8 +2376,17 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
         const int grid = (int)(tiles < (int64_t)num_cus() * per_cu ? tiles : (int64_t)num_cus() * per_cu);
         unsigned long long* part = nullptr;
         if (A.nfused) {
-            part = (unsigned long long*)dalloc((size_t)grid * kMaxFused * kAccWords * 8);
-            if (!part) return false;
+            const size_t part_bytes = (size_t)grid * kMaxFused * kAccWords * 8;
+            if (scratch_part) {
+                if (part_bytes > kNoAllocScratchBytes) {
+                    return fail("no-alloc probe: %zu bytes of partials exceed the %zu-byte scratch", part_bytes,
+                        (size_t)kNoAllocScratchBytes);
+                }
+                part = scratch_part;
+            } else {
+                part = (unsigned long long*)dalloc(part_bytes);
+                if (!part) return false;
+            }
             A.fused_part = part;
         }
         {
@@ -2387,7 +2403,7 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
                     A.nfused * kAccWords, ops, J->fused_acc);
             }
         }
-        dfree(part);
+        if (!scratch_part) dfree(part);
         if (!last_error_text().empty()) return false;
         if (!hip_ok(hipGetLastError(), "probe launch")) return false;
         // The pair count and the fused words in one readback.
@@ -2395,6 +2411,11 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs)
         if (!hip_ok(hipMemcpyAsync(J->readback, J->pair_count, rb, hipMemcpyDeviceToHost, stream()), "probe readback") || !sync("probe"))
             return false;
         const unsigned long long count = J->readback[0];
+        if (scratch_part) {
+            *pairs_seen = count;
+            J->fused_ready = A.nfused > 0;
+            return true;
+        }
         if (count <= A.pair_cap) {
             pairs->Count = (int64_t)count;
             J->fused_ready = A.nfused > 0;
@@ -2417,24 +2438,19 @@ std::unordered_map<JoinBuild*, FusedTotals>& fused_totals()
     return *m;
 }
 
-bool collect_fused(JoinBuild* J)
+// Folds one probe's fused words (the readback after the pair count) into
+// the results r[0..nf).
+// The words of a probe that saw no row (MIN/MAX words at their identities).
+const unsigned long long kZeroFusedWords[kMaxFused * kAccWords] = {
+    0, 0, 0, 0, 0x7fffffffffffffffull, 0x8000000000000000ull, 0, 0, 0, 0, 0x7fffffffffffffffull, 0x8000000000000000ull,
+    0, 0, 0, 0, 0x7fffffffffffffffull, 0x8000000000000000ull, 0, 0, 0, 0, 0x7fffffffffffffffull, 0x8000000000000000ull};
+
+void merge_fused_words(const JoinPlan* P, const unsigned long long* w, QdbFusedResult* res)
 {
-    JoinPlan* P = J->plan;
     const int nf = P->spec.NumFused;
-    if (!nf || !J->fused_ready) return true;
-    J->fused_ready = false;
-    const unsigned long long* w = J->readback + 2;
-    std::lock_guard<std::mutex> lock(g_fused_mu);
-    FusedTotals& T = fused_totals()[J];
-    if (T.r.empty()) {
-        T.r.resize((size_t)nf);
-        for (int f = 0; f < nf; ++f) {
-            memset(&T.r[(size_t)f], 0, sizeof(QdbFusedResult));
-        }
-    }
     for (int f = 0; f < nf; ++f) {
         const unsigned long long* a = &w[(size_t)f * kAccWords];
-        QdbFusedResult& r = T.r[(size_t)f];
+        QdbFusedResult& r = res[f];
         const FusedDesc d = P->fused[f];
         const bool first = r.Count == 0;
         r.Count += (int64_t)a[0];
@@ -2459,6 +2475,23 @@ bool collect_fused(JoinBuild* J)
         // COUNT is never NULL; SUM/MIN/MAX of no value is NULL.
         r.Valid = (d.kind == FK_COUNT_PAIRS || d.kind == FK_COUNT_VALID) ? 1 : (r.Count > 0);
     }
+}
+
+bool collect_fused(JoinBuild* J)
+{
+    JoinPlan* P = J->plan;
+    const int nf = P->spec.NumFused;
+    if (!nf || !J->fused_ready) return true;
+    J->fused_ready = false;
+    std::lock_guard<std::mutex> lock(g_fused_mu);
+    FusedTotals& T = fused_totals()[J];
+    if (T.r.empty()) {
+        T.r.resize((size_t)nf);
+        for (int f = 0; f < nf; ++f) {
+            memset(&T.r[(size_t)f], 0, sizeof(QdbFusedResult));
+        }
+    }
+    merge_fused_words(P, J->readback + 2, T.r.data());
     return true;
 }
 
@@ -2664,6 +2697,68 @@ extern "C" int qdb_gpu_compile_row_join(const QdbRowJoinSpec* spec, QdbRowJoinKe
     out->Slot = slot;
     out->NumOutputColumns = P->nout;
     for (int c = 0; c < P->nout; ++c) out->OutputTypes[c] = P->out_types[c];
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The probe that allocates nothing (qdb_gpu_join_probe_fused_noalloc). Its
+// scratch, the fused aggregates' per-workgroup partials, is this file-scope
+// device array: every device the library runs on holds one copy until the
+// process ends. Host code never uses the array's name as an address (on the
+// host it only names the symbol): the device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+// ---------------------------------------------------------------------------
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The current device's address of `scratch` (looked up at the first call on
+// that device), or nullptr with the error text.
+unsigned long long* noalloc_scratch()
+{
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+    if (dev < 0 || dev >= kMaxScratchDevices) return fail("device %d: no scratch slot", dev), nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        if (!hip_ok(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return nullptr;
+        g_scratch_ptr[dev] = ptr;
+    }
+    return (unsigned long long*)g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" int qdb_gpu_join_probe_fused_noalloc(void* arena, const TRowSet* batch, QdbFusedResult* out, int64_t max_out,
+    int64_t* pairs_out)
+{
+    clear_error();
+    if (!arena || !batch || !out || !pairs_out) return fail("null argument"), -1;
+    if (!ensure_init()) return -1;
+    RowArenaBlob* AB = (RowArenaBlob*)arena;
+    JoinBuild* J = AB->build;
+    if (!J || !J->built) return fail("no-alloc probe before Build"), -1;
+    JoinPlan* P = J->plan;
+    const int nf = P->spec.NumFused;
+    if (nf < 1) return fail("no-alloc probe: the join has no fused aggregate"), -1;
+    if (max_out < nf) return fail("no-alloc probe: %lld results requested, the join has %d", (long long)max_out, nf), -1;
+    unsigned long long* part = noalloc_scratch();
+    if (!part) return -1;
+    QdbPairBuffer none = {0, 0, nullptr};
+    unsigned long long seen = 0;
+    const bool ok = P->KW == 1 ? do_probe<1>(J, batch, &none, part, &seen) : do_probe<2>(J, batch, &none, part, &seen);
+    if (!ok) return -1;
+    memset(out, 0, (size_t)nf * sizeof(QdbFusedResult));
+    if (J->fused_ready) merge_fused_words(P, J->readback + 2, out);
+    else merge_fused_words(P, kZeroFusedWords, out);
+    J->fused_ready = false;
+    *pairs_out = (int64_t)seen;
     return 0;
 }
 
