@@ -508,6 +508,32 @@ int qdb_gpu_gen_join_probe(uint64_t seed, int64_t row0, int64_t rows, int64_t bu
 int qdb_gpu_gen_zipf_groups(uint64_t seed, int64_t row0, int64_t rows, const double* zipf_cdf,
     int64_t distinct, int64_t* key, int64_t* vi, double* vf);
 
+/* Runtime join filter (TRuntimeFilter, qdb/exec/runtime_filter.cpp:122-300;
+ * its kernel rf_build / rf_publish / rf_probe, qdb/kernel/join/
+ * runtime_filter.oz): the probe-side filter of a join, built from the
+ * selected build rows' join-key hashes (rh_hash of the key struct, as the
+ * group-hash entry point computes it). Defaults of the reference: max_keys
+ * 16384 (TRuntimeFilter::DefaultMaxKeys), bloom_bytes 2 MiB (rounded down to
+ * a power of two; below 64 bytes there is no Bloom storage). Up to max_keys
+ * distinct hashes the published filter is an exact set (kind 1), beyond that
+ * a blocked Bloom filter (kind 2), without Bloom storage it cannot answer
+ * (kind 0: every selected row is kept). Hashes, selections and outputs are
+ * device arrays of `count` entries; a NULL selection selects every row.
+ *   qdb_gpu_rf_new      -> an opaque filter, or NULL
+ *   qdb_gpu_rf_build    adds a build batch's selected hashes (0 or -1)
+ *   qdb_gpu_rf_publish  publishes; returns the kind, or -1
+ *   qdb_gpu_rf_probe    output[i] = 1 when row i is selected and its hash may
+ *                       be in the build (rf_probe); returns the kept count
+ *   qdb_gpu_rf_stats    [kind, exact keys, exact table slots, Bloom words,
+ *                       Bloom bits set]; returns the count written */
+void* qdb_gpu_rf_new(int64_t max_keys, int64_t bloom_bytes);
+int qdb_gpu_rf_build(void* filter, const uint64_t* hashes, const uint8_t* selection, int64_t count);
+int qdb_gpu_rf_publish(void* filter);
+int64_t qdb_gpu_rf_probe(const void* filter, const uint64_t* hashes, const uint8_t* selection, uint8_t* output,
+    int64_t count);
+int64_t qdb_gpu_rf_stats(const void* filter, int64_t* out, int64_t n);
+void qdb_gpu_rf_free(void* filter);
+
 /* Kernel timing of the last launches, for roofline reporting: the average
  * duration in milliseconds of the named kernel family ("agg_update",
  * "join_probe", ...) measured with HIP events on the stream it runs on since

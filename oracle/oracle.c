@@ -1505,3 +1505,163 @@ void oracle_gen_groups(uint64_t seed, int64_t row0, int64_t rows, const double* 
 {
     for (int64_t i = 0; i < rows; ++i) qdb_gen_groups_row(seed, row0 + i, cdf, distinct, &key[i], &vi[i], &vf[i]);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Runtime join filter (qdb/kernel/join/runtime_filter.oz; policy in         */
+/* qdb/exec/runtime_filter.cpp:122-300), for one producer                    */
+/* ------------------------------------------------------------------------ */
+
+/* rf_mix: the splitmix64 finalizer (runtime_filter.oz:79-86). */
+uint64_t oracle_rf_mix(uint64_t v)
+{
+    v ^= v >> 30;
+    v *= 0xBF58476D1CE4E5B9ull;
+    v ^= v >> 27;
+    v *= 0x94D049BB133111EBull;
+    return v ^ (v >> 31);
+}
+
+typedef struct {
+    int64_t max_keys;
+    int64_t bloom_words;   /* bit_floor(bloom bytes) / 8, 0 below 64 bytes */
+    uint64_t* keys;        /* every selected hash, in build order */
+    int64_t nkeys, cap;
+    int kind;              /* -1 building, 0 cannot answer, 1 exact, 2 Bloom */
+    uint64_t* words;
+    int64_t word_count;
+    int64_t exact_keys;
+} OracleRf;
+
+void* oracle_rf_new(int64_t max_keys, int64_t bloom_bytes)
+{
+    OracleRf* f = (OracleRf*)calloc(1, sizeof(OracleRf));
+    f->max_keys = max_keys;
+    int64_t b = 0;
+    if (bloom_bytes >= 64) { /* NormalizeBloomBytes, runtime_filter.cpp:20-22 */
+        b = 1;
+        while (b * 2 <= bloom_bytes) b *= 2;
+    }
+    f->bloom_words = b / 8;
+    f->kind = -1;
+    return f;
+}
+
+void oracle_rf_free(void* p)
+{
+    OracleRf* f = (OracleRf*)p;
+    if (!f) return;
+    free(f->keys);
+    free(f->words);
+    free(f);
+}
+
+/* rf_build: the selected hashes join the contribution. */
+void oracle_rf_build(void* p, const uint64_t* hashes, const uint8_t* sel, int64_t count)
+{
+    OracleRf* f = (OracleRf*)p;
+    for (int64_t i = 0; i < count; ++i) {
+        if (sel && !sel[i]) continue;
+        if (f->nkeys == f->cap) {
+            f->cap = f->cap ? 2 * f->cap : 1024;
+            f->keys = (uint64_t*)realloc(f->keys, (size_t)f->cap * 8);
+        }
+        f->keys[f->nkeys++] = hashes[i];
+    }
+}
+
+static uint64_t rf_exact_stored(uint64_t h) { return h == 0 ? 1 : h; }
+
+/* rf_exact_slot: the slot holding hash, or the empty slot where it belongs. */
+static int64_t rf_exact_slot(const uint64_t* table, uint64_t mask, uint64_t h)
+{
+    const uint64_t stored = rf_exact_stored(h);
+    int64_t slot = (int64_t)(h & mask);
+    while (table[slot] != 0 && table[slot] != stored) slot = (int64_t)((uint64_t)(slot + 1) & mask);
+    return slot;
+}
+
+static void rf_bloom_add(uint64_t* words, uint64_t block_mask, uint64_t h)
+{
+    const int64_t base = (int64_t)((oracle_rf_mix(h) & block_mask) * 8);
+    const uint64_t bits = oracle_rf_mix(h ^ 0x9E3779B97F4A7C15ull);
+    for (int w = 0; w < 8; ++w) words[base + w] |= 1ull << ((bits >> (8 * w)) & 63);
+}
+
+static int rf_bloom_contains(const uint64_t* words, uint64_t block_mask, uint64_t h)
+{
+    const int64_t base = (int64_t)((oracle_rf_mix(h) & block_mask) * 8);
+    const uint64_t bits = oracle_rf_mix(h ^ 0x9E3779B97F4A7C15ull);
+    for (int w = 0; w < 8; ++w) {
+        if ((words[base + w] & (1ull << ((bits >> (8 * w)) & 63))) == 0) return 0;
+    }
+    return 1;
+}
+
+/* rf_publish (runtime_filter.oz:503-557): distinct hashes counted up to the
+ * first one over MaxKeys; an exact table of pow2 >= 2 * distinct slots (at
+ * least 8) filled in first-seen order, else every hash into the Bloom words. */
+int oracle_rf_publish(void* p)
+{
+    OracleRf* f = (OracleRf*)p;
+    if (f->kind >= 0) return f->kind;
+    /* The set of distinct RAW hashes (the reference's deduplicating set
+     * holds raw hashes, so 0 and 1 are two keys), first-seen order: a plain
+     * open-addressing set with an occupancy byte per slot. */
+    int64_t cap = 16;
+    while (cap < 2 * (f->nkeys + 1)) cap *= 2;
+    uint64_t* set = (uint64_t*)calloc((size_t)cap, 8);
+    uint8_t* used = (uint8_t*)calloc((size_t)cap, 1);
+    uint64_t* distinct = (uint64_t*)malloc((size_t)(f->nkeys ? f->nkeys : 1) * 8);
+    int64_t nd = 0;
+    for (int64_t i = 0; i < f->nkeys && nd <= f->max_keys; ++i) {
+        const uint64_t h = f->keys[i];
+        int64_t s = (int64_t)(oracle_rf_mix(h) & (uint64_t)(cap - 1));
+        while (used[s] && set[s] != h) s = (s + 1) & (cap - 1);
+        if (!used[s]) {
+            used[s] = 1;
+            set[s] = h;
+            distinct[nd++] = h;
+        }
+    }
+    free(set);
+    free(used);
+    if (nd <= f->max_keys) {
+        int64_t size = 8;
+        while (size < nd * 2) size *= 2;
+        f->words = (uint64_t*)calloc((size_t)size, 8);
+        for (int64_t i = 0; i < nd; ++i) {
+            const int64_t s = rf_exact_slot(f->words, (uint64_t)(size - 1), distinct[i]);
+            f->words[s] = rf_exact_stored(distinct[i]);
+        }
+        f->word_count = size;
+        f->exact_keys = nd;
+        f->kind = 1;
+    } else if (f->bloom_words == 0) {
+        f->kind = 0;
+    } else {
+        f->words = (uint64_t*)calloc((size_t)f->bloom_words, 8);
+        f->word_count = f->bloom_words;
+        for (int64_t i = 0; i < f->nkeys; ++i) rf_bloom_add(f->words, (uint64_t)(f->bloom_words / 8 - 1), f->keys[i]);
+        f->kind = 2;
+    }
+    free(distinct);
+    return f->kind;
+}
+
+/* rf_probe: output[row] = selected and (maybe) in the filter; kept rows. */
+int64_t oracle_rf_probe(void* p, const uint64_t* hashes, const uint8_t* sel, uint8_t* out, int64_t count)
+{
+    const OracleRf* f = (const OracleRf*)p;
+    int64_t kept = 0;
+    for (int64_t i = 0; i < count; ++i) {
+        int keep = !sel || sel[i];
+        if (keep && f->kind == 1) {
+            keep = f->words[rf_exact_slot(f->words, (uint64_t)(f->word_count - 1), hashes[i])] == rf_exact_stored(hashes[i]);
+        } else if (keep && f->kind == 2) {
+            keep = rf_bloom_contains(f->words, (uint64_t)(f->word_count / 8 - 1), hashes[i]);
+        }
+        out[i] = (uint8_t)keep;
+        kept += keep;
+    }
+    return kept;
+}

@@ -55,6 +55,12 @@ def _lib() -> C.CDLL:
         "oracle_exact_sum": (C.c_double, [P, i64]),
         "oracle_kahan_sum": (C.c_double, [P, i64]),
         "oracle_filter": (None, [C.POINTER(abi.QdbFilterSpec), C.POINTER(abi.TRowSet), P, P]),
+        "oracle_rf_mix": (C.c_uint64, [C.c_uint64]),
+        "oracle_rf_new": (P, [i64, i64]),
+        "oracle_rf_build": (None, [P, P, P, i64]),
+        "oracle_rf_publish": (C.c_int, [P]),
+        "oracle_rf_probe": (i64, [P, P, P, P, i64]),
+        "oracle_rf_free": (None, [P]),
         "oracle_project": (C.c_int, [C.POINTER(abi.QdbProjectSpec), C.POINTER(abi.TRowSet), C.POINTER(P), P]),
         "oracle_key_hash": (C.c_uint64, [C.POINTER(abi.TColumn), C.POINTER(abi.QdbValueType), C.POINTER(C.c_int32), C.c_int, i64]),
         "oracle_mix_u64": (C.c_uint64, [C.c_uint64]),
@@ -341,3 +347,31 @@ def gen_groups(seed: int, row0: int, rows: int, cdf: np.ndarray):
     key, vi, vf = np.empty(rows, np.int64), np.empty(rows, np.int64), np.empty(rows, np.float64)
     LIB.oracle_gen_groups(seed, row0, rows, cdf.ctypes.data, len(cdf), key.ctypes.data, vi.ctypes.data, vf.ctypes.data)
     return key, vi, vf
+
+
+class OracleRuntimeFilter:
+    """The reference's runtime join filter for one producer (oracle.c)."""
+
+    def __init__(self, max_keys: int = 1 << 14, bloom_bytes: int = 2 << 20):
+        self.f = LIB.oracle_rf_new(max_keys, bloom_bytes)
+
+    def build(self, hashes, selection=None):
+        h = np.ascontiguousarray(np.asarray(hashes, dtype=np.uint64))
+        s = None if selection is None else np.ascontiguousarray(np.asarray(selection, dtype=np.uint8))
+        LIB.oracle_rf_build(self.f, h.ctypes.data, s.ctypes.data if s is not None else None, len(h))
+
+    def publish(self) -> int:
+        return LIB.oracle_rf_publish(self.f)
+
+    def probe(self, hashes, selection=None):
+        h = np.ascontiguousarray(np.asarray(hashes, dtype=np.uint64))
+        s = None if selection is None else np.ascontiguousarray(np.asarray(selection, dtype=np.uint8))
+        out = np.zeros(len(h), np.uint8)
+        kept = LIB.oracle_rf_probe(self.f, h.ctypes.data, s.ctypes.data if s is not None else None, out.ctypes.data, len(h))
+        return out, int(kept)
+
+    def __del__(self):
+        try:
+            LIB.oracle_rf_free(self.f)
+        except Exception:
+            pass

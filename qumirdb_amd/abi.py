@@ -211,6 +211,7 @@ EXPORTS = [
     "qdb_gpu_join_processor_arena", "qdb_gpu_join_processor_next", "qdb_gpu_join_processor_free",
     "qdb_gpu_rowset_release", "qdb_gpu_set_host_heap", "qdb_gpu_destroy_kernel_rowset", "qdb_gpu_adopt_kernel_rowset",
     "qdb_gpu_set_row_error_handler", "qdb_gpu_take_row_error", "qdb_gpu_join_probe_fused_noalloc",
+    "qdb_gpu_rf_new", "qdb_gpu_rf_build", "qdb_gpu_rf_publish", "qdb_gpu_rf_probe", "qdb_gpu_rf_stats", "qdb_gpu_rf_free",
 ]
 
 
@@ -283,6 +284,12 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "qdb_gpu_take_row_error": (i64, []),
         "qdb_gpu_join_probe_fused_noalloc": (C.c_int, [P, C.POINTER(TRowSet), C.POINTER(QdbFusedResult), i64,
                                                        C.POINTER(i64)]),
+        "qdb_gpu_rf_new": (P, [i64, i64]),
+        "qdb_gpu_rf_build": (C.c_int, [P, P, P, i64]),
+        "qdb_gpu_rf_publish": (C.c_int, [P]),
+        "qdb_gpu_rf_probe": (i64, [P, P, P, P, i64]),
+        "qdb_gpu_rf_stats": (i64, [P, C.POINTER(i64), i64]),
+        "qdb_gpu_rf_free": (None, [P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -72,6 +72,11 @@ int grid_for(int64_t items, int block, int per_cu);
 bool exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n);
 bool sort_pairs_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out, int64_t n,
     int end_bit);
+// Radix sort of u64 keys (all 64 bits), and a stable radix sort of
+// (u64 key, u32 value) pairs over key bits [begin_bit, end_bit).
+bool sort_keys_u64(const uint64_t* keys_in, uint64_t* keys_out, int64_t n);
+bool sort_pairs_u64_u32(const uint64_t* keys_in, uint64_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out, int64_t n,
+    int begin_bit, int end_bit);
 
 // Device-resident copy of a host struct (uploaded once, freed with dfree).
 void* upload(const void* host, size_t bytes);

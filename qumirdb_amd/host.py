@@ -762,3 +762,62 @@ class TRowJoinProcessor:
             self.close()
         except Exception:
             pass
+
+
+class RuntimeFilter:
+    """The runtime join filter over the C-ABI (qdb_gpu_rf_*): built from the
+    selected build rows' join-key hashes, published once, probed per probe
+    batch (TRuntimeFilter, qdb/exec/runtime_filter.cpp)."""
+
+    def __init__(self, max_keys: int = 1 << 14, bloom_bytes: int = 2 << 20):
+        self.f = lib().qdb_gpu_rf_new(max_keys, bloom_bytes)
+        if not self.f:
+            raise RuntimeError(f"runtime filter: {lib().qdb_gpu_last_error().decode()}")
+
+    def build(self, hashes: DeviceBuffer, count: int, selection: Optional[DeviceBuffer] = None) -> None:
+        check(lib().qdb_gpu_rf_build(self.f, hashes.ptr, selection.ptr if selection is not None else None, count), "rf_build")
+
+    def publish(self) -> int:
+        return check(lib().qdb_gpu_rf_publish(self.f), "rf_publish")
+
+    def probe(self, hashes: DeviceBuffer, count: int, selection: Optional[DeviceBuffer] = None):
+        """(selection buffer of the kept rows, kept count)."""
+        out = DeviceBuffer(max(count, 1))
+        kept = check(lib().qdb_gpu_rf_probe(self.f, hashes.ptr, selection.ptr if selection is not None else None, out.ptr,
+                                            count), "rf_probe")
+        return out, kept
+
+    def stats(self) -> dict:
+        v = (C.c_int64 * 5)()
+        lib().qdb_gpu_rf_stats(self.f, v, 5)
+        return {"kind": v[0], "exact_keys": v[1], "table_slots": v[2], "bloom_words": v[3], "bloom_bits": v[4]}
+
+    def close(self):
+        if self.f:
+            lib().qdb_gpu_rf_free(self.f)
+            self.f = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GroupHash:
+    """The group-hash entry point (rh_hash of the key columns of every
+    selected row) for one schema and key count."""
+
+    def __init__(self, types: Sequence[ValueType], nkeys: int):
+        self.schema = schema_c(types)
+        fn = C.c_void_p()
+        self.slot = check(lib().qdb_gpu_compile_group_hash(C.byref(self.schema), nkeys, C.byref(fn)), "compile_group_hash")
+        self.fn = abi.GROUP_HASH_FN(fn.value)
+        self.nkeys = nkeys
+
+    def __call__(self, rowset, key_cols: Sequence[int]) -> DeviceBuffer:
+        out = DeviceBuffer(max(rowset.rows, 1) * 8)
+        kc = (C.c_int64 * len(key_cols))(*key_cols)
+        if not self.fn(rowset.ptr, out.ptr, kc):
+            raise RuntimeError(f"group hash: {lib().qdb_gpu_last_error().decode()}")
+        return out

@@ -623,6 +623,84 @@ def test_stage_rowset_round_trip_and_aggregate(gpu_backend):
     assert got == wantagg
 
 
+@pytest.mark.parametrize("nkeys,max_keys", [(3000, 1 << 14), (60000, 1 << 14), (5000, 100)])
+def test_runtime_filter_equals_oracle(gpu_backend, nkeys, max_keys):
+    """The runtime join filter published and probed on the GPU keeps exactly
+    the rows the oracle's restatement of the reference keeps (exact set
+    below MaxKeys distinct hashes, blocked Bloom filter above), with build
+    and probe selections, repeated build hashes and a hash 0; its statistics
+    (kind, exact keys, Bloom bits set) agree."""
+    H = gpu_backend.H
+    rng = np.random.default_rng(nkeys)
+    build = rng.integers(0, 1 << 63, nkeys, dtype=np.uint64)
+    build[:3] = [0, 1, 1]
+    bsel = (rng.random(nkeys) > 0.2).astype(np.uint8)
+    probe = np.concatenate([build[rng.integers(0, nkeys, 50000)], rng.integers(0, 1 << 63, 150000, dtype=np.uint64)])
+    probe[:2] = [0, 2]
+    psel = (rng.random(len(probe)) > 0.1).astype(np.uint8)
+    f = H.RuntimeFilter(max_keys=max_keys)
+    half = nkeys // 2
+    for a, b in ((0, half), (half, nkeys)):
+        f.build(H.DeviceBuffer.from_numpy(build[a:b]), b - a, H.DeviceBuffer.from_numpy(bsel[a:b]))
+    kind = f.publish()
+    of = O.OracleRuntimeFilter(max_keys=max_keys)
+    of.build(build, bsel)
+    assert kind == of.publish() == (1 if len(np.unique(build[bsel.astype(bool)])) <= max_keys else 2)
+    out, kept = f.probe(H.DeviceBuffer.from_numpy(probe), len(probe), H.DeviceBuffer.from_numpy(psel))
+    got = out.to_numpy(np.uint8, len(probe))
+    want, wkept = of.probe(probe, psel)
+    bad = np.nonzero(got != want)[0]
+    assert len(bad) == 0 and kept == wkept, (
+        f"{len(bad)} rows differ (first {bad[:5].tolist()}: got {got[bad[:5]].tolist()}, want {want[bad[:5]].tolist()}, "
+        f"hashes {[hex(int(x)) for x in probe[bad[:5]]]}), kept {kept} vs {wkept}, stats {f.stats()}")
+    st = f.stats()
+    if kind == 1:
+        assert st["exact_keys"] == len(np.unique(build[bsel.astype(bool)]))
+    else:
+        assert st["bloom_words"] == (2 << 20) // 8 and st["bloom_bits"] > 0
+    f.close()
+
+
+def test_runtime_filter_on_the_join_keeps_every_match(gpu_backend):
+    """The filtered-build join with its runtime filter: build hashes are the
+    group-hash entry point's rh_hash of the selected build keys (segment = 1,
+    60K of 300K rows: a Bloom filter), the probe rows the filter rejects are
+    deselected before the probe, and the join's pairs and fused COUNT/SUM
+    equal those without the filter; the filter rejects most non-matching
+    probe rows."""
+    from qumirdb_amd import workloads as W
+    from qumirdb_amd.host import FilterKernel, GroupHash, RowJoinKernels, TRowJoinProcessor, RuntimeFilter
+    B, Pn = 300_000, 2_000_000
+    jd = W.JoinDevice(B, Pn)
+    bsel = FilterKernel(W.BUILD_TYPES, *W.join_build_predicate())(jd.build)
+    jd.build.set_selection_buffer(bsel)
+    bh = GroupHash(W.BUILD_TYPES, 1)(jd.build, [W.BUILD_KEY])
+    ph = GroupHash(W.PROBE_TYPES, 1)(jd.probe, [W.PROBE_FK])
+    rf = RuntimeFilter()
+    rf.build(bh, B, bsel)
+    assert rf.publish() == 2
+    keep, kept = rf.probe(ph, Pn)
+
+    def run(sel):
+        k = RowJoinKernels(W.join_spec(fused=True))
+        p = TRowJoinProcessor(k, build_is_left=False)
+        p.append(jd.build)
+        p.build()
+        probe = jd.probe
+        if sel is not None:
+            probe = jd.probe.view(0, Pn)
+            probe.set_selection_buffer(sel)
+        pairs = p.probe_pairs(probe)
+        fused = p.fused_results()
+        p.close()
+        return pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))], fused
+
+    plain, pf = run(None)
+    filtered, ff = run(keep)
+    assert np.array_equal(plain, filtered) and pf == ff
+    assert len(plain) <= kept < len(plain) + (Pn - len(plain)) // 10
+
+
 def test_group_hash_known_answers(gpu_backend):
     """The group-hash entry point returns the reference's rh_hash values,
     worked out by hand from qdb/kernel/gen.cpp:233-379 over the key struct

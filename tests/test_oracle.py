@@ -384,3 +384,77 @@ def test_key_hash_invariants():
     nans = np.array([0.0, -0.0, np.nan, -np.nan, np.frombuffer(np.uint64(0x7FF0000000000123).tobytes(), np.float64)[0], 1.5])
     h = _key_hash([T(FLOAT, 8)], [nans], None, [0])
     assert h[0] == h[1] and h[2] == h[3] == h[4] and len({h[0], h[2], h[5]}) == 3
+
+
+# ---------------------------------------------------------------------------
+# Runtime join filter (qdb/kernel/join/runtime_filter.oz)
+# ---------------------------------------------------------------------------
+
+def test_rf_mix_is_splitmix64():
+    """rf_mix is the splitmix64 finalizer: its first three outputs from seed
+    0 (the state stepped by 0x9E3779B97F4A7C15) are the published
+    0xe220a8397b1dcdaf, 0x6e789e6aa1b965f4, 0x06c45d188009454f."""
+    g = 0x9E3779B97F4A7C15
+    M = (1 << 64) - 1
+    assert [O.LIB.oracle_rf_mix((k * g) & M) for k in (1, 2, 3)] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4,
+                                                                      0x06C45D188009454F]
+
+
+def test_rf_exact_set_is_membership():
+    """Up to MaxKeys distinct hashes the filter is exact: a probe keeps a
+    selected row iff its hash is one of the selected build hashes (hash 0
+    aside, which the table stores as 1)."""
+    rng = np.random.default_rng(2)
+    build = rng.integers(1, 1 << 63, 5000, dtype=np.uint64)
+    bsel = (rng.random(5000) > 0.3).astype(np.uint8)
+    f = O.OracleRuntimeFilter(max_keys=1 << 14)
+    f.build(np.concatenate([build, build[:100]]), np.concatenate([bsel, bsel[:100]]))
+    assert f.publish() == 1
+    probe = np.concatenate([build, rng.integers(1, 1 << 63, 20000, dtype=np.uint64)])
+    psel = (rng.random(len(probe)) > 0.1).astype(np.uint8)
+    out, kept = f.probe(probe, psel)
+    want = np.isin(probe, build[bsel.astype(bool)]) & psel.astype(bool)
+    assert np.array_equal(out.astype(bool), want) and kept == int(want.sum())
+
+
+def test_rf_exact_counts_raw_hashes():
+    """The exact set's keys are raw hashes: 0 and 1 are two distinct keys
+    (both stored as 1 in the table, at their own slots), so a probe of either
+    is kept, and they count as two against MaxKeys."""
+    f = O.OracleRuntimeFilter(max_keys=1)
+    f.build(np.array([0, 1], np.uint64))
+    assert f.publish() == 2
+    f = O.OracleRuntimeFilter(max_keys=2)
+    f.build(np.array([0, 1], np.uint64))
+    assert f.publish() == 1
+    out, kept = f.probe(np.array([0, 1, 2], np.uint64))
+    assert list(out) == [1, 1, 0] and kept == 2
+
+
+def test_rf_switches_to_bloom_past_max_keys():
+    """MaxKeys distinct hashes keep the exact set; one more makes it a Bloom
+    filter, which never rejects a build hash and rejects most others; with
+    under 64 bytes of Bloom storage the filter cannot answer and keeps every
+    selected row."""
+    rng = np.random.default_rng(3)
+    keys = rng.integers(1, 1 << 63, 1001, dtype=np.uint64)
+    f = O.OracleRuntimeFilter(max_keys=1000)
+    f.build(keys[:1000])
+    assert f.publish() == 1
+    f = O.OracleRuntimeFilter(max_keys=1000, bloom_bytes=1 << 16)
+    f.build(keys)
+    assert f.publish() == 2
+    out, kept = f.probe(keys)
+    assert out.all() and kept == len(keys)
+    others = rng.integers(1, 1 << 63, 100000, dtype=np.uint64)
+    _, fp = f.probe(others)
+    # 1001 hashes in 1024 blocks of 512 bits, 8 bits each: a false positive
+    # needs 8 set bits in one block, about (1 - e^-(8 / 512))^8 ~ 3e-15 per
+    # lightly filled block; a handful at most.
+    assert fp < 100
+    f = O.OracleRuntimeFilter(max_keys=10, bloom_bytes=32)
+    f.build(keys)
+    assert f.publish() == 0
+    sel = (rng.random(len(others)) > 0.5).astype(np.uint8)
+    out, kept = f.probe(others, sel)
+    assert np.array_equal(out, sel) and kept == int(sel.sum())
