@@ -534,6 +534,37 @@ int64_t qdb_gpu_rf_probe(const void* filter, const uint64_t* hashes, const uint8
 int64_t qdb_gpu_rf_stats(const void* filter, int64_t* out, int64_t n);
 void qdb_gpu_rf_free(void* filter);
 
+/* Hash shuffle routing (THashShuffleTask::OutputOf, qdb/scheduler/tasks/
+ * runtime_adapter.cpp:998-1010): every selected row of `rs` goes to output
+ * hash % outputs, or to routes[hash % nroutes] with a routing table (host
+ * array; an entry outside [0, outputs) routes a partition nowhere, and a row
+ * of such a partition is an error with the reference's text). The hash is
+ * rs->Hash when the batch has one (computed upstream on the same key, the
+ * reference's UseInputHash), else `hashes` (device, e.g. the group-hash entry
+ * point's rh_hash). The routing is a stable device partition: row_index
+ * (device, RowCount entries) lists the selected rows grouped by output, each
+ * output's rows in input order, output o's at [offsets[o], offsets[o+1])
+ * (offsets: host, outputs + 1 entries). At most 4096 outputs. 0 or -1. */
+int qdb_gpu_shuffle_partition(const TRowSet* rs, const uint64_t* hashes, int32_t outputs, const int32_t* routes,
+    int32_t nroutes, uint32_t* row_index, int64_t* offsets);
+/* Materializes rows row_index[0..count) (device) of a device rowset as a new
+ * device rowset: every column (bool data and validity as bitmaps), and the
+ * hashes and selection when the batch has them; kernel-owned, Destroy set
+ * (qdb_gpu_destroy_kernel_rowset). 0 or -1. */
+int qdb_gpu_gather_rowset(const TRowSet* rs, const QdbColumnSchema* types, const uint32_t* row_index, int64_t count,
+    TRowSet* out);
+
+/* The single-row partial states of a batch for an aggregate: the projection
+ * that the reference's adaptive partial aggregate passes rows through once
+ * hashing stops paying (docs/arch/aggregation.md:376-389): the keys as they
+ * are, 1 for COUNT(*), IS NOT NULL (0/1) for COUNT(x), the argument of
+ * SUM/MIN/MAX in the state's type; one row per input row, the batch's
+ * selection copied (a final aggregate cannot tell these rows from emitted
+ * groups). Columns as agg_finish_rowset's output; kernel-owned. Not
+ * available for an aggregate with a fused predicate or computed columns.
+ * 0 or -1. */
+int qdb_gpu_agg_partial_states(const QdbAggregateKernels* kernels, const TRowSet* batch, TRowSet* out);
+
 /* Kernel timing of the last launches, for roofline reporting: the average
  * duration in milliseconds of the named kernel family ("agg_update",
  * "join_probe", ...) measured with HIP events on the stream it runs on since
@@ -630,6 +661,23 @@ void* qdb_gpu_join_processor_arena(void* processor);
 /* Next queued output batch: 1 when one was returned, 0 when none is left. */
 int qdb_gpu_join_processor_next(void* processor, TRowSet* out);
 void qdb_gpu_join_processor_free(void* processor);
+/* The partial aggregate (TPartialAggregateProcessor, qdb/exec/
+ * aggregate_exec.cpp:158-233; TPartialAggregatePolicy, qdb/exec/
+ * plan_data.h:234-250, defaults 131072 / 8 / 2^20): add consumes a batch;
+ * the table emits its groups into a pending output when it reaches
+ * max_groups and restarts; with pass_through, once sample_rows rows were
+ * hashed at fewer than min_reduction rows per produced group, the table is
+ * emitted and every later batch comes out as its partial states
+ * (qdb_gpu_agg_partial_states). next returns 1 with a pending output
+ * (adopted), 0 when none; finish returns the table's last groups' row count,
+ * 0 when rows were passed through (no table remains), -1 on error. */
+void* qdb_gpu_partial_agg_new(const QdbAggregateKernels* kernels, int64_t initial_capacity, uint64_t sample_rows,
+    uint64_t min_reduction, int64_t max_groups, int pass_through);
+int qdb_gpu_partial_agg_add(void* processor, const TRowSet* batch);
+int qdb_gpu_partial_agg_next(void* processor, TRowSet* out);
+int64_t qdb_gpu_partial_agg_finish(void* processor, TRowSet* out);
+int qdb_gpu_partial_agg_passing_through(void* processor);
+void qdb_gpu_partial_agg_free(void* processor);
 /* Calls rs->Destroy when set, else qdb_gpu_destroy_kernel_rowset. */
 void qdb_gpu_rowset_release(TRowSet* rs);
 

@@ -121,6 +121,9 @@ class HostRowSet:
     def __init__(self, types, columns, valid=None, selection=None, hashes=None):
         self.types = list(types)
         self.keep = []
+        self.columns = list(columns)
+        self.valid = list(valid) if valid is not None else [None] * len(self.types)
+        self.selection = selection
         n = None
         cols = (abi.TColumn * max(1, len(self.types)))()
         for i, (t, data) in enumerate(zip(self.types, columns)):
@@ -375,3 +378,119 @@ class OracleRuntimeFilter:
             LIB.oracle_rf_free(self.f)
         except Exception:
             pass
+
+
+def key_hashes(rowset: HostRowSet, types_c, key_cols: Sequence[int]) -> np.ndarray:
+    """rh_hash of the key columns of every row (oracle_key_hash); types_c:
+    the columns' QdbValueType structs."""
+    tt = (abi.QdbValueType * max(1, len(types_c)))(*[abi.QdbValueType.from_buffer_copy(t) for t in types_c])
+    kk = (C.c_int32 * max(1, len(key_cols)))(*key_cols)
+    n = rowset.rs.RowCount
+    return np.array([LIB.oracle_key_hash(rowset.rs.Columns, tt, kk, len(key_cols), i) for i in range(n)], dtype=np.uint64)
+
+
+def shuffle_partition(hashes: np.ndarray, selection: Optional[np.ndarray], outputs: int,
+                      routes: Optional[Sequence[int]] = None):
+    """The hash shuffle's routing of one batch: every selected row, in input
+    order, goes to output hash % outputs, or with a routing table to
+    routes[hash % len(routes)], where an entry outside [0, outputs) is an
+    error (THashShuffleTask::OutputOf, qdb/scheduler/tasks/
+    runtime_adapter.cpp:998-1010, over the row loop of :1100-1105).
+    Returns (row_index, offsets): output o's rows at
+    row_index[offsets[o]:offsets[o + 1]]."""
+    per = [[] for _ in range(outputs)]
+    for row in range(len(hashes)):
+        if selection is not None and not selection[row]:
+            continue
+        h = int(hashes[row])
+        if routes is None:
+            out = h % outputs
+        else:
+            part = h % len(routes)
+            out = int(routes[part])
+            if out < 0 or out >= outputs:
+                raise ValueError(f"hash shuffle got a row of partition {part}, which it routes nowhere")
+        per[out].append(row)
+    offsets = np.zeros(outputs + 1, dtype=np.int64)
+    for o in range(outputs):
+        offsets[o + 1] = offsets[o] + len(per[o])
+    rows = np.array([r for p in per for r in p], dtype=np.uint32)
+    return rows, offsets
+
+
+def partial_states(spec, rowset: HostRowSet):
+    """The single-row partial states of a batch's selected rows
+    (BuildAggregatePassThroughOperator, qdb/exec/planner_helpers.cpp:211-260):
+    the keys, 1 for COUNT(*), 0 for a NULL argument of COUNT(x) and 1
+    otherwise, the argument of SUM/MIN/MAX as it is (the state type widens
+    it). Returns per output column (values, valid) over the selected rows."""
+    sel = np.ones(rowset.rs.RowCount, dtype=bool) if rowset.selection is None else np.asarray(rowset.selection, bool)
+    out = []
+    for k in range(spec.NumKeys):
+        c = spec.KeyColumns[k]
+        v = rowset.valid[c]
+        out.append((np.asarray(rowset.columns[c])[sel] if not isinstance(rowset.columns[c], list)
+                    else [x for x, s in zip(rowset.columns[c], sel) if s], None if v is None else np.asarray(v)[sel]))
+    for a in range(spec.NumAggs):
+        fn = spec.Aggs[a]
+        if fn.Func == abi.AGG_COUNT_STAR:
+            out.append((np.ones(int(sel.sum()), dtype=np.int64), None))
+        elif fn.Func == abi.AGG_COUNT:
+            v = rowset.valid[fn.Column]
+            ones = np.ones(rowset.rs.RowCount, dtype=np.int64) if v is None else np.asarray(v).astype(np.int64)
+            out.append((ones[sel], None))
+        else:
+            c = fn.Column
+            v = rowset.valid[c]
+            col = rowset.columns[c]
+            vals = np.asarray(col)[sel] if not isinstance(col, list) else [x for x, s in zip(col, sel) if s]
+            out.append((vals, None if v is None else np.asarray(v)[sel]))
+    return out
+
+
+class PartialAggregate:
+    """The partial aggregate's policy (TPartialAggregateProcessor::Add,
+    qdb/exec/aggregate_exec.cpp:180-216): after each batch the table's group
+    count decides whether the table is emitted (max_groups) or whether the
+    lane stops hashing (sample_rows rows at fewer than min_reduction rows per
+    group, emitted groups included). The table is the oracle aggregate of
+    the batches since the last emission. outputs lists ("table", result) and
+    ("states", batch) in order."""
+
+    def __init__(self, spec, sample_rows=128 * 1024, min_reduction=8, max_groups=1 << 20, pass_through=True):
+        self.spec = spec
+        self.sample_rows = sample_rows
+        self.min_reduction = min_reduction
+        self.max_groups = max_groups
+        self.pass_through = pass_through
+        self.table = []
+        self.hashed_rows = 0
+        self.emitted_groups = 0
+        self.passing = False
+        self.outputs = []
+
+    def _groups(self):
+        if not self.table:
+            return 0
+        return len(aggregate(self.spec, self.table)[0][0]) if self.spec.NumKeys else 1
+
+    def _flush(self):
+        if self.table and self._groups() > 0:
+            self.outputs.append(("table", aggregate(self.spec, self.table)))
+        self.table = []
+
+    def add(self, batch: HostRowSet):
+        if self.passing:
+            self.outputs.append(("states", batch))
+            return
+        self.hashed_rows += batch.rs.RowCount
+        self.table.append(batch)
+        groups = self._groups()
+        if (self.pass_through and self.hashed_rows >= self.sample_rows
+                and self.hashed_rows < self.min_reduction * (self.emitted_groups + groups)):
+            self.passing = True
+            self._flush()
+            return
+        if groups >= self.max_groups:
+            self.emitted_groups += groups
+            self._flush()

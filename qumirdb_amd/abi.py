@@ -211,6 +211,9 @@ EXPORTS = [
     "qdb_gpu_join_processor_arena", "qdb_gpu_join_processor_next", "qdb_gpu_join_processor_free",
     "qdb_gpu_rowset_release", "qdb_gpu_set_host_heap", "qdb_gpu_destroy_kernel_rowset", "qdb_gpu_adopt_kernel_rowset",
     "qdb_gpu_set_row_error_handler", "qdb_gpu_take_row_error", "qdb_gpu_join_probe_fused_noalloc",
+    "qdb_gpu_shuffle_partition", "qdb_gpu_gather_rowset", "qdb_gpu_agg_partial_states", "qdb_gpu_partial_agg_new",
+    "qdb_gpu_partial_agg_add", "qdb_gpu_partial_agg_next", "qdb_gpu_partial_agg_finish", "qdb_gpu_partial_agg_passing_through",
+    "qdb_gpu_partial_agg_free",
     "qdb_gpu_rf_new", "qdb_gpu_rf_build", "qdb_gpu_rf_publish", "qdb_gpu_rf_probe", "qdb_gpu_rf_stats", "qdb_gpu_rf_free",
 ]
 
@@ -284,6 +287,15 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "qdb_gpu_take_row_error": (i64, []),
         "qdb_gpu_join_probe_fused_noalloc": (C.c_int, [P, C.POINTER(TRowSet), C.POINTER(QdbFusedResult), i64,
                                                        C.POINTER(i64)]),
+        "qdb_gpu_shuffle_partition": (C.c_int, [C.POINTER(TRowSet), P, C.c_int32, P, C.c_int32, P, C.POINTER(i64)]),
+        "qdb_gpu_gather_rowset": (C.c_int, [C.POINTER(TRowSet), C.POINTER(QdbColumnSchema), P, i64, C.POINTER(TRowSet)]),
+        "qdb_gpu_agg_partial_states": (C.c_int, [C.POINTER(QdbAggregateKernels), C.POINTER(TRowSet), C.POINTER(TRowSet)]),
+        "qdb_gpu_partial_agg_new": (P, [C.POINTER(QdbAggregateKernels), i64, C.c_uint64, C.c_uint64, i64, C.c_int]),
+        "qdb_gpu_partial_agg_add": (C.c_int, [P, C.POINTER(TRowSet)]),
+        "qdb_gpu_partial_agg_next": (C.c_int, [P, C.POINTER(TRowSet)]),
+        "qdb_gpu_partial_agg_finish": (i64, [P, C.POINTER(TRowSet)]),
+        "qdb_gpu_partial_agg_passing_through": (C.c_int, [P]),
+        "qdb_gpu_partial_agg_free": (None, [P]),
         "qdb_gpu_rf_new": (P, [i64, i64]),
         "qdb_gpu_rf_build": (C.c_int, [P, P, P, i64]),
         "qdb_gpu_rf_publish": (C.c_int, [P]),

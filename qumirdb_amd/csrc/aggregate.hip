@@ -272,11 +272,18 @@ struct AggPlan {
     QdbValueType slot_type[kMaxDevCols];
     std::string jit_src;
     bool jit_tried = false;
+    // The partial-state projection (qdb_gpu_agg_partial_states): compiled at
+    // its first use, its project slot released with the plan.
+    std::mutex partial_mu;
+    bool partial_tried = false;
+    QdbProjectFn partial_fn = nullptr;
+    int partial_slot = -1;
 
     ~AggPlan()
     {
         dfree(d_pred);
         dfree(d_progs);
+        if (partial_slot >= 0) qdb_gpu_release_slot(QDB_SLOT_PROJECT, partial_slot);
     }
 };
 
@@ -2029,6 +2036,138 @@ extern "C" int qdb_gpu_agg_finish_batched(const QdbAggregateKernels* kernels, vo
         HtBlob* B = (HtBlob*)hts[i];
         table_free(B->table);
         memset(B, 0, sizeof(*B));
+    }
+    return 0;
+}
+
+namespace {
+
+// The projection that turns one input row into the partial states of its
+// group (the pass-through of the reference's adaptive partial aggregate,
+// docs/arch/aggregation.md:376-389): the keys as they are, 1 for COUNT(*),
+// IS NOT NULL for COUNT(x), and the argument of SUM/MIN/MAX stored in the
+// state's type. False (with the error text) when the aggregate has fused
+// stages, which a projection of its input cannot reproduce.
+bool partial_spec_of(const AggPlan* P, QdbProjectSpec* ps)
+{
+    const QdbAggregateSpec& a = P->spec;
+    if (a.HasPredicate || a.NumComputed > 0) return fail("partial states: the aggregate has fused stages");
+    if (P->nout > QDB_PROJECT_MAX_OUTPUTS) {
+        return fail("partial states: %d output columns exceed the projection's %d", P->nout, QDB_PROJECT_MAX_OUTPUTS);
+    }
+    memset(ps, 0, sizeof(*ps));
+    ps->Input = a.Input;
+    ps->NumOutputs = P->nout;
+    auto col = [](QdbExprProgram& p, int c, int cls) {
+        QdbExprInsn& in = p.Insns[p.NumInsns++];
+        in.Op = QDB_EXPR_COLUMN;
+        in.Dst = 0;
+        in.Class = (uint8_t)cls;
+        in.Column = c;
+        p.Result = 0;
+    };
+    int k = 0;
+    for (int key = 0; key < a.NumKeys; ++key, ++k) {
+        const int c = a.KeyColumns[key];
+        col(ps->Outputs[k], c, a.Input.Types[c].Kind == QDB_KIND_FLOAT ? QDB_REG_F64 : QDB_REG_INT);
+        ps->OutputTypes[k] = P->out_types[k];
+    }
+    for (int g = 0; g < a.NumAggs; ++g, ++k) {
+        const QdbAggregateFn& fn = a.Aggs[g];
+        QdbExprProgram& p = ps->Outputs[k];
+        ps->OutputTypes[k] = P->out_types[k];
+        if (fn.Func == QDB_AGG_COUNT_STAR || (fn.Func == QDB_AGG_COUNT && !a.Input.Types[fn.Column].Nullable)) {
+            QdbExprInsn& in = p.Insns[p.NumInsns++];
+            in.Op = QDB_EXPR_CONST_INT;
+            in.Class = QDB_REG_INT;
+            in.ImmLo = 1;
+            p.Result = 0;
+        } else if (fn.Func == QDB_AGG_COUNT) {
+            col(p, fn.Column, a.Input.Types[fn.Column].Kind == QDB_KIND_FLOAT ? QDB_REG_F64 : QDB_REG_INT);
+            QdbExprInsn& isnull = p.Insns[p.NumInsns++];
+            isnull.Op = QDB_EXPR_IS_NULL;
+            isnull.Dst = 1;
+            isnull.A = 0;
+            isnull.Class = QDB_REG_BOOL;
+            QdbExprInsn& nt = p.Insns[p.NumInsns++];
+            nt.Op = QDB_EXPR_NOT;
+            nt.Dst = 2;
+            nt.A = 1;
+            nt.Class = QDB_REG_BOOL;
+            p.Result = 2;
+        } else {
+            col(p, fn.Column, a.Input.Types[fn.Column].Kind == QDB_KIND_FLOAT ? QDB_REG_F64 : QDB_REG_INT);
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" int qdb_gpu_agg_partial_states(const QdbAggregateKernels* kernels, const TRowSet* batch, TRowSet* out)
+{
+    clear_error();
+    if (!kernels || !batch || !out) return fail("null argument"), -1;
+    memset(out, 0, sizeof(*out));
+    if (kernels->Slot < 0 || kernels->Slot >= kSlots) return fail("bad aggregate slot %lld", (long long)kernels->Slot), -1;
+    AggPlan* P = agg_slots().get((int)kernels->Slot);
+    if (!P) return fail("aggregate kernels are not compiled"), -1;
+    {
+        std::lock_guard<std::mutex> lock(P->partial_mu);
+        if (!P->partial_tried) {
+            P->partial_tried = true;
+            QdbProjectSpec ps;
+            if (partial_spec_of(P, &ps)) {
+                QdbProjectFn fn = nullptr;
+                const int slot = qdb_gpu_compile_project(&ps, &fn);
+                if (slot >= 0) {
+                    P->partial_fn = fn;
+                    P->partial_slot = slot;
+                }
+            }
+        }
+        if (!P->partial_fn) return fail("partial states: no projection for this aggregate"), -1;
+    }
+    const int64_t n = batch->RowCount;
+    OwnedRowSet* o = init_owned_rowset(out, n, P->nout);
+    std::vector<void*> bufs((size_t)P->nout * 2, nullptr);
+    const size_t mask_bytes = (size_t)((n + 7) / 8) + 8;
+    bool ok = true;
+    for (int c = 0; ok && c < P->nout; ++c) {
+        const QdbValueType& t = P->out_types[c];
+        const size_t bytes = t.Kind == QDB_KIND_BOOL ? mask_bytes : (size_t)(n > 0 ? n : 1) * (size_t)t.Width + 8;
+        char* data = (char*)dalloc(bytes);
+        ok = data != nullptr;
+        if (!ok) break;
+        o->dev.push_back(data);
+        o->cols[(size_t)c].Data = data;
+        bufs[(size_t)c] = data;
+        if (t.Nullable) {
+            uint8_t* m = (uint8_t*)dalloc(mask_bytes);
+            ok = m != nullptr && zero_fill(m, mask_bytes);
+            if (!ok) break;
+            o->dev.push_back(m);
+            o->cols[(size_t)c].Mask = m;
+            bufs[(size_t)(P->nout + c)] = m;
+        }
+    }
+    // The states of the selected rows; the batch's selection comes along.
+    if (ok && batch->Selection && n > 0) {
+        uint8_t* sel = (uint8_t*)dalloc((size_t)n);
+        ok = sel && hip_ok(hipMemcpyAsync(sel, batch->Selection, (size_t)n, hipMemcpyDeviceToDevice, stream()), "selection copy");
+        if (sel) o->dev.push_back(sel);
+        out->Selection = sel;
+    }
+    if (ok && n > 0) {
+        TRowSet view = *batch;
+        P->partial_fn(&view, bufs.data(), nullptr, nullptr, batch->Selection);
+        ok = last_error_text().empty();
+    }
+    ok = ok && sync("partial states");
+    if (!ok || !publish_rowset(out)) {
+        destroy_kernel_rowset(out);
+        memset(out, 0, sizeof(*out));
+        return -1;
     }
     return 0;
 }

@@ -173,6 +173,79 @@ int64_t TAggregateDriver::Groups() const
 }
 
 // ---------------------------------------------------------------------------
+// Partial aggregate
+// ---------------------------------------------------------------------------
+
+TPartialAggregateDriver::TPartialAggregateDriver(const QdbAggregateKernels& kernels, const TAggregatePlan& plan,
+    const TPartialPolicy& policy, bool passThrough)
+    : Kernels_(kernels)
+    , Plan_(plan)
+    , Policy_(policy)
+    , PassThrough_(passThrough)
+    , Table_(new TAggregateDriver(kernels, plan))
+{
+    // An aggregate whose partial-state projection cannot be built keeps the
+    // bounded table and never passes rows through (the results are the same).
+    if (PassThrough_) {
+        TRowSet empty{};
+        TRowSet probe{};
+        PassThrough_ = qdb_gpu_agg_partial_states(&Kernels_, &empty, &probe) == 0;
+        if (PassThrough_) qdb_gpu_destroy_kernel_rowset(&probe);
+    }
+}
+
+void TPartialAggregateDriver::Flush()
+{
+    // Emits the table's groups (when it has any) and starts a new table.
+    if (Table_->Groups() > 0) {
+        TRowSet out{};
+        Table_->Emit(out);
+        Pending_.push_back(out);
+    }
+    Table_.reset(new TAggregateDriver(Kernels_, Plan_));
+}
+
+void TPartialAggregateDriver::Consume(const TRowSet& batch)
+{
+    if (PassingThrough_) {
+        TRowSet states{};
+        if (qdb_gpu_agg_partial_states(&Kernels_, &batch, &states) != 0) throw TDriverError("partial states failed");
+        AdoptOutput(states);
+        Pending_.push_back(states);
+        return;
+    }
+    HashedRows_ += (uint64_t)batch.RowCount;
+    Table_->Consume(batch);
+    const int64_t groups = Table_->Groups();
+    // Every group this lane produced counts, emitted or held.
+    const double produced = (double)(EmittedGroups_ + (uint64_t)groups);
+    if (PassThrough_ && HashedRows_ >= Policy_.SampleRows && (double)HashedRows_ < (double)Policy_.MinReduction * produced) {
+        PassingThrough_ = true;
+        Flush();
+        return;
+    }
+    if (groups >= Policy_.MaxGroups) {
+        EmittedGroups_ += (uint64_t)groups;
+        Flush();
+    }
+}
+
+bool TPartialAggregateDriver::Next(TRowSet& out)
+{
+    if (Pending_.empty()) return false;
+    out = Pending_.front();
+    Pending_.pop_front();
+    return true;
+}
+
+bool TPartialAggregateDriver::Finish(TRowSet& out)
+{
+    if (PassingThrough_) return false;
+    Table_->Emit(out);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // Row join
 // ---------------------------------------------------------------------------
 
@@ -373,6 +446,57 @@ int qdb_gpu_join_processor_next(void* p, TRowSet* out)
 void qdb_gpu_join_processor_free(void* p)
 {
     delete static_cast<TJoinDriver*>(p);
+}
+
+void* qdb_gpu_partial_agg_new(const QdbAggregateKernels* kernels, int64_t initial_capacity, uint64_t sample_rows,
+    uint64_t min_reduction, int64_t max_groups, int pass_through)
+{
+    NQdbGpu::TPartialAggregateDriver* d = nullptr;
+    Guard([&] {
+        NQdbGpu::TAggregatePlan plan;
+        plan.InitialCapacity = initial_capacity;
+        NQdbGpu::TPartialPolicy policy;
+        policy.SampleRows = sample_rows;
+        policy.MinReduction = min_reduction;
+        policy.MaxGroups = max_groups;
+        d = new NQdbGpu::TPartialAggregateDriver(*kernels, plan, policy, pass_through != 0);
+    });
+    return d;
+}
+
+int qdb_gpu_partial_agg_add(void* p, const TRowSet* batch)
+{
+    return Guard([&] { static_cast<NQdbGpu::TPartialAggregateDriver*>(p)->Consume(*batch); });
+}
+
+int qdb_gpu_partial_agg_next(void* p, TRowSet* out)
+{
+    return static_cast<NQdbGpu::TPartialAggregateDriver*>(p)->Next(*out) ? 1 : 0;
+}
+
+int64_t qdb_gpu_partial_agg_finish(void* p, TRowSet* out)
+{
+    int64_t rows = -1;
+    if (Guard([&] {
+            memset(out, 0, sizeof(*out));
+            rows = static_cast<NQdbGpu::TPartialAggregateDriver*>(p)->Finish(*out) ? out->RowCount : 0;
+        }) != 0) {
+        return -1;
+    }
+    return rows;
+}
+
+int qdb_gpu_partial_agg_passing_through(void* p)
+{
+    return static_cast<NQdbGpu::TPartialAggregateDriver*>(p)->PassingThrough() ? 1 : 0;
+}
+
+void qdb_gpu_partial_agg_free(void* p)
+{
+    auto* d = static_cast<NQdbGpu::TPartialAggregateDriver*>(p);
+    TRowSet rs{};
+    while (d->Next(rs)) qdb_gpu_rowset_release(&rs);
+    delete d;
 }
 
 void qdb_gpu_rowset_release(TRowSet* rs)

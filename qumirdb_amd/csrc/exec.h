@@ -26,6 +26,7 @@
 #include <array>
 #include <cstdint>
 #include <deque>
+#include <memory>
 #include <stdexcept>
 #include <vector>
 
@@ -82,6 +83,47 @@ private:
     TAggregatePlan Plan_;
     std::array<uint8_t, QDB_HASH_TABLE_SIZE> Table_ = {};
     EState State_ = EState::Fresh;
+};
+
+// The bounded, adaptive partial aggregate (TPartialAggregatePolicy,
+// qdb/exec/plan_data.h:234-250; behaviour of TPartialAggregateProcessor,
+// qdb/exec/aggregate_exec.cpp:158-233): a table that emits its groups and
+// restarts when it reaches MaxGroups, and that, once SampleRows input rows
+// have been hashed without an average of MinReduction rows per group
+// (emitted groups included), emits its table and passes every later batch
+// through as single-row partial states (qdb_gpu_agg_partial_states). The
+// decision is one-way.
+struct TPartialPolicy {
+    uint64_t SampleRows = 128 * 1024;
+    uint64_t MinReduction = 8;
+    int64_t MaxGroups = 1 << 20;
+};
+
+class TPartialAggregateDriver {
+public:
+    TPartialAggregateDriver(const QdbAggregateKernels& kernels, const TAggregatePlan& plan, const TPartialPolicy& policy,
+        bool passThrough);
+    // Consumes a batch; an output may then be ready (Next).
+    void Consume(const TRowSet& batch);
+    // The pending output (adopted); false when none.
+    bool Next(TRowSet& out);
+    // The table's remaining groups; false when rows are passed through (the
+    // table was emitted when that began).
+    bool Finish(TRowSet& out);
+    bool PassingThrough() const { return PassingThrough_; }
+
+private:
+    void Flush();
+
+    QdbAggregateKernels Kernels_;
+    TAggregatePlan Plan_;
+    TPartialPolicy Policy_;
+    bool PassThrough_;
+    std::unique_ptr<TAggregateDriver> Table_;
+    std::deque<TRowSet> Pending_;
+    uint64_t HashedRows_ = 0;
+    uint64_t EmittedGroups_ = 0;
+    bool PassingThrough_ = false;
 };
 
 class TJoinDriver {

@@ -537,6 +537,16 @@ class AggSpec:
         return list(self.types) + [p.result_type(r) for p, r in self.computed]
 
 
+def combine_spec(spec: AggSpec, partial_types: Sequence[ValueType]) -> AggSpec:
+    """The final aggregate over a partial aggregate's outputs
+    (BuildAggregateCombineOperator, qdb/exec/planner_helpers.cpp:191-209):
+    the same keys, COUNT and COUNT(*) become SUM, SUM/MIN/MAX stay, each
+    reading its own partial output column."""
+    nk = len(spec.keys)
+    aggs = [(abi.AGG_SUM if f in (abi.AGG_COUNT, abi.AGG_COUNT_STAR) else f, nk + i) for i, (f, _) in enumerate(spec.aggs)]
+    return AggSpec(types=list(partial_types), keys=list(range(nk)), aggs=aggs)
+
+
 class AggregateKernels:
     """Compiled agg_dispatch / agg_finish_rowset of one spec."""
 
@@ -821,3 +831,130 @@ class GroupHash:
         if not self.fn(rowset.ptr, out.ptr, kc):
             raise RuntimeError(f"group hash: {lib().qdb_gpu_last_error().decode()}")
         return out
+
+
+class KernelRowSet(RowSetView):
+    """A device rowset made by the library and owned here: released by
+    close() (or when collected)."""
+
+    def close(self):
+        if self.rs is not None:
+            lib().qdb_gpu_rowset_release(C.byref(self.rs))
+            self.rs = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def shuffle_partition(rowset, outputs: int, routes: Optional[Sequence[int]] = None,
+                      hashes: Optional[DeviceBuffer] = None):
+    """The hash shuffle's routing of one batch (qdb_gpu_shuffle_partition):
+    (row_index, offsets) with output o's rows, in input order, at
+    row_index[offsets[o]:offsets[o + 1]] (row_index a DeviceBuffer of uint32,
+    offsets a host int64 array)."""
+    idx = DeviceBuffer(max(rowset.rows, 1) * 4)
+    offs = np.zeros(outputs + 1, dtype=np.int64)
+    r = None
+    if routes is not None:
+        r = np.ascontiguousarray(np.asarray(routes, dtype=np.int32))
+    check(lib().qdb_gpu_shuffle_partition(rowset.ptr, hashes.ptr if hashes is not None else None, outputs,
+                                          r.ctypes.data if r is not None else None, len(r) if r is not None else 0,
+                                          idx.ptr, offs.ctypes.data_as(C.POINTER(C.c_int64))), "shuffle_partition")
+    return idx, offs
+
+
+def gather_rowset(rowset, row_index: DeviceBuffer, start: int, count: int) -> KernelRowSet:
+    """Rows row_index[start:start + count] of a device batch as a new device
+    batch (qdb_gpu_gather_rowset)."""
+    schema = schema_c(rowset.types)
+    out = abi.TRowSet()
+    check(lib().qdb_gpu_gather_rowset(rowset.ptr, C.byref(schema), (row_index.ptr or 0) + 4 * start, count,
+                                      C.byref(out)), "gather_rowset")
+    return KernelRowSet(out, rowset.types)
+
+
+class HashShuffle:
+    """THashShuffleTask (qdb/scheduler/tasks/runtime_adapter.cpp:960-1030)
+    over one batch on the device: each selected row goes to output
+    rh_hash(keys) % outputs (or through a routing table), each output's rows
+    in input order. The batch's own Hash column is used when it has one."""
+
+    def __init__(self, types: Sequence[ValueType], key_cols: Sequence[int], outputs: int,
+                 routes: Optional[Sequence[int]] = None):
+        self.types = list(types)
+        self.key_cols = list(key_cols)
+        self.outputs = int(outputs)
+        self.routes = routes
+        self.hash = GroupHash(self.types, len(self.key_cols))
+
+    def partition(self, rowset):
+        hashes = None if rowset.rs.Hash else self.hash(rowset, self.key_cols)
+        return shuffle_partition(rowset, self.outputs, self.routes, hashes)
+
+    def __call__(self, rowset) -> List[KernelRowSet]:
+        idx, offs = self.partition(rowset)
+        return [gather_rowset(rowset, idx, int(offs[o]), int(offs[o + 1] - offs[o])) for o in range(self.outputs)]
+
+
+def partial_states(kernels: AggregateKernels, batch) -> KernelRowSet:
+    """A batch's single-row partial states (qdb_gpu_agg_partial_states)."""
+    out = abi.TRowSet()
+    check(lib().qdb_gpu_agg_partial_states(C.byref(kernels.k), batch.ptr, C.byref(out)), "agg_partial_states")
+    lib().qdb_gpu_adopt_kernel_rowset(C.byref(out))
+    return KernelRowSet(out, kernels.out_types)
+
+
+class TPartialAggregateProcessor:
+    """TPartialAggregateProcessor (qdb/exec/aggregate_exec.cpp:158-233) over
+    the C-ABI: a bounded table that emits at max_groups groups and, with
+    pass_through, passes rows through as partial states once sample_rows
+    rows were hashed at fewer than min_reduction rows per group."""
+
+    def __init__(self, kernels: AggregateKernels, initial_capacity: int = 8, sample_rows: int = 128 * 1024,
+                 min_reduction: int = 8, max_groups: int = 1 << 20, pass_through: bool = True):
+        self.kernels = kernels
+        self.p = lib().qdb_gpu_partial_agg_new(C.byref(kernels.k), initial_capacity, sample_rows, min_reduction,
+                                               max_groups, int(pass_through))
+        if not self.p:
+            raise RuntimeError(f"partial aggregate: {(lib().qdb_gpu_exec_last_error() or b'').decode()}")
+
+    def add(self, batch) -> None:
+        if lib().qdb_gpu_partial_agg_add(self.p, batch.ptr) != 0:
+            raise RuntimeError((lib().qdb_gpu_exec_last_error() or b"").decode())
+
+    def take(self) -> List[KernelRowSet]:
+        """The outputs emitted so far."""
+        outs = []
+        while True:
+            rs = abi.TRowSet()
+            if not lib().qdb_gpu_partial_agg_next(self.p, C.byref(rs)):
+                return outs
+            outs.append(KernelRowSet(rs, self.kernels.out_types))
+
+    def finish(self) -> Optional[KernelRowSet]:
+        """The table's last groups; None when rows were passed through."""
+        rs = abi.TRowSet()
+        n = lib().qdb_gpu_partial_agg_finish(self.p, C.byref(rs))
+        if n < 0:
+            raise RuntimeError((lib().qdb_gpu_exec_last_error() or b"").decode())
+        if not lib().qdb_gpu_partial_agg_passing_through(self.p):
+            return KernelRowSet(rs, self.kernels.out_types)
+        return None
+
+    @property
+    def passing_through(self) -> bool:
+        return bool(lib().qdb_gpu_partial_agg_passing_through(self.p))
+
+    def close(self):
+        if self.p:
+            lib().qdb_gpu_partial_agg_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

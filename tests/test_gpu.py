@@ -1014,3 +1014,230 @@ def test_served_results_same_alone_and_in_the_mix(gpu_backend):
         assert j.result == (int(hit.sum()), int(hit.sum()), int(val[hit].sum()))
     finally:
         res.close()
+
+
+# ---------------------------------------------------------------------------
+# Hash shuffle and partial aggregate
+# ---------------------------------------------------------------------------
+
+def _per_output(idx, offs):
+    rows = idx.to_numpy(np.uint32, int(offs[-1])) if offs[-1] else np.zeros(0, np.uint32)
+    return [rows[offs[o]:offs[o + 1]].tolist() for o in range(len(offs) - 1)]
+
+
+def test_shuffle_reference_goldens(gpu_backend):
+    """The reference's routing cases (test/test_scheduler_runtime_adapter.cpp:
+    1496-1536, 1538-1562) through qdb_gpu_shuffle_partition, the batch's own
+    Hash column giving the hashes."""
+    H = gpu_backend.H
+
+    def batch_with_hashes(h):
+        d = gpu_backend.dev(Batch([T(SIGNED, 8)], [np.arange(len(h), dtype=np.int64)]))
+        hb = H.DeviceBuffer.from_numpy(np.array(h, np.uint64))
+        d.rs.Hash = hb.ptr
+        d.buffers.append(hb)
+        return d
+
+    d = batch_with_hashes([0, 1, 2, 3, 5])
+    assert _per_output(*H.shuffle_partition(d, 2)) == [[0, 2], [1, 3, 4]]
+    assert _per_output(*H.shuffle_partition(d, 2, routes=[1, 0, 0, 1])) == [[1, 2, 4], [0, 3]]
+    with pytest.raises(RuntimeError, match="partition 1, which it routes nowhere"):
+        H.shuffle_partition(batch_with_hashes([0, 3]), 2, routes=[0, -1])
+    # A routing table whose only bad entry no row reaches is fine.
+    assert _per_output(*H.shuffle_partition(batch_with_hashes([0, 2]), 2, routes=[1, -1])) == [[], [0, 1]]
+
+
+@pytest.mark.parametrize("outputs", [1, 3, 64, 300, 4096])
+def test_shuffle_partition_matches_oracle(gpu_backend, outputs):
+    """Routing of a 200K-row batch with a selection over a two-column key: the
+    device rh_hash and the device partition equal the oracle's, row for row."""
+    H = gpu_backend.H
+    rng = np.random.default_rng(outputs)
+    n = 200_003
+    types = [T(SIGNED, 8), T(SIGNED, 4, nullable=True)]
+    cols = [rng.integers(-10 ** 12, 10 ** 12, n).astype(np.int64), rng.integers(-1000, 1000, n).astype(np.int32)]
+    valid = [None, rng.random(n) > 0.2]
+    sel = (rng.random(n) > 0.3).astype(np.uint8)
+    batch = Batch(types, cols, valid, sel)
+    d = gpu_backend.dev(batch)
+    vts = [H.ValueType(t.kind, t.width, t.scale, t.nullable) for t in types]
+    gh = H.GroupHash(vts, 2)(d, [0, 1])
+    hb = OracleBackend().host(batch)
+    want_h = O.key_hashes(hb, [cases.value_type_c(t) for t in types], [0, 1])
+    # The entry point hashes the selected rows only.
+    keep = sel.astype(bool)
+    assert np.array_equal(gh.to_numpy(np.uint64, n)[keep], want_h[keep])
+    idx, offs = H.shuffle_partition(d, outputs, hashes=gh)
+    want_rows, want_offs = O.shuffle_partition(want_h, sel, outputs)
+    assert np.array_equal(offs, want_offs)
+    assert np.array_equal(idx.to_numpy(np.uint32, int(offs[-1])), want_rows)
+
+
+def test_shuffle_full_size_properties(gpu_backend):
+    """20M rows over 4096 outputs (a 13-bit sort key): each output's count is
+    the histogram of the selected rows' residues, each output's rows ascend,
+    and together they are exactly the selected rows."""
+    H = gpu_backend.H
+    rng = np.random.default_rng(11)
+    n, outputs = 20_000_000, 4096
+    h = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
+    sel = (rng.random(n) > 0.25).astype(np.uint8)
+    d = gpu_backend.dev(Batch([T(SIGNED, 8)], [np.zeros(n, np.int64)], None, sel))
+    hb = H.DeviceBuffer.from_numpy(h)
+    idx, offs = H.shuffle_partition(d, outputs, hashes=hb)
+    res = (h % np.uint64(outputs)).astype(np.int64)
+    counts = np.bincount(res[sel.astype(bool)], minlength=outputs)
+    assert np.array_equal(np.diff(offs), counts)
+    rows = idx.to_numpy(np.uint32, int(offs[-1])).astype(np.int64)
+    assert np.array_equal(np.sort(rows), np.flatnonzero(sel))
+    out_of = res[rows]
+    assert np.all(np.diff(out_of) >= 0)
+    same_out = np.diff(out_of) == 0
+    assert np.all(np.diff(rows)[same_out] > 0)
+
+
+def test_gather_rowset_equals_host_take(gpu_backend):
+    """Gathering each shuffle output of a batch view (bit offsets on the bool
+    column and the validity bitmaps) gives the host's rows at those indexes:
+    every column, the validity, the hashes and the selection."""
+    H = gpu_backend.H
+    rng = np.random.default_rng(21)
+    n, r0, outputs = 50_000, 37, 7
+    types = [T(SIGNED, 8), T(SIGNED, 4, nullable=True), T(DECIMAL, 16, 2, True, 30), T(3, 1), T(SIGNED, 1), T(FLOAT, 8)]
+    cols = [rng.integers(-10 ** 12, 10 ** 12, n).astype(np.int64), rng.integers(-50, 50, n).astype(np.int32),
+            [int(x) * 10 ** 20 + 7 for x in rng.integers(-10 ** 6, 10 ** 6, n)], rng.random(n) > 0.5,
+            rng.integers(-3, 4, n).astype(np.int8), rng.normal(0, 1, n)]
+    valid = [None, rng.random(n) > 0.2, rng.random(n) > 0.3, None, None, None]
+    sel = (rng.random(n) > 0.4).astype(np.uint8)
+    full = gpu_backend.dev(Batch(types, cols, valid, sel))
+    d = full.view(r0, n)
+    h = rng.integers(0, 1 << 62, n - r0).astype(np.uint64)
+    hb = H.DeviceBuffer.from_numpy(h)
+    d.rs.Hash = hb.ptr
+    idx, offs = H.shuffle_partition(d, outputs)
+    vts = [H.ValueType(t.kind, t.width, t.scale, t.nullable) for t in types]
+    host = OracleBackend().host(Batch(types, cols, valid, sel))  # holds the host buffers
+    want_all = O.read_host_rowset(host.rs, types)
+    total = 0
+    for o in range(outputs):
+        cnt = int(offs[o + 1] - offs[o])
+        rows = idx.to_numpy(np.uint32, int(offs[-1]))[offs[o]:offs[o + 1]].astype(np.int64)
+        g = H.gather_rowset(d, idx, int(offs[o]), cnt)
+        assert g.rows == cnt
+        got = H.download_rowset(g.rs, vts)
+        for c, ((gv, gm), (wv, wm)) in enumerate(zip(got, want_all)):
+            wv_rows = [wv[r0 + r] for r in rows]
+            assert list(gv) == list(wv_rows), f"output {o} column {c}"
+            if wm is not None:
+                assert np.array_equal(gm, np.asarray(wm)[r0 + rows]), f"output {o} mask {c}"
+        if cnt:
+            gs = np.empty(cnt, np.uint8)
+            gpu_backend.q.check(gpu_backend.q.lib().qdb_gpu_memcpy_d2h(gs.ctypes.data, g.rs.Selection, cnt), "sel")
+            gh = np.empty(cnt, np.uint64)
+            gpu_backend.q.check(gpu_backend.q.lib().qdb_gpu_memcpy_d2h(gh.ctypes.data, g.rs.Hash, cnt * 8), "hash")
+            assert gs.all() and np.array_equal(gh, h[rows])
+            assert np.all(h[rows] % np.uint64(outputs) == o)
+        total += cnt
+        g.close()
+    assert total == int(sel[r0:].sum())
+
+
+def _compile_agg(gpu_backend, case):
+    H = gpu_backend.H
+    kern = H.AggregateKernels.__new__(H.AggregateKernels)
+    kern.spec = None
+    kern.c_spec = gpu_backend.cast(case.to_c())
+    kern.k = gpu_backend.q.abi.QdbAggregateKernels()
+    gpu_backend.q.check(gpu_backend.q.lib().qdb_gpu_compile_aggregate(C.byref(kern.c_spec), C.byref(kern.k)), "compile")
+    kern.out_types = [H.ValueType(t.Kind, t.Width, t.Scale, bool(t.Nullable)) for t in kern.k.OutputTypes[: kern.k.NumOutputColumns]]
+    return kern
+
+
+def _selected_rows(cols, sel):
+    rows = result_rows(cols)
+    return [r for r, s in zip(rows, sel) if s] if sel is not None else rows
+
+
+@pytest.mark.parametrize("distinct,sample,max_groups,rows", [
+    (40, 50_000, 1 << 20, 20_000),          # reduces well: one table, kept to the end
+    (10 ** 9, 50_000, 1 << 20, 20_000),     # unique keys: emitted at 60K rows, then states
+    (25_000, 10 ** 9, 15_000, 20_000),      # bounded table: emits at 15K groups
+])
+def test_partial_aggregate_equals_oracle(gpu_backend, distinct, sample, max_groups, rows):
+    """The device partial aggregate makes the oracle's decisions after the
+    same batches and emits the same outputs (tables as sets of groups, states
+    row for row); the final aggregate (COUNT -> SUM) over its device outputs
+    equals the one-phase aggregate."""
+    import test_oracle as TO
+    H = gpu_backend.H
+    rng = np.random.default_rng(distinct % 1000 + 3)
+    types, case = TO._partial_cases()
+    batches = TO._partial_batches(rng, types, 6, rows, distinct)
+    kern = _compile_agg(gpu_backend, case)
+    pp = H.TPartialAggregateProcessor(kern, initial_capacity=8, sample_rows=sample, max_groups=max_groups)
+    spec = case.to_c()
+    pa = O.PartialAggregate(spec, sample_rows=sample, max_groups=max_groups)
+    be = OracleBackend()
+    devs = [gpu_backend.dev(b) for b in batches]
+    outs = []
+    for b, d in zip(batches, devs):
+        seen = len(pa.outputs)
+        pa.add(be.host(b))
+        pp.add(d)
+        got = pp.take()
+        want = pa.outputs[seen:]
+        assert len(got) == len(want) and pp.passing_through == pa.passing
+        for g, (kind, w) in zip(got, want):
+            gcols = H.download_rowset(g.rs, kern.out_types)
+            if kind == "table":
+                assert sorted(result_rows(gcols), key=sort_key) == sorted(result_rows(w), key=sort_key)
+            else:
+                gsel = np.empty(g.rows, np.uint8)
+                gpu_backend.q.check(gpu_backend.q.lib().qdb_gpu_memcpy_d2h(gsel.ctypes.data, g.rs.Selection, g.rows), "sel")
+                assert np.array_equal(gsel, b.selection)
+                assert _selected_rows(gcols, gsel) == result_rows(O.partial_states(spec, w))
+            outs.append(g)
+    last = pp.finish()
+    assert (last is None) == pa.passing
+    if last is not None:
+        want = O.aggregate(spec, pa.table) if pa.table else []
+        assert sorted(result_rows(H.download_rowset(last.rs, kern.out_types)), key=sort_key) == \
+            sorted(result_rows(want), key=sort_key)
+        outs.append(last)
+    ptypes = TO._partial_types(spec)
+    fk = _compile_agg(gpu_backend, TO._combine_case(case, ptypes))
+    fp = H.TAggregateProcessor(fk, initial_capacity=1 << 12)
+    for o in outs:
+        fp.add(o)
+    final = sorted(result_rows(fp.finish()), key=sort_key)
+    direct = sorted(result_rows(be.aggregate(case, batches)), key=sort_key)
+    assert final == direct
+    for o in outs:
+        o.close()
+    fp.close()
+    pp.close()
+    gpu_backend.q.lib().qdb_gpu_release_slot(gpu_backend.q.abi.SLOT_AGGREGATE, fk.k.Slot)
+    gpu_backend.q.lib().qdb_gpu_release_slot(gpu_backend.q.abi.SLOT_AGGREGATE, kern.k.Slot)
+
+
+def test_partial_aggregate_without_projection_keeps_hashing(gpu_backend):
+    """An aggregate with fused stages has no partial-state projection: the
+    entry point says so, and the partial aggregate keeps its bounded table
+    and never passes rows through, whatever its reduction."""
+    from qumirdb_amd import workloads as W
+    H = gpu_backend.H
+    ak = H.AggregateKernels(W.lineitem_fused_spec())
+    dev = W.LineitemDevice(10_000)
+    with pytest.raises(RuntimeError, match="fused stages|no projection"):
+        H.partial_states(ak, dev.rowset)
+    pp = H.TPartialAggregateProcessor(ak, sample_rows=1, min_reduction=1 << 40)
+    pp.add(dev.rowset)
+    assert not pp.passing_through and pp.take() == []
+    last = pp.finish()
+    ref = H.TAggregateProcessor(ak)
+    ref.add(dev.rowset)
+    assert last is not None and last.rows == ref.finish_in_hbm() > 0
+    ref.close()
+    last.close()
+    pp.close()
+    ak.release()

@@ -458,3 +458,126 @@ def test_rf_switches_to_bloom_past_max_keys():
     sel = (rng.random(len(others)) > 0.5).astype(np.uint8)
     out, kept = f.probe(others, sel)
     assert np.array_equal(out, sel) and kept == int(sel.sum())
+
+
+# ---------------------------------------------------------------------------
+# Hash shuffle and partial aggregate
+# ---------------------------------------------------------------------------
+
+def _rows_per_output(rows, offsets):
+    return [rows[offsets[o]:offsets[o + 1]].tolist() for o in range(len(offsets) - 1)]
+
+
+def test_shuffle_reference_goldens():
+    """The reference's routing cases (test/test_scheduler_runtime_adapter.cpp:
+    1496-1536, 1538-1562): hashes 0, 1, 2, 3, 5 over two outputs, without and
+    with the routing table 1, 0, 0, 1; a partition routed nowhere fails."""
+    h = np.array([0, 1, 2, 3, 5], np.uint64)
+    assert _rows_per_output(*O.shuffle_partition(h, None, 2)) == [[0, 2], [1, 3, 4]]
+    assert _rows_per_output(*O.shuffle_partition(h, None, 2, routes=[1, 0, 0, 1])) == [[1, 2, 4], [0, 3]]
+    with pytest.raises(ValueError, match="partition 1, which it routes nowhere"):
+        O.shuffle_partition(np.array([0, 3], np.uint64), None, 2, routes=[0, 0xFFFFFFFF])
+
+
+def test_shuffle_partition_properties():
+    """Each selected row lands in exactly one output, in input order, at its
+    hash's residue (checked with Python integers); 8 outputs take the low 3
+    bits; an identity routing table is no table, a permuted one permutes the
+    outputs, and a table of 12 partitions folded onto 3 outputs equals 3
+    plain outputs."""
+    rng = np.random.default_rng(5)
+    n = 3000
+    h = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
+    h[:4] = [0, (1 << 64) - 1, 1 << 63, 12345]
+    sel = rng.random(n) > 0.3
+    for outputs in (1, 3, 8, 257):
+        rows, offs = O.shuffle_partition(h, sel, outputs)
+        assert sorted(rows.tolist()) == np.flatnonzero(sel).tolist()
+        for o, part in enumerate(_rows_per_output(rows, offs)):
+            assert part == sorted(part)
+            assert all(int(h[r]) - (int(h[r]) // outputs) * outputs == o for r in part)
+    for o, part in enumerate(_rows_per_output(*O.shuffle_partition(h, None, 8))):
+        assert all(int(h[r]) & 7 == o for r in part)
+    plain = _rows_per_output(*O.shuffle_partition(h, sel, 5))
+    assert _rows_per_output(*O.shuffle_partition(h, sel, 5, routes=list(range(5)))) == plain
+    perm = [3, 0, 4, 1, 2]
+    routed = _rows_per_output(*O.shuffle_partition(h, sel, 5, routes=perm))
+    assert all(routed[perm[p]] == plain[p] for p in range(5))
+    assert _rows_per_output(*O.shuffle_partition(h, sel, 3, routes=[i % 3 for i in range(12)])) == \
+        _rows_per_output(*O.shuffle_partition(h, sel, 3))
+
+
+def _partial_types(spec_c):
+    return [T(t.Kind, t.Width, t.Scale, bool(t.Nullable), t.Precision) for t in O.output_types(spec_c)]
+
+
+def _combine_case(case: AggCase, ptypes):
+    nk = len(case.keys)
+    aggs = [(AGG_SUM if f in (AGG_COUNT, AGG_COUNT_STAR) else f, nk + i) for i, (f, _) in enumerate(case.aggs)]
+    return AggCase(ptypes, list(range(nk)), aggs)
+
+
+def _as_batch(ptypes, columns):
+    cols, valid = [], []
+    for t, (vals, v) in zip(ptypes, columns):
+        if t.width == 16:
+            vals = [int(x) for x in vals]
+        elif t.kind == FLOAT:
+            vals = np.asarray(vals, np.float64)
+        else:
+            vals = np.asarray(vals).astype(np.int64 if t.width == 8 else vals.dtype)
+        cols.append(vals)
+        valid.append(None if v is None else np.asarray(v, bool))
+    return Batch(ptypes, cols, valid if any(x is not None for x in valid) else None)
+
+
+def _partial_cases():
+    types = [T(SIGNED, 8), T(SIGNED, 4, nullable=True), T(DECIMAL, 16, 2, True, 30), T(FLOAT, 8)]
+    return types, AggCase(types, [0], [(AGG_COUNT_STAR, -1), (AGG_COUNT, 1), (AGG_SUM, 1), (AGG_SUM, 2), (AGG_MIN, 3),
+                                       (AGG_MAX, 1)])
+
+
+def _partial_batches(rng, types, nbatches, rows, distinct):
+    out = []
+    for _ in range(nbatches):
+        cols = [rng.integers(0, distinct, rows).astype(np.int64), rng.integers(-50, 50, rows).astype(np.int32),
+                [int(x) * 10 ** 20 + 3 for x in rng.integers(-10 ** 6, 10 ** 6, rows)], rng.normal(0, 1, rows)]
+        valid = [None, rng.random(rows) > 0.2, rng.random(rows) > 0.3, None]
+        out.append(Batch(types, cols, valid, (rng.random(rows) > 0.1).astype(np.uint8)))
+    return out
+
+
+@pytest.mark.parametrize("distinct,sample,max_groups,want_passing,want_tables", [
+    (40, 5000, 1 << 20, False, 0),        # reduces well: one table, kept to the end
+    (10 ** 9, 5000, 1 << 20, True, 1),    # unique keys: the table is emitted at 6000 rows, then states
+    (2500, 10 ** 9, 1500, False, None),   # bounded table: emits whenever it reaches 1500 groups
+])
+def test_partial_aggregate_policy_and_cascade(distinct, sample, max_groups, want_passing, want_tables):
+    """The partial aggregate's decisions (TPartialAggregateProcessor::Add),
+    and the cascade's invariant that makes them safe: the final aggregate
+    (COUNT -> SUM) over whatever the partial phase emitted, tables and
+    single-row states alike, equals the one-phase aggregate."""
+    rng = np.random.default_rng(distinct % 1000 + 7)
+    types, case = _partial_cases()
+    batches = _partial_batches(rng, types, 6, 2000, distinct)
+    be = OracleBackend()
+    spec = case.to_c()
+    pa = O.PartialAggregate(spec, sample_rows=sample, max_groups=max_groups)
+    for b in batches:
+        pa.add(be.host(b))
+    assert pa.passing == want_passing
+    tables = [o for kind, o in pa.outputs if kind == "table"]
+    if want_tables is not None:
+        assert len(tables) == want_tables
+    else:
+        assert len(tables) >= 2 and all(len(t[0][0]) >= max_groups for t in tables)
+    ptypes = _partial_types(spec)
+    parts = [_as_batch(ptypes, t) for t in tables]
+    for kind, hb in pa.outputs:
+        if kind == "states":
+            parts.append(_as_batch(ptypes, O.partial_states(spec, hb)))
+    if not pa.passing and pa.table:
+        parts.append(_as_batch(ptypes, O.aggregate(spec, pa.table)))
+    final = sorted(result_rows(be.aggregate(_combine_case(case, ptypes), parts)), key=sort_key)
+    direct = sorted(result_rows(be.aggregate(case, batches)), key=sort_key)
+    assert final == direct
