@@ -44,6 +44,22 @@ constexpr int kBlock = 256;
 constexpr int kProbeBlock = 1024;
 constexpr uint32_t kLockHead = 0xffffffffu;
 constexpr int kMaxFused = QDB_JOIN_MAX_FUSED;
+// Words per fused aggregate: [count, sum partial words x3, min, max].
+constexpr int kAccWords = 6;
+// Host readback words: the pair count, one spare, the fused words.
+constexpr int kReadbackWords = 2 + kMaxFused * kAccWords;
+
+// The probe's counters before a launch: the pair count 0 and the first
+// `words` fused words at their identities (sums 0, MIN INT64_MAX, MAX
+// INT64_MIN); thread t sets word t.
+__device__ __forceinline__ void probe_counters_init(unsigned long long* pair_count, unsigned long long* acc, int words, int t)
+{
+    if (t == 0) pair_count[0] = 0ull;
+    if (t < words) {
+        const int k = t % kAccWords;
+        acc[t] = k == 4 ? 0x7fffffffffffffffull : k == 5 ? 0x8000000000000000ull : 0ull;
+    }
+}
 
 struct JKeyField {
     int32_t slot;
@@ -106,29 +122,80 @@ __device__ __forceinline__ bool join_key(const DevCols& cols, const JKeyDesc& K,
 // Append: count, scan, scatter.
 // ---------------------------------------------------------------------------
 
+// Kept rows per block; with `block_range` (one-word keys) also the block's
+// largest ~key and largest key of its kept rows (0, 0 when none).
 template <int KW>
 __global__ void __launch_bounds__(kBlock) k_append_count(DevCols cols, JKeyDesc K, int64_t n, const uint8_t* sel,
-    unsigned int* block_counts)
+    unsigned int* block_counts, unsigned long long* block_range)
 {
     __shared__ unsigned int total;
+    __shared__ unsigned long long wave_range[kBlock / kWave][2];
     if (threadIdx.x == 0) total = 0;
     __syncthreads();
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     bool keep = false;
-    if (i < n && (!sel || sel[i])) {
-        uint64_t key[KW];
-        keep = join_key<KW>(cols, K, i, key);
-    }
+    uint64_t key[KW];
+    key[0] = 0;
+    if (i < n && (!sel || sel[i])) keep = join_key<KW>(cols, K, i, key);
     const uint64_t b = __ballot(keep);
     if (lane_id() == 0 && b) atomicAdd(&total, (unsigned int)__popcll(b));
+    if (KW == 1 && block_range) {
+        unsigned long long nmin = keep ? ~key[0] : 0ull, mx = keep ? key[0] : 0ull;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const unsigned long long a = shfl_xor_u64(nmin, o), c = shfl_xor_u64(mx, o);
+            nmin = a > nmin ? a : nmin;
+            mx = c > mx ? c : mx;
+        }
+        if (lane_id() == 0) {
+            wave_range[threadIdx.x / kWave][0] = nmin;
+            wave_range[threadIdx.x / kWave][1] = mx;
+        }
+    }
     __syncthreads();
-    if (threadIdx.x == 0) block_counts[blockIdx.x] = total;
+    if (threadIdx.x == 0) {
+        block_counts[blockIdx.x] = total;
+        if (KW == 1 && block_range) {
+            unsigned long long nmin = 0, mx = 0;
+            for (int w = 0; w < kBlock / kWave; ++w) {
+                nmin = wave_range[w][0] > nmin ? wave_range[w][0] : nmin;
+                mx = wave_range[w][1] > mx ? wave_range[w][1] : mx;
+            }
+            block_range[2 * blockIdx.x] = nmin;
+            block_range[2 * blockIdx.x + 1] = mx;
+        }
+    }
 }
 
-// Exclusive scan of `n` counts in place (one workgroup); writes the total.
-__global__ void __launch_bounds__(1024) k_scan_counts(unsigned int* counts, unsigned int n, unsigned long long* total)
+// Exclusive scan of `n` block counts in place (one workgroup), offset by
+// the arena's row count: each block's first arena row. The arena's words
+// then take the new row count and, with `block_range`, the key range of
+// the rows so far. `fresh`: the arena is empty and its words not yet set
+// (the first append writes all of them).
+__global__ void __launch_bounds__(1024) k_scan_counts(unsigned int* counts, unsigned int n, unsigned long long* arena,
+    const unsigned long long* block_range, int fresh)
 {
     __shared__ unsigned long long partial[1024];
+    __shared__ unsigned long long base;
+    __shared__ unsigned long long wrange[1024 / kWave][2];
+    if (threadIdx.x == 0) base = fresh ? 0ull : arena[0];
+    if (block_range) {
+        unsigned long long nmin = 0, mx = 0;
+        for (unsigned int b = threadIdx.x; b < n; b += blockDim.x) {
+            nmin = block_range[2 * b] > nmin ? block_range[2 * b] : nmin;
+            mx = block_range[2 * b + 1] > mx ? block_range[2 * b + 1] : mx;
+        }
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const unsigned long long a = shfl_xor_u64(nmin, o), c = shfl_xor_u64(mx, o);
+            nmin = a > nmin ? a : nmin;
+            mx = c > mx ? c : mx;
+        }
+        if (lane_id() == 0) {
+            wrange[threadIdx.x / kWave][0] = nmin;
+            wrange[threadIdx.x / kWave][1] = mx;
+        }
+    }
     const unsigned int per = (n + blockDim.x - 1) / blockDim.x;
     const unsigned int lo = threadIdx.x * per;
     const unsigned int hi = lo + per < n ? lo + per : n;
@@ -147,13 +214,23 @@ __global__ void __launch_bounds__(1024) k_scan_counts(unsigned int* counts, unsi
     if (lane == kWave - 1) partial[wave] = incl;
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long run = 0;
+        unsigned long long run = base;
         for (int w = 0; w < nw; ++w) {
             const unsigned long long v = partial[w];
             partial[w] = run;
             run += v;
         }
-        *total = run;
+        arena[0] = run;
+        unsigned long long nmin = fresh ? 0ull : arena[2], mx = fresh ? 0ull : arena[3];
+        if (block_range) {
+            for (int w = 0; w < nw; ++w) {
+                nmin = wrange[w][0] > nmin ? wrange[w][0] : nmin;
+                mx = wrange[w][1] > mx ? wrange[w][1] : mx;
+            }
+        }
+        if (fresh) arena[1] = 0ull;
+        arena[2] = nmin;
+        arena[3] = mx;
     }
     __syncthreads();
     unsigned long long run = partial[wave] + incl - s;
@@ -162,12 +239,6 @@ __global__ void __launch_bounds__(1024) k_scan_counts(unsigned int* counts, unsi
         counts[i] = (unsigned int)run;
         run += v;
     }
-}
-
-// The arena's row count grows by an append's kept rows (after its writes).
-__global__ void k_add_rows(unsigned long long* arena_rows, const unsigned long long* added)
-{
-    if (blockIdx.x == 0 && threadIdx.x == 0) *arena_rows += *added;
 }
 
 struct ArenaCols {
@@ -181,11 +252,10 @@ struct ArenaCols {
 // The arena payload is written for every build column the plan keeps.
 template <int KW>
 __global__ void __launch_bounds__(kBlock) k_append_write(DevCols cols, JKeyDesc K, int64_t n, const uint8_t* sel,
-    const unsigned int* block_offsets, const unsigned long long* arena_rows, int64_t ordinal_base, uint64_t* arena_keys,
-    int64_t* arena_ordinal, int32_t* ord_to_arena, DevCols payload_cols, int npayload, ArenaCols AC)
+    const unsigned int* block_offsets, int64_t ordinal_base, uint64_t* arena_keys, int64_t* arena_ordinal,
+    int32_t* ord_to_arena, DevCols payload_cols, int npayload, ArenaCols AC)
 {
     __shared__ unsigned int wave_base[kBlock / kWave];
-    const int64_t arena_base = (int64_t)*arena_rows;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int lane = lane_id();
     const int wave = threadIdx.x / kWave;
@@ -207,7 +277,7 @@ __global__ void __launch_bounds__(kBlock) k_append_write(DevCols cols, JKeyDesc 
     if (i < n) ord_to_arena[ordinal_base + i] = -1;
     if (!keep) return;
     const unsigned int rank = wave_base[wave] + (unsigned int)__popcll(b & ((1ull << lane) - 1ull));
-    const int64_t dst = arena_base + block_offsets[blockIdx.x] + rank;
+    const int64_t dst = (int64_t)block_offsets[blockIdx.x] + rank;
 #pragma unroll
     for (int w = 0; w < KW; ++w) arena_keys[dst * KW + w] = key[w];
     arena_ordinal[dst] = ordinal_base + i;
@@ -517,29 +587,32 @@ __global__ void __launch_bounds__(kBlock) k_reduce_partials(const unsigned long 
 
 constexpr uint32_t kAbsent = 0xffffffffu;   // dense table: no build row for this key
 
-// The key range [info[0], info[1]] of the dense table, when it holds at
-// most dense_limit(rows) keys.
-__device__ __forceinline__ bool dense_range(const unsigned long long* info, uint64_t rows, uint64_t* range)
+// The arena's device words: [0] the row count, [1] the dense build's flags
+// (u32 repeated key, u32 range too wide), [2] the largest ~key and [3] the
+// largest key of the rows appended (one-word keys; 0 and 0 while empty).
+constexpr int kArenaWords = 4;
+
+// The key range [min, max] of the dense table when it holds at most
+// dense_limit(rows) keys.
+__device__ __forceinline__ bool dense_range(const unsigned long long* arena, uint64_t* lo, uint64_t* range)
 {
-    const uint64_t lo = info[0], hi = info[1];
-    *range = hi - lo + 1;
-    return rows > 0 && hi >= lo && *range != 0 && *range <= (rows * 8 > (1ull << 16) ? rows * 8 : (1ull << 16));
+    const uint64_t rows = arena[0];
+    *lo = ~arena[2];
+    const uint64_t hi = arena[3];
+    *range = hi - *lo + 1;
+    return rows > 0 && hi >= *lo && *range != 0 && *range <= (rows * 8 > (1ull << 16) ? rows * 8 : (1ull << 16));
 }
 
 // Clears the dense table over the key range (no membership bits, kAbsent
-// ordinals), or flags a range too wide (info word 2, high half); copies the
-// row count to info[4].
-__global__ void __launch_bounds__(kBlock) k_dense_prepare(const unsigned long long* rows_dev, unsigned long long* info,
-    uint32_t* dbits, uint32_t* dtab)
+// ordinals), or flags a range too wide.
+__global__ void __launch_bounds__(kBlock) k_dense_prepare(unsigned long long* arena, uint32_t* dbits, uint32_t* dtab)
 {
-    const uint64_t rows = *rows_dev;
-    uint64_t range = 0;
-    const bool ok = dense_range(info, rows, &range);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        info[4] = rows;
-        if (!ok) ((unsigned int*)(info + 2))[1] = 1u;
+    uint64_t lo = 0, range = 0;
+    const bool ok = dense_range(arena, &lo, &range);
+    if (!ok) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) ((unsigned int*)(arena + 1))[1] = 1u;
+        return;
     }
-    if (!ok) return;
     const uint64_t words = (range + 31) / 32;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < range; i += stride) {
@@ -549,16 +622,16 @@ __global__ void __launch_bounds__(kBlock) k_dense_prepare(const unsigned long lo
 }
 
 // The dense table: a membership bit and the build ordinal of every row at
-// key - min (kAbsent where no row has the key); flags[0] = 1 when a key
-// repeats.
-__global__ void __launch_bounds__(kBlock) k_dense_fill(const uint64_t* keys, const int64_t* ordinal,
-    const unsigned long long* rows_dev, unsigned long long* info, uint32_t* dbits, uint32_t* dtab)
+// key - min (kAbsent where no row has the key); the repeated-key flag set
+// when a key repeats. Workgroup 0 also sets the first probe's counters.
+__global__ void __launch_bounds__(kBlock) k_dense_fill(const uint64_t* keys, const int64_t* ordinal, unsigned long long* arena,
+    uint32_t* dbits, uint32_t* dtab, unsigned long long* pair_count, unsigned long long* acc, int acc_words)
 {
-    const int64_t rows = (int64_t)*rows_dev;
-    uint64_t range = 0;
-    if (!dense_range(info, (uint64_t)rows, &range)) return;
-    unsigned int* flags = (unsigned int*)(info + 2);
-    const uint64_t dmin = info[0];
+    uint64_t dmin = 0, range = 0;
+    if (!dense_range(arena, &dmin, &range)) return;
+    if (blockIdx.x == 0) probe_counters_init(pair_count, acc, acc_words, (int)threadIdx.x);
+    const int64_t rows = (int64_t)arena[0];
+    unsigned int* flags = (unsigned int*)(arena + 1);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += stride) {
         const uint64_t idx = keys[r] - dmin;
@@ -634,12 +707,12 @@ struct ProbeArgs {
     const char* payload_data[kMaxFused];
     const uint8_t* payload_valid[kMaxFused];
     int32_t payload_width[kMaxFused];
-    // Per workgroup and fused agg: [count, p0, p1, p2, min, max] words;
-    // k_reduce_partials folds them into the join's totals.
+    // Per workgroup and fused agg: [count, p0, p1, p2, min, max] words,
+    // which k_reduce_partials folds into the join's totals; when null, each
+    // workgroup adds its words to fused_acc with atomics instead.
     unsigned long long* fused_part;
+    unsigned long long* fused_acc;
 };
-
-constexpr int kAccWords = 6;
 
 // Probe kernel modes: kGeneric reads keys through the column descriptors;
 // kDirectKey reads one non-nullable 8-byte integer key with a streaming
@@ -787,7 +860,21 @@ __device__ __forceinline__ void fused_flush(const ProbeArgs& P, const FusedLane 
             else if (k == 4) x = (int64_t)y < (int64_t)x ? y : x;
             else x = (int64_t)y > (int64_t)x ? y : x;
         }
-        P.fused_part[((size_t)blockIdx.x * kMaxFused + f) * kAccWords + k] = x;
+        if (P.fused_part) {
+            P.fused_part[((size_t)blockIdx.x * kMaxFused + f) * kAccWords + k] = x;
+        } else {
+            // The workgroup's words straight into the probe's totals (set to
+            // their identities before the launch): the sum words are partial
+            // words that add without overflow, MIN/MAX are signed.
+            unsigned long long* t = P.fused_acc + (size_t)f * kAccWords + k;
+            if (k < 4) {
+                if (x) atomicAdd(t, x);
+            } else if (k == 4) {
+                if ((int64_t)x != 0x7fffffffffffffffll) atomicMin((long long*)t, (long long)x);
+            } else {
+                if (x != 0x8000000000000000ull) atomicMax((long long*)t, (long long)x);
+            }
+        }
     }
 }
 
@@ -1030,7 +1117,10 @@ constexpr uint64_t kLeanBlockTile = (uint64_t)(kProbeBlock / kWave) * kLeanWaveT
 
 struct LeanState {
     unsigned long long pairs;   // this wave's pairs
-    uint64_t lo, hi;            // SUM of the probe column (two's complement, 128 bits)
+    // SUM of the probe column: the low 32 bits of each value (unsigned) and
+    // the high 32 bits (signed) summed apart, two adds per row without a
+    // carry; exact for fewer than 2^32 rows per lane.
+    uint64_t lo, hi;
     int64_t mn, mx;
 };
 
@@ -1099,9 +1189,8 @@ __device__ __forceinline__ void lean_tile(const ProbeArgs& P, const uint32_t* lb
 #pragma unroll
         for (int r = 0; r < kLeanRows; ++r) {
             const int64_t x = hit[r] ? val[r] : 0;
-            const uint64_t lo = st.lo + (uint64_t)x;
-            st.hi += (lo < st.lo ? 1ull : 0ull) + (x < 0 ? ~0ull : 0ull);
-            st.lo = lo;
+            st.lo += (uint64_t)(uint32_t)x;
+            st.hi += (uint64_t)(x >> 32);
             if (kMinMax) {
                 st.mn = hit[r] && x < st.mn ? x : st.mn;
                 st.mx = hit[r] && x > st.mx ? x : st.mx;
@@ -1179,7 +1268,10 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe_dense_lean(ProbeArgs P)
         none[0].count = 0;
         none[0].lo = 0;
         none[0].hi = 0;
-        fused_flush<1, true>(P, none, lane == 0 ? st.pairs : 0ull, st.lo, st.hi, st.mn, st.mx, lane);
+        // The 128-bit sum: lo + hi * 2^32.
+        const uint64_t sum_lo = st.lo + (st.hi << 32);
+        const uint64_t sum_hi = (uint64_t)((int64_t)st.hi >> 32) + (sum_lo < st.lo ? 1ull : 0ull);
+        fused_flush<1, true>(P, none, lane == 0 ? st.pairs : 0ull, sum_lo, sum_hi, st.mn, st.mx, lane);
     }
 }
 
@@ -1540,8 +1632,11 @@ struct JoinBuild {
     unsigned long long* fused_acc = nullptr;    // pair_count + 2
     // host: that readback; fused_ready when it holds a probe's fused words
     // not yet added to the totals
-    unsigned long long readback[2 + kMaxFused * kAccWords];
+    unsigned long long* readback = nullptr;     // pinned, kReadbackWords
     bool fused_ready = false;
+    // the pair counter and the fused words hold their identities (set by
+    // the build for the first probe)
+    bool counters_ready = false;
     // the probe batch the pair buffer refers to (for materialize)
     TRowSet last_probe;
 };
@@ -1741,6 +1836,7 @@ void build_free(JoinBuild* J)
     dfree(J->dtab);
     dfree(J->pair_count);
     dfree(J->rows_dev);
+    hfree_pinned(J->readback);
     delete J;
 }
 
@@ -1803,11 +1899,19 @@ bool do_append(JoinBuild* J, const TRowSet* batch)
     memset(&pcols, 0, sizeof(pcols));
     // Payload columns go through a second DevCols in chunks of kMaxDevCols.
     const int nblocks = (int)((n + kBlock - 1) / kBlock);
-    unsigned int* counts = (unsigned int*)dalloc((size_t)nblocks * 4 + 16);
-    unsigned long long* total = (unsigned long long*)dalloc(16);
-    if (!counts || !total) return false;
-    hipLaunchKernelGGL((k_append_count<KW>), dim3(nblocks), dim3(kBlock), 0, stream(), cols, P->build_key, n, batch->Selection, counts);
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, stream(), counts, (unsigned int)nblocks, total);
+    // Per block: the kept rows (then the first arena row) and, for one-word
+    // keys, the key range.
+    const size_t range_off = ((size_t)nblocks * 4 + 15) / 16 * 16;
+    char* work = (char*)dalloc(range_off + (KW == 1 ? (size_t)nblocks * 16 : 0) + 16);
+    if (!work) return false;
+    unsigned int* counts = (unsigned int*)work;
+    unsigned long long* brange = KW == 1 ? (unsigned long long*)(work + range_off) : nullptr;
+    hipLaunchKernelGGL((k_append_count<KW>), dim3(nblocks), dim3(kBlock), 0, stream(), cols, P->build_key, n, batch->Selection, counts,
+        brange);
+    // Each block's first arena row; the arena's row count and key range grow
+    // here (the first append sets every arena word).
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, stream(), counts, (unsigned int)nblocks, J->rows_dev,
+        (const unsigned long long*)brange, J->appended == 0 ? 1 : 0);
     bool ok = hip_ok(hipGetLastError(), "append count");
     for (int p0 = 0; ok && (p0 < P->npayload || p0 == 0); p0 += kMaxDevCols) {
         const int np = P->npayload - p0 < kMaxDevCols ? P->npayload - p0 : kMaxDevCols;
@@ -1823,19 +1927,13 @@ bool do_append(JoinBuild* J, const TRowSet* batch)
         }
         if (!bind_side(slot_col, np, P->spec.Build, batch, &pcols)) return false;
         hipLaunchKernelGGL((k_append_write<KW>), dim3(nblocks), dim3(kBlock), 0, stream(), cols, P->build_key, n,
-            batch->Selection, counts, (const unsigned long long*)J->rows_dev, J->appended, J->keys, J->ordinal, J->ord_to_arena,
-            pcols, np, AC);
+            batch->Selection, counts, J->appended, J->keys, J->ordinal, J->ord_to_arena, pcols, np, AC);
         ok = hip_ok(hipGetLastError(), "append write");
         if (P->npayload == 0) break;
     }
     // The kept rows are counted on the device; the host reads the count
     // back at Build (with the dense table's key range).
-    if (ok) {
-        hipLaunchKernelGGL(k_add_rows, dim3(1), dim3(64), 0, stream(), J->rows_dev, (const unsigned long long*)total);
-        ok = hip_ok(hipGetLastError(), "append rows");
-    }
-    dfree(counts);
-    dfree(total);
+    dfree(work);
     if (!ok) return false;
     J->rows_bound += n;
     J->rows_known = false;
@@ -1850,10 +1948,19 @@ bool do_append(JoinBuild* J, const TRowSet* batch)
 bool read_rows(JoinBuild* J)
 {
     if (J->rows_known) return true;
-    unsigned long long r = 0;
-    if (!hip_ok(hipMemcpyAsync(&r, J->rows_dev, 8, hipMemcpyDeviceToHost, stream()), "arena rows") || !sync("arena rows")) return false;
-    J->rows = (int64_t)r;
+    unsigned long long* w = J->readback;
+    J->fused_ready = false;
+    if (!hip_ok(hipMemcpyAsync(w, J->rows_dev, kArenaWords * 8, hipMemcpyDeviceToHost, stream()), "arena rows") || !sync("arena rows")) {
+        return false;
+    }
+    J->rows = (int64_t)w[0];
     J->rows_known = true;
+    // One-word keys: the appends kept their range.
+    if (J->plan->KW == 1 && J->rows > 0) {
+        J->have_range = true;
+        J->kmin = ~w[2];
+        J->kmax = w[3];
+    }
     return true;
 }
 
@@ -1865,16 +1972,12 @@ __host__ __device__ inline uint64_t dense_limit(uint64_t rows)
 
 bool try_dense_build(JoinBuild* J)
 {
-    // Sized for the bound on the rows: the key range, its validity, the
-    // repeated-key flag and the exact row count come back in one readback.
+    // Sized for the bound on the rows: the appends kept the key range on the
+    // device, the kernels read it there, and the row count, the range and
+    // the flags (repeated key, range too wide) come back in one readback.
     const int64_t bound = J->rows_known ? J->rows : J->rows_bound;
     const uint64_t limit_cap = dense_limit((uint64_t)bound);
     if (limit_cap >= (1ull << 32)) return false;
-    const int mgrid = grid_for(bound, kBlock, 2);
-    // info: [min, max] of the keys, the flags (repeated key, range too
-    // wide), the row count
-    unsigned long long* info = (unsigned long long*)dalloc(48);
-    unsigned long long* mpart = (unsigned long long*)dalloc((size_t)mgrid * 16);
     const size_t words_cap = (size_t)((limit_cap + 31) / 32);
     J->dbits = (uint32_t*)dalloc(words_cap * 4);
     J->dtab = (uint32_t*)dalloc((size_t)limit_cap * 4);
@@ -1884,50 +1987,40 @@ bool try_dense_build(JoinBuild* J)
         J->dbits = nullptr;
         J->dtab = nullptr;
     };
-    if (!info || !mpart || !J->dbits || !J->dtab) {
-        dfree(info);
-        dfree(mpart);
+    if (!J->dbits || !J->dtab) {
         drop();
         return false;
     }
-    unsigned long long h_info[5] = {0, 0, 0, 0, 0};
-    unsigned int* flags = (unsigned int*)(info + 2);
-    bool ok = zero_fill(info, 48);
-    if (ok) {
+    unsigned long long* h = J->readback;
+    {
         TimedRegion timer("join_build");
-        hipLaunchKernelGGL(k_key_minmax, dim3(mgrid), dim3(kBlock), 0, stream(), J->keys, (const unsigned long long*)J->rows_dev, mpart);
-        ReduceOps ops;
-        memset(&ops, 0, sizeof(ops));
-        ops.op[0] = 3;  // unsigned min
-        ops.op[1] = 4;  // unsigned max
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, stream(), mpart, mgrid, 2, 2, ops, info);
-        hipLaunchKernelGGL(k_dense_prepare, dim3(grid_for((int64_t)limit_cap, kBlock, 8)), dim3(kBlock), 0, stream(),
-            (const unsigned long long*)J->rows_dev, info, J->dbits, J->dtab);
+        hipLaunchKernelGGL(k_dense_prepare, dim3(grid_for((int64_t)limit_cap, kBlock, 8)), dim3(kBlock), 0, stream(), J->rows_dev,
+            J->dbits, J->dtab);
         hipLaunchKernelGGL(k_dense_fill, dim3(grid_for(bound, kBlock, 8)), dim3(kBlock), 0, stream(), J->keys, J->ordinal,
-            (const unsigned long long*)J->rows_dev, info, J->dbits, J->dtab);
+            J->rows_dev, J->dbits, J->dtab, J->pair_count, J->fused_acc, J->plan->spec.NumFused * kAccWords);
     }
-    ok = ok && hip_ok(hipGetLastError(), "dense fill") &&
-        hip_ok(hipMemcpyAsync(h_info, info, 40, hipMemcpyDeviceToHost, stream()), "dense info") && sync("dense fill");
-    dfree(info);
-    dfree(mpart);
+    J->fused_ready = false;
+    const bool ok = hip_ok(hipGetLastError(), "dense fill") &&
+        hip_ok(hipMemcpyAsync(h, J->rows_dev, kArenaWords * 8, hipMemcpyDeviceToHost, stream()), "dense info") && sync("dense fill");
     if (ok) {
-        J->rows = (int64_t)h_info[4];
+        J->rows = (int64_t)h[0];
         J->rows_known = true;
         J->have_range = J->rows > 0;
-        J->kmin = h_info[0];
-        J->kmax = h_info[1];
+        J->kmin = ~h[2];
+        J->kmax = h[3];
     }
-    const unsigned int* h_flags = (const unsigned int*)(h_info + 2);
-    const uint64_t range = h_info[1] - h_info[0] + 1;
+    const unsigned int* h_flags = (const unsigned int*)(h + 1);
+    const uint64_t range = J->kmax - J->kmin + 1;
     // Repeated keys or a key range too wide: the hash table serves.
-    if (!ok || J->rows <= 0 || h_flags[0] || h_flags[1] || h_info[1] < h_info[0] || range == 0 ||
+    if (!ok || J->rows <= 0 || h_flags[0] || h_flags[1] || J->kmax < J->kmin || range == 0 ||
         range > dense_limit((uint64_t)J->rows)) {
         drop();
         clear_error();
         return false;
     }
     J->dense = true;
-    J->dmin = h_info[0];
+    J->counters_ready = true;
+    J->dmin = J->kmin;
     J->drange = range;
     J->keys_distinct = J->rows;
     J->max_key_rows = J->rows > 0 ? 1 : 0;
@@ -2251,6 +2344,14 @@ void launch_probe(int mode, int tab, int grid, size_t lds, const ProbeArgs& A)
     launch_probe_tab<KW, kTabHash>(mode, grid, lds, A);
 }
 
+// The probe's counters before a launch: the pair count 0 and the first
+// `words` fused words at their identities (sums 0, MIN INT64_MAX, MAX
+// INT64_MIN).
+__global__ void k_probe_init(unsigned long long* pair_count, unsigned long long* acc, int words)
+{
+    probe_counters_init(pair_count, acc, words, (int)threadIdx.x);
+}
+
 template <int KW>
 // Probes one batch. With `scratch_part` (the allocation-free variant) no
 // pair is written and nothing is allocated: the fused aggregates' partials
@@ -2368,32 +2469,36 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs, unsigned
         A.pairs = scratch_part ? nullptr : pairs->Data;
         A.pair_cap = scratch_part ? 0ull : (unsigned long long)pairs->Capacity;
         A.pair_count = J->pair_count;
-        if (!zero_fill(J->pair_count, 8)) return false;
+        if (!J->counters_ready || attempt > 0) {
+            hipLaunchKernelGGL(k_probe_init, dim3(1), dim3(64), 0, stream(), J->pair_count, J->fused_acc, A.nfused * kAccWords);
+            if (!hip_ok(hipGetLastError(), "probe init")) return false;
+        }
+        J->counters_ready = false;
         // Fused accumulators are per probe call (k_reduce_partials writes
         // every word); they are summed on the host.
         const int64_t tiles = (n + (int64_t)kProbeBlock / kWave * kWaveTile - 1) / ((int64_t)kProbeBlock / kWave * kWaveTile);
         const int per_cu = lds_table ? 2 : 1;
         const int grid = (int)(tiles < (int64_t)num_cus() * per_cu ? tiles : (int64_t)num_cus() * per_cu);
+        // Fused totals: the workgroups add their words to the join's
+        // accumulators with atomics; the allocation-free variant writes
+        // per-workgroup partials into its scratch and reduces them.
         unsigned long long* part = nullptr;
-        if (A.nfused) {
+        A.fused_part = nullptr;
+        A.fused_acc = J->fused_acc;
+        if (A.nfused && scratch_part) {
             const size_t part_bytes = (size_t)grid * kMaxFused * kAccWords * 8;
-            if (scratch_part) {
-                if (part_bytes > kNoAllocScratchBytes) {
-                    return fail("no-alloc probe: %zu bytes of partials exceed the %zu-byte scratch", part_bytes,
-                        (size_t)kNoAllocScratchBytes);
-                }
-                part = scratch_part;
-            } else {
-                part = (unsigned long long*)dalloc(part_bytes);
-                if (!part) return false;
+            if (part_bytes > kNoAllocScratchBytes) {
+                return fail("no-alloc probe: %zu bytes of partials exceed the %zu-byte scratch", part_bytes,
+                    (size_t)kNoAllocScratchBytes);
             }
+            part = scratch_part;
             A.fused_part = part;
         }
         {
             TimedRegion timer("join_probe");
             if (lean) launch_dense_lean(mode == kDirectVal, minmax, A.probe_is_left != 0, grid, lds, A);
             else launch_probe<KW>(mode, tab, grid, lds, A);
-            if (A.nfused) {
+            if (part) {
                 ReduceOps ops;
                 for (int k = 0; k < kMaxFused * kAccWords; ++k) {
                     const int w = k % kAccWords;
@@ -2403,7 +2508,6 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs, unsigned
                     A.nfused * kAccWords, ops, J->fused_acc);
             }
         }
-        if (!scratch_part) dfree(part);
         if (!last_error_text().empty()) return false;
         if (!hip_ok(hipGetLastError(), "probe launch")) return false;
         // The pair count and the fused words in one readback.
@@ -2519,8 +2623,11 @@ bool jr_dispatch_impl(JoinPlan* P, void* arena, void* table, void* scratch, void
         J->pvalid.assign((size_t)P->npayload, nullptr);
         J->pair_count = (unsigned long long*)dalloc(16 + (size_t)kMaxFused * kAccWords * 8);
         J->fused_acc = J->pair_count ? J->pair_count + 2 : nullptr;
-        J->rows_dev = (unsigned long long*)dalloc(16);
-        if (!J->pair_count || !J->rows_dev || !zero_fill(J->rows_dev, 8)) {
+        // The arena words are set by the first append (an arena with no
+        // append has no rows, known on the host).
+        J->rows_dev = (unsigned long long*)dalloc(kArenaWords * 8);
+        J->readback = (unsigned long long*)halloc_pinned(sizeof(unsigned long long) * kReadbackWords);
+        if (!J->pair_count || !J->rows_dev || !J->readback) {
             build_free(J);
             return false;
         }
