@@ -46,6 +46,13 @@ using namespace agg;
 typedef long long ll2 __attribute__((ext_vector_type(2)));
 
 constexpr int kMaxStageWords = 8;     // staged row words the bucket sort moves
+// Rows of a batch from which the bucketed path pays (below, its fixed
+// costs -- four launches, three host synchronisations -- exceed the atomic
+// path's time), and table capacity
+// (slots, from Init / ExpectRowGroups or growth) that selects it.
+constexpr int64_t kSortedMinRows = (int64_t)1 << 22;
+constexpr uint64_t kSortedMinSlots = (uint64_t)1 << 20;
+constexpr int64_t kSortedItemRows = (int64_t)1 << 16;
 constexpr int kSortedBlock = 1024;    // threads of a bucket-aggregation workgroup
 
 struct StageLayout {
@@ -71,19 +78,40 @@ __device__ __forceinline__ void store_width(char* data, int width, unsigned int 
     }
 }
 
-// Validity bits are written a byte (8 groups) per thread to avoid races.
+// One group per lane: a wave takes 64 consecutive groups (its first a
+// multiple of 64), reads their records together (word k of 64 records per
+// load) and writes each validity bitmap's 64 bits, and a bool key's data
+// bits, as one word from its ballot.
 __global__ void __launch_bounds__(kBlock) k_agg_finalize(FinArgs F)
 {
-    const unsigned int nbytes = (F.groups + 7u) / 8u;
-    for (unsigned int b = blockIdx.x * blockDim.x + threadIdx.x; b < nbytes; b += gridDim.x * blockDim.x) {
-        uint8_t masks[QDB_AGG_MAX_KEYS + QDB_AGG_MAX_AGGS];
-        for (int c = 0; c < F.nkeys + F.nred; ++c) masks[c] = 0;
-        for (unsigned int g = b * 8u; g < b * 8u + 8u && g < F.groups; ++g) {
-            const uint64_t* r = F.recs + (size_t)(F.index ? F.index[g] : g) * (size_t)F.stride;
-            for (int f = 0; f < F.nkeys; ++f) {
-                const KeyFieldDesc kf = F.keys[f];
-                bool is_null = kf.nullable && ((r[kf.null_word] >> kf.null_bit) & 1ull);
-                u128 v = 0;
+    const int lane = lane_id();
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x - lane; base < F.groups; base += step) {
+        const unsigned int g = (unsigned int)(base + lane);
+        const bool live = g < F.groups;
+        const uint64_t* r = F.recs;
+        if (live) {
+            if (g < F.pend_n) {
+                // The run holding group g: the last k with pend_prefix[k] <= g.
+                int lo = 0, hi = F.pend_runs - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (F.pend_prefix[mid] <= g) lo = mid;
+                    else hi = mid - 1;
+                }
+                r = F.pend_recs + ((uint64_t)F.pend_items[lo].row0 + (g - F.pend_prefix[lo])) * (uint64_t)F.R;
+            } else {
+                const unsigned int gg = g - F.pend_n;
+                r = F.recs + (size_t)(F.index ? F.index[gg] : gg) * (size_t)F.stride;
+            }
+        }
+        const uint64_t word_at = base >> 6;  // the wave's 64-bit bitmap word
+        for (int f = 0; f < F.nkeys; ++f) {
+            const KeyFieldDesc kf = F.keys[f];
+            bool is_null = true;
+            u128 v = 0;
+            if (live) {
+                is_null = kf.nullable && ((r[kf.null_word] >> kf.null_bit) & 1ull);
                 if (!is_null) {
                     if (kf.width == 16) {
                         v = ((u128)r[kf.word + 1] << 64) | (u128)r[kf.word];
@@ -92,22 +120,27 @@ __global__ void __launch_bounds__(kBlock) k_agg_finalize(FinArgs F)
                         v = (r[kf.word] >> kf.shift) & m;
                     }
                 }
-                if (kf.kind == QDB_KIND_BOOL) {
-                    // bool keys are emitted as an LSB bitmap in the data buffer.
-                    if (v) F.out_data[f][g >> 3] |= (char)(1u << (g & 7));
-                } else {
-                    store_width(F.out_data[f], F.key_out_width[f], g, v);
-                }
-                if (!is_null) masks[f] |= (uint8_t)(1u << (g & 7));
             }
-            const uint64_t* s = r + F.KW;
-            const int64_t rows = (int64_t)s[F.rows_word];
-            for (int a = 0; a < F.nred; ++a) {
-                const RedOut ro = F.red[a];
-                const int col = F.nkeys + a;
+            if (kf.kind == QDB_KIND_BOOL) {
+                // bool keys are emitted as an LSB bitmap in the data buffer.
+                const uint64_t bits = __ballot(live && v != 0);
+                if (lane == 0) ((uint64_t*)F.out_data[f])[word_at] = bits;
+            } else if (live) {
+                store_width(F.out_data[f], F.key_out_width[f], g, v);
+            }
+            const uint64_t valid_bits = __ballot(live && !is_null);
+            if (F.out_mask[f] && lane == 0) ((uint64_t*)F.out_mask[f])[word_at] = valid_bits;
+        }
+        const uint64_t* s = r + F.KW;
+        const int64_t rows = live ? (int64_t)s[F.rows_word] : 0;
+        for (int a = 0; a < F.nred; ++a) {
+            const RedOut ro = F.red[a];
+            const int col = F.nkeys + a;
+            bool valid = false;
+            u128 v = 0;
+            if (live) {
                 const int64_t contributions = ro.wvalid >= 0 ? (int64_t)s[ro.wvalid] : rows;
-                bool valid = true;
-                u128 v = 0;
+                valid = true;
                 switch (ro.kind) {
                 case RK_COUNT_ROWS: v = (u128)(uint64_t)rows; break;
                 case RK_COUNT_VALID: v = (u128)s[ro.w0]; break;
@@ -152,11 +185,9 @@ __global__ void __launch_bounds__(kBlock) k_agg_finalize(FinArgs F)
                 }
                 if (!valid) v = 0;
                 store_width(F.out_data[col], ro.out_width, g, v);
-                if (valid) masks[col] |= (uint8_t)(1u << (g & 7));
             }
-        }
-        for (int c = 0; c < F.nkeys + F.nred; ++c) {
-            if (F.out_mask[c]) F.out_mask[c][b] = masks[c];
+            const uint64_t valid_bits = __ballot(valid);
+            if (F.out_mask[col] && lane == 0) ((uint64_t*)F.out_mask[col])[word_at] = valid_bits;
         }
     }
 }
@@ -297,10 +328,43 @@ struct AggTable {
     int rstride = 1;                    // words per record
     unsigned int extra_used = 0;        // the extra record holds a group
     unsigned int* counters = nullptr;   // device [4]
-    unsigned int groups = 0;
+    unsigned int groups = 0;            // groups in the slot table
     int64_t expect_groups = 0;          // ExpectRowGroups hint
     int match_rounds = 0;
+    // A table made with many slots gets its buffers at first need: a
+    // bucketed update into it while it is empty keeps the groups of the
+    // buckets that one workgroup held whole as record runs (pend_*), which
+    // finish reads directly and any other use first merges into the slot
+    // table. Their keys occur in no other run and not in the slot table.
+    bool allocated = true;
+    uint64_t cap_hint = 0;              // the slots asked for at Init
+    uint64_t* pend_recs = nullptr;      // records of R words
+    SortedItem* pend_items = nullptr;   // device: the runs (row0 = first record)
+    uint32_t* pend_counts = nullptr;    // device: records per run
+    uint32_t* pend_prefix = nullptr;    // device: exclusive prefix of the counts (runs + 1)
+    int pend_runs = 0;
+    unsigned int pend_groups = 0;
 };
+
+// Groups of a table: the slot table's and the pending runs'.
+inline unsigned int table_size(const AggTable* T)
+{
+    return T->groups + T->pend_groups;
+}
+
+void table_drop_pending(AggTable* T)
+{
+    dfree(T->pend_recs);
+    dfree(T->pend_items);
+    dfree(T->pend_counts);
+    dfree(T->pend_prefix);
+    T->pend_recs = nullptr;
+    T->pend_items = nullptr;
+    T->pend_counts = nullptr;
+    T->pend_prefix = nullptr;
+    T->pend_runs = 0;
+    T->pend_groups = 0;
+}
 
 // The HashTable blob (qdb/exec/aggregate_exec.cpp:18-39). The processor reads
 // Size; the first word holds the device table.
@@ -1075,7 +1139,16 @@ AggTable* table_new(AggPlan* P, int64_t slots_hint)
     while (cap < (uint64_t)slots_hint) cap *= 2;
     const int64_t groups_hint = slots_hint / 2;
     T->counters = (unsigned int*)dalloc(16);
-    if (!T->counters || !zero_fill(T->counters, 16) || !table_alloc(T, cap)) {
+    T->cap_hint = cap;
+    // Tables large enough for the bucketed path get their buffers at first
+    // need (see AggTable).
+    const bool lazy = cap >= kSortedMinSlots;
+    if (lazy) {
+        T->capacity = cap;
+        T->rstride = record_stride(P->KW + P->proto.nwords);
+        T->allocated = false;
+    }
+    if (!T->counters || !zero_fill(T->counters, 16) || (!lazy && !table_alloc(T, cap))) {
         dfree(T->counters);
         dfree(T->ctrl);
         dfree(T->recs);
@@ -1100,6 +1173,7 @@ void table_free(AggTable* T)
     dfree(T->occ);
     dfree(T->recs);
     dfree(T->counters);
+    table_drop_pending(T);
     delete T;
 }
 
@@ -1221,6 +1295,8 @@ __global__ void k_seed_groups(AggArgs A, SeedGroups G)
 }
 
 // Seeds the empty-input groups of a new table; T->groups counts them.
+bool table_ready(AggTable* T);
+
 bool table_seed(AggTable* T)
 {
     const AggPlan* P = T->plan;
@@ -1229,6 +1305,7 @@ bool table_seed(AggTable* T)
     G.n = P->spec.NumEmptyInputGroups;
     for (int i = 0; i < G.n; ++i) G.id[i] = P->spec.EmptyInputGroups[i];
     if (G.n == 0) return true;
+    if (!table_ready(T)) return false;
     AggArgs A = P->proto;
     table_args(T, &A);
     switch (P->KW) {
@@ -1290,6 +1367,7 @@ struct UpdateLaunch {
 
 bool prepare_launch(AggTable* T, const AggArgs& bound, int64_t base, int64_t n, size_t recs_cap, UpdateLaunch* L)
 {
+    if (!table_ready(T)) return false;
     const AggPlan* P = T->plan;
     const int R = P->KW + bound.nwords;
     L->T = T;
@@ -1360,13 +1438,59 @@ bool settle_launch(UpdateLaunch* L, unsigned int cnt[4])
     return true;
 }
 
-// Rows of a batch from which the bucketed path pays (below, its fixed
-// costs -- four launches, three host synchronisations -- exceed the atomic
-// path's time), and table capacity
-// (slots, from Init / ExpectRowGroups or growth) that selects it.
-constexpr int64_t kSortedMinRows = (int64_t)1 << 22;
-constexpr uint64_t kSortedMinSlots = (uint64_t)1 << 20;
-constexpr int64_t kSortedItemRows = (int64_t)1 << 16;
+// Merges the pending runs into the slot table: each of their keys occurs
+// once and not in the table, so the merge stores the words of a key it
+// inserts as they are.
+bool table_merge_pending(AggTable* T)
+{
+    if (T->pend_runs == 0) return true;
+    AggPlan* P = T->plan;
+    hipFunction_t f_merge = plan_jit_function(P, "qdb_agg_merge");
+    if (!f_merge) return fail("bucketed aggregate kernels unavailable");
+    // Taken off the table first: the launch below sees a table without them.
+    uint64_t* recs = T->pend_recs;
+    SortedItem* items = T->pend_items;
+    uint32_t* counts = T->pend_counts;
+    uint32_t* prefix = T->pend_prefix;
+    int nitems = T->pend_runs;
+    const unsigned int total = T->pend_groups;
+    T->pend_recs = nullptr;
+    T->pend_items = nullptr;
+    T->pend_counts = nullptr;
+    T->pend_prefix = nullptr;
+    T->pend_runs = 0;
+    T->pend_groups = 0;
+    unsigned int* head = (unsigned int*)dalloc(16);
+    bool ok = head != nullptr;
+    if (ok && (uint64_t)T->groups + total > T->capacity / 2) ok = table_grow(T, (uint64_t)T->groups + total);
+    UpdateLaunch L;
+    ok = ok && prepare_launch(T, P->proto, 0, 1, (size_t)total + 1024, &L) && set_counters(T) && zero_fill(head, 16);
+    if (ok) {
+        const int grid = nitems < num_cus() * 4 ? nitems : num_cus() * 4;
+        void* args[] = {&L.A, &recs, &items, &counts, &nitems, &head};
+        ok = hip_ok(hipModuleLaunchKernel(f_merge, (unsigned)grid, 1, 1, kBlock, 1, 1, 0, stream(), args, nullptr), "merge launch");
+    }
+    unsigned int cnt[4];
+    ok = ok && read_counters(T, cnt) && settle_launch(&L, cnt);
+    L.release();
+    dfree(head);
+    dfree(recs);
+    dfree(items);
+    dfree(counts);
+    dfree(prefix);
+    return ok;
+}
+
+// The slot table made (at its capacity) and every pending group in it.
+bool table_ready(AggTable* T)
+{
+    if (!T->allocated) {
+        if (!table_alloc(T, T->capacity)) return false;
+        T->allocated = true;
+    }
+    return table_merge_pending(T);
+}
+
 
 bool sorted_path_wanted(AggPlan* P, const AggTable* T, int64_t rows)
 {
@@ -1376,7 +1500,8 @@ bool sorted_path_wanted(AggPlan* P, const AggTable* T, int64_t rows)
         min_rows = atoll(e);
         if (min_rows <= 0) return false;
     }
-    return rows >= min_rows && T->capacity >= kSortedMinSlots && stage_layout(P).words > 0 &&
+    const uint64_t slots = T->capacity > T->cap_hint ? T->capacity : T->cap_hint;
+    return rows >= min_rows && slots >= kSortedMinSlots && stage_layout(P).words > 0 &&
         plan_jit_function(P, "qdb_agg_sorted") != nullptr;
 }
 
@@ -1386,6 +1511,11 @@ bool sorted_path_wanted(AggPlan* P, const AggTable* T, int64_t rows)
 bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n)
 {
     AggPlan* P = T->plan;
+    // An empty table whose buffers are not made yet keeps the groups of the
+    // buckets that one workgroup held whole as pending runs; otherwise the
+    // slot table takes every group.
+    const bool lazy = !T->allocated && T->groups == 0 && T->pend_runs == 0;
+    if (!lazy && !table_ready(T)) return false;
     const StageLayout SL = stage_layout(P);
     const int W = SL.words, KW = P->KW, R = KW + bound.nwords;
     // LDS table of the bucket workgroups: the most groups that fit next to
@@ -1531,6 +1661,69 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n)
     }
     uint64_t total = 0;
     for (uint32_t c : h_counts) total += c & ~kSharedItem;
+    if (lazy) {
+        // Owned runs stay pending; the records of split buckets (and of
+        // items whose rows overflowed the LDS table) may repeat keys and go
+        // to a slot table sized for them.
+        std::vector<SortedItem> own, shared;
+        std::vector<uint32_t> own_n, shared_n, prefix;
+        uint64_t own_total = 0, shared_total = 0;
+        for (int i = 0; i < nitems; ++i) {
+            const uint32_t c = h_counts[(size_t)i];
+            const uint32_t k = c & ~kSharedItem;
+            if (k == 0) continue;
+            if (items[(size_t)i].owned && !(c & kSharedItem)) {
+                prefix.push_back((uint32_t)own_total);
+                own.push_back(items[(size_t)i]);
+                own_n.push_back(k);
+                own_total += k;
+            } else {
+                shared.push_back(items[(size_t)i]);
+                shared_n.push_back(c);
+                shared_total += k;
+            }
+        }
+        prefix.push_back((uint32_t)own_total);
+        bool ok = true;
+        if (!shared.empty()) {
+            uint64_t cap = 1024;
+            while (cap / 2 < shared_total + 1024) cap *= 2;
+            ok = table_alloc(T, cap);
+            if (ok) T->allocated = true;
+            SortedItem* d_shared = ok ? (SortedItem*)upload(shared.data(), shared.size() * sizeof(SortedItem)) : nullptr;
+            unsigned int* d_shared_n = ok ? (unsigned int*)upload(shared_n.data(), shared_n.size() * 4) : nullptr;
+            int nshared = (int)shared.size();
+            UpdateLaunch L;
+            ok = ok && d_shared && d_shared_n && prepare_launch(T, bound, 0, 1, (size_t)shared_total + 1024, &L) && set_counters(T) &&
+                zero_fill(head, 16);
+            if (ok) {
+                const int grid = nshared < num_cus() * 4 ? nshared : num_cus() * 4;
+                void* args[] = {&L.A, &recs, &d_shared, &d_shared_n, &nshared, &head};
+                ok = hip_ok(hipModuleLaunchKernel(f_merge, (unsigned)grid, 1, 1, kBlock, 1, 1, 0, stream(), args, nullptr), "merge launch");
+            }
+            unsigned int cnt[4];
+            ok = ok && read_counters(T, cnt) && settle_launch(&L, cnt);
+            L.release();
+            dfree(d_shared);
+            dfree(d_shared_n);
+        }
+        if (ok && !own.empty()) {
+            T->pend_items = (SortedItem*)upload(own.data(), own.size() * sizeof(SortedItem));
+            T->pend_counts = (uint32_t*)upload(own_n.data(), own_n.size() * 4);
+            T->pend_prefix = (uint32_t*)upload(prefix.data(), prefix.size() * 4);
+            ok = T->pend_items && T->pend_counts && T->pend_prefix;
+            if (ok) {
+                T->pend_recs = (uint64_t*)recs;
+                recs = nullptr;  // kept by the table
+                T->pend_runs = (int)own.size();
+                T->pend_groups = (unsigned int)own_total;
+            } else {
+                table_drop_pending(T);
+            }
+        }
+        release();
+        return ok;
+    }
     // Room for every record as a new group keeps the merge free of spills.
     if ((uint64_t)T->groups + total > T->capacity / 2 && !table_grow(T, (uint64_t)T->groups + total)) {
         release();
@@ -1739,6 +1932,12 @@ int64_t table_finish_launch(AggTable* T, TRowSet* out, FinishTemps* tmp)
     F.recs = T->recs;
     F.stride = T->rstride;
     F.index = nullptr;
+    // The pending runs first, then the slot table's groups.
+    F.pend_recs = T->pend_recs;
+    F.pend_items = T->pend_items;
+    F.pend_prefix = T->pend_prefix;
+    F.pend_runs = T->pend_runs;
+    F.pend_n = T->pend_groups;
     uint32_t*& index = tmp->index;
     uint32_t*& counts = tmp->counts;
     uint32_t*& offsets = tmp->offsets;
@@ -1778,7 +1977,7 @@ int64_t table_finish_launch(AggTable* T, TRowSet* out, FinishTemps* tmp)
         }
         F.index = index;
     }
-    const unsigned int n = T->groups;
+    const unsigned int n = table_size(T);
     OwnedRowSet* o = init_owned_rowset(out, n, P->nout);
     F.groups = n;
     const size_t mask_bytes = (size_t)((n + 7) / 8) + 8;
@@ -1811,7 +2010,7 @@ int64_t table_finish_launch(AggTable* T, TRowSet* out, FinishTemps* tmp)
     }
     if (n > 0) {
         TimedRegion timer("agg_finalize");
-        const int grid = grid_for((n + 7) / 8, kBlock, 8);
+        const int grid = grid_for(n, kBlock, 8);
         hipLaunchKernelGGL(k_agg_finalize, dim3(grid), dim3(kBlock), 0, stream(), F);
         if (!hip_ok(hipGetLastError(), "agg finalize launch")) {
             release();
@@ -1892,11 +2091,11 @@ int64_t agg_dispatch_impl(AggPlan* P, void* ht, TRowSet* batch, int64_t arg, int
             fail("aggregate update before Init");
             return -1;
         }
-        if (!batch) return (int64_t)T->groups;
+        if (!batch) return (int64_t)table_size(T);
         if (!table_update(T, batch)) return -1;
-        B->Size = T->groups;
+        B->Size = table_size(T);
         B->Capacity = (int64_t)T->capacity;
-        return (int64_t)T->groups;
+        return (int64_t)table_size(T);
     }
     default:
         return 0;
@@ -1988,7 +2187,7 @@ extern "C" int qdb_gpu_agg_update_batched(const QdbAggregateKernels* kernels, vo
     if (!batched_update(P, tables.data(), batches, n)) return -1;
     for (int64_t i = 0; i < n; ++i) {
         HtBlob* B = (HtBlob*)hts[i];
-        B->Size = B->table->groups;
+        B->Size = table_size(B->table);
         B->Capacity = (int64_t)B->table->capacity;
     }
     return 0;

@@ -1192,7 +1192,15 @@ struct RedOut {
 
 struct FinArgs {
     const uint64_t* recs;
-    const uint32_t* index;    // slot of output group g (null: g itself)
+    const uint32_t* index;    // slot of output group g - pend_n (null: itself)
+    // Output groups [0, pend_n) come from record runs of R words each: run
+    // k is pend_items[k].row0 on, pend_prefix[k + 1] - pend_prefix[k]
+    // records; the groups after them from recs / index.
+    const uint64_t* pend_recs;
+    const SortedItem* pend_items;
+    const uint32_t* pend_prefix;
+    int pend_runs;
+    unsigned int pend_n;
     int stride;               // words between records
     int R;
     int KW;

@@ -129,8 +129,12 @@ __global__ void __launch_bounds__(kBlock) k_append_count(DevCols cols, JKeyDesc 
     unsigned int* block_counts, unsigned long long* block_range)
 {
     __shared__ unsigned int total;
-    __shared__ unsigned long long wave_range[kBlock / kWave][2];
-    if (threadIdx.x == 0) total = 0;
+    __shared__ unsigned long long range[2];
+    if (threadIdx.x == 0) {
+        total = 0;
+        range[0] = 0;
+        range[1] = 0;
+    }
     __syncthreads();
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     bool keep = false;
@@ -139,30 +143,16 @@ __global__ void __launch_bounds__(kBlock) k_append_count(DevCols cols, JKeyDesc 
     if (i < n && (!sel || sel[i])) keep = join_key<KW>(cols, K, i, key);
     const uint64_t b = __ballot(keep);
     if (lane_id() == 0 && b) atomicAdd(&total, (unsigned int)__popcll(b));
-    if (KW == 1 && block_range) {
-        unsigned long long nmin = keep ? ~key[0] : 0ull, mx = keep ? key[0] : 0ull;
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const unsigned long long a = shfl_xor_u64(nmin, o), c = shfl_xor_u64(mx, o);
-            nmin = a > nmin ? a : nmin;
-            mx = c > mx ? c : mx;
-        }
-        if (lane_id() == 0) {
-            wave_range[threadIdx.x / kWave][0] = nmin;
-            wave_range[threadIdx.x / kWave][1] = mx;
-        }
+    if (KW == 1 && block_range && keep) {
+        atomicMax(&range[0], (unsigned long long)~key[0]);
+        atomicMax(&range[1], (unsigned long long)key[0]);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         block_counts[blockIdx.x] = total;
         if (KW == 1 && block_range) {
-            unsigned long long nmin = 0, mx = 0;
-            for (int w = 0; w < kBlock / kWave; ++w) {
-                nmin = wave_range[w][0] > nmin ? wave_range[w][0] : nmin;
-                mx = wave_range[w][1] > mx ? wave_range[w][1] : mx;
-            }
-            block_range[2 * blockIdx.x] = nmin;
-            block_range[2 * blockIdx.x + 1] = mx;
+            block_range[2 * blockIdx.x] = range[0];
+            block_range[2 * blockIdx.x + 1] = range[1];
         }
     }
 }
