@@ -565,6 +565,28 @@ int qdb_gpu_gather_rowset(const TRowSet* rs, const QdbColumnSchema* types, const
  * 0 or -1. */
 int qdb_gpu_agg_partial_states(const QdbAggregateKernels* kernels, const TRowSet* batch, TRowSet* out);
 
+/* ORDER BY (TSortProcessor's radix sort, qdb/exec/sort_exec.cpp:393-427;
+ * kernels qdb/kernel/sort/radix.oz, radix_nullable.oz): the selected rows
+ * of a device batch ordered by the keys, first key most significant, equal
+ * keys in input order. Order keys as the reference's qumir_radix_key:
+ * signed integers, dates and decimals by value, floats in PostgreSQL order
+ * (NaN above +inf, -0 equal to +0), bool false < true; Desc reverses a key;
+ * Nulls places its NULL rows (DEFAULT: last ascending, first descending,
+ * qdb/plan/ops/sort_key.h:21-26). String keys are not supported. row_index
+ * (device, RowCount entries) receives the ordered row indexes, *count how
+ * many rows were selected; materialize them with qdb_gpu_gather_rowset
+ * (a [start, start + limit) slice of row_index for a LIMIT). 0 or -1. */
+#define QDB_SORT_MAX_KEYS 8
+enum { QDB_SORT_NULLS_DEFAULT = 0, QDB_SORT_NULLS_FIRST = 1, QDB_SORT_NULLS_LAST = 2 };
+typedef struct QdbSortKey {
+    int32_t Column;
+    int8_t Desc;
+    int8_t Nulls;
+    int8_t Pad[2];
+} QdbSortKey;
+int qdb_gpu_sort_rowset(const TRowSet* rs, const QdbColumnSchema* types, const QdbSortKey* keys, int32_t nkeys,
+    uint32_t* row_index, int64_t* count);
+
 /* Kernel timing of the last launches, for roofline reporting: the average
  * duration in milliseconds of the named kernel family ("agg_update",
  * "join_probe", ...) measured with HIP events on the stream it runs on since

@@ -494,3 +494,61 @@ class PartialAggregate:
         if groups >= self.max_groups:
             self.emitted_groups += groups
             self._flush()
+
+
+def _radix_key_values(t, vals):
+    """qumir_radix_key (qdb/kernel/sort/radix.oz:4-60) of every value, as
+    Python ints; with the key's bit count."""
+    out = []
+    if t.kind == abi.BOOL:
+        return [1 if v else 0 for v in vals], 8
+    if t.kind == abi.FLOAT:
+        w = 8 if t.width == 8 else 4
+        for v in vals:
+            b = int(np.array([v], dtype=np.float64 if w == 8 else np.float32).view(np.uint64 if w == 8 else np.uint32)[0])
+            top = 1 << (8 * w - 1)
+            mag = b & (top - 1)
+            inf = 0x7FF0000000000000 if w == 8 else 0x7F800000
+            if mag > inf:
+                out.append((1 << (8 * w)) - 1)
+                continue
+            if mag == 0:
+                b = 0
+            out.append(b ^ (((1 << (8 * w)) - 1) if b >> (8 * w - 1) else top))
+        return out, 8 * w
+    bits = 8 * t.width
+    for v in vals:
+        v = int(v)
+        if t.kind == abi.UNSIGNED:
+            out.append(v)
+        else:
+            out.append((v + (1 << (bits - 1))) & ((1 << bits) - 1))
+    return out, bits
+
+
+def sort_rows(types, columns, valid, selection, keys):
+    """ORDER BY as the reference's radix sort of row ids: the selected rows,
+    then for each key from last to first, for each 8-bit digit from the
+    lowest, a stable counting sort by the digit (255 - digit descending);
+    a nullable key's NULL rows in a bucket before (NULLS FIRST) or after
+    every digit (radix_count_sort_indices / _nullable,
+    qdb/kernel/sort/radix.oz:62-138, radix_nullable.oz:5-101; the default
+    placement of qdb/plan/ops/sort_key.h:21-26). keys: (column, desc,
+    nulls) with nulls 0 default, 1 first, 2 last. Returns row indexes."""
+    n = len(columns[0]) if columns else 0
+    order = [i for i in range(n) if selection is None or selection[i]]
+    for col, desc, nulls in reversed([tuple(k) for k in keys]):
+        t = types[col]
+        rk, bits = _radix_key_values(t, columns[col])
+        v = valid[col] if valid is not None else None
+        first = desc if nulls == 0 else nulls == 1
+        for digit in range(0, bits, 8):
+            def bucket(row):
+                if v is not None and not v[row]:
+                    return 0 if first else 256
+                d = (rk[row] >> digit) & 255
+                if desc:
+                    d = 255 - d
+                return d + 1 if (v is not None and first) else d
+            order = sorted(order, key=bucket)  # Python's sort is stable
+    return order

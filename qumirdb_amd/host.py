@@ -958,3 +958,36 @@ class TPartialAggregateProcessor:
             self.close()
         except Exception:
             pass
+
+
+class SortKernel:
+    """ORDER BY over a device batch (TSortProcessor's radix sort,
+    qdb/exec/sort_exec.cpp:393-427): keys are (column, desc, nulls) with
+    nulls one of abi.SORT_NULLS_*; the selected rows ordered, equal keys in
+    input order."""
+
+    def __init__(self, types: Sequence[ValueType], keys: Sequence[tuple]):
+        self.types = list(types)
+        self.schema = schema_c(self.types)
+        self.nkeys = len(keys)
+        self.keys = (abi.QdbSortKey * max(1, self.nkeys))()
+        for i, k in enumerate(keys):
+            col, desc, nulls = (list(k) + [False, abi.SORT_NULLS_DEFAULT])[:3]
+            self.keys[i].Column = int(col)
+            self.keys[i].Desc = int(bool(desc))
+            self.keys[i].Nulls = int(nulls)
+
+    def order(self, rowset):
+        """(row_index DeviceBuffer of uint32, selected row count)."""
+        idx = DeviceBuffer(max(rowset.rows, 1) * 4)
+        cnt = C.c_int64(0)
+        check(lib().qdb_gpu_sort_rowset(rowset.ptr, C.byref(self.schema), self.keys, self.nkeys, idx.ptr, C.byref(cnt)),
+              "sort_rowset")
+        return idx, int(cnt.value)
+
+    def __call__(self, rowset, start: int = 0, limit: Optional[int] = None) -> KernelRowSet:
+        """The ordered rows [start, start + limit) as a new device batch."""
+        idx, n = self.order(rowset)
+        start = max(0, min(start, n))
+        count = n - start if limit is None else max(0, min(limit, n - start))
+        return gather_rowset(rowset, idx, start, count)
