@@ -565,6 +565,15 @@ int qdb_gpu_gather_rowset(const TRowSet* rs, const QdbColumnSchema* types, const
  * 0 or -1. */
 int qdb_gpu_agg_partial_states(const QdbAggregateKernels* kernels, const TRowSet* batch, TRowSet* out);
 
+/* Decimal storage (DecimalStorageValues, qdb/io/parquet/decimal_storage.cpp:
+ * 36-60; DecimalStorageOf, qdb/plan/types/decimal.h:80-99): n unscaled
+ * DECIMAL(precision) values of from_width bytes (4, 8, 16 or 32, little-
+ * endian two's complement, as Arrow decodes them) at src (device) written to
+ * dst (device) in the storage width, 8 bytes for precision <= 18, else 16:
+ * the low bytes of the sign-extended value. Asynchronous on the library's
+ * stream. 0 or -1. */
+int qdb_gpu_decimal_storage(const void* src, int32_t from_width, int64_t n, int32_t precision, void* dst);
+
 /* ORDER BY (TSortProcessor's radix sort, qdb/exec/sort_exec.cpp:393-427;
  * kernels qdb/kernel/sort/radix.oz, radix_nullable.oz): the selected rows
  * of a device batch ordered by the keys, first key most significant, equal

@@ -219,7 +219,7 @@ EXPORTS = [
     "qdb_gpu_join_processor_arena", "qdb_gpu_join_processor_next", "qdb_gpu_join_processor_free",
     "qdb_gpu_rowset_release", "qdb_gpu_set_host_heap", "qdb_gpu_destroy_kernel_rowset", "qdb_gpu_adopt_kernel_rowset",
     "qdb_gpu_set_row_error_handler", "qdb_gpu_take_row_error", "qdb_gpu_join_probe_fused_noalloc",
-    "qdb_gpu_sort_rowset", "qdb_gpu_shuffle_partition", "qdb_gpu_gather_rowset", "qdb_gpu_agg_partial_states", "qdb_gpu_partial_agg_new",
+    "qdb_gpu_decimal_storage", "qdb_gpu_sort_rowset", "qdb_gpu_shuffle_partition", "qdb_gpu_gather_rowset", "qdb_gpu_agg_partial_states", "qdb_gpu_partial_agg_new",
     "qdb_gpu_partial_agg_add", "qdb_gpu_partial_agg_next", "qdb_gpu_partial_agg_finish", "qdb_gpu_partial_agg_passing_through",
     "qdb_gpu_partial_agg_free",
     "qdb_gpu_rf_new", "qdb_gpu_rf_build", "qdb_gpu_rf_publish", "qdb_gpu_rf_probe", "qdb_gpu_rf_stats", "qdb_gpu_rf_free",
@@ -295,6 +295,7 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "qdb_gpu_take_row_error": (i64, []),
         "qdb_gpu_join_probe_fused_noalloc": (C.c_int, [P, C.POINTER(TRowSet), C.POINTER(QdbFusedResult), i64,
                                                        C.POINTER(i64)]),
+        "qdb_gpu_decimal_storage": (C.c_int, [P, C.c_int32, i64, C.c_int32, P]),
         "qdb_gpu_sort_rowset": (C.c_int, [C.POINTER(TRowSet), C.POINTER(QdbColumnSchema), C.POINTER(QdbSortKey), C.c_int32, P,
                                          C.POINTER(i64)]),
         "qdb_gpu_shuffle_partition": (C.c_int, [C.POINTER(TRowSet), P, C.c_int32, P, C.c_int32, P, C.POINTER(i64)]),

@@ -23,7 +23,13 @@ TABLES = {
     "aggregate_empty.nulls": "cases/aggregate_empty/tables/nulls.json",
     "outer_join.c": "cases/outer_join/tables/c.json",
     "outer_join.o": "cases/outer_join/tables/o.json",
+    "decimal_parquet.layout": "cases/decimal_parquet/tables/layout.json",
 }
+
+# Data files of the reference's tests copied as they are (a Parquet file of
+# the decimal scan cases: FIXED_LEN_BYTE_ARRAY decimals of 4, 7, 8 and 16
+# bytes, 16 row groups).
+DATA_FILES = {"decimals.parquet": "cases/decimal_parquet/tables/decimals.parquet"}
 
 GOLDENS = [
     "decimal_parquet/aggregates", "decimal_parquet/group_by", "decimal_parquet/casts", "decimal_parquet/join",
@@ -34,6 +40,8 @@ GOLDENS = [
     "aggregate_empty/grouped_empty",
     "outer_join/duplicates", "outer_join/null_keys", "outer_join/all_matched", "outer_join/none_matched",
     "outer_join/right_join",
+    "decimal_parquet/order_limit", "decimal_parquet/order_wide", "decimal_parquet/filter_bounds",
+    "decimal_parquet/layouts_agree",
 ]
 
 # Goldens whose expected outcome is a runtime error (the error text).
@@ -58,6 +66,12 @@ def main():
             out["errors"][g] = {"text": f.read().strip(), "from": rel}
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1)
+    here = os.path.dirname(OUT)
+    for name, rel in DATA_FILES.items():
+        with open(os.path.join(REF, rel), "rb") as f:
+            data = f.read()
+        with open(os.path.join(here, name), "wb") as f:
+            f.write(data)
     print(f"wrote {OUT}")
 
 

@@ -253,3 +253,34 @@ def test_device_sort_full_size_properties(gpu):
     assert np.all(np.diff(ka) <= 0)
     same = np.diff(ka) == 0
     assert np.all(np.diff(kb)[same] > 0)
+
+
+def _order_wide_input():
+    """decimal_parquet/order_wide: SELECT id, d2, d10 FROM w WHERE d10 IS NOT
+    NULL ORDER BY d10 DESC, id LIMIT 4 -- the wide table with its selection."""
+    from helpers import table_columns
+    names, types, values, valid = table_columns(load_fixtures()["tables"]["decimal_parquet.wide"])
+    sel = valid[names.index("d10")].copy()
+    keys = [(names.index("d10"), True, 0), (names.index("id"), False, 0)]
+    return names, types, values, valid, sel, keys
+
+
+def _order_wide_rows(names, values, valid, order):
+    cols = [names.index(c) for c in ("id", "d2", "d10")]
+    return [tuple(None if (valid[c] is not None and not valid[c][r]) else int(values[c][r]) for c in cols) for r in order[:4]]
+
+
+def test_oracle_sort_order_wide_golden():
+    names, types, values, valid, sel, keys = _order_wide_input()
+    order = O.sort_rows(types, values, valid, sel, keys)
+    assert _order_wide_rows(names, values, valid, order) == golden_rows(load_fixtures()["goldens"]["decimal_parquet/order_wide"])
+
+
+@pytest.mark.gpu
+def test_device_sort_order_wide_golden(gpu):
+    """DESC on a 38-digit decimal, then id, with a selection, LIMIT 4."""
+    names, types, values, valid, sel, keys = _order_wide_input()
+    cols = [np_column(t, v) for t, v in zip(types, values)]
+    order, _, _ = _device_order(gpu, types, cols, valid, sel, keys)
+    assert _order_wide_rows(names, values, valid, order.tolist()) == \
+        golden_rows(load_fixtures()["goldens"]["decimal_parquet/order_wide"])
