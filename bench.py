@@ -229,19 +229,33 @@ def bench_groups(steps, warmup, rows=100_000_000, distinct=10_000_000):
             "achieved_gbs": nbytes / ((k_ms + f_ms) * 1e-3) / 1e9 if k_ms else 0.0}
 
 
-def bench_batched(n_instances=512):
-    """The batched form: 512 group-by instances of one aggregate (COUNT(*),
-    SUM(i64), MIN/MAX/SUM(f64) by an int64 key; rows log-uniform 10K..20M,
-    cardinality log-uniform 4..1M, uniform or Zipf(1.1) keys), whole queries
-    (table set-up, update, finish into HBM) one after another versus the
-    same instances with all updates in one persistent batched launch and
-    all finishes behind one synchronization."""
-    from qumirdb_amd import lib, workloads as W
-    from qumirdb_amd.host import AggregateKernels, TAggregateProcessor
+def bench_batched(n_groupby=256, n_join=256):
+    """The batched form: 512 instances, 256 group-bys (COUNT(*), SUM(i64),
+    MIN/MAX/SUM(f64) by an int64 key; rows log-uniform 10K..20M, cardinality
+    log-uniform 4..1M, uniform or Zipf(1.1) keys) and 256 equi joins (build
+    log-uniform 1K..2M unique int64 keys, probe log-uniform 10K..20M rows;
+    COUNT(*), exact SUM(value) and the matched pairs). Plain: whole queries
+    one after another (group-by: table set-up, update, finish into HBM; join:
+    append, build, probe). Batched: every group-by update in one persistent
+    launch and every finish behind one synchronization; every join build
+    behind one synchronization and every probe in one persistent launch.
+    Results are compared instance by instance afterwards: group-by outputs
+    in full (downloaded and sorted), joins by pair count, fused COUNT/SUM and
+    an order-independent checksum of the pairs."""
+    import ctypes
 
-    inst = W.batched_instances(n_instances)
+    import numpy as np
+
+    from qumirdb_amd import lib, workloads as W
+    from qumirdb_amd.host import (AggregateKernels, RowJoinKernels, TAggregateProcessor, TRowJoinProcessor,
+                                  download_rowset)
+
+    inst = W.batched_instances(n_groupby)
+    jinst = W.batched_join_instances(n_join)
     dev = W.BatchedDevice(inst)
+    jdev = W.BatchedJoinDevice(jinst)
     kern = AggregateKernels(W.groups_spec())
+    jk = RowJoinKernels(W.batched_join_spec())
 
     def capacity(rows, card):
         c = 8
@@ -252,48 +266,119 @@ def bench_batched(n_instances=512):
     caps = [capacity(r, c) for r, c, _ in inst]
 
     def plain():
-        groups = []
+        outs = []
         for d, cap in zip(dev.devs, caps):
             p = TAggregateProcessor(kern, initial_capacity=cap)
             p.add(d.rowset)
-            groups.append(p.finish_in_hbm())
+            outs.append(p.finish_raw())
             p.close()
-        return groups
+        jps = []
+        for jd in jdev.devs:
+            jp = TRowJoinProcessor(jk)
+            jp.append(jd.build)
+            jp.build()
+            jp.probe_only(jd.probe)
+            jp.fused_results()
+            jps.append(jp)
+        return outs, jps
 
     def batched():
         procs = [TAggregateProcessor(kern, initial_capacity=cap) for cap in caps]
         TAggregateProcessor.add_batched(procs, [d.rowset for d in dev.devs])
-        groups = TAggregateProcessor.finish_in_hbm_batched(procs)
+        raw = TAggregateProcessor._finish_batched_raw(procs)
+        outs = [raw[i] for i in range(len(procs))]
         for p in procs:
             p.close()
-        return groups
+        jps = [TRowJoinProcessor(jk) for _ in jdev.devs]
+        for jp, jd in zip(jps, jdev.devs):
+            jp.append(jd.build)
+        TRowJoinProcessor.build_batched(jps)
+        TRowJoinProcessor.probe_batched(jps, [jd.probe for jd in jdev.devs])
+        for jp in jps:
+            jp.fused_results()
+        return outs, jps
 
-    plain()
-    batched()  # warm-up: kernels compiled, allocator primed
+    def release(res):
+        outs, jps = res
+        for o in outs:
+            lib().qdb_gpu_rowset_release(ctypes.byref(o))
+        for jp in jps:
+            jp.close()
+
+    types = kern.out_types
+
+    def digest(res):
+        """Every group-by output downloaded and sorted by key (f64 as bits),
+        and every join's pair count, fused results and pair checksum; then
+        the results are released."""
+        outs, jps = res
+        gd = []
+        for o in outs:
+            r = download_rowset(o, types)
+            order = np.argsort(np.asarray(r[0][0]), kind="stable")
+            cols = []
+            for c in range(len(types)):
+                v = np.asarray(r[c][0])[order]
+                cols.append(v.view(np.int64) if v.dtype.kind == "f" else v)
+            gd.append(cols)
+        jd = [(jp.pair_count(), jp.fused_results(), jp.pairs_checksum()) for jp in jps]
+        release(res)
+        return gd, jd
+
+    # Autotuning on: the warm-up runs tune every shape bucket the instances
+    # fall in (plain and batched updates then run in the tuned configurations).
+    was_tuning = lib().qdb_gpu_autotune_enable(1)
+    release(plain())
+    release(batched())  # warm-up: kernels compiled, shapes tuned, allocator primed
     lib().qdb_gpu_synchronize()
     lib().qdb_gpu_timing_reset()
     lib().qdb_gpu_timing_enable(1)
     t0 = time.perf_counter()
-    g_plain = plain()
+    r_plain = plain()
     lib().qdb_gpu_synchronize()
     t1 = time.perf_counter()
     upd_ms, upd_n = timing("agg_update")
+    probe_ms, probe_n = timing("join_probe")
     lib().qdb_gpu_timing_reset()
+    d_plain = digest(r_plain)
     t2 = time.perf_counter()
-    g_batch = batched()
+    r_batch = batched()
     lib().qdb_gpu_synchronize()
     t3 = time.perf_counter()
     bat_ms, _ = timing("agg_batch")
+    bprobe_ms, _ = timing("join_probe_batched")
     lib().qdb_gpu_timing_enable(0)
     lib().qdb_gpu_timing_reset()
-    if g_plain != g_batch:
-        raise RuntimeError("batched and per-instance group counts differ")
-    rows = sum(r for r, _, _ in inst)
-    return {"instances": n_instances, "rows": rows, "groups": sum(g_batch),
-            "plain_s": t1 - t0, "batched_s": t3 - t2,
-            "plain_queries_per_s": n_instances / (t1 - t0), "batched_queries_per_s": n_instances / (t3 - t2),
+    d_batch = digest(r_batch)
+    lib().qdb_gpu_autotune_enable(was_tuning)
+    # Results, instance by instance.
+    groups = 0
+    for k, (a, b) in enumerate(zip(d_plain[0], d_batch[0])):
+        if len(a[0]) != len(b[0]) or not all(np.array_equal(x, y) for x, y in zip(a, b)):
+            raise RuntimeError(f"group-by instance {k} differs between plain and batched runs")
+        groups += len(a[0])
+    pairs = 0
+    for k, (a, b) in enumerate(zip(d_plain[1], d_batch[1])):
+        if a != b:
+            raise RuntimeError(f"join instance {k}: pairs or fused results differ between plain and batched runs")
+        pairs += a[0]
+    from qumirdb_amd import abi
+    ntune = lib().qdb_gpu_autotune_entries(None, 0)
+    ents = (abi.QdbTuneEntry * max(1, ntune))()
+    lib().qdb_gpu_autotune_entries(ents, ntune)
+    tuned = [{"key_words": e.KeyWords, "card_bucket": e.CardBucket, "rows_bucket": e.RowsBucket,
+              "config": ("bucketed, fan-out x%g" % (2.0 ** e.FanoutShift)) if e.Path == 1 else
+              f"LDS {e.LdsKiB} KiB x {e.GridPerCu}/CU", "ms": round(e.Ms, 3)} for e in ents[:ntune]]
+    n = n_groupby + n_join
+    rows = sum(r for r, _, _ in inst) + sum(b + p for b, p in jinst)
+    return {"instances": n, "groupby_instances": n_groupby, "join_instances": n_join, "rows": rows, "groups": groups,
+            "pairs": pairs, "plain_s": t1 - t0, "batched_s": t3 - t2,
+            "plain_queries_per_s": n / (t1 - t0), "batched_queries_per_s": n / (t3 - t2),
             "speedup": (t1 - t0) / (t3 - t2), "rows_per_s": rows / (t3 - t2), "ms_per_step": (t3 - t2) * 1e3,
-            "plain_update_kernels_ms": upd_ms * upd_n, "batched_update_kernel_ms": bat_ms}
+            "plain_update_kernels_ms": upd_ms * upd_n, "batched_update_kernel_ms": bat_ms,
+            "plain_probe_kernels_ms": probe_ms * probe_n, "batched_probe_kernel_ms": bprobe_ms,
+            "results": "equal (group-by outputs in full; joins: pair counts, fused COUNT/SUM, pair checksums)",
+            "autotuned_shapes": tuned}
 
 
 def bench_served(qps_levels=(50, 100, 200, 500, 1000, 2000), n_requests=120):

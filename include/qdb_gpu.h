@@ -640,6 +640,62 @@ int64_t qdb_gpu_project_kernel_source(const QdbProjectSpec* spec, char* out, int
  * agg_dispatch(Update) calls. Returns 0, or -1 with the error text. */
 int qdb_gpu_agg_update_batched(const QdbAggregateKernels* kernels, void* const* hts, const TRowSet* const* batches,
                                int64_t n);
+/* The aggregate autotuner (no counterpart in the reference, whose planner
+ * picks the table layout by its own rules): an update of a shape bucket
+ * (key words, ceil(log2(table slots)), floor(log2(batch rows))) that is not
+ * cached yet times every candidate configuration -- one pass over LDS
+ * tables of 32, 64 or 128 KiB per workgroup at 4, 2 or 1 workgroups per CU,
+ * or the bucketed path at 1/2x, 1x or 2x its bucket count -- as whole
+ * queries on scratch tables, caches the fastest, and every later update of
+ * that bucket (plain, batched, served) runs in it. Off by default (the
+ * sizing rule decides); QDB_GPU_AUTOTUNE=1 turns it on at start-up and
+ * QDB_GPU_TUNE_FORCE="path,lds_kib,grid_per_cu,fanout_shift" forces one
+ * configuration for every update. */
+typedef struct QdbTuneEntry {
+    int32_t KeyWords;
+    int32_t CardBucket;
+    int32_t RowsBucket;
+    int32_t Path;          /* 0: LDS tables + HBM table, 1: bucketed */
+    int32_t LdsKiB;        /* path 0 */
+    int32_t GridPerCu;     /* path 0 */
+    int32_t FanoutShift;   /* path 1: log2 of the bucket count relative to the sizing rule */
+    int32_t Pad;
+    double Ms;             /* the chosen configuration's time on the batch it was tuned on */
+} QdbTuneEntry;
+/* Turns tuning of uncached shapes on (1) or off (0); returns the previous state. */
+int qdb_gpu_autotune_enable(int on);
+/* Copies up to `max` cache entries into `out` (may be NULL); returns the count. */
+int64_t qdb_gpu_autotune_entries(QdbTuneEntry* out, int64_t max);
+void qdb_gpu_autotune_clear(void);
+/* Writes the cache as text; 0 or -1. */
+int qdb_gpu_autotune_save(const char* path);
+/* Adds the entries of a file written by qdb_gpu_autotune_save; returns how
+ * many were read, or -1. */
+int64_t qdb_gpu_autotune_load(const char* path);
+
+/* Batched join build (no counterpart in the reference, whose processors
+ * build one table per call): jr_dispatch(Build) of the RowArena / RowTable
+ * blobs arenas[i] / tables[i], for every i < n, all of `kernels` (one
+ * compiled join) and appended to. The dense tables' kernels and readbacks of
+ * every join are queued before ONE synchronization; a join whose keys the
+ * dense table does not take is built with the hash table as by its own
+ * Build. Returns 0, or -1 with the error text. */
+int qdb_gpu_join_build_batched(const QdbRowJoinKernels* kernels, void* const* arenas, void* const* tables, int64_t n);
+/* Order-independent checksum of n pairs at `pairs` (device, 2 i64 each):
+ * out[0] = sum and out[1] = xor of a 64-bit mix of every pair (no
+ * counterpart in the reference: it compares pair sets without moving them
+ * to the host). 0 or -1. */
+int qdb_gpu_pairs_checksum(const int64_t* pairs, int64_t n, uint64_t* out);
+/* Batched join probe (no counterpart in the reference): jr_dispatch(Probe)
+ * of batches[i] against the built join arenas[i] into the PairBuffer blob
+ * pairs[i], for every i < n, in ONE persistent launch whose workgroups take
+ * (join, row range) items from a device work queue (joins with one-word
+ * keys; two-word keys probe one join at a time). Pair buffers, their Count
+ * and the fused results (qdb_gpu_join_fused_results) are as after n
+ * separate probes; pairs[i] is materialized with batches[i]. Returns 0, or
+ * -1 with the error text. */
+int qdb_gpu_join_probe_batched(const QdbRowJoinKernels* kernels, void* const* arenas, const TRowSet* const* batches,
+                               void* const* pairs, int64_t n);
 /* Batched finish (no counterpart in the reference): agg_finish_rowset of
  * hts[i] into outs[i] for every i < n, every table's finalize launched
  * before ONE synchronization; rows[i] gets outs[i]'s row count and each
@@ -687,6 +743,12 @@ int64_t qdb_gpu_join_processor_probe(void* processor, const TRowSet* batch);
 /* Probes only; returns the pair count, pairs readable until the next probe. */
 int64_t qdb_gpu_join_processor_probe_only(void* processor, const TRowSet* batch);
 const QdbPairBuffer* qdb_gpu_join_processor_pairs(void* processor);
+/* Builds of processors[i] (all of one compiled join) with one
+ * synchronization (qdb_gpu_join_build_batched). 0 or -1. */
+int qdb_gpu_join_processor_build_batched(void* const* processors, int64_t n);
+/* probe_only of batches[i] by processors[i] (all of one compiled join) in
+ * one persistent launch (qdb_gpu_join_probe_batched). 0 or -1. */
+int qdb_gpu_join_processor_probe_batched(void* const* processors, const TRowSet* const* batches, int64_t n);
 /* The processor's RowArena blob (for qdb_gpu_join_fused_results). */
 void* qdb_gpu_join_processor_arena(void* processor);
 /* Next queued output batch: 1 when one was returned, 0 when none is left. */

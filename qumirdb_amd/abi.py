@@ -194,10 +194,16 @@ class QdbFusedResult(C.Structure):
     _fields_ = [("Lo", C.c_int64), ("Hi", C.c_int64), ("Count", C.c_int64), ("Valid", C.c_int64)]
 
 
+class QdbTuneEntry(C.Structure):
+    _fields_ = [("KeyWords", C.c_int32), ("CardBucket", C.c_int32), ("RowsBucket", C.c_int32), ("Path", C.c_int32),
+                ("LdsKiB", C.c_int32), ("GridPerCu", C.c_int32), ("FanoutShift", C.c_int32), ("Pad", C.c_int32),
+                ("Ms", C.c_double)]
+
+
 # Every struct the header declares, in the order qdb_gpu_struct_sizes reports.
 STRUCTS = [TColumn, TRowSet, QdbValueType, QdbExprInsn, QdbExprProgram, QdbColumnSchema, QdbFilterSpec,
            QdbProjectSpec, QdbAggregateFn, QdbAggregateSpec, QdbAggregateKernels, QdbRowJoinSpec,
-           QdbRowJoinKernels, QdbPairBuffer, QdbFusedResult]
+           QdbRowJoinKernels, QdbPairBuffer, QdbFusedResult, QdbTuneEntry]
 
 # Every extern "C" symbol include/qdb_gpu.h declares.
 EXPORTS = [
@@ -223,6 +229,10 @@ EXPORTS = [
     "qdb_gpu_partial_agg_add", "qdb_gpu_partial_agg_next", "qdb_gpu_partial_agg_finish", "qdb_gpu_partial_agg_passing_through",
     "qdb_gpu_partial_agg_free",
     "qdb_gpu_rf_new", "qdb_gpu_rf_build", "qdb_gpu_rf_publish", "qdb_gpu_rf_probe", "qdb_gpu_rf_stats", "qdb_gpu_rf_free",
+    "qdb_gpu_join_build_batched", "qdb_gpu_join_probe_batched", "qdb_gpu_join_processor_build_batched",
+    "qdb_gpu_join_processor_probe_batched", "qdb_gpu_pairs_checksum",
+    "qdb_gpu_autotune_enable", "qdb_gpu_autotune_entries", "qdb_gpu_autotune_clear", "qdb_gpu_autotune_save",
+    "qdb_gpu_autotune_load",
 ]
 
 
@@ -313,6 +323,17 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "qdb_gpu_rf_probe": (i64, [P, P, P, P, i64]),
         "qdb_gpu_rf_stats": (i64, [P, C.POINTER(i64), i64]),
         "qdb_gpu_rf_free": (None, [P]),
+        "qdb_gpu_join_build_batched": (C.c_int, [C.POINTER(QdbRowJoinKernels), C.POINTER(P), C.POINTER(P), i64]),
+        "qdb_gpu_join_probe_batched": (C.c_int, [C.POINTER(QdbRowJoinKernels), C.POINTER(P), C.POINTER(C.POINTER(TRowSet)),
+                                                 C.POINTER(P), i64]),
+        "qdb_gpu_join_processor_build_batched": (C.c_int, [C.POINTER(P), i64]),
+        "qdb_gpu_join_processor_probe_batched": (C.c_int, [C.POINTER(P), C.POINTER(C.POINTER(TRowSet)), i64]),
+        "qdb_gpu_pairs_checksum": (C.c_int, [P, i64, C.POINTER(C.c_uint64)]),
+        "qdb_gpu_autotune_enable": (C.c_int, [C.c_int]),
+        "qdb_gpu_autotune_entries": (i64, [C.POINTER(QdbTuneEntry), i64]),
+        "qdb_gpu_autotune_clear": (None, []),
+        "qdb_gpu_autotune_save": (C.c_int, [C.c_char_p]),
+        "qdb_gpu_autotune_load": (i64, [C.c_char_p]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

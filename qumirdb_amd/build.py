@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libqdb_gpu.so")
 
 SOURCES = ["runtime.hip", "filter_project.hip", "runtime_filter.hip", "shuffle.hip", "sort.hip", "staging.hip", "aggregate.hip", "aggregate_kw1.hip", "aggregate_kw2.hip",
-           "aggregate_kw3.hip", "aggregate_kw4.hip", "join.hip", "primitives.hip", "exec.cpp", "jit.cpp"]
+           "aggregate_kw3.hip", "aggregate_kw4.hip", "join.hip", "primitives.hip", "exec.cpp", "jit.cpp", "autotune.cpp"]
 
 # Device headers built into the library for kernels specialized at run time
 # (name seen by the generated source's #include, path in the tree).

@@ -31,9 +31,11 @@
 #include <mutex>
 #include <utility>
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include "aggregate_kernels.h"
+#include "autotune.h"
 #include "jit.h"
 #include "slots.h"
 
@@ -1329,13 +1331,14 @@ bool set_counters(AggTable* T)
     return hip_ok(hipGetLastError(), "counters");
 }
 
-// LDS table of an update launch: as many groups as fit in 64 KiB, a power
-// of two (none for i128 MIN/MAX, which live in HBM only).
-void lds_layout(const AggPlan* P, AggArgs* A, size_t* lds_bytes)
+// LDS table of an update launch: as many groups as fit in lds_kib KiB, a
+// power of two (none for i128 MIN/MAX, which live in HBM only).
+void lds_layout(const AggPlan* P, AggArgs* A, size_t* lds_bytes, int lds_kib = 64)
 {
     const int R = P->KW + A->nwords;
-    int lds_groups = 1024;
-    while (lds_groups > 16 && (size_t)lds_groups * R * 8 + (size_t)lds_groups * 2 * 4 + 16 > 64 * 1024) lds_groups /= 2;
+    const size_t budget = (size_t)lds_kib * 1024;
+    int lds_groups = 4096;
+    while (lds_groups > 16 && (size_t)lds_groups * R * 8 + (size_t)lds_groups * 2 * 4 + 16 > budget) lds_groups /= 2;
     if (P->has_pairs) lds_groups = 0;
     A->lds_groups = lds_groups;
     A->lds_slots = lds_groups * 2;
@@ -1508,7 +1511,7 @@ bool sorted_path_wanted(AggPlan* P, const AggTable* T, int64_t rows)
 // The bucketed update of rows [0, n) of a bound batch (n <= 2^30): stage,
 // sort by bucket, aggregate every bucket in LDS into records, grow the
 // table for them, merge them into the HBM table.
-bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n)
+bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n, int fanout_shift = 0)
 {
     AggPlan* P = T->plan;
     // An empty table whose buffers are not made yet keeps the groups of the
@@ -1544,6 +1547,8 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n)
     const uint64_t expect = (uint64_t)n < T->capacity / 2 ? (uint64_t)n : T->capacity / 2;
     int bits = 8;
     while (bits < 15 && ((uint64_t)1 << bits) * (uint64_t)(g * 6 / 10) < expect) ++bits;
+    bits += fanout_shift;
+    bits = bits < 6 ? 6 : bits > 15 ? 15 : bits;
     const uint32_t B = 1u << bits;
     hipFunction_t f_sorted = plan_jit_function(P, "qdb_agg_sorted");
     hipFunction_t f_merge = plan_jit_function(P, "qdb_agg_merge");
@@ -1751,25 +1756,44 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n)
     return ok;
 }
 
-// agg_dispatch(Update): every selected row of the batch, exactly once.
-bool table_update(AggTable* T, const TRowSet* batch)
+// Whether the bucketed path can take a batch of `rows` rows of this plan.
+bool bucketed_possible(AggPlan* P, int64_t rows)
+{
+    return rows <= ((int64_t)1 << 30) && stage_layout(P).words > 0 && plan_jit_function(P, "qdb_agg_sorted") != nullptr;
+}
+
+// The configuration of an update when nothing is tuned: the bucketed path
+// for large batches into large tables, else LDS tables of 64 KiB at two
+// workgroups per CU.
+TuneConfig default_config(AggPlan* P, const AggTable* T, int64_t rows)
+{
+    TuneConfig c = {0, 64, 2, 0};
+    if (rows <= ((int64_t)1 << 30) && sorted_path_wanted(P, T, rows)) c.path = 1;
+    return c;
+}
+
+TuneConfig choose_config(AggTable* T, const TRowSet* batch);
+
+// agg_dispatch(Update) in configuration `cfg`.
+bool table_update_cfg(AggTable* T, const TRowSet* batch, const TuneConfig& cfg)
 {
     AggPlan* P = T->plan;
     if (batch->RowCount <= 0) return true;
-    if (batch->RowCount <= ((int64_t)1 << 30) && sorted_path_wanted(P, T, batch->RowCount)) {
+    if (cfg.path == 1 && bucketed_possible(P, batch->RowCount)) {
         AggArgs bound;
         if (!bind_batch(P, batch, &bound)) return false;
-        return table_update_sorted(T, bound, batch->RowCount);
+        return table_update_sorted(T, bound, batch->RowCount, cfg.fanout_shift);
     }
     AggArgs bound;
     if (!bind_batch(P, batch, &bound)) return false;
     const int KW = P->KW, S = P->S;
     size_t lds = 0;
-    lds_layout(P, &bound, &lds);
+    lds_layout(P, &bound, &lds, cfg.lds_kib);
+    const int per_cu = cfg.grid_per_cu >= 1 ? cfg.grid_per_cu : 2;
     const int64_t chunk = (int64_t)1 << 30;
     for (int64_t base = 0; base < batch->RowCount; base += chunk) {
         const int64_t n = batch->RowCount - base < chunk ? batch->RowCount - base : chunk;
-        const int grid = grid_for(n, kBlock, 2);
+        const int grid = grid_for(n, kBlock, per_cu);
         UpdateLaunch L;
         if (!prepare_launch(T, bound, base, n, (size_t)grid * (bound.lds_groups ? bound.lds_groups : 1) + 1024, &L)) return false;
         if (!set_counters(T)) {
@@ -1796,6 +1820,14 @@ bool table_update(AggTable* T, const TRowSet* batch)
     return true;
 }
 
+// agg_dispatch(Update): every selected row of the batch, exactly once, in
+// the configuration chosen for its shape.
+bool table_update(AggTable* T, const TRowSet* batch)
+{
+    if (batch->RowCount <= 0) return true;
+    return table_update_cfg(T, batch, choose_config(T, batch));
+}
+
 // Rows per work item of a batched update.
 constexpr int64_t kBatchItemRows = 32768;
 
@@ -1815,14 +1847,19 @@ bool batched_update(AggPlan* P, AggTable* const* tables, const TRowSet* const* b
     bool ok = true;
     // One instance (after those updated on their own) gains nothing from
     // the queue: it takes the plain update.
+    // Each instance's configuration (tuned per shape when tuning is on):
+    // the bucketed ones, and batches above 2^30 rows, update on their own.
+    std::vector<TuneConfig> cfgs((size_t)n);
     int64_t queued = 0;
     for (int64_t i = 0; i < n; ++i) {
         const int64_t rows = batches[i]->RowCount;
-        queued += rows > 0 && rows <= ((int64_t)1 << 30) && !sorted_path_wanted(P, tables[i], rows);
+        if (rows <= 0) continue;
+        cfgs[(size_t)i] = choose_config(tables[i], batches[i]);
+        queued += rows <= ((int64_t)1 << 30) && cfgs[(size_t)i].path == 0;
     }
     if (queued <= 1) {
         for (int64_t i = 0; i < n && ok; ++i) {
-            if (batches[i]->RowCount > 0) ok = table_update(tables[i], batches[i]);
+            if (batches[i]->RowCount > 0) ok = table_update_cfg(tables[i], batches[i], cfgs[(size_t)i]);
         }
         return ok;
     }
@@ -1833,10 +1870,10 @@ bool batched_update(AggPlan* P, AggTable* const* tables, const TRowSet* const* b
         AggTable* T = tables[i];
         const TRowSet* b = batches[i];
         if (b->RowCount <= 0) continue;
-        // Very large batches, and large ones into large tables (the bucketed
-        // path), are updated on their own.
-        if (b->RowCount > ((int64_t)1 << 30) || sorted_path_wanted(P, T, b->RowCount)) {
-            ok = table_update(T, b);
+        // Very large batches, and those the bucketed path takes, are
+        // updated on their own.
+        if (b->RowCount > ((int64_t)1 << 30) || cfgs[(size_t)i].path != 0) {
+            ok = table_update_cfg(T, b, cfgs[(size_t)i]);
             continue;
         }
         AggArgs bound;
@@ -2027,6 +2064,70 @@ bool table_finish_done(AggTable* T, TRowSet* out, FinishTemps* tmp)
     tmp->release();
     if (T->plan->spec.OutputOnHost && !move_rowset_to_host(out, T->plan->out_types)) return false;
     return publish_rowset(out);
+}
+
+// Host milliseconds of one query in configuration `cfg` on a scratch table
+// of T's size: set-up, the update of `batch` and the finish into HBM (the
+// paths leave different work to the finish); < 0 when it fails.
+double time_config(AggTable* T, const TRowSet* batch, const TuneConfig& cfg)
+{
+    AggPlan* P = T->plan;
+    if (!sync("tune")) return -1.0;
+    const auto t0 = std::chrono::steady_clock::now();
+    AggTable* X = table_new(P, (int64_t)(T->cap_hint > T->capacity ? T->cap_hint : T->capacity));
+    if (!X) return -1.0;
+    bool ok = table_seed(X) && table_update_cfg(X, batch, cfg);
+    TRowSet out;
+    memset(&out, 0, sizeof(out));
+    FinishTemps tmp;
+    if (ok) ok = table_finish_launch(X, &out, &tmp) >= 0;
+    ok = sync("tune") && ok;
+    tmp.release();
+    destroy_kernel_rowset(&out);
+    table_free(X);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return ok ? ms : -1.0;
+}
+
+// The autotuner: the configuration of an update of `batch` into T. A forced
+// configuration (QDB_GPU_TUNE_FORCE) wins; else the one cached for the
+// shape bucket (key width, table slots, rows); else, with tuning on, every
+// candidate is timed twice on a scratch table (the second time counts: the
+// first may include one-time costs) and the fastest is cached; else the
+// sizing rule's default.
+TuneConfig choose_config(AggTable* T, const TRowSet* batch)
+{
+    AggPlan* P = T->plan;
+    const int64_t rows = batch->RowCount;
+    const bool bucketed_ok = bucketed_possible(P, rows);
+    TuneConfig cfg;
+    if (tune_forced(&cfg)) {
+        if (cfg.path == 1 && !bucketed_ok) cfg = TuneConfig{0, 64, 2, 0};
+        return cfg;
+    }
+    const uint64_t slots = T->cap_hint > T->capacity ? T->cap_hint : T->capacity;
+    const TuneKey key = tune_key(P->KW, slots, rows);
+    if (tune_lookup(key, &cfg)) {
+        if (cfg.path == 1 && !bucketed_ok) cfg = TuneConfig{0, 64, 2, 0};
+        return cfg;
+    }
+    if (!tune_enabled() || rows <= 0) return default_config(P, T, rows);
+    TuneConfig best = default_config(P, T, rows);
+    double best_ms = -1.0;
+    for (const TuneConfig& c : tune_candidates(bucketed_ok)) {
+        double ms = time_config(T, batch, c);
+        if (ms >= 0) ms = time_config(T, batch, c);
+        if (ms < 0) {
+            clear_error();
+            continue;
+        }
+        if (best_ms < 0 || ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    if (best_ms >= 0) tune_store(key, best, best_ms);
+    return best;
 }
 
 // agg_finish_rowset: groups -> columnar TRowSet; destroys the table.

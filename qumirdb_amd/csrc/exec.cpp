@@ -282,6 +282,41 @@ void TJoinDriver::Seal()
     Sealed_ = true;
 }
 
+void TJoinDriver::SealMany(TJoinDriver* const* drivers, int64_t n)
+{
+    std::vector<void*> arenas, tables;
+    std::vector<TJoinDriver*> open;
+    for (int64_t i = 0; i < n; ++i) {
+        TJoinDriver* d = drivers[i];
+        if (d->Kernels_.Slot != drivers[0]->Kernels_.Slot) throw std::invalid_argument("batched drivers of different joins");
+        if (d->Sealed_) continue;
+        arenas.push_back(d->Arena_.data());
+        tables.push_back(d->Table_.data());
+        open.push_back(d);
+    }
+    if (open.empty()) return;
+    if (qdb_gpu_join_build_batched(&open[0]->Kernels_, arenas.data(), tables.data(), (int64_t)open.size()) != 0) {
+        throw TDriverError("batched row join table build failed");
+    }
+    for (TJoinDriver* d : open) d->Sealed_ = true;
+}
+
+void TJoinDriver::ProbeMany(TJoinDriver* const* drivers, const TRowSet* const* batches, int64_t n)
+{
+    if (n <= 0) return;
+    std::vector<void*> arenas((size_t)n), pairs((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        TJoinDriver* d = drivers[i];
+        if (d->Kernels_.Slot != drivers[0]->Kernels_.Slot) throw std::invalid_argument("batched drivers of different joins");
+        if (!d->Sealed_) throw std::logic_error("row join probe before build");
+        arenas[(size_t)i] = d->Arena_.data();
+        pairs[(size_t)i] = &d->Pairs_;
+    }
+    if (qdb_gpu_join_probe_batched(&drivers[0]->Kernels_, arenas.data(), batches, pairs.data(), n) != 0) {
+        throw TDriverError("batched row join probe failed");
+    }
+}
+
 int64_t TJoinDriver::Probe(const TRowSet& batch, bool materialize)
 {
     if (!Sealed_) throw std::logic_error("row join probe before build");
@@ -426,6 +461,16 @@ int64_t qdb_gpu_join_processor_probe_only(void* p, const TRowSet* batch)
     int64_t pairs = -1;
     if (Guard([&] { pairs = static_cast<TJoinDriver*>(p)->Probe(*batch, false); }) != 0) return -1;
     return pairs;
+}
+
+int qdb_gpu_join_processor_build_batched(void* const* procs, int64_t n)
+{
+    return Guard([&] { TJoinDriver::SealMany(reinterpret_cast<TJoinDriver* const*>(procs), n); });
+}
+
+int qdb_gpu_join_processor_probe_batched(void* const* procs, const TRowSet* const* batches, int64_t n)
+{
+    return Guard([&] { TJoinDriver::ProbeMany(reinterpret_cast<TJoinDriver* const*>(procs), batches, n); });
 }
 
 const QdbPairBuffer* qdb_gpu_join_processor_pairs(void* p)

@@ -144,6 +144,13 @@ public:
     const QdbPairBuffer& Pairs() const { return Pairs_; }
     void* Arena() { return Arena_.data(); }
 
+    // Seal() of every driver (one compiled join) with one synchronisation
+    // for all their dense tables (qdb_gpu_join_build_batched).
+    static void SealMany(TJoinDriver* const* drivers, int64_t n);
+    // Probe(batches[i], false) of every driver in one persistent launch
+    // (qdb_gpu_join_probe_batched); each driver's pairs stay readable.
+    static void ProbeMany(TJoinDriver* const* drivers, const TRowSet* const* batches, int64_t n);
+
 private:
     bool Op(int64_t op, const TRowSet* batch);
 

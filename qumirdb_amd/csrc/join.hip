@@ -932,18 +932,18 @@ __device__ __forceinline__ unsigned long long reserve_block_tile(unsigned long l
 // (row i = tile start + r * 64 + lane, r-major). A key matches at most one
 // build row, so a row's output is at most one pair (or its unmatched pair in
 // an outer join).
+//
+// probe_dense_tiles runs the block tiles starting at b_begin, b_begin +
+// b_step, ... below b_end (every thread of the block calls it with the same
+// range) and adds the block's fused words to the totals at the end; lbits
+// holds the first lds_range bits of the membership bitmap in LDS (with
+// lds_range 0 every key reads the ordinal table, and lbits must still point
+// at one readable LDS word).
 template <int kMode>
-__global__ void __launch_bounds__(kProbeBlock) k_probe_dense(ProbeArgs P)
+__device__ __forceinline__ void probe_dense_tiles(const ProbeArgs& P, const uint32_t* lbits, uint64_t lds_range, int64_t b_begin,
+    int64_t b_end, int64_t b_step)
 {
     constexpr bool kDirect = kMode != kGeneric;
-    extern __shared__ __align__(16) uint64_t lds64[];
-    const uint32_t* lbits = (const uint32_t*)lds64;
-    {
-        uint32_t* b = (uint32_t*)lds64;
-        fill_lds_words(b, P.dbits, P.lds_words);
-    }
-    __syncthreads();
-    const uint64_t lds_range = (uint64_t)P.lds_words * 32u;
     const int lane = lane_id();
     const int wave = threadIdx.x / kWave;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -959,9 +959,8 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe_dense(ProbeArgs P)
     unsigned long long v_pairs = 0;
     uint64_t v_lo = 0, v_hi = 0;
     int64_t v_min = 0x7fffffffffffffffll, v_max = (int64_t)0x8000000000000000ll;
-    constexpr int64_t kBlockTile = (int64_t)(kProbeBlock / kWave) * kWaveTile;
     // Every thread of the block runs the same number of block tiles.
-    for (int64_t b0 = blockIdx.x * kBlockTile; b0 < P.n; b0 += (int64_t)gridDim.x * kBlockTile) {
+    for (int64_t b0 = b_begin; b0 < b_end; b0 += b_step) {
         const int64_t t0 = b0 + (int64_t)wave * kWaveTile;
         // Round 1: keys and the fused aggregates' probe column, unconditional
         // loads at clamped rows (the column is streamed with the keys: a
@@ -1077,6 +1076,18 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe_dense(ProbeArgs P)
         }
     }
     if (P.nfused) fused_flush<kAcc, kMode == kDirectVal>(P, acc, v_pairs, v_lo, v_hi, v_min, v_max, lane);
+}
+
+constexpr int64_t kProbeBlockTile = (int64_t)(kProbeBlock / kWave) * kWaveTile;
+
+template <int kMode>
+__global__ void __launch_bounds__(kProbeBlock) k_probe_dense(ProbeArgs P)
+{
+    extern __shared__ __align__(16) uint64_t lds64[];
+    fill_lds_words((uint32_t*)lds64, P.dbits, P.lds_words);
+    __syncthreads();
+    probe_dense_tiles<kMode>(P, (const uint32_t*)lds64, (uint64_t)P.lds_words * 32u, (int64_t)blockIdx.x * kProbeBlockTile, P.n,
+        (int64_t)gridDim.x * kProbeBlockTile);
 }
 
 __device__ __forceinline__ uint64_t uniform_u64(uint64_t x)
@@ -1265,22 +1276,16 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe_dense_lean(ProbeArgs P)
     }
 }
 
+// probe_hash_tiles runs the block tiles b_begin, b_begin + b_step, ...
+// below b_end over the slot table T (in LDS for kTabLds), behind the Bloom
+// filter words in LDS at lds_bloom when P.bloom_words is not 0, and adds the
+// block's fused words to the totals at the end.
 template <int KW, int kTab, int kMode>
-__global__ void __launch_bounds__(kProbeBlock) k_probe(ProbeArgs P)
+__device__ __forceinline__ void probe_hash_tiles(const ProbeArgs& P, const Slot<KW>* T, const uint32_t* lds_bloom, int64_t b_begin,
+    int64_t b_end, int64_t b_step)
 {
     constexpr bool kDirect = kMode != kGeneric;
     constexpr bool kLds = kTab == kTabLds;
-    extern __shared__ __align__(16) uint64_t lds64[];
-    const Slot<KW>* T = kLds ? (const Slot<KW>*)lds64 : (const Slot<KW>*)P.table;
-    const uint32_t* lds_bloom = (const uint32_t*)lds64;
-    if (kLds) {
-        const uint64_t words = (P.cmask + 1) * (sizeof(Slot<KW>) / 8);
-        fill_lds_words((uint32_t*)lds64, (const uint32_t*)P.table, words * 2);
-    } else if (P.bloom_words) {
-        uint32_t* b = (uint32_t*)lds64;
-        fill_lds_words(b, P.bloom, P.bloom_words);
-    }
-    __syncthreads();
     const int lane = lane_id();
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     // Fused state per lane. kDirectVal: the pair count and the sum, min and
@@ -1299,10 +1304,9 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe(ProbeArgs P)
     uint64_t v_lo = 0, v_hi = 0;
     int64_t v_min = 0x7fffffffffffffffll, v_max = (int64_t)0x8000000000000000ll;
     const int wave = threadIdx.x / kWave;
-    constexpr int64_t kBlockTile = (int64_t)(kProbeBlock / kWave) * kWaveTile;
     // Every thread of the block runs the same number of block tiles (two
     // barriers per tile place the block's pairs with one atomic).
-    for (int64_t b0 = blockIdx.x * kBlockTile; b0 < P.n; b0 += (int64_t)gridDim.x * kBlockTile) {
+    for (int64_t b0 = b_begin; b0 < b_end; b0 += b_step) {
         const int64_t t0 = b0 + (int64_t)wave * kWaveTile;
         // Per row of the tile: match count and slot head; and whether the
         // row is selected.
@@ -1458,6 +1462,105 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe(ProbeArgs P)
         }
     }
     if (P.nfused) fused_flush<kAcc, kMode == kDirectVal>(P, acc, v_pairs, v_lo, v_hi, v_min, v_max, lane);
+}
+
+template <int KW, int kTab, int kMode>
+__global__ void __launch_bounds__(kProbeBlock) k_probe(ProbeArgs P)
+{
+    constexpr bool kLds = kTab == kTabLds;
+    extern __shared__ __align__(16) uint64_t lds64[];
+    if (kLds) {
+        const uint64_t words = (P.cmask + 1) * (sizeof(Slot<KW>) / 8);
+        fill_lds_words((uint32_t*)lds64, (const uint32_t*)P.table, words * 2);
+    } else if (P.bloom_words) {
+        fill_lds_words((uint32_t*)lds64, P.bloom, P.bloom_words);
+    }
+    __syncthreads();
+    const Slot<KW>* T = kLds ? (const Slot<KW>*)lds64 : (const Slot<KW>*)P.table;
+    probe_hash_tiles<KW, kTab, kMode>(P, T, (const uint32_t*)lds64, (int64_t)blockIdx.x * kProbeBlockTile, P.n,
+        (int64_t)gridDim.x * kProbeBlockTile);
+}
+
+// Batched probe: many joins' probes (of one compiled join, one-word keys) in
+// ONE persistent launch. Each workgroup takes items (instance, block tile
+// range) from a device queue and probes that range of the instance's batch
+// with its table: the dense table reading the ordinal table for every key
+// (no bitmap in LDS), or the hash table without the Bloom filter (nothing is
+// staged into LDS per item). Pairs and fused words go to each instance's own
+// pair buffer and counters. Every workgroup leaves the loop when the queue
+// is drained.
+struct ProbeItem {
+    int32_t inst;
+    int32_t pad;
+    int64_t b_begin;   // first row of the item (a multiple of the block tile)
+    int64_t b_end;     // one past its last block tile's first row
+};
+
+template <int kMode>
+__global__ void __launch_bounds__(kProbeBlock) k_probe_batched(const ProbeArgs* __restrict__ insts,
+    const ProbeItem* __restrict__ items, int nitems, unsigned int* head)
+{
+    __shared__ int s_item;
+    __shared__ uint32_t no_bits[4];
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(head, 1u);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= nitems) break;
+        const ProbeItem w = items[it];
+        const ProbeArgs& P = insts[w.inst];
+        if (P.dtab) probe_dense_tiles<kMode>(P, no_bits, 0, w.b_begin, w.b_end, kProbeBlockTile);
+        else probe_hash_tiles<1, kTabHash, kMode>(P, (const Slot<1>*)P.table, no_bits, w.b_begin, w.b_end, kProbeBlockTile);
+    }
+}
+
+// Every instance's pair counter and fused words at their identities.
+__global__ void k_probe_init_batched(const ProbeArgs* __restrict__ insts, int n, int words)
+{
+    for (int i = blockIdx.x; i < n; i += gridDim.x) probe_counters_init(insts[i].pair_count, insts[i].fused_acc, words, (int)threadIdx.x);
+}
+
+// Every instance's pair count and fused words into out[i * stride ...].
+__global__ void k_probe_collect_batched(const ProbeArgs* __restrict__ insts, int n, int words, unsigned long long* out)
+{
+    const int stride = 1 + words;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)n * stride; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t / stride), k = (int)(t % stride);
+        out[t] = k == 0 ? insts[i].pair_count[0] : insts[i].fused_acc[k - 1];
+    }
+}
+
+// Order-independent checksum of n pairs: the sum and the xor of a 64-bit
+// mix of every pair (out[0], out[1], added to), so that two pair sets can be
+// compared without moving them off the device.
+__device__ __forceinline__ uint64_t pair_mix(uint64_t x)
+{
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+
+__global__ void __launch_bounds__(kBlock) k_pairs_checksum(const int64_t* pairs, int64_t n, unsigned long long* out)
+{
+    unsigned long long sum = 0, x = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t h = pair_mix((uint64_t)pairs[2 * i] ^ pair_mix((uint64_t)pairs[2 * i + 1] + 0x9e3779b97f4a7c15ull));
+        sum += h;
+        x ^= h;
+    }
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        sum += shfl_xor_u64(sum, o);
+        x ^= shfl_xor_u64(x, o);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(&out[0], sum);
+        atomicXor(&out[1], x);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1954,34 +2057,39 @@ bool read_rows(JoinBuild* J)
     return true;
 }
 
+// Frees the dense table's buffers.
+void dense_drop(JoinBuild* J)
+{
+    dfree(J->dbits);
+    dfree(J->dtab);
+    J->dbits = nullptr;
+    J->dtab = nullptr;
+}
+
 // The widest key range the dense table takes for `rows` build rows.
 __host__ __device__ inline uint64_t dense_limit(uint64_t rows)
 {
     return rows * 8 > (1ull << 16) ? rows * 8 : (1ull << 16);
 }
 
-bool try_dense_build(JoinBuild* J)
+// The dense table's first half: its buffers (sized for the bound on the
+// rows: the appends kept the key range on the device, the kernels read it
+// there), its kernels, and the readback of the row count, the range and the
+// flags (repeated key, range too wide), queued on the stream. False (nothing
+// queued, no error) when the dense table cannot serve.
+bool dense_build_launch(JoinBuild* J)
 {
-    // Sized for the bound on the rows: the appends kept the key range on the
-    // device, the kernels read it there, and the row count, the range and
-    // the flags (repeated key, range too wide) come back in one readback.
     const int64_t bound = J->rows_known ? J->rows : J->rows_bound;
     const uint64_t limit_cap = dense_limit((uint64_t)bound);
     if (limit_cap >= (1ull << 32)) return false;
     const size_t words_cap = (size_t)((limit_cap + 31) / 32);
     J->dbits = (uint32_t*)dalloc(words_cap * 4);
     J->dtab = (uint32_t*)dalloc((size_t)limit_cap * 4);
-    auto drop = [&]() {
-        dfree(J->dbits);
-        dfree(J->dtab);
-        J->dbits = nullptr;
-        J->dtab = nullptr;
-    };
     if (!J->dbits || !J->dtab) {
-        drop();
+        dense_drop(J);
+        clear_error();
         return false;
     }
-    unsigned long long* h = J->readback;
     {
         TimedRegion timer("join_build");
         hipLaunchKernelGGL(k_dense_prepare, dim3(grid_for((int64_t)limit_cap, kBlock, 8)), dim3(kBlock), 0, stream(), J->rows_dev,
@@ -1990,8 +2098,21 @@ bool try_dense_build(JoinBuild* J)
             J->rows_dev, J->dbits, J->dtab, J->pair_count, J->fused_acc, J->plan->spec.NumFused * kAccWords);
     }
     J->fused_ready = false;
-    const bool ok = hip_ok(hipGetLastError(), "dense fill") &&
-        hip_ok(hipMemcpyAsync(h, J->rows_dev, kArenaWords * 8, hipMemcpyDeviceToHost, stream()), "dense info") && sync("dense fill");
+    if (!hip_ok(hipGetLastError(), "dense fill") ||
+        !hip_ok(hipMemcpyAsync(J->readback, J->rows_dev, kArenaWords * 8, hipMemcpyDeviceToHost, stream()), "dense info")) {
+        dense_drop(J);
+        clear_error();
+        return false;
+    }
+    return true;
+}
+
+// The dense table's second half, once the stream has synchronised (`ok`
+// false: it failed): the readback decides. False (buffers dropped, no
+// error) when the hash table must serve.
+bool dense_build_finish(JoinBuild* J, bool ok)
+{
+    const unsigned long long* h = J->readback;
     if (ok) {
         J->rows = (int64_t)h[0];
         J->rows_known = true;
@@ -2004,7 +2125,7 @@ bool try_dense_build(JoinBuild* J)
     // Repeated keys or a key range too wide: the hash table serves.
     if (!ok || J->rows <= 0 || h_flags[0] || h_flags[1] || J->kmax < J->kmin || range == 0 ||
         range > dense_limit((uint64_t)J->rows)) {
-        drop();
+        dense_drop(J);
         clear_error();
         return false;
     }
@@ -2018,13 +2139,27 @@ bool try_dense_build(JoinBuild* J)
     return true;
 }
 
+bool try_dense_build(JoinBuild* J)
+{
+    if (!dense_build_launch(J)) return false;
+    return dense_build_finish(J, sync("dense fill"));
+}
+
+// Whether J's build tries the dense table first.
+bool dense_wanted(const JoinBuild* J)
+{
+    const int64_t bound = J->rows_known ? J->rows : J->rows_bound;
+    return J->plan->KW == 1 && J->plan->dense_ok && bound > 0 && bound < (int64_t)0x7fffffff;
+}
+
+// Builds J's table: the dense table when it serves (unless dense_tried:
+// the dense attempt was already made), else the hash table.
 template <int KW>
-bool do_build(JoinBuild* J)
+bool do_build(JoinBuild* J, bool dense_tried = false)
 {
     JoinPlan* P = J->plan;
     if (J->appended > (int64_t)0xffffffff) return fail("join builds are limited to 2^32 appended rows");
-    const int64_t bound = J->rows_known ? J->rows : J->rows_bound;
-    if (KW == 1 && P->dense_ok && bound > 0 && bound < (int64_t)0x7fffffff && try_dense_build(J)) return true;
+    if (!dense_tried && dense_wanted(J) && try_dense_build(J)) return true;
     if (!read_rows(J)) return false;
     const int64_t rows = J->rows;
     if (rows >= (int64_t)0x7fffffff) return fail("join builds are limited to 2^31 rows");
@@ -2342,21 +2477,13 @@ __global__ void k_probe_init(unsigned long long* pair_count, unsigned long long*
     probe_counters_init(pair_count, acc, words, (int)threadIdx.x);
 }
 
+// The probe's arguments for one batch against J's table (everything but
+// the LDS staging, the pair buffer and the counters) and the kernel mode.
 template <int KW>
-// Probes one batch. With `scratch_part` (the allocation-free variant) no
-// pair is written and nothing is allocated: the fused aggregates' partials
-// go to that device scratch, the pair count to *pairs_seen.
-bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs, unsigned long long* scratch_part = nullptr,
-    unsigned long long* pairs_seen = nullptr)
+bool probe_args_of(JoinBuild* J, const TRowSet* batch, ProbeArgs& A, int* mode_out)
 {
     JoinPlan* P = J->plan;
     const int64_t n = batch->RowCount;
-    pairs->Count = 0;
-    J->last_probe = *batch;
-    J->fused_ready = false;
-    if (n <= 0) return true;
-    if (n >= (int64_t)1 << 32) return fail("probe batches are limited to 2^32 rows");
-    ProbeArgs A;
     memset(&A, 0, sizeof(A));
     if (!bind_side(P->probe_slot_col, P->probe_nslots, P->spec.Probe, batch, &A.cols)) return false;
     A.K = P->probe_key;
@@ -2409,6 +2536,53 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs, unsigned
             }
         }
     }
+    *mode_out = mode;
+    return true;
+}
+
+// Pairs a probe of n rows against J can emit, and the capacity its first
+// attempt allocates: a probe row has at most max_key_rows matches (one pair
+// when unmatched in an outer join), and the first attempt takes that bound
+// up to kFirstPairBudget pairs (the kernels count every pair whatever the
+// capacity, so a larger result is probed again at its exact size).
+unsigned long long first_pair_capacity(const JoinBuild* J, int64_t n)
+{
+    const bool outer = J->plan->spec.JoinType != QDB_JOIN_INNER;
+    const u128 bound = (u128)(uint64_t)n * (u128)(uint64_t)(J->max_key_rows > 1 ? J->max_key_rows : 1)
+        + (outer ? (u128)(uint64_t)n : (u128)0);
+    const unsigned long long budget = (unsigned long long)n > kFirstPairBudget ? (unsigned long long)n : kFirstPairBudget;
+    return bound < (u128)budget ? (unsigned long long)bound : budget;
+}
+
+// A pair buffer of at least `cap` pairs.
+bool reserve_pairs(QdbPairBuffer* pairs, unsigned long long cap)
+{
+    if ((unsigned long long)pairs->Capacity >= cap) return true;
+    if (cap > (unsigned long long)(SIZE_MAX / 16)) return fail("probe: %llu pairs do not fit in memory", cap);
+    dfree(pairs->Data);
+    pairs->Capacity = 0;
+    pairs->Data = (int64_t*)dalloc((size_t)cap * 16);
+    if (!pairs->Data) return false;
+    pairs->Capacity = (int64_t)cap;
+    return true;
+}
+
+template <int KW>
+// Probes one batch. With `scratch_part` (the allocation-free variant) no
+// pair is written and nothing is allocated: the fused aggregates' partials
+// go to that device scratch, the pair count to *pairs_seen.
+bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs, unsigned long long* scratch_part = nullptr,
+    unsigned long long* pairs_seen = nullptr)
+{
+    const int64_t n = batch->RowCount;
+    pairs->Count = 0;
+    J->last_probe = *batch;
+    J->fused_ready = false;
+    if (n <= 0) return true;
+    if (n >= (int64_t)1 << 32) return fail("probe batches are limited to 2^32 rows");
+    ProbeArgs A;
+    int mode = kGeneric;
+    if (!probe_args_of<KW>(J, batch, A, &mode)) return false;
     // The dense probe's lean form: direct key, no probe selection, inner
     // join, fused aggregates that count pairs or read the one direct probe
     // column (QDB_GPU_JOIN_LEAN=0 keeps the general form).
@@ -2438,24 +2612,9 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs, unsigned
         A.dtab = J->dtab;
         lds = (size_t)A.lds_words * 4;
     }
-    // Pair capacity: a probe row has at most max_key_rows matches (one
-    // pair when unmatched in an outer join). The first attempt takes that
-    // bound up to kFirstPairBudget pairs; the kernels count every pair
-    // whatever the capacity, so a larger result is probed again into a
-    // buffer of its exact size.
-    const u128 bound = (u128)(uint64_t)n * (u128)(uint64_t)(J->max_key_rows > 1 ? J->max_key_rows : 1)
-        + (A.emit_unmatched ? (u128)(uint64_t)n : (u128)0);
-    const unsigned long long budget = (unsigned long long)n > kFirstPairBudget ? (unsigned long long)n : kFirstPairBudget;
-    unsigned long long cap = bound < (u128)budget ? (unsigned long long)bound : budget;
+    unsigned long long cap = first_pair_capacity(J, n);
     for (int attempt = 0; attempt < 2; ++attempt) {
-        if (!scratch_part && (unsigned long long)pairs->Capacity < cap) {
-            if (cap > (unsigned long long)(SIZE_MAX / 16)) return fail("probe: %llu pairs do not fit in memory", cap);
-            dfree(pairs->Data);
-            pairs->Capacity = 0;
-            pairs->Data = (int64_t*)dalloc((size_t)cap * 16);
-            if (!pairs->Data) return false;
-            pairs->Capacity = (int64_t)cap;
-        }
+        if (!scratch_part && !reserve_pairs(pairs, cap)) return false;
         A.pairs = scratch_part ? nullptr : pairs->Data;
         A.pair_cap = scratch_part ? 0ull : (unsigned long long)pairs->Capacity;
         A.pair_count = J->pair_count;
@@ -2589,6 +2748,189 @@ bool collect_fused(JoinBuild* J)
     return true;
 }
 
+// The blobs' fields after a Build.
+void publish_build(RowArenaBlob* AB, RowTableBlob* TB, const JoinBuild* J)
+{
+    AB->Rows = J->rows;
+    TB->entries = J->table;
+    TB->Capacity = (int64_t)J->capacity;
+    TB->Mask = J->capacity - 1;
+    int shift = 64;
+    for (uint64_t c = J->capacity; c > 1; c >>= 1) --shift;
+    TB->Shift = shift;
+    TB->Rows = J->rows;
+    TB->Keys = J->keys_distinct;
+}
+
+void register_pairs(const void* pairs, JoinBuild* J)
+{
+    std::lock_guard<std::mutex> lock(g_pairs_mu);
+    pair_owner()[pairs] = J;
+}
+
+// Build of every join arenas[i] (one compiled join): the dense tables'
+// kernels and readbacks of all of them are queued before ONE
+// synchronisation; a join whose keys the dense table does not take gets
+// the hash table as by its own Build.
+bool batched_build(JoinPlan* P, void* const* arenas, void* const* tables, int64_t n)
+{
+    std::vector<char> queued((size_t)n, 0);
+    bool any = false;
+    for (int64_t i = 0; i < n; ++i) {
+        JoinBuild* J = ((RowArenaBlob*)arenas[i])->build;
+        if (!J) return fail("batched build of join %lld before Init", (long long)i);
+        if (J->plan != P) return fail("join %lld belongs to another compiled join", (long long)i);
+        if (J->built) return fail("join %lld is already built", (long long)i);
+        if (J->appended > (int64_t)0xffffffff) return fail("join builds are limited to 2^32 appended rows");
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        JoinBuild* J = ((RowArenaBlob*)arenas[i])->build;
+        if (dense_wanted(J)) {
+            queued[(size_t)i] = dense_build_launch(J) ? 1 : 0;
+            any = any || queued[(size_t)i];
+        }
+    }
+    const bool synced = !any || sync("batched dense fill");
+    bool ok = true;
+    for (int64_t i = 0; i < n; ++i) {
+        JoinBuild* J = ((RowArenaBlob*)arenas[i])->build;
+        bool built = queued[(size_t)i] && dense_build_finish(J, synced);
+        if (!built && ok) {
+            ok = P->KW == 1 ? do_build<1>(J, queued[(size_t)i] != 0) : do_build<2>(J, queued[(size_t)i] != 0);
+            built = ok;
+        }
+        if (built) publish_build((RowArenaBlob*)arenas[i], (RowTableBlob*)tables[i], J);
+    }
+    return ok;
+}
+
+// Rows per work item of the batched probe: four block tiles.
+constexpr int64_t kProbeItemRows = 4 * kProbeBlockTile;
+
+// Probe of batches[i] against join arenas[i] into pairs[i] for every i (one
+// compiled join with a one-word key), in ONE persistent launch
+// (k_probe_batched) between one counter-initialising launch and one launch
+// that collects every instance's pair count and fused words for a single
+// readback. A probe whose pairs outgrow the first capacity is probed again
+// on its own at the exact size. Each pair buffer and the fused results are
+// then as after jr_dispatch(Probe) of that batch.
+bool batched_probe(JoinPlan* P, void* const* arenas, const TRowSet* const* batches, void* const* pairs, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        JoinBuild* J = ((RowArenaBlob*)arenas[i])->build;
+        if (!J || !J->built) return fail("batched probe of join %lld before Build", (long long)i);
+        if (J->plan != P) return fail("join %lld belongs to another compiled join", (long long)i);
+        if (!batches[i] || !pairs[i]) return fail("batched probe %lld: null batch or pair buffer", (long long)i);
+        if (batches[i]->RowCount >= (int64_t)1 << 32) return fail("probe batches are limited to 2^32 rows");
+    }
+    if (P->KW != 1) {
+        // Two-word keys: one probe at a time.
+        for (int64_t i = 0; i < n; ++i) {
+            JoinBuild* J = ((RowArenaBlob*)arenas[i])->build;
+            if (!do_probe<2>(J, batches[i], (QdbPairBuffer*)pairs[i]) || !collect_fused(J)) return false;
+            register_pairs(pairs[i], J);
+        }
+        return true;
+    }
+    std::vector<ProbeArgs> insts;
+    std::vector<ProbeItem> items;
+    std::vector<int64_t> which;
+    int mode = -1;
+    for (int64_t i = 0; i < n; ++i) {
+        JoinBuild* J = ((RowArenaBlob*)arenas[i])->build;
+        QdbPairBuffer* PB = (QdbPairBuffer*)pairs[i];
+        const TRowSet* b = batches[i];
+        PB->Count = 0;
+        J->last_probe = *b;
+        J->fused_ready = false;
+        register_pairs(PB, J);
+        if (b->RowCount <= 0) continue;
+        ProbeArgs A;
+        int m = kGeneric;
+        if (!probe_args_of<1>(J, b, A, &m)) return false;
+        if (mode >= 0 && m != mode) return fail("batched probe: kernel modes differ");
+        mode = m;
+        // Nothing staged in LDS per item: the dense table reads its ordinal
+        // table for every key, the hash table has no Bloom filter.
+        A.lds_words = 0;
+        A.bloom_words = 0;
+        if (J->dense) {
+            A.dmin = J->dmin;
+            A.drange = J->drange;
+            A.dbits = J->dbits;
+            A.dtab = J->dtab;
+        }
+        if (!reserve_pairs(PB, first_pair_capacity(J, b->RowCount))) return false;
+        A.pairs = PB->Data;
+        A.pair_cap = (unsigned long long)PB->Capacity;
+        A.pair_count = J->pair_count;
+        A.fused_acc = J->fused_acc;
+        A.fused_part = nullptr;
+        const int k = (int)insts.size();
+        insts.push_back(A);
+        which.push_back(i);
+        for (int64_t r = 0; r < b->RowCount; r += kProbeItemRows) {
+            ProbeItem it;
+            it.inst = k;
+            it.pad = 0;
+            it.b_begin = r;
+            it.b_end = r + kProbeItemRows < b->RowCount ? r + kProbeItemRows : b->RowCount;
+            items.push_back(it);
+        }
+    }
+    if (insts.empty()) return true;
+    const int nk = (int)insts.size();
+    const int nitems = (int)items.size();
+    const int words = P->spec.NumFused * kAccWords;
+    const int stride = 1 + words;
+    ProbeArgs* d_insts = (ProbeArgs*)upload(insts.data(), insts.size() * sizeof(ProbeArgs));
+    ProbeItem* d_items = (ProbeItem*)upload(items.data(), items.size() * sizeof(ProbeItem));
+    unsigned int* d_head = (unsigned int*)dalloc(16);
+    unsigned long long* d_out = (unsigned long long*)dalloc((size_t)nk * stride * 8);
+    unsigned long long* h_out = (unsigned long long*)halloc_pinned((size_t)nk * stride * 8);
+    auto release = [&]() {
+        dfree(d_insts);
+        dfree(d_items);
+        dfree(d_head);
+        dfree(d_out);
+        hfree_pinned(h_out);
+    };
+    bool ok = d_insts && d_items && d_head && d_out && h_out && zero_fill(d_head, 16);
+    if (ok) {
+        hipLaunchKernelGGL(k_probe_init_batched, dim3(nk < 1024 ? nk : 1024), dim3(64), 0, stream(), d_insts, nk, words);
+        const int grid = nitems < num_cus() ? nitems : num_cus();
+        {
+            TimedRegion timer("join_probe_batched");
+            if (mode == kDirectVal) hipLaunchKernelGGL(k_probe_batched<kDirectVal>, dim3(grid), dim3(kProbeBlock), 0, stream(), d_insts, d_items, nitems, d_head);
+            else if (mode == kDirectKey) hipLaunchKernelGGL(k_probe_batched<kDirectKey>, dim3(grid), dim3(kProbeBlock), 0, stream(), d_insts, d_items, nitems, d_head);
+            else hipLaunchKernelGGL(k_probe_batched<kGeneric>, dim3(grid), dim3(kProbeBlock), 0, stream(), d_insts, d_items, nitems, d_head);
+        }
+        hipLaunchKernelGGL(k_probe_collect_batched, dim3(grid_for((int64_t)nk * stride, kBlock, 1)), dim3(kBlock), 0, stream(), d_insts,
+            nk, words, d_out);
+        ok = hip_ok(hipGetLastError(), "batched probe launch") &&
+            hip_ok(hipMemcpyAsync(h_out, d_out, (size_t)nk * stride * 8, hipMemcpyDeviceToHost, stream()), "batched probe readback") &&
+            sync("batched probe");
+    }
+    for (int k = 0; ok && k < nk; ++k) {
+        const int64_t i = which[(size_t)k];
+        JoinBuild* J = ((RowArenaBlob*)arenas[i])->build;
+        QdbPairBuffer* PB = (QdbPairBuffer*)pairs[i];
+        const unsigned long long* w = h_out + (size_t)k * stride;
+        J->counters_ready = false;
+        if (w[0] > insts[(size_t)k].pair_cap) {
+            // More pairs than the first capacity: this probe again, alone.
+            ok = do_probe<1>(J, batches[i], PB) && collect_fused(J);
+            continue;
+        }
+        PB->Count = (int64_t)w[0];
+        memcpy(J->readback + 2, w + 1, (size_t)words * 8);
+        J->fused_ready = words > 0;
+        ok = collect_fused(J);
+    }
+    release();
+    return ok;
+}
+
 bool jr_dispatch_impl(JoinPlan* P, void* arena, void* table, void* scratch, void* pairs, TRowSet* batch, const int64_t* key_columns,
     int64_t op)
 {
@@ -2640,15 +2982,7 @@ bool jr_dispatch_impl(JoinPlan* P, void* arena, void* table, void* scratch, void
         if (!J) return fail("join build before Init");
         bool ok = P->KW == 1 ? do_build<1>(J) : do_build<2>(J);
         if (!ok) return false;
-        AB->Rows = J->rows;
-        TB->entries = J->table;
-        TB->Capacity = (int64_t)J->capacity;
-        TB->Mask = J->capacity - 1;
-        int shift = 64;
-        for (uint64_t c = J->capacity; c > 1; c >>= 1) --shift;
-        TB->Shift = shift;
-        TB->Rows = J->rows;
-        TB->Keys = J->keys_distinct;
+        publish_build(AB, TB, J);
         return true;
     }
     case QDB_JR_OP_PROBE: {
@@ -2657,10 +2991,7 @@ bool jr_dispatch_impl(JoinPlan* P, void* arena, void* table, void* scratch, void
         if (!batch) return fail("join probe without a batch");
         bool ok = P->KW == 1 ? do_probe<1>(J, batch, PB) : do_probe<2>(J, batch, PB);
         if (ok) ok = collect_fused(J);
-        if (ok) {
-            std::lock_guard<std::mutex> lock(g_pairs_mu);
-            pair_owner()[pairs] = J;
-        }
+        if (ok) register_pairs(pairs, J);
         return ok;
     }
     default:
@@ -2831,6 +3162,49 @@ unsigned long long* noalloc_scratch()
 }
 
 }  // namespace
+
+extern "C" int qdb_gpu_join_build_batched(const QdbRowJoinKernels* kernels, void* const* arenas, void* const* tables, int64_t n)
+{
+    clear_error();
+    if (!kernels || (n > 0 && (!arenas || !tables))) return fail("null argument"), -1;
+    if (kernels->Slot < 0 || kernels->Slot >= kSlots) return fail("bad join slot %d", kernels->Slot), -1;
+    JoinPlan* P = join_slots().get(kernels->Slot);
+    if (!P) return fail("join kernels are not compiled"), -1;
+    for (int64_t i = 0; i < n; ++i) {
+        if (!arenas[i] || !tables[i]) return fail("null arena or table %lld", (long long)i), -1;
+    }
+    return batched_build(P, arenas, tables, n) ? 0 : -1;
+}
+
+extern "C" int qdb_gpu_join_probe_batched(const QdbRowJoinKernels* kernels, void* const* arenas, const TRowSet* const* batches,
+    void* const* pairs, int64_t n)
+{
+    clear_error();
+    if (!kernels || (n > 0 && (!arenas || !batches || !pairs))) return fail("null argument"), -1;
+    if (kernels->Slot < 0 || kernels->Slot >= kSlots) return fail("bad join slot %d", kernels->Slot), -1;
+    JoinPlan* P = join_slots().get(kernels->Slot);
+    if (!P) return fail("join kernels are not compiled"), -1;
+    for (int64_t i = 0; i < n; ++i) {
+        if (!arenas[i]) return fail("null arena %lld", (long long)i), -1;
+    }
+    return batched_probe(P, arenas, batches, pairs, n) ? 0 : -1;
+}
+
+extern "C" int qdb_gpu_pairs_checksum(const int64_t* pairs, int64_t n, uint64_t* out)
+{
+    clear_error();
+    if (!out || (n > 0 && !pairs) || n < 0) return fail("bad argument"), -1;
+    if (!ensure_init()) return -1;
+    unsigned long long* d = (unsigned long long*)dalloc(16);
+    bool ok = d && zero_fill(d, 16);
+    if (ok && n > 0) {
+        hipLaunchKernelGGL(k_pairs_checksum, dim3(grid_for(n, kBlock, 4)), dim3(kBlock), 0, stream(), pairs, n, d);
+        ok = hip_ok(hipGetLastError(), "pairs checksum");
+    }
+    ok = ok && hip_ok(hipMemcpyAsync(out, d, 16, hipMemcpyDeviceToHost, stream()), "pairs checksum") && sync("pairs checksum");
+    dfree(d);
+    return ok ? 0 : -1;
+}
 
 extern "C" int qdb_gpu_join_probe_fused_noalloc(void* arena, const TRowSet* batch, QdbFusedResult* out, int64_t max_out,
     int64_t* pairs_out)

@@ -1022,7 +1022,7 @@ extern "C" int64_t qdb_gpu_struct_sizes(int64_t* out, int64_t n)
         (int64_t)sizeof(QdbExprProgram), (int64_t)sizeof(QdbColumnSchema), (int64_t)sizeof(QdbFilterSpec),
         (int64_t)sizeof(QdbProjectSpec), (int64_t)sizeof(QdbAggregateFn), (int64_t)sizeof(QdbAggregateSpec),
         (int64_t)sizeof(QdbAggregateKernels), (int64_t)sizeof(QdbRowJoinSpec), (int64_t)sizeof(QdbRowJoinKernels),
-        (int64_t)sizeof(QdbPairBuffer), (int64_t)sizeof(QdbFusedResult),
+        (int64_t)sizeof(QdbPairBuffer), (int64_t)sizeof(QdbFusedResult), (int64_t)sizeof(QdbTuneEntry),
     };
     const int64_t count = (int64_t)(sizeof(sizes) / sizeof(sizes[0]));
     for (int64_t i = 0; i < n && i < count; ++i) out[i] = sizes[i];

@@ -751,6 +751,52 @@ class TRowJoinProcessor:
             check(lib().qdb_gpu_memcpy_d2h(out.ctypes.data, pb.Data, out.nbytes), "pairs d2h")
         return out.reshape(n, 2)
 
+    @staticmethod
+    def build_batched(procs: Sequence["TRowJoinProcessor"]) -> None:
+        """build() of every processor (one compiled join) with one
+        synchronization for all their dense tables."""
+        arr = (C.c_void_p * max(1, len(procs)))(*[p.p for p in procs])
+        if lib().qdb_gpu_join_processor_build_batched(arr, len(procs)) != 0:
+            raise RuntimeError((lib().qdb_gpu_exec_last_error() or b"").decode())
+
+    @staticmethod
+    def probe_batched(procs: Sequence["TRowJoinProcessor"], batches: Sequence[DeviceRowSet]) -> None:
+        """Probe-only of batches[i] by procs[i] (one compiled join) in one
+        persistent launch; each processor's pairs stay readable
+        (pairs, pairs_checksum, fused_results)."""
+        n = len(procs)
+        arr = (C.c_void_p * max(1, n))(*[p.p for p in procs])
+        if n != len(batches):
+            raise ValueError("one batch per processor")
+        bs = (C.POINTER(abi.TRowSet) * max(1, n))(*[C.pointer(b.rs) for b in batches])
+        if lib().qdb_gpu_join_processor_probe_batched(arr, bs, n) != 0:
+            raise RuntimeError((lib().qdb_gpu_exec_last_error() or b"").decode())
+
+    def probe_only(self, batch: DeviceRowSet) -> int:
+        """Probes without materializing; returns the pair count (the pairs
+        stay on the device until the next probe)."""
+        return self._ok(lib().qdb_gpu_join_processor_probe_only(self.p, batch.ptr))
+
+    def pairs_checksum(self) -> tuple:
+        """(sum, xor) of a mix of every pair of the last probe-only call,
+        computed on the device (equal for equal pair sets in any order)."""
+        pb = lib().qdb_gpu_join_processor_pairs(self.p).contents
+        out = (C.c_uint64 * 2)()
+        check(lib().qdb_gpu_pairs_checksum(pb.Data, int(pb.Count), out), "pairs_checksum")
+        return int(out[0]), int(out[1])
+
+    def pair_count(self) -> int:
+        return int(lib().qdb_gpu_join_processor_pairs(self.p).contents.Count)
+
+    def pairs(self) -> np.ndarray:
+        """The pairs of the last probe-only call as an (n, 2) int64 array."""
+        pb = lib().qdb_gpu_join_processor_pairs(self.p).contents
+        n = int(pb.Count)
+        out = np.empty(2 * n, dtype=np.int64)
+        if n:
+            check(lib().qdb_gpu_memcpy_d2h(out.ctypes.data, pb.Data, out.nbytes), "pairs d2h")
+        return out.reshape(n, 2)
+
     def fused_results(self):
         arena = lib().qdb_gpu_join_processor_arena(self.p)
         n = len(self.kernels.spec.fused)

@@ -274,3 +274,34 @@ class BatchedDevice:
         self.devs = []
         for i, (rows, card, s) in enumerate(self.instances):
             self.devs.append(GroupsDevice(rows, card, seed=seed * 1000003 + i, cdf=zipf_cdf(card, s)))
+
+
+# --------------------------------------------------------------------------
+# batched form: join instances
+# --------------------------------------------------------------------------
+
+
+def batched_join_instances(n: int = 256, seed: int = SEED_BATCHED + 1, build_lo: int = 1_000, build_hi: int = 2_000_000,
+                           rows_lo: int = 10_000, rows_hi: int = 20_000_000):
+    """(build rows, probe rows) per join instance, both log-uniform: build
+    keys 1..B shuffled (unique), probe foreign keys uniform over 1..B."""
+    rng = np.random.default_rng(seed)
+    b = np.exp(rng.uniform(np.log(build_lo), np.log(build_hi), n)).astype(np.int64)
+    p = np.exp(rng.uniform(np.log(rows_lo), np.log(rows_hi), n)).astype(np.int64)
+    return [(int(x), int(y)) for x, y in zip(b, p)]
+
+
+def batched_join_spec() -> JoinSpec:
+    """The join of every batched join instance: build (key i64, segment i32)
+    unfiltered, probe (fk i64, value DECIMAL(15,2)); COUNT(*) and SUM(value)
+    fused, and the matched pairs."""
+    return join_spec(fused=True)
+
+
+class BatchedJoinDevice:
+    """Every join instance's build and probe columns generated in HBM."""
+
+    def __init__(self, instances, seed: int = SEED_BATCHED + 1):
+        self.instances = list(instances)
+        self.devs = [JoinDevice(b, p, seed=seed * 1000003 + i) for i, (b, p) in enumerate(self.instances)]
+

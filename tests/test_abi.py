@@ -70,3 +70,36 @@ def test_package_import_fails_loudly_without_library(tmp_path):
     code = "import sys; sys.path.insert(0, %r); import qumirdb_amd" % str(tmp_path)
     proc = subprocess.run(["python3", "-c", code], capture_output=True, text=True)
     assert proc.returncode != 0 and "no CPU fallback" in proc.stderr
+
+
+def test_autotune_cache_file_round_trip(tmp_path):
+    """The tuner's cache file: valid lines load, invalid ones (unknown LDS
+    budget, fan-out out of range, malformed) are skipped, and a saved cache
+    loads back to the same entries. No GPU call."""
+    from qumirdb_amd import abi
+    lib = abi.declare(ctypes.CDLL(LIB))
+    lib.qdb_gpu_autotune_clear()
+    src = tmp_path / "tune.txt"
+    src.write_text("# kw card rows path lds grid shift ms\n"
+                   "1 14 18 0 64 2 0 1.5\n"
+                   "1 21 24 1 0 0 -1 4.25\n"
+                   "2 10 12 0 48 2 0 1.0\n"     # 48 KiB is not a candidate
+                   "1 22 25 1 0 0 5 1.0\n"      # fan-out shift out of range
+                   "garbage\n")
+    assert lib.qdb_gpu_autotune_load(str(src).encode()) == 2
+    n = lib.qdb_gpu_autotune_entries(None, 0)
+    assert n == 2
+    ents = (abi.QdbTuneEntry * n)()
+    lib.qdb_gpu_autotune_entries(ents, n)
+    got = sorted((e.KeyWords, e.CardBucket, e.RowsBucket, e.Path, e.LdsKiB, e.GridPerCu, e.FanoutShift, e.Ms) for e in ents)
+    assert got == [(1, 14, 18, 0, 64, 2, 0, 1.5), (1, 21, 24, 1, 0, 0, -1, 4.25)]
+    out = tmp_path / "saved.txt"
+    assert lib.qdb_gpu_autotune_save(str(out).encode()) == 0
+    lib.qdb_gpu_autotune_clear()
+    assert lib.qdb_gpu_autotune_entries(None, 0) == 0
+    assert lib.qdb_gpu_autotune_load(str(out).encode()) == 2
+    ents2 = (abi.QdbTuneEntry * 2)()
+    lib.qdb_gpu_autotune_entries(ents2, 2)
+    assert sorted((e.KeyWords, e.CardBucket, e.RowsBucket, e.Path, e.LdsKiB, e.GridPerCu, e.FanoutShift, e.Ms)
+                  for e in ents2) == got
+    lib.qdb_gpu_autotune_clear()

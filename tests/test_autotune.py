@@ -1,0 +1,87 @@
+"""The aggregate autotuner: every configuration it can pick gives the oracle's
+results (forced one at a time, on small and large tables, one batch and two
+batches into one table), and tuning caches one configuration per shape
+bucket that later updates of that bucket reuse."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from cases import Batch, GpuBackend, OracleBackend, random_agg_case
+from helpers import result_rows, sort_key
+
+pytestmark = pytest.mark.gpu
+
+# (path, lds_kib, grid_per_cu, fanout_shift): every candidate of the tuner.
+CONFIGS = [(0, 32, 4, 0), (0, 64, 2, 0), (0, 128, 1, 0), (1, 0, 0, -1), (1, 0, 0, 0), (1, 0, 0, 1)]
+
+
+@pytest.fixture(scope="module")
+def gpu_backend():
+    return GpuBackend()
+
+
+def staged_case(case):
+    """Aggregates the bucketed path stages (no i128 MIN/MAX)."""
+    keep = [(f, c) for f, c in case.aggs if not (f in (cases.AGG_MIN, cases.AGG_MAX) and case.types[c].width == 16)]
+    return cases.AggCase(case.types, case.keys, keep)
+
+
+@pytest.mark.parametrize("cfg", CONFIGS, ids=lambda c: "p%d_l%d_g%d_f%d" % c)
+@pytest.mark.parametrize("shape", ["small_table", "large_table"])
+def test_forced_config_equals_oracle(gpu_backend, monkeypatch, cfg, shape):
+    monkeypatch.setenv("QDB_GPU_TUNE_FORCE", ",".join(str(v) for v in cfg))
+    if shape == "small_table":
+        # 200K rows over a few thousand groups into a 4096-slot table, two batches.
+        case, b = random_agg_case(41, 200_000)
+        capacity = 1 << 12
+    else:
+        # 1.1M rows into a 2^21-slot table (made lazily), two batches.
+        case, b = random_agg_case(42, 1_100_000)
+        capacity = 1 << 21
+    case = staged_case(case)
+    got = sorted(result_rows(gpu_backend.aggregate(case, [b, b], capacity=capacity)), key=sort_key)
+    want = sorted(result_rows(OracleBackend(fsum_mode=2).aggregate(case, [b, b])), key=sort_key)
+    assert len(got) == len(want)
+    for g, w in zip(got, want):
+        assert g == w, f"{cfg} {shape}: {g} vs {w}"
+
+
+def test_tuner_caches_one_config_per_shape(gpu_backend, monkeypatch):
+    monkeypatch.delenv("QDB_GPU_TUNE_FORCE", raising=False)
+    lib = gpu_backend.q.lib()
+    abi = gpu_backend.q.abi
+    lib.qdb_gpu_autotune_clear()
+    was = lib.qdb_gpu_autotune_enable(1)
+    try:
+        case, b = random_agg_case(43, 300_000)
+        case = staged_case(case)
+        want = sorted(result_rows(OracleBackend(fsum_mode=2).aggregate(case, [b])), key=sort_key)
+        got = sorted(result_rows(gpu_backend.aggregate(case, [b], capacity=1 << 14)), key=sort_key)
+        assert got == want
+        n = lib.qdb_gpu_autotune_entries(None, 0)
+        assert n == 1
+        ents = (abi.QdbTuneEntry * n)()
+        assert lib.qdb_gpu_autotune_entries(ents, n) == n
+        e = ents[0]
+        assert e.CardBucket == 14 and e.RowsBucket == 18 and e.Ms > 0
+        assert (e.Path, e.LdsKiB, e.GridPerCu, e.FanoutShift) in CONFIGS
+        # The same shape again: no new entry, same results.
+        got2 = sorted(result_rows(gpu_backend.aggregate(case, [b], capacity=1 << 14)), key=sort_key)
+        assert got2 == want and lib.qdb_gpu_autotune_entries(None, 0) == 1
+        # Save, clear, load.
+        path = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"qdb_tune_{os.getpid()}.txt")
+        assert lib.qdb_gpu_autotune_save(path.encode()) == 0
+        lib.qdb_gpu_autotune_clear()
+        assert lib.qdb_gpu_autotune_entries(None, 0) == 0
+        assert lib.qdb_gpu_autotune_load(path.encode()) == 1
+        ents2 = (abi.QdbTuneEntry * 1)()
+        lib.qdb_gpu_autotune_entries(ents2, 1)
+        assert (ents2[0].Path, ents2[0].LdsKiB, ents2[0].GridPerCu, ents2[0].FanoutShift) == (e.Path, e.LdsKiB, e.GridPerCu,
+                                                                                              e.FanoutShift)
+        os.unlink(path)
+    finally:
+        lib.qdb_gpu_autotune_enable(was)
+        lib.qdb_gpu_autotune_clear()

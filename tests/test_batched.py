@@ -1,0 +1,127 @@
+"""The batched form against per-instance plain runs: many joins built with one
+synchronization and probed in one persistent launch from a device work queue
+give, instance by instance, the pairs and fused results of separate
+Append/Build/Probe calls (bit-exact), and one instance equals the oracle."""
+import numpy as np
+import pytest
+
+import cases
+from cases import AGG_COUNT_STAR, AGG_MIN, AGG_SUM, Batch, GpuBackend, JoinCase
+from helpers import SIGNED, T
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu_backend():
+    return GpuBackend()
+
+
+def sorted_pairs(p):
+    return p[np.lexsort((p[:, 1], p[:, 0]))]
+
+
+def join_instances(rng):
+    """(build keys, build values, build selection, probe keys, probe values,
+    probe selection) per instance: dense unique keys, duplicate keys (hash
+    table), a wide unique key range (hash table), an empty probe, a probe
+    selection, a one-row build, and a large dense instance."""
+    out = []
+    b = rng.permutation(5000).astype(np.int64) + 1
+    out.append((b, None, rng.integers(-100, 5100, 100_000)))
+    b = np.repeat(np.arange(3000, dtype=np.int64), rng.integers(1, 4, 3000))
+    out.append((b[rng.permutation(len(b))], None, rng.integers(-50, 3050, 60_000)))
+    b = rng.choice(np.iinfo(np.int64).max // 4, 20_000, replace=False).astype(np.int64) - (1 << 60)
+    out.append((b, None, np.concatenate([b[rng.integers(0, len(b), 30_000)], rng.integers(-10 ** 18, 10 ** 18, 30_000)])))
+    out.append((rng.permutation(700).astype(np.int64), None, np.empty(0, np.int64)))
+    b = rng.permutation(40_000).astype(np.int64)
+    out.append((b, (rng.random(len(b)) > 0.4).astype(np.uint8), rng.integers(0, 40_000, 250_000)))
+    out.append((np.array([42], np.int64), None, rng.integers(40, 45, 5_000)))
+    b = rng.permutation(200_000).astype(np.int64) + 7
+    out.append((b, None, rng.integers(0, 200_010, 2_000_000)))
+    return out
+
+
+def make_batches(rng, inst, probe_sel_every=4):
+    types_b = [T(SIGNED, 8), T(SIGNED, 8)]
+    types_p = [T(SIGNED, 8), T(SIGNED, 8)]
+    res = []
+    for k, (bk, bsel, pk) in enumerate(inst):
+        bv = rng.integers(-10 ** 12, 10 ** 12, len(bk)).astype(np.int64)
+        pk = np.asarray(pk, np.int64)
+        pv = rng.integers(-10 ** 9, 10 ** 9, len(pk)).astype(np.int64)
+        psel = (rng.random(len(pk)) > 0.5).astype(np.uint8) if (k % probe_sel_every == 3 and len(pk)) else None
+        res.append((Batch(types_b, [bk, bv], None, bsel), Batch(types_p, [pk, pv], None, psel)))
+    return types_b, types_p, res
+
+
+@pytest.mark.parametrize("join_type", [cases.JOIN_INNER, cases.JOIN_LEFT_OUTER])
+def test_batched_join_equals_plain(gpu_backend, join_type):
+    rng = np.random.default_rng(31 + join_type)
+    inst = join_instances(rng)
+    types_b, types_p, batches = make_batches(rng, inst)
+    case = JoinCase(types_b, types_p, [0], [0], join_type=join_type, probe_is_left=True,
+                    fused=[(AGG_COUNT_STAR, -1), (AGG_SUM, 1), (AGG_SUM, -3), (AGG_MIN, -3)])
+    devs = [(gpu_backend.dev(b), gpu_backend.dev(p)) for b, p in batches]
+    H = gpu_backend.H
+    kern, first = gpu_backend.join_processor(case)
+    first.close()
+    # Plain: one processor per instance, Append / Build / Probe each.
+    plain = []
+    for db, dp in devs:
+        proc = H.TRowJoinProcessor(kern)
+        proc.append(db)
+        proc.build()
+        n = proc.probe_only(dp)
+        plain.append((n, sorted_pairs(proc.pairs()), proc.fused_results(), proc.pairs_checksum()))
+        proc.close()
+    # Batched: every build with one synchronization, every probe in one launch.
+    procs = [H.TRowJoinProcessor(kern) for _ in devs]
+    for p, (db, _) in zip(procs, devs):
+        p.append(db)
+    H.TRowJoinProcessor.build_batched(procs)
+    for rnd in range(2):
+        H.TRowJoinProcessor.probe_batched(procs, [dp for _, dp in devs])
+        for k, p in enumerate(procs):
+            n, pairs, fused, checksum = plain[k]
+            assert p.pair_count() == n, f"instance {k}, round {rnd}: {p.pair_count()} pairs vs {n}"
+            assert np.array_equal(sorted_pairs(p.pairs()), pairs), f"instance {k}, round {rnd}: pairs differ"
+            assert p.pairs_checksum() == checksum
+            # Fused results accumulate over the probes since Init.
+            got = p.fused_results()
+            for f, (g, w) in enumerate(zip(got, fused)):
+                if w["count"] == 0:
+                    assert g["count"] == 0 and g["valid"] == w["valid"]
+                    continue
+                # fused[3] is a MIN: the same after a second probe; the
+                # counts and sums double.
+                want_val = w["value"] if f == 3 else w["value"] * (rnd + 1)
+                assert g["count"] == w["count"] * (rnd + 1), f"instance {k}, round {rnd}: fused count"
+                assert g["value"] == want_val, f"instance {k}, round {rnd}: fused {f}: {g} vs {w}"
+    # One instance against the oracle.
+    b, p = batches[1]
+    oj = O.OracleJoin(case.to_c())
+    oj.append(O.HostRowSet(types_b, b.columns, selection=b.selection))
+    oj.build()
+    want = oj.probe(O.HostRowSet(types_p, p.columns, selection=p.selection))
+    assert np.array_equal(plain[1][1], sorted_pairs(want))
+    for p in procs:
+        p.close()
+
+
+def test_batched_join_rejects_mixed_joins(gpu_backend):
+    """Processors of two compiled joins cannot share a batched launch."""
+    types = [T(SIGNED, 8), T(SIGNED, 8)]
+    c1 = JoinCase(types, types, [0], [0], fused=[(AGG_COUNT_STAR, -1)])
+    c2 = JoinCase(types, types, [0], [0], fused=[(AGG_SUM, 1)])
+    p1 = gpu_backend.join_processor(c1)[1]
+    p2 = gpu_backend.join_processor(c2)[1]
+    b = gpu_backend.dev(Batch(types, [np.arange(10, dtype=np.int64), np.arange(10, dtype=np.int64)]))
+    p1.append(b)
+    p2.append(b)
+    with pytest.raises(RuntimeError, match="different joins"):
+        gpu_backend.H.TRowJoinProcessor.build_batched([p1, p2])
+    p1.close()
+    p2.close()
