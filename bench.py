@@ -388,7 +388,7 @@ def bench_served(qps_levels=(50, 100, 200, 500, 1000, 2000), n_requests=120):
     row range), continuous batching versus one request at a time, with the
     same results checked request by request."""
     from qumirdb_amd import serve as SV
-    res = SV.Resident()
+    res = SV.Resident(tenants=8)
     srv = SV.Server(res)
     warm = SV.make_stream(res, 12, 1e6, seed=1)
     srv.serve(warm, batched=True)
@@ -402,6 +402,7 @@ def bench_served(qps_levels=(50, 100, 200, 500, 1000, 2000), n_requests=120):
             if a.result != b.result:
                 raise RuntimeError(f"request {a.rid} ({a.shape}) differs between plain and batched serving")
         levels.append({"offered_qps": qps, "plain": SV.summarize(plain), "batched": SV.summarize(batched)})
+    srv.close()
     res.close()
     return {"levels": levels, "requests_per_level": n_requests,
             "resident_bytes": res.table_bytes()}

@@ -681,6 +681,27 @@ int64_t qdb_gpu_autotune_load(const char* path);
  * dense table does not take is built with the hash table as by its own
  * Build. Returns 0, or -1 with the error text. */
 int qdb_gpu_join_build_batched(const QdbRowJoinKernels* kernels, void* const* arenas, void* const* tables, int64_t n);
+/* Detached batched probe (no counterpart in the reference): batches[i]
+ * (i < n) probed against ONE built join `arena` (one-word keys), each into
+ * its own pair buffer pairs[i] (pairs NULL, or pairs[i] NULL: the pairs are
+ * counted, not written), with the pair count in counts[i] and the fused
+ * results of that batch alone in fused[i * NumFused + f] (fused may be
+ * NULL). The join's own pair buffer, counters and fused totals are not
+ * touched, so threads that each issue on their own stream
+ * (qdb_gpu_set_thread_stream) may probe one resident build table at the
+ * same time. Each call replays a hipGraph captured once per stream and
+ * kernel mode (queue reset, counter set-up, the persistent probe over
+ * (batch, row range) items, collection of the counts and fused words, one
+ * copy to pinned host memory) after one descriptor upload, up to 256
+ * batches per replay. Returns 0, or -1 with the error text. */
+int qdb_gpu_join_probe_detached(const QdbRowJoinKernels* kernels, void* arena, const TRowSet* const* batches,
+                                QdbPairBuffer* const* pairs, int64_t n, int64_t* counts, QdbFusedResult* fused);
+/* Makes the calling thread issue its work on library stream `index`
+ * (0 = the default library stream, up to 3 more, all at the default
+ * priority). Device blocks freed on one stream are reused by another only
+ * after the work queued before the free has completed. One thread per
+ * stream index at a time. 0 or -1. */
+int qdb_gpu_set_thread_stream(int index);
 /* Order-independent checksum of n pairs at `pairs` (device, 2 i64 each):
  * out[0] = sum and out[1] = xor of a 64-bit mix of every pair (no
  * counterpart in the reference: it compares pair sets without moving them

@@ -232,7 +232,7 @@ EXPORTS = [
     "qdb_gpu_join_build_batched", "qdb_gpu_join_probe_batched", "qdb_gpu_join_processor_build_batched",
     "qdb_gpu_join_processor_probe_batched", "qdb_gpu_pairs_checksum",
     "qdb_gpu_autotune_enable", "qdb_gpu_autotune_entries", "qdb_gpu_autotune_clear", "qdb_gpu_autotune_save",
-    "qdb_gpu_autotune_load",
+    "qdb_gpu_autotune_load", "qdb_gpu_join_probe_detached", "qdb_gpu_set_thread_stream",
 ]
 
 
@@ -334,6 +334,10 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "qdb_gpu_autotune_clear": (None, []),
         "qdb_gpu_autotune_save": (C.c_int, [C.c_char_p]),
         "qdb_gpu_autotune_load": (i64, [C.c_char_p]),
+        "qdb_gpu_join_probe_detached": (C.c_int, [C.POINTER(QdbRowJoinKernels), P, C.POINTER(C.POINTER(TRowSet)),
+                                                  C.POINTER(C.POINTER(QdbPairBuffer)), i64, C.POINTER(i64),
+                                                  C.POINTER(QdbFusedResult)]),
+        "qdb_gpu_set_thread_stream": (C.c_int, [C.c_int]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -308,6 +308,7 @@ struct AggPlan {
     // The partial-state projection (qdb_gpu_agg_partial_states): compiled at
     // its first use, its project slot released with the plan.
     std::mutex partial_mu;
+    std::mutex jit_mu;      // jit_tried / jit_src, set by the first caller
     bool partial_tried = false;
     QdbProjectFn partial_fn = nullptr;
     int partial_slot = -1;
@@ -1125,9 +1126,12 @@ std::string agg_jit_source(const AggPlan* P)
 hipFunction_t plan_jit_function(AggPlan* P, const char* name)
 {
     if (!jit_enabled()) return nullptr;
-    if (!P->jit_tried) {
-        P->jit_tried = true;
-        P->jit_src = agg_jit_source(P);
+    {
+        std::lock_guard<std::mutex> lock(P->jit_mu);
+        if (!P->jit_tried) {
+            P->jit_tried = true;
+            P->jit_src = agg_jit_source(P);
+        }
     }
     std::string err;
     return jit_function(P->jit_src, name, &err);

@@ -1496,10 +1496,13 @@ struct ProbeItem {
     int64_t b_end;     // one past its last block tile's first row
 };
 
+// ctl[0] = instances, ctl[1] = items (device words, so that a captured graph
+// serves any count up to its buffers').
 template <int kMode>
 __global__ void __launch_bounds__(kProbeBlock) k_probe_batched(const ProbeArgs* __restrict__ insts,
-    const ProbeItem* __restrict__ items, int nitems, unsigned int* head)
+    const ProbeItem* __restrict__ items, const int* __restrict__ ctl, unsigned int* head)
 {
+    const int nitems = ctl[1];
     __shared__ int s_item;
     __shared__ uint32_t no_bits[4];
     for (;;) {
@@ -1516,14 +1519,17 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe_batched(const ProbeArgs* 
 }
 
 // Every instance's pair counter and fused words at their identities.
-__global__ void k_probe_init_batched(const ProbeArgs* __restrict__ insts, int n, int words)
+__global__ void k_probe_init_batched(const ProbeArgs* __restrict__ insts, const int* __restrict__ ctl, int words)
 {
+    const int n = ctl[0];
     for (int i = blockIdx.x; i < n; i += gridDim.x) probe_counters_init(insts[i].pair_count, insts[i].fused_acc, words, (int)threadIdx.x);
 }
 
 // Every instance's pair count and fused words into out[i * stride ...].
-__global__ void k_probe_collect_batched(const ProbeArgs* __restrict__ insts, int n, int words, unsigned long long* out)
+__global__ void k_probe_collect_batched(const ProbeArgs* __restrict__ insts, const int* __restrict__ ctl, int words,
+    unsigned long long* out)
 {
+    const int n = ctl[0];
     const int stride = 1 + words;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)n * stride; t += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(t / stride), k = (int)(t % stride);
@@ -1671,7 +1677,54 @@ __global__ void __launch_bounds__(kBlock) k_materialize(GatherArgs G)
 // Host.
 // ---------------------------------------------------------------------------
 
+// A captured probe pipeline (the detached batched probe): device and pinned
+// buffers of fixed capacity and the graph that reads them -- reset the
+// queue, set every instance's counters, the persistent probe, collect the
+// counts and fused words, copy them to the host. A call writes the
+// instances' descriptors into the buffers and replays the graph.
+struct ProbePipeline {
+    int mode = 0;
+    int words = 0;
+    static constexpr int kMaxInst = 256;
+    static constexpr int kMaxItems = 16384;
+    static constexpr int kStride = 2 + kMaxFused * kAccWords;   // counter words per instance
+    char* d_buf = nullptr;       // [ctl 16 B | insts | items]
+    char* h_stage = nullptr;     // pinned copy of d_buf's layout
+    unsigned int* d_head = nullptr;
+    unsigned long long* d_counters = nullptr;   // kMaxInst x kStride
+    unsigned long long* d_out = nullptr;        // kMaxInst x (1 + words)
+    unsigned long long* h_out = nullptr;        // pinned
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    static size_t insts_off() { return 16; }
+    static size_t items_off() { return 16 + (size_t)kMaxInst * sizeof(ProbeArgs); }
+    static size_t buf_bytes() { return items_off() + (size_t)kMaxItems * sizeof(ProbeItem); }
+    ~ProbePipeline()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        dfree(d_buf);
+        dfree(d_head);
+        dfree(d_counters);
+        dfree(d_out);
+        hfree_pinned(h_stage);
+        hfree_pinned(h_out);
+    }
+};
+
 struct JoinPlan {
+    ~JoinPlan()
+    {
+        for (auto& per_stream : pipes) {
+            for (ProbePipeline*& pp : per_stream) {
+                delete pp;
+                pp = nullptr;
+            }
+        }
+    }
+    std::mutex pipe_mu;
+    ProbePipeline* pipes[kMaxStreams][3] = {};   // per stream and kernel mode
     QdbRowJoinSpec spec;
     int KW = 1;
     JKeyDesc build_key;   // slots index build-side DevCols
@@ -2804,6 +2857,13 @@ bool batched_build(JoinPlan* P, void* const* arenas, void* const* tables, int64_
     return ok;
 }
 
+void launch_probe_batched(int mode, int grid, const ProbeArgs* insts, const ProbeItem* items, const int* ctl, unsigned int* head)
+{
+    if (mode == kDirectVal) hipLaunchKernelGGL(k_probe_batched<kDirectVal>, dim3(grid), dim3(kProbeBlock), 0, stream(), insts, items, ctl, head);
+    else if (mode == kDirectKey) hipLaunchKernelGGL(k_probe_batched<kDirectKey>, dim3(grid), dim3(kProbeBlock), 0, stream(), insts, items, ctl, head);
+    else hipLaunchKernelGGL(k_probe_batched<kGeneric>, dim3(grid), dim3(kProbeBlock), 0, stream(), insts, items, ctl, head);
+}
+
 // Rows per work item of the batched probe: four block tiles.
 constexpr int64_t kProbeItemRows = 4 * kProbeBlockTile;
 
@@ -2883,30 +2943,31 @@ bool batched_probe(JoinPlan* P, void* const* arenas, const TRowSet* const* batch
     const int nitems = (int)items.size();
     const int words = P->spec.NumFused * kAccWords;
     const int stride = 1 + words;
+    const int ctl[4] = {nk, nitems, 0, 0};
     ProbeArgs* d_insts = (ProbeArgs*)upload(insts.data(), insts.size() * sizeof(ProbeArgs));
     ProbeItem* d_items = (ProbeItem*)upload(items.data(), items.size() * sizeof(ProbeItem));
+    int* d_ctl = (int*)upload(ctl, sizeof(ctl));
     unsigned int* d_head = (unsigned int*)dalloc(16);
     unsigned long long* d_out = (unsigned long long*)dalloc((size_t)nk * stride * 8);
     unsigned long long* h_out = (unsigned long long*)halloc_pinned((size_t)nk * stride * 8);
     auto release = [&]() {
         dfree(d_insts);
         dfree(d_items);
+        dfree(d_ctl);
         dfree(d_head);
         dfree(d_out);
         hfree_pinned(h_out);
     };
-    bool ok = d_insts && d_items && d_head && d_out && h_out && zero_fill(d_head, 16);
+    bool ok = d_insts && d_items && d_ctl && d_head && d_out && h_out && zero_fill(d_head, 16);
     if (ok) {
-        hipLaunchKernelGGL(k_probe_init_batched, dim3(nk < 1024 ? nk : 1024), dim3(64), 0, stream(), d_insts, nk, words);
+        hipLaunchKernelGGL(k_probe_init_batched, dim3(nk < 1024 ? nk : 1024), dim3(64), 0, stream(), d_insts, d_ctl, words);
         const int grid = nitems < num_cus() ? nitems : num_cus();
         {
             TimedRegion timer("join_probe_batched");
-            if (mode == kDirectVal) hipLaunchKernelGGL(k_probe_batched<kDirectVal>, dim3(grid), dim3(kProbeBlock), 0, stream(), d_insts, d_items, nitems, d_head);
-            else if (mode == kDirectKey) hipLaunchKernelGGL(k_probe_batched<kDirectKey>, dim3(grid), dim3(kProbeBlock), 0, stream(), d_insts, d_items, nitems, d_head);
-            else hipLaunchKernelGGL(k_probe_batched<kGeneric>, dim3(grid), dim3(kProbeBlock), 0, stream(), d_insts, d_items, nitems, d_head);
+            launch_probe_batched(mode, grid, d_insts, d_items, d_ctl, d_head);
         }
         hipLaunchKernelGGL(k_probe_collect_batched, dim3(grid_for((int64_t)nk * stride, kBlock, 1)), dim3(kBlock), 0, stream(), d_insts,
-            nk, words, d_out);
+            d_ctl, words, d_out);
         ok = hip_ok(hipGetLastError(), "batched probe launch") &&
             hip_ok(hipMemcpyAsync(h_out, d_out, (size_t)nk * stride * 8, hipMemcpyDeviceToHost, stream()), "batched probe readback") &&
             sync("batched probe");
@@ -2929,6 +2990,186 @@ bool batched_probe(JoinPlan* P, void* const* arenas, const TRowSet* const* batch
     }
     release();
     return ok;
+}
+
+// The calling thread's stream's pipeline for `mode`, made and captured at
+// first use.
+ProbePipeline* probe_pipeline(JoinPlan* P, int mode)
+{
+    const int si = thread_stream_index();
+    {
+        std::lock_guard<std::mutex> lock(P->pipe_mu);
+        if (P->pipes[si][mode]) return P->pipes[si][mode];
+    }
+    ProbePipeline* pp = new ProbePipeline();
+    pp->mode = mode;
+    pp->words = P->spec.NumFused * kAccWords;
+    pp->d_buf = (char*)dalloc(ProbePipeline::buf_bytes());
+    pp->h_stage = (char*)halloc_pinned(ProbePipeline::buf_bytes());
+    pp->d_head = (unsigned int*)dalloc(16);
+    pp->d_counters = (unsigned long long*)dalloc((size_t)ProbePipeline::kMaxInst * ProbePipeline::kStride * 8);
+    const size_t out_bytes = (size_t)ProbePipeline::kMaxInst * (1 + pp->words) * 8;
+    pp->d_out = (unsigned long long*)dalloc(out_bytes);
+    pp->h_out = (unsigned long long*)halloc_pinned(out_bytes);
+    if (!pp->d_buf || !pp->h_stage || !pp->d_head || !pp->d_counters || !pp->d_out || !pp->h_out || !sync("probe pipeline")) {
+        delete pp;
+        return nullptr;
+    }
+    const ProbeArgs* d_insts = (const ProbeArgs*)(pp->d_buf + ProbePipeline::insts_off());
+    const ProbeItem* d_items = (const ProbeItem*)(pp->d_buf + ProbePipeline::items_off());
+    const int* d_ctl = (const int*)pp->d_buf;
+    hipStream_t s = stream();
+    // Capture: kernels and one copy; the queue head is cleared by a kernel
+    // (no memset node).
+    bool ok = hip_ok(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "probe pipeline capture");
+    if (ok) {
+        zero_fill(pp->d_head, 16);
+        hipLaunchKernelGGL(k_probe_init_batched, dim3(ProbePipeline::kMaxInst), dim3(64), 0, s, d_insts, d_ctl, pp->words);
+        launch_probe_batched(mode, num_cus(), d_insts, d_items, d_ctl, pp->d_head);
+        hipLaunchKernelGGL(k_probe_collect_batched, dim3(64), dim3(kBlock), 0, s, d_insts, d_ctl, pp->words, pp->d_out);
+        (void)hipMemcpyAsync(pp->h_out, pp->d_out, out_bytes, hipMemcpyDeviceToHost, s);
+        const hipError_t e = hipStreamEndCapture(s, &pp->graph);
+        ok = hip_ok(e, "probe pipeline capture") && pp->graph &&
+            hip_ok(hipGraphInstantiate(&pp->exec, pp->graph, nullptr, nullptr, 0), "probe pipeline instantiate");
+    }
+    if (!ok) {
+        delete pp;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(P->pipe_mu);
+    if (P->pipes[si][mode]) {
+        delete pp;  // another caller on this stream made one meanwhile
+        return P->pipes[si][mode];
+    }
+    P->pipes[si][mode] = pp;
+    return pp;
+}
+
+// Detached batched probe: batches[i] (i < n) against ONE built join J,
+// each into its own pair buffer pairs[i] (NULL: pairs counted, not written),
+// counts[i] and fused[i * NumFused + f] returned per batch; J's own counters,
+// pair buffer and fused totals are not touched, so calls on different
+// threads (each on its own stream) may probe one resident table together.
+// Runs as replays of the thread's captured pipeline, up to 256 batches and
+// 16384 items per replay.
+bool detached_probe(JoinPlan* P, JoinBuild* J, const TRowSet* const* batches, QdbPairBuffer* const* pairs, int64_t n,
+    int64_t* counts, QdbFusedResult* fused)
+{
+    const int nf = P->spec.NumFused;
+    for (int64_t i = 0; i < n; ++i) {
+        counts[i] = 0;
+        if (fused && nf) {
+            // The results of a batch without rows (COUNT 0, the rest NULL).
+            for (int f = 0; f < nf; ++f) memset(&fused[i * nf + f], 0, sizeof(QdbFusedResult));
+            merge_fused_words(P, kZeroFusedWords, &fused[i * nf]);
+        }
+        if (!batches[i]) return fail("detached probe %lld: null batch", (long long)i);
+        if (batches[i]->RowCount >= (int64_t)1 << 32) return fail("probe batches are limited to 2^32 rows");
+    }
+    std::vector<int64_t> todo;
+    for (int64_t i = 0; i < n; ++i) {
+        if (batches[i]->RowCount > 0) todo.push_back(i);
+    }
+    std::vector<unsigned long long> caps((size_t)n, 0);
+    ProbePipeline* pp = nullptr;
+    int pp_mode = -1;
+    for (int pass = 0; pass < 2 && !todo.empty(); ++pass) {
+        std::vector<int64_t> again;
+        size_t at = 0;
+        while (at < todo.size()) {
+            // One replay: as many batches as the buffers hold.
+            std::vector<ProbeArgs> insts;
+            std::vector<ProbeItem> items;
+            std::vector<int64_t> which;
+            while (at < todo.size() && (int)insts.size() < ProbePipeline::kMaxInst) {
+                const int64_t i = todo[at];
+                const TRowSet* b = batches[i];
+                const int64_t nitems_i = (b->RowCount + kProbeItemRows - 1) / kProbeItemRows;
+                if (!insts.empty() && (int64_t)items.size() + nitems_i > ProbePipeline::kMaxItems) break;
+                if (nitems_i > ProbePipeline::kMaxItems) return fail("detached probe: batch %lld has too many rows", (long long)i);
+                ProbeArgs A;
+                int m = kGeneric;
+                if (!probe_args_of<1>(J, b, A, &m)) return false;
+                if (pp_mode >= 0 && m != pp_mode) return fail("detached probe: kernel modes differ");
+                if (!pp) {
+                    pp = probe_pipeline(P, m);
+                    if (!pp) return false;
+                    pp_mode = m;
+                }
+                A.lds_words = 0;
+                A.bloom_words = 0;
+                if (J->dense) {
+                    A.dmin = J->dmin;
+                    A.drange = J->drange;
+                    A.dbits = J->dbits;
+                    A.dtab = J->dtab;
+                }
+                QdbPairBuffer* PB = pairs ? pairs[i] : nullptr;
+                if (PB) {
+                    PB->Count = 0;
+                    const unsigned long long want = pass == 0 ? first_pair_capacity(J, b->RowCount) : caps[(size_t)i];
+                    if (!reserve_pairs(PB, want)) return false;
+                    A.pairs = PB->Data;
+                    A.pair_cap = (unsigned long long)PB->Capacity;
+                } else {
+                    A.pairs = nullptr;
+                    A.pair_cap = 0;
+                }
+                const int k = (int)insts.size();
+                A.pair_count = pp->d_counters + (size_t)k * ProbePipeline::kStride;
+                A.fused_acc = A.pair_count + 2;
+                A.fused_part = nullptr;
+                insts.push_back(A);
+                which.push_back(i);
+                for (int64_t r = 0; r < b->RowCount; r += kProbeItemRows) {
+                    ProbeItem it;
+                    it.inst = k;
+                    it.pad = 0;
+                    it.b_begin = r;
+                    it.b_end = r + kProbeItemRows < b->RowCount ? r + kProbeItemRows : b->RowCount;
+                    items.push_back(it);
+                }
+                ++at;
+            }
+            int* ctl = (int*)pp->h_stage;
+            ctl[0] = (int)insts.size();
+            ctl[1] = (int)items.size();
+            ctl[2] = ctl[3] = 0;
+            memcpy(pp->h_stage + ProbePipeline::insts_off(), insts.data(), insts.size() * sizeof(ProbeArgs));
+            memcpy(pp->h_stage + ProbePipeline::items_off(), items.data(), items.size() * sizeof(ProbeItem));
+            const size_t head_bytes = ProbePipeline::insts_off() + insts.size() * sizeof(ProbeArgs);
+            bool ok = hip_ok(hipMemcpyAsync(pp->d_buf, pp->h_stage, head_bytes, hipMemcpyHostToDevice, stream()), "pipeline descriptors") &&
+                hip_ok(hipMemcpyAsync(pp->d_buf + ProbePipeline::items_off(), pp->h_stage + ProbePipeline::items_off(),
+                           items.size() * sizeof(ProbeItem), hipMemcpyHostToDevice, stream()),
+                    "pipeline items");
+            {
+                TimedRegion timer("join_probe_detached");
+                ok = ok && hip_ok(hipGraphLaunch(pp->exec, stream()), "pipeline launch");
+            }
+            ok = ok && sync("detached probe");
+            if (!ok) return false;
+            const int stride = 1 + pp->words;
+            for (size_t k = 0; k < which.size(); ++k) {
+                const int64_t i = which[k];
+                const unsigned long long* w = pp->h_out + k * stride;
+                if (pairs && pairs[i] && w[0] > insts[k].pair_cap) {
+                    // More pairs than the first capacity: probed again at the exact size.
+                    caps[(size_t)i] = w[0];
+                    again.push_back(i);
+                    continue;
+                }
+                counts[i] = (int64_t)w[0];
+                if (pairs && pairs[i]) pairs[i]->Count = (int64_t)w[0];
+                if (fused && nf) {
+                    for (int f = 0; f < nf; ++f) memset(&fused[i * nf + f], 0, sizeof(QdbFusedResult));
+                    merge_fused_words(P, w + 1, &fused[i * nf]);
+                }
+            }
+        }
+        todo.swap(again);
+    }
+    if (!todo.empty()) return fail("detached probe: pair buffer overflow");
+    return true;
 }
 
 bool jr_dispatch_impl(JoinPlan* P, void* arena, void* table, void* scratch, void* pairs, TRowSet* batch, const int64_t* key_columns,
@@ -3204,6 +3445,21 @@ extern "C" int qdb_gpu_pairs_checksum(const int64_t* pairs, int64_t n, uint64_t*
     ok = ok && hip_ok(hipMemcpyAsync(out, d, 16, hipMemcpyDeviceToHost, stream()), "pairs checksum") && sync("pairs checksum");
     dfree(d);
     return ok ? 0 : -1;
+}
+
+extern "C" int qdb_gpu_join_probe_detached(const QdbRowJoinKernels* kernels, void* arena, const TRowSet* const* batches,
+    QdbPairBuffer* const* pairs, int64_t n, int64_t* counts, QdbFusedResult* fused)
+{
+    clear_error();
+    if (!kernels || !arena || (n > 0 && (!batches || !counts))) return fail("null argument"), -1;
+    if (kernels->Slot < 0 || kernels->Slot >= kSlots) return fail("bad join slot %d", kernels->Slot), -1;
+    JoinPlan* P = join_slots().get(kernels->Slot);
+    if (!P) return fail("join kernels are not compiled"), -1;
+    JoinBuild* J = ((RowArenaBlob*)arena)->build;
+    if (!J || !J->built) return fail("detached probe before Build"), -1;
+    if (J->plan != P) return fail("the join belongs to another compiled join"), -1;
+    if (P->KW != 1) return fail("detached probes take one-word join keys"), -1;
+    return detached_probe(P, J, batches, pairs, n, counts, fused) ? 0 : -1;
 }
 
 extern "C" int qdb_gpu_join_probe_fused_noalloc(void* arena, const TRowSet* batch, QdbFusedResult* out, int64_t max_out,

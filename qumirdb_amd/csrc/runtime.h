@@ -22,7 +22,14 @@ const std::string& last_error_text();
 
 // Lazily selects the device and creates the library stream; false on error.
 bool ensure_init();
+// Streams the library issues work on: [0] the library stream, [1..3] more
+// (at the default priority), one per thread that selects it.
+constexpr int kMaxStreams = 4;
+// The calling thread's stream (the library stream unless the thread chose
+// another with set_thread_stream).
 hipStream_t stream();
+bool set_thread_stream(int index);
+int thread_stream_index();
 int num_cus();
 // Largest LDS allocation of one workgroup on the device, in bytes.
 size_t max_lds_per_block();

@@ -21,6 +21,8 @@ namespace qdbgpu {
 namespace {
 
 thread_local std::string g_error;
+// The stream index this thread issues work on (0: the library stream).
+thread_local int t_stream = 0;
 
 struct Runtime {
     std::mutex mu;
@@ -34,7 +36,17 @@ struct Runtime {
     struct FreeBlock {
         void* p;
         uint64_t freed;
+        // The stream that freed it and, once more than one stream is in
+        // use, an event recorded there at the free: another stream takes
+        // the block only after the event has completed.
+        hipStream_t s;
+        hipEvent_t ev;
     };
+    // Streams: [0] the library stream; [1..] created at first selection
+    // (qdb_gpu_set_thread_stream), all at the default priority.
+    hipStream_t streams[kMaxStreams] = {};
+    bool multi = false;
+    std::vector<hipEvent_t> pool_events;   // events of freed blocks (no timing)
     std::multimap<size_t, FreeBlock> free_dev;
     std::unordered_map<void*, size_t> dev_sizes;
     // Blocks in free_dev: a second dfree of one of them is refused (two
@@ -134,13 +146,35 @@ bool ensure_init()
     }
     // Default priority, non-blocking with respect to the legacy stream.
     if (!hip_ok(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
+    r.streams[0] = r.stream;
     r.ready = true;
     return true;
 }
 
 hipStream_t stream()
 {
-    return R().stream;
+    Runtime& r = R();
+    return t_stream == 0 ? r.stream : r.streams[t_stream];
+}
+
+bool set_thread_stream(int index)
+{
+    if (index < 0 || index >= kMaxStreams) return fail("stream index %d outside [0, %d)", index, kMaxStreams);
+    if (!ensure_init()) return false;
+    Runtime& r = R();
+    std::lock_guard<std::mutex> lock(r.mu);
+    if (!r.streams[index]) {
+        // Default priority, non-blocking with respect to the legacy stream.
+        if (!hip_ok(hipStreamCreateWithFlags(&r.streams[index], hipStreamNonBlocking), "hipStreamCreate")) return false;
+    }
+    if (index != 0) r.multi = true;
+    t_stream = index;
+    return true;
+}
+
+int thread_stream_index()
+{
+    return t_stream;
 }
 
 size_t max_lds_per_block()
@@ -186,6 +220,7 @@ static void evict_oldest_locked(Runtime& r)
     r.cached -= oldest->first;
     r.dev_sizes.erase(oldest->second.p);
     r.cached_set.erase(oldest->second.p);
+    if (oldest->second.ev) r.pool_events.push_back(oldest->second.ev);
     (void)hipFree(oldest->second.p);
     r.n_free += 1;
     r.free_dev.erase(oldest);
@@ -211,6 +246,7 @@ static void release_cached_locked(Runtime& r)
 {
     for (auto& kv : r.free_dev) {
         r.dev_sizes.erase(kv.second.p);
+        if (kv.second.ev) r.pool_events.push_back(kv.second.ev);
         (void)hipFree(kv.second.p);
         r.n_free += 1;
     }
@@ -226,10 +262,20 @@ void* dalloc(size_t bytes)
     const size_t c = size_class(bytes);
     {
         std::lock_guard<std::mutex> lock(r.mu);
-        // Best fit: the smallest cached block of at least class c.
-        auto it = r.free_dev.lower_bound(c);
-        if (it != r.free_dev.end() && it->first <= reuse_limit(c)) {
-            void* p = it->second.p;
+        // Best fit: the smallest cached block of at least class c that
+        // this stream may use now (freed on it, or its free has completed).
+        const hipStream_t cur = stream();
+        for (auto it = r.free_dev.lower_bound(c); it != r.free_dev.end() && it->first <= reuse_limit(c); ++it) {
+            Runtime::FreeBlock& b = it->second;
+            if (b.ev && b.s != cur) {
+                const hipError_t q = hipEventQuery(b.ev);
+                if (q != hipSuccess) {
+                    (void)hipGetLastError();
+                    continue;
+                }
+            }
+            void* p = b.p;
+            if (b.ev) r.pool_events.push_back(b.ev);
             r.cached -= it->first;
             r.free_dev.erase(it);
             r.cached_set.erase(p);
@@ -274,7 +320,24 @@ void dfree(void* p)
         return;
     }
     const size_t sz = it->second;
-    r.free_dev.emplace(sz, Runtime::FreeBlock{p, ++r.frees});
+    hipEvent_t ev = nullptr;
+    if (r.multi) {
+        // Work queued on this stream may still use the block.
+        if (!r.pool_events.empty()) {
+            ev = r.pool_events.back();
+            r.pool_events.pop_back();
+        } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            ev = nullptr;
+        }
+        if (ev && hipEventRecord(ev, stream()) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipStreamSynchronize(stream());
+            r.pool_events.push_back(ev);
+            ev = nullptr;
+        }
+    }
+    r.free_dev.emplace(sz, Runtime::FreeBlock{p, ++r.frees, stream(), ev});
     r.cached += sz;
     // Freed blocks stay cached while the device keeps the reserve free (the
     // HIP runtime and other allocators need memory of their own); past
@@ -1015,6 +1078,12 @@ extern "C" int qdb_gpu_gen_zipf_groups(uint64_t seed, int64_t row0, int64_t rows
 
 // Sizes of the ABI structs, in the order the header declares them, so that
 // bindings can check their layouts.
+extern "C" int qdb_gpu_set_thread_stream(int index)
+{
+    qdbgpu::clear_error();
+    return qdbgpu::set_thread_stream(index) ? 0 : -1;
+}
+
 extern "C" int64_t qdb_gpu_struct_sizes(int64_t* out, int64_t n)
 {
     const int64_t sizes[] = {

@@ -785,6 +785,32 @@ class TRowJoinProcessor:
         check(lib().qdb_gpu_pairs_checksum(pb.Data, int(pb.Count), out), "pairs_checksum")
         return int(out[0]), int(out[1])
 
+    def probe_detached(self, batches: Sequence[DeviceRowSet], pair_buffers: Optional[Sequence] = None):
+        """Probes every batch against this processor's built table in one
+        replay of the calling thread's captured pipeline, without touching
+        the processor's own pairs or fused totals. pair_buffers: one
+        abi.QdbPairBuffer per batch (or None: pairs counted, not written).
+        Returns [(pair count, fused results)] per batch."""
+        n = len(batches)
+        bs = (C.POINTER(abi.TRowSet) * max(1, n))(*[C.pointer(b.rs) for b in batches])
+        pbs = None
+        if pair_buffers is not None:
+            pbs = (C.POINTER(abi.QdbPairBuffer) * max(1, n))(*[C.pointer(p) if p is not None else C.POINTER(abi.QdbPairBuffer)()
+                                                               for p in pair_buffers])
+        counts = (C.c_int64 * max(1, n))()
+        nf = len(self.kernels.spec.fused)
+        fused = (abi.QdbFusedResult * max(1, n * nf))()
+        arena = lib().qdb_gpu_join_processor_arena(self.p)
+        check(lib().qdb_gpu_join_probe_detached(C.byref(self.kernels.k), arena, bs, pbs, n, counts, fused), "join_probe_detached")
+        out = []
+        for i in range(n):
+            res = []
+            for r in fused[i * nf:(i + 1) * nf]:
+                v = (int(r.Hi) << 64) + (int(r.Lo) & ((1 << 64) - 1))
+                res.append({"value": v, "count": int(r.Count), "valid": bool(r.Valid)})
+            out.append((int(counts[i]), res))
+        return out
+
     def pair_count(self) -> int:
         return int(lib().qdb_gpu_join_processor_pairs(self.p).contents.Count)
 
@@ -1037,3 +1063,19 @@ class SortKernel:
         start = max(0, min(start, n))
         count = n - start if limit is None else max(0, min(limit, n - start))
         return gather_rowset(rowset, idx, start, count)
+
+
+def set_thread_stream(index: int) -> None:
+    """The calling thread issues its library work on stream `index` (0: the
+    library stream; up to 3 more)."""
+    check(lib().qdb_gpu_set_thread_stream(index), "set_thread_stream")
+
+
+def free_pair_buffer(pb) -> None:
+    """Returns the device block of a PairBuffer filled by probe_detached."""
+    if pb.Data:
+        lib().qdb_gpu_free(pb.Data)
+        pb.Data = None
+        pb.Capacity = 0
+        pb.Count = 0
+

@@ -125,3 +125,91 @@ def test_batched_join_rejects_mixed_joins(gpu_backend):
         gpu_backend.H.TRowJoinProcessor.build_batched([p1, p2])
     p1.close()
     p2.close()
+
+
+def pairs_of(gpu_backend, pb):
+    n = int(pb.Count)
+    out = np.empty(2 * n, dtype=np.int64)
+    if n:
+        gpu_backend.q.check(gpu_backend.q.lib().qdb_gpu_memcpy_d2h(out.ctypes.data, pb.Data, out.nbytes), "d2h")
+    return sorted_pairs(out.reshape(n, 2))
+
+
+@pytest.mark.parametrize("dup", [False, True], ids=["dense", "hash"])
+def test_detached_probe_equals_plain(gpu_backend, dup):
+    """Many probe batches against ONE built join through the captured probe
+    pipeline (qdb_gpu_join_probe_detached): each batch's pairs, count and
+    fused results equal a plain probe of that batch alone; the graph is
+    replayed several times with plain probes between the replays, with and
+    without pair buffers, and from two threads on their own streams at once."""
+    import threading
+    H = gpu_backend.H
+    abi = gpu_backend.q.abi
+    rng = np.random.default_rng(5 + dup)
+    types_b = [T(SIGNED, 8), T(SIGNED, 8)]
+    types_p = [T(SIGNED, 8), T(SIGNED, 8)]
+    nb = 60_000
+    bk = rng.permutation(nb).astype(np.int64) + 3
+    if dup:
+        bk = np.concatenate([bk, bk[: nb // 3]])
+    bv = rng.integers(-10 ** 12, 10 ** 12, len(bk)).astype(np.int64)
+    case = JoinCase(types_b, types_p, [0], [0], fused=[(AGG_COUNT_STAR, -1), (AGG_SUM, 1), (AGG_SUM, -3)])
+    kern, proc = gpu_backend.join_processor(case)
+    proc.append(gpu_backend.dev(Batch(types_b, [bk, bv])))
+    proc.build()
+    sizes = [0, 1, 700, 50_000, 333_333, 1_200_000, 64, 90_000]
+    probes = []
+    for k, n in enumerate(sizes):
+        pk = rng.integers(-1000, nb + 1000, n).astype(np.int64)
+        pv = rng.integers(-10 ** 9, 10 ** 9, n).astype(np.int64)
+        sel = (rng.random(n) > 0.3).astype(np.uint8) if k == 4 else None
+        probes.append(gpu_backend.dev(Batch(types_p, [pk, pv], None, sel)))
+    # Plain: one probe per batch on a processor of its own.
+    want = []
+    for d in probes:
+        p = H.TRowJoinProcessor(kern)
+        p.append(gpu_backend.dev(Batch(types_b, [bk, bv])))
+        p.build()
+        n = p.probe_only(d)
+        f = p.fused_results()
+        want.append((n, [x["value"] for x in f], [x["count"] for x in f], [x["valid"] for x in f], sorted_pairs(p.pairs())))
+        p.close()
+
+    def check(out, pbs):
+        for k, (n, fused) in enumerate(out):
+            wn, wv, wc, wvalid, wp = want[k]
+            assert n == wn, f"batch {k}: {n} pairs vs {wn}"
+            assert [x["value"] for x in fused] == wv and [x["count"] for x in fused] == wc, f"batch {k}: fused"
+            assert [x["valid"] for x in fused] == wvalid
+            if pbs is not None:
+                assert np.array_equal(pairs_of(gpu_backend, pbs[k]), wp), f"batch {k}: pairs"
+
+    for rnd in range(3):
+        pbs = [abi.QdbPairBuffer() for _ in probes]
+        check(proc.probe_detached(probes, pbs), pbs)
+        for pb in pbs:
+            H.free_pair_buffer(pb)
+        check(proc.probe_detached(probes), None)
+        # Eager work between replays.
+        assert proc.probe_only(probes[3]) == want[3][0]
+    # Two threads, each on its own stream, probing the same table at once.
+    errors = []
+
+    def worker(idx):
+        try:
+            H.set_thread_stream(idx)
+            for _ in range(3):
+                pbs = [abi.QdbPairBuffer() for _ in probes]
+                check(proc.probe_detached(probes, pbs), pbs)
+                for pb in pbs:
+                    H.free_pair_buffer(pb)
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    ts = [threading.Thread(target=worker, args=(i,)) for i in (1, 2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
+    proc.close()

@@ -990,29 +990,39 @@ def test_f64_sum_edge_values(gpu_backend):
 
 
 def test_served_results_same_alone_and_in_the_mix(gpu_backend):
-    """Requests served with continuous batching (every request that has
-    arrived runs in the next round; group-bys of a shape share one batched
-    launch) return exactly what each returns alone, one at a time."""
+    """Requests served with continuous batching (admitted on arrival,
+    dispatched in units to worker threads that each issue on their own
+    stream; group-bys of a shape share one batched launch, joins of a tenant
+    one replay of the captured probe pipeline) return exactly what each
+    returns alone, one at a time."""
     from qumirdb_amd import serve as SV
-    res = SV.Resident(lineitem_rows=200_000, build_rows=50_000, probe_rows=400_000, group_rows=300_000, distinct=40_000)
+    res = SV.Resident(lineitem_rows=200_000, build_rows=50_000, probe_rows=400_000, group_rows=300_000, distinct=40_000,
+                      tenants=2)
+    srv = SV.Server(res, full_results=True)
     try:
-        srv = SV.Server(res, full_results=True)
-        stream = SV.make_stream(res, 36, 20_000.0, seed=7)
+        stream = SV.make_stream(res, 48, 20_000.0, seed=7)
         mixed = srv.serve(SV.fresh(stream), batched=True)
         alone = srv.serve(SV.fresh(stream), batched=False)
         assert {r.shape for r in stream} == set(SV.SHAPES)
+        assert {r.tenant for r in stream} == {0, 1}
         for a, b in zip(alone, mixed):
-            assert a.result == b.result, f"request {a.rid} ({a.shape}, rows {a.r0}..{a.r1})"
+            assert a.result == b.result, f"request {a.rid} ({a.shape}, tenant {a.tenant}, rows {a.r0}..{a.r1})"
+        # Requests ran together: some unit held more than one request, and
+        # the units of different workers overlapped in time.
+        spans = sorted((r.start, r.done) for r in mixed)
+        assert any(s2 < d1 for (s1, d1), (s2, d2) in zip(spans, spans[1:]))
         # and a join request's fused aggregate against the oracle's pairs
-        j = next(r for r in alone if r.shape == "join")
-        fk = res.jd.fk.to_numpy(np.int64, res.jd.probe_rows)[j.r0:j.r1]
-        key = res.jd.key.to_numpy(np.int64, res.jd.build_rows)
-        seg = res.jd.segment.to_numpy(np.int32, res.jd.build_rows)
-        val = res.jd.value.to_numpy(np.int64, res.jd.probe_rows)[j.r0:j.r1]
-        build = set(key[seg == 1].tolist())
-        hit = np.array([k in build for k in fk.tolist()], dtype=bool)
-        assert j.result == (int(hit.sum()), int(hit.sum()), int(val[hit].sum()))
+        for j in [r for r in alone if r.shape == "join"][:2]:
+            jd = res.tenants[j.tenant].jd
+            fk = jd.fk.to_numpy(np.int64, jd.probe_rows)[j.r0:j.r1]
+            key = jd.key.to_numpy(np.int64, jd.build_rows)
+            seg = jd.segment.to_numpy(np.int32, jd.build_rows)
+            val = jd.value.to_numpy(np.int64, jd.probe_rows)[j.r0:j.r1]
+            build = set(key[seg == 1].tolist())
+            hit = np.array([k in build for k in fk.tolist()], dtype=bool)
+            assert j.result == (int(hit.sum()), int(hit.sum()), int(val[hit].sum()))
     finally:
+        srv.close()
         res.close()
 
 
