@@ -63,6 +63,29 @@ def timing_per_step(family, steps):
     return ms * n / steps if steps else 0.0
 
 
+def run_steps(step, steps):
+    """Times `steps` steps on the wall clock with the library's HIP-event
+    regions off (the throughput figure), then `steps` more with them on (the
+    per-kernel durations, read with timing() afterwards; recording the events
+    costs time of its own). Returns (ms per step, ms per step with events)."""
+    from qumirdb_amd import lib
+    lib().qdb_gpu_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    lib().qdb_gpu_synchronize()
+    t1 = time.perf_counter()
+    lib().qdb_gpu_timing_reset()
+    lib().qdb_gpu_timing_enable(1)
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    lib().qdb_gpu_synchronize()
+    t3 = time.perf_counter()
+    lib().qdb_gpu_timing_enable(0)
+    return (t1 - t0) * 1e3 / steps, (t3 - t2) * 1e3 / steps
+
+
 def bench_join(steps, warmup, build_rows=1_500_000, probe_rows=60_000_000, table_mode=0, jd=None):
     """table_mode 0: the build picks its table (dense direct table for these
     keys); 1: the hash table."""
@@ -94,26 +117,17 @@ def bench_join(steps, warmup, build_rows=1_500_000, probe_rows=60_000_000, table
 
     for _ in range(warmup):
         step()
-    lib().qdb_gpu_synchronize()
-    lib().qdb_gpu_timing_reset()
-    lib().qdb_gpu_timing_enable(1)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        step()
-    lib().qdb_gpu_synchronize()
-    t1 = time.perf_counter()
-    lib().qdb_gpu_timing_enable(0)
+    ms_per_step, ms_events = run_steps(step, steps)
     probe_ms, launches = timing("join_probe")
     build_ms, nb = timing("join_build")
     place_ms, npl = timing("join_build_place")
     build_ms = (build_ms * nb + place_ms * npl) / steps
-    ms_per_step = (t1 - t0) * 1e3 / steps
     pairs = results["pairs"]
     probe_bytes = probe_rows * 16 + pairs * 16
     achieved = probe_bytes / (probe_ms * 1e-3) / 1e9 if probe_ms else 0.0
     lib().qdb_gpu_timing_reset()
     return {
-        "rows": build_rows + probe_rows, "ms_per_step": ms_per_step,
+        "rows": build_rows + probe_rows, "ms_per_step": ms_per_step, "ms_per_step_with_events": ms_events,
         "rows_per_s": (build_rows + probe_rows) / (ms_per_step * 1e-3),
         "probe_kernel_ms": probe_ms, "probe_launches": launches, "build_kernel_ms": build_ms,
         "pairs": pairs, "count": results["count"], "sum_value_unscaled": str(results["sum"]),
@@ -138,19 +152,11 @@ def bench_lineitem(steps, warmup, rows=6_000_000):
 
     for _ in range(warmup):
         step()
-    lib().qdb_gpu_timing_reset()
-    lib().qdb_gpu_timing_enable(1)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        step()
-    lib().qdb_gpu_synchronize()
-    t1 = time.perf_counter()
-    lib().qdb_gpu_timing_enable(0)
+    ms, ms_events = run_steps(step, steps)
     k_ms = timing_per_step("agg_update", steps)
     lib().qdb_gpu_timing_reset()
-    ms = (t1 - t0) * 1e3 / steps
     nbytes = rows * (1 + 1 + 4 + 8 * 4)
-    return {"rows": rows, "ms_per_step": ms, "rows_per_s": rows / (ms * 1e-3), "kernel_ms": k_ms,
+    return {"rows": rows, "ms_per_step": ms, "ms_per_step_with_events": ms_events, "rows_per_s": rows / (ms * 1e-3), "kernel_ms": k_ms,
             "algorithmic_bytes": nbytes, "achieved_gbs": nbytes / (k_ms * 1e-3) / 1e9 if k_ms else 0.0,
             "groups": len(out["rows"])}
 
@@ -173,25 +179,16 @@ def bench_project(steps, warmup, rows=60_000_000):
 
     for _ in range(warmup):
         step()
-    lib().qdb_gpu_synchronize()
-    lib().qdb_gpu_timing_reset()
-    lib().qdb_gpu_timing_enable(1)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        step()
-    lib().qdb_gpu_synchronize()
-    t1 = time.perf_counter()
-    lib().qdb_gpu_timing_enable(0)
+    ms, ms_events = run_steps(step, steps)
     err = lib().qdb_gpu_last_error()
     if err:
         raise RuntimeError(f"project: {err.decode()}")
     k_ms = timing_per_step("project", steps)
     lib().qdb_gpu_timing_reset()
     del bufs
-    ms = (t1 - t0) * 1e3 / steps
     nbytes = rows * (8 * 3 + 16 * 2)
-    return {"rows": rows, "ms_per_step": ms, "rows_per_s": rows / (ms * 1e-3), "kernel_ms": k_ms,
-            "algorithmic_bytes": nbytes, "achieved_gbs": nbytes / (k_ms * 1e-3) / 1e9 if k_ms else 0.0}
+    return {"rows": rows, "ms_per_step": ms, "ms_per_step_with_events": ms_events, "rows_per_s": rows / (ms * 1e-3),
+            "kernel_ms": k_ms, "algorithmic_bytes": nbytes, "achieved_gbs": nbytes / (k_ms * 1e-3) / 1e9 if k_ms else 0.0}
 
 
 def bench_groups(steps, warmup, rows=100_000_000, distinct=10_000_000):
@@ -208,25 +205,34 @@ def bench_groups(steps, warmup, rows=100_000_000, distinct=10_000_000):
         out["groups"] = p.finish_in_hbm()
         p.close()
 
+    # The first warm-up query tunes the shape (the autotuner times its
+    # candidate configurations on scratch tables and caches the fastest).
+    was_tuning = lib().qdb_gpu_autotune_enable(1)
     for _ in range(warmup):
         step()
-    lib().qdb_gpu_timing_reset()
-    lib().qdb_gpu_timing_enable(1)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        step()
-    lib().qdb_gpu_synchronize()
-    t1 = time.perf_counter()
-    lib().qdb_gpu_timing_enable(0)
+    lib().qdb_gpu_autotune_enable(was_tuning)
+    tuned = tuned_configs()
+    ms, ms_events = run_steps(step, steps)
     k_ms = timing_per_step("agg_update", steps)
     f_ms = timing_per_step("agg_finalize", steps)  # occupancy scan, then the output rows
     lib().qdb_gpu_timing_reset()
-    ms = (t1 - t0) * 1e3 / steps
     groups = out["groups"]
     nbytes = rows * 24 + groups * 48
-    return {"rows": rows, "ms_per_step": ms, "rows_per_s": rows / (ms * 1e-3), "update_kernel_ms": k_ms,
+    return {"rows": rows, "ms_per_step": ms, "ms_per_step_with_events": ms_events, "rows_per_s": rows / (ms * 1e-3),
+            "update_kernel_ms": k_ms,
             "finalize_kernel_ms": f_ms, "groups": groups, "algorithmic_bytes": nbytes,
-            "achieved_gbs": nbytes / ((k_ms + f_ms) * 1e-3) / 1e9 if k_ms else 0.0}
+            "achieved_gbs": nbytes / ((k_ms + f_ms) * 1e-3) / 1e9 if k_ms else 0.0, "autotuned": tuned}
+
+
+def tuned_configs():
+    """The autotuner's cache: the configuration chosen per shape bucket."""
+    from qumirdb_amd import abi, lib
+    n = lib().qdb_gpu_autotune_entries(None, 0)
+    ents = (abi.QdbTuneEntry * max(1, n))()
+    lib().qdb_gpu_autotune_entries(ents, n)
+    return [{"key_words": e.KeyWords, "card_bucket": e.CardBucket, "rows_bucket": e.RowsBucket,
+             "config": ("bucketed, fan-out x%g, partition over <= %d workgroups" % (2.0 ** e.FanoutShift, e.PartBlocks or 2048))
+             if e.Path == 1 else f"LDS {e.LdsKiB} KiB x {e.GridPerCu}/CU", "ms": round(e.Ms, 3)} for e in ents[:n]]
 
 
 def bench_batched(n_groupby=256, n_join=256):
@@ -362,13 +368,7 @@ def bench_batched(n_groupby=256, n_join=256):
         if a != b:
             raise RuntimeError(f"join instance {k}: pairs or fused results differ between plain and batched runs")
         pairs += a[0]
-    from qumirdb_amd import abi
-    ntune = lib().qdb_gpu_autotune_entries(None, 0)
-    ents = (abi.QdbTuneEntry * max(1, ntune))()
-    lib().qdb_gpu_autotune_entries(ents, ntune)
-    tuned = [{"key_words": e.KeyWords, "card_bucket": e.CardBucket, "rows_bucket": e.RowsBucket,
-              "config": ("bucketed, fan-out x%g" % (2.0 ** e.FanoutShift)) if e.Path == 1 else
-              f"LDS {e.LdsKiB} KiB x {e.GridPerCu}/CU", "ms": round(e.Ms, 3)} for e in ents[:ntune]]
+    tuned = tuned_configs()
     n = n_groupby + n_join
     rows = sum(r for r, _, _ in inst) + sum(b + p for b, p in jinst)
     return {"instances": n, "groupby_instances": n_groupby, "join_instances": n_join, "rows": rows, "groups": groups,
@@ -381,7 +381,7 @@ def bench_batched(n_groupby=256, n_join=256):
             "autotuned_shapes": tuned}
 
 
-def bench_served(qps_levels=(50, 100, 200, 500, 1000, 2000), n_requests=120):
+def bench_served(qps_levels=(50, 200, 1000, 2000, 4000, 8000), n_requests=300):
     """The served form: Poisson request streams at each offered QPS over the
     resident tables (lineitem 6M rows, join build 1.5M / probe 60M rows,
     groups 100M rows; each request one shape over a log-uniform 1%..100%

@@ -1012,6 +1012,26 @@ void emit_stage(std::string& o, const AggPlan* P)
     }
     if (L.valid_word >= 0) o += "        st[" + std::to_string(L.valid_word) + "] = vb;\n";
     o += "        (void)vb;\n        return true;\n    }\n";
+    // stage_key(): the row's key alone (the partition's counting pass reads
+    // only the key and predicate columns).
+    o += "    template <int KW>\n";
+    o += "    __device__ static __forceinline__ bool stage_key(const AggArgs& A, int64_t row, uint64_t (&key)[KW])\n    {\n";
+    o += "        const DevCols& C = A.cols;\n";
+    o += "#pragma unroll\n        for (int w = 0; w < KW; ++w) key[w] = 0;\n";
+    o += "        if (A.sel && !A.sel[row]) return false;\n";
+    if (P->has_pred) {
+        o += "        {\n";
+        const std::string r = emit_program(o, "p", P->h_pred, P);
+        o += "    if (!(" + r + ".valid && " + r + ".i != 0)) return false;\n        }\n";
+    }
+    for (int f = 0; f < A.nkeys; ++f) {
+        const KeyFieldDesc& k = A.keys[f];
+        char b[300];
+        snprintf(b, sizeof(b), "        pack_key_field<KW>(KeyFieldDesc{%d, %d, %d, %d, %d, %d, %d, %d}, C.c[%d], row, key);\n", k.slot, k.kind,
+            k.width, k.nullable, k.word, k.shift, k.null_word, k.null_bit, k.slot);
+        o += b;
+    }
+    o += "        return true;\n    }\n";
     o += "    template <int KW, int S>\n";
     o += "    __device__ static __forceinline__ void unstage(const AggArgs& A, const uint64_t (&st)[kStage], uint64_t (&key)[KW], uint64_t (&c)[S])\n    {\n";
     o += "#pragma unroll\n        for (int w = 0; w < KW; ++w) key[w] = st[w];\n";
@@ -1515,7 +1535,7 @@ bool sorted_path_wanted(AggPlan* P, const AggTable* T, int64_t rows)
 // The bucketed update of rows [0, n) of a bound batch (n <= 2^30): stage,
 // sort by bucket, aggregate every bucket in LDS into records, grow the
 // table for them, merge them into the HBM table.
-bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n, int fanout_shift = 0)
+bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n, int fanout_shift = 0, int part_blocks = 0)
 {
     AggPlan* P = T->plan;
     // An empty table whose buffers are not made yet keeps the groups of the
@@ -1561,8 +1581,10 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n, int fanou
     // Partition: per-workgroup bucket counts over contiguous row ranges, a
     // scan, then every row's staged words at its bucket run's next slot.
     const unsigned int nb = B + 1;  // the last bucket takes rows that do not take part
+    // Partition workgroups: one per 16K rows, at most 2048 (or the tuned count).
+    const int64_t max_blk = part_blocks > 0 ? part_blocks : 2048;
     int64_t nblk = (n + 16383) / 16384;
-    if (nblk > 2048) nblk = 2048;
+    if (nblk > max_blk) nblk = max_blk;
     long long chunk = (long long)((n + nblk - 1) / nblk);
     nblk = (n + chunk - 1) / chunk;
     const size_t nhist = (size_t)nb * (size_t)nblk;
@@ -1771,7 +1793,7 @@ bool bucketed_possible(AggPlan* P, int64_t rows)
 // workgroups per CU.
 TuneConfig default_config(AggPlan* P, const AggTable* T, int64_t rows)
 {
-    TuneConfig c = {0, 64, 2, 0};
+    TuneConfig c = {0, 64, 2, 0, 0};
     if (rows <= ((int64_t)1 << 30) && sorted_path_wanted(P, T, rows)) c.path = 1;
     return c;
 }
@@ -1786,7 +1808,7 @@ bool table_update_cfg(AggTable* T, const TRowSet* batch, const TuneConfig& cfg)
     if (cfg.path == 1 && bucketed_possible(P, batch->RowCount)) {
         AggArgs bound;
         if (!bind_batch(P, batch, &bound)) return false;
-        return table_update_sorted(T, bound, batch->RowCount, cfg.fanout_shift);
+        return table_update_sorted(T, bound, batch->RowCount, cfg.fanout_shift, cfg.part_blocks);
     }
     AggArgs bound;
     if (!bind_batch(P, batch, &bound)) return false;
@@ -2106,13 +2128,13 @@ TuneConfig choose_config(AggTable* T, const TRowSet* batch)
     const bool bucketed_ok = bucketed_possible(P, rows);
     TuneConfig cfg;
     if (tune_forced(&cfg)) {
-        if (cfg.path == 1 && !bucketed_ok) cfg = TuneConfig{0, 64, 2, 0};
+        if (cfg.path == 1 && !bucketed_ok) cfg = TuneConfig{0, 64, 2, 0, 0};
         return cfg;
     }
     const uint64_t slots = T->cap_hint > T->capacity ? T->cap_hint : T->capacity;
     const TuneKey key = tune_key(P->KW, slots, rows);
     if (tune_lookup(key, &cfg)) {
-        if (cfg.path == 1 && !bucketed_ok) cfg = TuneConfig{0, 64, 2, 0};
+        if (cfg.path == 1 && !bucketed_ok) cfg = TuneConfig{0, 64, 2, 0, 0};
         return cfg;
     }
     if (!tune_enabled() || rows <= 0) return default_config(P, T, rows);

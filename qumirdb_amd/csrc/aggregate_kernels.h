@@ -866,23 +866,18 @@ __device__ __forceinline__ void agg_count_body(const AggArgs& A, unsigned int* h
     // kStageUnroll rows per thread per step, staged (loads issued) before
     // any is used, so that their memory latencies overlap.
     for (int64_t i0 = r0 + threadIdx.x; i0 < r1; i0 += (int64_t)blockDim.x * kStageUnroll) {
-        uint64_t st[kStageUnroll][W];
+        // Only the key (and predicate) columns are read here.
+        uint64_t key[kStageUnroll][KW];
         bool ok[kStageUnroll];
 #pragma unroll
         for (int u = 0; u < kStageUnroll; ++u) {
             const int64_t i = i0 + (int64_t)u * blockDim.x;
-            ok[u] = i < r1 && P::template stage<KW>(A, A.row_base + (i < r1 ? i : r0), st[u]);
+            ok[u] = i < r1 && P::template stage_key<KW>(A, A.row_base + (i < r1 ? i : r0), key[u]);
         }
 #pragma unroll
         for (int u = 0; u < kStageUnroll; ++u) {
             if (i0 + (int64_t)u * blockDim.x >= r1) continue;
-            unsigned int b = nb - 1u;
-            if (ok[u]) {
-                uint64_t key[KW];
-#pragma unroll
-                for (int w = 0; w < KW; ++w) key[w] = st[u][w];
-                b = (unsigned int)(key_place_hash<KW>(key) >> shift);
-            }
+            const unsigned int b = ok[u] ? (unsigned int)(key_place_hash<KW>(key[u]) >> shift) : nb - 1u;
             atomicAdd(&lh[b], 1u);
         }
     }

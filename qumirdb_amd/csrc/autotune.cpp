@@ -79,7 +79,9 @@ bool tune_forced(TuneConfig* cfg)
     const char* e = getenv("QDB_GPU_TUNE_FORCE");
     if (!e || !*e) return false;
     TuneConfig c;
-    if (sscanf(e, "%d,%d,%d,%d", &c.path, &c.lds_kib, &c.grid_per_cu, &c.fanout_shift) != 4) return false;
+    c.part_blocks = 0;
+    const int got = sscanf(e, "%d,%d,%d,%d,%d", &c.path, &c.lds_kib, &c.grid_per_cu, &c.fanout_shift, &c.part_blocks);
+    if (got < 4) return false;
     *cfg = c;
     return true;
 }
@@ -89,14 +91,18 @@ std::vector<TuneConfig> tune_candidates(bool bucketed_ok)
     // LDS budget x workgroups per CU within the 160 KiB of a CU, and the
     // bucket count at 1/2x, 1x and 2x the sizing rule's.
     std::vector<TuneConfig> v = {
-        {0, 32, 4, 0},
-        {0, 64, 2, 0},
-        {0, 128, 1, 0},
+        {0, 32, 4, 0, 0},
+        {0, 64, 2, 0, 0},
+        {0, 128, 1, 0, 0},
     };
     if (bucketed_ok) {
-        v.push_back({1, 0, 0, -1});
-        v.push_back({1, 0, 0, 0});
-        v.push_back({1, 0, 0, 1});
+        // and the partition at up to 2048 or 512 workgroups (fewer, longer
+        // row ranges per workgroup: longer runs per bucket).
+        for (int blocks : {2048, 512}) {
+            v.push_back({1, 0, 0, -1, blocks});
+            v.push_back({1, 0, 0, 0, blocks});
+            v.push_back({1, 0, 0, 1, blocks});
+        }
     }
     return v;
 }
@@ -128,6 +134,7 @@ extern "C" int64_t qdb_gpu_autotune_entries(QdbTuneEntry* out, int64_t max)
             e.LdsKiB = kv.second.cfg.lds_kib;
             e.GridPerCu = kv.second.cfg.grid_per_cu;
             e.FanoutShift = kv.second.cfg.fanout_shift;
+            e.PartBlocks = kv.second.cfg.part_blocks;
             e.Ms = kv.second.ms;
         }
         ++n;
@@ -147,11 +154,11 @@ extern "C" int qdb_gpu_autotune_save(const char* path)
     std::lock_guard<std::mutex> lock(g_mu);
     FILE* f = fopen(path, "w");
     if (!f) return -1;
-    fprintf(f, "# kw card_bucket rows_bucket path lds_kib grid_per_cu fanout_shift ms\n");
+    fprintf(f, "# kw card_bucket rows_bucket path lds_kib grid_per_cu fanout_shift part_blocks ms\n");
     for (const auto& kv : cache()) {
         const TuneConfig& c = kv.second.cfg;
-        fprintf(f, "%d %d %d %d %d %d %d %.6f\n", std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), c.path,
-            c.lds_kib, c.grid_per_cu, c.fanout_shift, kv.second.ms);
+        fprintf(f, "%d %d %d %d %d %d %d %d %.6f\n", std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), c.path,
+            c.lds_kib, c.grid_per_cu, c.fanout_shift, c.part_blocks, kv.second.ms);
     }
     return fclose(f) == 0 ? 0 : -1;
 }
@@ -168,13 +175,13 @@ extern "C" int64_t qdb_gpu_autotune_load(const char* path)
         TuneKey k;
         TuneConfig c;
         double ms = 0;
-        if (sscanf(line, "%d %d %d %d %d %d %d %lf", &k.kw, &k.card_bucket, &k.rows_bucket, &c.path, &c.lds_kib, &c.grid_per_cu,
-                &c.fanout_shift, &ms) != 8) {
+        if (sscanf(line, "%d %d %d %d %d %d %d %d %lf", &k.kw, &k.card_bucket, &k.rows_bucket, &c.path, &c.lds_kib, &c.grid_per_cu,
+                &c.fanout_shift, &c.part_blocks, &ms) != 9) {
             continue;
         }
         const bool valid = (c.path == 0 && (c.lds_kib == 32 || c.lds_kib == 64 || c.lds_kib == 128) && c.grid_per_cu >= 1 &&
                                c.grid_per_cu <= 8) ||
-            (c.path == 1 && c.fanout_shift >= -2 && c.fanout_shift <= 2);
+            (c.path == 1 && c.fanout_shift >= -2 && c.fanout_shift <= 2 && c.part_blocks >= 0 && c.part_blocks <= 2048);
         if (!valid) continue;
         tune_store(k, c, ms);
         ++n;
