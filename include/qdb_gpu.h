@@ -697,6 +697,11 @@ int qdb_gpu_join_build_batched(const QdbRowJoinKernels* kernels, void* const* ar
  * batches per replay. Returns 0, or -1 with the error text. */
 int qdb_gpu_join_probe_detached(const QdbRowJoinKernels* kernels, void* arena, const TRowSet* const* batches,
                                 QdbPairBuffer* const* pairs, int64_t n, int64_t* counts, QdbFusedResult* fused);
+/* Diagnostics of a join's table (no counterpart in the reference): [built,
+ * dense direct table, dense key range, bits per packed ordinal (0: not
+ * packed), rows kept, rows appended]; returns how many values were written,
+ * or -1. */
+int64_t qdb_gpu_join_table_info(void* arena, int64_t* out, int64_t n);
 /* Makes the calling thread issue its work on library stream `index`
  * (0 = the default library stream, up to 3 more, all at the default
  * priority). Device blocks freed on one stream are reused by another only

@@ -232,7 +232,7 @@ EXPORTS = [
     "qdb_gpu_join_build_batched", "qdb_gpu_join_probe_batched", "qdb_gpu_join_processor_build_batched",
     "qdb_gpu_join_processor_probe_batched", "qdb_gpu_pairs_checksum",
     "qdb_gpu_autotune_enable", "qdb_gpu_autotune_entries", "qdb_gpu_autotune_clear", "qdb_gpu_autotune_save",
-    "qdb_gpu_autotune_load", "qdb_gpu_join_probe_detached", "qdb_gpu_set_thread_stream",
+    "qdb_gpu_autotune_load", "qdb_gpu_join_probe_detached", "qdb_gpu_set_thread_stream", "qdb_gpu_join_table_info",
 ]
 
 
@@ -338,6 +338,7 @@ def declare(lib: C.CDLL) -> C.CDLL:
                                                   C.POINTER(C.POINTER(QdbPairBuffer)), i64, C.POINTER(i64),
                                                   C.POINTER(QdbFusedResult)]),
         "qdb_gpu_set_thread_stream": (C.c_int, [C.c_int]),
+        "qdb_gpu_join_table_info": (i64, [P, C.POINTER(i64), i64]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

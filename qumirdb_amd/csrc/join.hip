@@ -631,6 +631,27 @@ __global__ void __launch_bounds__(kBlock) k_dense_fill(const uint64_t* keys, con
     }
 }
 
+// The ordinal table packed at w bits per key (w < 32; kAbsent becomes all
+// ones): thread per output word j assembles the entries that overlap bits
+// [32 j, 32 j + 32). The last word is slack for two-word reads.
+__global__ void __launch_bounds__(kBlock) k_dense_pack(const uint32_t* dtab, uint64_t range, uint32_t w, uint32_t* pack,
+    uint64_t nwords)
+{
+    const uint64_t mask = (1ull << w) - 1ull;
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nwords; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t s = j * 32;
+        uint64_t word = 0;
+        const uint64_t e1 = (s + 31) / w;
+        for (uint64_t e = s / w; e <= e1 && e < range; ++e) {
+            const uint32_t o = dtab[e];
+            const uint64_t v = o == kAbsent ? mask : (uint64_t)o;
+            const uint64_t at = e * w;
+            word |= at >= s ? (v << (at - s)) : (v >> (s - at));
+        }
+        pack[j] = (uint32_t)word;
+    }
+}
+
 // One payload column (and its validity bytes) gathered into CSR order.
 __global__ void __launch_bounds__(kBlock) k_permute_column(const uint32_t* perm, int64_t rows, const char* src, char* dst,
     int width, const uint8_t* vsrc, uint8_t* vdst)
@@ -702,6 +723,11 @@ struct ProbeArgs {
     // workgroup adds its words to fused_acc with atomics instead.
     unsigned long long* fused_part;
     unsigned long long* fused_acc;
+    // dense, lean probe: the ordinals packed in dpack_w bits each (all ones:
+    // no build row), or null
+    const uint32_t* dpack;
+    uint32_t dpack_w;
+    uint32_t pad3;
 };
 
 // Probe kernel modes: kGeneric reads keys through the column descriptors;
@@ -1148,7 +1174,19 @@ __device__ __forceinline__ void lean_load(const ProbeArgs& P, uint64_t tl, int l
 // One wave tile (rows tl + r * 64 + lane; those before t0 left out) whose
 // keys and probe column are in key/val; loads the tile at ntl into
 // nkey/nval on the way.
-template <bool kVal, bool kMinMax, bool kLeft>
+// The ordinal of dense index ix from the packed table: w bits at bit
+// ix * w, read as two adjacent words (the table has one word of slack).
+__device__ __forceinline__ uint32_t packed_ordinal(const uint32_t* pack, uint32_t w, uint32_t ix)
+{
+    const uint64_t bit = (uint64_t)ix * w;
+    const uint64_t wi = bit >> 5;
+    const uint64_t two = (uint64_t)pack[wi] | ((uint64_t)pack[wi + 1] << 32);
+    const uint64_t mask = (1ull << w) - 1ull;
+    const uint64_t v = (two >> (bit & 31u)) & mask;
+    return v == mask ? kAbsent : (uint32_t)v;
+}
+
+template <bool kVal, bool kMinMax, bool kLeft, bool kPacked>
 __device__ __forceinline__ void lean_tile(const ProbeArgs& P, const uint32_t* lbits, uint32_t lds_range, int wave, int lane,
     uint64_t t0, uint64_t tl, const uint64_t (&key)[kLeanRows], const int64_t (&val)[kLeanRows], uint64_t ntl,
     uint64_t (&nkey)[kLeanRows], int64_t (&nval)[kLeanRows], LeanState& st)
@@ -1178,7 +1216,7 @@ __device__ __forceinline__ void lean_tile(const ProbeArgs& P, const uint32_t* lb
         need[r] = inr[r] & (!cov[r] | bit);
         // Only the lanes that may match issue the gather.
         head[r] = kAbsent;
-        if (need[r]) head[r] = P.dtab[ixs[r]];
+        if (need[r]) head[r] = kPacked ? packed_ordinal(P.dpack, P.dpack_w, ixs[r]) : P.dtab[ixs[r]];
     }
     __builtin_amdgcn_sched_barrier(0);
     lean_load<kVal>(P, ntl, lane, nkey, nval);
@@ -1228,7 +1266,7 @@ __device__ __forceinline__ void lean_tile(const ProbeArgs& P, const uint32_t* lb
     }
 }
 
-template <bool kVal, bool kMinMax, bool kLeft>
+template <bool kVal, bool kMinMax, bool kLeft, bool kPacked>
 __global__ void __launch_bounds__(kProbeBlock) k_probe_dense_lean(ProbeArgs P)
 {
     extern __shared__ __align__(16) uint64_t lds64[];
@@ -1258,10 +1296,12 @@ __global__ void __launch_bounds__(kProbeBlock) k_probe_dense_lean(ProbeArgs P)
     lean_load<kVal>(P, lean_base(b0 + woff, n), lane, ka, va);
     while (b0 < n) {
         const uint64_t t0 = b0 + woff, t1 = b0 + stride + woff, t2 = b0 + 2 * stride + woff;
-        lean_tile<kVal, kMinMax, kLeft>(P, lbits, lds_range, wave, lane, t0, lean_base(t0, n), ka, va, lean_base(t1, n), kb, vb, st);
+        lean_tile<kVal, kMinMax, kLeft, kPacked>(P, lbits, lds_range, wave, lane, t0, lean_base(t0, n), ka, va, lean_base(t1, n), kb, vb,
+            st);
         b0 += stride;
         if (b0 >= n) break;
-        lean_tile<kVal, kMinMax, kLeft>(P, lbits, lds_range, wave, lane, t1, lean_base(t1, n), kb, vb, lean_base(t2, n), ka, va, st);
+        lean_tile<kVal, kMinMax, kLeft, kPacked>(P, lbits, lds_range, wave, lane, t1, lean_base(t1, n), kb, vb, lean_base(t2, n), ka, va,
+            st);
         b0 += stride;
     }
     if (P.nfused) {
@@ -1759,6 +1799,8 @@ struct JoinBuild {
     uint64_t dmin = 0, drange = 0;
     uint32_t* dbits = nullptr;
     uint32_t* dtab = nullptr;
+    uint32_t* dpack = nullptr;    // dense: the ordinals at dpack_w bits (large ranges)
+    uint32_t dpack_w = 0;
     uint32_t bloom_words = 0;
     int64_t keys_distinct = 0;
     int64_t max_key_rows = 0;   // most build rows of one key (a probe row's most matches)
@@ -1980,6 +2022,7 @@ void build_free(JoinBuild* J)
     dfree(J->bloom);
     dfree(J->dbits);
     dfree(J->dtab);
+    dfree(J->dpack);
     dfree(J->pair_count);
     dfree(J->rows_dev);
     hfree_pinned(J->readback);
@@ -2110,6 +2153,8 @@ bool read_rows(JoinBuild* J)
     return true;
 }
 
+void dense_pack(JoinBuild* J);
+
 // Frees the dense table's buffers.
 void dense_drop(JoinBuild* J)
 {
@@ -2189,7 +2234,41 @@ bool dense_build_finish(JoinBuild* J, bool ok)
     J->keys_distinct = J->rows;
     J->max_key_rows = J->rows > 0 ? 1 : 0;
     J->built = true;
+    dense_pack(J);
     return true;
+}
+
+// Ordinal tables larger than this are packed for the lean probe: at
+// ceil(log2(appended rows + 1)) bits per key the table of the 1.5M-key
+// range of 1.5M appended build rows takes 3.9 MB instead of 6 MB, within
+// one XCD's 4 MB L2 (QDB_GPU_JOIN_PACK=0: never).
+constexpr uint64_t kPackMinTableBytes = (uint64_t)2 << 20;
+
+void dense_pack(JoinBuild* J)
+{
+    const char* e = getenv("QDB_GPU_JOIN_PACK");
+    if (e && e[0] == '0') return;
+    if (J->drange * 4 <= kPackMinTableBytes) return;
+    // Ordinals are appended-row indices (rows a selection left out count
+    // too): w bits hold every ordinal below J->appended and the all-ones
+    // absent marker.
+    uint32_t w = 1;
+    while (w < 32 && ((uint64_t)1 << w) <= (uint64_t)J->appended) ++w;
+    if (w >= 25) return;  // too little to gain
+    const uint64_t nwords = (J->drange * w + 31) / 32 + 1;
+    J->dpack = (uint32_t*)dalloc(nwords * 4);
+    if (!J->dpack) {
+        clear_error();
+        return;
+    }
+    J->dpack_w = w;
+    hipLaunchKernelGGL(k_dense_pack, dim3(grid_for((int64_t)nwords, kBlock, 4)), dim3(kBlock), 0, stream(), J->dtab, J->drange, w,
+        J->dpack, nwords);
+    if (hipGetLastError() != hipSuccess) {
+        dfree(J->dpack);
+        J->dpack = nullptr;
+        J->dpack_w = 0;
+    }
 }
 
 bool try_dense_build(JoinBuild* J)
@@ -2449,11 +2528,11 @@ void launch_dense_one(int grid, size_t lds, const ProbeArgs& A)
     hipLaunchKernelGGL((k_probe_dense<kMode>), dim3(grid), dim3(kProbeBlock), lds, stream(), A);
 }
 
-template <bool kVal, bool kMinMax, bool kLeft>
+template <bool kVal, bool kMinMax, bool kLeft, bool kPacked>
 void launch_lean_one(int grid, size_t lds, const ProbeArgs& A)
 {
     static bool attr_set = false;
-    const void* fn = (const void*)k_probe_dense_lean<kVal, kMinMax, kLeft>;
+    const void* fn = (const void*)k_probe_dense_lean<kVal, kMinMax, kLeft, kPacked>;
     static const size_t s_lds = static_lds(fn);  // the kernel's static LDS, read once
     if (!attr_set) {
         (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(max_lds_per_block() - s_lds));
@@ -2464,14 +2543,19 @@ void launch_lean_one(int grid, size_t lds, const ProbeArgs& A)
         fail("dense probe: %zu bytes of LDS exceed the workgroup limit", lds + s_lds);
         return;
     }
-    hipLaunchKernelGGL((k_probe_dense_lean<kVal, kMinMax, kLeft>), dim3(grid), dim3(kProbeBlock), lds, stream(), A);
+    hipLaunchKernelGGL((k_probe_dense_lean<kVal, kMinMax, kLeft, kPacked>), dim3(grid), dim3(kProbeBlock), lds, stream(), A);
 }
 
 template <bool kVal, bool kMinMax>
 void launch_lean_left(bool left, int grid, size_t lds, const ProbeArgs& A)
 {
-    if (left) launch_lean_one<kVal, kMinMax, true>(grid, lds, A);
-    else launch_lean_one<kVal, kMinMax, false>(grid, lds, A);
+    if (A.dpack) {
+        if (left) launch_lean_one<kVal, kMinMax, true, true>(grid, lds, A);
+        else launch_lean_one<kVal, kMinMax, false, true>(grid, lds, A);
+        return;
+    }
+    if (left) launch_lean_one<kVal, kMinMax, true, false>(grid, lds, A);
+    else launch_lean_one<kVal, kMinMax, false, false>(grid, lds, A);
 }
 
 void launch_dense_lean(bool val, bool minmax, bool left, int grid, size_t lds, const ProbeArgs& A)
@@ -2493,9 +2577,12 @@ size_t dense_lds_budget()
     if (cached) return cached;
     size_t s = static_lds((const void*)k_probe_dense<kMode>);
     // The lean forms (same LDS use) fit the same budget.
-    const void* lean[] = {(const void*)k_probe_dense_lean<true, true, true>, (const void*)k_probe_dense_lean<true, true, false>,
-        (const void*)k_probe_dense_lean<true, false, true>, (const void*)k_probe_dense_lean<true, false, false>,
-        (const void*)k_probe_dense_lean<false, false, true>, (const void*)k_probe_dense_lean<false, false, false>};
+    const void* lean[] = {(const void*)k_probe_dense_lean<true, true, true, false>, (const void*)k_probe_dense_lean<true, true, false, false>,
+        (const void*)k_probe_dense_lean<true, false, true, false>, (const void*)k_probe_dense_lean<true, false, false, false>,
+        (const void*)k_probe_dense_lean<false, false, true, false>, (const void*)k_probe_dense_lean<false, false, false, false>,
+        (const void*)k_probe_dense_lean<true, true, true, true>, (const void*)k_probe_dense_lean<true, true, false, true>,
+        (const void*)k_probe_dense_lean<true, false, true, true>, (const void*)k_probe_dense_lean<true, false, false, true>,
+        (const void*)k_probe_dense_lean<false, false, true, true>, (const void*)k_probe_dense_lean<false, false, false, true>};
     for (const void* f : lean) s = s > static_lds(f) ? s : static_lds(f);
     cached = max_lds_per_block() > s ? max_lds_per_block() - s : 0;
     return cached;
@@ -2663,6 +2750,8 @@ bool do_probe(JoinBuild* J, const TRowSet* batch, QdbPairBuffer* pairs, unsigned
         A.drange = J->drange;
         A.dbits = J->dbits;
         A.dtab = J->dtab;
+        A.dpack = J->dpack;
+        A.dpack_w = J->dpack_w;
         lds = (size_t)A.lds_words * 4;
     }
     unsigned long long cap = first_pair_capacity(J, n);
@@ -3460,6 +3549,17 @@ extern "C" int qdb_gpu_join_probe_detached(const QdbRowJoinKernels* kernels, voi
     if (J->plan != P) return fail("the join belongs to another compiled join"), -1;
     if (P->KW != 1) return fail("detached probes take one-word join keys"), -1;
     return detached_probe(P, J, batches, pairs, n, counts, fused) ? 0 : -1;
+}
+
+extern "C" int64_t qdb_gpu_join_table_info(void* arena, int64_t* out, int64_t n)
+{
+    clear_error();
+    JoinBuild* J = arena ? ((RowArenaBlob*)arena)->build : nullptr;
+    if (!J || !out) return fail("join table info: no join"), -1;
+    const int64_t v[6] = {J->built ? 1 : 0, J->dense ? 1 : 0, (int64_t)J->drange, (int64_t)J->dpack_w, J->rows, J->appended};
+    int64_t k = 0;
+    for (; k < n && k < 6; ++k) out[k] = v[k];
+    return k;
 }
 
 extern "C" int qdb_gpu_join_probe_fused_noalloc(void* arena, const TRowSet* batch, QdbFusedResult* out, int64_t max_out,

@@ -811,6 +811,13 @@ class TRowJoinProcessor:
             out.append((int(counts[i]), res))
         return out
 
+    def table_info(self) -> dict:
+        """The built table: dense or hash, its key range, packed ordinal bits."""
+        v = (C.c_int64 * 6)()
+        lib().qdb_gpu_join_table_info(lib().qdb_gpu_join_processor_arena(self.p), v, 6)
+        return {"built": bool(v[0]), "dense": bool(v[1]), "range": int(v[2]), "packed_bits": int(v[3]), "rows": int(v[4]),
+                "appended": int(v[5])}
+
     def pair_count(self) -> int:
         return int(lib().qdb_gpu_join_processor_pairs(self.p).contents.Count)
 

@@ -327,6 +327,48 @@ def test_join_distinct_dense_keys_vs_oracle(gpu_backend, join_type, table_mode):
     assert fused[3]["value"] == int(pval[prow].sum(dtype=np.int64))
 
 
+@pytest.mark.parametrize("nb", [600_000, 2_100_000])
+def test_join_dense_packed_ordinals_equal_unpacked(gpu_backend, monkeypatch, nb):
+    """A dense key range above 512K keys gets its ordinals packed at
+    ceil(log2(appended rows + 1)) bits (20 and 22 bits here, entries
+    straddling word boundaries; a build selection leaves rows out, so the
+    ordinals run past the kept row count) for the lean probe: pairs and
+    fused results equal the
+    unpacked table's (QDB_GPU_JOIN_PACK=0) and the oracle's, with probe keys
+    below, inside and above the range."""
+    rng = np.random.default_rng(nb)
+    types_b = [T(SIGNED, 8), T(SIGNED, 4)]
+    types_p = [T(SIGNED, 8), T(SIGNED, 8)]
+    # Non-negative keys: the dense table spans the keys' unsigned range.
+    bk = rng.permutation(nb).astype(np.int64) + 1234
+    bsel = (rng.random(nb) > 0.4).astype(np.uint8)
+    npr = 2_000_003
+    pk = rng.integers(-2000, nb + 4000, npr).astype(np.int64)
+    pval = rng.integers(-10 ** 12, 10 ** 12, npr).astype(np.int64)
+    case = JoinCase(types_b, types_p, [0], [0], probe_is_left=True, fused=[(AGG_COUNT_STAR, -1), (AGG_SUM, 1)])
+    build = Batch(types_b, [bk, rng.integers(0, 5, nb).astype(np.int32)], None, bsel)
+    probe = Batch(types_p, [pk, pval])
+    got = {}
+    for pack in ("1", "0"):
+        monkeypatch.setenv("QDB_GPU_JOIN_PACK", pack)
+        kern, proc = gpu_backend.join_processor(case)
+        proc.append(gpu_backend.dev(build))
+        proc.build()
+        info = proc.table_info()
+        bits = int(nb).bit_length()   # ordinals index the appended rows
+        assert info["dense"] and info["appended"] == nb and info["packed_bits"] == (bits if pack == "1" else 0), info
+        pairs = proc.probe_pairs(gpu_backend.dev(probe))
+        got[pack] = (pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))], [f["value"] for f in proc.fused_results()])
+        proc.close()
+    oj = O.OracleJoin(case.to_c())
+    oj.append(OracleBackend().host(build))
+    oj.build()
+    opairs = oj.probe(OracleBackend().host(probe))
+    opairs = opairs[np.lexsort((opairs[:, 1], opairs[:, 0]))]
+    assert np.array_equal(got["1"][0], opairs) and np.array_equal(got["0"][0], opairs)
+    assert got["1"][1] == got["0"][1] == [len(opairs), int(pval[(opairs[:, 0] + (1 << 32)).astype(np.int64)].sum())]
+
+
 @pytest.mark.parametrize("probe_is_left", [True, False])
 @pytest.mark.parametrize("fused", ["count", "sum", "minmax"])
 @pytest.mark.parametrize("npr", [1024, 300_007])
