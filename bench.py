@@ -244,17 +244,20 @@ def bench_batched(n_groupby=256, n_join=256):
     one after another (group-by: table set-up, update, finish into HBM; join:
     append, build, probe). Batched: every group-by update in one persistent
     launch and every finish behind one synchronization; every join build
-    behind one synchronization and every probe in one persistent launch.
-    Results are compared instance by instance afterwards: group-by outputs
+    behind one synchronization and every probe in one persistent launch;
+    the joins run on a stream of their own beside the group-bys, and the
+    group-by instances on the bucketed path on the library stream and two
+    helper streams. Results are compared instance by instance afterwards: group-by outputs
     in full (downloaded and sorted), joins by pair count, fused COUNT/SUM and
     an order-independent checksum of the pairs."""
     import ctypes
+    import threading
 
     import numpy as np
 
     from qumirdb_amd import lib, workloads as W
     from qumirdb_amd.host import (AggregateKernels, RowJoinKernels, TAggregateProcessor, TRowJoinProcessor,
-                                  download_rowset)
+                                  download_rowset, set_thread_stream)
 
     inst = W.batched_instances(n_groupby)
     jinst = W.batched_join_instances(n_join)
@@ -288,21 +291,36 @@ def bench_batched(n_groupby=256, n_join=256):
             jps.append(jp)
         return outs, jps
 
+    def joins_batched(out):
+        # On a stream of its own, beside the group-bys.
+        try:
+            set_thread_stream(3)
+            jps = [TRowJoinProcessor(jk) for _ in jdev.devs]
+            for jp, jd in zip(jps, jdev.devs):
+                jp.append(jd.build)
+            TRowJoinProcessor.build_batched(jps)
+            TRowJoinProcessor.probe_batched(jps, [jd.probe for jd in jdev.devs])
+            for jp in jps:
+                jp.fused_results()
+            lib().qdb_gpu_synchronize()
+            out.append(jps)
+        except Exception as e:  # re-raised by the caller
+            out.append(e)
+
     def batched():
+        jout = []
+        jt = threading.Thread(target=joins_batched, args=(jout,))
+        jt.start()
         procs = [TAggregateProcessor(kern, initial_capacity=cap) for cap in caps]
         TAggregateProcessor.add_batched(procs, [d.rowset for d in dev.devs])
         raw = TAggregateProcessor._finish_batched_raw(procs)
         outs = [raw[i] for i in range(len(procs))]
         for p in procs:
             p.close()
-        jps = [TRowJoinProcessor(jk) for _ in jdev.devs]
-        for jp, jd in zip(jps, jdev.devs):
-            jp.append(jd.build)
-        TRowJoinProcessor.build_batched(jps)
-        TRowJoinProcessor.probe_batched(jps, [jd.probe for jd in jdev.devs])
-        for jp in jps:
-            jp.fused_results()
-        return outs, jps
+        jt.join()
+        if isinstance(jout[0], Exception):
+            raise jout[0]
+        return outs, jout[0]
 
     def release(res):
         outs, jps = res

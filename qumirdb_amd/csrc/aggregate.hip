@@ -31,7 +31,9 @@
 #include <mutex>
 #include <utility>
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <thread>
 #include <vector>
 
 #include "aggregate_kernels.h"
@@ -1857,6 +1859,69 @@ bool table_update(AggTable* T, const TRowSet* batch)
 // Rows per work item of a batched update.
 constexpr int64_t kBatchItemRows = 32768;
 
+// The batched update's instances that run on their own (the bucketed path,
+// batches above 2^30 rows), largest first. Called from the library stream,
+// they are spread over that stream and up to two more, each driven by a
+// thread of its own (QDB_GPU_BATCH_STREAMS = 1: one after another), so that
+// one instance's host round trips overlap another's kernels; tables are
+// independent, and the pool reuses a block across streams only after its
+// free has completed.
+bool update_own(AggTable* const* tables, const TRowSet* const* batches, const std::vector<TuneConfig>& cfgs,
+    std::vector<int64_t> own)
+{
+    if (own.empty()) return true;
+    std::sort(own.begin(), own.end(), [&](int64_t a, int64_t b) { return batches[a]->RowCount > batches[b]->RowCount; });
+    int helpers = 2;
+    if (const char* e = getenv("QDB_GPU_BATCH_STREAMS")) helpers = atoi(e) - 1;
+    if (helpers > kMaxStreams - 2) helpers = kMaxStreams - 2;
+    if (thread_stream_index() != 0 || own.size() < 2 || helpers < 1) {
+        for (int64_t i : own) {
+            if (!table_update_cfg(tables[i], batches[i], cfgs[(size_t)i])) return false;
+        }
+        return true;
+    }
+    if ((size_t)helpers > own.size() - 1) helpers = (int)own.size() - 1;
+    std::atomic<size_t> next(0);
+    std::atomic<bool> failed(false);
+    std::vector<std::string> errors((size_t)helpers + 1);
+    auto work = [&](int slot) {
+        for (;;) {
+            const size_t k = next.fetch_add(1);
+            if (k >= own.size() || failed.load()) break;
+            const int64_t i = own[k];
+            if (!table_update_cfg(tables[i], batches[i], cfgs[(size_t)i])) {
+                errors[(size_t)slot] = last_error_text().empty() ? std::string("aggregate update failed") : last_error_text();
+                failed.store(true);
+                break;
+            }
+        }
+        if (slot > 0 && !sync("batched update helper") && errors[(size_t)slot].empty()) {
+            errors[(size_t)slot] = last_error_text();
+            failed.store(true);
+        }
+    };
+    std::vector<std::thread> threads;
+    for (int h = 1; h <= helpers; ++h) {
+        threads.emplace_back([&, h]() {
+            if (!set_thread_stream(h)) {
+                errors[(size_t)h] = last_error_text();
+                failed.store(true);
+                return;
+            }
+            work(h);
+        });
+    }
+    work(0);
+    for (auto& t : threads) t.join();
+    if (failed.load()) {
+        for (const std::string& e : errors) {
+            if (!e.empty()) return fail("%s", e.c_str());
+        }
+        return fail("batched update failed");
+    }
+    return true;
+}
+
 // Updates tables[i] with batches[i] for every i in ONE persistent launch:
 // workgroups take (instance, row range) items from a device queue. Leftover
 // rows and records are replayed per table afterwards, as after a plain
@@ -1892,14 +1957,15 @@ bool batched_update(AggPlan* P, AggTable* const* tables, const TRowSet* const* b
     auto release_all = [&]() {
         for (auto& L : launches) L.release();
     };
+    // Very large batches, and those the bucketed path takes, are updated on
+    // their own (after the persistent launch is queued, see below).
+    std::vector<int64_t> own;
     for (int64_t i = 0; i < n && ok; ++i) {
         AggTable* T = tables[i];
         const TRowSet* b = batches[i];
         if (b->RowCount <= 0) continue;
-        // Very large batches, and those the bucketed path takes, are
-        // updated on their own.
         if (b->RowCount > ((int64_t)1 << 30) || cfgs[(size_t)i].path != 0) {
-            ok = table_update_cfg(T, b, cfgs[(size_t)i]);
+            own.push_back(i);
             continue;
         }
         AggArgs bound;
@@ -1931,7 +1997,7 @@ bool batched_update(AggPlan* P, AggTable* const* tables, const TRowSet* const* b
     }
     if (!ok || insts.empty()) {
         release_all();
-        return ok;
+        return ok && update_own(tables, batches, cfgs, own);
     }
     AggArgs* d_insts = (AggArgs*)upload(insts.data(), insts.size() * sizeof(AggArgs));
     BatchItem* d_items = (BatchItem*)upload(items.data(), items.size() * sizeof(BatchItem));
@@ -1953,6 +2019,8 @@ bool batched_update(AggPlan* P, AggTable* const* tables, const TRowSet* const* b
         }
         ok = hip_ok(launched, "agg batch launch") && hip_ok(hipGetLastError(), "agg batch launch");
     }
+    // The instances on their own run while the persistent launch does.
+    ok = ok && update_own(tables, batches, cfgs, own);
     // Every table's counters in one synchronisation.
     std::vector<unsigned int> cnt(launches.size() * 4, 0u);
     for (size_t k = 0; ok && k < launches.size(); ++k) {

@@ -162,6 +162,8 @@ bool set_thread_stream(int index)
     if (index < 0 || index >= kMaxStreams) return fail("stream index %d outside [0, %d)", index, kMaxStreams);
     if (!ensure_init()) return false;
     Runtime& r = R();
+    // A new thread's current device is device 0: select the library's.
+    if (!hip_ok(hipSetDevice(r.device), "hipSetDevice")) return false;
     std::lock_guard<std::mutex> lock(r.mu);
     if (!r.streams[index]) {
         // Default priority, non-blocking with respect to the legacy stream.

@@ -128,8 +128,12 @@ def test_generated_kernel_equals_generic(gpu_backend, monkeypatch):
     assert outs[0] == outs[1]
 
 
-def test_batched_update_equals_plain(gpu_backend):
-    """Many tables of one aggregate updated in one persistent launch, and
+@pytest.mark.parametrize("own_path", [False, True], ids=["queued", "own_streams"])
+def test_batched_update_equals_plain(gpu_backend, monkeypatch, own_path):
+    """Many tables of one aggregate updated in one persistent launch (or,
+    own_streams: every instance forced onto the bucketed path, which runs
+    each instance on its own, spread over the library stream and two helper
+    streams), and
     finished with one synchronization, give exactly the results of one
     update and one finish per table: instances from empty to 300K rows, few
     to many groups, nullable multi-column keys, every reducer; the small
@@ -140,6 +144,10 @@ def test_batched_update_equals_plain(gpu_backend):
     sizes = [0, 1, 7, 100, 2000, 5000, 33000, 65536, 100000, 300000, 50, 40000]
     insts = [random_agg_case(11 + i, n) for i, n in enumerate(sizes)]
     case = insts[0][0]
+    if own_path:
+        # The bucketed path stages every argument: no MIN/MAX of 16-byte decimals.
+        case = cases.AggCase(case.types, case.keys,
+                             [(f, c) for f, c in case.aggs if not (f in (AGG_MIN, AGG_MAX) and case.types[c].width == 16)])
     kern = q.AggregateKernels.__new__(q.AggregateKernels)
     kern.spec = None
     kern.c_spec = gpu_backend.cast(case.to_c())
@@ -156,9 +164,12 @@ def test_batched_update_equals_plain(gpu_backend):
         plain.append(sorted(result_rows(p.finish()), key=sort_key))
         p.close()
     procs = [q.TAggregateProcessor(kern, initial_capacity=cap) for cap in caps]
+    if own_path:
+        monkeypatch.setenv("QDB_GPU_TUNE_FORCE", "1,0,0,0,512")
     q.TAggregateProcessor.add_batched(procs, devs)
     # a second batched round on the same tables accumulates
     q.TAggregateProcessor.add_batched(procs[:3], devs[:3])
+    monkeypatch.delenv("QDB_GPU_TUNE_FORCE", raising=False)
     finished = q.TAggregateProcessor.finish_batched(procs)
     for i, p in enumerate(procs):
         got = sorted(result_rows(finished[i]), key=sort_key)
