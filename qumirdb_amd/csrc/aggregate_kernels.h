@@ -573,6 +573,13 @@ __device__ __forceinline__ void apply_words(const AggArgs& A, LdsTable& T, int w
             if (w >= P::nwords(A)) continue;
             const int op = P::op(A, w);
             if (op == OP_PAIR_MIN || op == OP_PAIR_MAX || op == OP_PAIR_HI || op == OP_LOCK) continue;
+            // A MIN/MAX word only moves one way: a plain read that already
+            // beats the contribution (even a stale one) makes the atomic
+            // unnecessary -- after a group's first rows, most rows.
+            if (op == OP_MIN_I64 || op == OP_MAX_I64 || op == OP_MIN_U64 || op == OP_MAX_U64) {
+                const uint64_t cur = __hip_atomic_load(&r[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (combine((uint8_t)op, cur, c[w]) == cur) continue;
+            }
             atomic_apply_lds((uint8_t)op, &r[w], c[w]);
         }
     } else {
