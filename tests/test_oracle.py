@@ -581,3 +581,72 @@ def test_partial_aggregate_policy_and_cascade(distinct, sample, max_groups, want
     final = sorted(result_rows(be.aggregate(_combine_case(case, ptypes), parts)), key=sort_key)
     direct = sorted(result_rows(be.aggregate(case, batches)), key=sort_key)
     assert final == direct
+
+
+CMP_PY = [("EQ", lambda a, b: a == b), ("NE", lambda a, b: a != b), ("LT", lambda a, b: a < b),
+          ("LE", lambda a, b: a <= b), ("GT", lambda a, b: a > b), ("GE", lambda a, b: a >= b)]
+
+
+@pytest.mark.parametrize("name,fn", CMP_PY, ids=[n for n, _ in CMP_PY])
+def test_comparisons_against_python(name, fn):
+    """Every comparison op of the filter language, pinned against Python's
+    own comparison of the same values under SQL's rule that a comparison
+    with NULL is NULL (and NULL is not TRUE): int64 columns with NULLs and
+    equal pairs, DECIMAL(15,2) against DECIMAL(15,4) after the scale
+    alignment (compared as exact fractions), and f64 without NaN (with -0
+    equal to +0); then NOT and AND/OR over the comparison's NULLs."""
+    from fractions import Fraction
+    E = cases._abi()
+    op = getattr(E, "CMP_" + name)
+    rng = np.random.default_rng(hash(name) % 1000)
+    n = 4000
+    x = rng.integers(-5, 6, n).astype(np.int64)
+    y = rng.integers(-5, 6, n).astype(np.int64)
+    xv, yv = rng.random(n) > 0.2, rng.random(n) > 0.2
+    d1 = rng.integers(-500, 500, n).astype(np.int64)             # DECIMAL(15,2)
+    d2 = (d1 * 100 + rng.integers(-1, 2, n) * rng.integers(0, 2, n)).astype(np.int64)  # DECIMAL(15,4), often equal
+    f = rng.choice([-1.5, -0.0, 0.0, 2.25, 1e300, -1e-300], n)
+    g = rng.choice([-1.5, 0.0, -0.0, 2.25, 1e300, -1e-300], n)
+    types = [T(SIGNED, 8, nullable=True), T(SIGNED, 8, nullable=True), T(DECIMAL, 8, 2, False, 15), T(DECIMAL, 8, 4, False, 15),
+             T(FLOAT, 8), T(FLOAT, 8)]
+    batch = Batch(types, [x, y, d1, d2, f, g], [xv, yv, None, None, None, None])
+    ob = OracleBackend()
+
+    def run(build):
+        p = cases.Prog(types)
+        r = build(p)
+        return ob.filter(types, p, r, batch).astype(bool)
+
+    # int64 with NULLs
+    got = run(lambda p: p.cmp(op, p.col(0), p.col(1)))
+    want = np.array([bool(xv[i] and yv[i] and fn(int(x[i]), int(y[i]))) for i in range(n)])
+    assert np.array_equal(got, want)
+    assert want.any() and (~want).any()
+    # decimals at different scales
+    got = run(lambda p: p.cmp(op, p.scale_up(p.col(2), 2, T(DECIMAL, 8, 4, False, 17)), p.col(3)))
+    want = np.array([fn(Fraction(int(d1[i]), 100), Fraction(int(d2[i]), 10000)) for i in range(n)])
+    assert np.array_equal(got, want)
+    # f64 (no NaN)
+    got = run(lambda p: p.cmp(op, p.col(4), p.col(5)))
+    assert np.array_equal(got, np.array([fn(float(f[i]), float(g[i])) for i in range(n)]))
+    # NOT of a comparison with NULLs: NULL stays NULL (not selected)
+    got = run(lambda p: p._emit(E.EXPR_NOT, E.REG_BOOL, T(3, 1, nullable=True), a=p.cmp(op, p.col(0), p.col(1))))
+    want = np.array([bool(xv[i] and yv[i] and not fn(int(x[i]), int(y[i]))) for i in range(n)])
+    assert np.array_equal(got, want)
+    # AND / OR with a NULL operand: FALSE AND NULL = FALSE, TRUE OR NULL = TRUE
+    def and3(a, b):
+        return False if (a is False or b is False) else (None if (a is None or b is None) else True)
+
+    def or3(a, b):
+        return True if (a is True or b is True) else (None if (a is None or b is None) else False)
+
+    for lop, py in ((E.EXPR_AND, and3), (E.EXPR_OR, or3)):
+        got = run(lambda p: p._emit(lop, E.REG_BOOL, T(3, 1, nullable=True), a=p.cmp(op, p.col(0), p.col(1)),
+                                    b=p.cmp(E.CMP_GE, p.col(4), p.const_f64(0.0))))
+        want = []
+        for i in range(n):
+            a = fn(int(x[i]), int(y[i])) if (xv[i] and yv[i]) else None
+            b = float(f[i]) >= 0.0
+            r = py(a, b)
+            want.append(r is True)
+        assert np.array_equal(got, np.array(want))
