@@ -232,7 +232,8 @@ def tuned_configs():
     lib().qdb_gpu_autotune_entries(ents, n)
     return [{"key_words": e.KeyWords, "card_bucket": e.CardBucket, "rows_bucket": e.RowsBucket,
              "config": ("bucketed, fan-out x%g, partition over <= %d workgroups" % (2.0 ** e.FanoutShift, e.PartBlocks or 2048))
-             if e.Path == 1 else f"LDS {e.LdsKiB} KiB x {e.GridPerCu}/CU", "ms": round(e.Ms, 3)} for e in ents[:n]]
+             if e.Path == 1 else f"LDS {e.LdsKiB} KiB x {e.GridPerCu}/CU" + (f", {e.LdsCopies} copies" if e.LdsCopies > 1 else ""),
+             "ms": round(e.Ms, 3)} for e in ents[:n]]
 
 
 def bench_batched(n_groupby=256, n_join=256):

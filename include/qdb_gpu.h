@@ -644,13 +644,14 @@ int qdb_gpu_agg_update_batched(const QdbAggregateKernels* kernels, void* const* 
  * picks the table layout by its own rules): an update of a shape bucket
  * (key words, ceil(log2(table slots)), floor(log2(batch rows))) that is not
  * cached yet times every candidate configuration -- one pass over LDS
- * tables of 32, 64 or 128 KiB per workgroup at 4, 2 or 1 workgroups per CU,
- * or the bucketed path at 1/2x, 1x or 2x its bucket count with its
+ * tables of 32, 64 or 128 KiB per workgroup at 4, 2 or 1 workgroups per CU
+ * (and 64 KiB with eight copies of the state words, for a few groups), or
+ * the bucketed path at 1/2x, 1x or 2x its bucket count with its
  * partition over up to 2048 or 512 workgroups -- as whole
  * queries on scratch tables, caches the fastest, and every later update of
  * that bucket (plain, batched, served) runs in it. Off by default (the
  * sizing rule decides); QDB_GPU_AUTOTUNE=1 turns it on at start-up and
- * QDB_GPU_TUNE_FORCE="path,lds_kib,grid_per_cu,fanout_shift[,part_blocks]" forces one
+ * QDB_GPU_TUNE_FORCE="path,lds_kib,grid_per_cu,fanout_shift[,part_blocks[,lds_copies]]" forces one
  * configuration for every update. */
 typedef struct QdbTuneEntry {
     int32_t KeyWords;
@@ -661,6 +662,8 @@ typedef struct QdbTuneEntry {
     int32_t GridPerCu;     /* path 0 */
     int32_t FanoutShift;   /* path 1: log2 of the bucket count relative to the sizing rule */
     int32_t PartBlocks;    /* path 1: most partition workgroups (0: 2048) */
+    int32_t LdsCopies;     /* path 0: copies of the LDS state words (1: none) */
+    int32_t Pad;
     double Ms;             /* the chosen configuration's time on the batch it was tuned on */
 } QdbTuneEntry;
 /* Turns tuning of uncached shapes on (1) or off (0); returns the previous state. */

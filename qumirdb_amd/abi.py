@@ -196,7 +196,7 @@ class QdbFusedResult(C.Structure):
 
 class QdbTuneEntry(C.Structure):
     _fields_ = [("KeyWords", C.c_int32), ("CardBucket", C.c_int32), ("RowsBucket", C.c_int32), ("Path", C.c_int32),
-                ("LdsKiB", C.c_int32), ("GridPerCu", C.c_int32), ("FanoutShift", C.c_int32), ("PartBlocks", C.c_int32),
+                ("LdsKiB", C.c_int32), ("GridPerCu", C.c_int32), ("FanoutShift", C.c_int32), ("PartBlocks", C.c_int32), ("LdsCopies", C.c_int32), ("Pad", C.c_int32),
                 ("Ms", C.c_double)]
 
 

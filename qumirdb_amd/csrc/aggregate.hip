@@ -94,15 +94,24 @@ __global__ void __launch_bounds__(kBlock) k_agg_finalize(FinArgs F)
         const unsigned int g = (unsigned int)(base + lane);
         const bool live = g < F.groups;
         const uint64_t* r = F.recs;
+        // The run holding the wave's first pending group (one binary search
+        // per wave: the last k with pend_prefix[k] <= base); runs hold
+        // hundreds of groups, so a lane walks at most a step or two from it.
+        int run0 = 0;
+        if (base < F.pend_n) {
+            int lo = 0, hi = F.pend_runs - 1;
+            const unsigned int b0 = (unsigned int)base;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (F.pend_prefix[mid] <= b0) lo = mid;
+                else hi = mid - 1;
+            }
+            run0 = lo;
+        }
         if (live) {
             if (g < F.pend_n) {
-                // The run holding group g: the last k with pend_prefix[k] <= g.
-                int lo = 0, hi = F.pend_runs - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (F.pend_prefix[mid] <= g) lo = mid;
-                    else hi = mid - 1;
-                }
+                int lo = run0;
+                while (lo + 1 < F.pend_runs && F.pend_prefix[lo + 1] <= g) ++lo;
                 r = F.pend_recs + ((uint64_t)F.pend_items[lo].row0 + (g - F.pend_prefix[lo])) * (uint64_t)F.R;
             } else {
                 const unsigned int gg = g - F.pend_n;
@@ -342,7 +351,8 @@ struct AggTable {
     // finish reads directly and any other use first merges into the slot
     // table. Their keys occur in no other run and not in the slot table.
     bool allocated = true;
-    uint64_t cap_hint = 0;              // the slots asked for at Init
+    uint64_t cap_hint = 0;              // the slots asked for at Init (at least 1024)
+    int64_t init_slots = 0;             // the Init argument itself
     uint64_t* pend_recs = nullptr;      // records of R words
     SortedItem* pend_items = nullptr;   // device: the runs (row0 = first record)
     uint32_t* pend_counts = nullptr;    // device: records per run
@@ -1168,6 +1178,7 @@ AggTable* table_new(AggPlan* P, int64_t slots_hint)
     const int64_t groups_hint = slots_hint / 2;
     T->counters = (unsigned int*)dalloc(16);
     T->cap_hint = cap;
+    T->init_slots = slots_hint;
     // Tables large enough for the bucketed path get their buffers at first
     // need (see AggTable).
     const bool lazy = cap >= kSortedMinSlots;
@@ -1359,16 +1370,27 @@ bool set_counters(AggTable* T)
 
 // LDS table of an update launch: as many groups as fit in lds_kib KiB, a
 // power of two (none for i128 MIN/MAX, which live in HBM only).
-void lds_layout(const AggPlan* P, AggArgs* A, size_t* lds_bytes, int lds_kib = 64)
+void lds_layout(const AggPlan* P, AggArgs* A, size_t* lds_bytes, int lds_kib = 64, int copies = 1)
 {
     const int R = P->KW + A->nwords;
     const size_t budget = (size_t)lds_kib * 1024;
+    if (copies < 1 || P->has_pairs) copies = 1;
+    // Records, hash slots (two per group), the counter, then the copies of
+    // the state words: an odd number of words per group and per copy, so
+    // that a word's copies, and one word of different groups, fall in
+    // different LDS banks.
+    const int G = A->nwords | 1;  // words per group in a copy (odd)
+    auto bytes_of = [&](int g) {
+        return (size_t)g * R * 8 + (size_t)g * 2 * 4 + 16 + (size_t)(copies - 1) * ((size_t)g * G + 1) * 8;
+    };
     int lds_groups = 4096;
-    while (lds_groups > 16 && (size_t)lds_groups * R * 8 + (size_t)lds_groups * 2 * 4 + 16 > budget) lds_groups /= 2;
+    while (lds_groups > 16 && bytes_of(lds_groups) > budget) lds_groups /= 2;
     if (P->has_pairs) lds_groups = 0;
     A->lds_groups = lds_groups;
     A->lds_slots = lds_groups * 2;
-    *lds_bytes = lds_groups ? (size_t)lds_groups * R * 8 + (size_t)A->lds_slots * 4 + 16 : 0;
+    A->lds_copies = copies;
+    A->lds_copy_stride = lds_groups * G + 1;
+    *lds_bytes = lds_groups ? bytes_of(lds_groups) : 0;
 }
 
 // One update launch over rows [base, base + n) of a batch into table T: its
@@ -1569,6 +1591,7 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n, int fanou
     }
     A.lds_groups = g;
     A.lds_slots = slots;
+    A.lds_copies = 1;
     const size_t lds = (size_t)g * R * 8 + (size_t)A.lds_slots * 4 + 16;
     const uint64_t expect = (uint64_t)n < T->capacity / 2 ? (uint64_t)n : T->capacity / 2;
     int bits = 8;
@@ -1795,8 +1818,11 @@ bool bucketed_possible(AggPlan* P, int64_t rows)
 // workgroups per CU.
 TuneConfig default_config(AggPlan* P, const AggTable* T, int64_t rows)
 {
-    TuneConfig c = {0, 64, 2, 0, 0};
+    TuneConfig c = {0, 64, 2, 0, 0, 1};
     if (rows <= ((int64_t)1 << 30) && sorted_path_wanted(P, T, rows)) c.path = 1;
+    // A table made for a handful of groups: eight copies of the LDS state
+    // words spread the atomics of a wave's rows on the same group.
+    if (T->init_slots <= 64) c.lds_copies = 8;
     return c;
 }
 
@@ -1816,7 +1842,7 @@ bool table_update_cfg(AggTable* T, const TRowSet* batch, const TuneConfig& cfg)
     if (!bind_batch(P, batch, &bound)) return false;
     const int KW = P->KW, S = P->S;
     size_t lds = 0;
-    lds_layout(P, &bound, &lds, cfg.lds_kib);
+    lds_layout(P, &bound, &lds, cfg.lds_kib, cfg.lds_copies);
     const int per_cu = cfg.grid_per_cu >= 1 ? cfg.grid_per_cu : 2;
     const int64_t chunk = (int64_t)1 << 30;
     for (int64_t base = 0; base < batch->RowCount; base += chunk) {
@@ -2196,13 +2222,13 @@ TuneConfig choose_config(AggTable* T, const TRowSet* batch)
     const bool bucketed_ok = bucketed_possible(P, rows);
     TuneConfig cfg;
     if (tune_forced(&cfg)) {
-        if (cfg.path == 1 && !bucketed_ok) cfg = TuneConfig{0, 64, 2, 0, 0};
+        if (cfg.path == 1 && !bucketed_ok) cfg = TuneConfig{0, 64, 2, 0, 0, 1};
         return cfg;
     }
     const uint64_t slots = T->cap_hint > T->capacity ? T->cap_hint : T->capacity;
     const TuneKey key = tune_key(P->KW, slots, rows);
     if (tune_lookup(key, &cfg)) {
-        if (cfg.path == 1 && !bucketed_ok) cfg = TuneConfig{0, 64, 2, 0, 0};
+        if (cfg.path == 1 && !bucketed_ok) cfg = TuneConfig{0, 64, 2, 0, 0, 1};
         return cfg;
     }
     if (!tune_enabled() || rows <= 0) return default_config(P, T, rows);

@@ -91,6 +91,15 @@ struct AggArgs {
     WordDesc words[kMaxWords];
     int lds_slots;             // power of two
     int lds_groups;            // max groups in LDS
+    // Copies of the LDS table's state words (1: none): lane l applies its
+    // row to copy l % lds_copies, so that the lanes of a wave whose rows
+    // share a group spread their atomics over that many addresses; the
+    // copies are combined when the table is flushed. Copy c >= 1 of group
+    // g's word w is at extra + (c - 1) * lds_copy_stride + g * (nwords | 1)
+    // + w (an odd number of words per group and per copy: one word of
+    // different groups or copies falls in different LDS banks).
+    int lds_copies;
+    int lds_copy_stride;
     int match_rounds;
     int has_pairs;             // i128 MIN/MAX words present
     // HBM table: one record per slot, `rstride` words apart (KW key words,
@@ -381,6 +390,7 @@ struct LdsTable {
     unsigned int* ctrl;
     unsigned int* count;
     uint64_t* recs;
+    uint64_t* extra;          // copies 1.. of the state words (lds_copies > 1)
     unsigned int* inserted;   // HBM groups this workgroup inserted
     int fresh;                // this thread's last HBM lookup inserted its key
 };
@@ -498,6 +508,10 @@ __device__ __forceinline__ int lds_find_or_insert(const AggArgs& A, LdsTable& T,
 #pragma unroll
                 for (int w = 0; w < KW; ++w) r[w] = key[w];
                 for (int w = 0; w < P::nwords(A); ++w) r[KW + w] = P::init(A, w);
+                for (int c = 1; c < A.lds_copies; ++c) {
+                    uint64_t* e = T.extra + (size_t)(c - 1) * A.lds_copy_stride + (size_t)lg * (P::nwords(A) | 1);
+                    for (int w = 0; w < P::nwords(A); ++w) e[w] = P::init(A, w);
+                }
                 __hip_atomic_store(&T.ctrl[slot], lg + 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                 return (int)lg;
             }
@@ -568,6 +582,10 @@ __device__ __forceinline__ void apply_words(const AggArgs& A, LdsTable& T, int w
     // either would make every access a flat one).
     if (where == AT_LDS) {
         uint64_t* r = T.recs + (size_t)id * (KW + P::nwords(A)) + KW;
+        if (A.lds_copies > 1) {
+            const int c = lane_id() % A.lds_copies;
+            if (c) r = T.extra + (size_t)(c - 1) * A.lds_copy_stride + (size_t)id * (P::nwords(A) | 1);
+        }
 #pragma unroll
         for (int w = 0; w < S; ++w) {
             if (w >= P::nwords(A)) continue;
@@ -674,6 +692,8 @@ __device__ __forceinline__ LdsTable lds_table_at(const AggArgs& A, uint64_t* lds
     T.recs = lds_raw;
     T.ctrl = (unsigned int*)(lds_raw + (size_t)A.lds_groups * (KW + P::nwords(A)));
     T.count = T.ctrl + A.lds_slots;
+    // 8-byte aligned past the counter (lds_slots is even).
+    T.extra = (uint64_t*)(T.ctrl + A.lds_slots + 2);
     T.inserted = inserted;
     T.fresh = 0;
     return T;
@@ -766,6 +786,14 @@ __device__ __forceinline__ void lds_table_flush(const AggArgs& A, LdsTable& T, b
             for (int w = 0; w < KW; ++w) key[w] = r[w];
 #pragma unroll
             for (int w = 0; w < S; ++w) c[w] = w < P::nwords(A) ? r[KW + w] : 0;
+            // The copies' words combined into the first copy's.
+            for (int k = 1; k < A.lds_copies; ++k) {
+                const uint64_t* e = T.extra + (size_t)(k - 1) * A.lds_copy_stride + (size_t)(want ? lg : 0u) * (P::nwords(A) | 1);
+#pragma unroll
+                for (int w = 0; w < S; ++w) {
+                    if (w < P::nwords(A)) c[w] = combine((uint8_t)P::op(A, w), c[w], e[w]);
+                }
+            }
             global_upsert_or_spill<KW, S, P>(A, T, want, key, c);
         }
         __syncthreads();

@@ -80,7 +80,9 @@ bool tune_forced(TuneConfig* cfg)
     if (!e || !*e) return false;
     TuneConfig c;
     c.part_blocks = 0;
-    const int got = sscanf(e, "%d,%d,%d,%d,%d", &c.path, &c.lds_kib, &c.grid_per_cu, &c.fanout_shift, &c.part_blocks);
+    c.lds_copies = 1;
+    const int got = sscanf(e, "%d,%d,%d,%d,%d,%d", &c.path, &c.lds_kib, &c.grid_per_cu, &c.fanout_shift, &c.part_blocks,
+        &c.lds_copies);
     if (got < 4) return false;
     *cfg = c;
     return true;
@@ -91,17 +93,18 @@ std::vector<TuneConfig> tune_candidates(bool bucketed_ok)
     // LDS budget x workgroups per CU within the 160 KiB of a CU, and the
     // bucket count at 1/2x, 1x and 2x the sizing rule's.
     std::vector<TuneConfig> v = {
-        {0, 32, 4, 0, 0},
-        {0, 64, 2, 0, 0},
-        {0, 128, 1, 0, 0},
+        {0, 32, 4, 0, 0, 1},
+        {0, 64, 2, 0, 0, 1},
+        {0, 128, 1, 0, 0, 1},
+        {0, 64, 2, 0, 0, 8},   // eight copies of the LDS state words (a few groups)
     };
     if (bucketed_ok) {
         // and the partition at up to 2048 or 512 workgroups (fewer, longer
         // row ranges per workgroup: longer runs per bucket).
         for (int blocks : {2048, 512}) {
-            v.push_back({1, 0, 0, -1, blocks});
-            v.push_back({1, 0, 0, 0, blocks});
-            v.push_back({1, 0, 0, 1, blocks});
+            v.push_back({1, 0, 0, -1, blocks, 1});
+            v.push_back({1, 0, 0, 0, blocks, 1});
+            v.push_back({1, 0, 0, 1, blocks, 1});
         }
     }
     return v;
@@ -135,6 +138,7 @@ extern "C" int64_t qdb_gpu_autotune_entries(QdbTuneEntry* out, int64_t max)
             e.GridPerCu = kv.second.cfg.grid_per_cu;
             e.FanoutShift = kv.second.cfg.fanout_shift;
             e.PartBlocks = kv.second.cfg.part_blocks;
+            e.LdsCopies = kv.second.cfg.lds_copies;
             e.Ms = kv.second.ms;
         }
         ++n;
@@ -154,11 +158,11 @@ extern "C" int qdb_gpu_autotune_save(const char* path)
     std::lock_guard<std::mutex> lock(g_mu);
     FILE* f = fopen(path, "w");
     if (!f) return -1;
-    fprintf(f, "# kw card_bucket rows_bucket path lds_kib grid_per_cu fanout_shift part_blocks ms\n");
+    fprintf(f, "# kw card_bucket rows_bucket path lds_kib grid_per_cu fanout_shift part_blocks lds_copies ms\n");
     for (const auto& kv : cache()) {
         const TuneConfig& c = kv.second.cfg;
-        fprintf(f, "%d %d %d %d %d %d %d %d %.6f\n", std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), c.path,
-            c.lds_kib, c.grid_per_cu, c.fanout_shift, c.part_blocks, kv.second.ms);
+        fprintf(f, "%d %d %d %d %d %d %d %d %d %.6f\n", std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), c.path,
+            c.lds_kib, c.grid_per_cu, c.fanout_shift, c.part_blocks, c.lds_copies, kv.second.ms);
     }
     return fclose(f) == 0 ? 0 : -1;
 }
@@ -175,12 +179,12 @@ extern "C" int64_t qdb_gpu_autotune_load(const char* path)
         TuneKey k;
         TuneConfig c;
         double ms = 0;
-        if (sscanf(line, "%d %d %d %d %d %d %d %d %lf", &k.kw, &k.card_bucket, &k.rows_bucket, &c.path, &c.lds_kib, &c.grid_per_cu,
-                &c.fanout_shift, &c.part_blocks, &ms) != 9) {
+        if (sscanf(line, "%d %d %d %d %d %d %d %d %d %lf", &k.kw, &k.card_bucket, &k.rows_bucket, &c.path, &c.lds_kib,
+                &c.grid_per_cu, &c.fanout_shift, &c.part_blocks, &c.lds_copies, &ms) != 10) {
             continue;
         }
         const bool valid = (c.path == 0 && (c.lds_kib == 32 || c.lds_kib == 64 || c.lds_kib == 128) && c.grid_per_cu >= 1 &&
-                               c.grid_per_cu <= 8) ||
+                               c.grid_per_cu <= 8 && c.lds_copies >= 1 && c.lds_copies <= 16) ||
             (c.path == 1 && c.fanout_shift >= -2 && c.fanout_shift <= 2 && c.part_blocks >= 0 && c.part_blocks <= 2048);
         if (!valid) continue;
         tune_store(k, c, ms);

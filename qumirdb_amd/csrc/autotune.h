@@ -17,6 +17,7 @@ struct TuneConfig {
     int32_t grid_per_cu;   // path 0: workgroups per CU
     int32_t fanout_shift;  // path 1: log2 of the bucket count relative to the sizing rule
     int32_t part_blocks;   // path 1: most partition workgroups (0: the sizing rule's 2048)
+    int32_t lds_copies;    // path 0: copies of the LDS state words (1: none)
 };
 
 // A shape bucket: key words, ceil(log2(table slots)), floor(log2(rows)).
@@ -34,7 +35,8 @@ void tune_store(const TuneKey& key, const TuneConfig& cfg, double ms);
 // or QDB_GPU_AUTOTUNE=1 in the environment).
 bool tune_enabled();
 // A configuration forced for every update (QDB_GPU_TUNE_FORCE =
-// "path,lds_kib,grid_per_cu,fanout_shift[,part_blocks]"); false when none is.
+// "path,lds_kib,grid_per_cu,fanout_shift[,part_blocks[,lds_copies]]"); false
+// when none is.
 bool tune_forced(TuneConfig* cfg);
 // The candidates for a shape: path-0 LDS budgets and grids, and path-1
 // fan-outs when the bucketed path can serve.

@@ -80,21 +80,22 @@ def test_autotune_cache_file_round_trip(tmp_path):
     lib = abi.declare(ctypes.CDLL(LIB))
     lib.qdb_gpu_autotune_clear()
     src = tmp_path / "tune.txt"
-    src.write_text("# kw card rows path lds grid shift blocks ms\n"
-                   "1 14 18 0 64 2 0 0 1.5\n"
-                   "1 21 24 1 0 0 -1 512 4.25\n"
-                   "2 10 12 0 48 2 0 0 1.0\n"     # 48 KiB is not a candidate
-                   "1 22 25 1 0 0 5 0 1.0\n"      # fan-out shift out of range
-                   "1 23 25 1 0 0 0 4096 1.0\n"   # more partition workgroups than 2048
+    src.write_text("# kw card rows path lds grid shift blocks copies ms\n"
+                   "1 14 18 0 64 2 0 0 8 1.5\n"
+                   "1 21 24 1 0 0 -1 512 1 4.25\n"
+                   "2 10 12 0 48 2 0 0 1 1.0\n"     # 48 KiB is not a candidate
+                   "1 22 25 1 0 0 5 0 1 1.0\n"      # fan-out shift out of range
+                   "1 23 25 1 0 0 0 4096 1 1.0\n"   # more partition workgroups than 2048
+                   "1 24 25 0 64 2 0 0 64 1.0\n"    # more LDS copies than 16
                    "garbage\n")
     assert lib.qdb_gpu_autotune_load(str(src).encode()) == 2
     n = lib.qdb_gpu_autotune_entries(None, 0)
     assert n == 2
     ents = (abi.QdbTuneEntry * n)()
     lib.qdb_gpu_autotune_entries(ents, n)
-    got = sorted((e.KeyWords, e.CardBucket, e.RowsBucket, e.Path, e.LdsKiB, e.GridPerCu, e.FanoutShift, e.PartBlocks, e.Ms)
-                 for e in ents)
-    assert got == [(1, 14, 18, 0, 64, 2, 0, 0, 1.5), (1, 21, 24, 1, 0, 0, -1, 512, 4.25)]
+    got = sorted((e.KeyWords, e.CardBucket, e.RowsBucket, e.Path, e.LdsKiB, e.GridPerCu, e.FanoutShift, e.PartBlocks, e.LdsCopies,
+                  e.Ms) for e in ents)
+    assert got == [(1, 14, 18, 0, 64, 2, 0, 0, 8, 1.5), (1, 21, 24, 1, 0, 0, -1, 512, 1, 4.25)]
     out = tmp_path / "saved.txt"
     assert lib.qdb_gpu_autotune_save(str(out).encode()) == 0
     lib.qdb_gpu_autotune_clear()
@@ -102,6 +103,6 @@ def test_autotune_cache_file_round_trip(tmp_path):
     assert lib.qdb_gpu_autotune_load(str(out).encode()) == 2
     ents2 = (abi.QdbTuneEntry * 2)()
     lib.qdb_gpu_autotune_entries(ents2, 2)
-    assert sorted((e.KeyWords, e.CardBucket, e.RowsBucket, e.Path, e.LdsKiB, e.GridPerCu, e.FanoutShift, e.PartBlocks, e.Ms)
-                  for e in ents2) == got
+    assert sorted((e.KeyWords, e.CardBucket, e.RowsBucket, e.Path, e.LdsKiB, e.GridPerCu, e.FanoutShift, e.PartBlocks,
+                   e.LdsCopies, e.Ms) for e in ents2) == got
     lib.qdb_gpu_autotune_clear()

@@ -14,9 +14,9 @@ from helpers import result_rows, sort_key
 
 pytestmark = pytest.mark.gpu
 
-# (path, lds_kib, grid_per_cu, fanout_shift, part_blocks): every candidate of the tuner.
-CONFIGS = [(0, 32, 4, 0, 0), (0, 64, 2, 0, 0), (0, 128, 1, 0, 0)] + \
-    [(1, 0, 0, f, b) for b in (2048, 512) for f in (-1, 0, 1)]
+# (path, lds_kib, grid_per_cu, fanout_shift, part_blocks, lds_copies): every candidate of the tuner.
+CONFIGS = [(0, 32, 4, 0, 0, 1), (0, 64, 2, 0, 0, 1), (0, 128, 1, 0, 0, 1), (0, 64, 2, 0, 0, 8)] + \
+    [(1, 0, 0, f, b, 1) for b in (2048, 512) for f in (-1, 0, 1)]
 
 
 @pytest.fixture(scope="module")
@@ -30,7 +30,7 @@ def staged_case(case):
     return cases.AggCase(case.types, case.keys, keep)
 
 
-@pytest.mark.parametrize("cfg", CONFIGS, ids=lambda c: "p%d_l%d_g%d_f%d_b%d" % c)
+@pytest.mark.parametrize("cfg", CONFIGS, ids=lambda c: "p%d_l%d_g%d_f%d_b%d_c%d" % c)
 @pytest.mark.parametrize("shape", ["small_table", "large_table"])
 def test_forced_config_equals_oracle(gpu_backend, monkeypatch, cfg, shape):
     monkeypatch.setenv("QDB_GPU_TUNE_FORCE", ",".join(str(v) for v in cfg))
@@ -68,7 +68,7 @@ def test_tuner_caches_one_config_per_shape(gpu_backend, monkeypatch):
         assert lib.qdb_gpu_autotune_entries(ents, n) == n
         e = ents[0]
         assert e.CardBucket == 14 and e.RowsBucket == 18 and e.Ms > 0
-        assert (e.Path, e.LdsKiB, e.GridPerCu, e.FanoutShift, e.PartBlocks) in CONFIGS
+        assert (e.Path, e.LdsKiB, e.GridPerCu, e.FanoutShift, e.PartBlocks, e.LdsCopies) in CONFIGS
         # The same shape again: no new entry, same results.
         got2 = sorted(result_rows(gpu_backend.aggregate(case, [b], capacity=1 << 14)), key=sort_key)
         assert got2 == want and lib.qdb_gpu_autotune_entries(None, 0) == 1
@@ -80,8 +80,8 @@ def test_tuner_caches_one_config_per_shape(gpu_backend, monkeypatch):
         assert lib.qdb_gpu_autotune_load(path.encode()) == 1
         ents2 = (abi.QdbTuneEntry * 1)()
         lib.qdb_gpu_autotune_entries(ents2, 1)
-        assert (ents2[0].Path, ents2[0].LdsKiB, ents2[0].GridPerCu, ents2[0].FanoutShift, ents2[0].PartBlocks) == \
-            (e.Path, e.LdsKiB, e.GridPerCu, e.FanoutShift, e.PartBlocks)
+        assert (ents2[0].Path, ents2[0].LdsKiB, ents2[0].GridPerCu, ents2[0].FanoutShift, ents2[0].PartBlocks,
+                ents2[0].LdsCopies) == (e.Path, e.LdsKiB, e.GridPerCu, e.FanoutShift, e.PartBlocks, e.LdsCopies)
         os.unlink(path)
     finally:
         lib.qdb_gpu_autotune_enable(was)
