@@ -144,7 +144,7 @@ class Server:
     batching on `workers` streams or one request at a time."""
 
     def __init__(self, res: Resident, hbm_budget: float = 200e9, full_results: bool = False, workers: int = 3,
-                 max_unit: int = 64):
+                 max_unit: int = 16):
         self.res = res
         self.budget = hbm_budget - res.table_bytes()
         self.full = full_results
@@ -308,10 +308,14 @@ class Server:
             while queue and len(inflight) < self.workers:
                 unit = self._take_unit(queue)
                 inflight[pool.submit(self.run_unit, unit, clock)] = unit
-            # Wait for a completion or the next arrival.
+            # Wait for a completion or the next arrival. A request that has
+            # arrived but does not fit the budget waits for a completion: the
+            # loop must not spin, since a spinning thread holds the GIL the
+            # workers need after each library call.
             wait_s = 0.0005
             if pending:
-                wait_s = max(0.0, min(wait_s, pending[0].arrival - clock()))
+                ahead = pending[0].arrival - clock()
+                wait_s = min(wait_s, ahead) if ahead > 0 else 0.05
             if inflight:
                 done, _ = cf.wait(list(inflight), timeout=wait_s, return_when=cf.FIRST_COMPLETED)
                 for fut in done:
