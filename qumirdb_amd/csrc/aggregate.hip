@@ -58,6 +58,13 @@ constexpr int64_t kSortedMinRows = (int64_t)1 << 22;
 constexpr uint64_t kSortedMinSlots = (uint64_t)1 << 20;
 constexpr int64_t kSortedItemRows = (int64_t)1 << 16;
 constexpr int kSortedBlock = 1024;    // threads of a bucket-aggregation workgroup
+// Partial records of an update (AggArgs::part_recs): at most kPartCap
+// groups per workgroup slot, kPartsPerReducer slots per workgroup of the
+// combining launch; used for tables of at most kPartMaxSlots slots that
+// hold at most kPartCap groups.
+constexpr int kPartCap = 64;
+constexpr int kPartsPerReducer = 32;
+constexpr uint64_t kPartMaxSlots = 4096;
 
 struct StageLayout {
     int words = 0;                    // 0: the plan does not stage
@@ -978,6 +985,21 @@ void emit_contribs(std::string& o, const AggPlan* P, int j, const std::string& v
     o += "        }\n";
 }
 
+// keep_: whether the row takes part (its selection byte and the predicate).
+// The generated functions decide at their end, not by an early return, so
+// that the loads of every column the row needs issue together instead of
+// waiting for the predicate's columns first; the code after it only loads
+// in-bounds values of the same row and computes, without side effects.
+void emit_keep(std::string& o, const AggPlan* P)
+{
+    o += "        bool keep_ = !(A.sel && !A.sel[row]);\n";
+    if (P->has_pred) {
+        o += "        {\n";
+        const std::string r = emit_program(o, "p", P->h_pred, P);
+        o += "    keep_ = keep_ && " + r + ".valid && " + r + ".i != 0;\n        }\n";
+    }
+}
+
 // stage(): the row's staged words (false: the row does not take part);
 // unstage(): the key and contributions back from them.
 void emit_stage(std::string& o, const AggPlan* P)
@@ -991,12 +1013,7 @@ void emit_stage(std::string& o, const AggPlan* P)
     o += "    __device__ static __forceinline__ bool stage(const AggArgs& A, int64_t row, uint64_t (&st)[kStage])\n    {\n";
     o += "        const DevCols& C = A.cols;\n";
     o += "#pragma unroll\n        for (int w = 0; w < kStage; ++w) st[w] = 0;\n";
-    o += "        if (A.sel && !A.sel[row]) return false;\n";
-    if (P->has_pred) {
-        o += "        {\n";
-        const std::string r = emit_program(o, "p", P->h_pred, P);
-        o += "    if (!(" + r + ".valid && " + r + ".i != 0)) return false;\n        }\n";
-    }
+    emit_keep(o, P);
     o += "        uint64_t key[KW];\n#pragma unroll\n        for (int w = 0; w < KW; ++w) key[w] = 0;\n";
     for (int f = 0; f < A.nkeys; ++f) {
         const KeyFieldDesc& k = A.keys[f];
@@ -1023,19 +1040,14 @@ void emit_stage(std::string& o, const AggPlan* P)
         o += "        }\n";
     }
     if (L.valid_word >= 0) o += "        st[" + std::to_string(L.valid_word) + "] = vb;\n";
-    o += "        (void)vb;\n        return true;\n    }\n";
+    o += "        (void)vb;\n        return keep_;\n    }\n";
     // stage_key(): the row's key alone (the partition's counting pass reads
     // only the key and predicate columns).
     o += "    template <int KW>\n";
     o += "    __device__ static __forceinline__ bool stage_key(const AggArgs& A, int64_t row, uint64_t (&key)[KW])\n    {\n";
     o += "        const DevCols& C = A.cols;\n";
     o += "#pragma unroll\n        for (int w = 0; w < KW; ++w) key[w] = 0;\n";
-    o += "        if (A.sel && !A.sel[row]) return false;\n";
-    if (P->has_pred) {
-        o += "        {\n";
-        const std::string r = emit_program(o, "p", P->h_pred, P);
-        o += "    if (!(" + r + ".valid && " + r + ".i != 0)) return false;\n        }\n";
-    }
+    emit_keep(o, P);
     for (int f = 0; f < A.nkeys; ++f) {
         const KeyFieldDesc& k = A.keys[f];
         char b[300];
@@ -1043,7 +1055,7 @@ void emit_stage(std::string& o, const AggPlan* P)
             k.width, k.nullable, k.word, k.shift, k.null_word, k.null_bit, k.slot);
         o += b;
     }
-    o += "        return true;\n    }\n";
+    o += "        return keep_;\n    }\n";
     o += "    template <int KW, int S>\n";
     o += "    __device__ static __forceinline__ void unstage(const AggArgs& A, const uint64_t (&st)[kStage], uint64_t (&key)[KW], uint64_t (&c)[S])\n    {\n";
     o += "#pragma unroll\n        for (int w = 0; w < KW; ++w) key[w] = st[w];\n";
@@ -1094,12 +1106,7 @@ std::string agg_jit_source(const AggPlan* P)
     o += "    template <int KW, int S>\n";
     o += "    __device__ static __forceinline__ bool contrib(const AggArgs& A, int64_t row, uint64_t (&key)[KW], uint64_t (&c)[S])\n    {\n";
     o += "        const DevCols& C = A.cols;\n";
-    o += "        if (A.sel && !A.sel[row]) return false;\n";
-    if (P->has_pred) {
-        o += "        {\n";
-        const std::string r = emit_program(o, "p", P->h_pred, P);
-        o += "    if (!(" + r + ".valid && " + r + ".i != 0)) return false;\n        }\n";
-    }
+    emit_keep(o, P);
     o += "#pragma unroll\n        for (int w = 0; w < S; ++w) c[w] = 0;\n";
     for (int w = 0; w < nw; ++w) {
         const WordDesc& d = A.words[w];
@@ -1120,7 +1127,7 @@ std::string agg_jit_source(const AggPlan* P)
             k.width, k.nullable, k.word, k.shift, k.null_word, k.null_bit, k.slot);
         o += b;
     }
-    o += "        return true;\n    }\n";
+    o += "        return keep_;\n    }\n";
     emit_stage(o, P);
     o += "};\n}  // namespace agg\n}  // namespace qdbgpu\n";
     o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kBlock) + ") qdb_agg_update(qdbgpu::agg::AggArgs A)\n{\n";
@@ -1129,6 +1136,10 @@ std::string agg_jit_source(const AggPlan* P)
         ") qdb_agg_batch(const qdbgpu::agg::AggArgs* insts, const qdbgpu::agg::BatchItem* items, int nitems, unsigned int* head)\n{\n";
     o += "    qdbgpu::agg::agg_batch_body<" + std::to_string(P->KW) + ", " + std::to_string(P->S) +
         ", qdbgpu::agg::JitPolicy>(insts, items, nitems, head);\n}\n";
+    o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kBlock) +
+        ") qdb_agg_reduce(qdbgpu::agg::AggArgs A, const unsigned long long* parts, const unsigned int* part_n, int nparts, int per)\n{\n";
+    o += "    qdbgpu::agg::agg_reduce_parts_body<" + std::to_string(P->KW) + ", " + std::to_string(P->S) +
+        ", qdbgpu::agg::JitPolicy>(A, (const uint64_t*)parts, part_n, nparts, per);\n}\n";
     const StageLayout L = stage_layout(P);
     if (L.words > 0) {
         const std::string kw = std::to_string(P->KW), sw = std::to_string(P->S), w = std::to_string(L.words);
@@ -1820,9 +1831,13 @@ TuneConfig default_config(AggPlan* P, const AggTable* T, int64_t rows)
 {
     TuneConfig c = {0, 64, 2, 0, 0, 1};
     if (rows <= ((int64_t)1 << 30) && sorted_path_wanted(P, T, rows)) c.path = 1;
-    // A table made for a handful of groups: eight copies of the LDS state
-    // words spread the atomics of a wave's rows on the same group.
-    if (T->init_slots <= 64) c.lds_copies = 8;
+    // A table made for a handful of groups: four workgroups per CU with
+    // small LDS tables (their groups leave through partial records, so the
+    // larger grid adds no same-line atomics on the table).
+    if (T->init_slots <= 64) {
+        c.lds_kib = 32;
+        c.grid_per_cu = 4;
+    }
     return c;
 }
 
@@ -1855,12 +1870,45 @@ bool table_update_cfg(AggTable* T, const TRowSet* batch, const TuneConfig& cfg)
             return false;
         }
         hipError_t launched = hipSuccess;
+        hipFunction_t fn = plan_jit_function(P, "qdb_agg_update");
+        // A table of few groups updated by a large grid: the workgroups
+        // write their groups as partial records to slots of their own, and
+        // a second launch (one workgroup per kPartsPerReducer slots)
+        // combines them in LDS before its atomics on the table.
+        uint64_t* parts = nullptr;
+        unsigned int* part_n = nullptr;
+        hipFunction_t f_reduce = nullptr;
+        const int part_cap = bound.lds_groups < kPartCap ? bound.lds_groups : kPartCap;
+        if (fn && grid >= 4 * kPartsPerReducer && part_cap > 0 && T->groups <= (uint64_t)part_cap && T->capacity <= kPartMaxSlots &&
+            (f_reduce = plan_jit_function(P, "qdb_agg_reduce")) != nullptr) {
+            parts = (uint64_t*)dalloc((size_t)grid * (size_t)part_cap * (size_t)(KW + bound.nwords) * 8);
+            part_n = (unsigned int*)dalloc((size_t)grid * 4);
+            if (!parts || !part_n) {
+                dfree(parts);
+                dfree(part_n);
+                L.release();
+                return false;
+            }
+        }
         {
-            hipFunction_t fn = plan_jit_function(P, "qdb_agg_update");
             TimedRegion timer("agg_update");
             if (fn) {
-                void* args[] = {&L.A};
+                AggArgs A = L.A;
+                if (parts) {
+                    A.part_recs = parts;
+                    A.part_n = part_n;
+                    A.part_cap = part_cap;
+                }
+                void* args[] = {&A};
                 launched = hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, kBlock, 1, 1, (unsigned)lds, stream(), args, nullptr);
+                if (parts && launched == hipSuccess) {
+                    AggArgs B = L.A;
+                    B.part_cap = part_cap;
+                    int nparts = grid, per = kPartsPerReducer;
+                    const int nred = (grid + per - 1) / per;
+                    void* rargs[] = {&B, &parts, &part_n, &nparts, &per};
+                    launched = hipModuleLaunchKernel(f_reduce, (unsigned)nred, 1, 1, kBlock, 1, 1, (unsigned)lds, stream(), rargs, nullptr);
+                }
             } else {
                 agg::launch_update(KW, S, L.A, grid, lds);
             }
@@ -1868,6 +1916,8 @@ bool table_update_cfg(AggTable* T, const TRowSet* batch, const TuneConfig& cfg)
         unsigned int cnt[4];
         const bool ok = hip_ok(launched, "agg update launch") && hip_ok(hipGetLastError(), "agg update launch") &&
             read_counters(T, cnt) && settle_launch(&L, cnt);
+        dfree(parts);
+        dfree(part_n);
         L.release();
         if (!ok) return false;
     }

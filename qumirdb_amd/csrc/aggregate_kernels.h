@@ -117,6 +117,17 @@ struct AggArgs {
     uint32_t* ov_rows;
     uint64_t* ov_recs;
     unsigned int ov_rec_cap;
+    // Partial records (null: off). A workgroup whose LDS table ends with at
+    // most part_cap groups writes them as records (key words, state words)
+    // to its own slot, part_recs + blockIdx.x * part_cap records, and their
+    // count to part_n[blockIdx.x] (0 when it moved them into the HBM table
+    // itself); a second launch combines the slots. Device-scope atomics on
+    // one cache line serialize at about 10 ns each, so a few groups' words
+    // updated by every workgroup of a large grid would take longer than
+    // the rows.
+    uint64_t* part_recs;
+    unsigned int* part_n;
+    int part_cap;
 };
 
 __device__ __forceinline__ uint64_t identity_of(uint8_t op)
@@ -776,6 +787,8 @@ __device__ __forceinline__ void lds_table_flush(const AggArgs& A, LdsTable& T, b
         // Every thread runs the same number of iterations.
         const unsigned int n = *T.count < (unsigned int)A.lds_groups ? *T.count : (unsigned int)A.lds_groups;
         const int R = KW + P::nwords(A);
+        const bool to_parts = A.part_recs != nullptr && n <= (unsigned int)A.part_cap;  // uniform
+        if (A.part_recs != nullptr && threadIdx.x == 0) A.part_n[blockIdx.x] = to_parts ? n : 0u;
         for (unsigned int b = 0; b < n; b += blockDim.x) {
             const unsigned int lg = b + threadIdx.x;
             const bool want = lg < n;
@@ -794,9 +807,23 @@ __device__ __forceinline__ void lds_table_flush(const AggArgs& A, LdsTable& T, b
                     if (w < P::nwords(A)) c[w] = combine((uint8_t)P::op(A, w), c[w], e[w]);
                 }
             }
-            global_upsert_or_spill<KW, S, P>(A, T, want, key, c);
+            if (to_parts) {
+                if (want) {
+                    uint64_t* r = A.part_recs + ((size_t)blockIdx.x * (size_t)A.part_cap + lg) * R;
+#pragma unroll
+                    for (int w = 0; w < KW; ++w) r[w] = key[w];
+#pragma unroll
+                    for (int w = 0; w < S; ++w) {
+                        if (w < P::nwords(A)) r[KW + w] = c[w];
+                    }
+                }
+            } else {
+                global_upsert_or_spill<KW, S, P>(A, T, want, key, c);
+            }
         }
         __syncthreads();
+    } else if (A.part_recs != nullptr && threadIdx.x == 0) {
+        A.part_n[blockIdx.x] = 0u;
     }
     // One atomic per workgroup for the groups it inserted.
     if (threadIdx.x == 0 && *T.inserted) atomicAdd(&A.counters[0], *T.inserted);
@@ -818,6 +845,58 @@ __device__ __forceinline__ void agg_update_body(const AggArgs& A)
     lds_table_reset<KW, P>(A, T, use_lds);
     const int64_t start = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     agg_rows<KW, S, P>(A, T, use_lds, start - lane_id(), A.nrows, (int64_t)gridDim.x * blockDim.x);
+    lds_table_flush<KW, S, P>(A, T, use_lds);
+}
+
+// The second launch of an update with partial records: workgroup b
+// combines the records of the update's workgroups [b * per, (b + 1) * per)
+// (slot p holds part_n[p] records of KW + nwords words, part_cap records
+// apart) in its LDS table and moves its groups into the HBM table; a record
+// that finds no room there is kept for the replay after growth. A is the
+// update's, without part_recs.
+template <int KW, int S, class P>
+__device__ __forceinline__ void agg_reduce_parts_body(const AggArgs& A, const uint64_t* parts, const unsigned int* part_n, int nparts,
+    int per)
+{
+    extern __shared__ uint64_t lds_raw[];
+    __shared__ unsigned int s_inserted;
+    const bool use_lds = A.lds_slots > 0;
+    LdsTable T = lds_table_at<KW, P>(A, lds_raw, &s_inserted);
+    lds_table_reset<KW, P>(A, T, use_lds);
+    const int R = KW + P::nwords(A);
+    const int p0 = (int)blockIdx.x * per;
+    const int p1 = p0 + per < nparts ? p0 + per : nparts;
+    const int64_t cap = A.part_cap;
+    const int64_t total = p1 > p0 ? (int64_t)(p1 - p0) * cap : 0;
+    const int lane = lane_id();
+    // Every lane of a wave runs the same number of iterations (the lookups
+    // inside are wave-uniform).
+    for (int64_t base = (int64_t)threadIdx.x - lane; base < total; base += blockDim.x) {
+        const int64_t k = base + lane;
+        const bool in = k < total;
+        const int p = p0 + (int)((in ? k : 0) / cap);
+        const unsigned int j = (unsigned int)((in ? k : 0) % cap);
+        const bool want = in && j < part_n[p];
+        const uint64_t* r = parts + ((size_t)p * (size_t)cap + (want ? j : 0u)) * R;
+        uint64_t key[KW];
+        uint64_t c[S];
+#pragma unroll
+        for (int w = 0; w < KW; ++w) key[w] = want ? r[w] : 0ull;
+#pragma unroll
+        for (int w = 0; w < S; ++w) c[w] = (want && w < P::nwords(A)) ? r[KW + w] : 0ull;
+        if (!apply_contrib<KW, S, P>(A, T, use_lds, want, key, c)) {
+            const unsigned int pos = atomicAdd(&A.counters[2], 1u);
+            if (pos < A.ov_rec_cap) {
+                uint64_t* o = A.ov_recs + (size_t)pos * R;
+#pragma unroll
+                for (int w = 0; w < KW; ++w) o[w] = key[w];
+#pragma unroll
+                for (int w = 0; w < S; ++w) {
+                    if (w < P::nwords(A)) o[KW + w] = c[w];
+                }
+            }
+        }
+    }
     lds_table_flush<KW, S, P>(A, T, use_lds);
 }
 

@@ -86,3 +86,34 @@ def test_tuner_caches_one_config_per_shape(gpu_backend, monkeypatch):
     finally:
         lib.qdb_gpu_autotune_enable(was)
         lib.qdb_gpu_autotune_clear()
+
+
+@pytest.mark.parametrize("mixed", [False, True], ids=["few_groups", "mixed"])
+def test_partial_records_equal_oracle(gpu_backend, monkeypatch, mixed):
+    """Tables of few groups under a large grid: workgroups write their groups
+    as partial records and a second launch combines them. Mixed: the rows of
+    half the workgroups (rows are dealt to workgroups in 256-row chunks,
+    round robin) carry a few keys, the other half's thousands, so some
+    workgroups write records and others update the 64-slot table themselves,
+    which then grows and replays what did not fit (rows and combined records)."""
+    import ctypes as C
+    lib = gpu_backend.q.lib()
+    limits = (C.c_int64 * 2)()
+    lib.qdb_gpu_device_limits(limits, 2)
+    grid = int(limits[0]) * 4
+    monkeypatch.setenv("QDB_GPU_TUNE_FORCE", "0,32,4,0,0,1")
+    n = grid * 256 * 2 + 1234
+    case, b = random_agg_case(44, n)
+    case = staged_case(case)
+    rng = np.random.default_rng(45)
+    chunk_wg = (np.arange(n) // 256) % grid
+    k2 = rng.integers(0, 4, n).astype(np.int64)
+    if mixed:
+        many = chunk_wg >= grid // 2
+        k2[many] = rng.integers(0, 3000, int(many.sum()))
+    b.columns[1] = k2
+    got = sorted(result_rows(gpu_backend.aggregate(case, [b, b], capacity=64)), key=sort_key)
+    want = sorted(result_rows(OracleBackend(fsum_mode=2).aggregate(case, [b, b])), key=sort_key)
+    assert len(got) == len(want)
+    for g, w in zip(got, want):
+        assert g == w, f"{g} vs {w}"
