@@ -538,7 +538,7 @@ __global__ void __launch_bounds__(kBlock) k_key_minmax(const uint64_t* keys, con
 // Reduces `n` partials of `words` u64 words each (partial p, word k at
 // part[p * stride + k]) into out[k]: op[k] 0 = sum, 1 = signed min, 2 =
 // signed max, 3 = unsigned min, 4 = unsigned max. One workgroup of kBlock
-// threads; every thread reads its share of the partials.
+// threads; each wave folds whole words.
 struct ReduceOps {
     int32_t op[kMaxFused * 6];
 };
@@ -546,8 +546,10 @@ struct ReduceOps {
 __global__ void __launch_bounds__(kBlock) k_reduce_partials(const unsigned long long* part, int n, int stride, int words,
     ReduceOps ops, unsigned long long* out)
 {
-    __shared__ unsigned long long red[kBlock / kWave];
-    for (int k = 0; k < words; ++k) {
+    // Wave w folds words w, w + kBlock / kWave, ...: its lanes' loads of a
+    // word are independent, and no barrier is needed.
+    const int lane = lane_id();
+    for (int k = threadIdx.x / kWave; k < words; k += kBlock / kWave) {
         const int op = ops.op[k];
         auto comb = [op](unsigned long long a, unsigned long long b) -> unsigned long long {
             switch (op) {
@@ -561,17 +563,10 @@ __global__ void __launch_bounds__(kBlock) k_reduce_partials(const unsigned long 
         const unsigned long long id = op == 0 ? 0ull : op == 1 ? 0x7fffffffffffffffull : op == 2 ? 0x8000000000000000ull
                                                    : op == 3 ? ~0ull : 0ull;
         unsigned long long x = id;
-        for (int p = threadIdx.x; p < n; p += blockDim.x) x = comb(x, part[(size_t)p * stride + k]);
+        for (int p = lane; p < n; p += kWave) x = comb(x, part[(size_t)p * stride + k]);
 #pragma unroll
         for (int o = 1; o < kWave; o <<= 1) x = comb(x, shfl_xor_u64(x, o));
-        if (lane_id() == 0) red[threadIdx.x / kWave] = x;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long t = id;
-            for (int w = 0; w < kBlock / kWave; ++w) t = comb(t, red[w]);
-            out[k] = t;
-        }
-        __syncthreads();
+        if (lane == 0) out[k] = x;
     }
 }
 
