@@ -626,14 +626,26 @@ __device__ __forceinline__ double limbs_to_f64(const int64_t (&limb)[kLimbs])
         if (acc[w]) top = w * 64 + 63 - __builtin_clzll(acc[w]);
     }
     if (top < 0) return 0.0;
-    int lsb = top - 52;
-    if (lsb < 0) lsb = 0;
-    auto bit = [&](int p) -> uint64_t { return p < 0 ? 0 : (acc[p / 64] >> (p % 64)) & 1ull; };
-    uint64_t q = 0;
-    for (int p = top; p >= lsb; --p) q = (q << 1) | bit(p);
-    uint64_t guard = bit(lsb - 1);
-    uint64_t sticky = 0;
-    for (int p = lsb - 2; p >= 0 && !sticky; --p) sticky = bit(p);
+    // The 53 bits from `top` down (fewer when top < 52), then the guard bit
+    // below them and the sticky OR of every bit below the guard, by whole
+    // words (a word index outside the accumulator reads as zero).
+    auto word = [&](int i) -> uint64_t { return i == 0 ? acc[0] : i == 1 ? acc[1] : i == 2 ? acc[2] : i == 3 ? acc[3] : 0ull; };
+    const int lsb = top > 52 ? top - 52 : 0;
+    const int nbits = top - lsb + 1;
+    const int wq = lsb >> 6, bq = lsb & 63;
+    uint64_t q = (word(wq) >> bq) | (bq ? word(wq + 1) << (64 - bq) : 0ull);
+    q &= (1ull << nbits) - 1ull;
+    uint64_t guard = 0, sticky = 0;
+    if (lsb >= 1) {
+        const int gp = lsb - 1;
+        guard = (word(gp >> 6) >> (gp & 63)) & 1ull;
+        const int full = gp >> 6, rem = gp & 63;  // whole words, then bits, below the guard
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            if (w < full) sticky |= acc[w];
+        }
+        if (rem) sticky |= word(full) & ((1ull << rem) - 1ull);
+    }
     if (guard && (sticky || (q & 1))) q += 1;
     double r = ldexp((double)q, lsb + kLimbLo);
     return neg ? -r : r;
