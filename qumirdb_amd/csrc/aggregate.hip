@@ -65,6 +65,7 @@ constexpr int kSortedBlock = 1024;    // threads of a bucket-aggregation workgro
 constexpr int kPartCap = 64;
 constexpr int kPartsPerReducer = 32;
 constexpr uint64_t kPartMaxSlots = 4096;
+static_assert(kPartsPerReducer <= agg::kMaxPartsPerReducer, "combining workgroups take at most kMaxPartsPerReducer slots");
 
 struct StageLayout {
     int words = 0;                    // 0: the plan does not stage

@@ -848,6 +848,8 @@ __device__ __forceinline__ void agg_update_body(const AggArgs& A)
     lds_table_flush<KW, S, P>(A, T, use_lds);
 }
 
+constexpr int kMaxPartsPerReducer = 64;  // slots one workgroup of the combining launch takes at most
+
 // The second launch of an update with partial records: workgroup b
 // combines the records of the update's workgroups [b * per, (b + 1) * per)
 // (slot p holds part_n[p] records of KW + nwords words, part_cap records
@@ -860,24 +862,40 @@ __device__ __forceinline__ void agg_reduce_parts_body(const AggArgs& A, const ui
 {
     extern __shared__ uint64_t lds_raw[];
     __shared__ unsigned int s_inserted;
+    __shared__ unsigned int s_first[kMaxPartsPerReducer + 1];  // exclusive prefix of the slots' record counts
     const bool use_lds = A.lds_slots > 0;
     LdsTable T = lds_table_at<KW, P>(A, lds_raw, &s_inserted);
     lds_table_reset<KW, P>(A, T, use_lds);
     const int R = KW + P::nwords(A);
     const int p0 = (int)blockIdx.x * per;
-    const int p1 = p0 + per < nparts ? p0 + per : nparts;
-    const int64_t cap = A.part_cap;
-    const int64_t total = p1 > p0 ? (int64_t)(p1 - p0) * cap : 0;
+    const int np = p0 + per < nparts ? per : (nparts > p0 ? nparts - p0 : 0);
+    if (threadIdx.x == 0) {
+        unsigned int t = 0;
+        for (int q = 0; q < np; ++q) {
+            s_first[q] = t;
+            t += part_n[p0 + q];
+        }
+        s_first[np] = t;
+    }
+    __syncthreads();
+    const unsigned int total = s_first[np];
+    const uint64_t cap = (uint64_t)A.part_cap;
     const int lane = lane_id();
-    // Every lane of a wave runs the same number of iterations (the lookups
-    // inside are wave-uniform).
-    for (int64_t base = (int64_t)threadIdx.x - lane; base < total; base += blockDim.x) {
-        const int64_t k = base + lane;
-        const bool in = k < total;
-        const int p = p0 + (int)((in ? k : 0) / cap);
-        const unsigned int j = (unsigned int)((in ? k : 0) % cap);
-        const bool want = in && j < part_n[p];
-        const uint64_t* r = parts + ((size_t)p * (size_t)cap + (want ? j : 0u)) * R;
+    // Only the records the slots hold, one per thread; every lane of a wave
+    // runs the same number of iterations (the lookups inside are
+    // wave-uniform).
+    for (unsigned int base = threadIdx.x - (unsigned int)lane; base < total; base += blockDim.x) {
+        const unsigned int k = base + (unsigned int)lane;
+        const bool want = k < total;
+        // The slot of record k: the last q with s_first[q] <= k.
+        int lo = 0, hi = np - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_first[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        const unsigned int j = want ? k - s_first[lo] : 0u;
+        const uint64_t* r = parts + ((uint64_t)(p0 + lo) * cap + j) * (uint64_t)R;
         uint64_t key[KW];
         uint64_t c[S];
 #pragma unroll
