@@ -356,25 +356,32 @@ def bench_batched(n_groupby=256, n_join=256):
     release(plain())
     release(batched())  # warm-up: kernels compiled, shapes tuned, allocator primed
     lib().qdb_gpu_synchronize()
-    lib().qdb_gpu_timing_reset()
-    lib().qdb_gpu_timing_enable(1)
+    # The timed runs: HIP-event regions off (each region's events cost tens
+    # of microseconds of host time, and the two forms open different
+    # numbers of them).
+    lib().qdb_gpu_timing_enable(0)
     t0 = time.perf_counter()
     r_plain = plain()
     lib().qdb_gpu_synchronize()
     t1 = time.perf_counter()
-    upd_ms, upd_n = timing("agg_update")
-    probe_ms, probe_n = timing("join_probe")
-    lib().qdb_gpu_timing_reset()
     d_plain = digest(r_plain)
     t2 = time.perf_counter()
     r_batch = batched()
     lib().qdb_gpu_synchronize()
     t3 = time.perf_counter()
+    d_batch = digest(r_batch)
+    # Kernel durations from one more run of each form with the regions on.
+    lib().qdb_gpu_timing_reset()
+    lib().qdb_gpu_timing_enable(1)
+    release(plain())
+    upd_ms, upd_n = timing("agg_update")
+    probe_ms, probe_n = timing("join_probe")
+    lib().qdb_gpu_timing_reset()
+    release(batched())
     bat_ms, _ = timing("agg_batch")
     bprobe_ms, _ = timing("join_probe_batched")
     lib().qdb_gpu_timing_enable(0)
     lib().qdb_gpu_timing_reset()
-    d_batch = digest(r_batch)
     lib().qdb_gpu_autotune_enable(was_tuning)
     # Results, instance by instance.
     groups = 0

@@ -90,15 +90,15 @@ __device__ __forceinline__ void store_width(char* data, int width, unsigned int 
     }
 }
 
-// One group per lane: a wave takes 64 consecutive groups (its first a
-// multiple of 64), reads their records together (word k of 64 records per
+// Output groups [base, base + 64) of F, one per lane (base a multiple of
+// 64): the wave reads their records together (word k of 64 records per
 // load) and writes each validity bitmap's 64 bits, and a bool key's data
-// bits, as one word from its ballot.
-__global__ void __launch_bounds__(kBlock) k_agg_finalize(FinArgs F)
+// bits, as one word from its ballot. `index`: the slot of output group
+// g - pend_n (null: the group's own number).
+__device__ __forceinline__ void finalize_groups_wave(const FinArgs& F, const uint32_t* index, uint64_t base)
 {
     const int lane = lane_id();
-    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x - lane; base < F.groups; base += step) {
+    {
         const unsigned int g = (unsigned int)(base + lane);
         const bool live = g < F.groups;
         const uint64_t* r = F.recs;
@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(kBlock) k_agg_finalize(FinArgs F)
                 r = F.pend_recs + ((uint64_t)F.pend_items[lo].row0 + (g - F.pend_prefix[lo])) * (uint64_t)F.R;
             } else {
                 const unsigned int gg = g - F.pend_n;
-                r = F.recs + (size_t)(F.index ? F.index[gg] : gg) * (size_t)F.stride;
+                r = F.recs + (size_t)(index ? index[gg] : gg) * (size_t)F.stride;
             }
         }
         const uint64_t word_at = base >> 6;  // the wave's 64-bit bitmap word
@@ -211,6 +211,84 @@ __global__ void __launch_bounds__(kBlock) k_agg_finalize(FinArgs F)
             if (F.out_mask[col] && lane == 0) ((uint64_t*)F.out_mask[col])[word_at] = valid_bits;
         }
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_agg_finalize(FinArgs F)
+{
+    const int lane = lane_id();
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x - lane; base < F.groups; base += step) {
+        finalize_groups_wave(F, F.index, base);
+    }
+}
+
+// The finalize of many small tables of one aggregate in one launch (a
+// batched finish): workgroup b takes table b whole. It lists the slots
+// that hold groups, in slot order, in LDS (the occupancy words split into
+// one contiguous range per thread, counted, scanned, then each thread
+// writes its range's slots; the extra record's slot last), then its waves
+// write the output groups as k_agg_finalize does. One-word keys only, no
+// pending runs, at most kSmallFinMaxSlots slots and kSmallFinMaxGroups
+// groups.
+constexpr int kSmallFinBlock = 1024;
+constexpr unsigned int kSmallFinMaxGroups = 16384;
+constexpr uint64_t kSmallFinMaxSlots = 65536;
+
+struct SmallFin {
+    const uint32_t* occ;   // one bit per slot
+    uint64_t slots;        // the slot table's capacity (the extra record is slot `slots`)
+    int extra_used;
+    int pad;
+};
+
+__global__ void __launch_bounds__(kSmallFinBlock) k_agg_finalize_small(const FinArgs* tabs, const SmallFin* info)
+{
+    __shared__ uint32_t s_index[kSmallFinMaxGroups];
+    __shared__ unsigned int s_wave[kSmallFinBlock / kWave];
+    const FinArgs& F = tabs[blockIdx.x];
+    const SmallFin I = info[blockIdx.x];
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    const uint32_t nw = (uint32_t)((I.slots + 31) / 32);
+    const uint32_t per = (nw + blockDim.x - 1) / blockDim.x;
+    const uint32_t w0 = threadIdx.x * per, w1 = w0 + per < nw ? w0 + per : nw;
+    auto word_at = [&](uint32_t w) -> uint32_t {
+        uint32_t v = I.occ[w];
+        const uint64_t first = (uint64_t)w * 32;
+        if (first + 32 > I.slots) v &= (uint32_t)((1ull << (I.slots - first)) - 1ull);
+        return v;
+    };
+    unsigned int cnt = 0;
+    for (uint32_t w = w0; w < w1; ++w) cnt += (unsigned int)__popc(word_at(w));
+    // Exclusive scan of the threads' counts: within the wave, then the
+    // waves before it.
+    unsigned int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const unsigned int y = (unsigned int)__shfl_up((int)incl, o, kWave);
+        if (lane >= o) incl += y;
+    }
+    if (lane == kWave - 1) s_wave[wave] = incl;
+    __syncthreads();
+    unsigned int before = incl - cnt;
+    unsigned int all = 0;
+    for (int v = 0; v < (int)(blockDim.x / kWave); ++v) {
+        before += v < wave ? s_wave[v] : 0u;
+        all += s_wave[v];
+    }
+    unsigned int at = before;
+    for (uint32_t w = w0; w < w1; ++w) {
+        uint32_t bits = word_at(w);
+        while (bits) {
+            const int b = __ffs(bits) - 1;
+            bits &= bits - 1u;
+            if (at < kSmallFinMaxGroups) s_index[at] = w * 32u + (uint32_t)b;
+            ++at;
+        }
+    }
+    if (threadIdx.x == 0 && I.extra_used && all < kSmallFinMaxGroups) s_index[all] = (uint32_t)I.slots;
+    __syncthreads();
+    const uint64_t step = (uint64_t)blockDim.x;
+    for (uint64_t base = (uint64_t)wave * kWave; base < F.groups; base += step) finalize_groups_wave(F, s_index, base);
 }
 
 // Fills `n` records of `stride` words with the empty-table pattern (the
@@ -2133,19 +2211,67 @@ struct FinishTemps {
 
 // agg_finish_rowset, first half: groups -> columnar TRowSet, every kernel
 // launched, nothing waited for; returns the row count or -1.
+// The output rowset of T (every column's data and validity bitmap in one
+// block of 256-byte aligned parts: one pool call per output instead of one
+// per buffer) and the finalize arguments that write it, the index aside;
+// returns the group count or -1.
+int64_t table_finish_outputs(AggTable* T, TRowSet* out, FinArgs* F)
+{
+    AggPlan* P = T->plan;
+    *F = P->fin;
+    F->recs = T->recs;
+    F->stride = T->rstride;
+    F->index = nullptr;
+    // The pending runs first, then the slot table's groups.
+    F->pend_recs = T->pend_recs;
+    F->pend_items = T->pend_items;
+    F->pend_prefix = T->pend_prefix;
+    F->pend_runs = T->pend_runs;
+    F->pend_n = T->pend_groups;
+    const unsigned int n = table_size(T);
+    OwnedRowSet* o = init_owned_rowset(out, n, P->nout);
+    F->groups = n;
+    // Every column's data and validity bitmap in one block, 256-byte aligned
+    // parts (one pool call per output instead of one per buffer).
+    const size_t mask_bytes = (size_t)((n + 7) / 8) + 8;
+    auto aligned = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t total = 0;
+    for (int c = 0; c < P->nout; ++c) {
+        const QdbValueType& t = P->out_types[c];
+        total += aligned(t.Kind == QDB_KIND_BOOL ? mask_bytes : (size_t)n * (size_t)t.Width + 8);
+        if (t.Nullable) total += aligned(mask_bytes);
+    }
+    char* block = (char*)dalloc(total ? total : 256);
+    if (!block) {
+        return -1;
+    }
+    o->dev.push_back(block);
+    size_t at = 0;
+    for (int c = 0; c < P->nout; ++c) {
+        const QdbValueType& t = P->out_types[c];
+        const size_t bytes = t.Kind == QDB_KIND_BOOL ? mask_bytes : (size_t)n * (size_t)t.Width + 8;
+        char* data = block + at;
+        at += aligned(bytes);
+        if (t.Kind == QDB_KIND_BOOL && !zero_fill(data, bytes)) {
+            return -1;
+        }
+        F->out_data[c] = data;
+        o->cols[(size_t)c].Data = data;
+        uint8_t* mask = nullptr;
+        if (t.Nullable) {
+            mask = (uint8_t*)(block + at);
+            at += aligned(mask_bytes);
+        }
+        F->out_mask[c] = mask;
+        o->cols[(size_t)c].Mask = mask;
+    }
+    return (int64_t)n;
+}
+
 int64_t table_finish_launch(AggTable* T, TRowSet* out, FinishTemps* tmp)
 {
     AggPlan* P = T->plan;
-    FinArgs F = P->fin;
-    F.recs = T->recs;
-    F.stride = T->rstride;
-    F.index = nullptr;
-    // The pending runs first, then the slot table's groups.
-    F.pend_recs = T->pend_recs;
-    F.pend_items = T->pend_items;
-    F.pend_prefix = T->pend_prefix;
-    F.pend_runs = T->pend_runs;
-    F.pend_n = T->pend_groups;
+    FinArgs F;
     uint32_t*& index = tmp->index;
     uint32_t*& counts = tmp->counts;
     uint32_t*& offsets = tmp->offsets;
@@ -2183,39 +2309,13 @@ int64_t table_finish_launch(AggTable* T, TRowSet* out, FinishTemps* tmp)
             release();
             return -1;
         }
-        F.index = index;
     }
-    const unsigned int n = table_size(T);
-    OwnedRowSet* o = init_owned_rowset(out, n, P->nout);
-    F.groups = n;
-    const size_t mask_bytes = (size_t)((n + 7) / 8) + 8;
-    for (int c = 0; c < P->nout; ++c) {
-        const QdbValueType& t = P->out_types[c];
-        const size_t bytes = t.Kind == QDB_KIND_BOOL ? mask_bytes : (size_t)n * (size_t)t.Width + 8;
-        char* data = (char*)dalloc(bytes);
-        if (!data) {
-            release();
-            return -1;
-        }
-        o->dev.push_back(data);
-        if (t.Kind == QDB_KIND_BOOL && !zero_fill(data, bytes)) {
-            release();
-            return -1;
-        }
-        F.out_data[c] = data;
-        o->cols[(size_t)c].Data = data;
-        uint8_t* mask = nullptr;
-        if (t.Nullable) {
-            mask = (uint8_t*)dalloc(mask_bytes);
-            if (!mask) {
-                release();
-                return -1;
-            }
-            o->dev.push_back(mask);
-        }
-        F.out_mask[c] = mask;
-        o->cols[(size_t)c].Mask = mask;
+    const int64_t n = table_finish_outputs(T, out, &F);
+    if (n < 0) {
+        release();
+        return -1;
     }
+    F.index = T->groups > 0 ? index : nullptr;
     if (n > 0) {
         TimedRegion timer("agg_finalize");
         const int grid = grid_for(n, kBlock, 8);
@@ -2465,6 +2565,110 @@ extern "C" int qdb_gpu_agg_update_batched(const QdbAggregateKernels* kernels, vo
     return 0;
 }
 
+// The finalize kernels of every table of a batched finish, launched without
+// waiting. Called from the library stream with at least kSpreadMinTables
+// tables, the tables (largest first) are spread over that stream and up to
+// two more, each driven by a thread of its own that synchronizes its stream
+// at the end (QDB_GPU_BATCH_STREAMS = 1: all on the calling stream): one
+// table's launches are a handful of small kernels, which a single stream
+// runs one after another. rows[i] >= 0 marks a table whose kernels were
+// launched; false after the first failure (the error is kept).
+constexpr int64_t kSpreadMinTables = 8;
+
+bool finish_launch_spread(void* const* hts, TRowSet* outs, int64_t* rows, std::vector<FinishTemps>& tmp, std::vector<int64_t> order)
+{
+    int helpers = 2;
+    if (const char* e = getenv("QDB_GPU_BATCH_STREAMS")) helpers = atoi(e) - 1;
+    if (helpers > kMaxStreams - 2) helpers = kMaxStreams - 2;
+    if (thread_stream_index() != 0 || (int64_t)order.size() < kSpreadMinTables || helpers < 1) {
+        for (int64_t i : order) {
+            rows[i] = table_finish_launch(((HtBlob*)hts[i])->table, &outs[i], &tmp[(size_t)i]);
+            if (rows[i] < 0) return false;
+        }
+        return true;
+    }
+    std::sort(order.begin(), order.end(),
+        [&](int64_t a, int64_t b) { return table_size(((HtBlob*)hts[a])->table) > table_size(((HtBlob*)hts[b])->table); });
+    std::atomic<size_t> next(0);
+    std::atomic<bool> failed(false);
+    std::vector<std::string> errors((size_t)helpers + 1);
+    auto work = [&](int slot) {
+        for (;;) {
+            const size_t k = next.fetch_add(1);
+            if (k >= order.size() || failed.load()) break;
+            const int64_t i = order[k];
+            rows[i] = table_finish_launch(((HtBlob*)hts[i])->table, &outs[i], &tmp[(size_t)i]);
+            if (rows[i] < 0) {
+                errors[(size_t)slot] = last_error_text().empty() ? std::string("aggregate finalize failed") : last_error_text();
+                failed.store(true);
+                break;
+            }
+        }
+        if (slot > 0 && !sync("batched finalize helper") && errors[(size_t)slot].empty()) {
+            errors[(size_t)slot] = last_error_text();
+            failed.store(true);
+        }
+    };
+    std::vector<std::thread> threads;
+    for (int h = 1; h <= helpers; ++h) {
+        threads.emplace_back([&, h]() {
+            if (!set_thread_stream(h)) {
+                errors[(size_t)h] = last_error_text();
+                failed.store(true);
+                return;
+            }
+            work(h);
+        });
+    }
+    work(0);
+    for (auto& t : threads) t.join();
+    if (failed.load()) {
+        for (const std::string& e : errors) {
+            if (!e.empty()) return fail("%s", e.c_str());
+        }
+        return fail("batched finalize failed");
+    }
+    return true;
+}
+
+// Whether table T's finalize can take the one-workgroup form
+// (k_agg_finalize_small).
+bool small_finish_ok(const AggTable* T)
+{
+    return T->plan->KW == 1 && T->pend_runs == 0 && T->groups > 0 && T->occ != nullptr && T->capacity <= kSmallFinMaxSlots &&
+        table_size(T) <= kSmallFinMaxGroups;
+}
+
+// The outputs of tables hts[i], i in `which` (every one small_finish_ok),
+// and one launch of k_agg_finalize_small for all of them, not waited for.
+bool small_finish_launch(void* const* hts, TRowSet* outs, int64_t* rows, const std::vector<int64_t>& which)
+{
+    if (which.empty()) return true;
+    std::vector<FinArgs> fa(which.size());
+    std::vector<SmallFin> sf(which.size());
+    for (size_t k = 0; k < which.size(); ++k) {
+        const int64_t i = which[k];
+        AggTable* T = ((HtBlob*)hts[i])->table;
+        rows[i] = table_finish_outputs(T, &outs[i], &fa[k]);
+        if (rows[i] < 0) return false;
+        sf[k].occ = T->occ;
+        sf[k].slots = T->capacity;
+        sf[k].extra_used = T->extra_used ? 1 : 0;
+        sf[k].pad = 0;
+    }
+    FinArgs* d_fa = (FinArgs*)upload(fa.data(), fa.size() * sizeof(FinArgs));
+    SmallFin* d_sf = (SmallFin*)upload(sf.data(), sf.size() * sizeof(SmallFin));
+    bool ok = d_fa && d_sf;
+    if (ok) {
+        TimedRegion timer("agg_finalize");
+        hipLaunchKernelGGL(k_agg_finalize_small, dim3((unsigned)which.size()), dim3(kSmallFinBlock), 0, stream(), d_fa, d_sf);
+        ok = hip_ok(hipGetLastError(), "small finalize launch");
+    }
+    dfree(d_fa);  // stream-ordered: reused only by work queued after the launch
+    dfree(d_sf);
+    return ok;
+}
+
 // agg_finish_rowset of hts[i] into outs[i] for every i: every table's
 // finalize kernels are launched before one synchronization; each table is
 // destroyed as by agg_finish_rowset. rows[i] gets the row count. Returns 0,
@@ -2483,21 +2687,27 @@ extern "C" int qdb_gpu_agg_finish_batched(const QdbAggregateKernels* kernels, vo
         if (B->table->plan != P) return fail("table %lld belongs to another aggregate", (long long)i), -1;
     }
     std::vector<FinishTemps> tmp((size_t)n);
-    bool ok = true;
-    int64_t made = 0;
-    for (; made < n && ok; ++made) {
-        memset(&outs[made], 0, sizeof(TRowSet));
-        rows[made] = table_finish_launch(((HtBlob*)hts[made])->table, &outs[made], &tmp[(size_t)made]);
-        ok = rows[made] >= 0;
+    for (int64_t i = 0; i < n; ++i) {
+        memset(&outs[i], 0, sizeof(TRowSet));
+        rows[i] = -1;
     }
+    // Small tables: their outputs made here and their finalize in one
+    // launch; the others on up to three streams.
+    std::vector<int64_t> small, rest;
+    for (int64_t i = 0; i < n; ++i) (small_finish_ok(((HtBlob*)hts[i])->table) ? small : rest).push_back(i);
+    if (small.size() < 2) {
+        rest.insert(rest.end(), small.begin(), small.end());
+        small.clear();
+    }
+    bool ok = small_finish_launch(hts, outs, rows, small) && finish_launch_spread(hts, outs, rows, tmp, rest);
     ok = sync("agg finalize") && ok;
-    for (int64_t i = 0; i < made; ++i) {
+    for (int64_t i = 0; i < n; ++i) {
         HtBlob* B = (HtBlob*)hts[i];
-        if (rows[i] >= 0 && !table_finish_done(B->table, &outs[i], &tmp[(size_t)i])) ok = false;
+        if (ok && rows[i] >= 0 && !table_finish_done(B->table, &outs[i], &tmp[(size_t)i])) ok = false;
         tmp[(size_t)i].release();
     }
     if (!ok) {
-        for (int64_t i = 0; i < made; ++i) {
+        for (int64_t i = 0; i < n; ++i) {
             destroy_kernel_rowset(&outs[i]);
             memset(&outs[i], 0, sizeof(TRowSet));
         }

@@ -2,6 +2,8 @@
 synchronization and probed in one persistent launch from a device work queue
 give, instance by instance, the pairs and fused results of separate
 Append/Build/Probe calls (bit-exact), and one instance equals the oracle."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -213,3 +215,55 @@ def test_detached_probe_equals_plain(gpu_backend, dup):
         t.join()
     assert not errors, errors
     proc.close()
+
+
+def test_batched_finish_small_tables_equal_plain(gpu_backend):
+    """A batched finish of many tables with one-word keys: the tables of at
+    most 65,536 slots and 16,384 groups are finished by one launch (one
+    workgroup per table lists its occupied slots in LDS), the others as by
+    their own finish; every output equals that table's plain update and
+    finish (as sorted rows: concurrent inserts place groups in slots in any
+    order), and one equals the oracle. Some instances
+    hold the key equal to the slot table's empty marker (INT64_MIN, kept in
+    the extra record), some grow from an 8-slot table, one is empty."""
+    import qumirdb_amd as q
+    from cases import AGG_MAX, OracleBackend, AggCase
+    from helpers import FLOAT, result_rows, sort_key
+    rng = np.random.default_rng(77)
+    types = [T(SIGNED, 8), T(SIGNED, 8), T(FLOAT, 8)]
+    case = AggCase(types, [0], [(AGG_COUNT_STAR, -1), (AGG_SUM, 1), (AGG_MIN, 2), (AGG_MAX, 2), (AGG_SUM, 2)])
+    specs = [(3000, 40), (1, 1), (0, 1), (50_000, 9000), (200_000, 15_000), (120_000, 30_000), (80_000, 300),
+             (10_000, 10_000), (64_000, 5), (2_000, 2_000), (300_000, 60_000), (40_000, 4_000)]
+    batches, caps = [], []
+    for k, (n, card) in enumerate(specs):
+        keys = rng.integers(-card, card, n).astype(np.int64) * 7919
+        if k % 3 == 1 and n:
+            keys[rng.integers(0, n, max(1, n // 10))] = np.iinfo(np.int64).min
+        vi = rng.integers(-10 ** 12, 10 ** 12, n).astype(np.int64)
+        vf = rng.normal(0, 1e3, n)
+        batches.append(Batch(types, [keys, vi, vf]))
+        caps.append(8 if k % 4 == 0 else 1 << max(3, int(2 * min(n, 2 * card)).bit_length()))
+    kern = q.AggregateKernels.__new__(q.AggregateKernels)
+    kern.spec = None
+    kern.c_spec = gpu_backend.cast(case.to_c())
+    kern.k = q.abi.QdbAggregateKernels()
+    q.check(q.lib().qdb_gpu_compile_aggregate(C.byref(kern.c_spec), C.byref(kern.k)), "compile_aggregate")
+    kern.out_types = [gpu_backend.H.ValueType(t.Kind, t.Width, t.Scale, bool(t.Nullable))
+                      for t in kern.k.OutputTypes[: kern.k.NumOutputColumns]]
+    devs = [gpu_backend.dev(b) for b in batches]
+    plain = []
+    for d, cap in zip(devs, caps):
+        p = q.TAggregateProcessor(kern, initial_capacity=cap)
+        p.add(d)
+        plain.append(sorted(result_rows(p.finish()), key=sort_key))
+        p.close()
+    procs = [q.TAggregateProcessor(kern, initial_capacity=cap) for cap in caps]
+    for p, d in zip(procs, devs):
+        p.add(d)
+    finished = q.TAggregateProcessor.finish_batched(procs)
+    for i, p in enumerate(procs):
+        p.close()
+        assert sorted(result_rows(finished[i]), key=sort_key) == plain[i], f"instance {i} ({specs[i]})"
+    want = sorted(result_rows(OracleBackend(fsum_mode=2).aggregate(case, [batches[1 + 3]])), key=sort_key)
+    assert plain[4] == want
+    q.lib().qdb_gpu_release_slot(q.abi.SLOT_AGGREGATE, kern.k.Slot)
