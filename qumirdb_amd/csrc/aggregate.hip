@@ -58,6 +58,10 @@ constexpr int64_t kSortedMinRows = (int64_t)1 << 22;
 constexpr uint64_t kSortedMinSlots = (uint64_t)1 << 20;
 constexpr int64_t kSortedItemRows = (int64_t)1 << 16;
 constexpr int kSortedBlock = 1024;    // threads of a bucket-aggregation workgroup
+// Rows of a bucketed batch from which its hot keys are sampled
+// (QDB_GPU_HOT_MIN_ROWS moves it, down to the sample's size).
+constexpr int64_t kHotMinRows = (int64_t)1 << 22;
+bool hot_keys_enabled();
 // Partial records of an update (AggArgs::part_recs): at most kPartCap
 // groups per workgroup slot, kPartsPerReducer slots per workgroup of the
 // combining launch; used for tables of at most kPartMaxSlots slots that
@@ -1223,12 +1227,23 @@ std::string agg_jit_source(const AggPlan* P)
     if (L.words > 0) {
         const std::string kw = std::to_string(P->KW), sw = std::to_string(P->S), w = std::to_string(L.words);
         o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kSortedBlock) +
-            ") qdb_agg_count(qdbgpu::agg::AggArgs A, unsigned int* hist, int shift, unsigned int nb, long long chunk)\n{\n";
-        o += "    qdbgpu::agg::agg_count_body<" + kw + ", " + w + ", qdbgpu::agg::JitPolicy>(A, hist, shift, nb, chunk);\n}\n";
+            ") qdb_agg_count(qdbgpu::agg::AggArgs A, unsigned int* hist, int shift, unsigned int nb, long long chunk, "
+            "const unsigned long long* hot_keys, const unsigned int* hot_n)\n{\n";
+        o += "    qdbgpu::agg::agg_count_body<" + kw + ", " + w + ", qdbgpu::agg::JitPolicy>(A, hist, shift, nb, chunk, hot_keys, hot_n);\n}\n";
         o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kSortedBlock) +
             ") qdb_agg_scatter(qdbgpu::agg::AggArgs A, const unsigned int* offs, unsigned long long* staged, int shift, unsigned int nb, "
-            "long long chunk)\n{\n";
-        o += "    qdbgpu::agg::agg_scatter_body<" + kw + ", " + w + ", qdbgpu::agg::JitPolicy>(A, offs, staged, shift, nb, chunk);\n}\n";
+            "long long chunk, const unsigned long long* hot_keys, const unsigned int* hot_n, unsigned long long* hot_parts)\n{\n";
+        o += "    qdbgpu::agg::agg_scatter_body<" + kw + ", " + sw + ", " + w +
+            ", qdbgpu::agg::JitPolicy>(A, offs, staged, shift, nb, chunk, hot_keys, hot_n, hot_parts);\n}\n";
+        if (P->KW == 1) {
+            o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(agg::kHotSampleBlock) +
+                ") qdb_agg_hot_sample(qdbgpu::agg::AggArgs A, long long n, unsigned long long* hot_keys, unsigned int* hot_n)\n{\n";
+            o += "    qdbgpu::agg::agg_hot_sample_body<1, qdbgpu::agg::JitPolicy>(A, n, hot_keys, hot_n);\n}\n";
+            o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kBlock) +
+                ") qdb_agg_hot_reduce(qdbgpu::agg::AggArgs A, const unsigned long long* parts, int nblk, const unsigned int* hot_n, "
+                "unsigned long long* out)\n{\n";
+            o += "    qdbgpu::agg::agg_hot_reduce_body<1, qdbgpu::agg::JitPolicy>(A, parts, nblk, hot_n, out);\n}\n";
+        }
         o += "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(kSortedBlock) +
             ") qdb_agg_sorted(qdbgpu::agg::AggArgs A, const unsigned long long* staged, const qdbgpu::agg::SortedItem* items, int nitems, "
             "unsigned int* head, unsigned long long* recs, unsigned int* counts)\n{\n";
@@ -1711,6 +1726,23 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n, int fanou
     unsigned long long* recs = nullptr;
     SortedItem* d_items = nullptr;
     unsigned int* counts = nullptr;
+    // Hot keys (one-word keys, batches of at least kHotMinRows rows;
+    // QDB_GPU_HOT_KEYS=0 turns them off): a sample names them, the count
+    // leaves their rows out and the scatter aggregates them in LDS.
+    const size_t hot_base = ((size_t)nb * 4 + 7) & ~(size_t)7;
+    const size_t scatter_hot_lds = hot_base + agg::kHotSetBytes + (size_t)agg::kHotMax * (size_t)R * 8;
+    hipFunction_t f_hot_sample = nullptr, f_hot_reduce = nullptr;
+    int64_t hot_min_rows = kHotMinRows;
+    if (const char* e = getenv("QDB_GPU_HOT_MIN_ROWS")) hot_min_rows = atoll(e) > agg::kHotSamples ? atoll(e) : agg::kHotSamples;
+    bool hot = KW == 1 && n >= hot_min_rows && hot_keys_enabled() && scatter_hot_lds + 1024 <= max_lds_per_block();
+    if (hot) {
+        f_hot_sample = plan_jit_function(P, "qdb_agg_hot_sample");
+        f_hot_reduce = plan_jit_function(P, "qdb_agg_hot_reduce");
+        hot = f_hot_sample && f_hot_reduce;
+    }
+    unsigned long long* hot_keys = hot ? (unsigned long long*)dalloc((size_t)agg::kHotMax * 8) : nullptr;
+    unsigned int* hot_n = hot ? (unsigned int*)dalloc(16) : nullptr;
+    unsigned long long* hot_parts = hot ? (unsigned long long*)dalloc((size_t)nblk * agg::kHotMax * (size_t)R * 8) : nullptr;
     auto release = [&]() {
         dfree(hist);
         dfree(offs);
@@ -1720,8 +1752,11 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n, int fanou
         dfree(recs);
         dfree(d_items);
         dfree(counts);
+        dfree(hot_keys);
+        dfree(hot_n);
+        dfree(hot_parts);
     };
-    if (!hist || !offs || !staged || !starts || !head) {
+    if (!hist || !offs || !staged || !starts || !head || (hot && (!hot_keys || !hot_n || !hot_parts))) {
         release();
         return false;
     }
@@ -1734,10 +1769,22 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n, int fanou
     TimedRegion timer("agg_update");
     int shift = 64 - bits;
     const unsigned int hist_lds = nb * 4;
+    if (hot) {
+        long long nn = (long long)n;
+        void* args[] = {&A, &nn, &hot_keys, &hot_n};
+        if (!hip_ok(hipModuleLaunchKernel(f_hot_sample, 1, 1, 1, agg::kHotSampleBlock, 1, 1, 0, stream(), args, nullptr),
+                "hot sample launch")) {
+            release();
+            return false;
+        }
+    }
     {
         unsigned int nbv = nb;
-        void* args[] = {&A, &hist, &shift, &nbv, &chunk};
-        if (!hip_ok(hipModuleLaunchKernel(f_count, (unsigned)nblk, 1, 1, kSortedBlock, 1, 1, hist_lds, stream(), args, nullptr),
+        const unsigned long long* hk = hot_keys;
+        const unsigned int* hn = hot_n;
+        void* args[] = {&A, &hist, &shift, &nbv, &chunk, &hk, &hn};
+        const unsigned int lds = hot ? (unsigned int)(hot_base + agg::kHotSetBytes) : hist_lds;
+        if (!hip_ok(hipModuleLaunchKernel(f_count, (unsigned)nblk, 1, 1, kSortedBlock, 1, 1, lds, stream(), args, nullptr),
                 "count launch")) {
             release();
             return false;
@@ -1749,8 +1796,11 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n, int fanou
     }
     {
         unsigned int nbv = nb;
-        void* args[] = {&A, &offs, &staged, &shift, &nbv, &chunk};
-        if (!hip_ok(hipModuleLaunchKernel(f_scatter, (unsigned)nblk, 1, 1, kSortedBlock, 1, 1, hist_lds, stream(), args, nullptr),
+        const unsigned long long* hk = hot_keys;
+        const unsigned int* hn = hot_n;
+        void* args[] = {&A, &offs, &staged, &shift, &nbv, &chunk, &hk, &hn, &hot_parts};
+        const unsigned int lds = hot ? (unsigned int)scatter_hot_lds : hist_lds;
+        if (!hip_ok(hipModuleLaunchKernel(f_scatter, (unsigned)nblk, 1, 1, kSortedBlock, 1, 1, lds, stream(), args, nullptr),
                 "scatter launch")) {
             release();
             return false;
@@ -1759,8 +1809,10 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n, int fanou
     // Bucket b starts where the first range's run of it starts.
     hipLaunchKernelGGL(k_gather_starts, dim3(grid_for(nb, kBlock, 1)), dim3(kBlock), 0, stream(), offs, nb, (unsigned int)nblk, starts);
     std::vector<uint32_t> h_starts((size_t)nb + 1);
+    unsigned int nhot = 0;
     if (!hip_ok(hipGetLastError(), "bucket launch") ||
-        !hip_ok(hipMemcpyAsync(h_starts.data(), starts, (size_t)nb * 4, hipMemcpyDeviceToHost, stream()), "starts") || !sync("starts")) {
+        !hip_ok(hipMemcpyAsync(h_starts.data(), starts, (size_t)nb * 4, hipMemcpyDeviceToHost, stream()), "starts") ||
+        (hot && !hip_ok(hipMemcpyAsync(&nhot, hot_n, 4, hipMemcpyDeviceToHost, stream()), "hot count")) || !sync("starts")) {
         release();
         return false;
     }
@@ -1779,21 +1831,44 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n, int fanou
             items.push_back(it);
         }
     }
+    // The bucket aggregate takes items [0, nsorted); the hot keys' records,
+    // when there are any, are one more item after them (records n, n + 1,
+    // ...), owned: no bucket holds a hot key.
+    int nsorted = (int)items.size();
+    if (nhot > 0) {
+        SortedItem it;
+        it.row0 = n;
+        it.row1 = n + (int64_t)nhot;
+        it.owned = 1;
+        it.pad = 0;
+        items.push_back(it);
+    }
     int nitems = (int)items.size();
     if (nitems == 0) {
         release();
         return true;
     }
-    recs = (unsigned long long*)dalloc((size_t)n * R * 8 + 16);
+    recs = (unsigned long long*)dalloc((size_t)(n + (hot ? agg::kHotMax : 0)) * R * 8 + 16);
     d_items = (SortedItem*)upload(items.data(), items.size() * sizeof(SortedItem));
     counts = (unsigned int*)dalloc((size_t)nitems * 4 + 16);
     if (!recs || !d_items || !counts || !zero_fill(head, 16)) {
         release();
         return false;
     }
-    {
-        const int grid = nitems < num_cus() ? nitems : num_cus();
-        void* args[] = {&A, &staged, &d_items, &nitems, &head, &recs, &counts};
+    if (nhot > 0) {
+        unsigned long long* hot_out = recs + (size_t)n * R;
+        int nblk_i = (int)nblk;
+        void* args[] = {&A, &hot_parts, &nblk_i, &hot_n, &hot_out};
+        const unsigned int grid = (nhot * kWave + kBlock - 1) / kBlock;
+        if (!hip_ok(hipModuleLaunchKernel(f_hot_reduce, grid, 1, 1, kBlock, 1, 1, 0, stream(), args, nullptr), "hot reduce launch") ||
+            !hip_ok(hipMemcpyAsync(counts + nsorted, &nhot, 4, hipMemcpyHostToDevice, stream()), "hot item count")) {
+            release();
+            return false;
+        }
+    }
+    if (nsorted > 0) {
+        const int grid = nsorted < num_cus() ? nsorted : num_cus();
+        void* args[] = {&A, &staged, &d_items, &nsorted, &head, &recs, &counts};
         if (!hip_ok(hipModuleLaunchKernel(f_sorted, (unsigned)grid, 1, 1, kSortedBlock, 1, 1, (unsigned)lds, stream(), args, nullptr),
                 "bucket aggregate launch")) {
             release();
@@ -1895,6 +1970,13 @@ bool table_update_sorted(AggTable* T, const AggArgs& bound, int64_t n, int fanou
     L.release();
     release();
     return ok;
+}
+
+// Whether bucketed updates look for hot keys (QDB_GPU_HOT_KEYS=0: no).
+bool hot_keys_enabled()
+{
+    const char* e = getenv("QDB_GPU_HOT_KEYS");
+    return !(e && e[0] == '0');
 }
 
 // Whether the bucketed path can take a batch of `rows` rows of this plan.

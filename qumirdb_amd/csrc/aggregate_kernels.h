@@ -987,11 +987,197 @@ constexpr int kStageUnroll = 4;                    // rows per thread per step i
 // k]. After an exclusive scan of hist, scattering: each row's staged words
 // at the next position of its bucket's run for the range. A row that does
 // not take part counts in the last bucket and is not written.
+// Hot keys of a bucketed update (one-word keys): the rows of the keys that a
+// sample of the batch shows most often are aggregated in the scatter's LDS
+// instead of being staged, sorted and merged (Zipf-like inputs put a large
+// share of the rows on a few hundred keys). Each scatter workgroup writes
+// its states of the hot keys as records to a slot of its own, and one wave
+// per hot key combines the slots into that key's record. No other record
+// of the update holds a hot key.
+constexpr int kHotMax = 512;                 // hot keys at most
+constexpr int kHotSlots = 4096;              // LDS hash slots of the hot set (a power of two; 1/8 full at most)
+constexpr int kHotSamples = 32768;           // rows sampled, evenly spaced
+constexpr int kHotSampleSlots = 8192;        // LDS hash slots of the sample's counts
+constexpr unsigned int kHotMinCount = 4;     // sample count that makes a key hot
+constexpr int kHotSampleBlock = 1024;        // threads of the sampling workgroup
+constexpr size_t kHotSetBytes = (size_t)kHotSlots * 8 + (size_t)kHotSlots * 2;
+
+// The LDS hot set: keys by hash slot (kEmptyKey: free) and their indexes in
+// the hot list.
+struct HotSet {
+    unsigned long long* keys;
+    unsigned short* idx;
+};
+
+__device__ __forceinline__ unsigned int hot_slot(uint64_t h)
+{
+    return (unsigned int)(h >> 20) & (unsigned int)(kHotSlots - 1);
+}
+
+// The hot set at `base` (8-byte aligned) filled from hot_keys[0, n) (every
+// thread; ends with a barrier). The keys are distinct and none is kEmptyKey.
+__device__ __forceinline__ HotSet hot_set_load(void* base, const unsigned long long* hot_keys, unsigned int n)
+{
+    HotSet H;
+    H.keys = (unsigned long long*)base;
+    H.idx = (unsigned short*)(H.keys + kHotSlots);
+    for (int i = threadIdx.x; i < kHotSlots; i += blockDim.x) H.keys[i] = kEmptyKey;
+    __syncthreads();
+    for (unsigned int i = threadIdx.x; i < n; i += blockDim.x) {
+        const unsigned long long k = hot_keys[i];
+        uint64_t key1[1] = {k};
+        unsigned int slot = hot_slot(key_place_hash<1>(key1));
+        for (int p = 0; p < kHotSlots; ++p) {
+            if (atomicCAS(&H.keys[slot], kEmptyKey, k) == kEmptyKey) {
+                H.idx[slot] = (unsigned short)i;
+                break;
+            }
+            slot = (slot + 1u) & (unsigned int)(kHotSlots - 1);
+        }
+    }
+    __syncthreads();
+    return H;
+}
+
+// The hot index of `key` (hash h), or -1. kEmptyKey marks free slots and
+// is never hot.
+__device__ __forceinline__ int hot_find(const HotSet& H, uint64_t key, uint64_t h)
+{
+    if (key == kEmptyKey) return -1;
+    unsigned int slot = hot_slot(h);
+    for (int p = 0; p < kHotSlots; ++p) {
+        const unsigned long long k = H.keys[slot];
+        if (k == key) return (int)H.idx[slot];
+        if (k == kEmptyKey) return -1;
+        slot = (slot + 1u) & (unsigned int)(kHotSlots - 1);
+    }
+    return -1;
+}
+
+// One workgroup of kHotSampleBlock threads: the keys of kHotSamples evenly
+// spaced rows of [row_base, row_base + n) counted in an LDS table; every key
+// seen at least kHotMinCount times goes to hot_keys (at most kHotMax), and
+// their number to *hot_n. A key that finds no slot within 8 probes is not
+// counted: the table fills up with a skewed sample's rare keys, and a hot
+// key claims its slot among the first samples.
+template <int KW, class P>
+__device__ __forceinline__ void agg_hot_sample_body(const AggArgs& A, int64_t n, unsigned long long* hot_keys, unsigned int* hot_n)
+{
+    __shared__ unsigned long long sk[kHotSampleSlots];
+    __shared__ unsigned int sc[kHotSampleSlots];
+    __shared__ unsigned int s_n;
+    for (int i = threadIdx.x; i < kHotSampleSlots; i += blockDim.x) {
+        sk[i] = kEmptyKey;
+        sc[i] = 0u;
+    }
+    if (threadIdx.x == 0) s_n = 0u;
+    __syncthreads();
+    if constexpr (KW == 1) {
+        constexpr int kU = 8;  // rows per thread whose loads are in flight together
+        const int per = kHotSamples / kHotSampleBlock;
+        for (int k0 = 0; k0 < per; k0 += kU) {
+            uint64_t key[kU][1];
+            bool ok[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int64_t sidx = (int64_t)threadIdx.x + (int64_t)(k0 + u) * kHotSampleBlock;
+                const int64_t row = sidx * n / kHotSamples;
+                ok[u] = P::template stage_key<KW>(A, A.row_base + row, key[u]);
+            }
+            // Adds `cnt` to the count of key k: a plain read finds a key
+            // already in its slot, a CAS claims a free one.
+            auto add = [&](unsigned long long k, unsigned int cnt) {
+                uint64_t k1[1] = {k};
+                unsigned int slot = (unsigned int)(key_place_hash<1>(k1) >> 40) & (unsigned int)(kHotSampleSlots - 1);
+                for (int p = 0; p < 8; ++p) {
+                    unsigned long long cur = sk[slot];
+                    if (cur == kEmptyKey) cur = atomicCAS(&sk[slot], kEmptyKey, k);
+                    if (cur == kEmptyKey || cur == k) {
+                        atomicAdd(&sc[slot], cnt);
+                        break;
+                    }
+                    slot = (slot + 1u) & (unsigned int)(kHotSampleSlots - 1);
+                }
+            };
+            const int lane = lane_id();
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                bool want = ok[u] && key[u][0] != kEmptyKey;
+                // The lanes that share the first wanting lane's key count once
+                // (the most frequent keys fill whole waves of the sample).
+                const uint64_t wm = __ballot(want);
+                if (wm) {
+                    const int leader = __ffsll((unsigned long long)wm) - 1;
+                    const unsigned long long lk = shfl_u64(key[u][0], leader);
+                    const bool same = want && key[u][0] == lk;
+                    const uint64_t sm = __ballot(same);
+                    if (same) {
+                        want = false;
+                        if (lane == leader) add(lk, (unsigned int)__popcll(sm));
+                    }
+                }
+                if (want) add(key[u][0], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHotSampleSlots; i += blockDim.x) {
+        if (sk[i] != kEmptyKey && sc[i] >= kHotMinCount) {
+            const unsigned int pos = atomicAdd(&s_n, 1u);
+            if (pos < (unsigned int)kHotMax) hot_keys[pos] = sk[i];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *hot_n = s_n < (unsigned int)kHotMax ? s_n : (unsigned int)kHotMax;
+}
+
+// One wave per hot key h < *hot_n: the state words of the key's records in
+// the scatter workgroups' slots (parts + (b * kHotMax + h) * R, b < nblk)
+// combined into out + h * R (key word first).
+template <int KW, class P>
+__device__ __forceinline__ void agg_hot_reduce_body(const AggArgs& A, const unsigned long long* parts, int nblk,
+    const unsigned int* hot_n, unsigned long long* out)
+{
+    const int R = KW + P::nwords(A);
+    const int lane = lane_id();
+    const int nh = (int)*hot_n;
+    const int waves = (int)(gridDim.x * blockDim.x / kWave);
+    for (int h = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave); h < nh; h += waves) {
+        for (int w = 0; w < P::nwords(A); ++w) {
+            const uint8_t op = P::op(A, w);
+            uint64_t acc = identity_of(op);
+            for (int b = lane; b < nblk; b += kWave) acc = combine(op, acc, parts[((size_t)b * kHotMax + h) * R + KW + w]);
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) acc = combine(op, acc, shfl_xor_u64(acc, o));
+            if (lane == 0) out[(size_t)h * R + KW + w] = acc;
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int w = 0; w < KW; ++w) out[(size_t)h * R + w] = parts[(size_t)h * R + w];
+        }
+    }
+}
+
+// Bucket partition of the staged rows in two passes over the same
+// contiguous row ranges (workgroup k: rows [k * chunk, (k + 1) * chunk)).
+// Counting: per-bucket counts of the range into hist[bucket * gridDim.x +
+// k]. After an exclusive scan of hist, scattering: each row's staged words
+// at the next position of its bucket's run for the range. A row that does
+// not take part, or whose key is hot (hot_keys non-null), counts in the
+// last bucket and is not written; the scatter aggregates the hot rows in
+// LDS and writes its states of the hot keys to hot_parts + blockIdx.x *
+// kHotMax * (KW + nwords).
 template <int KW, int W, class P>
-__device__ __forceinline__ void agg_count_body(const AggArgs& A, unsigned int* hist, int shift, unsigned int nb, int64_t chunk)
+__device__ __forceinline__ void agg_count_body(const AggArgs& A, unsigned int* hist, int shift, unsigned int nb, int64_t chunk,
+    const unsigned long long* hot_keys, const unsigned int* hot_n)
 {
     extern __shared__ unsigned int lh[];
     for (unsigned int b = threadIdx.x; b < nb; b += blockDim.x) lh[b] = 0u;
+    const unsigned int nhot = (KW == 1 && hot_keys) ? *hot_n : 0u;
+    HotSet H;
+    H.keys = nullptr;
+    H.idx = nullptr;
+    if (nhot) H = hot_set_load(lh + ((nb + 1u) & ~1u), hot_keys, nhot);  // barrier inside
     __syncthreads();
     const int64_t r0 = blockIdx.x * chunk;
     const int64_t r1 = r0 + chunk < A.nrows ? r0 + chunk : A.nrows;
@@ -1009,7 +1195,10 @@ __device__ __forceinline__ void agg_count_body(const AggArgs& A, unsigned int* h
 #pragma unroll
         for (int u = 0; u < kStageUnroll; ++u) {
             if (i0 + (int64_t)u * blockDim.x >= r1) continue;
-            const unsigned int b = ok[u] ? (unsigned int)(key_place_hash<KW>(key[u]) >> shift) : nb - 1u;
+            const uint64_t h = key_place_hash<KW>(key[u]);
+            bool keep = ok[u];
+            if (keep && nhot) keep = hot_find(H, key[u][0], h) < 0;
+            const unsigned int b = keep ? (unsigned int)(h >> shift) : nb - 1u;
             atomicAdd(&lh[b], 1u);
         }
     }
@@ -1017,12 +1206,29 @@ __device__ __forceinline__ void agg_count_body(const AggArgs& A, unsigned int* h
     for (unsigned int b = threadIdx.x; b < nb; b += blockDim.x) hist[(uint64_t)b * gridDim.x + blockIdx.x] = lh[b];
 }
 
-template <int KW, int W, class P>
+template <int KW, int S, int W, class P>
 __device__ __forceinline__ void agg_scatter_body(const AggArgs& A, const unsigned int* offs, unsigned long long* staged, int shift,
-    unsigned int nb, int64_t chunk)
+    unsigned int nb, int64_t chunk, const unsigned long long* hot_keys, const unsigned int* hot_n, unsigned long long* hot_parts)
 {
     extern __shared__ unsigned int cur[];
     for (unsigned int b = threadIdx.x; b < nb; b += blockDim.x) cur[b] = offs[(uint64_t)b * gridDim.x + blockIdx.x];
+    const unsigned int nhot = (KW == 1 && hot_keys) ? *hot_n : 0u;
+    const int R = KW + P::nwords(A);
+    HotSet H;
+    H.keys = nullptr;
+    H.idx = nullptr;
+    struct {
+        uint64_t* recs;
+    } HT = {nullptr};  // the hot keys' states as LDS records (key word, state words)
+    if (nhot) {
+        void* hb = cur + ((nb + 1u) & ~1u);
+        HT.recs = (uint64_t*)((char*)hb + kHotSetBytes);
+        for (unsigned int k = threadIdx.x; k < nhot * (unsigned int)R; k += blockDim.x) {
+            const unsigned int h = k / (unsigned int)R, w = k % (unsigned int)R;
+            HT.recs[k] = w < (unsigned int)KW ? hot_keys[h] : P::init(A, (int)w - KW);
+        }
+        H = hot_set_load(hb, hot_keys, nhot);  // barrier inside
+    }
     __syncthreads();
     const int64_t r0 = blockIdx.x * chunk;
     const int64_t r1 = r0 + chunk < A.nrows ? r0 + chunk : A.nrows;
@@ -1040,11 +1246,38 @@ __device__ __forceinline__ void agg_scatter_body(const AggArgs& A, const unsigne
             uint64_t key[KW];
 #pragma unroll
             for (int w = 0; w < KW; ++w) key[w] = st[u][w];
-            const unsigned int b = (unsigned int)(key_place_hash<KW>(key) >> shift);
-            const unsigned int pos = atomicAdd(&cur[b], 1u);
+            const uint64_t h = key_place_hash<KW>(key);
+            const int hi = nhot ? hot_find(H, key[0], h) : -1;
+            if (hi >= 0) {
+                uint64_t k2[KW];
+                uint64_t c[S];
+                P::template unstage<KW, S>(A, st[u], k2, c);
+                // The hot key's LDS record (the bucketed path has no i128
+                // MIN/MAX pairs): atomics, a MIN/MAX word skipped when a plain
+                // read already dominates. (apply_words inlined here crashes
+                // ROCm 7.2's SimplifyCFG.)
+                uint64_t* hr = HT.recs + (size_t)hi * R + KW;
 #pragma unroll
-            for (int w = 0; w < W; ++w) staged[(uint64_t)pos * W + w] = st[u][w];
+                for (int w = 0; w < S; ++w) {
+                    if (w < P::nwords(A)) {
+                        const int op = P::op(A, w);
+                        const bool mm = op == OP_MIN_I64 || op == OP_MAX_I64 || op == OP_MIN_U64 || op == OP_MAX_U64;
+                        const uint64_t now = mm ? __hip_atomic_load(&hr[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0ull;
+                        if (!mm || combine((uint8_t)op, now, c[w]) != now) atomic_apply_lds((uint8_t)op, &hr[w], c[w]);
+                    }
+                }
+            } else {
+                const unsigned int b = (unsigned int)(h >> shift);
+                const unsigned int pos = atomicAdd(&cur[b], 1u);
+#pragma unroll
+                for (int w = 0; w < W; ++w) staged[(uint64_t)pos * W + w] = st[u][w];
+            }
         }
+    }
+    if (nhot) {
+        __syncthreads();
+        unsigned long long* dst = hot_parts + (size_t)blockIdx.x * kHotMax * R;
+        for (unsigned int k = threadIdx.x; k < nhot * (unsigned int)R; k += blockDim.x) dst[k] = HT.recs[k];
     }
 }
 

@@ -1304,3 +1304,31 @@ def test_partial_aggregate_without_projection_keeps_hashing(gpu_backend):
     last.close()
     pp.close()
     ak.release()
+
+
+@pytest.mark.parametrize("hot", ["1", "0"], ids=["hot_keys", "no_hot_keys"])
+def test_bucketed_one_word_key_hot_keys_vs_oracle(gpu_backend, monkeypatch, hot):
+    """The bucketed path with a one-word key whose rows are skewed (40% on
+    one key, 25% over 30 more, the rest over 500K keys), a selection, and
+    one key equal to INT64_MIN: with hot keys (QDB_GPU_HOT_MIN_ROWS lowers
+    their row threshold) a sample names the frequent keys, the scatter
+    aggregates their rows in LDS and one record per hot key joins the
+    update; results equal the oracle with and without, in an empty table
+    (pending runs) and then in a table that already holds groups (merged)."""
+    monkeypatch.setenv("QDB_GPU_BUCKETED_MIN_ROWS", str(1 << 20))
+    monkeypatch.setenv("QDB_GPU_HOT_MIN_ROWS", str(1 << 20))
+    monkeypatch.setenv("QDB_GPU_HOT_KEYS", hot)
+    rng = np.random.default_rng(91)
+    n = 1_300_000
+    u = rng.random(n)
+    key = np.where(u < 0.4, 11, np.where(u < 0.65, rng.integers(100, 131, n), rng.integers(-250_000, 250_000, n))).astype(np.int64)
+    key[rng.integers(0, n, 5000)] = np.iinfo(np.int64).min
+    vi = rng.integers(-10 ** 12, 10 ** 12, n).astype(np.int64)
+    vf = rng.normal(0, 1e3, n)
+    sel = (rng.random(n) > 0.1).astype(np.uint8)
+    types = [T(SIGNED, 8), T(SIGNED, 8), T(FLOAT, 8)]
+    case = AggCase(types, [0], [(AGG_COUNT_STAR, -1), (AGG_SUM, 1), (AGG_MIN, 1), (AGG_MIN, 2), (AGG_MAX, 2), (AGG_SUM, 2)])
+    b = Batch(types, [key, vi, vf], None, sel)
+    got = sorted(result_rows(gpu_backend.aggregate(case, [b, b], capacity=1 << 21)), key=sort_key)
+    want = sorted(result_rows(OracleBackend(fsum_mode=2).aggregate(case, [b, b])), key=sort_key)
+    same(got, want)
