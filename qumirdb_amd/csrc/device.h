@@ -353,7 +353,17 @@ __device__ __forceinline__ DVal exec_insn(const DevInsn& in, const DVal& a, cons
             o.f = o.valid ? z : 0.0;
         } else if (o.valid) {
             u128 x = (u128)a.i, y = (u128)b.i;
-            o.i = (i128)(in.op == QDB_EXPR_ADD ? x + y : in.op == QDB_EXPR_SUB ? x - y : x * y);
+            if (in.op == QDB_EXPR_MUL) {
+                // Operands that fit in 32 bits (decimal prices, rates) have
+                // an exact 64-bit product; the full 128-bit multiply only for
+                // the others.
+                const bool small = a.i >= -(i128)2147483647 && a.i <= (i128)2147483647 && b.i >= -(i128)2147483647 &&
+                    b.i <= (i128)2147483647;
+                if (small) o.i = (i128)((int64_t)a.i * (int64_t)b.i);
+                else o.i = (i128)(x * y);
+            } else {
+                o.i = (i128)(in.op == QDB_EXPR_ADD ? x + y : x - y);
+            }
         }
         break;
     case QDB_EXPR_NEG:
