@@ -3105,8 +3105,18 @@ ProbePipeline* probe_pipeline(JoinPlan* P, int mode)
     hipStream_t s = stream();
     // Capture: kernels and one copy; the queue head is cleared by a kernel
     // (no memset node).
-    bool ok = hip_ok(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "probe pipeline capture");
-    if (ok) {
+    // No other thread allocates or frees device memory meanwhile (either can
+    // invalidate the capture); a capture that fails anyway is made again.
+    bool ok = false;
+    for (int attempt = 0; attempt < 3 && !ok; ++attempt) {
+        std::unique_lock<std::shared_mutex> g(capture_mutex());
+        if (attempt > 0) {
+            (void)hipGetLastError();
+            clear_error();
+            if (pp->graph) (void)hipGraphDestroy(pp->graph);
+            pp->graph = nullptr;
+        }
+        if (!hip_ok(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "probe pipeline capture")) continue;
         zero_fill(pp->d_head, 16);
         hipLaunchKernelGGL(k_probe_init_batched, dim3(ProbePipeline::kMaxInst), dim3(64), 0, s, d_insts, d_ctl, pp->words);
         launch_probe_batched(mode, num_cus(), d_insts, d_items, d_ctl, pp->d_head);

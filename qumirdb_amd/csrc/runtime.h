@@ -7,6 +7,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -72,6 +73,10 @@ private:
 // Grid size for a grid-stride kernel over `items` with `block` threads: at
 // most `per_cu` blocks per CU and no more than the items need.
 int grid_for(int64_t items, int block, int per_cu);
+// Held exclusively while a thread captures a graph, and shared around the
+// device pool's hipMalloc and hipFree: on ROCm an allocation or a free by
+// another thread can invalidate a capture in progress.
+std::shared_mutex& capture_mutex();
 
 // Exclusive prefix sum of n u32 values (out may alias nothing in in), and a
 // stable LSD radix sort of (key, value) u32 pairs over key bits [0, end_bit).

@@ -8,6 +8,7 @@
 
 #include <map>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <unordered_map>
 #include <unordered_set>
@@ -208,6 +209,12 @@ int num_cus()
 // 64 MiB serve requests down to 2/3 of their size (per-query tables and
 // staging vary in size from call to call); smaller blocks serve their own
 // class only.
+std::shared_mutex& capture_mutex()
+{
+    static std::shared_mutex m;
+    return m;
+}
+
 static size_t reuse_limit(size_t c)
 {
     return c > ((size_t)64 << 20) ? c + c / 2 : c;
@@ -223,7 +230,10 @@ static void evict_oldest_locked(Runtime& r)
     r.dev_sizes.erase(oldest->second.p);
     r.cached_set.erase(oldest->second.p);
     if (oldest->second.ev) r.pool_events.push_back(oldest->second.ev);
-    (void)hipFree(oldest->second.p);
+    {
+        std::shared_lock<std::shared_mutex> g(capture_mutex());
+        (void)hipFree(oldest->second.p);
+    }
     r.n_free += 1;
     r.free_dev.erase(oldest);
 }
@@ -249,7 +259,10 @@ static void release_cached_locked(Runtime& r)
     for (auto& kv : r.free_dev) {
         r.dev_sizes.erase(kv.second.p);
         if (kv.second.ev) r.pool_events.push_back(kv.second.ev);
-        (void)hipFree(kv.second.p);
+        {
+            std::shared_lock<std::shared_mutex> g(capture_mutex());
+            (void)hipFree(kv.second.p);
+        }
         r.n_free += 1;
     }
     r.free_dev.clear();
@@ -291,13 +304,20 @@ void* dalloc(size_t bytes)
         std::lock_guard<std::mutex> lock(r.mu);
         if (r.cached) make_room_locked(r, c);
     }
-    hipError_t e = hipMalloc(&p, c);
+    hipError_t e;
+    {
+        std::shared_lock<std::shared_mutex> g(capture_mutex());
+        e = hipMalloc(&p, c);
+    }
     if (e != hipSuccess) {
         // Return cached blocks to the HIP runtime and retry once.
         std::lock_guard<std::mutex> lock(r.mu);
         release_cached_locked(r);
         (void)hipGetLastError();
-        e = hipMalloc(&p, c);
+        {
+            std::shared_lock<std::shared_mutex> g(capture_mutex());
+            e = hipMalloc(&p, c);
+        }
         if (e != hipSuccess) {
             fail("hipMalloc(%zu): %s", c, hipGetErrorString(e));
             return nullptr;
